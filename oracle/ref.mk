@@ -1,0 +1,39 @@
+# Builds the reference LMie C library from its own sources, where they lie, into
+# oracle/_ref/ (git-ignored).  Test infrastructure only: the library is used as a
+# checker and as the CPU baseline, never by the product path.
+#
+#   make -f oracle/ref.mk            (from the repository root)
+#
+# The reference's own build system is not used: this compiles the C files of the
+# path directly with gcc and the reference's stock flags (make.inc.example:12-20),
+# with -DUSE_PTHREADS for its pthreads scheduler (src/Makefile:23-25).
+# -Bsymbolic keeps the reference's calls to its own lmie_* symbols inside this
+# library, so it can share a process with the product library that exports the
+# same names.
+
+REF      ?= /root/reference
+OUT      := oracle/_ref
+CC       := gcc
+CFLAGS   := -O2 -fPIC -Wall -Wno-cpp -Wno-misleading-indentation -Wno-uninitialized \
+            -Wno-unused-function -fno-strict-aliasing -fopenmp -DUSE_PTHREADS \
+            -D'VERSION="0.91"' -I$(REF)/src -I$(REF)/misc
+SRCS     := $(REF)/src/lmie_core.c $(REF)/src/lmie_interface.c $(REF)/src/lmie_schedule.c \
+            $(REF)/misc/gutil_alloc.c $(REF)/misc/gutil_error.c $(REF)/misc/gutil_parse.c \
+            $(REF)/misc/gindex_name_value.c $(REF)/misc/rtutil_math.c \
+            $(REF)/misc/rtutil_scat_int.c $(REF)/misc/rtutil_size_dist.c \
+            $(REF)/misc/rtutil_support.c $(REF)/misc/rtutil_scat_io.c $(REF)/misc/zeroin.c
+OBJS     := $(patsubst $(REF)/%.c,$(OUT)/obj/%.o,$(SRCS))
+
+all: $(OUT)/liblmie_ref.so
+
+$(OUT)/obj/%.o: $(REF)/%.c
+	@mkdir -p $(dir $@)
+	$(CC) $(CFLAGS) -c $< -o $@
+
+$(OUT)/liblmie_ref.so: $(OBJS)
+	$(CC) -shared -fopenmp -Wl,-Bsymbolic -o $@ $(OBJS) -lpthread -lm
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean
