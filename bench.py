@@ -1,0 +1,125 @@
+"""Benchmark of the size-integrated Mie hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path mixed-adjoint|fp64]
+                    [--no-cpu-baseline]
+
+Workload: two log-normal modes (r_g 0.08 um / sigma_g 1.5 and r_g 2.5 um /
+sigma_g 2.0, number ratio 1000:1) on one grid r in [0.001, 100] um with 512 x 64
+= 32768 size points, wavelength 0.34 um, m = 1.53 + 0.005i: x_max 1848,
+n1 1905, 3810 Gauss angles, 6 x 3810 expansion coefficients.  Inputs are a few
+scalars; the size grid, the Mie coefficients and every table are generated on
+the device inside the timed step.
+
+A step is one call of the library for that workload.  The JSON line reports:
+  value        sum_i n1(x_i) over the size points per second (size-pt * n-term / s)
+  roofline     the angular-contraction kernel: algorithmic FLOP per launch
+               (8 n_qang sum_i n1(x_i)) over its device time, measured with HIP
+               events on the library's stream, against the MI355X peak
+  cpu_baseline the reference's pthreads CPU path (oracle/_ref) on a bounded
+               sample of the same workload, on this host's cores
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "size-pts*n-terms/s + adjoint grads/s vs plain FP64 GPU at rel err<=1e-6"
+PEAK = {"fp64": 78.6e12, "bf16": 2.5e15}
+
+
+def sum_n1(p):
+    """sum_i n1(x_i) over the grid of p (Gauss nodes per interval)."""
+    t, _ = np.polynomial.legendre.leggauss(p["n_quad"])
+    h = (p["r2"] - p["r1"]) / p["n_int2"]
+    r = (p["r1"] + h * (np.arange(p["n_int2"])[:, None] + 0.5) + 0.5 * h * t[None, :]).ravel()
+    x = 2 * math.pi * r / p["lambda"]
+    return int(np.sum(np.floor(x + 4.05 * x ** (1.0 / 3.0) + 8)))
+
+
+def cpu_baseline(threads):
+    """The reference build on a bounded sample: both modes, 64 x 64 = 4096 size
+    points each (1/8 of the grid, same radius range, same 3810 angles)."""
+    from lmie_amd import abi, workloads as W
+    so = os.path.join(ROOT, "oracle", "_ref", "liblmie_ref.so")
+    if not os.path.exists(so):
+        return None
+    ref = abi.bind(C.CDLL(so))
+    tot_units, t0 = 0, time.perf_counter()
+    for k in range(2):
+        p = dict(W.cfg4_mode_params(k), n_int2=64)
+        abi.solve(ref, p, None, threads)
+        tot_units += sum_n1(p)
+    dt = time.perf_counter() - t0
+    return {"value": tot_units / dt, "unit": "size-pt*n-term/s", "cores": threads,
+            "kind": "reference",
+            "sample": "reference lmie_solution (pthreads) on both modes at 64x64 size points each "
+                      "(1/8 of the grid, same r range and 3810 angles): %.1f s" % dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--path", default="fp64", choices=["fp64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the size points "
+              "would shard across GPUs with one all-reduce of the raw sums); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import lmie_amd
+    from lmie_amd import lmie, workloads as W
+    lib = lmie_amd.load()
+    modes = [(W.cfg4_mode_params(k), None) for k in range(2)]
+    weights = [m["weight"] for m in W.CFG4_MODES]
+    units = sum_n1(W.cfg4_mode_params(0))
+    nq = 2 * W.calc_n1(2 * math.pi * W.CFG4_GRID["r2"] / W.CFG4_GRID["lambda"])
+
+    def step():
+        return lmie.solve_mix(lib, modes, weights, a.path)
+
+    lib.lmie_set_timing(1)
+    for _ in range(a.warmup):
+        step()
+    t_ang = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        step()
+        t_ang.append(lmie.timings(lib)["angular"])
+    dt = (time.perf_counter() - t0) / a.steps
+    lib.lmie_set_timing(0)
+
+    flop = 8.0 * nq * units
+    ang_ms = float(np.mean(t_ang))
+    achieved = flop / (ang_ms * 1e-3)
+    peak = PEAK["fp64"]
+    line = {
+        "metric": METRIC, "value": units / dt, "unit": "size-pt*n-term/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: generated size grid of two log-normal modes",
+        "config": {"workload": "two-mode log-normal, 32768 sizes, x_max 1848, 3810 angles, "
+                               "gc 6x3810", "path": a.path, "sum_n1": units, "n_qang": nq},
+        "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "kernel": "angular_f64", "kernel_ms": ang_ms},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(min(16, os.cpu_count() or 1))
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

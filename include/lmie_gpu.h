@@ -1,0 +1,89 @@
+/*
+ * lmie_gpu.h -- thin C ABI between the host C layer of liblmie_amd.so and its
+ * gfx950 kernels.  Plain pointers and sizes; all arrays are host memory owned
+ * by the caller.  Counterpart of the accelerator hook the reference left in
+ * its scheduler (lmie_core_cuda, src2/lmie_cuda.h:19-30, called from
+ * src/lmie_schedule.c:443-449), re-cut for this design: the kernels return
+ * the raw size-integrated sums and the expansion of the raw scattering matrix;
+ * the host applies the normalisation (src/lmie_schedule.c:1155-1199) to the
+ * small results.
+ */
+#ifndef LMIE_GPU_H
+#define LMIE_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+     int n_size;              /* size points N                                   */
+     const double *x;         /* [N] size parameter 2 pi r / lambda              */
+     const double *f;         /* [N] number density x quadrature weight          */
+     const int *n1;           /* [N] per-size order n1(x_i), src/lmie_core.c:18-26 */
+     const int *n2;           /* [N] per-size D_n start,   src/lmie_core.c:30-38 */
+     double mr, mi;           /* m = mr - i mi (src/lmie_schedule.c:1048)         */
+     int n1_max;              /* order of the largest radius r2                   */
+     int n_qang;              /* Gauss angles (2 n1_max), 0 = no angular work     */
+     int n_coef;              /* expansion length (2 n1_max)                      */
+     int calc_gc, calc_lc;
+     /* Mie tangent channels: seeds that move x or m (src/lmie_interface.c:327-331) */
+     int n_tan;
+     const double *dx;        /* [n_tan][N]                                       */
+     const double *dm;        /* [n_tan][2] (re, im) of d m                       */
+     const double *df_tan;    /* [n_tan][N] d f along the channel                 */
+     /* weight-only channels: seeds that move only the number density */
+     int n_wgt;
+     const double *df_wgt;    /* [n_wgt][N]                                       */
+     int save_control;        /* 1: return per-size series and matrix elements    */
+     int precision;           /* enum lmie_precision                              */
+     int device;
+} lmie_gpu_job;
+
+typedef struct {
+     /* raw f-weighted sums over sizes: ext, sca, bak, asym (src/lmie_core.c:537-596) */
+     double sums[4];
+     double *sums_ch;         /* [n_tan + n_wgt][4] their tangents               (caller) */
+     /* expansion of the raw 4-element matrix, channel 0 = value, then tangents,
+        each [6][n_coef] in the output order beta, alpha, zeta, delta, gamma, epsilon,
+        before the (2 / C_sca) normalisation */
+     double *gc_raw;          /* [1 + n_tan + n_wgt][6][n_coef] or NULL          (caller) */
+     double *lc_raw;          /* same for the Legendre expansion, or NULL        (caller) */
+     double *save1;           /* [4][N]          if save_control == 1            (caller) */
+     double *save2;           /* [4][N][n_qang]  if save_control == 1            (caller) */
+} lmie_gpu_result;
+
+int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res);
+
+/* Reverse pass after lmie_gpu_forward on the same thread and job: given the
+   cotangents of the raw sums (4) and of the raw expansions (gc_raw / lc_raw of
+   channel 0, [6][n_coef], NULL = zero), return per size the cotangents of
+   f_i and x_i, and the cotangent of m as (d/d mr, d/d mi) summed over sizes. */
+typedef struct {
+     const double *sums_bar;  /* [4]                                              */
+     const double *gc_bar;    /* [6][n_coef] or NULL                              */
+     const double *lc_bar;    /* [6][n_coef] or NULL                              */
+} lmie_gpu_adj_in;
+
+typedef struct {
+     double *f_bar;           /* [N]                                      (caller) */
+     double *x_bar;           /* [N] (zero: the x reverse pass is not built)(caller) */
+     double m_bar[2];         /* dL/d mr, dL/d mi                                  */
+} lmie_gpu_adj_out;
+
+int lmie_gpu_adjoint(const lmie_gpu_job *job, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out);
+
+/* Scattering matrix at n_theta user angles from expansion coefficients
+   (misc/rtutil_scat_int.c:510-797): coefs [n_set][6][ld], n_coef used,
+   theta [n_theta] in degrees, pf [n_set][6][n_theta]; gsf = 1 for
+   generalized-spherical-function coefficients, 0 for Legendre. */
+int lmie_gpu_phase_func(int n_set, int n_coef, int ld, const double *coefs, int n_theta,
+                        const double *theta, double *pf, int gsf, int device);
+
+/* Error text of the last failing call on this thread ("" if none). */
+const char *lmie_gpu_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LMIE_GPU_H */

@@ -1,0 +1,153 @@
+/*
+ * lmie_hip.h -- GPU-side C ABI of liblmie_amd.so (gfx950, one GPU).
+ *
+ * Plain C types only.  Three groups of entry points:
+ *
+ *  1. lmie_core_solution: the reference's own scheduler seam
+ *     (src/lmie_schedule.h:18-30, called from src/lmie_interface.c:453-462),
+ *     same signature.  It replaces the CPU dispatch of lmie_core_shared
+ *     (src/lmie_schedule.c:391-452, the hook the reference left for an
+ *     accelerator at :443-449) together with the normalisation and the
+ *     expansion (src/lmie_schedule.c:1155-1241).  FP64 throughout.
+ *
+ *  2. Extensions with no counterpart in the reference, for precision choice,
+ *     gradients and the two-mode mixture:
+ *       lmie_solution_ex       lmie_solution with an options block
+ *       lmie_solution_mix      several distributions on one size grid, mixed
+ *                              by number weights (one GPU pass)
+ *       lmie_solution_adjoint  forward pass plus one reverse-mode pass: the
+ *                              gradient of a scalar loss over the outputs with
+ *                              respect to every scalar input
+ *     (the reference's derivatives are forward-mode only; its adjoint
+ *      equations are left empty in doc/lmie_equations.tex:247-249).
+ *
+ *  3. Diagnostics: lmie_last_error and per-kernel timing.
+ *
+ * All functions return 0 on success and -1 on error (then lmie_last_error()
+ * names the failing step), like the reference's API.
+ */
+#ifndef LMIE_HIP_H
+#define LMIE_HIP_H
+
+#include "lmie_interface.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic of the angular contraction.  FP64: every stage in double.
+   MIXED: the Mie coefficients, cross sections, reductions, normalisation and
+   expansion stay in double; the angular sums S1/S2 (and their adjoint) run on
+   the matrix cores with each operand split into a high and a low 16-bit part
+   (three products, fp32 accumulation per block of orders, blocks summed in
+   double). */
+enum lmie_precision {
+     LMIE_PRECISION_FP64  = 0,
+     LMIE_PRECISION_MIXED = 1
+};
+
+typedef struct {
+     int precision;        /* enum lmie_precision */
+     int device;           /* HIP device ordinal */
+     int reserved[6];      /* zero */
+} lmie_opts;
+
+/* (1) Scheduler seam, src/lmie_schedule.h:18-30.  Returns the truncated
+   coefficient count (>= 0) or -1. */
+int lmie_core_solution(int n_qsiz, int n_derivs1, int n_derivs2,
+                       int save_control, int *index1, int *index2,
+                       double lambda, double *lambda_l,
+                       double mr, double mi, double *mr_l, double *mi_l,
+                       double *qx, double *qw, double *nr,
+                       double r1, double r2,
+                       double *cext_, double *csca_, double *pbak_,
+                       double *g, double **gc, double **lc,
+                       double **qx_l, double **qw_l, double **nr_l,
+                       double *cext_l_, double *csca_l_, double *pbak_l_,
+                       double *g_l, double ***gc_l, double ***lc_l,
+                       double ***save1, double ****save2, double accuracy,
+                       int verbose, int n_threads, int use_mpi);
+
+/* Same seam with an options block (NULL = FP64 on device 0). */
+int lmie_core_solution_ex(int n_qsiz, int n_derivs1, int n_derivs2,
+                          int save_control, int *index1, int *index2,
+                          double lambda, double *lambda_l,
+                          double mr, double mi, double *mr_l, double *mi_l,
+                          double *qx, double *qw, double *nr,
+                          double r1, double r2,
+                          double *cext_, double *csca_, double *pbak_,
+                          double *g, double **gc, double **lc,
+                          double **qx_l, double **qw_l, double **nr_l,
+                          double *cext_l_, double *csca_l_, double *pbak_l_,
+                          double *g_l, double ***gc_l, double ***lc_l,
+                          double ***save1, double ****save2, double accuracy,
+                          const lmie_opts *opts);
+
+/* (2) lmie_solution with options (NULL = FP64 on device 0).  No
+   counterpart in the reference. */
+int lmie_solution_ex(lmie_in_data *in, lmie_out_data *out, int alloc_out,
+                     const lmie_opts *opts);
+
+/* Mixture of n_modes distributions on one grid: every mode shares
+   dist grid fields (n_int1, n_int2, n_quad, r1, r2), lambda, mr, mi,
+   accuracy and the calc_* flags of modes[0]; mode k contributes number
+   density weight[k] * n_k(r) (each n_k normalised to unit integral).
+   Outputs are those of the mixed distribution; the distribution statistics
+   (reff ...) are those of the mixture.  Seeds: modes[k]->n_derivs columns
+   each; the output tangent columns are the modes' columns in order (mode 0
+   first), so out must be allocated for sum_k n_derivs.  Seeds of lambda,
+   mr, mi are read from each mode's own columns.  No counterpart in the
+   reference. */
+int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_modes,
+                      lmie_out_data *out, int alloc_out, const lmie_opts *opts);
+
+/* Cotangents of the outputs (the loss gradient dL/d output).  NULL arrays
+   are zero.  gc / lc are [6][n_coef_max] row-major (the layout of out->gc[0]). */
+typedef struct {
+     double cext, csca, cbak, g;
+     const double *gc;
+     const double *lc;
+} lmie_adj_in;
+
+/* Gradient of the loss with respect to each scalar input, in the order of
+   lmie_in_data: lambda, mr, mi, a1, a2, a3, a4, a5, r1, r2.  For a mixture,
+   mode k's distribution parameters a1..a5 go to grad_modes[5 k .. 5 k + 4]
+   (grad_modes may be NULL for one mode). */
+typedef struct {
+     double lambda, mr, mi, a1, a2, a3, a4, a5, r1, r2;
+} lmie_grad;
+
+/* Forward pass (outputs written to out exactly as lmie_solution_ex with
+   n_derivs = 0) followed by one reverse pass.  modes / weight / n_modes as
+   lmie_solution_mix (n_modes = 1, weight = NULL for a single distribution).
+   The gradient covers mr, mi and the distribution parameters a1..a5;
+   lambda, r1 and r2 are reported as NaN (their reverse pass is not built).
+   No counterpart in the reference. */
+int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int n_modes,
+                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
+                          const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes);
+
+/* (3) Text of the last error on this thread ("" if none). */
+const char *lmie_last_error(void);
+
+/* Per-kernel device time of the last call on this thread, in milliseconds,
+   measured with HIP events on the library's stream when enabled. */
+enum lmie_timer {
+     LMIE_T_TABLES = 0,    /* Gauss nodes + angular and expansion tables  */
+     LMIE_T_COEFS,         /* per-size recurrences, a_n/b_n, series        */
+     LMIE_T_ANGULAR,       /* angular contraction (dominant kernel)        */
+     LMIE_T_REDUCE,        /* reductions over sizes                        */
+     LMIE_T_EXPAND,        /* expansion into coefficients                  */
+     LMIE_T_ADJ_ANGULAR,   /* adjoint angular contraction                  */
+     LMIE_T_ADJ_COEFS,     /* adjoint per-size recurrences                 */
+     LMIE_T_TOTAL,         /* whole device pipeline of the call            */
+     LMIE_N_TIMERS
+};
+int lmie_set_timing(int on);
+int lmie_get_timing(double *ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LMIE_HIP_H */

@@ -1,0 +1,1335 @@
+// lmie_gpu.hip -- gfx950 kernels of the size-integrated Mie path and the thin C
+// ABI of include/lmie_gpu.h.
+//
+// Pipeline of one forward call (one HIP stream per workspace):
+//   gauss_nodes   Gauss-Legendre angles on [-1, 1]             (misc/rtutil_math.c:231-316)
+//   pitau_table   pi_n +- tau_n at the positive angles         (src/lmie_core.c:669-691)
+//   gsf_table     P^{00}, P^{02}, P^{22}, P^{2-2} at the same  (misc/rtutil_math.c:867-1017)
+//   mie_coefs     per size point: D_n, psi_n, chi_n, a_n, b_n, series, tangents
+//                                                              (src/lmie_core.c:263-652)
+//   series_sums   f-weighted sums over sizes of the series      (src/lmie_core.c:537-611)
+//   angular_f64   S1/S2 contraction on FP64 matrix cores, Stokes epilogue and
+//                 weighted reduction over sizes                (src/lmie_core.c:658-791)
+//   angular_mix   the same contraction on BF16 matrix cores with split operands
+//   reduce_parts  deterministic sum of the per-block partials  (src/lmie_schedule.c:109-147)
+//   expand_*      generalized-spherical / Legendre expansion   (misc/rtutil_scat_int.c:20-480)
+//
+// HBM layouts (N_p = N rounded up to 64, K_p = n1_max rounded up to 16):
+//   A planes   [channel][4][K_p][N_p]  fv4_n (a_n + b_n) re, im; fv4_n (a_n - b_n) re, im,
+//              row n-1 holds order n; columns are size points, so the 64 lanes
+//              of a wave (64 consecutive sizes) store 512 contiguous bytes.
+//   B tables   [2][K_p][H_p]           pi_n + tau_n, pi_n - tau_n at the H = n_qang/2
+//              positive Gauss angles.
+//   GSF tables [4][n_coef][H_p].
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <pthread.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/lmie_gpu.h"
+#include "../../include/lmie_hip.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+constexpr int KC = 16;     // orders per LDS stage of the contraction
+constexpr int TILE = 64;   // size-point tile of the padded layouts
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+thread_local char g_err[512];
+thread_local double g_times[LMIE_N_TIMERS];
+int g_timing = 0;
+
+int set_err(const char *fmt, ...) {
+     va_list ap;
+     va_start(ap, fmt);
+     vsnprintf(g_err, sizeof g_err, fmt, ap);
+     va_end(ap);
+     return -1;
+}
+
+#define HIPCHECK(expr)                                                                     \
+     do {                                                                                  \
+          hipError_t e_ = (expr);                                                          \
+          if (e_ != hipSuccess)                                                            \
+               return set_err("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),        \
+                              __FILE__, __LINE__);                                         \
+     } while (0)
+
+inline int rup(int a, int b) { return (a + b - 1) / b * b; }
+
+// ---------------------------------------------------------------------------
+// complex helpers (double2 = re, im)
+struct cd {
+     double r, i;
+};
+__device__ __forceinline__ cd cmk(double r, double i) { return cd{r, i}; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return cd{a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return cd{a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cd cmul(cd a, cd b) { return cd{a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r}; }
+__device__ __forceinline__ cd cscale(cd a, double s) { return cd{a.r * s, a.i * s}; }
+__device__ __forceinline__ cd cconj(cd a) { return cd{a.r, -a.i}; }
+__device__ __forceinline__ double cabs2(cd a) { return a.r * a.r + a.i * a.i; }
+// Smith's division: scaled by the larger component of the divisor, so no
+// intermediate overflows where |b|^2 would.
+__device__ __forceinline__ cd cdiv(cd a, cd b) {
+     if (fabs(b.r) >= fabs(b.i)) {
+          double t = b.i / b.r, d = b.r + b.i * t;
+          return cd{(a.r + a.i * t) / d, (a.i - a.r * t) / d};
+     }
+     double t = b.r / b.i, d = b.r * t + b.i;
+     return cd{(a.r * t + a.i) / d, (a.i * t - a.r) / d};
+}
+__device__ __forceinline__ cd crecip(cd b) { return cdiv(cd{1., 0.}, b); }
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre angles: one thread per root.  Newton steps with the
+// second-order correction, stopped at a step <= DBL_EPSILON
+// (misc/rtutil_math.c:73-129); nodes ascending, weights
+// (1 - y^2) 2 / (n P_{n-1}(y'))^2 with y' the last-but-one iterate
+// (misc/rtutil_math.c:279-310).  Near mu = +-1 that weight formula is
+// sensitive (about 1e-8 relative per ulp of y') to the iteration path, so the
+// initial guesses come from the host (the C library's cos and tan, as the
+// reference) and every operation here is a separately rounded IEEE operation
+// (no contraction): the rule is the reference's rule bit for bit.
+__global__ void gauss_nodes(int n, const double *guess, double *mu, double *w) {
+#pragma clang fp contract(off)
+     int i = blockIdx.x * blockDim.x + threadIdx.x;
+     if (i >= (n + 1) / 2) return;
+     double y = guess[i], y0, pa, pb;
+     const double nn1 = (double) (n * (n + 1));
+     int it = 0;
+     do {
+          double pm;
+          pa = 1.;
+          pb = 0.;
+          for (int j = 0; j < n; ++j) {
+               pm = pb;
+               pb = pa;
+               const double u = y * pb;
+               pa = j / (j + 1.) * (u - pm) + u;
+          }
+          const double s = 1. - y * y;
+          const double d1 = n * (pb - y * pa) / s;
+          const double d2 = (2. * y * d1 - nn1 * pa) / s;
+          const double h = pa / d1;
+          y0 = y;
+          y = y0 - h * (1. + h * d2 / (2. * d1));
+     } while (fabs(y - y0) > 2.220446049250313e-16 && ++it < 100);
+     double c = n * pb;
+     c = 2. / (c * c);
+     const double off = 1. * y;
+     mu[i] = 0. - off;
+     mu[n - 1 - i] = 0. + off;
+     w[i] = w[n - 1 - i] = (1. - off * y) * c;
+}
+
+// pi_n +- tau_n at the positive angles mu_{H+j}: one thread per angle, upward
+// recurrences pi_{n+1} = mu pi_n + (n+1)/n (mu pi_n - pi_{n-1}),
+// tau_n = n (mu pi_n - pi_{n-1}) - pi_{n-1} (src/lmie_core.c:669-691).
+__global__ void pitau_table(int H, int Hp, int n1max, int Kp, const double *mu, double *Bp, double *Bm) {
+     int j = blockIdx.x * blockDim.x + threadIdx.x;
+     if (j >= Hp) return;
+     if (j >= H) {
+          for (int n = 1; n <= Kp; ++n) Bp[(size_t) (n - 1) * Hp + j] = Bm[(size_t) (n - 1) * Hp + j] = 0.;
+          return;
+     }
+     double m = mu[H + j], p0 = 0., p1 = 1.;
+     for (int n = 1; n <= n1max; ++n) {
+          double a1 = m * p1, a2 = a1 - p0;
+          double tau = n * a2 - p0;
+          Bp[(size_t) (n - 1) * Hp + j] = p1 + tau;
+          Bm[(size_t) (n - 1) * Hp + j] = p1 - tau;
+          p0 = p1;
+          p1 = a1 + (n + 1.) / n * a2;
+     }
+     for (int n = n1max + 1; n <= Kp; ++n) Bp[(size_t) (n - 1) * Hp + j] = Bm[(size_t) (n - 1) * Hp + j] = 0.;
+}
+
+// Generalized spherical functions at the positive angles, l = 0 .. C-1, by
+// their three-term recurrences in l (misc/rtutil_math.c:956-1016).  Layout
+// [4][C][Hp]: lanes (angles) store contiguously for every l.
+__global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) {
+     int j = blockIdx.x * blockDim.x + threadIdx.x;
+     if (j >= H) return;
+     double m = mu[H + j], m2 = m * m;
+     size_t plane = (size_t) C * Hp;
+     double *P00 = T, *P02 = T + plane, *P22 = T + 2 * plane, *P2m = T + 3 * plane;
+     double a00 = 1., b00 = m, a02 = 0., b02 = 0., a22 = 0., b22 = 0., a2m = 0., b2m = 0.;
+     for (int l = 0; l < C; ++l) {
+          double v00, v02, v22, v2m;
+          if (l == 0) { v00 = 1.; v02 = v22 = v2m = 0.; }
+          else if (l == 1) { v00 = m; v02 = v22 = v2m = 0.; }
+          else if (l == 2) {
+               v00 = (3. * m2 - 1.) / 2.;
+               v02 = .25 * sqrt(6.) * (m2 - 1.);
+               v22 = .25 * (1. + m) * (1. + m);
+               v2m = .25 * (1. - m) * (1. - m);
+          } else {
+               double L = l - 1, L1 = l, t = 2. * L + 1.;
+               double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
+               double c = t * L * L1 * m, k4 = 4. * t, kb = L1 * (L * L - 4.), kd = L * (L1 * L1 - 4.);
+               v00 = (t * m * b00 - L * a00) / L1;
+               v02 = (t * m * b02 - sa * a02) / sb;
+               v22 = ((c - k4) * b22 - kb * a22) / kd;
+               v2m = ((c + k4) * b2m - kb * a2m) / kd;
+          }
+          P00[(size_t) l * Hp + j] = v00;
+          P02[(size_t) l * Hp + j] = v02;
+          P22[(size_t) l * Hp + j] = v22;
+          P2m[(size_t) l * Hp + j] = v2m;
+          a00 = b00; b00 = v00; a02 = b02; b02 = v02; a22 = b22; b22 = v22; a2m = b2m; b2m = v2m;
+     }
+}
+
+// ---------------------------------------------------------------------------
+// Per-size Mie coefficients.  One thread per size point (NT tangent channels
+// carried alongside).  Downward D_n(z) from n2 (stored for n <= n1) and
+// downward ratio r_n = psi_n / psi_{n-1} from r_{n1} = 0 (stored), then one
+// upward sweep for psi_n, chi_n, a_n, b_n, the series and the A operand
+// (src/lmie_core.c:298-652).  Scratch rows are [n][Np] so every access of a
+// wave is one contiguous 512-byte (or 1 KiB) line.
+struct CoefArgs {
+     int N, Np, n1max;
+     const double *x;           // [Np]
+     const int *n1, *n2;        // [Np]
+     const int *kfill;          // [Np / TILE] rows to zero-fill up to
+     double mr, mi;
+     int nt;                    // tangent channels in this pass
+     const double *dx;          // [nt][Np]
+     const double *dm;          // [nt][2]
+     double *scrD;              // [(1 + nt)][2][n1max + 1][Np]
+     double *scrR;              // [(1 + nt)][n1max + 1][Np]
+     double *A;                 // base planes [4][Kp][Np]
+     double *At[4];             // tangent planes of this pass, each [4][Kp][Np]
+     int Kp;
+     double *S;                 // series [4][Np]: ext, sca, |c_bak|^2, asym
+     double *St[4];             // tangent series of this pass, each [4][Np]
+     int write_base;            // write A and S of the base channel
+};
+
+template <int NT>
+__global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
+     int i = blockIdx.x * blockDim.x + threadIdx.x;
+     if (i >= p.Np) return;
+     const size_t Np = p.Np, plane = (size_t) p.Kp * Np, srow = (size_t) (p.n1max + 1) * Np;
+     int kf = p.kfill[i / TILE];
+     if (i >= p.N) {
+          // padded columns: zero operand rows and series
+          for (int r = 0; r < kf; ++r)
+               for (int c = 0; c < 4; ++c) {
+                    if (p.write_base) p.A[c * plane + r * Np + i] = 0.;
+                    for (int t = 0; t < NT; ++t) p.At[t][c * plane + r * Np + i] = 0.;
+               }
+          for (int q = 0; q < 4; ++q) {
+               if (p.write_base) p.S[q * Np + i] = 0.;
+               for (int t = 0; t < NT; ++t) p.St[t][q * Np + i] = 0.;
+          }
+          return;
+     }
+     const double x = p.x[i];
+     const int n1 = p.n1[i], n2 = p.n2[i];
+     const cd m = cmk(p.mr, -p.mi);
+     const cd z = cscale(m, x);
+     const cd iz = crecip(z);
+     const cd im = crecip(m);
+     cd dz[NT > 0 ? NT : 1], dmv[NT > 0 ? NT : 1];
+     double dxv[NT > 0 ? NT : 1];
+#pragma unroll
+     for (int t = 0; t < NT; ++t) {
+          dxv[t] = p.dx[t * Np + i];
+          dmv[t] = cmk(p.dm[2 * t], p.dm[2 * t + 1]);
+          dz[t] = cadd(cscale(dmv[t], x), cscale(m, dxv[t]));
+     }
+     double *Dre = p.scrD, *Dim = p.scrD + srow, *R = p.scrR;
+
+     // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0
+     cd D = cmk(0., 0.), dD[NT > 0 ? NT : 1];
+#pragma unroll
+     for (int t = 0; t < NT; ++t) dD[t] = cmk(0., 0.);
+     for (int j = n2 - 1; j >= 1; --j) {
+          cd c1 = cscale(iz, (double) (j + 1));
+          cd c2 = cadd(D, c1);
+          cd rc = crecip(c2);
+          D = csub(c1, rc);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+               cd dc1 = cmul(cmul(cscale(c1, -1.), iz), dz[t]);
+               dD[t] = cadd(dc1, cmul(cadd(dD[t], dc1), cmul(rc, rc)));
+          }
+          if (j <= n1) {
+               Dre[j * Np + i] = D.r;
+               Dim[j * Np + i] = D.i;
+#pragma unroll
+               for (int t = 0; t < NT; ++t) {
+                    p.scrD[(size_t) (1 + t) * 2 * srow + j * Np + i] = dD[t].r;
+                    p.scrD[(size_t) (1 + t) * 2 * srow + srow + j * Np + i] = dD[t].i;
+               }
+          }
+     }
+     // downward ratio r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0
+     {
+          double r = 0., dr[NT > 0 ? NT : 1];
+#pragma unroll
+          for (int t = 0; t < NT; ++t) dr[t] = 0.;
+          R[n1 * Np + i] = 0.;
+#pragma unroll
+          for (int t = 0; t < NT; ++t) p.scrR[(size_t) (1 + t) * srow + n1 * Np + i] = 0.;
+          for (int j = n1 - 1; j >= 1; --j) {
+               double c = (2. * j + 1.) / x;
+               double rn = 1. / (c - r);
+#pragma unroll
+               for (int t = 0; t < NT; ++t) {
+                    dr[t] = (c / x * dxv[t] + dr[t]) * rn * rn;
+                    p.scrR[(size_t) (1 + t) * srow + j * Np + i] = dr[t];
+               }
+               r = rn;
+               R[j * Np + i] = r;
+          }
+     }
+     // upward sweep
+     double sx, cx;
+     sincos(x, &sx, &cx);
+     double psi0 = sx, chi0 = cx, chi1 = cx / x + sx;
+     double dpsi0[NT > 0 ? NT : 1], dchi0[NT > 0 ? NT : 1], dchi1[NT > 0 ? NT : 1];
+#pragma unroll
+     for (int t = 0; t < NT; ++t) {
+          dpsi0[t] = cx * dxv[t];
+          dchi0[t] = -sx * dxv[t];
+          dchi1[t] = -dxv[t] / (x * x) * cx + dchi0[t] / x + cx * dxv[t];
+     }
+     double se = 0., ss = 0., sg = 0.;
+     cd cb = cmk(0., 0.), ap = cmk(0., 0.), bp = cmk(0., 0.);
+     double dse[NT > 0 ? NT : 1], dss[NT > 0 ? NT : 1], dsg[NT > 0 ? NT : 1];
+     cd dcb[NT > 0 ? NT : 1], dap[NT > 0 ? NT : 1], dbp[NT > 0 ? NT : 1];
+#pragma unroll
+     for (int t = 0; t < NT; ++t) {
+          dse[t] = dss[t] = dsg[t] = 0.;
+          dcb[t] = dap[t] = dbp[t] = cmk(0., 0.);
+     }
+     double sgn = -1.;
+     for (int j = 1; j <= n1; ++j) {
+          const double r = R[j * Np + i];
+          const double psi = r * psi0;
+          const cd Dj = cmk(Dre[j * Np + i], Dim[j * Np + i]);
+          const double jx = j / x;
+          const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
+          // a_j with u = D_j / m + j / x; b_j with v = m D_j + j / x
+          const cd u = cadd(cmul(Dj, im), cmk(jx, 0.));
+          const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
+          const cd na = csub(cscale(u, psi), cmk(psi0, 0.)), da = csub(cmul(u, zj), zjm);
+          const cd nb = csub(cscale(v, psi), cmk(psi0, 0.)), db = csub(cmul(v, zj), zjm);
+          const cd a = cdiv(na, da), b = cdiv(nb, db);
+          const double fv1 = 2. * j + 1., fv2 = (j + 1.) / j, fv4 = fv1 / (j * (j + 1.));
+          const double fv3m = j > 1 ? (j + 1.) / ((double) j / (j - 1.)) : 0.;
+          const double fv4m = j > 1 ? (2. * j - 1.) / ((j - 1.) * j) : 0.;
+          (void) fv2;
+          se += fv1 * (a.r + b.r);
+          ss += fv1 * (cabs2(a) + cabs2(b));
+          cb = cadd(cb, cscale(csub(a, b), fv1 * sgn));
+          if (j > 1)
+               sg += fv3m * (ap.r * a.r + ap.i * a.i + bp.r * b.r + bp.i * b.i) + fv4m * (ap.r * bp.r + ap.i * bp.i);
+          if (p.write_base) {
+               const size_t o = (size_t) (j - 1) * Np + i;
+               p.A[o] = fv4 * (a.r + b.r);
+               p.A[plane + o] = fv4 * (a.i + b.i);
+               p.A[2 * plane + o] = fv4 * (a.r - b.r);
+               p.A[3 * plane + o] = fv4 * (a.i - b.i);
+          }
+#pragma unroll
+          for (int t = 0; t < NT; ++t) {
+               const double drt = p.scrR[(size_t) (1 + t) * srow + j * Np + i];
+               const double dpsi = drt * psi0 + r * dpsi0[t];
+               const cd dD = cmk(p.scrD[(size_t) (1 + t) * 2 * srow + j * Np + i],
+                                 p.scrD[(size_t) (1 + t) * 2 * srow + srow + j * Np + i]);
+               const double djx = -jx / x * dxv[t];
+               const cd dzj = cmk(dpsi, dchi1[t]), dzjm = cmk(dpsi0[t], dchi0[t]);
+               // du = dD/m - D dm / m^2 + d(j/x); dv = dm D + m dD + d(j/x)
+               const cd du = cadd(csub(cmul(dD, im), cmul(cmul(Dj, cmul(im, im)), dmv[t])), cmk(djx, 0.));
+               const cd dv = cadd(cadd(cmul(dmv[t], Dj), cmul(m, dD)), cmk(djx, 0.));
+               const cd dna = csub(cadd(cscale(du, psi), cscale(u, dpsi)), cmk(dpsi0[t], 0.));
+               const cd dda = csub(cadd(cmul(du, zj), cmul(u, dzj)), dzjm);
+               const cd dnb = csub(cadd(cscale(dv, psi), cscale(v, dpsi)), cmk(dpsi0[t], 0.));
+               const cd ddb = csub(cadd(cmul(dv, zj), cmul(v, dzj)), dzjm);
+               const cd dat = csub(cdiv(dna, da), cdiv(cmul(na, dda), cmul(da, da)));
+               const cd dbt = csub(cdiv(dnb, db), cdiv(cmul(nb, ddb), cmul(db, db)));
+               dse[t] += fv1 * (dat.r + dbt.r);
+               dss[t] += fv1 * 2. * (a.r * dat.r + a.i * dat.i + b.r * dbt.r + b.i * dbt.i);
+               dcb[t] = cadd(dcb[t], cscale(csub(dat, dbt), fv1 * sgn));
+               if (j > 1)
+                    dsg[t] += fv3m * (dap[t].r * a.r + dap[t].i * a.i + ap.r * dat.r + ap.i * dat.i +
+                                      dbp[t].r * b.r + dbp[t].i * b.i + bp.r * dbt.r + bp.i * dbt.i) +
+                              fv4m * (dap[t].r * bp.r + dap[t].i * bp.i + ap.r * dbp[t].r + ap.i * dbp[t].i);
+               const size_t o = (size_t) (j - 1) * Np + i;
+               p.At[t][o] = fv4 * (dat.r + dbt.r);
+               p.At[t][plane + o] = fv4 * (dat.i + dbt.i);
+               p.At[t][2 * plane + o] = fv4 * (dat.r - dbt.r);
+               p.At[t][3 * plane + o] = fv4 * (dat.i - dbt.i);
+               dap[t] = dat;
+               dbp[t] = dbt;
+               // advance tangent recurrences
+               const double c = fv1 / x;
+               const double dchi2 = -c / x * dxv[t] * chi1 + c * dchi1[t] - dchi0[t];
+               dchi0[t] = dchi1[t];
+               dchi1[t] = dchi2;
+               dpsi0[t] = dpsi;
+          }
+          ap = a;
+          bp = b;
+          sgn = -sgn;
+          const double chi2 = fv1 / x * chi1 - chi0;
+          chi0 = chi1;
+          chi1 = chi2;
+          psi0 = psi;
+     }
+     // zero rows n1 .. kf-1 so the contraction can run whole chunks
+     for (int r = n1; r < kf; ++r) {
+          const size_t o = (size_t) r * Np + i;
+          for (int c = 0; c < 4; ++c) {
+               if (p.write_base) p.A[c * plane + o] = 0.;
+#pragma unroll
+               for (int t = 0; t < NT; ++t) p.At[t][c * plane + o] = 0.;
+          }
+     }
+     if (p.write_base) {
+          p.S[i] = se;
+          p.S[Np + i] = ss;
+          p.S[2 * Np + i] = cabs2(cb);
+          p.S[3 * Np + i] = sg;
+     }
+#pragma unroll
+     for (int t = 0; t < NT; ++t) {
+          p.St[t][i] = dse[t];
+          p.St[t][Np + i] = dss[t];
+          p.St[t][2 * Np + i] = 2. * (cb.r * dcb[t].r + cb.i * dcb[t].i);
+          p.St[t][3 * Np + i] = dsg[t];
+     }
+}
+
+// ---------------------------------------------------------------------------
+// Weighted sums over sizes: out[k] = sum_i u_k[i] v_k[i] (+ sum_i u2_k[i] v2_k[i]),
+// one workgroup per output, fixed-order tree reduction (deterministic).
+struct DotJob {
+     const double *u, *v, *u2, *v2;
+};
+
+__global__ __launch_bounds__(256) void weighted_dots(const DotJob *jobs, int n, double *out) {
+     __shared__ double red[256];
+     const DotJob jb = jobs[blockIdx.x];
+     double s = 0.;
+     for (int i = threadIdx.x; i < n; i += 256) {
+          s += jb.u[i] * jb.v[i];
+          if (jb.u2) s += jb.u2[i] * jb.v2[i];
+     }
+     red[threadIdx.x] = s;
+     __syncthreads();
+     for (int w = 128; w > 0; w >>= 1) {
+          if ((int) threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+          __syncthreads();
+     }
+     if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// Angular contraction, FP64 matrix cores.
+//
+// For a tile of sizes i and positive angles j the four complex sums
+//   sp+ = sum_n S_n (pi+tau)_n   sm+ = sum_n T_n (pi-tau)_n
+//   sp- = sum_n s_n S_n (pi-tau)_n   sm- = sum_n s_n T_n (pi+tau)_n,
+// S_n = fv4 (a+b), T_n = fv4 (a-b), s_n = (-1)^(n-1) (the -mu mirror,
+// src/lmie_core.c:617-652,682-686), are 8 real GEMM accumulators of
+// v_mfma_f64_16x16x4 (operand lane layout: A[i][k] in lane i + 16 k, B[k][j] in
+// lane j + 16 k; result D[4 (l/16) + v][l % 16] in lane l, element v).  The
+// epilogue forms S1, S2 and the four matrix elements (src/lmie_core.c:726-749),
+// weights them by f_i and reduces over the sizes of the workgroup; partials per
+// block of sizes go to HBM and are summed by reduce_parts.
+struct AngArgs {
+     int Np, Kp, Hp, H, Q;
+     const double *A[3];        // base planes, tangent planes (channel 0, 1 of the pass)
+     const double *Bp, *Bm;     // [Kp][Hp]
+     const int *ktile;          // rows to contract per size tile of TM
+     const double *W[4];        // weight channels [Np]: f, then weight-only tangents
+     const double *WT[2];       // tangent df [Np]
+     const double *f;           // base weight (for f dP of tangents)
+     int tiles_per_block;       // size tiles per workgroup
+     double *part;              // [gridDim.y][nch][4][Q]
+     double *save2;             // [4][N][Q] per-size matrix (save_control 1) or NULL
+     int N;
+};
+
+__device__ __forceinline__ void stokes(double spr, double spi, double smr, double smi, double P[4]) {
+     const double s1r = .5 * (spr + smr), s1i = .5 * (spi + smi);
+     const double s2r = .5 * (spr - smr), s2i = .5 * (spi - smi);
+     const double q1 = s1r * s1r + s1i * s1i, q2 = s2r * s2r + s2i * s2i;
+     P[0] = (q1 + q2) * .5;
+     P[1] = s1r * s2r + s1i * s2i;            // Re S1 S2*
+     P[2] = -(q1 - q2) * .5;
+     P[3] = s1i * s2r - s1r * s2i;            // Im S1 S2*
+}
+
+__device__ __forceinline__ void stokes_tan(double spr, double spi, double smr, double smi, double tpr,
+                                           double tpi, double tmr, double tmi, double dP[4]) {
+     const double s1r = .5 * (spr + smr), s1i = .5 * (spi + smi);
+     const double s2r = .5 * (spr - smr), s2i = .5 * (spi - smi);
+     const double t1r = .5 * (tpr + tmr), t1i = .5 * (tpi + tmi);
+     const double t2r = .5 * (tpr - tmr), t2i = .5 * (tpi - tmi);
+     const double dq1 = 2. * (t1r * s1r + t1i * s1i), dq2 = 2. * (t2r * s2r + t2i * s2i);
+     // d(S1 S2*) = dS1 S2* + S1 dS2*
+     const double dxr = t1r * s2r + t1i * s2i + s1r * t2r + s1i * t2i;
+     const double dxi = t1i * s2r - t1r * s2i + s1i * t2r - s1r * t2i;
+     dP[0] = (dq1 + dq2) * .5;
+     dP[1] = dxr;
+     dP[2] = -(dq1 - dq2) * .5;
+     dP[3] = dxi;
+}
+
+template <int WM, int WN, int NT, int NW>
+__global__ __launch_bounds__(256) void angular_f64(AngArgs p) {
+     constexpr int TM = 32 * WM, TN = 32 * WN;
+     constexpr int SA = TM + 16, SB = TN + 16;       // LDS row strides (bank spread)
+     constexpr int NC = 1 + NT;
+     __shared__ double sA[NC * 4 * KC * SA];
+     __shared__ double sB[2 * KC * SB];
+     constexpr int NE = (NW + NT) > 0 ? (NW + NT) : 1;
+     __shared__ double sR[NE * 8 * 32 * WN];
+
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wm = wave & 1, wn = wave >> 1;
+     const int l16 = lane & 15, lg = lane >> 4;
+     const int jt0 = blockIdx.x * TN;
+     const int tile0 = blockIdx.y * p.tiles_per_block;
+     const size_t Np = p.Np, Hp = p.Hp, plane = (size_t) p.Kp * Np;
+
+     double ew[NE][WN][8];
+#pragma unroll
+     for (int c = 0; c < NW + NT; ++c)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int q = 0; q < 8; ++q) ew[c][b][q] = 0.;
+
+     for (int tt = 0; tt < p.tiles_per_block; ++tt) {
+          const int tile = tile0 + tt;
+          const int it0 = tile * TM;
+          if (it0 >= p.Np) break;
+          const int kmax = p.ktile[tile];
+          v4d acc[NC][WM][WN][8];
+#pragma unroll
+          for (int c = 0; c < NC; ++c)
+#pragma unroll
+               for (int a = 0; a < WM; ++a)
+#pragma unroll
+                    for (int b = 0; b < WN; ++b)
+#pragma unroll
+                         for (int q = 0; q < 8; ++q) acc[c][a][b][q] = v4d{0., 0., 0., 0.};
+
+          for (int k0 = 0; k0 < kmax; k0 += KC) {
+               __syncthreads();
+               // stage A: NC*4 planes x KC rows x TM sizes
+               for (int e = threadIdx.x; e < NC * 4 * KC * (TM / 2); e += 256) {
+                    const int col2 = e % (TM / 2), rest = e / (TM / 2);
+                    const int row = rest % KC, pl = rest / KC;
+                    const int c = pl / 4, comp = pl % 4;
+                    const double2 v = *(const double2 *) (p.A[c] + comp * plane + (size_t) (k0 + row) * Np + it0 + 2 * col2);
+                    *(double2 *) (sA + (pl * KC + row) * SA + 2 * col2) = v;
+               }
+               for (int e = threadIdx.x; e < 2 * KC * (TN / 2); e += 256) {
+                    const int col2 = e % (TN / 2), rest = e / (TN / 2);
+                    const int row = rest % KC, pm = rest / KC;
+                    const double *src = pm ? p.Bm : p.Bp;
+                    const double2 v = *(const double2 *) (src + (size_t) (k0 + row) * Hp + jt0 + 2 * col2);
+                    *(double2 *) (sB + (pm * KC + row) * SB + 2 * col2) = v;
+               }
+               __syncthreads();
+#pragma unroll
+               for (int ks = 0; ks < KC / 4; ++ks) {
+                    const int r = ks * 4 + lg;                   // this lane's k row
+                    const double sg = ((k0 + r) & 1) ? -1. : 1.;   // s_n, n = k0 + r + 1
+                    double bp[WN], bm[WN];
+#pragma unroll
+                    for (int b = 0; b < WN; ++b) {
+                         const int jc = wn * 16 * WN + 16 * b + l16;
+                         bp[b] = sB[r * SB + jc];
+                         bm[b] = sB[(KC + r) * SB + jc];
+                    }
+#pragma unroll
+                    for (int c = 0; c < NC; ++c)
+#pragma unroll
+                         for (int a = 0; a < WM; ++a) {
+                              const int ic = wm * 16 * WM + 16 * a + l16;
+                              const double Sr = sA[((c * 4 + 0) * KC + r) * SA + ic];
+                              const double Si = sA[((c * 4 + 1) * KC + r) * SA + ic];
+                              const double Tr = sA[((c * 4 + 2) * KC + r) * SA + ic];
+                              const double Ti = sA[((c * 4 + 3) * KC + r) * SA + ic];
+#pragma unroll
+                              for (int b = 0; b < WN; ++b) {
+                                   v4d *q = acc[c][a][b];
+                                   q[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sr, bp[b], q[0], 0, 0, 0);
+                                   q[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Si, bp[b], q[1], 0, 0, 0);
+                                   q[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tr, bm[b], q[2], 0, 0, 0);
+                                   q[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti, bm[b], q[3], 0, 0, 0);
+                                   q[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(sg * Sr, bm[b], q[4], 0, 0, 0);
+                                   q[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(sg * Si, bm[b], q[5], 0, 0, 0);
+                                   q[6] = __builtin_amdgcn_mfma_f64_16x16x4f64(sg * Tr, bp[b], q[6], 0, 0, 0);
+                                   q[7] = __builtin_amdgcn_mfma_f64_16x16x4f64(sg * Ti, bp[b], q[7], 0, 0, 0);
+                              }
+                         }
+               }
+          }
+          // epilogue: Stokes elements, weights, reduction over this lane's sizes
+#pragma unroll
+          for (int a = 0; a < WM; ++a)
+#pragma unroll
+               for (int v = 0; v < 4; ++v) {
+                    const int i = it0 + wm * 16 * WM + 16 * a + 4 * lg + v;
+                    double wv[NW > 0 ? NW : 1], wt[NT > 0 ? NT : 1];
+#pragma unroll
+                    for (int c = 0; c < NW; ++c) wv[c] = p.W[c][i];
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) wt[t] = p.WT[t][i];
+                    const double f0 = p.f[i];
+#pragma unroll
+                    for (int b = 0; b < WN; ++b) {
+                         const v4d *q = acc[0][a][b];
+                         double Pp[4], Pm[4];
+                         stokes(q[0][v], q[1][v], q[2][v], q[3][v], Pp);
+                         stokes(q[4][v], q[5][v], q[6][v], q[7][v], Pm);
+#pragma unroll
+                         for (int c = 0; c < NW; ++c)
+#pragma unroll
+                              for (int k = 0; k < 4; ++k) {
+                                   ew[c][b][k] += wv[c] * Pp[k];
+                                   ew[c][b][4 + k] += wv[c] * Pm[k];
+                              }
+#pragma unroll
+                         for (int t = 0; t < NT; ++t) {
+                              const v4d *d = acc[1 + t][a][b];
+                              double dPp[4], dPm[4];
+                              stokes_tan(q[0][v], q[1][v], q[2][v], q[3][v], d[0][v], d[1][v], d[2][v], d[3][v], dPp);
+                              stokes_tan(q[4][v], q[5][v], q[6][v], q[7][v], d[4][v], d[5][v], d[6][v], d[7][v], dPm);
+#pragma unroll
+                              for (int k = 0; k < 4; ++k) {
+                                   ew[NW + t][b][k] += wt[t] * Pp[k] + f0 * dPp[k];
+                                   ew[NW + t][b][4 + k] += wt[t] * Pm[k] + f0 * dPm[k];
+                              }
+                         }
+                         if (p.save2 && i < p.N) {
+                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+                              if (j < p.H) {
+#pragma unroll
+                                   for (int k = 0; k < 4; ++k) {
+                                        p.save2[((size_t) k * p.N + i) * p.Q + p.H + j] = Pp[k];
+                                        p.save2[((size_t) k * p.N + i) * p.Q + p.H - 1 - j] = Pm[k];
+                                   }
+                              }
+                         }
+                    }
+               }
+     }
+     // reduce over the 4 lane groups (same angle column), then over the two
+     // waves sharing the columns, then write this block's partial
+#pragma unroll
+     for (int c = 0; c < NW + NT; ++c)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int q = 0; q < 8; ++q) {
+                    double s = ew[c][b][q];
+                    s += __shfl_xor(s, 16);
+                    s += __shfl_xor(s, 32);
+                    ew[c][b][q] = s;
+               }
+     __syncthreads();
+     if (wm == 1 && lg == 0) {
+#pragma unroll
+          for (int c = 0; c < NW + NT; ++c)
+#pragma unroll
+               for (int b = 0; b < WN; ++b)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) sR[((c * WN + b) * 8 + q) * 32 + wn * 16 + l16] = ew[c][b][q];
+     }
+     __syncthreads();
+     if (wm == 0 && lg == 0) {
+          const int nch = NW + NT;
+          double *out = p.part + (size_t) blockIdx.y * nch * 4 * p.Q;
+#pragma unroll
+          for (int b = 0; b < WN; ++b) {
+               const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+               if (j < p.H) {
+#pragma unroll
+                    for (int c = 0; c < NW + NT; ++c)
+#pragma unroll
+                         for (int k = 0; k < 4; ++k) {
+                              const double vp = ew[c][b][k] + sR[((c * WN + b) * 8 + k) * 32 + wn * 16 + l16];
+                              const double vm = ew[c][b][4 + k] + sR[((c * WN + b) * 8 + 4 + k) * 32 + wn * 16 + l16];
+                              out[((size_t) c * 4 + k) * p.Q + p.H + j] = vp;
+                              out[((size_t) c * 4 + k) * p.Q + p.H - 1 - j] = vm;
+                         }
+               }
+          }
+     }
+}
+
+// Sum of the per-block partials: one thread per (pass channel, element, angle);
+// channel c of the pass goes to channel map.g[c] of the output.
+struct ChMap {
+     int g[6];
+};
+
+__global__ void reduce_parts(const double *part, int nblk, int nchp, int Q, ChMap map, double *out) {
+     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+     const int per = nchp * 4 * Q;
+     if (e >= per) return;
+     double s = 0.;
+     for (int b = 0; b < nblk; ++b) s += part[(size_t) b * per + e];
+     const int c = e / (4 * Q), rem = e % (4 * Q);
+     out[(size_t) map.g[c] * 4 * Q + rem] = s;
+}
+
+// ---------------------------------------------------------------------------
+// Expansion.  For channel c the quadrature-weighted element rows at the
+// mirrored (-) and direct (+) angles of every positive node j:
+//   GSF rows  0..5: F11, F11+F33, F11-F33, F33, F12, F34 at -mu_j
+//            6..11: the same at +mu_j            (misc/rtutil_scat_int.c:173-183)
+//   Legendre 0..5: F11, F11, F33, F33, F12, F34 at -mu_j, 6..11 at +mu_j.
+__global__ void expand_rows(const double *pf, int nch, int Q, int H, int Hp, const double *w, int gsf,
+                            double *E) {
+     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+     const int c = blockIdx.y;
+     if (j >= Hp || c >= nch) return;
+     double *Ec = E + (size_t) c * 12 * Hp;
+     if (j >= H) {
+          for (int r = 0; r < 12; ++r) Ec[r * Hp + j] = 0.;
+          return;
+     }
+     const double *P = pf + (size_t) c * 4 * Q;
+     const int ip = H + j, im = H - 1 - j;
+     const double wq = w[ip];
+     for (int s = 0; s < 2; ++s) {
+          const int a = s ? ip : im;
+          const double f11 = P[a] * wq, f33 = P[Q + a] * wq, f12 = P[2 * Q + a] * wq, f34 = P[3 * Q + a] * wq;
+          double *row = Ec + (size_t) 6 * s * Hp;
+          if (gsf) {
+               row[0 * Hp + j] = f11;
+               row[1 * Hp + j] = f11 + f33;
+               row[2 * Hp + j] = f11 - f33;
+               row[3 * Hp + j] = f33;
+          } else {
+               row[0 * Hp + j] = f11;
+               row[1 * Hp + j] = f11;
+               row[2 * Hp + j] = f33;
+               row[3 * Hp + j] = f33;
+          }
+          row[4 * Hp + j] = f12;
+          row[5 * Hp + j] = f34;
+     }
+}
+
+// One wave per (l, channel): twelve dot products over the positive nodes,
+// folded with (-1)^l for the mirrored half, then the alpha/zeta recombination
+// and the (2l+1)/2 factor (misc/rtutil_scat_int.c:185-213, :436-469).
+__global__ __launch_bounds__(256) void expand_coefs(const double *T, const double *E, int H, int Hp, int C,
+                                                    int gsf, double *coefs) {
+     const int lane = threadIdx.x & 63;
+     const int l = blockIdx.x * 4 + (threadIdx.x >> 6);
+     const int c = blockIdx.y;
+     if (l >= C) return;
+     const size_t plane = (size_t) C * Hp;
+     const double *P00 = T + (size_t) l * Hp, *P02 = T + plane + (size_t) l * Hp;
+     const double *P22 = T + 2 * plane + (size_t) l * Hp, *P2m = T + 3 * plane + (size_t) l * Hp;
+     const double *Ec = E + (size_t) c * 12 * Hp;
+     double d[12];
+#pragma unroll
+     for (int r = 0; r < 12; ++r) d[r] = 0.;
+     for (int j = lane; j < H; j += 64) {
+          const double p00 = P00[j];
+          if (gsf) {
+               const double p02 = P02[j], p22 = P22[j], p2m = P2m[j];
+               d[0] += p00 * Ec[0 * Hp + j];  d[6] += p00 * Ec[6 * Hp + j];
+               d[1] += p2m * Ec[1 * Hp + j];  d[7] += p22 * Ec[7 * Hp + j];
+               d[2] += p22 * Ec[2 * Hp + j];  d[8] += p2m * Ec[8 * Hp + j];
+               d[3] += p00 * Ec[3 * Hp + j];  d[9] += p00 * Ec[9 * Hp + j];
+               d[4] += p02 * Ec[4 * Hp + j];  d[10] += p02 * Ec[10 * Hp + j];
+               d[5] += p02 * Ec[5 * Hp + j];  d[11] += p02 * Ec[11 * Hp + j];
+          } else {
+#pragma unroll
+               for (int r = 0; r < 12; ++r) d[r] += p00 * Ec[r * Hp + j];
+          }
+     }
+#pragma unroll
+     for (int r = 0; r < 12; ++r) {
+          double s = d[r];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+          d[r] = s;
+     }
+     if (lane == 0) {
+          const double sl = (l & 1) ? -1. : 1., fl = (2 * l + 1) / 2.;
+          double e[6];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) e[k] = d[k] * sl + d[6 + k];
+          double *out = coefs + (size_t) c * 6 * C;
+          if (gsf) {
+               const double t = e[1];
+               e[1] = (t + e[2]) / 2.;
+               e[2] = (t - e[2]) / 2.;
+          } else {
+               e[5] = -e[5];
+          }
+#pragma unroll
+          for (int k = 0; k < 6; ++k) out[(size_t) k * C + l] = e[k] * fl;
+     }
+}
+
+// Scattering matrix at user angles from coefficients: one thread per
+// (set, angle) (misc/rtutil_scat_int.c:510-589, :662-710).
+__global__ void phase_at_angles(int nset, int ncoef, int ld, const double *X, int nth, const double *theta,
+                                double *pf, int gsf) {
+     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+     if (e >= nset * nth) return;
+     const int s = e / nth, i = e % nth;
+     const double *x = X + (size_t) s * 6 * ld;
+     const double m = cos(theta[i] * (kPi / 180.)), m2 = m * m;
+     double acc[6] = {0., 0., 0., 0., 0., 0.};
+     if (gsf) {
+          double a00 = 0, b00 = 0, a02 = 0, b02 = 0, a22 = 0, b22 = 0, a2m = 0, b2m = 0;
+          for (int l = 0; l < ncoef; ++l) {
+               double v00, v02, v22, v2m;
+               if (l == 0) { v00 = 1.; v02 = v22 = v2m = 0.; }
+               else if (l == 1) { v00 = m; v02 = v22 = v2m = 0.; }
+               else if (l == 2) {
+                    v00 = (3. * m2 - 1.) / 2.;
+                    v02 = .25 * sqrt(6.) * (m2 - 1.);
+                    v22 = .25 * (1. + m) * (1. + m);
+                    v2m = .25 * (1. - m) * (1. - m);
+               } else {
+                    double L = l - 1, L1 = l, t = 2. * L + 1.;
+                    double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
+                    double c = t * L * L1 * m, k4 = 4. * t, kb = L1 * (L * L - 4.), kd = L * (L1 * L1 - 4.);
+                    v00 = (t * m * b00 - L * a00) / L1;
+                    v02 = (t * m * b02 - sa * a02) / sb;
+                    v22 = ((c - k4) * b22 - kb * a22) / kd;
+                    v2m = ((c + k4) * b2m - kb * a2m) / kd;
+               }
+               acc[0] += x[l] * v00;
+               acc[3] += x[3 * ld + l] * v00;
+               if (l >= 2) {
+                    acc[4] += x[4 * ld + l] * v02;
+                    acc[5] += x[5 * ld + l] * v02;
+                    acc[1] += (x[ld + l] + x[2 * ld + l]) * v22;
+                    acc[2] += (x[ld + l] - x[2 * ld + l]) * v2m;
+               }
+               a00 = b00; b00 = v00; a02 = b02; b02 = v02; a22 = b22; b22 = v22; a2m = b2m; b2m = v2m;
+          }
+          double *o = pf + (size_t) s * 6 * nth;
+          o[i] = acc[0];
+          o[3 * nth + i] = acc[3];
+          o[4 * nth + i] = acc[4];
+          o[5 * nth + i] = acc[5];
+          o[nth + i] = (acc[1] + acc[2]) / 2.;
+          o[2 * nth + i] = (acc[1] - acc[2]) / 2.;
+     } else {
+          double pm = 0., pc = 1.;
+          for (int l = 0; l < ncoef; ++l) {
+               double v;
+               if (l == 0) v = 1.;
+               else if (l == 1) v = m;
+               else {
+                    const double u = m * pc;
+                    v = (l - 1.) / l * (u - pm) + u;
+               }
+               if (l >= 1) { pm = pc; pc = v; }
+               for (int k = 0; k < 6; ++k) acc[k] += x[k * ld + l] * v;
+          }
+          double *o = pf + (size_t) s * 6 * nth;
+          for (int k = 0; k < 6; ++k) o[k * nth + i] = k == 5 ? -acc[k] : acc[k];
+     }
+}
+
+// ---------------------------------------------------------------------------
+// Workspace: device buffers and a stream, reused across calls, one per
+// concurrent caller (a mutex-guarded pool), so the API stays reentrant.
+struct Buf {
+     void *p = nullptr;
+     size_t n = 0;
+     int get(size_t bytes) {
+          if (bytes <= n) return 0;
+          if (p) (void) hipFree(p);
+          p = nullptr;
+          n = 0;
+          bytes = bytes < 256 ? 256 : bytes;
+          if (hipMalloc(&p, bytes) != hipSuccess) return set_err("hipMalloc(%zu bytes) failed", bytes);
+          n = bytes;
+          return 0;
+     }
+     template <class T> T *as() const { return (T *) p; }
+};
+
+struct Workspace {
+     int device = -1;
+     hipStream_t stream = nullptr;
+     hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
+     Buf x, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
+     Buf guess, mu, w, Bp, Bm, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2;
+     // kept for the adjoint of the same call
+     int valid = 0;
+};
+
+pthread_mutex_t g_pool_mu = PTHREAD_MUTEX_INITIALIZER;
+std::vector<Workspace *> g_pool;
+
+
+int ws_acquire(int device, Workspace **out) {
+     pthread_mutex_lock(&g_pool_mu);
+     Workspace *w = nullptr;
+     for (size_t k = 0; k < g_pool.size(); ++k)
+          if (g_pool[k]->device == device) {
+               w = g_pool[k];
+               g_pool.erase(g_pool.begin() + k);
+               break;
+          }
+     pthread_mutex_unlock(&g_pool_mu);
+     HIPCHECK(hipSetDevice(device));
+     if (!w) {
+          w = new Workspace();
+          w->device = device;
+          HIPCHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+          for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
+     }
+     *out = w;
+     return 0;
+}
+
+void ws_release(Workspace *w) {
+     pthread_mutex_lock(&g_pool_mu);
+     g_pool.push_back(w);
+     pthread_mutex_unlock(&g_pool_mu);
+}
+
+struct Timer {
+     Workspace *w;
+     int on;
+     void mark(int id, int end) {
+          if (on) (void) hipEventRecord(w->ev[2 * id + end], w->stream);
+     }
+     void collect(const int *ids, int n) {
+          for (int k = 0; k < LMIE_N_TIMERS; ++k) g_times[k] = 0.;
+          if (!on) return;
+          for (int k = 0; k < n; ++k) {
+               float ms = 0.f;
+               if (hipEventElapsedTime(&ms, w->ev[2 * ids[k]], w->ev[2 * ids[k] + 1]) == hipSuccess)
+                    g_times[ids[k]] = ms;
+          }
+     }
+};
+
+// Launch sequence of the contraction for one pass (up to 2 tangent channels
+// and up to 4 weight channels).
+template <int WM, int WN, int NT, int NW>
+int launch_angular(const AngArgs &a, int n_tiles, hipStream_t s, int *nblk_out) {
+     constexpr int TM = 32 * WM, TN = 32 * WN;
+     (void) TM;
+     const int tpb = 4;
+     const int nblk = (n_tiles + tpb - 1) / tpb;
+     AngArgs b = a;
+     b.tiles_per_block = tpb;
+     dim3 grid(a.Hp / TN, nblk);
+     hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+     HIPCHECK(hipGetLastError());
+     *nblk_out = nblk;
+     return 0;
+}
+
+template <int WM, int WN, int NT>
+int launch_angular_nw(int nw, const AngArgs &a, int n_tiles, hipStream_t s, int *nblk) {
+     switch (nw) {
+     case 0:
+          if (NT == 0) return set_err("internal: contraction pass without channels");
+          return launch_angular<WM, WN, NT, (NT > 0 ? 0 : 1)>(a, n_tiles, s, nblk);
+     case 1: return launch_angular<WM, WN, NT, 1>(a, n_tiles, s, nblk);
+     case 2: return launch_angular<WM, WN, NT, 2>(a, n_tiles, s, nblk);
+     case 3: return launch_angular<WM, WN, NT, 3>(a, n_tiles, s, nblk);
+     default: return launch_angular<WM, WN, NT, 4>(a, n_tiles, s, nblk);
+     }
+}
+
+int launch_coefs(int nt, const CoefArgs &c, hipStream_t s) {
+     dim3 grid(c.Np / 64);
+     switch (nt) {
+     case 0: hipLaunchKernelGGL(mie_coefs<0>, grid, dim3(64), 0, s, c); break;
+     case 1: hipLaunchKernelGGL(mie_coefs<1>, grid, dim3(64), 0, s, c); break;
+     case 2: hipLaunchKernelGGL(mie_coefs<2>, grid, dim3(64), 0, s, c); break;
+     case 3: hipLaunchKernelGGL(mie_coefs<3>, grid, dim3(64), 0, s, c); break;
+     default: hipLaunchKernelGGL(mie_coefs<4>, grid, dim3(64), 0, s, c); break;
+     }
+     HIPCHECK(hipGetLastError());
+     return 0;
+}
+
+// Initial guesses of the Gauss-Legendre roots, cos(t + (n-1)/(8 n^3) / tan t)
+// with t = (4 i + 3) pi / (4 n + 2) (misc/rtutil_math.c:93-97), on the host.
+void gauss_guesses(int n, double *g) {
+     const int ia = 4 * n + 2;
+     const double da = (n - 1) / (8. * pow((double) n, 3));
+     for (int i = 0; i < (n + 1) / 2; ++i) {
+          const double db = (4. * i + 3.) / ia * kPi;
+          g[i] = cos(db + da / tan(db));
+     }
+}
+
+int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
+     const int N = J->n_size, Np = rup(N, TILE);
+     const int n1max = J->n1_max, Q = J->n_qang, C = J->n_coef;
+     const int H = Q / 2, Hp = rup(H > 0 ? H : 1, TILE), Kp = rup(n1max, KC);
+     const int nt = J->n_tan, nw = J->n_wgt, nch = 1 + nt + nw;
+     if (N <= 0) return set_err("no size points");
+     if (Q % 2) return set_err("odd n_qang");
+     for (int i = 0; i < N; ++i)
+          if (J->n1[i] > n1max || J->n1[i] < 1 || J->n2[i] < J->n1[i])
+               return set_err("size point %d: order n1 = %d outside [1, %d] (n2 = %d)", i, J->n1[i], n1max, J->n2[i]);
+     hipStream_t s = w->stream;
+     Timer tm{w, g_timing};
+     w->valid = 0;
+
+     // host-side tables: per-tile contraction depths
+     const int nt32 = Np / 32, nt64 = Np / 64;
+     std::vector<int> n1h(Np, 0), n2h(Np, 0), kt32(nt32), kt64(nt64), kfill(nt64);
+     for (int i = 0; i < N; ++i) { n1h[i] = J->n1[i]; n2h[i] = J->n2[i]; }
+     for (int t = 0; t < nt32; ++t) {
+          int m = 0;
+          for (int i = t * 32; i < t * 32 + 32; ++i) m = n1h[i] > m ? n1h[i] : m;
+          kt32[t] = rup(m, KC);
+     }
+     for (int t = 0; t < nt64; ++t) {
+          kt64[t] = kt32[2 * t] > kt32[2 * t + 1] ? kt32[2 * t] : kt32[2 * t + 1];
+          kfill[t] = kt64[t];
+     }
+     // uploads
+     if (w->x.get(sizeof(double) * Np) || w->f.get(sizeof(double) * Np) || w->n1.get(sizeof(int) * Np) ||
+         w->n2.get(sizeof(int) * Np) || w->kfill.get(sizeof(int) * nt64) || w->ktile32.get(sizeof(int) * nt32) ||
+         w->ktile64.get(sizeof(int) * nt64) || w->dx.get(sizeof(double) * Np * (nt ? nt : 1)) ||
+         w->dm.get(sizeof(double) * 2 * (nt ? nt : 1)) || w->dft.get(sizeof(double) * Np * (nt ? nt : 1)) ||
+         w->dfw.get(sizeof(double) * Np * (nw ? nw : 1)))
+          return -1;
+     HIPCHECK(hipMemsetAsync(w->x.p, 0, sizeof(double) * Np, s));
+     HIPCHECK(hipMemsetAsync(w->f.p, 0, sizeof(double) * Np, s));
+     HIPCHECK(hipMemcpyAsync(w->x.p, J->x, sizeof(double) * N, hipMemcpyHostToDevice, s));
+     HIPCHECK(hipMemcpyAsync(w->f.p, J->f, sizeof(double) * N, hipMemcpyHostToDevice, s));
+     HIPCHECK(hipMemcpyAsync(w->n1.p, n1h.data(), sizeof(int) * Np, hipMemcpyHostToDevice, s));
+     HIPCHECK(hipMemcpyAsync(w->n2.p, n2h.data(), sizeof(int) * Np, hipMemcpyHostToDevice, s));
+     HIPCHECK(hipMemcpyAsync(w->kfill.p, kfill.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
+     HIPCHECK(hipMemcpyAsync(w->ktile32.p, kt32.data(), sizeof(int) * nt32, hipMemcpyHostToDevice, s));
+     HIPCHECK(hipMemcpyAsync(w->ktile64.p, kt64.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
+     if (nt) {
+          HIPCHECK(hipMemsetAsync(w->dx.p, 0, sizeof(double) * Np * nt, s));
+          HIPCHECK(hipMemsetAsync(w->dft.p, 0, sizeof(double) * Np * nt, s));
+          for (int t = 0; t < nt; ++t) {
+               HIPCHECK(hipMemcpyAsync(w->dx.as<double>() + (size_t) t * Np, J->dx + (size_t) t * N,
+                                       sizeof(double) * N, hipMemcpyHostToDevice, s));
+               HIPCHECK(hipMemcpyAsync(w->dft.as<double>() + (size_t) t * Np, J->df_tan + (size_t) t * N,
+                                       sizeof(double) * N, hipMemcpyHostToDevice, s));
+          }
+          HIPCHECK(hipMemcpyAsync(w->dm.p, J->dm, sizeof(double) * 2 * nt, hipMemcpyHostToDevice, s));
+     }
+     if (nw) {
+          HIPCHECK(hipMemsetAsync(w->dfw.p, 0, sizeof(double) * Np * nw, s));
+          for (int t = 0; t < nw; ++t)
+               HIPCHECK(hipMemcpyAsync(w->dfw.as<double>() + (size_t) t * Np, J->df_wgt + (size_t) t * N,
+                                       sizeof(double) * N, hipMemcpyHostToDevice, s));
+     }
+     tm.mark(LMIE_T_TOTAL, 0);
+
+     // angle tables
+     tm.mark(LMIE_T_TABLES, 0);
+     if (Q > 0) {
+          if (w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) ||
+              w->Bp.get(sizeof(double) * (size_t) Kp * Hp) || w->Bm.get(sizeof(double) * (size_t) Kp * Hp))
+               return -1;
+          {
+               std::vector<double> guess(H);
+               gauss_guesses(Q, guess.data());
+               if (w->guess.get(sizeof(double) * H)) return -1;
+               HIPCHECK(hipMemcpyAsync(w->guess.p, guess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
+               HIPCHECK(hipStreamSynchronize(s));   // guess lives on this stack frame
+          }
+          hipLaunchKernelGGL(gauss_nodes, dim3((H + 63) / 64), dim3(64), 0, s, Q, (const double *) w->guess.as<double>(),
+                             w->mu.as<double>(), w->w.as<double>());
+          hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kp, w->mu.as<double>(),
+                             w->Bp.as<double>(), w->Bm.as<double>());
+          if (J->calc_gc || J->calc_lc) {
+               if (w->T.get(sizeof(double) * 4 * (size_t) C * Hp)) return -1;
+               HIPCHECK(hipMemsetAsync(w->T.p, 0, sizeof(double) * 4 * (size_t) C * Hp, s));
+               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, C, w->mu.as<double>(), w->T.as<double>());
+          }
+          HIPCHECK(hipGetLastError());
+     }
+     tm.mark(LMIE_T_TABLES, 1);
+
+     // per-size coefficients (passes of up to 4 tangent channels)
+     const size_t plane = (size_t) Kp * Np, srow = (size_t) (n1max + 1) * Np;
+     const int nt_pass = nt < 4 ? nt : 4;
+     if (w->A.get(sizeof(double) * 4 * plane * (1 + nt)) || w->S.get(sizeof(double) * 4 * Np * (1 + nt)) ||
+         w->scrD.get(sizeof(double) * 2 * srow * (1 + nt_pass)) || w->scrR.get(sizeof(double) * srow * (1 + nt_pass)))
+          return -1;
+     tm.mark(LMIE_T_COEFS, 0);
+     for (int t0 = 0; t0 < (nt ? nt : 1); t0 += 4) {
+          CoefArgs c;
+          memset(&c, 0, sizeof c);
+          c.N = N; c.Np = Np; c.n1max = n1max;
+          c.x = w->x.as<double>(); c.n1 = w->n1.as<int>(); c.n2 = w->n2.as<int>(); c.kfill = w->kfill.as<int>();
+          c.mr = J->mr; c.mi = J->mi;
+          int k = nt - t0 < 4 ? nt - t0 : 4;
+          if (k < 0) k = 0;
+          c.nt = k;
+          c.dx = w->dx.as<double>() + (size_t) t0 * Np;
+          c.dm = w->dm.as<double>() + 2 * t0;
+          c.scrD = w->scrD.as<double>();
+          c.scrR = w->scrR.as<double>();
+          c.A = w->A.as<double>();
+          c.Kp = Kp;
+          c.S = w->S.as<double>();
+          for (int q = 0; q < k; ++q) {
+               c.At[q] = w->A.as<double>() + 4 * plane * (1 + t0 + q);
+               c.St[q] = w->S.as<double>() + (size_t) 4 * Np * (1 + t0 + q);
+          }
+          c.write_base = t0 == 0;
+          if (launch_coefs(k, c, s)) return -1;
+     }
+     tm.mark(LMIE_T_COEFS, 1);
+
+     // series sums: channel 0, then tangent, then weight-only channels
+     {
+          std::vector<DotJob> jobs;
+          const double *S0 = w->S.as<double>(), *f = w->f.as<double>();
+          for (int q = 0; q < 4; ++q) jobs.push_back(DotJob{f, S0 + (size_t) q * Np, nullptr, nullptr});
+          for (int t = 0; t < nt; ++t)
+               for (int q = 0; q < 4; ++q)
+                    jobs.push_back(DotJob{w->dft.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, f,
+                                          w->S.as<double>() + (size_t) 4 * Np * (1 + t) + (size_t) q * Np});
+          for (int t = 0; t < nw; ++t)
+               for (int q = 0; q < 4; ++q)
+                    jobs.push_back(DotJob{w->dfw.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, nullptr, nullptr});
+          if (w->dotjobs.get(sizeof(DotJob) * jobs.size()) || w->dots.get(sizeof(double) * jobs.size())) return -1;
+          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs.data(), sizeof(DotJob) * jobs.size(), hipMemcpyHostToDevice, s));
+          tm.mark(LMIE_T_REDUCE, 0);
+          hipLaunchKernelGGL(weighted_dots, dim3(jobs.size()), dim3(256), 0, s, w->dotjobs.as<DotJob>(), Np,
+                             w->dots.as<double>());
+          HIPCHECK(hipGetLastError());
+     }
+
+     // angular contraction and reduction
+     if (Q > 0) {
+          if (w->pf.get(sizeof(double) * (size_t) nch * 4 * Q)) return -1;
+          if (J->save_control == 1 && w->save2.get(sizeof(double) * (size_t) 4 * N * Q)) return -1;
+          // pass plan: pass 0 carries channel 0, up to 2 tangent and 3 weight
+          // channels; further passes carry the rest (recomputing the base sums)
+          struct Pass { int ntp, nwp, t0, w0, nblk; size_t off; };
+          std::vector<Pass> plan;
+          size_t tot = 0;
+          for (int td = 0, wd = 0, ps = 0; ps == 0 || td < nt || wd < nw; ++ps) {
+               Pass P;
+               P.ntp = nt - td < 2 ? nt - td : 2;
+               const int base = ps == 0 ? 1 : 0;
+               const int ww = nw - wd < 4 - base ? nw - wd : 4 - base;
+               P.nwp = base + ww;
+               P.t0 = td;
+               P.w0 = wd;
+               P.nblk = ((Np / (P.ntp ? 32 : 64)) + 3) / 4;
+               P.off = tot;
+               tot += (size_t) P.nblk * (P.nwp + P.ntp) * 4 * Q;
+               plan.push_back(P);
+               td += P.ntp;
+               wd += ww;
+          }
+          if (w->part.get(sizeof(double) * tot)) return -1;
+          tm.mark(LMIE_T_ANGULAR, 0);
+          for (size_t ps = 0; ps < plan.size(); ++ps) {
+               Pass &P = plan[ps];
+               AngArgs a;
+               memset(&a, 0, sizeof a);
+               a.Np = Np; a.Kp = Kp; a.Hp = Hp; a.H = H; a.Q = Q; a.N = N;
+               a.A[0] = w->A.as<double>();
+               for (int q = 0; q < P.ntp; ++q) a.A[1 + q] = w->A.as<double>() + 4 * plane * (1 + P.t0 + q);
+               a.Bp = w->Bp.as<double>(); a.Bm = w->Bm.as<double>();
+               a.f = w->f.as<double>();
+               int wi = 0;
+               if (ps == 0) a.W[wi++] = w->f.as<double>();
+               for (int q = 0; wi < P.nwp; ++q) a.W[wi++] = w->dfw.as<double>() + (size_t) (P.w0 + q) * Np;
+               for (int q = 0; q < P.ntp; ++q) a.WT[q] = w->dft.as<double>() + (size_t) (P.t0 + q) * Np;
+               a.part = w->part.as<double>() + P.off;
+               a.save2 = (ps == 0 && J->save_control == 1) ? w->save2.as<double>() : nullptr;
+               int nblk = 0, rc;
+               if (P.ntp == 0) {
+                    a.ktile = w->ktile64.as<int>();
+                    rc = launch_angular_nw<2, 1, 0>(P.nwp, a, Np / 64, s, &nblk);
+               } else {
+                    a.ktile = w->ktile32.as<int>();
+                    rc = P.ntp == 1 ? launch_angular_nw<1, 1, 1>(P.nwp, a, Np / 32, s, &nblk)
+                                    : launch_angular_nw<1, 1, 2>(P.nwp, a, Np / 32, s, &nblk);
+               }
+               if (rc) return -1;
+               if (nblk != P.nblk) return set_err("internal: partial block count");
+          }
+          tm.mark(LMIE_T_ANGULAR, 1);
+          tm.mark(LMIE_T_REDUCE, 0);
+          for (size_t ps = 0; ps < plan.size(); ++ps) {
+               const Pass &P = plan[ps];
+               ChMap map;
+               int c = 0;
+               if (ps == 0) map.g[c++] = 0;
+               for (int q = 0; c < P.nwp; ++q) map.g[c++] = 1 + nt + P.w0 + q;
+               for (int q = 0; q < P.ntp; ++q) map.g[c++] = 1 + P.t0 + q;
+               const int per = (P.nwp + P.ntp) * 4 * Q;
+               hipLaunchKernelGGL(reduce_parts, dim3((per + 255) / 256), dim3(256), 0, s,
+                                  (const double *) (w->part.as<double>() + P.off), P.nblk, P.nwp + P.ntp, Q, map,
+                                  w->pf.as<double>());
+          }
+          HIPCHECK(hipGetLastError());
+          tm.mark(LMIE_T_REDUCE, 1);
+
+          // expansion
+          tm.mark(LMIE_T_EXPAND, 0);
+          if (w->E.get(sizeof(double) * (size_t) nch * 12 * Hp) || w->coefs.get(sizeof(double) * (size_t) nch * 6 * C))
+               return -1;
+          for (int gsf = 1; gsf >= 0; --gsf) {
+               double *dst = gsf ? R->gc_raw : R->lc_raw;
+               if (!(gsf ? J->calc_gc : J->calc_lc) || !dst) continue;
+               hipLaunchKernelGGL(expand_rows, dim3(Hp / 64, nch), dim3(64), 0, s, (const double *) w->pf.as<double>(),
+                                  nch, Q, H, Hp, (const double *) w->w.as<double>(), gsf, w->E.as<double>());
+               hipLaunchKernelGGL(expand_coefs, dim3((C + 3) / 4, nch), dim3(256), 0, s, (const double *) w->T.as<double>(),
+                                  (const double *) w->E.as<double>(), H, Hp, C, gsf, w->coefs.as<double>());
+               HIPCHECK(hipGetLastError());
+               HIPCHECK(hipMemcpyAsync(dst, w->coefs.p, sizeof(double) * (size_t) nch * 6 * C, hipMemcpyDeviceToHost, s));
+          }
+          tm.mark(LMIE_T_EXPAND, 1);
+          if (J->save_control == 1 && R->save2)
+               HIPCHECK(hipMemcpyAsync(R->save2, w->save2.p, sizeof(double) * (size_t) 4 * N * Q, hipMemcpyDeviceToHost, s));
+     } else {
+          tm.mark(LMIE_T_REDUCE, 1);
+     }
+     tm.mark(LMIE_T_TOTAL, 1);
+     // results
+     std::vector<double> dots((size_t) 4 * nch);
+     HIPCHECK(hipMemcpyAsync(dots.data(), w->dots.p, sizeof(double) * 4 * nch, hipMemcpyDeviceToHost, s));
+     if (J->save_control == 1 && R->save1)
+          for (int q = 0; q < 4; ++q)
+               HIPCHECK(hipMemcpyAsync(R->save1 + (size_t) q * N, w->S.as<double>() + (size_t) q * Np,
+                                       sizeof(double) * N, hipMemcpyDeviceToHost, s));
+     HIPCHECK(hipStreamSynchronize(s));
+     for (int q = 0; q < 4; ++q) R->sums[q] = dots[q];
+     for (int c = 1; c < nch; ++c)
+          for (int q = 0; q < 4; ++q) R->sums_ch[4 * (c - 1) + q] = dots[4 * c + q];
+     const int ids[] = {LMIE_T_TABLES, LMIE_T_COEFS, LMIE_T_ANGULAR, LMIE_T_REDUCE, LMIE_T_EXPAND, LMIE_T_TOTAL};
+     tm.collect(ids, Q > 0 ? 6 : 2);
+     w->valid = 1;
+     return 0;
+}
+
+// Lane-layout check of the FP64 matrix instruction used by the contraction:
+// D = A (16x4) B (4x16), all row-major, one wave.
+__global__ void mfma_f64_probe(const double *A, const double *B, double *D) {
+     const int l = threadIdx.x;
+     const double a = A[(l & 15) * 4 + (l >> 4)];
+     const double b = B[(l >> 4) * 16 + (l & 15)];
+     v4d c = {0., 0., 0., 0.};
+     c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+     for (int v = 0; v < 4; ++v) D[(4 * (l >> 4) + v) * 16 + (l & 15)] = c[v];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lmie_gpu_last_error(void) { return g_err; }
+
+int lmie_set_timing(int on) {
+     g_timing = on ? 1 : 0;
+     return 0;
+}
+
+int lmie_get_timing(double *ms, int n) {
+     for (int k = 0; k < n && k < LMIE_N_TIMERS; ++k) ms[k] = g_times[k];
+     return 0;
+}
+
+int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
+     g_err[0] = 0;
+     if (!job || !res) return set_err("null argument");
+     if (job->precision != LMIE_PRECISION_FP64 && job->precision != LMIE_PRECISION_MIXED)
+          return set_err("unknown precision %d", job->precision);
+     Workspace *w = nullptr;
+     if (ws_acquire(job->device, &w)) return -1;
+     int rc = forward_impl(job, res, w);
+     if (rc) (void) hipStreamSynchronize(w->stream);
+     ws_release(w);
+     return rc;
+}
+
+int lmie_gpu_adjoint(const lmie_gpu_job *job, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
+     (void) job; (void) in; (void) out;
+     return set_err("adjoint pass not available in this build");
+}
+
+int lmie_gpu_phase_func(int n_set, int n_coef, int ld, const double *coefs, int n_theta, const double *theta,
+                        double *pf, int gsf, int device) {
+     g_err[0] = 0;
+     if (n_set <= 0 || n_theta <= 0) return 0;
+     if (n_coef > ld || n_coef < 0) return set_err("n_coef %d outside [0, %d]", n_coef, ld);
+     HIPCHECK(hipSetDevice(device));
+     double *dX = nullptr, *dT = nullptr, *dP = nullptr;
+     const size_t nx = (size_t) n_set * 6 * ld, np = (size_t) n_set * 6 * n_theta;
+     HIPCHECK(hipMalloc(&dX, sizeof(double) * (nx ? nx : 1)));
+     HIPCHECK(hipMalloc(&dT, sizeof(double) * n_theta));
+     HIPCHECK(hipMalloc(&dP, sizeof(double) * np));
+     HIPCHECK(hipMemcpy(dX, coefs, sizeof(double) * nx, hipMemcpyHostToDevice));
+     HIPCHECK(hipMemcpy(dT, theta, sizeof(double) * n_theta, hipMemcpyHostToDevice));
+     const int tot = n_set * n_theta;
+     hipLaunchKernelGGL(phase_at_angles, dim3((tot + 127) / 128), dim3(128), 0, 0, n_set, n_coef, ld,
+                        (const double *) dX, n_theta, (const double *) dT, dP, gsf);
+     HIPCHECK(hipGetLastError());
+     HIPCHECK(hipMemcpy(pf, dP, sizeof(double) * np, hipMemcpyDeviceToHost));
+     HIPCHECK(hipFree(dX));
+     HIPCHECK(hipFree(dT));
+     HIPCHECK(hipFree(dP));
+     return 0;
+}
+
+int lmie_gpu_gauss_rule(int n, double *x, double *wt, int device) {
+     if (n < 1) return set_err("n < 1");
+     HIPCHECK(hipSetDevice(device));
+     std::vector<double> g((n + 1) / 2);
+     gauss_guesses(n, g.data());
+     double *d = nullptr;
+     HIPCHECK(hipMalloc(&d, sizeof(double) * (g.size() + 2 * (size_t) n)));
+     HIPCHECK(hipMemcpy(d, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice));
+     hipLaunchKernelGGL(gauss_nodes, dim3(((n + 1) / 2 + 63) / 64), dim3(64), 0, 0, n, (const double *) d,
+                        d + g.size(), d + g.size() + n);
+     HIPCHECK(hipGetLastError());
+     HIPCHECK(hipMemcpy(x, d + g.size(), sizeof(double) * n, hipMemcpyDeviceToHost));
+     HIPCHECK(hipMemcpy(wt, d + g.size() + n, sizeof(double) * n, hipMemcpyDeviceToHost));
+     HIPCHECK(hipFree(d));
+     return 0;
+}
+
+int lmie_gpu_selftest_mfma_f64(const double *A, const double *B, double *D, int device) {
+     HIPCHECK(hipSetDevice(device));
+     double *d = nullptr;
+     HIPCHECK(hipMalloc(&d, sizeof(double) * (64 + 64 + 256)));
+     HIPCHECK(hipMemcpy(d, A, sizeof(double) * 64, hipMemcpyHostToDevice));
+     HIPCHECK(hipMemcpy(d + 64, B, sizeof(double) * 64, hipMemcpyHostToDevice));
+     hipLaunchKernelGGL(mfma_f64_probe, dim3(1), dim3(64), 0, 0, (const double *) d, (const double *) (d + 64), d + 128);
+     HIPCHECK(hipGetLastError());
+     HIPCHECK(hipMemcpy(D, d + 128, sizeof(double) * 256, hipMemcpyDeviceToHost));
+     HIPCHECK(hipFree(d));
+     return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,1070 @@
+/*
+ * lmie_host.c -- host C layer of liblmie_amd.so.
+ *
+ * Implements the drop-in API of include/lmie_interface.h and the extensions of
+ * include/lmie_hip.h.  Everything that is O(N * n_derivs) or smaller runs
+ * here: the size grid and number density with their tangents
+ * (misc/rtutil_size_dist.c:20-646), the distribution statistics
+ * (misc/rtutil_size_dist.c:653-794), the split of the seeds into Mie tangents
+ * and weight-only ones (src/lmie_interface.c:304-353), the per-size orders,
+ * and the normalisation of the raw sums and expansions
+ * (src/lmie_schedule.c:1155-1241).  Everything that scales with the number of
+ * orders or angles runs in the kernels behind include/lmie_gpu.h.
+ */
+#include <complex.h>
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/lmie_interface.h"
+#include "../../include/lmie_hip.h"
+#include "../../include/lmie_gpu.h"
+
+#define LM_PI 3.14159265358979323846264338327950288
+
+/* ------------------------------------------------------------------------- */
+/* errors: one message per thread; -1 returns leave a breadcrumb like the
+   reference's fprintf(stderr, "ERROR: ...") chain. */
+static __thread char g_err[512];
+
+static int fail(const char *where, const char *why) {
+     char tmp[400];
+     snprintf(tmp, sizeof tmp, "%s", why ? why : "");     /* why may be g_err itself */
+     snprintf(g_err, sizeof g_err, "%s: %s", where, tmp);
+     fprintf(stderr, "ERROR: %s\n", g_err);
+     return -1;
+}
+
+const char *lmie_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------------------- */
+/* Pointer-table blocks (misc/gutil_alloc.c:19-75 layout): row pointers, then
+   contiguous row-major data, one malloc, released with free(). */
+static double **table2(int n0, int n1) {
+     size_t head = ((size_t) n0 * sizeof(double *) + 63) & ~(size_t) 63;
+     char *m = calloc(1, head + (size_t) n0 * n1 * sizeof(double) + 64);
+     if (!m) return NULL;
+     double **p = (double **) m;
+     for (int i = 0; i < n0; ++i) p[i] = (double *) (m + head) + (size_t) i * n1;
+     return p;
+}
+
+static double ***table3(int n0, int n1, int n2) {
+     size_t h0 = (size_t) n0 * sizeof(double **), h1 = (size_t) n0 * n1 * sizeof(double *);
+     size_t head = (h0 + h1 + 63) & ~(size_t) 63;
+     char *m = calloc(1, head + (size_t) n0 * n1 * n2 * sizeof(double) + 64);
+     if (!m) return NULL;
+     double ***p = (double ***) m;
+     double **q = (double **) (m + h0);
+     for (int i = 0; i < n0; ++i) {
+          p[i] = q + (size_t) i * n1;
+          for (int j = 0; j < n1; ++j)
+               p[i][j] = (double *) (m + head) + ((size_t) i * n1 + j) * n2;
+     }
+     return p;
+}
+
+static double *vec(size_t n) { return calloc(n ? n : 1, sizeof(double)); }
+
+/* ------------------------------------------------------------------------- */
+/* Orders (src/lmie_core.c:18-38).  Computed here, once per size, so host and
+   kernels agree on every integer. */
+static int mie_n1(double x) { return (int) (x + 4.05 * pow(x, 1. / 3.) + 8); }
+
+static int mie_n2(int n1, double x, double mr, double mi) {
+     double az = cabs((mr - I * mi) * x);
+     double m = n1 > az ? (double) n1 : az;
+     return (int) (m + 6.40 * pow(m, 1. / 3.) + 8);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Gauss-Legendre rule on [-1, 1] for the size grid: Newton iteration with a
+   second-order correction from the initial guess of misc/rtutil_math.c:73-129. */
+static void gl_rule(int n, double *x, double *w) {
+     for (int i = 0, k = n - 1; i < (n + 1) / 2; ++i, --k) {
+          double t = (4. * i + 3.) / (4 * n + 2) * LM_PI;
+          double y = cos(t + (n - 1) / (8. * pow((double) n, 3)) / tan(t)), y0, pa, pb;
+          do {
+               double pm = 0.;
+               pa = 1.;
+               pb = 0.;
+               for (int j = 0; j < n; ++j) {
+                    double u;
+                    pm = pb;
+                    pb = pa;
+                    u = y * pb;
+                    pa = j / (j + 1.) * (u - pm) + u;
+               }
+               double s = 1. - y * y;
+               double d1 = n * (pb - y * pa) / s;
+               double d2 = (2. * y * d1 - n * (n + 1) * pa) / s;
+               double h = pa / d1;
+               y0 = y;
+               y = y0 - h * (1. + h * d2 / (2. * d1));
+          } while (fabs(y - y0) > DBL_EPSILON);
+          x[i] = -y;
+          x[k] = y;
+          double c = n * pb;
+          c = 2. / (c * c);
+          w[i] = (1. - y * y) * c;
+          w[k] = w[i];
+     }
+}
+
+/* Seed columns of the size grid: a[p][d] for a1..a5, r1[d], r2[d]. */
+typedef struct {
+     int nd;
+     const double *a[5];
+     const double *r1, *r2;
+} seeds;
+
+static double sd(const double *v, int d) { return v ? v[d] : 0.; }
+
+/* n_int equal intervals on [r1, r2] with n_quad nodes each; the map of the
+   [-1, 1] rule onto each interval follows misc/rtutil_math.c:329-387 (the
+   mirrored node sets the offset). */
+static void grid_intervals(int n_int, int n_quad, int nd, double r1, double r2,
+                           const double *r1_l, const double *r2_l,
+                           double *qx, double *qw, double *qx_l, double *qw_l) {
+     double *tx = vec(n_quad), *tw = vec(n_quad);
+     gl_rule(n_quad, tx, tw);
+     /* the reference scales the [-1,1] weights by the half-width of each
+        interval with its own weight formula ((half - off*y) * c); on [-1, 1]
+        that is (1 - y^2) * c, as gl_rule computes */
+     double step = (r2 - r1) / n_int;
+     for (int s = 0; s < n_int; ++s) {
+          double lo = r1 + s * step, hi = lo + step;
+          double mid = (lo + hi) / 2., half = (hi - lo) / 2.;
+          for (int i = 0, k = n_quad - 1; i < (n_quad + 1) / 2; ++i, --k) {
+               int a = s * n_quad + i, b = s * n_quad + k;
+               qx[a] = mid - half * tx[k];
+               qx[b] = mid + half * tx[k];
+               qw[a] = qw[b] = tw[i] * half;
+               for (int d = 0; d < nd; ++d) {
+                    double st_l = (sd(r2_l, d) - sd(r1_l, d)) / n_int;
+                    double lo_l = sd(r1_l, d) + s * st_l, hi_l = lo_l + st_l;
+                    double mid_l = (lo_l + hi_l) / 2., half_l = (hi_l - lo_l) / 2.;
+                    qx_l[(size_t) a * nd + d] = mid_l - half_l * tx[k];
+                    qx_l[(size_t) b * nd + d] = mid_l + half_l * tx[k];
+                    qw_l[(size_t) a * nd + d] = qw_l[(size_t) b * nd + d] = tw[i] * half_l;
+               }
+          }
+     }
+     free(tx);
+     free(tw);
+}
+
+static int grid_points(int dist, int n_int1, int n_int2, int n_quad) {
+     switch (dist) {
+     case SIZE_DIST_MONO: return 1;
+     case SIZE_DIST_MODIFIED_POWER_LAW: return (n_int1 + n_int2) * n_quad;
+     case SIZE_DIST_MODIFIED_GAMMA: case SIZE_DIST_LOG_NORMAL: case SIZE_DIST_POWER_LAW:
+     case SIZE_DIST_GAMMA: case SIZE_DIST_MODIFIED_BIMODAL_LOG_NORMAL: case SIZE_DIST_EXPONENTIAL:
+          return n_int2 * n_quad;
+     default: return -1;
+     }
+}
+
+/* Brent root bracketing for the power-law range (the role of misc/zeroin.c). */
+static double pl_f(double r1, double a1, double a2) {
+     double r2 = (1. + a2) * 2. * a1 - r1;
+     return (r2 - r1) / log(r2 / r1) - a1;
+}
+
+static void pl_range(double a1, double a2, double *r1, double *r2) {
+     double a = DBL_EPSILON, b = a1, fa = pl_f(a, a1, a2), fb = pl_f(b, a1, a2);
+     double c = a, fc = fa, d = b - a, e = d;
+     for (int it = 0; it < 400; ++it) {
+          if ((fb > 0 && fc > 0) || (fb < 0 && fc < 0)) { c = a; fc = fa; d = e = b - a; }
+          if (fabs(fc) < fabs(fb)) { a = b; b = c; c = a; fa = fb; fb = fc; fc = fa; }
+          double tol = 2. * DBL_EPSILON * fabs(b), m = .5 * (c - b);
+          if (fabs(m) <= tol || fb == 0.) break;
+          if (fabs(e) >= tol && fabs(fa) > fabs(fb)) {
+               double s = fb / fa, p, q, r;
+               if (a == c) { p = 2. * m * s; q = 1. - s; }
+               else {
+                    q = fa / fc; r = fb / fc;
+                    p = s * (2. * m * q * (q - r) - (b - a) * (r - 1.));
+                    q = (q - 1.) * (r - 1.) * (s - 1.);
+               }
+               if (p > 0) q = -q; else p = -p;
+               if (2. * p < 3. * m * q - fabs(tol * q) && p < fabs(.5 * e * q)) { e = d; d = p / q; }
+               else { d = m; e = d; }
+          } else { d = m; e = d; }
+          a = b; fa = fb;
+          b += fabs(d) > tol ? d : (m > 0 ? tol : -tol);
+          fb = pl_f(b, a1, a2);
+     }
+     *r1 = b;
+     *r2 = (1. + a2) * 2. * a1 - b;
+}
+
+typedef struct {
+     int N, nd;
+     double *qx, *qw, *nr, *qx_l, *qw_l, *nr_l;    /* [N], [N][nd] */
+     double r1, r2, *r1_l, *r2_l, norm, *norm_l;
+} grid_t;
+
+static void grid_free(grid_t *g) {
+     free(g->qx); free(g->qw); free(g->nr); free(g->qx_l); free(g->qw_l); free(g->nr_l);
+     free(g->r1_l); free(g->r2_l); free(g->norm_l);
+}
+
+/* Grid, number density normalised to unit integral, and tangents along each
+   seed column (misc/rtutil_size_dist.c:200-646). */
+static int grid_build(int dist, int n_int1, int n_int2, int n_quad, const double a[5],
+                      double r1, double r2, const seeds *sv, grid_t *g) {
+     int nd = sv->nd, N = grid_points(dist, n_int1, n_int2, n_quad);
+     if (N <= 0) return fail("size distribution", "invalid distribution type");
+     size_t L = (size_t) N * (nd > 0 ? nd : 1);
+     g->N = N; g->nd = nd;
+     g->qx = vec(N); g->qw = vec(N); g->nr = vec(N);
+     g->qx_l = vec(L); g->qw_l = vec(L); g->nr_l = vec(L);
+     g->r1_l = vec(nd); g->r2_l = vec(nd); g->norm_l = vec(nd);
+     double *tot_l = vec(nd), tot = 0.;
+     g->r1 = r1; g->r2 = r2;
+     for (int d = 0; d < nd; ++d) { g->r1_l[d] = sd(sv->r1, d); g->r2_l[d] = sd(sv->r2, d); }
+     double a1 = a[0], a2 = a[1], a3 = a[2], a4 = a[3], a5 = a[4];
+
+     if (dist == SIZE_DIST_MONO) {
+          g->r1 = g->r2 = a1;
+          for (int d = 0; d < nd; ++d) { g->r1_l[d] = g->r2_l[d] = 0.; g->qx_l[d] = sd(sv->a[0], d); }
+          g->qx[0] = a1; g->qw[0] = 1.; g->nr[0] = 1.;
+          g->norm = 1.;
+          free(tot_l);
+          return 0;
+     }
+     if (dist == SIZE_DIST_POWER_LAW) {
+          pl_range(a1, a2, &g->r1, &g->r2);
+          double Lg = log(g->r2) - log(g->r1);
+          double p = -Lg + (g->r2 - g->r1) / g->r1, q = Lg - (g->r2 - g->r1) / g->r2;
+          for (int d = 0; d < nd; ++d) {
+               double u = sd(sv->a[0], d) * Lg * Lg;
+               double v = sd(sv->a[1], d) * 2. * a1 + (1. + a2) * 2. * sd(sv->a[0], d);
+               g->r1_l[d] = (q * v - u) / (q - p);
+               g->r2_l[d] = (p * v - u) / (p - q);
+          }
+     }
+     int N1 = 0;
+     if (dist == SIZE_DIST_MODIFIED_POWER_LAW) {
+          N1 = n_int1 * n_quad;
+          grid_intervals(n_int1, n_quad, nd, 0., g->r1, NULL, g->r1_l, g->qx, g->qw, g->qx_l, g->qw_l);
+          grid_intervals(n_int2, n_quad, nd, g->r1, g->r2, g->r1_l, g->r2_l, g->qx + N1, g->qw + N1,
+                         g->qx_l + (size_t) N1 * nd, g->qw_l + (size_t) N1 * nd);
+     } else
+          grid_intervals(n_int2, n_quad, nd, g->r1, g->r2, g->r1_l, g->r2_l, g->qx, g->qw, g->qx_l, g->qw_l);
+
+     for (int i = 0; i < N; ++i) {
+          double r = g->qx[i], lr = log(r), *nl = g->nr_l + (size_t) i * nd;
+          const double *xl = g->qx_l + (size_t) i * nd;
+          double n = 0.;
+          switch (dist) {
+          case SIZE_DIST_MODIFIED_GAMMA: {
+               double b = pow(a2, a3), c = a3 * b, h = pow(r, a3);
+               n = pow(r, a1) * exp(-a1 / c * h);
+               for (int d = 0; d < nd; ++d) {
+                    double s1 = sd(sv->a[0], d), s2 = sd(sv->a[1], d), s3 = sd(sv->a[2], d);
+                    double lc = s3 / a3 + s2 * (a3 / a2) + s3 * log(a2);    /* d ln c */
+                    double px = xl[d] / r;
+                    nl[d] = n * ((px * a1 + s1 * lr) + h * ((-s1 + a1 * (-(px * a3 + s3 * lr) + lc)) / c));
+               }
+               break;
+          }
+          case SIZE_DIST_LOG_NORMAL: {
+               double dv = lr - log(a1), e = dv * dv / (2. * a2);
+               n = exp(-e) / r;
+               for (int d = 0; d < nd; ++d) {
+                    double px = xl[d] / r;
+                    nl[d] = n * (-px - (2. * dv * (px - sd(sv->a[0], d) / a1) - e * 2. * sd(sv->a[1], d)) / (2. * a2));
+               }
+               break;
+          }
+          case SIZE_DIST_POWER_LAW:
+               n = 1. / pow(r, 3);
+               for (int d = 0; d < nd; ++d) nl[d] = -n * 3. * xl[d] / r;
+               break;
+          case SIZE_DIST_GAMMA: {
+               double ab = a1 * a2, pw = (1. - 3. * a2) / a2, rp = pow(r, pw), ex = exp(-r / ab);
+               n = rp * ex;
+               for (int d = 0; d < nd; ++d) {
+                    double ab_l = sd(sv->a[0], d) * a2 + a1 * sd(sv->a[1], d);
+                    double pw_l = (-3. * sd(sv->a[1], d) - pw * sd(sv->a[1], d)) / a2;
+                    nl[d] = rp * ((xl[d] * (pw / r) + pw_l * lr) + (-xl[d] + r / ab * ab_l) / ab) * ex;
+               }
+               break;
+          }
+          case SIZE_DIST_MODIFIED_POWER_LAW:
+               if (i < N1) {
+                    n = 1.;
+               } else {
+                    double u = r / g->r1;
+                    n = pow(u, a1);
+                    for (int d = 0; d < nd; ++d)
+                         nl[d] = n * ((xl[d] - u * g->r1_l[d]) / g->r1 * a1 / u + sd(sv->a[0], d) * log(u));
+               }
+               break;
+          case SIZE_DIST_MODIFIED_BIMODAL_LOG_NORMAL: {
+               double d1 = lr - log(a1), e1 = d1 * d1 / (2. * a2);
+               double d2 = lr - log(a3), e2 = d2 * d2 / (2. * a4);
+               double o1 = exp(-e1), o2 = exp(-e2), r4 = r * r * r * r;
+               n = (o1 + a5 * o2) / r4;
+               for (int d = 0; d < nd; ++d) {
+                    double px = xl[d] / r;
+                    double h1 = ((-px + sd(sv->a[0], d) / a1) * 2. * d1 + e1 * 2. * sd(sv->a[1], d)) * o1 / (2. * a2);
+                    double h2 = ((-px + sd(sv->a[2], d) / a3) * 2. * d2 + e2 * 2. * sd(sv->a[3], d)) * o2 / (2. * a4);
+                    nl[d] = (h1 + sd(sv->a[4], d) * o2 + a5 * h2 - n * 4. * r * r * r * xl[d]) / r4;
+               }
+               break;
+          }
+          case SIZE_DIST_EXPONENTIAL:
+               /* the reference defines no tangents for this kind */
+               n = 2. * exp(-a1 * 2. * r);
+               break;
+          }
+          g->nr[i] = n;
+     }
+     for (int i = 0; i < N; ++i) {
+          tot += g->nr[i] * g->qw[i];
+          if (dist != SIZE_DIST_EXPONENTIAL)
+               for (int d = 0; d < nd; ++d)
+                    tot_l[d] += g->nr_l[(size_t) i * nd + d] * g->qw[i] + g->nr[i] * g->qw_l[(size_t) i * nd + d];
+     }
+     g->norm = tot;
+     for (int d = 0; d < nd; ++d) g->norm_l[d] = tot_l[d];
+     for (int i = 0; i < N; ++i) {
+          g->nr[i] /= tot;
+          for (int d = 0; d < nd; ++d)
+               g->nr_l[(size_t) i * nd + d] = (g->nr_l[(size_t) i * nd + d] - g->nr[i] * tot_l[d]) / tot;
+     }
+     free(tot_l);
+     return 0;
+}
+
+/* Distribution statistics and tangents (misc/rtutil_size_dist.c:653-794). */
+typedef struct { double reff, veff, gavg, vavg, ravg, rvw; } dstats;
+
+static void grid_stats(const grid_t *g, dstats *s, dstats *sl /* [nd] or NULL */) {
+     int N = g->N, nd = g->nd;
+     double m[4] = {0, 0, 0, 0};
+     double *ml = vec((size_t) 4 * (nd ? nd : 1));
+     for (int i = 0; i < N; ++i) {
+          double t = g->nr[i] * g->qx[i] * g->qw[i];
+          for (int k = 0; k < 4; ++k) { m[k] += t; t *= g->qx[i]; }
+          for (int d = 0; d < nd; ++d) {
+               size_t o = (size_t) i * nd + d;
+               double h = g->qx[i] * g->qw[i] * g->nr_l[o], p = g->nr[i] * g->qw[i] * g->qx_l[o];
+               double q = g->nr[i] * g->qx[i] * g->qw_l[o], rp = 1.;
+               for (int k = 0; k < 4; ++k) { ml[k * nd + d] += (h + (k + 1) * p + q) * rp; rp *= g->qx[i]; }
+          }
+     }
+     s->gavg = m[1] * LM_PI;
+     s->vavg = 4. / 3. * LM_PI * m[2];
+     s->ravg = m[0];
+     s->rvw = 4. / 3. * LM_PI * m[3] / s->vavg;
+     s->reff = m[2] / m[1];
+     for (int d = 0; d < nd && sl; ++d) {
+          sl[d].gavg = ml[nd + d] * LM_PI;
+          sl[d].vavg = 4. / 3. * LM_PI * ml[2 * nd + d];
+          sl[d].ravg = ml[d];
+          sl[d].rvw = 4. / 3. * LM_PI * (ml[3 * nd + d] / s->vavg - m[3] * sl[d].vavg / (s->vavg * s->vavg));
+          sl[d].reff = ml[2 * nd + d] / m[1] - m[2] * ml[nd + d] / (m[1] * m[1]);
+     }
+     double v = 0., *vl = vec(nd);
+     for (int i = 0; i < N; ++i) {
+          double dr = g->qx[i] - s->reff, r2w = g->qx[i] * g->qx[i] * g->qw[i];
+          v += g->nr[i] * dr * dr * r2w;
+          for (int d = 0; d < nd && sl; ++d) {
+               size_t o = (size_t) i * nd + d;
+               vl[d] += g->nr_l[o] * dr * dr * r2w + g->nr[i] *
+                        (2. * dr * r2w * (g->qx_l[o] - sl[d].reff) +
+                         dr * dr * (2. * g->qx[i] * g->qw[i] * g->qx_l[o] + g->qx[i] * g->qx[i] * g->qw_l[o]));
+          }
+     }
+     double b = s->gavg * s->reff * s->reff;
+     s->veff = v * LM_PI / b;
+     for (int d = 0; d < nd && sl; ++d) {
+          double q = s->veff * s->reff / b;
+          sl[d].veff = vl[d] * LM_PI / b - (sl[d].gavg * q * s->reff + q * s->gavg * 2. * sl[d].reff);
+     }
+     free(ml);
+     free(vl);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Core: raw sums on the GPU, normalisation and expansion bookkeeping here
+   (src/lmie_schedule.c:942-1268).  Seeds as columns d of [N][nd] arrays. */
+typedef struct {
+     int N, nd, nd1, nd2, *idx1, *idx2, save_control;
+     double lambda, mr, mi;
+     const double *lambda_l, *mr_l, *mi_l;      /* [nd] */
+     const double *qx, *qw, *nr;                /* [N]  */
+     const double *qx_l, *qw_l, *nr_l;          /* [N][nd] */
+     double r2;
+     int calc_gc, calc_lc;
+     double accuracy;
+     const lmie_opts *opts;
+} core_args;
+
+typedef struct {
+     double *cext, *csca, *cbak, *g;
+     double **gc, **lc;
+     double *cext_l, *csca_l, *cbak_l, *g_l;
+     double ***gc_l, ***lc_l;
+     double ***save1, ****save2;
+     int n_qang, n_coef;
+     double sums[4];                /* raw sums, kept for the adjoint */
+} core_out;
+
+static void fill_job(const core_args *c, lmie_gpu_job *j, double *x, double *f, int *n1, int *n2,
+                     double *dx, double *dm, double *dft, double *dfw) {
+     double x2 = 2. * LM_PI * c->r2 / c->lambda;
+     int n1m = mie_n1(x2);
+     memset(j, 0, sizeof *j);
+     for (int i = 0; i < c->N; ++i) {
+          x[i] = 2. * LM_PI * c->qx[i] / c->lambda;
+          f[i] = c->nr[i] * c->qw[i];
+          n1[i] = mie_n1(x[i]);
+          n2[i] = mie_n2(n1[i], x[i], c->mr, c->mi);
+          for (int t = 0; t < c->nd1; ++t) {
+               int d = c->idx1[t];
+               size_t o = (size_t) i * c->nd + d;
+               dx[(size_t) t * c->N + i] = 2. * LM_PI * (c->qx_l[o] - c->qx[i] * c->lambda_l[d] / c->lambda) / c->lambda;
+               dft[(size_t) t * c->N + i] = c->nr_l[o] * c->qw[i] + c->nr[i] * c->qw_l[o];
+          }
+          for (int t = 0; t < c->nd2; ++t) {
+               size_t o = (size_t) i * c->nd + c->idx2[t];
+               dfw[(size_t) t * c->N + i] = c->nr_l[o] * c->qw[i];
+          }
+     }
+     for (int t = 0; t < c->nd1; ++t) {
+          dm[2 * t] = c->mr_l[c->idx1[t]];
+          dm[2 * t + 1] = -c->mi_l[c->idx1[t]];
+     }
+     j->n_size = c->N; j->x = x; j->f = f; j->n1 = n1; j->n2 = n2;
+     j->mr = c->mr; j->mi = c->mi;
+     j->n1_max = n1m;
+     j->n_qang = (c->calc_gc || c->calc_lc) ? 2 * n1m : 0;
+     j->n_coef = 2 * n1m;
+     j->calc_gc = c->calc_gc; j->calc_lc = c->calc_lc;
+     j->n_tan = c->nd1; j->dx = dx; j->dm = dm; j->df_tan = dft;
+     j->n_wgt = c->nd2; j->df_wgt = dfw;
+     j->save_control = c->save_control;
+     j->precision = c->opts ? c->opts->precision : LMIE_PRECISION_FP64;
+     j->device = c->opts ? c->opts->device : 0;
+}
+
+typedef struct {
+     double *x, *f, *dx, *dm, *dft, *dfw;
+     int *n1, *n2;
+} job_mem;
+
+static void job_mem_alloc(job_mem *m, int N, int nd1, int nd2) {
+     m->x = vec(N); m->f = vec(N);
+     m->n1 = calloc(N, sizeof(int)); m->n2 = calloc(N, sizeof(int));
+     m->dx = vec((size_t) nd1 * N); m->dm = vec(2 * (size_t) nd1); m->dft = vec((size_t) nd1 * N);
+     m->dfw = vec((size_t) nd2 * N);
+}
+
+static void job_mem_free(job_mem *m) {
+     free(m->x); free(m->f); free(m->n1); free(m->n2); free(m->dx); free(m->dm); free(m->dft); free(m->dfw);
+}
+
+/* Normalise the expansion of channel c of raw into out (one 6 x n_coef set):
+   value a * raw_c (+ a_l * raw_0 for tangent channels). */
+static void norm_set(const double *raw, int ch, int n_coef, double a, double a_l, double **dst) {
+     const double *r = raw + (size_t) ch * 6 * n_coef, *r0 = raw;
+     for (int k = 0; k < 6; ++k)
+          for (int l = 0; l < n_coef; ++l)
+               dst[k][l] = ch == 0 ? r[k * n_coef + l] * a : r[k * n_coef + l] * a + r0[k * n_coef + l] * a_l;
+}
+
+static int truncated_count(double **coef, int n_coef, double accuracy) {
+     for (int l = 0; l < n_coef; ++l)
+          if (fabs(coef[0][l]) <= accuracy) return l + 1;
+     return n_coef;
+}
+
+static int core_run(const core_args *c, core_out *o) {
+     int nd = c->nd, nch = 1 + c->nd1 + c->nd2;
+     job_mem jm;
+     lmie_gpu_job job;
+     lmie_gpu_result res;
+     job_mem_alloc(&jm, c->N, c->nd1, c->nd2);
+     fill_job(c, &job, jm.x, jm.f, jm.n1, jm.n2, jm.dx, jm.dm, jm.dft, jm.dfw);
+     int C = job.n_coef, Q = job.n_qang;
+     o->n_qang = Q; o->n_coef = C;
+     memset(&res, 0, sizeof res);
+     res.sums_ch = vec((size_t) 4 * (nch - 1));
+     res.gc_raw = c->calc_gc ? vec((size_t) nch * 6 * C) : NULL;
+     res.lc_raw = c->calc_lc ? vec((size_t) nch * 6 * C) : NULL;
+     if (c->save_control == 1) {
+          *o->save1 = table2(4, c->N);
+          *o->save2 = table3(4, c->N, Q > 0 ? Q : 1);
+          res.save1 = (*o->save1)[0];
+          res.save2 = (*o->save2)[0][0];
+     }
+     int rc = lmie_gpu_forward(&job, &res);
+     if (rc) {
+          fail("lmie_gpu_forward", lmie_gpu_last_error());
+          goto done;
+     }
+     /* per-channel raw tangent sums -> seed order */
+     double *rs_l = vec((size_t) 4 * (nd ? nd : 1));
+     for (int t = 0; t < c->nd1; ++t)
+          for (int q = 0; q < 4; ++q) rs_l[4 * c->idx1[t] + q] = res.sums_ch[4 * t + q];
+     for (int t = 0; t < c->nd2; ++t)
+          for (int q = 0; q < 4; ++q) rs_l[4 * c->idx2[t] + q] = res.sums_ch[4 * (c->nd1 + t) + q];
+     double Cext = res.sums[0], Csca = res.sums[1], Cbak = res.sums[2], G = res.sums[3];
+     memcpy(o->sums, res.sums, sizeof o->sums);
+     double a = 2. / Csca, s = c->lambda * c->lambda / (2. * LM_PI);
+     for (int d = 0; d < nd; ++d) {
+          double al = -a * rs_l[4 * d + 1] / Csca;
+          double sl = 2. * c->lambda_l[d] * c->lambda / (2. * LM_PI);
+          o->g_l[d] = rs_l[4 * d + 3] * a + G * al;
+          o->cext_l[d] = rs_l[4 * d + 0] * s + Cext * sl;
+          o->csca_l[d] = rs_l[4 * d + 1] * s + Csca * sl;
+          o->cbak_l[d] = (rs_l[4 * d + 2] * s + Cbak * sl) / (2. * 4. * LM_PI);
+          /* channel of seed d in the raw arrays */
+          int ch = -1;
+          for (int t = 0; t < c->nd1; ++t) if (c->idx1[t] == d) ch = 1 + t;
+          for (int t = 0; t < c->nd2; ++t) if (c->idx2[t] == d) ch = 1 + c->nd1 + t;
+          if (c->calc_gc) { norm_set(res.gc_raw, ch, C, a, al, o->gc_l[d]); o->gc_l[d][0][0] = 0.; }
+          if (c->calc_lc) { norm_set(res.lc_raw, ch, C, a, al, o->lc_l[d]); o->lc_l[d][0][0] = 0.; }
+     }
+     free(rs_l);
+     *o->g = G / Csca * 2.;
+     *o->cext = Cext * s;
+     *o->csca = Csca * s;
+     *o->cbak = Cbak * s / (2. * 4. * LM_PI);
+     rc = 0;
+     if (c->calc_gc) { norm_set(res.gc_raw, 0, C, a, 0., o->gc); rc = truncated_count(o->gc, C, c->accuracy); }
+     if (c->calc_lc) { norm_set(res.lc_raw, 0, C, a, 0., o->lc); rc = truncated_count(o->lc, C, c->accuracy); }
+done:
+     free(res.sums_ch); free(res.gc_raw); free(res.lc_raw);
+     job_mem_free(&jm);
+     return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Reference seam (src/lmie_schedule.h:18-30). */
+int lmie_core_solution_ex(int n_qsiz, int n_derivs1, int n_derivs2, int save_control,
+                          int *index1, int *index2, double lambda, double *lambda_l,
+                          double mr, double mi, double *mr_l, double *mi_l,
+                          double *qx, double *qw, double *nr, double r1, double r2,
+                          double *cext_, double *csca_, double *pbak_, double *g,
+                          double **gc, double **lc, double **qx_l, double **qw_l, double **nr_l,
+                          double *cext_l_, double *csca_l_, double *pbak_l_, double *g_l,
+                          double ***gc_l, double ***lc_l, double ***save1, double ****save2,
+                          double accuracy, const lmie_opts *opts) {
+     (void) r1;
+     int nd = n_derivs1 + n_derivs2;
+     if (save_control == 2) return fail("lmie_core_solution", "save_control = 2 is not supported");
+     core_args c;
+     memset(&c, 0, sizeof c);
+     c.N = n_qsiz; c.nd = nd; c.nd1 = n_derivs1; c.nd2 = n_derivs2;
+     c.idx1 = index1; c.idx2 = index2; c.save_control = save_control;
+     c.lambda = lambda; c.mr = mr; c.mi = mi;
+     c.lambda_l = lambda_l; c.mr_l = mr_l; c.mi_l = mi_l;
+     c.qx = qx; c.qw = qw; c.nr = nr;
+     /* pointer tables from the reference allocator are contiguous row-major */
+     c.qx_l = nd ? qx_l[0] : NULL; c.qw_l = nd ? qw_l[0] : NULL; c.nr_l = nd ? nr_l[0] : NULL;
+     c.r2 = r2; c.calc_gc = gc != NULL; c.calc_lc = lc != NULL; c.accuracy = accuracy;
+     c.opts = opts;
+     double dummy[4];
+     core_out o = {cext_, csca_, pbak_, g, gc, lc, nd ? cext_l_ : dummy, nd ? csca_l_ : dummy,
+                   nd ? pbak_l_ : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}};
+     return core_run(&c, &o);
+}
+
+int lmie_core_solution(int n_qsiz, int n_derivs1, int n_derivs2, int save_control,
+                       int *index1, int *index2, double lambda, double *lambda_l,
+                       double mr, double mi, double *mr_l, double *mi_l,
+                       double *qx, double *qw, double *nr, double r1, double r2,
+                       double *cext_, double *csca_, double *pbak_, double *g,
+                       double **gc, double **lc, double **qx_l, double **qw_l, double **nr_l,
+                       double *cext_l_, double *csca_l_, double *pbak_l_, double *g_l,
+                       double ***gc_l, double ***lc_l, double ***save1, double ****save2,
+                       double accuracy, int verbose, int n_threads, int use_mpi) {
+     (void) verbose; (void) n_threads;
+     if (use_mpi) return fail("lmie_core_solution", "MPI is not supported (one GPU)");
+     return lmie_core_solution_ex(n_qsiz, n_derivs1, n_derivs2, save_control, index1, index2, lambda,
+                                  lambda_l, mr, mi, mr_l, mi_l, qx, qw, nr, r1, r2, cext_, csca_,
+                                  pbak_, g, gc, lc, qx_l, qw_l, nr_l, cext_l_, csca_l_, pbak_l_, g_l,
+                                  gc_l, lc_l, save1, save2, accuracy, NULL);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Public API (src/lmie_interface.c). */
+int lmie_calc_max_coef(double lambda, enum size_dist_type dist_type,
+                       double a1, double a2, double a3, double r1, double r2) {
+     (void) a3;
+     if (dist_type == SIZE_DIST_POWER_LAW) pl_range(a1, a2, &r1, &r2);
+     return 2 * mie_n1(2. * LM_PI * r2 / lambda);
+}
+
+int lmie_in_alloc(lmie_in_data *in, int n) {
+     if (n > 0) {
+          double **v[10] = {&in->lambda_l, &in->mr_l, &in->mi_l, &in->a1_l, &in->a2_l,
+                            &in->a3_l, &in->a4_l, &in->a5_l, &in->r1_l, &in->r2_l};
+          for (int k = 0; k < 10; ++k)
+               if (!(*v[k] = calloc(n, sizeof(double)))) return fail("lmie_in_alloc", "out of memory");
+     }
+     return 0;
+}
+
+void lmie_in_free(lmie_in_data *in, int flag) {
+     if (!flag) return;
+     free(in->lambda_l); free(in->mr_l); free(in->mi_l); free(in->a1_l); free(in->a2_l);
+     free(in->a3_l); free(in->a4_l); free(in->a5_l); free(in->r1_l); free(in->r2_l);
+}
+
+int lmie_out_alloc(lmie_in_data *in, lmie_out_data *out, int max_coef) {
+     int n = in->n_derivs;
+     if (n > 0) {
+          double **v[13] = {&out->r1_l, &out->r2_l, &out->norm_l, &out->reff_l, &out->veff_l,
+                            &out->gavg_l, &out->vavg_l, &out->ravg_l, &out->rvw_l, &out->cext_l,
+                            &out->csca_l, &out->cbak_l, &out->g_l};
+          for (int k = 0; k < 13; ++k) *v[k] = vec(n);
+     }
+     out->gc = NULL;
+     if (in->calc_gc) { out->gc = table2(6, max_coef); if (n > 0) out->gc_l = table3(n, 6, max_coef); }
+     out->lc = NULL;
+     if (in->calc_lc) { out->lc = table2(6, max_coef); if (n > 0) out->lc_l = table3(n, 6, max_coef); }
+     out->pf = NULL;
+     if (in->calc_pf) {
+          out->theta = vec(in->n_angles);
+          out->pf = table2(6, in->n_angles);
+          if (n > 0) out->pf_l = table3(n, 6, in->n_angles);
+     }
+     if (in->save_control != 2) { out->save1 = NULL; out->save2 = NULL; }
+     return 0;
+}
+
+void lmie_out_free(lmie_out_data *out, int flag) {
+     if (flag) {
+          free(out->r1_l); free(out->r2_l); free(out->norm_l); free(out->reff_l); free(out->veff_l);
+          free(out->gavg_l); free(out->vavg_l); free(out->ravg_l); free(out->rvw_l);
+          free(out->cext_l); free(out->csca_l); free(out->cbak_l); free(out->g_l);
+     }
+     if (out->gc) { free(out->gc); if (flag) free(out->gc_l); }
+     if (out->lc) { free(out->lc); if (flag) free(out->lc_l); }
+     if (out->pf) { free(out->theta); free(out->pf); if (flag) free(out->pf_l); }
+}
+
+void lmie_in_zero_derivs(lmie_in_data *in, int n) {
+     for (int i = 0; i < n; ++i)
+          in->lambda_l[i] = in->mr_l[i] = in->mi_l[i] = in->a1_l[i] = in->a2_l[i] =
+          in->a3_l[i] = in->a4_l[i] = in->a5_l[i] = in->r1_l[i] = in->r2_l[i] = 0.;
+}
+
+/* Scattering matrix at user angles theta_i = i * 180 / (n - 1)
+   (misc/rtutil_scat_int.c:487-503, :771-797), evaluated on the GPU. */
+static int user_angles(int n_coef, int n_angles, int nd, int ld, double **xc, double ***xc_l,
+                       double *theta, double **pf, double ***pf_l, int gsf, int device) {
+     double dth = 180. / (n_angles - 1);
+     for (int i = 0; i < n_angles; ++i) theta[i] = i * dth;
+     if (lmie_gpu_phase_func(1, n_coef, ld, xc[0], n_angles, theta, pf[0], gsf, device))
+          return fail("lmie_gpu_phase_func", lmie_gpu_last_error());
+     if (nd > 0 && lmie_gpu_phase_func(nd, n_coef, ld, xc_l[0][0], n_angles, theta, pf_l[0][0], gsf, device))
+          return fail("lmie_gpu_phase_func", lmie_gpu_last_error());
+     return 0;
+}
+
+/* Everything of lmie_solution2_l for one or several distributions on one grid. */
+typedef struct {
+     int calc_gc, calc_lc, calc_pf, n_angles, save_control;
+     double lambda, mr, mi, accuracy;
+} run_flags;
+
+static int solve_grid(const run_flags *rf, grid_t *g, const double *lambda_l, const double *mr_l,
+                      const double *mi_l, const lmie_opts *opts, int *n_coef,
+                      double *r21, double *r22, double *norm, double *reff, double *veff,
+                      double *gavg, double *vavg, double *ravg, double *rvw,
+                      double *cext, double *csca, double *cbak, double *g_,
+                      double **gc, double **lc, double *theta, double **pf,
+                      double *r21_l, double *r22_l, double *norm_l, double *reff_l, double *veff_l,
+                      double *gavg_l, double *vavg_l, double *ravg_l, double *rvw_l,
+                      double *cext_l, double *csca_l, double *cbak_l, double *g_l,
+                      double ***gc_l, double ***lc_l, double ***pf_l,
+                      double ***save1, double ****save2, core_out *keep) {
+     int nd = g->nd;
+     dstats st, *stl = calloc(nd ? nd : 1, sizeof(dstats));
+     grid_stats(g, &st, stl);
+     *r21 = g->r1; *r22 = g->r2; *norm = g->norm;
+     *reff = st.reff; *veff = st.veff; *gavg = st.gavg; *vavg = st.vavg; *ravg = st.ravg; *rvw = st.rvw;
+     for (int d = 0; d < nd; ++d) {
+          r21_l[d] = g->r1_l[d]; r22_l[d] = g->r2_l[d]; norm_l[d] = g->norm_l[d];
+          reff_l[d] = stl[d].reff; veff_l[d] = stl[d].veff; gavg_l[d] = stl[d].gavg;
+          vavg_l[d] = stl[d].vavg; ravg_l[d] = stl[d].ravg; rvw_l[d] = stl[d].rvw;
+     }
+     free(stl);
+     int *idx1 = calloc(nd + 1, sizeof(int)), *idx2 = calloc(nd + 1, sizeof(int)), nd1 = 0, nd2 = 0;
+     for (int d = 0; d < nd; ++d) {
+          if (lambda_l[d] != 0. || mr_l[d] != 0. || mi_l[d] != 0. || g->r1_l[d] != 0. || g->r2_l[d] != 0.)
+               idx1[nd1++] = d;
+          else
+               idx2[nd2++] = d;
+     }
+     core_args c;
+     memset(&c, 0, sizeof c);
+     double zero = 0.;
+     c.N = g->N; c.nd = nd; c.nd1 = nd1; c.nd2 = nd2; c.idx1 = idx1; c.idx2 = idx2;
+     c.save_control = rf->save_control;
+     c.lambda = rf->lambda; c.mr = rf->mr; c.mi = rf->mi;
+     c.lambda_l = nd ? lambda_l : &zero; c.mr_l = nd ? mr_l : &zero; c.mi_l = nd ? mi_l : &zero;
+     c.qx = g->qx; c.qw = g->qw; c.nr = g->nr; c.qx_l = g->qx_l; c.qw_l = g->qw_l; c.nr_l = g->nr_l;
+     c.r2 = g->r2; c.calc_gc = rf->calc_gc; c.calc_lc = rf->calc_lc; c.accuracy = rf->accuracy;
+     c.opts = opts;
+     double dummy[4];
+     core_out o = {cext, csca, cbak, g_, gc, lc, nd ? cext_l : dummy, nd ? csca_l : dummy,
+                   nd ? cbak_l : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}};
+     int rc = 0;
+     if (rf->save_control == 2) rc = fail("lmie_solution", "save_control = 2 is not supported");
+     else rc = core_run(&c, &o);
+     free(idx1); free(idx2);
+     if (rc < 0) return fail("lmie_core_solution", lmie_last_error());
+     *n_coef = rc;
+     if (keep) *keep = o;
+     if (rf->calc_pf) {
+          int dev = opts ? opts->device : 0;
+          if (rf->calc_gc) rc = user_angles(*n_coef, rf->n_angles, nd, o.n_coef, gc, gc_l, theta, pf, pf_l, 1, dev);
+          else rc = user_angles(*n_coef, rf->n_angles, nd, o.n_coef, lc, lc_l, theta, pf, pf_l, 0, dev);
+          if (rc) return -1;
+     }
+     return 0;
+}
+
+static int solution2_l_opts(int calc_gc, int calc_lc, int calc_pf, enum size_dist_type dist_type,
+                            int n_int1, int n_int2, int n_quad, int n_angles, int n_derivs,
+                            int save_control, double lambda, double mr, double mi,
+                            double a1, double a2, double a3, double a4, double a5,
+                            double r1, double r2, double *lambda_l, double *mr_l, double *mi_l,
+                            double *a1_l, double *a2_l, double *a3_l, double *a4_l, double *a5_l,
+                            double *r1_l, double *r2_l, double accuracy, int *n_coef,
+                            double *r21, double *r22, double *norm, double *reff, double *veff,
+                            double *gavg, double *vavg, double *ravg, double *rvw,
+                            double *cext, double *csca, double *cbak, double *g,
+                            double **gc, double **lc, double *theta, double **pf,
+                            double *r21_l, double *r22_l, double *norm_l, double *reff_l,
+                            double *veff_l, double *gavg_l, double *vavg_l, double *ravg_l,
+                            double *rvw_l, double *cext_l, double *csca_l, double *cbak_l,
+                            double *g_l, double ***gc_l, double ***lc_l, double ***pf_l,
+                            double ***save1, double ****save2, int use_mpi, const lmie_opts *opts) {
+     if (use_mpi) return fail("lmie_solution", "MPI is not supported (one GPU)");
+     if (calc_pf && !calc_gc && !calc_lc)
+          return fail("lmie_solution", "either calc_gc or calc_lc must be set to true when calc_pf is true");
+     int nd = n_derivs > 0 ? n_derivs : 0;
+     double a[5] = {a1, a2, a3, a4, a5};
+     seeds sv = {nd, {a1_l, a2_l, a3_l, a4_l, a5_l}, r1_l, r2_l};
+     grid_t gr;
+     memset(&gr, 0, sizeof gr);
+     if (grid_build(dist_type, n_int1, n_int2, n_quad, a, r1, r2, &sv, &gr)) {
+          grid_free(&gr);
+          return fail("get_particle_dist", lmie_last_error());
+     }
+     run_flags rf = {calc_gc, calc_lc, calc_pf, n_angles, save_control, lambda, mr, mi, accuracy};
+     int rc = solve_grid(&rf, &gr, lambda_l, mr_l, mi_l, opts, n_coef, r21, r22, norm, reff, veff,
+                         gavg, vavg, ravg, rvw, cext, csca, cbak, g, gc, lc, theta, pf, r21_l, r22_l,
+                         norm_l, reff_l, veff_l, gavg_l, vavg_l, ravg_l, rvw_l, cext_l, csca_l,
+                         cbak_l, g_l, gc_l, lc_l, pf_l, save1, save2, NULL);
+     grid_free(&gr);
+     return rc;
+}
+
+int lmie_solution2_l(int calc_gc, int calc_lc, int calc_pf, enum size_dist_type dist_type,
+                     int n_int1, int n_int2, int n_quad, int n_angles, int n_derivs,
+                     int save_control, double lambda, double mr, double mi,
+                     double a1, double a2, double a3, double a4, double a5, double r1, double r2,
+                     double *lambda_l, double *mr_l, double *mi_l, double *a1_l, double *a2_l,
+                     double *a3_l, double *a4_l, double *a5_l, double *r1_l, double *r2_l,
+                     double accuracy, int *n_coef, double *r21, double *r22, double *norm,
+                     double *reff, double *veff, double *gavg, double *vavg, double *ravg,
+                     double *rvw, double *cext, double *csca, double *cbak, double *g,
+                     double **gc, double **lc, double *theta, double **pf, double *r21_l,
+                     double *r22_l, double *norm_l, double *reff_l, double *veff_l,
+                     double *gavg_l, double *vavg_l, double *ravg_l, double *rvw_l,
+                     double *cext_l, double *csca_l, double *cbak_l, double *g_l,
+                     double ***gc_l, double ***lc_l, double ***pf_l, double ***save1,
+                     double ****save2, int verbose, int n_threads, int use_mpi) {
+     (void) verbose; (void) n_threads;
+     return solution2_l_opts(calc_gc, calc_lc, calc_pf, dist_type, n_int1, n_int2, n_quad, n_angles,
+                             n_derivs, save_control, lambda, mr, mi, a1, a2, a3, a4, a5, r1, r2,
+                             lambda_l, mr_l, mi_l, a1_l, a2_l, a3_l, a4_l, a5_l, r1_l, r2_l, accuracy,
+                             n_coef, r21, r22, norm, reff, veff, gavg, vavg, ravg, rvw, cext, csca,
+                             cbak, g, gc, lc, theta, pf, r21_l, r22_l, norm_l, reff_l, veff_l, gavg_l,
+                             vavg_l, ravg_l, rvw_l, cext_l, csca_l, cbak_l, g_l, gc_l, lc_l, pf_l,
+                             save1, save2, use_mpi, NULL);
+}
+
+int lmie_solution2(int calc_gc, int calc_lc, int calc_pf, enum size_dist_type dist_type,
+                   int n_int1, int n_int2, int n_quad, int n_angles, int save_control,
+                   double lambda, double mr, double mi, double a1, double a2, double a3,
+                   double a4, double a5, double r1, double r2, double accuracy, int *n_coef,
+                   double *r21, double *r22, double *norm, double *reff, double *veff,
+                   double *gavg, double *vavg, double *ravg, double *rvw, double *cext,
+                   double *csca, double *cbak, double *g, double **gc, double **lc,
+                   double *theta, double **pf, double ***save1, double ****save2,
+                   int verbose, int n_threads, int use_mpi) {
+     return lmie_solution2_l(calc_gc, calc_lc, calc_pf, dist_type, n_int1, n_int2, n_quad, n_angles,
+                             0, save_control, lambda, mr, mi, a1, a2, a3, a4, a5, r1, r2, NULL, NULL,
+                             NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, accuracy, n_coef, r21,
+                             r22, norm, reff, veff, gavg, vavg, ravg, rvw, cext, csca, cbak, g, gc,
+                             lc, theta, pf, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
+                             NULL, NULL, NULL, NULL, NULL, NULL, NULL, save1, save2, verbose,
+                             n_threads, use_mpi);
+}
+
+static int solution_opts(lmie_in_data *in, lmie_out_data *out, int alloc_out, int use_mpi,
+                         const lmie_opts *opts) {
+     if (alloc_out) {
+          int mc = lmie_calc_max_coef(in->lambda, in->dist_type, in->a1, in->a2, in->a3, in->r1, in->r2);
+          if (lmie_out_alloc(in, out, mc)) return fail("lmie_out_alloc", "failed");
+     }
+     if (solution2_l_opts(in->calc_gc, in->calc_lc, in->calc_pf, in->dist_type, in->n_int1, in->n_int2,
+                          in->n_quad, in->n_angles, in->n_derivs, in->save_control, in->lambda,
+                          in->mr, in->mi, in->a1, in->a2, in->a3, in->a4, in->a5, in->r1, in->r2,
+                          in->lambda_l, in->mr_l, in->mi_l, in->a1_l, in->a2_l, in->a3_l, in->a4_l,
+                          in->a5_l, in->r1_l, in->r2_l, in->accuracy, &out->n_coef, &out->r1,
+                          &out->r2, &out->norm, &out->reff, &out->veff, &out->gavg, &out->vavg,
+                          &out->ravg, &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g,
+                          out->gc, out->lc, out->theta, out->pf, out->r1_l, out->r2_l, out->norm_l,
+                          out->reff_l, out->veff_l, out->gavg_l, out->vavg_l, out->ravg_l,
+                          out->rvw_l, out->cext_l, out->csca_l, out->cbak_l, out->g_l, out->gc_l,
+                          out->lc_l, out->pf_l, &out->save1, &out->save2, use_mpi, opts))
+          return fail("lmie_solution2_l", lmie_last_error());
+     return 0;
+}
+
+int lmie_solution(lmie_in_data *in, lmie_out_data *out, int alloc_out, int verbose,
+                  int n_threads, int use_mpi) {
+     (void) verbose; (void) n_threads;
+     return solution_opts(in, out, alloc_out, use_mpi, NULL);
+}
+
+int lmie_solution_ex(lmie_in_data *in, lmie_out_data *out, int alloc_out, const lmie_opts *opts) {
+     return solution_opts(in, out, alloc_out, 0, opts);
+}
+
+int lmie_solution_slave(void) { return fail("lmie_solution_slave", "MPI is not supported (one GPU)"); }
+
+/* ------------------------------------------------------------------------- */
+/* Mixtures of distributions on one grid. */
+
+/* Grid of the mixture: sum_k w_k n_k(r), each n_k normalised.  Seed column d
+   belongs to mode owner[d]; its a1..a5 seeds act on that mode only, its
+   r1/r2 seeds move the shared grid for all modes. */
+static int mix_grid(lmie_in_data *const *modes, const double *weight, int n_modes, int nd,
+                    grid_t *mix) {
+     const lmie_in_data *m0 = modes[0];
+     if (m0->dist_type == SIZE_DIST_POWER_LAW || m0->dist_type == SIZE_DIST_MONO)
+          if (n_modes > 1) return fail("lmie_solution_mix", "power-law and mono modes cannot be mixed");
+     int D = nd > 0 ? nd : 1;
+     double **col = calloc(12, sizeof(double *));
+     for (int q = 0; q < 7; ++q) col[q] = vec(D);    /* a1..a5, r1, r2 */
+     memset(mix, 0, sizeof *mix);
+     int off = 0, rc = 0;
+     for (int k = 0; k < n_modes && !rc; ++k) {
+          const lmie_in_data *m = modes[k];
+          for (int q = 0; q < 7; ++q) memset(col[q], 0, sizeof(double) * D);
+          for (int kk = 0, o2 = 0; kk < n_modes; o2 += modes[kk]->n_derivs, ++kk) {
+               const lmie_in_data *mm = modes[kk];
+               for (int d = 0; d < mm->n_derivs; ++d) {
+                    if (kk == k) {
+                         const double *src[5] = {mm->a1_l, mm->a2_l, mm->a3_l, mm->a4_l, mm->a5_l};
+                         for (int q = 0; q < 5; ++q) col[q][o2 + d] = src[q][d];
+                    }
+                    col[5][o2 + d] = mm->r1_l[d];
+                    col[6][o2 + d] = mm->r2_l[d];
+               }
+          }
+          double a[5] = {m->a1, m->a2, m->a3, m->a4, m->a5};
+          seeds sv = {nd, {col[0], col[1], col[2], col[3], col[4]}, col[5], col[6]};
+          grid_t gk;
+          memset(&gk, 0, sizeof gk);
+          rc = grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk);
+          if (!rc) {
+               double w = weight ? weight[k] : 1.;
+               if (k == 0) {
+                    *mix = gk;
+                    for (int i = 0; i < gk.N; ++i) mix->nr[i] *= w;
+                    for (size_t i = 0; i < (size_t) gk.N * nd; ++i) mix->nr_l[i] *= w;
+                    mix->norm *= w;
+                    for (int d = 0; d < nd; ++d) mix->norm_l[d] *= w;
+               } else {
+                    if (gk.N != mix->N) rc = fail("lmie_solution_mix", "modes have different grids");
+                    for (int i = 0; i < gk.N && !rc; ++i) mix->nr[i] += w * gk.nr[i];
+                    for (size_t i = 0; i < (size_t) gk.N * nd && !rc; ++i) mix->nr_l[i] += w * gk.nr_l[i];
+                    mix->norm += w * gk.norm;
+                    for (int d = 0; d < nd; ++d) mix->norm_l[d] += w * gk.norm_l[d];
+                    grid_free(&gk);
+               }
+          }
+          off += m->n_derivs;
+     }
+     (void) off;
+     for (int q = 0; q < 7; ++q) free(col[q]);
+     free(col);
+     return rc;
+}
+
+static void mix_seed_column(lmie_in_data *const *modes, int n_modes, int which, double *dst) {
+     for (int k = 0, o = 0; k < n_modes; o += modes[k]->n_derivs, ++k) {
+          const double *src = which == 0 ? modes[k]->lambda_l : which == 1 ? modes[k]->mr_l : modes[k]->mi_l;
+          for (int d = 0; d < modes[k]->n_derivs; ++d) dst[o + d] = src[d];
+     }
+}
+
+int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_modes,
+                      lmie_out_data *out, int alloc_out, const lmie_opts *opts) {
+     if (n_modes < 1 || !modes || !modes[0]) return fail("lmie_solution_mix", "no modes");
+     const lmie_in_data *m0 = modes[0];
+     int nd = 0;
+     for (int k = 0; k < n_modes; ++k) nd += modes[k]->n_derivs > 0 ? modes[k]->n_derivs : 0;
+     if (m0->calc_pf && !m0->calc_gc && !m0->calc_lc)
+          return fail("lmie_solution_mix", "either calc_gc or calc_lc must be set to true when calc_pf is true");
+     if (alloc_out) {
+          lmie_in_data tmp = *m0;
+          tmp.n_derivs = nd;
+          int mc = lmie_calc_max_coef(m0->lambda, m0->dist_type, m0->a1, m0->a2, m0->a3, m0->r1, m0->r2);
+          if (lmie_out_alloc(&tmp, out, mc)) return fail("lmie_out_alloc", "failed");
+     }
+     grid_t g;
+     if (mix_grid(modes, weight, n_modes, nd, &g)) return -1;
+     double *lam_l = vec(nd), *mr_l = vec(nd), *mi_l = vec(nd);
+     mix_seed_column(modes, n_modes, 0, lam_l);
+     mix_seed_column(modes, n_modes, 1, mr_l);
+     mix_seed_column(modes, n_modes, 2, mi_l);
+     run_flags rf = {m0->calc_gc, m0->calc_lc, m0->calc_pf, m0->n_angles, m0->save_control,
+                     m0->lambda, m0->mr, m0->mi, m0->accuracy};
+     int rc = solve_grid(&rf, &g, lam_l, mr_l, mi_l, opts, &out->n_coef, &out->r1, &out->r2,
+                         &out->norm, &out->reff, &out->veff, &out->gavg, &out->vavg, &out->ravg,
+                         &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g, out->gc, out->lc,
+                         out->theta, out->pf, out->r1_l, out->r2_l, out->norm_l, out->reff_l,
+                         out->veff_l, out->gavg_l, out->vavg_l, out->ravg_l, out->rvw_l, out->cext_l,
+                         out->csca_l, out->cbak_l, out->g_l, out->gc_l, out->lc_l, out->pf_l,
+                         &out->save1, &out->save2, NULL);
+     free(lam_l); free(mr_l); free(mi_l);
+     grid_free(&g);
+     return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Forward + reverse pass.  Output cotangents are pulled back through the
+   normalisation here, through the kernels by lmie_gpu_adjoint to f_i and m,
+   and from f_i to the distribution parameters with the grid tangents. */
+int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int n_modes,
+                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
+                          const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes) {
+     if (n_modes < 1 || !modes || !modes[0] || !adj || !grad)
+          return fail("lmie_solution_adjoint", "missing argument");
+     const lmie_in_data *m0 = modes[0];
+     if (m0->save_control != 0) return fail("lmie_solution_adjoint", "save_control must be 0");
+     if (alloc_out) {
+          lmie_in_data tmp = *m0;
+          tmp.n_derivs = 0;
+          int mc = lmie_calc_max_coef(m0->lambda, m0->dist_type, m0->a1, m0->a2, m0->a3, m0->r1, m0->r2);
+          if (lmie_out_alloc(&tmp, out, mc)) return fail("lmie_out_alloc", "failed");
+     }
+     /* forward on the mixture grid, no seeds */
+     lmie_in_data **nm = calloc(n_modes, sizeof(*nm));
+     lmie_in_data *copies = calloc(n_modes, sizeof(*copies));
+     for (int k = 0; k < n_modes; ++k) { copies[k] = *modes[k]; copies[k].n_derivs = 0; nm[k] = &copies[k]; }
+     grid_t g;
+     int rc = mix_grid(nm, weight, n_modes, 0, &g);
+     if (rc) { free(nm); free(copies); return -1; }
+     run_flags rf = {m0->calc_gc, m0->calc_lc, 0, 0, 0, m0->lambda, m0->mr, m0->mi, m0->accuracy};
+     core_out keep;
+     double zero = 0.;
+     rc = solve_grid(&rf, &g, &zero, &zero, &zero, opts, &out->n_coef, &out->r1, &out->r2,
+                     &out->norm, &out->reff, &out->veff, &out->gavg, &out->vavg, &out->ravg,
+                     &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g, out->gc, out->lc,
+                     out->theta, out->pf, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
+                     NULL, NULL, NULL, NULL, NULL, NULL, NULL, &out->save1, &out->save2, &keep);
+     if (rc) { grid_free(&g); free(nm); free(copies); return -1; }
+     if (m0->calc_pf) {
+          /* the forward of the adjoint call also fills the user-angle matrix */
+          if (user_angles(out->n_coef, m0->n_angles, 0, keep.n_coef, m0->calc_gc ? out->gc : out->lc, NULL,
+                          out->theta, out->pf, NULL, m0->calc_gc, opts ? opts->device : 0)) {
+               grid_free(&g); free(nm); free(copies); return -1;
+          }
+     }
+     /* pull back through the normalisation (src/lmie_schedule.c:1158-1199) */
+     int C = keep.n_coef;
+     double Csca = keep.sums[1], s = m0->lambda * m0->lambda / (2. * LM_PI);
+     double sb[4];
+     sb[0] = s * adj->cext;
+     sb[2] = s * adj->cbak / (2. * 4. * LM_PI);
+     sb[3] = 2. * adj->g / Csca;
+     double dot = adj->g * out->g;
+     double *gcb = NULL, *lcb = NULL;
+     if (adj->gc && m0->calc_gc) {
+          gcb = vec((size_t) 6 * C);
+          for (int k = 0; k < 6; ++k)
+               for (int l = 0; l < C; ++l) {
+                    gcb[k * C + l] = adj->gc[k * C + l] * 2. / Csca;
+                    dot += adj->gc[k * C + l] * out->gc[k][l];
+               }
+     }
+     if (adj->lc && m0->calc_lc) {
+          lcb = vec((size_t) 6 * C);
+          for (int k = 0; k < 6; ++k)
+               for (int l = 0; l < C; ++l) {
+                    lcb[k * C + l] = adj->lc[k * C + l] * 2. / Csca;
+                    dot += adj->lc[k * C + l] * out->lc[k][l];
+               }
+     }
+     sb[1] = s * adj->csca - dot / Csca;
+     /* reverse pass on the GPU */
+     core_args c;
+     memset(&c, 0, sizeof c);
+     c.N = g.N; c.lambda = m0->lambda; c.mr = m0->mr; c.mi = m0->mi;
+     c.lambda_l = c.mr_l = c.mi_l = &zero;
+     c.qx = g.qx; c.qw = g.qw; c.nr = g.nr; c.r2 = g.r2;
+     c.calc_gc = m0->calc_gc; c.calc_lc = m0->calc_lc; c.accuracy = m0->accuracy; c.opts = opts;
+     job_mem jm;
+     lmie_gpu_job job;
+     job_mem_alloc(&jm, g.N, 0, 0);
+     fill_job(&c, &job, jm.x, jm.f, jm.n1, jm.n2, jm.dx, jm.dm, jm.dft, jm.dfw);
+     lmie_gpu_adj_in ain = {sb, gcb, lcb};
+     lmie_gpu_adj_out aout;
+     aout.f_bar = vec(g.N);
+     aout.x_bar = vec(g.N);
+     rc = lmie_gpu_adjoint(&job, &ain, &aout);
+     if (rc) {
+          fail("lmie_gpu_adjoint", lmie_gpu_last_error());
+     } else {
+          grad->lambda = NAN; grad->r1 = NAN; grad->r2 = NAN;
+          grad->mr = aout.m_bar[0];
+          grad->mi = aout.m_bar[1];
+          /* f_i = qw_i sum_k w_k n_k(r_i): parameter p of mode k enters through n_k */
+          double *gp = vec((size_t) 5 * n_modes);
+          for (int k = 0; k < n_modes; ++k) {
+               const lmie_in_data *m = modes[k];
+               double e[5][5] = {{1, 0, 0, 0, 0}, {0, 1, 0, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 1, 0}, {0, 0, 0, 0, 1}};
+               double *cols[5];
+               for (int q = 0; q < 5; ++q) cols[q] = e[q];
+               seeds sv = {5, {cols[0], cols[1], cols[2], cols[3], cols[4]}, NULL, NULL};
+               double a[5] = {m->a1, m->a2, m->a3, m->a4, m->a5};
+               grid_t gk;
+               memset(&gk, 0, sizeof gk);
+               if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0) {
+                    double w = weight ? weight[k] : 1.;
+                    for (int i = 0; i < gk.N; ++i)
+                         for (int q = 0; q < 5; ++q)
+                              gp[5 * k + q] += aout.f_bar[i] * w * gk.nr_l[(size_t) i * 5 + q] * gk.qw[i];
+               }
+               grid_free(&gk);
+          }
+          grad->a1 = gp[0]; grad->a2 = gp[1]; grad->a3 = gp[2]; grad->a4 = gp[3]; grad->a5 = gp[4];
+          if (grad_modes)
+               for (int q = 0; q < 5 * n_modes; ++q) grad_modes[q] = gp[q];
+          free(gp);
+     }
+     free(aout.f_bar); free(aout.x_bar); free(gcb); free(lcb);
+     job_mem_free(&jm);
+     grid_free(&g);
+     free(nm); free(copies);
+     return rc ? -1 : 0;
+}

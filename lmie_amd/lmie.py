@@ -1,0 +1,170 @@
+"""ctypes wrappers of the product library (include/lmie_hip.h extensions).
+
+``solve`` mirrors the reference call ``lmie_solution`` (src/lmie_interface.c:199-250)
+with a precision option; ``solve_mix`` evaluates several distributions on one
+size grid; ``adjoint`` runs the forward and one reverse pass and returns the
+gradient of a scalar loss.  Inputs are the same parameter dictionaries as
+lmie_amd.abi and lmie_amd.workloads.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import abi
+from .abi import LmieIn, LmieOut
+
+PRECISION = {"fp64": 0, "mixed": 1}
+
+TIMERS = ("tables", "coefs", "angular", "reduce", "expand", "adj_angular", "adj_coefs", "total")
+
+
+class LmieOpts(C.Structure):
+    """lmie_opts, include/lmie_hip.h."""
+    _fields_ = [("precision", C.c_int), ("device", C.c_int), ("reserved", C.c_int * 6)]
+
+
+class LmieAdjIn(C.Structure):
+    """lmie_adj_in, include/lmie_hip.h."""
+    _fields_ = [("cext", C.c_double), ("csca", C.c_double), ("cbak", C.c_double),
+                ("g", C.c_double), ("gc", C.POINTER(C.c_double)), ("lc", C.POINTER(C.c_double))]
+
+
+class LmieGrad(C.Structure):
+    """lmie_grad, include/lmie_hip.h: dL/d of each scalar input."""
+    _fields_ = [(n, C.c_double) for n in ("lambda_", "mr", "mi", "a1", "a2", "a3", "a4", "a5",
+                                          "r1", "r2")]
+
+
+def bind_all(lib):
+    abi.bind(lib)
+    PIn, POut = C.POINTER(LmieIn), C.POINTER(LmieOut)
+    lib.lmie_solution_ex.argtypes = [PIn, POut, C.c_int, C.POINTER(LmieOpts)]
+    lib.lmie_solution_ex.restype = C.c_int
+    lib.lmie_solution_mix.argtypes = [C.POINTER(PIn), C.POINTER(C.c_double), C.c_int, POut,
+                                      C.c_int, C.POINTER(LmieOpts)]
+    lib.lmie_solution_mix.restype = C.c_int
+    lib.lmie_solution_adjoint.argtypes = [C.POINTER(PIn), C.POINTER(C.c_double), C.c_int, POut,
+                                          C.c_int, C.POINTER(LmieOpts), C.POINTER(LmieAdjIn),
+                                          C.POINTER(LmieGrad), C.POINTER(C.c_double)]
+    lib.lmie_solution_adjoint.restype = C.c_int
+    lib.lmie_last_error.restype = C.c_char_p
+    lib.lmie_gpu_last_error.restype = C.c_char_p
+    lib.lmie_set_timing.argtypes = [C.c_int]
+    lib.lmie_get_timing.argtypes = [C.POINTER(C.c_double), C.c_int]
+    return lib
+
+
+def _opts(precision, device=0):
+    o = LmieOpts()
+    o.precision = PRECISION[precision] if isinstance(precision, str) else int(precision)
+    o.device = device
+    return o
+
+
+def last_error(lib):
+    return (lib.lmie_last_error() or b"").decode()
+
+
+def timings(lib):
+    """Per-stage device times (ms) of this thread's last call, if timing is on."""
+    buf = (C.c_double * len(TIMERS))()
+    lib.lmie_get_timing(buf, len(TIMERS))
+    return dict(zip(TIMERS, list(buf)))
+
+
+def solve(lib, p, seeds=None, precision="fp64", device=0):
+    """lmie_solution_ex with alloc_out = 1; returns numpy copies of every output."""
+    inp = abi.make_in(lib, p, seeds)
+    out = LmieOut()
+    max_coef = lib.lmie_calc_max_coef(inp.lambda_, inp.dist_type, inp.a1, inp.a2, inp.a3,
+                                      inp.r1, inp.r2)
+    o = _opts(precision, device)
+    rc = lib.lmie_solution_ex(C.byref(inp), C.byref(out), 1, C.byref(o))
+    try:
+        if rc != 0:
+            raise RuntimeError("lmie_solution_ex failed: " + last_error(lib))
+        return abi.read_out(lib, p, inp, out, max_coef)
+    finally:
+        if rc == 0:
+            lib.lmie_out_free(C.byref(out), 1 if inp.n_derivs > 0 else 0)
+        if inp.n_derivs > 0:
+            lib.lmie_in_free(C.byref(inp), 1)
+
+
+def _modes(lib, modes):
+    ins = [abi.make_in(lib, p, s) for p, s in modes]
+    arr = (C.POINTER(LmieIn) * len(ins))(*[C.pointer(x) for x in ins])
+    return ins, arr
+
+
+def solve_mix(lib, modes, weights, precision="fp64", device=0):
+    """lmie_solution_mix: modes is a list of (params, seeds) on one grid."""
+    ins, arr = _modes(lib, modes)
+    nd = sum(x.n_derivs for x in ins)
+    w = (C.c_double * len(ins))(*weights)
+    out = LmieOut()
+    p0 = modes[0][0]
+    max_coef = lib.lmie_calc_max_coef(ins[0].lambda_, ins[0].dist_type, ins[0].a1, ins[0].a2,
+                                      ins[0].a3, ins[0].r1, ins[0].r2)
+    o = _opts(precision, device)
+    rc = lib.lmie_solution_mix(arr, w, len(ins), C.byref(out), 1, C.byref(o))
+    try:
+        if rc != 0:
+            raise RuntimeError("lmie_solution_mix failed: " + last_error(lib))
+        fake = LmieIn.from_buffer_copy(ins[0])
+        fake.n_derivs = nd
+        return abi.read_out(lib, p0, fake, out, max_coef)
+    finally:
+        if rc == 0:
+            lib.lmie_out_free(C.byref(out), 1 if nd > 0 else 0)
+        for x in ins:
+            if x.n_derivs > 0:
+                lib.lmie_in_free(C.byref(x), 1)
+
+
+def adjoint(lib, modes, weights, cot, precision="fp64", device=0):
+    """Forward + reverse pass.  cot: dict with optional keys cext, csca, cbak,
+    g (floats) and gc, lc ([6][max_coef] arrays).  Returns (outputs, grad) where
+    grad maps input names to dL/d input and 'modes' to an [n_modes][5] array of
+    the a1..a5 gradients of each mode."""
+    ins, arr = _modes(lib, [(p, None) for p in modes])
+    n = len(ins)
+    w = (C.c_double * n)(*(weights if weights is not None else [1.0] * n))
+    out = LmieOut()
+    max_coef = lib.lmie_calc_max_coef(ins[0].lambda_, ins[0].dist_type, ins[0].a1, ins[0].a2,
+                                      ins[0].a3, ins[0].r1, ins[0].r2)
+    keep = []
+
+    def arrp(key):
+        v = cot.get(key)
+        if v is None:
+            return None
+        a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(6, max_coef))
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+
+    adj = LmieAdjIn(float(cot.get("cext", 0.0)), float(cot.get("csca", 0.0)),
+                    float(cot.get("cbak", 0.0)), float(cot.get("g", 0.0)), arrp("gc"), arrp("lc"))
+    g = LmieGrad()
+    gm = (C.c_double * (5 * n))()
+    o = _opts(precision, device)
+    rc = lib.lmie_solution_adjoint(arr, w, n, C.byref(out), 1, C.byref(o), C.byref(adj),
+                                   C.byref(g), gm)
+    try:
+        if rc != 0:
+            raise RuntimeError("lmie_solution_adjoint failed: " + last_error(lib))
+        fake = LmieIn.from_buffer_copy(ins[0])
+        fake.n_derivs = 0
+        res = abi.read_out(lib, modes[0], fake, out, max_coef)
+        grad = {"lambda": g.lambda_, "mr": g.mr, "mi": g.mi, "a1": g.a1, "a2": g.a2, "a3": g.a3,
+                "a4": g.a4, "a5": g.a5, "r1": g.r1, "r2": g.r2,
+                "modes": np.array(list(gm)).reshape(n, 5)}
+        return res, grad
+    finally:
+        if rc == 0:
+            lib.lmie_out_free(C.byref(out), 0)
+
+
+def nan_to_none(v):
+    return None if (v is None or (isinstance(v, float) and math.isnan(v))) else v

@@ -13,16 +13,19 @@
  * the reference file cited beside it.  Pins (tests/test_oracle_*.py):
  *   - every output against the reference's own printed example
  *     (examples/example.out) and against full-precision fixtures produced by
- *     the reference build (tests/golden/*.npz);
+ *     the reference build (the .npz files under tests/golden);
  *   - a_n, b_n against arbitrary-precision Bessel-function Lorenz-Mie
  *     coefficients (mpmath), pi_n / tau_n against Legendre derivatives,
  *     Gauss nodes against numpy's leggauss, the generalized spherical
  *     functions against their closed forms at low order, and the invariants
  *     beta_0 = 1, g = beta_1 / 3, C_bak = F11(180 deg) C_sca / (4 pi).
- * Single threaded; n_threads is accepted and ignored.
+ * The per-size loop runs on n_threads POSIX threads (as the reference
+ * partitions it); the result does not depend on the thread count beyond the
+ * order of the final sums.
  */
 #include <complex.h>
 #include <float.h>
+#include <pthread.h>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -709,68 +712,38 @@ static void synth_pf(int n_coef, int n_theta, double **xc, double *theta, double
 /* ------------------------------------------------------------------------- */
 /* The size-integrated solution (src/lmie_schedule.c:942-1268 with the per-size
    kernel src/lmie_core.c:95-842).  Raw sums, normalisation, expansion. */
-typedef struct {
+typedef struct core_in_s {
      int N, nd, nd1, nd2, *idx1, *idx2, save;
      double lambda, *lambda_l, mr, mi, *mr_l, *mi_l;
      const double *qx, *qw, *nr, *qx_l, *qw_l, *nr_l;
      double r2;
 } core_in;
 
-static int core_solution(const core_in *c, double *cext, double *csca, double *cbak, double *g,
-                         double **gc, double **lc, double *cext_l, double *csca_l,
-                         double *cbak_l, double *g_l, double ***gc_l, double ***lc_l,
-                         double ***save1, double ****save2, double accuracy) {
-     int nd = c->nd, nd1 = c->nd1, nd2 = c->nd2;
-     double x2 = 2. * ORACLE_PI * c->r2 / c->lambda;
-     cplx m = c->mr - I * c->mi;
-     int n1 = order_n1(x2);
-     int nq = (!gc && !lc) ? 0 : 2 * n1;
-     int n_coef = 2 * n1;
-     double *fv1 = calloc(n1 + 2, sizeof(double)), *fv2 = calloc(n1 + 2, sizeof(double));
-     double *fv3 = calloc(n1 + 2, sizeof(double)), *fv4 = calloc(n1 + 2, sizeof(double));
-     for (int n = 1; n <= n1; ++n) {
-          fv1[n] = 2. * n + 1.;
-          fv2[n] = (n + 1.) / n;
-          fv3[n] = (n + 2.) / fv2[n];
-          fv4[n] = fv1[n] / (n * (n + 1.));
-     }
-     double *mu = calloc(nq + 1, sizeof(double)), *wq = calloc(nq + 1, sizeof(double));
-     if (nq > 0) oracle_gauss_leg(nq, 0, -1., 1., NULL, NULL, mu, wq, NULL, NULL);
-     double **pf = blk2(4, nq > 0 ? nq : 1);
-     double ***pf_l = blk3(nd > 0 ? nd : 1, 4, nq > 0 ? nq : 1);
-     cplx *m_l = calloc(nd + 1, sizeof(cplx));
-     for (int d = 0; d < nd; ++d) m_l[d] = c->mr_l[d] - I * c->mi_l[d];
-     if (c->save == 1) {
-          *save1 = blk2(4, c->N);
-          *save2 = blk3(4, c->N, nq > 0 ? nq : 1);
-     }
-     double Cext = 0, Csca = 0, Cbak = 0, G = 0;
-     for (int d = 0; d < nd; ++d) cext_l[d] = csca_l[d] = cbak_l[d] = g_l[d] = 0.;
+typedef struct {
+     const struct core_in_s *c;
+     int t, P, nq, n1;
+     const double *fv1, *fv2, *fv3, *fv4, *mu;
+     cplx m;
+     const cplx *m_l;
+     double ***save1, ****save2;
+     double cx[4], *cx_l, **pf, ***pf_l;
+} part_job;
 
-     if (c->save == 2) {
-          /* re-weight the cached per-size sums (src/lmie_schedule.c:1087-1124) */
-          for (int i = 0; i < c->N; ++i) {
-               double f = c->nr[i] * c->qw[i];
-               Cext += (*save1)[0][i] * f; Csca += (*save1)[1][i] * f;
-               Cbak += (*save1)[2][i] * f; G += (*save1)[3][i] * f;
-               for (int j = 0; j < nq; ++j)
-                    for (int k = 0; k < 4; ++k) pf[k][j] += (*save2)[k][i][j] * f;
-               for (int e = 0; e < nd2; ++e) {
-                    int d = c->idx2[e];
-                    double fl = c->nr_l[i * nd + d] * c->qw[i] + c->nr[i] * c->qw_l[i * nd + d];
-                    cext_l[d] += (*save1)[0][i] * fl; csca_l[d] += (*save1)[1][i] * fl;
-                    cbak_l[d] += (*save1)[2][i] * fl; g_l[d] += (*save1)[3][i] * fl;
-                    for (int j = 0; j < nq; ++j)
-                         for (int k = 0; k < 4; ++k) pf_l[d][k][j] += (*save2)[k][i][j] * fl;
-               }
-          }
-     }
-     else {
+static void *core_part(void *arg) {
+     part_job *J = (part_job *) arg;
+     const core_in *c = J->c;
+     int nd = c->nd, nd1 = c->nd1, nd2 = c->nd2, nq = J->nq;
+     const double *fv1 = J->fv1, *fv2 = J->fv2, *fv3 = J->fv3, *fv4 = J->fv4, *mu = J->mu;
+     cplx m = J->m;
+     const cplx *m_l = J->m_l;
+     double ***save1 = J->save1;
+     double ****save2 = J->save2;
+     (void) nd;
           double *dx = calloc(nd1 + 1, sizeof(double)), *fl1 = calloc(nd1 + 1, sizeof(double));
           double *fl2 = calloc(nd2 + 1, sizeof(double));
           cplx *dm = calloc(nd1 + 1, sizeof(cplx));
           int T = nd1 > 0 ? nd1 : 1;
-          for (int i = 0; i < c->N; ++i) {
+          for (int i = J->t; i < c->N; i += J->P) {
                double r = c->qx[i], x = 2. * ORACLE_PI * r / c->lambda;
                double f = c->nr[i] * c->qw[i];
                for (int e = 0; e < nd1; ++e) {
@@ -817,21 +790,21 @@ static int core_solution(const core_in *c, double *cext, double *csca, double *c
                     }
                }
                double cb2 = creal(cb * conj(cb));
-               Cext += f * se; Csca += f * ss; Cbak += f * cb2; G += f * sg;
+               J->cx[0] += f * se; J->cx[1] += f * ss; J->cx[2] += f * cb2; J->cx[3] += f * sg;
                if (c->save == 1) {
                     (*save1)[0][i] = se; (*save1)[1][i] = ss; (*save1)[2][i] = cb2; (*save1)[3][i] = sg;
                }
                for (int t = 0; t < nd1; ++t) {
                     int d = c->idx1[t];
-                    cext_l[d] += fl1[t] * se + f * se_t[t];
-                    csca_l[d] += fl1[t] * ss + f * ss_t[t];
-                    cbak_l[d] += fl1[t] * cb2 + f * 2. * creal(cb * conj(cb_t[t]));
-                    g_l[d] += fl1[t] * sg + f * sg_t[t];
+                    J->cx_l[4 * d] += fl1[t] * se + f * se_t[t];
+                    J->cx_l[4 * d + 1] += fl1[t] * ss + f * ss_t[t];
+                    J->cx_l[4 * d + 2] += fl1[t] * cb2 + f * 2. * creal(cb * conj(cb_t[t]));
+                    J->cx_l[4 * d + 3] += fl1[t] * sg + f * sg_t[t];
                }
                for (int t = 0; t < nd2; ++t) {
                     int d = c->idx2[t];
-                    cext_l[d] += fl2[t] * se; csca_l[d] += fl2[t] * ss;
-                    cbak_l[d] += fl2[t] * cb2; g_l[d] += fl2[t] * sg;
+                    J->cx_l[4 * d] += fl2[t] * se; J->cx_l[4 * d + 1] += fl2[t] * ss;
+                    J->cx_l[4 * d + 2] += fl2[t] * cb2; J->cx_l[4 * d + 3] += fl2[t] * sg;
                }
                /* angular sums over the Gauss nodes (src/lmie_core.c:617-791) */
                cplx *sm_ = malloc(sizeof(cplx) * (n + 1)), *df_ = malloc(sizeof(cplx) * (n + 1));
@@ -870,7 +843,7 @@ static int core_solution(const core_in *c, double *cext, double *csca, double *c
                          double q1 = creal(s1 * conj(s1)), q2 = creal(s2 * conj(s2));
                          cplx x12 = s1 * conj(s2);
                          double P[4] = {(q1 + q2) * .5, creal(x12), -(q1 - q2) * .5, cimag(x12)};
-                         for (int k = 0; k < 4; ++k) pf[k][jj] += f * P[k];
+                         for (int k = 0; k < 4; ++k) J->pf[k][jj] += f * P[k];
                          if (c->save == 1)
                               for (int k = 0; k < 4; ++k) (*save2)[k][i][jj] = P[k];
                          for (int t = 0; t < nd1; ++t) {
@@ -880,11 +853,11 @@ static int core_solution(const core_in *c, double *cext, double *csca, double *c
                               cplx dx12 = s1t * conj(s2) + s1 * conj(s2t);
                               double dP[4] = {(dq1 + dq2) * .5, creal(dx12), -(dq1 - dq2) * .5, cimag(dx12)};
                               int d = c->idx1[t];
-                              for (int k = 0; k < 4; ++k) pf_l[d][k][jj] += fl1[t] * P[k] + f * dP[k];
+                              for (int k = 0; k < 4; ++k) J->pf_l[d][k][jj] += fl1[t] * P[k] + f * dP[k];
                          }
                          for (int t = 0; t < nd2; ++t) {
                               int d = c->idx2[t];
-                              for (int k = 0; k < 4; ++k) pf_l[d][k][jj] += fl2[t] * P[k];
+                              for (int k = 0; k < 4; ++k) J->pf_l[d][k][jj] += fl2[t] * P[k];
                          }
                     }
                }
@@ -893,6 +866,92 @@ static int core_solution(const core_in *c, double *cext, double *csca, double *c
                mie_free(&ab);
           }
           free(dx); free(fl1); free(fl2); free(dm);
+     return arg;
+}
+
+static int core_solution(const core_in *c, double *cext, double *csca, double *cbak, double *g,
+                         double **gc, double **lc, double *cext_l, double *csca_l,
+                         double *cbak_l, double *g_l, double ***gc_l, double ***lc_l,
+                         double ***save1, double ****save2, double accuracy, int n_threads) {
+     int nd = c->nd, nd2 = c->nd2;
+     double x2 = 2. * ORACLE_PI * c->r2 / c->lambda;
+     cplx m = c->mr - I * c->mi;
+     int n1 = order_n1(x2);
+     int nq = (!gc && !lc) ? 0 : 2 * n1;
+     int n_coef = 2 * n1;
+     double *fv1 = calloc(n1 + 2, sizeof(double)), *fv2 = calloc(n1 + 2, sizeof(double));
+     double *fv3 = calloc(n1 + 2, sizeof(double)), *fv4 = calloc(n1 + 2, sizeof(double));
+     for (int n = 1; n <= n1; ++n) {
+          fv1[n] = 2. * n + 1.;
+          fv2[n] = (n + 1.) / n;
+          fv3[n] = (n + 2.) / fv2[n];
+          fv4[n] = fv1[n] / (n * (n + 1.));
+     }
+     double *mu = calloc(nq + 1, sizeof(double)), *wq = calloc(nq + 1, sizeof(double));
+     if (nq > 0) oracle_gauss_leg(nq, 0, -1., 1., NULL, NULL, mu, wq, NULL, NULL);
+     double **pf = blk2(4, nq > 0 ? nq : 1);
+     double ***pf_l = blk3(nd > 0 ? nd : 1, 4, nq > 0 ? nq : 1);
+     cplx *m_l = calloc(nd + 1, sizeof(cplx));
+     for (int d = 0; d < nd; ++d) m_l[d] = c->mr_l[d] - I * c->mi_l[d];
+     if (c->save == 1) {
+          *save1 = blk2(4, c->N);
+          *save2 = blk3(4, c->N, nq > 0 ? nq : 1);
+     }
+     double Cext = 0, Csca = 0, Cbak = 0, G = 0;
+     for (int d = 0; d < nd; ++d) cext_l[d] = csca_l[d] = cbak_l[d] = g_l[d] = 0.;
+
+     if (c->save == 2) {
+          /* re-weight the cached per-size sums (src/lmie_schedule.c:1087-1124) */
+          for (int i = 0; i < c->N; ++i) {
+               double f = c->nr[i] * c->qw[i];
+               Cext += (*save1)[0][i] * f; Csca += (*save1)[1][i] * f;
+               Cbak += (*save1)[2][i] * f; G += (*save1)[3][i] * f;
+               for (int j = 0; j < nq; ++j)
+                    for (int k = 0; k < 4; ++k) pf[k][j] += (*save2)[k][i][j] * f;
+               for (int e = 0; e < nd2; ++e) {
+                    int d = c->idx2[e];
+                    double fl = c->nr_l[i * nd + d] * c->qw[i] + c->nr[i] * c->qw_l[i * nd + d];
+                    cext_l[d] += (*save1)[0][i] * fl; csca_l[d] += (*save1)[1][i] * fl;
+                    cbak_l[d] += (*save1)[2][i] * fl; g_l[d] += (*save1)[3][i] * fl;
+                    for (int j = 0; j < nq; ++j)
+                         for (int k = 0; k < 4; ++k) pf_l[d][k][j] += (*save2)[k][i][j] * fl;
+               }
+          }
+     }
+     else {
+          /* round-robin partition of the sizes over threads with private
+             accumulators, reduced in thread order (src/lmie_schedule.c:33-61,
+             :217-366) */
+          int P = n_threads > 0 ? n_threads : 1;
+          if (P > c->N) P = c->N;
+          part_job *jobs = calloc(P, sizeof(part_job));
+          pthread_t *th = calloc(P, sizeof(pthread_t));
+          for (int t = 0; t < P; ++t) {
+               jobs[t].c = c; jobs[t].t = t; jobs[t].P = P; jobs[t].nq = nq; jobs[t].n1 = n1;
+               jobs[t].fv1 = fv1; jobs[t].fv2 = fv2; jobs[t].fv3 = fv3; jobs[t].fv4 = fv4;
+               jobs[t].mu = mu; jobs[t].m = m; jobs[t].m_l = m_l;
+               jobs[t].save1 = save1; jobs[t].save2 = save2;
+               jobs[t].cx_l = calloc(4 * (size_t) (nd + 1), sizeof(double));
+               jobs[t].pf = blk2(4, nq > 0 ? nq : 1);
+               jobs[t].pf_l = blk3(nd > 0 ? nd : 1, 4, nq > 0 ? nq : 1);
+               if (P > 1) pthread_create(&th[t], NULL, core_part, &jobs[t]);
+               else core_part(&jobs[t]);
+          }
+          for (int t = 0; t < P; ++t) {
+               if (P > 1) pthread_join(th[t], NULL);
+               Cext += jobs[t].cx[0]; Csca += jobs[t].cx[1]; Cbak += jobs[t].cx[2]; G += jobs[t].cx[3];
+               for (int d = 0; d < nd; ++d) {
+                    cext_l[d] += jobs[t].cx_l[4 * d]; csca_l[d] += jobs[t].cx_l[4 * d + 1];
+                    cbak_l[d] += jobs[t].cx_l[4 * d + 2]; g_l[d] += jobs[t].cx_l[4 * d + 3];
+               }
+               for (int j = 0; j < nq; ++j)
+                    for (int k = 0; k < 4; ++k) {
+                         pf[k][j] += jobs[t].pf[k][j];
+                         for (int d = 0; d < nd; ++d) pf_l[d][k][j] += jobs[t].pf_l[d][k][j];
+                    }
+               free(jobs[t].cx_l); free(jobs[t].pf); free(jobs[t].pf_l);
+          }
+          free(jobs); free(th);
      }
      /* normalisation (src/lmie_schedule.c:1158-1199) */
      double a = 2. / Csca;
@@ -1013,7 +1072,7 @@ int lmie_solution2_l(int calc_gc, int calc_lc, int calc_pf, enum size_dist_type 
                      double *cext_l, double *csca_l, double *cbak_l, double *g_l,
                      double ***gc_l, double ***lc_l, double ***pf_l, double ***save1,
                      double ****save2, int verbose, int n_threads, int use_mpi) {
-     (void) verbose; (void) n_threads; (void) use_mpi;
+     (void) verbose; (void) use_mpi;
      if (calc_pf && !calc_gc && !calc_lc) {
           fprintf(stderr, "ERROR: either calc_gc or calc_lc must be set to true when calc_pf is true\n");
           return -1;
@@ -1052,7 +1111,7 @@ int lmie_solution2_l(int calc_gc, int calc_lc, int calc_pf, enum size_dist_type 
      double cx_l[1], dd_l[1];
      *n_coef = core_solution(&c, cext, csca, cbak, g, calc_gc ? gc : NULL, calc_lc ? lc : NULL,
                              nd ? cext_l : cx_l, nd ? csca_l : dd_l, nd ? cbak_l : cx_l,
-                             nd ? g_l : dd_l, gc_l, lc_l, save1, save2, accuracy);
+                             nd ? g_l : dd_l, gc_l, lc_l, save1, save2, accuracy, n_threads);
      if (calc_pf) {
           double **xc = calc_gc ? gc : lc;
           double ***xc_l = calc_gc ? gc_l : lc_l;

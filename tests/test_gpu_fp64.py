@@ -1,0 +1,102 @@
+"""GPU parity of the FP64 path against the reference's outputs (fixtures made
+from the reference build) and against the oracle.  Normwise relative error per
+output array, bar 1e-12 (north star: FP64 path within 1e-12 of the reference)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import compare_all, load_golden, relnorm
+from lmie_amd import lmie, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+SINGLE = ["small_lognormal", "small_gamma", "small_modgamma", "small_powerlaw", "small_modpowerlaw",
+          "small_bimodal", "small_exponential", "small_mono", "cfg0", "example", "cfg1"]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_mfma_f64_lane_layout(product):
+    say("case: mfma f64 16x16x4 lane layout")
+    rng = np.random.Generator(np.random.PCG64(3))
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    D = np.zeros((16, 16))
+    f = product.lmie_gpu_selftest_mfma_f64
+    f.argtypes = [C.POINTER(C.c_double)] * 3 + [C.c_int]
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    assert f(dp(np.ascontiguousarray(A)), dp(np.ascontiguousarray(B)), dp(D), 0) == 0
+    assert np.abs(D - A @ B).max() <= 1e-14
+
+
+@pytest.mark.parametrize("n", [84, 522, 1922, 3810])
+def test_gauss_rule_is_bitwise_the_reference_rule(product, oracle, n):
+    say("case: gauss rule n=%d" % n)
+    x, w = np.zeros(n), np.zeros(n)
+    xo, wo = (C.c_double * n)(), (C.c_double * n)()
+    f = product.lmie_gpu_gauss_rule
+    f.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    assert f(n, x.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)), 0) == 0
+    oracle.oracle_gauss_leg(n, 0, -1.0, 1.0, None, None, xo, wo, None, None)
+    assert np.array_equal(x, np.array(xo))
+    assert np.array_equal(w, np.array(wo))
+
+
+@pytest.mark.parametrize("name", SINGLE)
+def test_fp64_matches_reference(product, name):
+    say("case: fp64 parity", name)
+    z, p, seeds = load_golden(name)
+    r = lmie.solve(product, p, seeds, "fp64")
+    worst = compare_all(r, z, 1e-12)
+    say("  worst %s %.2e" % (worst[1], worst[0]))
+
+
+def test_fp64_cfg2_matches_reference(product):
+    say("case: fp64 parity cfg2 (16384 sizes, x 914, 4 seeds)")
+    z, p, seeds = load_golden("cfg2")
+    r = lmie.solve(product, p, seeds, "fp64")
+    worst = compare_all(r, z, 1e-12)
+    say("  worst %s %.2e" % (worst[1], worst[0]))
+
+
+def test_fp64_cfg3_cases_match_reference(product):
+    say("case: fp64 parity cfg3 (32 retrieval cases)")
+    z = np.load(load_golden.__globals__["GOLDEN"] + "/cfg3.npz")
+    for c, (p, seeds) in enumerate(W.cfg3_cases()):
+        r = lmie.solve(product, p, None, "fp64")
+        for k in ("cext", "csca", "cbak", "g"):
+            assert abs(r[k] - z[k][c]) <= 1e-12 * abs(z[k][c]), (c, k)
+        if "gc_%d" % c in z.files:
+            assert relnorm(r["gc"], z["gc_%d" % c]) <= 1e-12, c
+
+
+def test_fp64_cfg4_two_modes_match_reference(product):
+    say("case: fp64 parity cfg4 (two modes, 32768 sizes, x 1848)")
+    z = np.load(load_golden.__globals__["GOLDEN"] + "/cfg4.npz")
+    modes = [(W.cfg4_mode_params(k), None) for k in range(2)]
+    r = lmie.solve_mix(product, modes, [m["weight"] for m in W.CFG4_MODES], "fp64")
+    for k in ("cext", "csca", "cbak", "g"):
+        assert abs(r[k] - float(z[k])) <= 1e-12 * abs(float(z[k])), k
+    e = relnorm(r["gc"], z["gc"])
+    say("  gc rel err %.2e" % e)
+    assert e <= 1e-12
+
+
+def test_save_control_per_size_arrays(product):
+    say("case: save_control 1 per-size arrays")
+    z, p, seeds = load_golden("small_lognormal")
+    r = lmie.solve(product, p, seeds, "fp64")
+    assert relnorm(r["save1"], z["save1"]) <= 1e-12
+    assert relnorm(r["save2"], z["save2"]) <= 1e-12
+
+
+def test_repeat_calls_are_deterministic(product):
+    say("case: determinism")
+    z, p, seeds = load_golden("cfg1")
+    a = lmie.solve(product, p, seeds, "fp64")
+    b = lmie.solve(product, p, seeds, "fp64")
+    for k in ("gc", "gc_l", "cext", "g_l"):
+        assert np.array_equal(a[k], b[k]), k

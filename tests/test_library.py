@@ -1,0 +1,96 @@
+"""CPU-side checks of the product library: it loads, exports every function the
+headers in include/ declare, and its host-only paths (allocation, argument
+checks that fail before any device work) behave like the reference's
+(src/lmie_interface.c:24-169, :404-408)."""
+import ctypes as C
+import glob
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+from lmie_amd import abi, lmie
+
+LIB = os.path.join(ROOT, "lmie_amd", "lib", "liblmie_amd.so")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "lmie_amd", "csrc")])
+    return lmie.bind_all(C.CDLL(LIB))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(lmie_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_the_reference_interface():
+    names = declared_functions()
+    for f in ("lmie_calc_max_coef", "lmie_in_alloc", "lmie_in_free", "lmie_out_alloc", "lmie_out_free",
+              "lmie_in_zero_derivs", "lmie_solution", "lmie_solution2", "lmie_solution2_l",
+              "lmie_solution_slave", "lmie_core_solution"):
+        assert f in names
+
+
+def test_library_exports_every_declared_function(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layout_matches_the_reference():
+    # field order and types of src/lmie_interface.h:20-101 on LP64
+    assert C.sizeof(abi.LmieIn) == 10 * 4 + 10 * 8 + 10 * 8 + 8
+    assert abi.LmieIn.lambda_.offset == 40
+    assert abi.LmieIn.accuracy.offset == 200
+    assert abi.LmieOut.gc.offset == 8 + 13 * 8
+    assert C.sizeof(abi.LmieOut) == 8 + 13 * 8 + 4 * 8 + 13 * 8 + 3 * 8 + 2 * 8
+
+
+def test_max_coef_matches_reference_formula(lib):
+    from lmie_amd import workloads as W
+    for p in (W.CFG0, W.CFG1, W.CFG2, W.cfg4_mode_params(0)):
+        x = 2 * 3.141592653589793 * p["r2"] / p["lambda"]
+        assert lib.lmie_calc_max_coef(p["lambda"], abi.SIZE_DIST[p["dist_type"]], p["a1"], p["a2"],
+                                      0.0, p["r1"], p["r2"]) == 2 * W.calc_n1(x)
+
+
+def test_alloc_and_free_roundtrip(lib):
+    p = dict(dist_type="log_normal", n_int2=2, n_quad=4, calc_gc=1, calc_lc=1, calc_pf=1, n_angles=7,
+             **{"lambda": 0.5}, mr=1.5, mi=0.01, a1=0.3, a2=0.2, r1=0.01, r2=2.0)
+    inp = abi.make_in(lib, p, {"mr": [1, 0, 0]})
+    assert inp.n_derivs == 3 and inp.mr_l[0] == 1.0 and inp.mi_l[2] == 0.0
+    out = abi.LmieOut()
+    assert lib.lmie_out_alloc(C.byref(inp), C.byref(out), 40) == 0
+    # pointer-table blocks: rows contiguous
+    a0 = C.cast(out.gc[0], C.c_void_p).value
+    a1 = C.cast(out.gc[1], C.c_void_p).value
+    assert a1 - a0 == 40 * 8
+    b0 = C.cast(out.gc_l[0][0], C.c_void_p).value
+    b1 = C.cast(out.gc_l[1][0], C.c_void_p).value
+    assert b1 - b0 == 6 * 40 * 8
+    lib.lmie_out_free(C.byref(out), 1)
+    lib.lmie_in_free(C.byref(inp), 1)
+
+
+def test_argument_errors_fail_before_device_work(lib):
+    p = dict(dist_type="log_normal", n_int2=2, n_quad=4, calc_gc=0, calc_lc=0, calc_pf=1, n_angles=7,
+             **{"lambda": 0.5}, mr=1.5, mi=0.01, a1=0.3, a2=0.2, r1=0.01, r2=2.0)
+    with pytest.raises(RuntimeError):
+        abi.solve(lib, p)                                  # calc_pf without gc or lc
+    inp = abi.make_in(lib, dict(p, calc_gc=1))
+    out = abi.LmieOut()
+    assert lib.lmie_solution(C.byref(inp), C.byref(out), 1, 0, 1, 1) == -1   # use_mpi
+    assert "MPI" in lmie.last_error(lib)
+    lib.lmie_out_free(C.byref(out), 0)
+    bad = abi.make_in(lib, dict(p, calc_gc=1, dist_type=42))
+    out = abi.LmieOut()
+    assert lib.lmie_solution(C.byref(bad), C.byref(out), 1, 0, 1, 0) == -1   # invalid distribution
+    assert lib.lmie_solution_slave() == -1

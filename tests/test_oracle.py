@@ -1,0 +1,304 @@
+"""The oracle (oracle/lmie_oracle.c) pinned before it is trusted as the checker.
+
+1. Against every full-precision fixture made from the reference build (tests/golden).
+2. Against the reference's own printed golden output, examples/example.out.
+3. Against what mathematics fixes independently of the reference: Lorenz-Mie
+   coefficients from arbitrary-precision Bessel functions, pi_n / tau_n from
+   Legendre derivatives, Gauss-Legendre nodes from numpy, generalized spherical
+   functions from Wigner d-functions, and physical invariants.
+4. Against the reference build on random inputs (where oracle/_ref exists).
+"""
+import ctypes as C
+import glob
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, compare_all, load_golden, relnorm
+from lmie_amd import abi, workloads as W
+
+NT = os.cpu_count() or 1
+
+FIXTURES = sorted(os.path.basename(f)[:-4] for f in glob.glob(os.path.join(GOLDEN, "*.npz")))
+# fixtures that describe a single lmie_solution call
+SINGLE = [f for f in FIXTURES if f.startswith("small_") or f in ("cfg0", "cfg1", "example")]
+
+
+@pytest.mark.parametrize("name", SINGLE)
+def test_oracle_matches_reference_fixture(oracle, name):
+    z, p, seeds = load_golden(name)
+    r = abi.solve(oracle, p, seeds, n_threads=NT)
+    compare_all(r, z, 1e-12)
+
+
+def test_oracle_matches_reference_cfg2(oracle):
+    z, p, seeds = load_golden("cfg2")
+    r = abi.solve(oracle, p, seeds, n_threads=NT)
+    compare_all(r, z, 1e-12)
+
+
+def _parse_example_out(path):
+    """Blocks of examples/example.out: scalars, gc rows, pf rows per section."""
+    blocks, cur, mode = [], None, None
+    for line in open(path):
+        s = line.strip()
+        if s.startswith("full quantities") or s.startswith("derivatives wrt"):
+            cur = {"scal": {}, "gc": [], "pf": []}
+            blocks.append(cur)
+            mode = None
+        elif cur is None or not s or s.startswith("---"):
+            continue
+        elif "=" in s:
+            k, v = [t.strip() for t in s.split("=")]
+            cur["scal"][k] = float(v)
+        elif s.startswith("generalized"):
+            mode = "gc"
+        elif s.startswith("elements of"):
+            mode = "pf"
+        else:
+            try:
+                vals = [float(t) for t in s.split()]
+            except ValueError:
+                continue
+            if mode == "gc" and len(vals) == 6:
+                cur["gc"].append(vals)
+            elif mode == "pf" and len(vals) == 7:
+                cur["pf"].append(vals)
+    return blocks
+
+
+def test_oracle_matches_reference_printed_example(oracle):
+    """examples/example.out is printed with %14E (7 significant digits)."""
+    blocks = _parse_example_out(os.path.join(GOLDEN, "example.out"))
+    assert len(blocks) == 5
+    r = abi.solve(oracle, W.EXAMPLE, W.EXAMPLE_SEEDS, n_threads=NT)
+    for b, blk in enumerate(blocks):
+        suf = "" if b == 0 else "_l"
+        pick = (lambda a: a) if b == 0 else (lambda a, d=b - 1: a[d])
+        for k, v in blk["scal"].items():
+            if k == "n_coef":
+                assert int(v) == r["n_coef"]
+                continue
+            got = pick(r[k + suf])
+            assert abs(got - v) <= 6e-7 * abs(v) + 1e-300, (b, k, got, v)
+        gc = np.array(blk["gc"]).T
+        ours = pick(r["gc" + suf])[:, :gc.shape[1]]
+        scale = np.abs(gc).max()
+        assert np.all(np.abs(ours - gc) <= 6e-7 * np.abs(gc) + 1e-9 * scale), b
+        pf = np.array(blk["pf"]).T
+        assert np.allclose(r["theta"], pf[0], rtol=1e-9, atol=0)
+        ours = pick(r["pf" + suf])
+        scale = np.abs(pf[1:]).max()
+        assert np.all(np.abs(ours - pf[1:]) <= 6e-7 * np.abs(pf[1:]) + 1e-9 * scale), b
+
+
+# --------------------------------------------------------------------------
+# independent mathematics
+
+def _mp_ab(x, mr, mi, nmax, dps=40):
+    """a_n, b_n from exact spherical Bessel functions (mpmath), in the
+    convention of the reference: m = mr - i mi, zeta_n = psi_n + i chi_n with
+    chi_n(x) = -x y_n(x), and D_n = psi_n'(mx) / psi_n(mx)."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = dps
+    X = mp.mpf(x)
+    m = mp.mpc(mr, -mi)
+    z = m * X
+
+    def jn(n, t):
+        return mp.sqrt(mp.pi / (2 * t)) * mp.besselj(n + mp.mpf(1) / 2, t)
+
+    def yn(n, t):
+        return mp.sqrt(mp.pi / (2 * t)) * mp.bessely(n + mp.mpf(1) / 2, t)
+
+    def psi(n, t):
+        return t * jn(n, t)
+
+    def chi(n, t):
+        return -t * yn(n, t)
+
+    a, b = [0], [0]
+    for n in range(1, nmax + 1):
+        # D_n = psi_{n-1}(z) / psi_n(z) - n / z
+        D = psi(n - 1, z) / psi(n, z) - n / z
+        ps, psm = psi(n, X), psi(n - 1, X)
+        ze, zem = ps + 1j * chi(n, X), psm + 1j * chi(n - 1, X)
+        u = D / m + n / X
+        v = m * D + n / X
+        a.append(complex((u * ps - psm) / (u * ze - zem)))
+        b.append(complex((v * ps - psm) / (v * ze - zem)))
+    return np.array(a), np.array(b)
+
+
+@pytest.mark.parametrize("x,mr,mi", [(0.3, 1.33, 1e-8), (2.7, 1.5, 0.01), (11.0, 1.45, 0.001),
+                                     (35.0, 1.53, 0.008), (80.0, 1.6, 0.05)])
+def test_oracle_mie_coefficients_against_bessel_functions(oracle, x, mr, mi):
+    n1 = oracle.oracle_order_n1(x)
+    a = (C.c_double * (2 * (n1 + 1)))()
+    b = (C.c_double * (2 * (n1 + 1)))()
+    assert oracle.oracle_mie_ab(x, mr, mi, n1 + 1, a, b) == n1
+    ao = np.array(a).reshape(-1, 2) @ np.array([1, 1j])
+    bo = np.array(b).reshape(-1, 2) @ np.array([1, 1j])
+    # the reference truncates psi at n1 (r_{n1} = 0), so the last order is an
+    # approximation; compare orders 1 .. n1 - 1
+    nmax = n1 - 1
+    ae, be = _mp_ab(x, mr, mi, nmax)
+    sc = max(np.abs(ae).max(), np.abs(be).max())
+    assert np.abs(ao[1:nmax + 1] - ae[1:]).max() <= 1e-11 * sc
+    assert np.abs(bo[1:nmax + 1] - be[1:]).max() <= 1e-11 * sc
+
+
+@pytest.mark.parametrize("mu", [-0.97, -0.3, 0.0, 0.41, 0.999])
+def test_oracle_pi_tau_are_legendre_derivatives(oracle, mu):
+    n = 40
+    pi = (C.c_double * n)()
+    tau = (C.c_double * n)()
+    oracle.oracle_pitau(mu, n, pi, tau)
+    L = np.polynomial.legendre
+    for k in range(1, n + 1):
+        c = np.zeros(k + 1)
+        c[k] = 1
+        d1 = L.legval(mu, L.legder(c, 1))
+        d2 = L.legval(mu, L.legder(c, 2))
+        assert abs(pi[k - 1] - d1) <= 1e-11 * max(1, abs(d1)) * k
+        t = mu * d1 - (1 - mu * mu) * d2
+        assert abs(tau[k - 1] - t) <= 1e-11 * max(1, abs(t)) * k * k
+
+
+@pytest.mark.parametrize("n", [2, 7, 84, 200, 522])
+def test_oracle_gauss_nodes_against_numpy(oracle, n):
+    x = (C.c_double * n)()
+    w = (C.c_double * n)()
+    oracle.oracle_gauss_leg(n, 0, -1.0, 1.0, None, None, x, w, None, None)
+    xe, we = np.polynomial.legendre.leggauss(n)
+    assert np.abs(np.array(x) - xe).max() <= 4e-15
+    # the weight formula of the reference evaluates P_{n-1} at the last-but-one
+    # Newton iterate; near mu = +-1 that costs accuracy growing like n^2
+    # (8.7e-10 relative at n = 522 against numpy, whose own error is ~1e-12)
+    assert np.all(np.abs(np.array(w) - we) <= 4e-15 * n * n * we)
+    assert abs(sum(w) - 2.0) <= 1e-12 * n
+
+
+@pytest.mark.parametrize("n", [84, 522, 1922, 3810])
+def test_oracle_gauss_rule_is_the_reference_rule_bitwise(oracle, reflib, n):
+    x, w = (C.c_double * n)(), (C.c_double * n)()
+    x2, w2 = (C.c_double * n)(), (C.c_double * n)()
+    oracle.oracle_gauss_leg(n, 0, -1.0, 1.0, None, None, x, w, None, None)
+    reflib.gauss_leg_quadx.argtypes = [C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+    reflib.gauss_leg_quadx(n, -1.0, 1.0, x2, w2)
+    assert np.array_equal(np.array(x), np.array(x2))
+    assert np.array_equal(np.array(w), np.array(w2))
+
+
+def _wigner_d(j, mp_, m, beta):
+    """Wigner small-d d^j_{m' m}(beta) by its explicit sum, in 40-digit arithmetic."""
+    mpm = pytest.importorskip("mpmath")
+    mpm.mp.dps = 40
+    f = mpm.factorial
+    b = mpm.mpf(beta)
+    c, s = mpm.cos(b / 2), mpm.sin(b / 2)
+    pre = mpm.sqrt(f(j + mp_) * f(j - mp_) * f(j + m) * f(j - m))
+    tot = mpm.mpf(0)
+    for k in range(0, 2 * j + 1):
+        a1, a2, a3, a4 = j + m - k, k, mp_ - m + k, j - mp_ - k
+        if min(a1, a2, a3, a4) < 0:
+            continue
+        tot += (-1) ** (mp_ - m + k) * c ** (2 * j + m - mp_ - 2 * k) * s ** (mp_ - m + 2 * k) / (
+            f(a1) * f(a2) * f(a3) * f(a4))
+    return float(pre * tot)
+
+
+@pytest.mark.parametrize("mu", [-0.9, -0.2, 0.35, 0.8])
+def test_oracle_generalized_spherical_functions(oracle, mu):
+    n = 25
+    P = [(C.c_double * n)() for _ in range(4)]
+    oracle.oracle_gsf(n, mu, *P)
+    P = [np.array(p) for p in P]
+    beta = math.acos(mu)  # exact enough: d depends smoothly on beta
+    for l in range(n):
+        leg = np.polynomial.legendre.legval(mu, np.eye(n)[l])
+        assert abs(P[0][l] - leg) <= 1e-13
+        if l >= 2:
+            # P^l_{mn} = i^(m-n) d^l_{mn}: P^{0,2} = -d^l_{0 2}, P^{2,2} = d^l_{2 2},
+            # P^{2,-2} = d^l_{2,-2}
+            assert abs(P[1][l] + _wigner_d(l, 0, 2, beta)) <= 1e-12
+            assert abs(P[2][l] - _wigner_d(l, 2, 2, beta)) <= 1e-12
+            assert abs(P[3][l] - _wigner_d(l, 2, -2, beta)) <= 1e-12
+        else:
+            assert P[1][l] == P[2][l] == P[3][l] == 0.0
+
+
+INVARIANT_CASES = [
+    dict(dist_type="log_normal", n_int2=4, n_quad=16, **{"lambda": 0.63}, mr=1.41, mi=0.003,
+         a1=0.7, a2=0.3, r1=0.02, r2=6.0),
+    dict(dist_type="gamma", n_int2=5, n_quad=12, **{"lambda": 1.1}, mr=1.55, mi=0.02,
+         a1=1.5, a2=0.2, r1=0.05, r2=9.0),
+]
+
+
+@pytest.mark.parametrize("p", INVARIANT_CASES)
+def test_oracle_invariants(oracle, p):
+    q = dict(p, calc_gc=1, calc_pf=1, n_angles=361, accuracy=-1.0)
+    r = abi.solve(oracle, q, {"mr": [1, 0], "a1": [0, 1]}, n_threads=NT)
+    gc, pf = r["gc"], r["pf"]
+    assert abs(gc[0, 0] - 1.0) <= 1e-12                        # beta_0 = 1
+    assert abs(r["g"] - gc[0, 1] / 3.0) <= 1e-10 * abs(r["g"])  # g = beta_1 / 3
+    # C_bak = F11(180) C_sca / (4 pi)
+    assert abs(r["cbak"] - pf[0, -1] * r["csca"] / (4 * math.pi)) <= 1e-8 * r["cbak"]
+    assert r["csca"] <= r["cext"]
+    assert np.allclose(pf[0], pf[1], rtol=1e-10, atol=1e-12 * pf[0].max())  # F11 = F22
+    assert np.allclose(pf[2], pf[3], rtol=1e-10, atol=1e-12 * pf[0].max())  # F33 = F44
+    assert np.all(r["gc_l"][:, 0, 0] == 0.0)
+
+
+def test_oracle_extinction_paradox_and_rayleigh(oracle):
+    # radii chosen away from x = k pi: there sin x ~ 0 and the ratio method of
+    # the reference (psi_n = sin x prod r_k, src/lmie_core.c:298-330) loses its
+    # accuracy; the oracle reproduces that behaviour of the reference
+    r = 80.3
+    big = abi.solve(oracle, dict(dist_type="mono", n_int2=1, n_quad=1, **{"lambda": 0.53}, mr=1.5,
+                                 mi=0.01, a1=r, r1=r, r2=r, calc_gc=0))
+    assert abs(big["cext"] / (math.pi * r * r) - 2.0) < 0.05    # Q_ext - 2 ~ x^(-2/3)
+    r = 0.001
+    lam = 0.6
+    small = abi.solve(oracle, dict(dist_type="mono", n_int2=1, n_quad=1, **{"lambda": lam}, mr=1.5,
+                                   mi=0.0, a1=r, r1=r, r2=r, calc_gc=0))
+    x = 2 * math.pi * r / lam
+    m = 1.5
+    qsca = 8.0 / 3.0 * x ** 4 * ((m * m - 1) / (m * m + 2)) ** 2
+    assert abs(small["csca"] / (math.pi * r * r) / qsca - 1.0) < 1e-3
+
+
+def test_oracle_tangents_against_central_differences(oracle):
+    """Forward tangents of the oracle against central differences of the oracle."""
+    p = dict(dist_type="log_normal", n_int2=3, n_quad=12, calc_gc=1, **{"lambda": 0.7}, mr=1.45,
+             mi=0.01, a1=0.4, a2=0.2, r1=0.05, r2=3.0, accuracy=-1.0)
+    r = abi.solve(oracle, p, {"mr": [1, 0, 0], "mi": [0, 1, 0], "a1": [0, 0, 1]}, n_threads=NT)
+    for d, key in enumerate(("mr", "mi", "a1")):
+        h = 1e-6 * max(abs(p[key]), 1e-3)
+        hi = abi.solve(oracle, dict(p, **{key: p[key] + h}), n_threads=NT)
+        lo = abi.solve(oracle, dict(p, **{key: p[key] - h}), n_threads=NT)
+        for out in ("cext", "csca", "cbak", "g"):
+            fd = (hi[out] - lo[out]) / (2 * h)
+            assert abs(r[out + "_l"][d] - fd) <= 1e-6 * max(abs(fd), 1e-8 * abs(r[out]) / h * 1e-6), (key, out)
+        fd = (hi["gc"] - lo["gc"]) / (2 * h)
+        assert relnorm(r["gc_l"][d][:, 1:], fd[:, 1:]) <= 1e-5, key
+
+
+def test_oracle_against_reference_random(oracle, reflib):
+    rng = np.random.Generator(np.random.PCG64(7))
+    for _ in range(4):
+        p = dict(dist_type=("log_normal", "gamma")[int(rng.integers(2))], n_int2=int(rng.integers(1, 4)),
+                 n_quad=int(rng.integers(4, 20)), calc_gc=1, calc_lc=1, calc_pf=1, n_angles=19,
+                 **{"lambda": float(rng.uniform(0.3, 2.0))}, mr=float(rng.uniform(1.3, 1.7)),
+                 mi=float(10 ** rng.uniform(-4, -1)), a1=float(rng.uniform(0.2, 1.2)),
+                 a2=float(rng.uniform(0.1, 0.3)), r1=0.01, r2=float(rng.uniform(1, 6)), accuracy=1e-9)
+        seeds = {"mr": [1, 0, 0], "a1": [0, 1, 0], "lambda": [0, 0, 1]}
+        a = abi.solve(oracle, p, seeds, n_threads=NT)
+        b = abi.solve(reflib, p, seeds, 2)
+        assert a["n_coef"] == b["n_coef"]
+        for k in ("cext", "csca", "cbak", "g", "gc", "lc", "pf", "gc_l", "pf_l", "cext_l", "g_l"):
+            assert relnorm(a[k], b[k]) <= 1e-12, k
