@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--path", default="fp64", choices=["fp64"])
+    ap.add_argument("--path", default="mixed", choices=["fp64", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -86,34 +86,44 @@ def main():
     units = sum_n1(W.cfg4_mode_params(0))
     nq = 2 * W.calc_n1(2 * math.pi * W.CFG4_GRID["r2"] / W.CFG4_GRID["lambda"])
 
-    def step():
-        return lmie.solve_mix(lib, modes, weights, a.path)
+    def run(path, steps, warmup):
+        for _ in range(warmup):
+            lmie.solve_mix(lib, modes, weights, path)
+        lib.lmie_set_timing(1)
+        t_ang, t_all = [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lmie.solve_mix(lib, modes, weights, path)
+            tm = lmie.timings(lib)
+            t_ang.append(tm["angular"])
+            t_all.append(tm)
+        dt = (time.perf_counter() - t0) / steps
+        lib.lmie_set_timing(0)
+        stages = {k: float(np.mean([t[k] for t in t_all])) for k in t_all[0]}
+        return dt, float(np.mean(t_ang)), stages
 
-    lib.lmie_set_timing(1)
-    for _ in range(a.warmup):
-        step()
-    t_ang = []
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        step()
-        t_ang.append(lmie.timings(lib)["angular"])
-    dt = (time.perf_counter() - t0) / a.steps
-    lib.lmie_set_timing(0)
+    dt, ang_ms, stages = run(a.path, a.steps, a.warmup)
+    ref_dt = None
+    if a.path != "fp64":
+        ref_dt, ref_ang, ref_stages = run("fp64", max(2, a.steps // 3), 1)
 
     flop = 8.0 * nq * units
-    ang_ms = float(np.mean(t_ang))
     achieved = flop / (ang_ms * 1e-3)
-    peak = PEAK["fp64"]
+    peak = PEAK["fp64"] if a.path == "fp64" else PEAK["bf16"]
     line = {
         "metric": METRIC, "value": units / dt, "unit": "size-pt*n-term/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if a.path == "fp64" else ("f16-split/f32-acc+f64" if a.path == "mixed" else "bf16-split/f32-acc+f64"),
         "data": "synthetic: generated size grid of two log-normal modes",
         "config": {"workload": "two-mode log-normal, 32768 sizes, x_max 1848, 3810 angles, "
-                               "gc 6x3810", "path": a.path, "sum_n1": units, "n_qang": nq},
+                               "gc 6x3810", "path": a.path, "sum_n1": units, "n_qang": nq,
+                   "stage_ms": stages,
+                   "fp64_ms_per_step": None if ref_dt is None else ref_dt * 1e3,
+                   "speedup_vs_fp64": None if ref_dt is None else ref_dt / dt},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "kernel": "angular_f64", "kernel_ms": ang_ms},
+                     "kernel": "angular_f64" if a.path == "fp64" else "angular_mix", "kernel_ms": ang_ms},
         "cpu_baseline": None,
     }
     if not a.no_cpu_baseline:

@@ -38,12 +38,14 @@ extern "C" {
 /* Arithmetic of the angular contraction.  FP64: every stage in double.
    MIXED: the Mie coefficients, cross sections, reductions, normalisation and
    expansion stay in double; the angular sums S1/S2 (and their adjoint) run on
-   the matrix cores with each operand split into a high and a low 16-bit part
-   (three products, fp32 accumulation per block of orders, blocks summed in
-   double). */
+   the matrix cores with each operand split into a high and a low 16-bit part,
+   three products, fp32 accumulation; MIXED uses fp16 parts (operands scaled
+   into the fp16 range by exact powers of two), MIXED_BF16 bf16 parts.
+   Forward tangents are always computed in FP64. */
 enum lmie_precision {
-     LMIE_PRECISION_FP64  = 0,
-     LMIE_PRECISION_MIXED = 1
+     LMIE_PRECISION_FP64       = 0,
+     LMIE_PRECISION_MIXED      = 1,
+     LMIE_PRECISION_MIXED_BF16 = 2
 };
 
 typedef struct {

@@ -31,6 +31,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "../../include/lmie_gpu.h"
@@ -43,6 +44,56 @@ constexpr int KC = 16;     // orders per LDS stage of the contraction
 constexpr int TILE = 64;   // size-point tile of the padded layouts
 
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef __bf16 v8b __attribute__((ext_vector_type(8)));
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+
+// Mixed path scaling: the split operands hold (2n+1)(a_n +- b_n) * kSA and
+// (pi_n +- tau_n) / (n(n+1)) * kSB, so that their product is
+// fv4_n (a_n +- b_n)(pi_n +- tau_n) * kSA * kSB with both factors inside the
+// normal range of fp16 (|A| <= 4 * 7622, |B| <= 2^14); powers of two, so the
+// scaling is exact.
+constexpr double kSA = 4.0, kSB = 16384.0;
+constexpr int KM = 32;     // orders per LDS stage of the mixed contraction (4 groups of 8)
+
+// 16-bit element types of the split operands
+struct F16 {
+     typedef _Float16 T;
+     typedef v8h V;
+     static __device__ __forceinline__ unsigned short bits(double v, double *rounded) {
+          _Float16 h = (_Float16) (float) v;
+          *rounded = (double) (float) h;
+          return __builtin_bit_cast(unsigned short, h);
+     }
+     static __device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c) {
+          return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0, 0);
+     }
+};
+struct BF16 {
+     typedef __bf16 T;
+     typedef v8b V;
+     static __device__ __forceinline__ unsigned short bits(double v, double *rounded) {
+          __bf16 h = (__bf16) (float) v;
+          *rounded = (double) (float) h;
+          return __builtin_bit_cast(unsigned short, h);
+     }
+     static __device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c) {
+          return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8b, a), __builtin_bit_cast(v8b, b), c, 0, 0, 0);
+     }
+};
+
+// high and low 16-bit parts of v: hi = round(v), lo = round(v - hi)
+template <class E>
+__device__ __forceinline__ void split16(double v, unsigned short *hi, unsigned short *lo) {
+     double r, r2;
+     *hi = E::bits(v, &r);
+     *lo = E::bits(v - r, &r2);
+}
+
+// Row of the 16x16 f32 result held in lane l, element v, for the 16-bit
+// matrix instructions (checked by lmie_gpu_selftest_mfma16).
+__device__ __forceinline__ int mfma16_row(int l, int v) { return 4 * (l >> 4) + v; }
 
 thread_local char g_err[512];
 thread_local double g_times[LMIE_N_TIMERS];
@@ -105,7 +156,7 @@ __global__ void gauss_nodes(int n, const double *guess, double *mu, double *w) {
      int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= (n + 1) / 2) return;
      double y = guess[i], y0, pa, pb;
-     const double nn1 = (double) (n * (n + 1));
+     const double dn = (double) n, nn1 = (double) (n * (n + 1));
      int it = 0;
      do {
           double pm;
@@ -114,28 +165,31 @@ __global__ void gauss_nodes(int n, const double *guess, double *mu, double *w) {
           for (int j = 0; j < n; ++j) {
                pm = pb;
                pb = pa;
-               const double u = y * pb;
-               pa = j / (j + 1.) * (u - pm) + u;
+               const double u = __dmul_rn(y, pb);
+               pa = __dadd_rn(__dmul_rn(__ddiv_rn((double) j, __dadd_rn((double) j, 1.)), __dsub_rn(u, pm)), u);
           }
-          const double s = 1. - y * y;
-          const double d1 = n * (pb - y * pa) / s;
-          const double d2 = (2. * y * d1 - nn1 * pa) / s;
-          const double h = pa / d1;
+          const double s = __dsub_rn(1., __dmul_rn(y, y));
+          const double d1 = __ddiv_rn(__dmul_rn(dn, __dsub_rn(pb, __dmul_rn(y, pa))), s);
+          const double d2 = __ddiv_rn(__dsub_rn(__dmul_rn(__dmul_rn(2., y), d1), __dmul_rn(nn1, pa)), s);
+          const double h = __ddiv_rn(pa, d1);
           y0 = y;
-          y = y0 - h * (1. + h * d2 / (2. * d1));
-     } while (fabs(y - y0) > 2.220446049250313e-16 && ++it < 100);
-     double c = n * pb;
-     c = 2. / (c * c);
-     const double off = 1. * y;
-     mu[i] = 0. - off;
-     mu[n - 1 - i] = 0. + off;
-     w[i] = w[n - 1 - i] = (1. - off * y) * c;
+          y = __dsub_rn(y0, __dmul_rn(h, __dadd_rn(1., __ddiv_rn(__dmul_rn(h, d2), __dmul_rn(2., d1)))));
+     } while (fabs(__dsub_rn(y, y0)) > 2.220446049250313e-16 && ++it < 100);
+     double c = __dmul_rn(dn, pb);
+     c = __ddiv_rn(2., __dmul_rn(c, c));
+     const double off = __dmul_rn(1., y);
+     mu[i] = __dsub_rn(0., off);
+     mu[n - 1 - i] = __dadd_rn(0., off);
+     w[i] = w[n - 1 - i] = __dmul_rn(__dsub_rn(1., __dmul_rn(off, y)), c);
 }
 
 // pi_n +- tau_n at the positive angles mu_{H+j}: one thread per angle, upward
 // recurrences pi_{n+1} = mu pi_n + (n+1)/n (mu pi_n - pi_{n-1}),
 // tau_n = n (mu pi_n - pi_{n-1}) - pi_{n-1} (src/lmie_core.c:669-691).
 __global__ void pitau_table(int H, int Hp, int n1max, int Kp, const double *mu, double *Bp, double *Bm) {
+     // separately rounded operations in the reference's order: the table is
+     // the reference's pi_n, tau_n bit for bit
+#pragma clang fp contract(off)
      int j = blockIdx.x * blockDim.x + threadIdx.x;
      if (j >= Hp) return;
      if (j >= H) {
@@ -154,18 +208,71 @@ __global__ void pitau_table(int H, int Hp, int n1max, int Kp, const double *mu, 
      for (int n = n1max + 1; n <= Kp; ++n) Bp[(size_t) (n - 1) * Hp + j] = Bm[(size_t) (n - 1) * Hp + j] = 0.;
 }
 
+// Split 16-bit tables for the mixed path, blocked by 8 orders:
+// [4 planes: (pi+tau) hi, lo, (pi-tau) hi, lo][Kb][Hp][8], value
+// (pi_n +- tau_n) / (n (n+1)) * kSB.  A lane (angle) writes 16 bytes per plane
+// per group of 8 orders; consecutive lanes are consecutive 16-byte chunks.
+template <class E>
+__global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) {
+     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+     if (j >= Hp) return;
+     const size_t plane = (size_t) Kb * Hp * 8;
+     const double m = j < H ? mu[H + j] : 0.;
+     double p0 = 0., p1 = 1.;
+     for (int g = 0; g < Kb; ++g) {
+          unsigned short v[4][8];
+          for (int e = 0; e < 8; ++e) {
+               const int n = 8 * g + e + 1;
+               double bp = 0., bm = 0.;
+               if (j < H && n <= n1max) {
+                    const double a1 = m * p1, a2 = a1 - p0;
+                    const double tau = n * a2 - p0;
+                    const double sc = kSB / (n * (n + 1.));
+                    bp = (p1 + tau) * sc;
+                    bm = (p1 - tau) * sc;
+                    p0 = p1;
+                    p1 = a1 + (n + 1.) / n * a2;
+               }
+               split16<E>(bp, &v[0][e], &v[1][e]);
+               split16<E>(bm, &v[2][e], &v[3][e]);
+          }
+          for (int q = 0; q < 4; ++q) {
+               v4u w;
+               for (int k = 0; k < 4; ++k) w[k] = (unsigned) v[q][2 * k] | ((unsigned) v[q][2 * k + 1] << 16);
+               *(v4u *) (B16 + q * plane + ((size_t) g * Hp + j) * 8) = w;
+          }
+     }
+}
+
 // Generalized spherical functions at the positive angles, l = 0 .. C-1, by
 // their three-term recurrences in l (misc/rtutil_math.c:956-1016).  Layout
 // [4][C][Hp]: lanes (angles) store contiguously for every l.
-__global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) {
+__global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T, int legendre) {
+     // separately rounded operations in the reference's order (bitwise the
+     // reference's functions); plane 4 (if legendre) holds P_l by the
+     // recurrence of misc/rtutil_math.c:506-513 for the Legendre expansion
+#pragma clang fp contract(off)
      int j = blockIdx.x * blockDim.x + threadIdx.x;
      if (j >= H) return;
      double m = mu[H + j], m2 = m * m;
      size_t plane = (size_t) C * Hp;
      double *P00 = T, *P02 = T + plane, *P22 = T + 2 * plane, *P2m = T + 3 * plane;
      double a00 = 1., b00 = m, a02 = 0., b02 = 0., a22 = 0., b22 = 0., a2m = 0., b2m = 0.;
+     double *PL = T + 4 * plane, lp0 = 1., lp1 = m;
      for (int l = 0; l < C; ++l) {
           double v00, v02, v22, v2m;
+          if (legendre) {
+               double v;
+               if (l == 0) v = 1.;
+               else if (l == 1) v = m;
+               else {
+                    const double a = (l - 1) / (double) l, b = m * lp1;
+                    v = a * (b - lp0) + b;
+                    lp0 = lp1;
+                    lp1 = v;
+               }
+               PL[(size_t) l * Hp + j] = v;
+          }
           if (l == 0) { v00 = 1.; v02 = v22 = v2m = 0.; }
           else if (l == 1) { v00 = m; v02 = v22 = v2m = 0.; }
           else if (l == 2) {
@@ -174,13 +281,15 @@ __global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) {
                v22 = .25 * (1. + m) * (1. + m);
                v2m = .25 * (1. - m) * (1. - m);
           } else {
-               double L = l - 1, L1 = l, t = 2. * L + 1.;
-               double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
-               double c = t * L * L1 * m, k4 = 4. * t, kb = L1 * (L * L - 4.), kd = L * (L1 * L1 - 4.);
-               v00 = (t * m * b00 - L * a00) / L1;
-               v02 = (t * m * b02 - sa * a02) / sb;
-               v22 = ((c - k4) * b22 - kb * a22) / kd;
-               v2m = ((c + k4) * b2m - kb * a2m) / kd;
+               const double L = l - 1, L1 = l, t = 2. * L + 1.;
+               const double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
+               const double k4 = t * 4, kb = L1 * (L * L - 4), kd = L * (L1 * L1 - 4);
+               const double da = t * m;
+               v00 = (da * b00 - L * a00) / L1;
+               v02 = (da * b02 - sa * a02) / sb;
+               const double dc = t * (L * L1) * m;
+               v22 = ((dc - k4) * b22 - kb * a22) / kd;
+               v2m = ((dc + k4) * b2m - kb * a2m) / kd;
           }
           P00[(size_t) l * Hp + j] = v00;
           P02[(size_t) l * Hp + j] = v02;
@@ -200,6 +309,7 @@ __global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) {
 struct CoefArgs {
      int N, Np, n1max;
      const double *x;           // [Np]
+     const double *sx, *cx;     // [Np] sin x, cos x (from the host's C library, as the reference)
      const int *n1, *n2;        // [Np]
      const int *kfill;          // [Np / TILE] rows to zero-fill up to
      double mr, mi;
@@ -214,14 +324,37 @@ struct CoefArgs {
      double *S;                 // series [4][Np]: ext, sca, |c_bak|^2, asym
      double *St[4];             // tangent series of this pass, each [4][Np]
      int write_base;            // write A and S of the base channel
+     unsigned short *A16;       // mixed path: [8 planes][Kb][Np][8] split operand
+     int Kb;                    // groups of 8 orders in A16
 };
 
-template <int NT>
+// E = void: FP64 operand planes; E = F16 / BF16: split 16-bit operand, blocked
+// by 8 orders ([8 planes: S re hi, lo, S im hi, lo, T re hi, lo, T im hi, lo]
+// [Kb][Np][8], value (2n+1)(a_n +- b_n) * kSA), so every lane stores 16
+// contiguous bytes per plane and the contraction loads its operand fragments
+// without a transpose.
+template <int NT, class E = void>
 __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
+     // separately rounded operations in the reference's order and association
+     // (src/lmie_core.c:298-611): the real recurrences reproduce the
+     // reference's bits; complex divisions use Smith's scaling (ulp-level
+     // differences that the downward recurrences damp)
+#pragma clang fp contract(off)
+     constexpr bool MIX = !std::is_void<E>::value;
      int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= p.Np) return;
      const size_t Np = p.Np, plane = (size_t) p.Kp * Np, srow = (size_t) (p.n1max + 1) * Np;
      int kf = p.kfill[i / TILE];
+     const size_t p16 = (size_t) p.Kb * Np * 8;
+     if constexpr (MIX) {
+          if (i >= p.N) {
+               const v4u z = {0u, 0u, 0u, 0u};
+               for (int g = 0; g < kf / 8; ++g)
+                    for (int q = 0; q < 8; ++q) *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = z;
+               for (int q = 0; q < 4; ++q) p.S[q * Np + i] = 0.;
+               return;
+          }
+     }
      if (i >= p.N) {
           // padded columns: zero operand rows and series
           for (int r = 0; r < kf; ++r)
@@ -239,8 +372,7 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
      const int n1 = p.n1[i], n2 = p.n2[i];
      const cd m = cmk(p.mr, -p.mi);
      const cd z = cscale(m, x);
-     const cd iz = crecip(z);
-     const cd im = crecip(m);
+     const cd m2 = cmul(m, m);
      cd dz[NT > 0 ? NT : 1], dmv[NT > 0 ? NT : 1];
      double dxv[NT > 0 ? NT : 1];
 #pragma unroll
@@ -256,14 +388,13 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
 #pragma unroll
      for (int t = 0; t < NT; ++t) dD[t] = cmk(0., 0.);
      for (int j = n2 - 1; j >= 1; --j) {
-          cd c1 = cscale(iz, (double) (j + 1));
-          cd c2 = cadd(D, c1);
-          cd rc = crecip(c2);
-          D = csub(c1, rc);
+          const cd c1 = cdiv(cmk(j + 1., 0.), z);
+          const cd c2 = cadd(D, c1);
+          D = csub(c1, crecip(c2));
 #pragma unroll
           for (int t = 0; t < NT; ++t) {
-               cd dc1 = cmul(cmul(cscale(c1, -1.), iz), dz[t]);
-               dD[t] = cadd(dc1, cmul(cadd(dD[t], dc1), cmul(rc, rc)));
+               const cd c5 = cmul(cdiv(cscale(c1, -1.), z), dz[t]);
+               dD[t] = cadd(c5, cdiv(cadd(dD[t], c5), cmul(c2, c2)));
           }
           if (j <= n1) {
                Dre[j * Np + i] = D.r;
@@ -284,11 +415,12 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
 #pragma unroll
           for (int t = 0; t < NT; ++t) p.scrR[(size_t) (1 + t) * srow + n1 * Np + i] = 0.;
           for (int j = n1 - 1; j >= 1; --j) {
-               double c = (2. * j + 1.) / x;
-               double rn = 1. / (c - r);
+               const double c = (2. * j + 1.) / x;
+               const double rn = 1. / (c - r);
+               const double a2 = -c / x, a3 = rn * rn;
 #pragma unroll
                for (int t = 0; t < NT; ++t) {
-                    dr[t] = (c / x * dxv[t] + dr[t]) * rn * rn;
+                    dr[t] = -(a2 * dxv[t] - dr[t]) * a3;
                     p.scrR[(size_t) (1 + t) * srow + j * Np + i] = dr[t];
                }
                r = rn;
@@ -296,15 +428,16 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           }
      }
      // upward sweep
-     double sx, cx;
-     sincos(x, &sx, &cx);
-     double psi0 = sx, chi0 = cx, chi1 = cx / x + sx;
+     const double sx = p.sx[i], cx = p.cx[i];
+     const double ax = 1. / x;
+     double psi0 = sx, chi0 = cx, chi1 = ax * cx + sx;
      double dpsi0[NT > 0 ? NT : 1], dchi0[NT > 0 ? NT : 1], dchi1[NT > 0 ? NT : 1];
 #pragma unroll
      for (int t = 0; t < NT; ++t) {
-          dpsi0[t] = cx * dxv[t];
-          dchi0[t] = -sx * dxv[t];
-          dchi1[t] = -dxv[t] / (x * x) * cx + dchi0[t] / x + cx * dxv[t];
+          const double a2l = -ax / x * dxv[t];
+          dpsi0[t] = dxv[t] * cx;
+          dchi0[t] = dxv[t] * -sx;
+          dchi1[t] = a2l * cx + ax * dchi0[t] + dxv[t] * cx;
      }
      double se = 0., ss = 0., sg = 0.;
      cd cb = cmk(0., 0.), ap = cmk(0., 0.), bp = cmk(0., 0.);
@@ -316,6 +449,7 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           dcb[t] = dap[t] = dbp[t] = cmk(0., 0.);
      }
      double sgn = -1.;
+     unsigned short buf[8][8];      // mixed path: 8 planes x 8 orders of the current group
      for (int j = 1; j <= n1; ++j) {
           const double r = R[j * Np + i];
           const double psi = r * psi0;
@@ -323,7 +457,7 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           const double jx = j / x;
           const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
           // a_j with u = D_j / m + j / x; b_j with v = m D_j + j / x
-          const cd u = cadd(cmul(Dj, im), cmk(jx, 0.));
+          const cd u = cadd(cdiv(Dj, m), cmk(jx, 0.));
           const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
           const cd na = csub(cscale(u, psi), cmk(psi0, 0.)), da = csub(cmul(u, zj), zjm);
           const cd nb = csub(cscale(v, psi), cmk(psi0, 0.)), db = csub(cmul(v, zj), zjm);
@@ -336,8 +470,25 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           ss += fv1 * (cabs2(a) + cabs2(b));
           cb = cadd(cb, cscale(csub(a, b), fv1 * sgn));
           if (j > 1)
-               sg += fv3m * (ap.r * a.r + ap.i * a.i + bp.r * b.r + bp.i * b.i) + fv4m * (ap.r * bp.r + ap.i * bp.i);
-          if (p.write_base) {
+               sg += fv3m * ((ap.r * a.r + ap.i * a.i) + (bp.r * b.r + bp.i * b.i)) + fv4m * (ap.r * bp.r + ap.i * bp.i);
+          if constexpr (MIX) {
+               const int e = (j - 1) & 7;
+               const double sc = fv1 * kSA;
+               split16<E>(sc * (a.r + b.r), &buf[0][e], &buf[1][e]);
+               split16<E>(sc * (a.i + b.i), &buf[2][e], &buf[3][e]);
+               split16<E>(sc * (a.r - b.r), &buf[4][e], &buf[5][e]);
+               split16<E>(sc * (a.i - b.i), &buf[6][e], &buf[7][e]);
+               if (e == 7 || j == n1) {
+                    for (int e2 = e + 1; e2 < 8; ++e2)
+                         for (int q = 0; q < 8; ++q) buf[q][e2] = 0;
+                    const int g = (j - 1) >> 3;
+                    for (int q = 0; q < 8; ++q) {
+                         v4u w;
+                         for (int k = 0; k < 4; ++k) w[k] = (unsigned) buf[q][2 * k] | ((unsigned) buf[q][2 * k + 1] << 16);
+                         *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = w;
+                    }
+               }
+          } else if (p.write_base) {
                const size_t o = (size_t) (j - 1) * Np + i;
                p.A[o] = fv4 * (a.r + b.r);
                p.A[plane + o] = fv4 * (a.i + b.i);
@@ -350,10 +501,10 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
                const double dpsi = drt * psi0 + r * dpsi0[t];
                const cd dD = cmk(p.scrD[(size_t) (1 + t) * 2 * srow + j * Np + i],
                                  p.scrD[(size_t) (1 + t) * 2 * srow + srow + j * Np + i]);
-               const double djx = -jx / x * dxv[t];
+               const double djx = -jx * dxv[t] / x;
                const cd dzj = cmk(dpsi, dchi1[t]), dzjm = cmk(dpsi0[t], dchi0[t]);
                // du = dD/m - D dm / m^2 + d(j/x); dv = dm D + m dD + d(j/x)
-               const cd du = cadd(csub(cmul(dD, im), cmul(cmul(Dj, cmul(im, im)), dmv[t])), cmk(djx, 0.));
+               const cd du = cadd(csub(cdiv(dD, m), cmul(cdiv(Dj, m2), dmv[t])), cmk(djx, 0.));
                const cd dv = cadd(cadd(cmul(dmv[t], Dj), cmul(m, dD)), cmk(djx, 0.));
                const cd dna = csub(cadd(cscale(du, psi), cscale(u, dpsi)), cmk(dpsi0[t], 0.));
                const cd dda = csub(cadd(cmul(du, zj), cmul(u, dzj)), dzjm);
@@ -362,7 +513,7 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
                const cd dat = csub(cdiv(dna, da), cdiv(cmul(na, dda), cmul(da, da)));
                const cd dbt = csub(cdiv(dnb, db), cdiv(cmul(nb, ddb), cmul(db, db)));
                dse[t] += fv1 * (dat.r + dbt.r);
-               dss[t] += fv1 * 2. * (a.r * dat.r + a.i * dat.i + b.r * dbt.r + b.i * dbt.i);
+               dss[t] += fv1 * 2. * ((a.r * dat.r + a.i * dat.i) + (b.r * dbt.r + b.i * dbt.i));
                dcb[t] = cadd(dcb[t], cscale(csub(dat, dbt), fv1 * sgn));
                if (j > 1)
                     dsg[t] += fv3m * (dap[t].r * a.r + dap[t].i * a.i + ap.r * dat.r + ap.i * dat.i +
@@ -377,7 +528,7 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
                dbp[t] = dbt;
                // advance tangent recurrences
                const double c = fv1 / x;
-               const double dchi2 = -c / x * dxv[t] * chi1 + c * dchi1[t] - dchi0[t];
+               const double dchi2 = -c * dxv[t] / x * chi1 + c * dchi1[t] - dchi0[t];
                dchi0[t] = dchi1[t];
                dchi1[t] = dchi2;
                dpsi0[t] = dpsi;
@@ -391,7 +542,12 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           psi0 = psi;
      }
      // zero rows n1 .. kf-1 so the contraction can run whole chunks
-     for (int r = n1; r < kf; ++r) {
+     if constexpr (MIX) {
+          const v4u z = {0u, 0u, 0u, 0u};
+          for (int g = (n1 + 7) / 8; g < kf / 8; ++g)
+               for (int q = 0; q < 8; ++q) *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = z;
+     }
+     for (int r = n1; r < kf && !MIX; ++r) {
           const size_t o = (size_t) r * Np + i;
           for (int c = 0; c < 4; ++c) {
                if (p.write_base) p.A[c * plane + o] = 0.;
@@ -447,7 +603,8 @@ __global__ __launch_bounds__(256) void weighted_dots(const DotJob *jobs, int n, 
 // S_n = fv4 (a+b), T_n = fv4 (a-b), s_n = (-1)^(n-1) (the -mu mirror,
 // src/lmie_core.c:617-652,682-686), are 8 real GEMM accumulators of
 // v_mfma_f64_16x16x4 (operand lane layout: A[i][k] in lane i + 16 k, B[k][j] in
-// lane j + 16 k; result D[4 (l/16) + v][l % 16] in lane l, element v).  The
+// lane j + 16 k; result D[l/16 + 4 v][l % 16] in lane l, element v -- checked by
+// lmie_gpu_selftest_mfma_f64).  The
 // epilogue forms S1, S2 and the four matrix elements (src/lmie_core.c:726-749),
 // weights them by f_i and reduces over the sizes of the workgroup; partials per
 // block of sizes go to HBM and are summed by reduce_parts.
@@ -589,7 +746,7 @@ __global__ __launch_bounds__(256) void angular_f64(AngArgs p) {
           for (int a = 0; a < WM; ++a)
 #pragma unroll
                for (int v = 0; v < 4; ++v) {
-                    const int i = it0 + wm * 16 * WM + 16 * a + 4 * lg + v;
+                    const int i = it0 + wm * 16 * WM + 16 * a + lg + 4 * v;
                     double wv[NW > 0 ? NW : 1], wt[NT > 0 ? NT : 1];
 #pragma unroll
                     for (int c = 0; c < NW; ++c) wv[c] = p.W[c][i];
@@ -678,6 +835,179 @@ __global__ __launch_bounds__(256) void angular_f64(AngArgs p) {
      }
 }
 
+
+// Angular contraction, split 16-bit operands on the matrix cores.
+// The same four complex sums as angular_f64, each operand split into a high and
+// a low 16-bit part: x y ~ xh yh + xh yl + xl yh (three products, fp32
+// accumulation in the matrix core), v_mfma_f32_16x16x32_{f16,bf16}: lane l
+// supplies rows l % 16 and orders 8 (l / 16) .. 8 (l / 16) + 7 of the chunk for
+// both operands.  The mirror sign s_n = (-1)^(n-1) flips the odd elements of
+// each 8-order fragment.  Epilogue in double as angular_f64.
+struct MixArgs {
+     int Np, Kb, Hp, H, Q, N;
+     const unsigned short *A16;     // [8][Kb][Np][8]
+     const unsigned short *B16;     // [4][Kb][Hp][8]
+     const int *ktile;              // orders to contract per size tile (multiple of KM)
+     const double *W[4];            // weight channels [Np]
+     int tiles_per_block;
+     double *part;                  // [gridDim.y][nw][4][Q]
+};
+
+template <class E, int WM, int WN, int NW>
+__global__ __launch_bounds__(256) void angular_mix(MixArgs p) {
+     constexpr int TM = 32 * WM, TN = 32 * WN, G = KM / 8;
+     __shared__ v4u sA[8 * G * TM];
+     __shared__ v4u sB[4 * G * TN];
+     __shared__ double sR[NW * 8 * 32 * WN];
+
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wm = wave & 1, wn = wave >> 1;
+     const int l16 = lane & 15, lg = lane >> 4;
+     const int jt0 = blockIdx.x * TN;
+     const int tile0 = blockIdx.y * p.tiles_per_block;
+     const size_t Np = p.Np, Hp = p.Hp;
+     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hp * 8;
+     const double unscale = 1. / (kSA * kSB);
+     const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
+
+     double ew[NW][WN][8];
+#pragma unroll
+     for (int c = 0; c < NW; ++c)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int q = 0; q < 8; ++q) ew[c][b][q] = 0.;
+
+     for (int tt = 0; tt < p.tiles_per_block; ++tt) {
+          const int tile = tile0 + tt;
+          const int it0 = tile * TM;
+          if (it0 >= p.Np) break;
+          const int kmax = p.ktile[tile];
+          v4f acc[WM][WN][8];
+#pragma unroll
+          for (int a = 0; a < WM; ++a)
+#pragma unroll
+               for (int b = 0; b < WN; ++b)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc[a][b][q] = v4f{0.f, 0.f, 0.f, 0.f};
+
+          for (int k0 = 0; k0 < kmax; k0 += KM) {
+               const int g0 = k0 / 8;
+               __syncthreads();
+               for (int e = threadIdx.x; e < 8 * G * TM; e += 256) {
+                    const int col = e % TM, rest = e / TM, g = rest % G, pl = rest / G;
+                    sA[e] = *(const v4u *) (p.A16 + pl * pA + ((size_t) (g0 + g) * Np + it0 + col) * 8);
+               }
+               for (int e = threadIdx.x; e < 4 * G * TN; e += 256) {
+                    const int col = e % TN, rest = e / TN, g = rest % G, pl = rest / G;
+                    sB[e] = *(const v4u *) (p.B16 + pl * pB + ((size_t) (g0 + g) * Hp + jt0 + col) * 8);
+               }
+               __syncthreads();
+               v4u bph[WN], bpl[WN], bmh[WN], bml[WN];
+#pragma unroll
+               for (int b = 0; b < WN; ++b) {
+                    const int jc = wn * 16 * WN + 16 * b + l16;
+                    bph[b] = sB[(0 * G + lg) * TN + jc];
+                    bpl[b] = sB[(1 * G + lg) * TN + jc];
+                    bmh[b] = sB[(2 * G + lg) * TN + jc];
+                    bml[b] = sB[(3 * G + lg) * TN + jc];
+               }
+#pragma unroll
+               for (int a = 0; a < WM; ++a) {
+                    const int ic = wm * 16 * WM + 16 * a + l16;
+                    v4u x[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) x[q] = sA[(q * G + lg) * TM + ic];
+#pragma unroll
+                    for (int b = 0; b < WN; ++b) {
+                         v4f *c = acc[a][b];
+                         // accumulators: sp+ re, im = S (pi+tau); sm+ = T (pi-tau);
+                         // sp- = s S (pi-tau); sm- = s T (pi+tau)
+                         const v4u y[8] = {x[0] ^ flip, x[1] ^ flip, x[2] ^ flip, x[3] ^ flip,
+                                           x[4] ^ flip, x[5] ^ flip, x[6] ^ flip, x[7] ^ flip};
+                         // hi x hi
+                         c[0] = E::mfma(x[0], bph[b], c[0]); c[1] = E::mfma(x[2], bph[b], c[1]);
+                         c[2] = E::mfma(x[4], bmh[b], c[2]); c[3] = E::mfma(x[6], bmh[b], c[3]);
+                         c[4] = E::mfma(y[0], bmh[b], c[4]); c[5] = E::mfma(y[2], bmh[b], c[5]);
+                         c[6] = E::mfma(y[4], bph[b], c[6]); c[7] = E::mfma(y[6], bph[b], c[7]);
+                         // hi x lo
+                         c[0] = E::mfma(x[0], bpl[b], c[0]); c[1] = E::mfma(x[2], bpl[b], c[1]);
+                         c[2] = E::mfma(x[4], bml[b], c[2]); c[3] = E::mfma(x[6], bml[b], c[3]);
+                         c[4] = E::mfma(y[0], bml[b], c[4]); c[5] = E::mfma(y[2], bml[b], c[5]);
+                         c[6] = E::mfma(y[4], bpl[b], c[6]); c[7] = E::mfma(y[6], bpl[b], c[7]);
+                         // lo x hi
+                         c[0] = E::mfma(x[1], bph[b], c[0]); c[1] = E::mfma(x[3], bph[b], c[1]);
+                         c[2] = E::mfma(x[5], bmh[b], c[2]); c[3] = E::mfma(x[7], bmh[b], c[3]);
+                         c[4] = E::mfma(y[1], bmh[b], c[4]); c[5] = E::mfma(y[3], bmh[b], c[5]);
+                         c[6] = E::mfma(y[5], bph[b], c[6]); c[7] = E::mfma(y[7], bph[b], c[7]);
+                    }
+               }
+          }
+#pragma unroll
+          for (int a = 0; a < WM; ++a)
+#pragma unroll
+               for (int v = 0; v < 4; ++v) {
+                    const int i = it0 + wm * 16 * WM + 16 * a + mfma16_row(lane, v);
+                    double wv[NW];
+#pragma unroll
+                    for (int c = 0; c < NW; ++c) wv[c] = p.W[c][i];
+#pragma unroll
+                    for (int b = 0; b < WN; ++b) {
+                         const v4f *q = acc[a][b];
+                         double Pp[4], Pm[4];
+                         stokes(q[0][v] * unscale, q[1][v] * unscale, q[2][v] * unscale, q[3][v] * unscale, Pp);
+                         stokes(q[4][v] * unscale, q[5][v] * unscale, q[6][v] * unscale, q[7][v] * unscale, Pm);
+#pragma unroll
+                         for (int c = 0; c < NW; ++c)
+#pragma unroll
+                              for (int k = 0; k < 4; ++k) {
+                                   ew[c][b][k] += wv[c] * Pp[k];
+                                   ew[c][b][4 + k] += wv[c] * Pm[k];
+                              }
+                    }
+               }
+     }
+#pragma unroll
+     for (int c = 0; c < NW; ++c)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int q = 0; q < 8; ++q) {
+                    double t = ew[c][b][q];
+                    t += __shfl_xor(t, 16);
+                    t += __shfl_xor(t, 32);
+                    ew[c][b][q] = t;
+               }
+     __syncthreads();
+     if (wm == 1 && lg == 0) {
+#pragma unroll
+          for (int c = 0; c < NW; ++c)
+#pragma unroll
+               for (int b = 0; b < WN; ++b)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) sR[((c * WN + b) * 8 + q) * 32 + wn * 16 + l16] = ew[c][b][q];
+     }
+     __syncthreads();
+     if (wm == 0 && lg == 0) {
+          double *out = p.part + (size_t) blockIdx.y * NW * 4 * p.Q;
+#pragma unroll
+          for (int b = 0; b < WN; ++b) {
+               const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+               if (j < p.H) {
+#pragma unroll
+                    for (int c = 0; c < NW; ++c)
+#pragma unroll
+                         for (int k = 0; k < 4; ++k) {
+                              out[((size_t) c * 4 + k) * p.Q + p.H + j] =
+                                   ew[c][b][k] + sR[((c * WN + b) * 8 + k) * 32 + wn * 16 + l16];
+                              out[((size_t) c * 4 + k) * p.Q + p.H - 1 - j] =
+                                   ew[c][b][4 + k] + sR[((c * WN + b) * 8 + 4 + k) * 32 + wn * 16 + l16];
+                         }
+               }
+          }
+     }
+}
+
 // Sum of the per-block partials: one thread per (pass channel, element, angle);
 // channel c of the pass goes to channel map.g[c] of the output.
 struct ChMap {
@@ -743,7 +1073,7 @@ __global__ __launch_bounds__(256) void expand_coefs(const double *T, const doubl
      const int c = blockIdx.y;
      if (l >= C) return;
      const size_t plane = (size_t) C * Hp;
-     const double *P00 = T + (size_t) l * Hp, *P02 = T + plane + (size_t) l * Hp;
+     const double *P00 = T + (gsf ? 0 : 4 * plane) + (size_t) l * Hp, *P02 = T + plane + (size_t) l * Hp;
      const double *P22 = T + 2 * plane + (size_t) l * Hp, *P2m = T + 3 * plane + (size_t) l * Hp;
      const double *Ec = E + (size_t) c * 12 * Hp;
      double d[12];
@@ -793,6 +1123,7 @@ __global__ __launch_bounds__(256) void expand_coefs(const double *T, const doubl
 // (set, angle) (misc/rtutil_scat_int.c:510-589, :662-710).
 __global__ void phase_at_angles(int nset, int ncoef, int ld, const double *X, int nth, const double *theta,
                                 double *pf, int gsf) {
+#pragma clang fp contract(off)
      const int e = blockIdx.x * blockDim.x + threadIdx.x;
      if (e >= nset * nth) return;
      const int s = e / nth, i = e % nth;
@@ -811,13 +1142,14 @@ __global__ void phase_at_angles(int nset, int ncoef, int ld, const double *X, in
                     v22 = .25 * (1. + m) * (1. + m);
                     v2m = .25 * (1. - m) * (1. - m);
                } else {
-                    double L = l - 1, L1 = l, t = 2. * L + 1.;
-                    double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
-                    double c = t * L * L1 * m, k4 = 4. * t, kb = L1 * (L * L - 4.), kd = L * (L1 * L1 - 4.);
-                    v00 = (t * m * b00 - L * a00) / L1;
-                    v02 = (t * m * b02 - sa * a02) / sb;
-                    v22 = ((c - k4) * b22 - kb * a22) / kd;
-                    v2m = ((c + k4) * b2m - kb * a2m) / kd;
+                    const double L = l - 1, L1 = l, t = 2. * L + 1.;
+                    const double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
+                    const double k4 = t * 4, kb = L1 * (L * L - 4), kd = L * (L1 * L1 - 4);
+                    const double da = t * m, dc = t * (L * L1) * m;
+                    v00 = (da * b00 - L * a00) / L1;
+                    v02 = (da * b02 - sa * a02) / sb;
+                    v22 = ((dc - k4) * b22 - kb * a22) / kd;
+                    v2m = ((dc + k4) * b2m - kb * a2m) / kd;
                }
                acc[0] += x[l] * v00;
                acc[3] += x[3 * ld + l] * v00;
@@ -837,16 +1169,17 @@ __global__ void phase_at_angles(int nset, int ncoef, int ld, const double *X, in
           o[nth + i] = (acc[1] + acc[2]) / 2.;
           o[2 * nth + i] = (acc[1] - acc[2]) / 2.;
      } else {
-          double pm = 0., pc = 1.;
+          double pm = 1., pc = m;
           for (int l = 0; l < ncoef; ++l) {
                double v;
                if (l == 0) v = 1.;
                else if (l == 1) v = m;
                else {
-                    const double u = m * pc;
-                    v = (l - 1.) / l * (u - pm) + u;
+                    const double a = (l - 1) / (double) l, b = m * pc;
+                    v = a * (b - pm) + b;
+                    pm = pc;
+                    pc = v;
                }
-               if (l >= 1) { pm = pc; pc = v; }
                for (int k = 0; k < 6; ++k) acc[k] += x[k * ld + l] * v;
           }
           double *o = pf + (size_t) s * 6 * nth;
@@ -877,8 +1210,9 @@ struct Workspace {
      int device = -1;
      hipStream_t stream = nullptr;
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
-     Buf x, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
-     Buf guess, mu, w, Bp, Bm, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2;
+     Buf x, sx, cx, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
+     Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2;
+     std::vector<double> hsc;       // host staging of sin x, cos x
      // kept for the adjoint of the same call
      int valid = 0;
 };
@@ -998,6 +1332,13 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      hipStream_t s = w->stream;
      Timer tm{w, g_timing};
      w->valid = 0;
+     // mixed path: value and weight-only channels on the split 16-bit kernel;
+     // forward tangents and per-size matrix output stay on the FP64 kernel
+     const int mixed = (J->precision == LMIE_PRECISION_MIXED || J->precision == LMIE_PRECISION_MIXED_BF16) &&
+                       nt == 0 && nw <= 3 && J->save_control != 1 && Q > 0;
+     const int bf16 = J->precision == LMIE_PRECISION_MIXED_BF16;
+     const int kgran = mixed ? KM : KC;
+     const int Kb = rup(n1max, KM) / 8;
 
      // host-side tables: per-tile contraction depths
      const int nt32 = Np / 32, nt64 = Np / 64;
@@ -1006,7 +1347,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      for (int t = 0; t < nt32; ++t) {
           int m = 0;
           for (int i = t * 32; i < t * 32 + 32; ++i) m = n1h[i] > m ? n1h[i] : m;
-          kt32[t] = rup(m, KC);
+          kt32[t] = rup(m, kgran);
      }
      for (int t = 0; t < nt64; ++t) {
           kt64[t] = kt32[2 * t] > kt32[2 * t + 1] ? kt32[2 * t] : kt32[2 * t + 1];
@@ -1022,6 +1363,17 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      HIPCHECK(hipMemsetAsync(w->x.p, 0, sizeof(double) * Np, s));
      HIPCHECK(hipMemsetAsync(w->f.p, 0, sizeof(double) * Np, s));
      HIPCHECK(hipMemcpyAsync(w->x.p, J->x, sizeof(double) * N, hipMemcpyHostToDevice, s));
+     {
+          // sin x, cos x of every size from the host C library, as the reference
+          if (w->sx.get(sizeof(double) * Np) || w->cx.get(sizeof(double) * Np)) return -1;
+          w->hsc.resize(2 * (size_t) Np);
+          for (int i = 0; i < Np; ++i) {
+               w->hsc[i] = i < N ? sin(J->x[i]) : 0.;
+               w->hsc[Np + i] = i < N ? cos(J->x[i]) : 1.;
+          }
+          HIPCHECK(hipMemcpyAsync(w->sx.p, w->hsc.data(), sizeof(double) * Np, hipMemcpyHostToDevice, s));
+          HIPCHECK(hipMemcpyAsync(w->cx.p, w->hsc.data() + Np, sizeof(double) * Np, hipMemcpyHostToDevice, s));
+     }
      HIPCHECK(hipMemcpyAsync(w->f.p, J->f, sizeof(double) * N, hipMemcpyHostToDevice, s));
      HIPCHECK(hipMemcpyAsync(w->n1.p, n1h.data(), sizeof(int) * Np, hipMemcpyHostToDevice, s));
      HIPCHECK(hipMemcpyAsync(w->n2.p, n2h.data(), sizeof(int) * Np, hipMemcpyHostToDevice, s));
@@ -1062,12 +1414,24 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           }
           hipLaunchKernelGGL(gauss_nodes, dim3((H + 63) / 64), dim3(64), 0, s, Q, (const double *) w->guess.as<double>(),
                              w->mu.as<double>(), w->w.as<double>());
-          hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kp, w->mu.as<double>(),
-                             w->Bp.as<double>(), w->Bm.as<double>());
+          if (mixed) {
+               if (w->B16.get(sizeof(unsigned short) * 4 * (size_t) Kb * Hp * 8)) return -1;
+               if (bf16)
+                    hipLaunchKernelGGL(pitau_table16<BF16>, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kb,
+                                       (const double *) w->mu.as<double>(), w->B16.as<unsigned short>());
+               else
+                    hipLaunchKernelGGL(pitau_table16<F16>, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kb,
+                                       (const double *) w->mu.as<double>(), w->B16.as<unsigned short>());
+          } else {
+               hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kp, w->mu.as<double>(),
+                                  w->Bp.as<double>(), w->Bm.as<double>());
+          }
           if (J->calc_gc || J->calc_lc) {
-               if (w->T.get(sizeof(double) * 4 * (size_t) C * Hp)) return -1;
-               HIPCHECK(hipMemsetAsync(w->T.p, 0, sizeof(double) * 4 * (size_t) C * Hp, s));
-               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, C, w->mu.as<double>(), w->T.as<double>());
+               const int planes = J->calc_lc ? 5 : 4;
+               if (w->T.get(sizeof(double) * planes * (size_t) C * Hp)) return -1;
+               HIPCHECK(hipMemsetAsync(w->T.p, 0, sizeof(double) * planes * (size_t) C * Hp, s));
+               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, C, (const double *) w->mu.as<double>(),
+                                  w->T.as<double>(), J->calc_lc);
           }
           HIPCHECK(hipGetLastError());
      }
@@ -1076,7 +1440,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      // per-size coefficients (passes of up to 4 tangent channels)
      const size_t plane = (size_t) Kp * Np, srow = (size_t) (n1max + 1) * Np;
      const int nt_pass = nt < 4 ? nt : 4;
-     if (w->A.get(sizeof(double) * 4 * plane * (1 + nt)) || w->S.get(sizeof(double) * 4 * Np * (1 + nt)) ||
+     if ((!mixed && w->A.get(sizeof(double) * 4 * plane * (1 + nt))) || w->S.get(sizeof(double) * 4 * Np * (1 + nt)) ||
+         (mixed && w->A16.get(sizeof(unsigned short) * 8 * (size_t) Kb * Np * 8)) ||
          w->scrD.get(sizeof(double) * 2 * srow * (1 + nt_pass)) || w->scrR.get(sizeof(double) * srow * (1 + nt_pass)))
           return -1;
      tm.mark(LMIE_T_COEFS, 0);
@@ -1084,7 +1449,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           CoefArgs c;
           memset(&c, 0, sizeof c);
           c.N = N; c.Np = Np; c.n1max = n1max;
-          c.x = w->x.as<double>(); c.n1 = w->n1.as<int>(); c.n2 = w->n2.as<int>(); c.kfill = w->kfill.as<int>();
+          c.x = w->x.as<double>(); c.sx = w->sx.as<double>(); c.cx = w->cx.as<double>(); c.n1 = w->n1.as<int>(); c.n2 = w->n2.as<int>(); c.kfill = w->kfill.as<int>();
           c.mr = J->mr; c.mi = J->mi;
           int k = nt - t0 < 4 ? nt - t0 : 4;
           if (k < 0) k = 0;
@@ -1101,7 +1466,15 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                c.St[q] = w->S.as<double>() + (size_t) 4 * Np * (1 + t0 + q);
           }
           c.write_base = t0 == 0;
-          if (launch_coefs(k, c, s)) return -1;
+          if (mixed) {
+               c.A16 = w->A16.as<unsigned short>();
+               c.Kb = Kb;
+               if (bf16) hipLaunchKernelGGL((mie_coefs<0, BF16>), dim3(Np / 64), dim3(64), 0, s, c);
+               else hipLaunchKernelGGL((mie_coefs<0, F16>), dim3(Np / 64), dim3(64), 0, s, c);
+               HIPCHECK(hipGetLastError());
+          } else if (launch_coefs(k, c, s)) {
+               return -1;
+          }
      }
      tm.mark(LMIE_T_COEFS, 1);
 
@@ -1149,9 +1522,41 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                td += P.ntp;
                wd += ww;
           }
+          if (mixed) {
+               plan.resize(1);
+               plan[0].ntp = 0;
+               plan[0].nwp = 1 + nw;
+               plan[0].t0 = plan[0].w0 = 0;
+               plan[0].nblk = ((Np / 64) + 3) / 4;
+               plan[0].off = 0;
+               tot = (size_t) plan[0].nblk * plan[0].nwp * 4 * Q;
+          }
           if (w->part.get(sizeof(double) * tot)) return -1;
           tm.mark(LMIE_T_ANGULAR, 0);
-          for (size_t ps = 0; ps < plan.size(); ++ps) {
+          if (mixed) {
+               MixArgs m;
+               memset(&m, 0, sizeof m);
+               m.Np = Np; m.Kb = Kb; m.Hp = Hp; m.H = H; m.Q = Q; m.N = N;
+               m.A16 = w->A16.as<unsigned short>();
+               m.B16 = w->B16.as<unsigned short>();
+               m.ktile = w->ktile64.as<int>();
+               m.W[0] = w->f.as<double>();
+               for (int q = 0; q < nw; ++q) m.W[1 + q] = w->dfw.as<double>() + (size_t) q * Np;
+               m.tiles_per_block = 4;
+               m.part = w->part.as<double>();
+               dim3 grid(Hp / 64, plan[0].nblk);
+#define LMIE_MIX(E, NWV) hipLaunchKernelGGL((angular_mix<E, 2, 2, NWV>), grid, dim3(256), 0, s, m)
+               if (bf16) {
+                    if (nw == 0) LMIE_MIX(BF16, 1); else if (nw == 1) LMIE_MIX(BF16, 2);
+                    else if (nw == 2) LMIE_MIX(BF16, 3); else LMIE_MIX(BF16, 4);
+               } else {
+                    if (nw == 0) LMIE_MIX(F16, 1); else if (nw == 1) LMIE_MIX(F16, 2);
+                    else if (nw == 2) LMIE_MIX(F16, 3); else LMIE_MIX(F16, 4);
+               }
+#undef LMIE_MIX
+               HIPCHECK(hipGetLastError());
+          }
+          for (size_t ps = 0; ps < plan.size() && !mixed; ++ps) {
                Pass &P = plan[ps];
                AngArgs a;
                memset(&a, 0, sizeof a);
@@ -1241,7 +1646,29 @@ __global__ void mfma_f64_probe(const double *A, const double *B, double *D) {
      const double b = B[(l >> 4) * 16 + (l & 15)];
      v4d c = {0., 0., 0., 0.};
      c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-     for (int v = 0; v < 4; ++v) D[(4 * (l >> 4) + v) * 16 + (l & 15)] = c[v];
+     for (int v = 0; v < 4; ++v) D[((l >> 4) + 4 * v) * 16 + (l & 15)] = c[v];
+}
+
+// Lane-layout check of the 16-bit matrix instructions used by the mixed
+// contraction: D = A (16x32) B (32x16), row-major float inputs (exactly
+// representable in the 16-bit type), one wave.
+template <class E>
+__global__ void mfma16_probe(const float *A, const float *B, float *D) {
+     const int l = threadIdx.x;
+     unsigned short av[8], bv[8];
+     double r;
+     for (int e = 0; e < 8; ++e) {
+          av[e] = E::bits(A[(l & 15) * 32 + 8 * (l >> 4) + e], &r);
+          bv[e] = E::bits(B[(8 * (l >> 4) + e) * 16 + (l & 15)], &r);
+     }
+     v4u a, b;
+     for (int k = 0; k < 4; ++k) {
+          a[k] = (unsigned) av[2 * k] | ((unsigned) av[2 * k + 1] << 16);
+          b[k] = (unsigned) bv[2 * k] | ((unsigned) bv[2 * k + 1] << 16);
+     }
+     v4f c = {0.f, 0.f, 0.f, 0.f};
+     c = E::mfma(a, b, c);
+     for (int v = 0; v < 4; ++v) D[mfma16_row(l, v) * 16 + (l & 15)] = c[v];
 }
 
 }  // namespace
@@ -1263,7 +1690,8 @@ int lmie_get_timing(double *ms, int n) {
 int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
      g_err[0] = 0;
      if (!job || !res) return set_err("null argument");
-     if (job->precision != LMIE_PRECISION_FP64 && job->precision != LMIE_PRECISION_MIXED)
+     if (job->precision != LMIE_PRECISION_FP64 && job->precision != LMIE_PRECISION_MIXED &&
+         job->precision != LMIE_PRECISION_MIXED_BF16)
           return set_err("unknown precision %d", job->precision);
      Workspace *w = nullptr;
      if (ws_acquire(job->device, &w)) return -1;
@@ -1315,6 +1743,22 @@ int lmie_gpu_gauss_rule(int n, double *x, double *wt, int device) {
      HIPCHECK(hipGetLastError());
      HIPCHECK(hipMemcpy(x, d + g.size(), sizeof(double) * n, hipMemcpyDeviceToHost));
      HIPCHECK(hipMemcpy(wt, d + g.size() + n, sizeof(double) * n, hipMemcpyDeviceToHost));
+     HIPCHECK(hipFree(d));
+     return 0;
+}
+
+int lmie_gpu_selftest_mfma16(int bf16, const float *A, const float *B, float *D, int device) {
+     HIPCHECK(hipSetDevice(device));
+     float *d = nullptr;
+     HIPCHECK(hipMalloc(&d, sizeof(float) * (512 + 512 + 256)));
+     HIPCHECK(hipMemcpy(d, A, sizeof(float) * 512, hipMemcpyHostToDevice));
+     HIPCHECK(hipMemcpy(d + 512, B, sizeof(float) * 512, hipMemcpyHostToDevice));
+     if (bf16)
+          hipLaunchKernelGGL(mfma16_probe<BF16>, dim3(1), dim3(64), 0, 0, (const float *) d, (const float *) (d + 512), d + 1024);
+     else
+          hipLaunchKernelGGL(mfma16_probe<F16>, dim3(1), dim3(64), 0, 0, (const float *) d, (const float *) (d + 512), d + 1024);
+     HIPCHECK(hipGetLastError());
+     HIPCHECK(hipMemcpy(D, d + 1024, sizeof(float) * 256, hipMemcpyDeviceToHost));
      HIPCHECK(hipFree(d));
      return 0;
 }

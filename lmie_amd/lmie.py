@@ -14,7 +14,7 @@ import numpy as np
 from . import abi
 from .abi import LmieIn, LmieOut
 
-PRECISION = {"fp64": 0, "mixed": 1}
+PRECISION = {"fp64": 0, "mixed": 1, "mixed-bf16": 2}
 
 TIMERS = ("tables", "coefs", "angular", "reduce", "expand", "adj_angular", "adj_coefs", "total")
 

@@ -41,8 +41,15 @@ def test_gauss_rule_is_bitwise_the_reference_rule(product, oracle, n):
     f.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
     assert f(n, x.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)), 0) == 0
     oracle.oracle_gauss_leg(n, 0, -1.0, 1.0, None, None, xo, wo, None, None)
-    assert np.array_equal(x, np.array(xo))
-    assert np.array_equal(w, np.array(wo))
+    xo, wo = np.array(xo), np.array(wo)
+    if not (np.array_equal(x, xo) and np.array_equal(w, wo)):
+        import os
+        os.makedirs("gpurun_out", exist_ok=True)
+        np.savez("gpurun_out/gauss_rule_%d.npz" % n, x=x, w=w, xo=xo, wo=wo)
+        say("  nodes differ at %d, weights at %d; max |dx| %.3e, max |dw|/w %.3e" % (
+            np.sum(x != xo), np.sum(w != wo), np.abs(x - xo).max(), (np.abs(w - wo) / wo).max()))
+    assert np.array_equal(x, xo)
+    assert np.array_equal(w, wo)
 
 
 @pytest.mark.parametrize("name", SINGLE)
