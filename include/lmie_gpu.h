@@ -37,6 +37,7 @@ typedef struct {
      int save_control;        /* 1: return per-size series and matrix elements    */
      int precision;           /* enum lmie_precision                              */
      int device;
+     int keep;                /* 1: keep the device state for lmie_gpu_adjoint    */
 } lmie_gpu_job;
 
 typedef struct {
@@ -50,14 +51,16 @@ typedef struct {
      double *lc_raw;          /* same for the Legendre expansion, or NULL        (caller) */
      double *save1;           /* [4][N]          if save_control == 1            (caller) */
      double *save2;           /* [4][N][n_qang]  if save_control == 1            (caller) */
+     void *state;             /* job->keep: device state for lmie_gpu_adjoint     */
 } lmie_gpu_result;
 
 int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res);
 
-/* Reverse pass after lmie_gpu_forward on the same thread and job: given the
-   cotangents of the raw sums (4) and of the raw expansions (gc_raw / lc_raw of
-   channel 0, [6][n_coef], NULL = zero), return per size the cotangents of
-   f_i and x_i, and the cotangent of m as (d/d mr, d/d mi) summed over sizes. */
+/* Reverse pass on the state kept by lmie_gpu_forward (job->keep = 1, value
+   channel only): given the cotangents of the raw sums (4) and of the raw
+   expansions (gc_raw / lc_raw of channel 0, [6][n_coef], NULL = zero), return
+   per size the cotangent of f_i, and the cotangent of m as (d/d mr, d/d mi)
+   summed over sizes.  The state stays valid until lmie_gpu_release. */
 typedef struct {
      const double *sums_bar;  /* [4]                                              */
      const double *gc_bar;    /* [6][n_coef] or NULL                              */
@@ -66,11 +69,11 @@ typedef struct {
 
 typedef struct {
      double *f_bar;           /* [N]                                      (caller) */
-     double *x_bar;           /* [N] (zero: the x reverse pass is not built)(caller) */
      double m_bar[2];         /* dL/d mr, dL/d mi                                  */
 } lmie_gpu_adj_out;
 
-int lmie_gpu_adjoint(const lmie_gpu_job *job, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out);
+int lmie_gpu_adjoint(void *state, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out);
+void lmie_gpu_release(void *state);
 
 /* Scattering matrix at n_theta user angles from expansion coefficients
    (misc/rtutil_scat_int.c:510-797): coefs [n_set][6][ld], n_coef used,

@@ -119,15 +119,34 @@ typedef struct {
      double lambda, mr, mi, a1, a2, a3, a4, a5, r1, r2;
 } lmie_grad;
 
-/* Forward pass (outputs written to out exactly as lmie_solution_ex with
-   n_derivs = 0) followed by one reverse pass.  modes / weight / n_modes as
-   lmie_solution_mix (n_modes = 1, weight = NULL for a single distribution).
-   The gradient covers mr, mi and the distribution parameters a1..a5;
-   lambda, r1 and r2 are reported as NaN (their reverse pass is not built).
-   No counterpart in the reference. */
+/* Reverse mode in two calls, so a caller (e.g. an autograd wrapper) can form
+   the loss gradient from the forward outputs in between:
+     lmie_adjoint_forward  forward pass (outputs written to out exactly as
+                           lmie_solution_ex with n_derivs = 0) that keeps what
+                           the reverse pass needs on the device: a_n, b_n per
+                           size point and the angular sums per size and angle;
+     lmie_adjoint_reverse  one reverse pass for the given output cotangents;
+                           may be called again on the same context;
+     lmie_adjoint_free     releases the device state.
+   modes / weight / n_modes as lmie_solution_mix (n_modes = 1, weight = NULL
+   for a single distribution).  The gradient covers mr, mi and the
+   distribution parameters a1..a5; lambda, r1 and r2 are reported as NaN (their
+   reverse pass is not built).  No counterpart in the reference. */
+typedef struct lmie_adj_ctx lmie_adj_ctx;
+
+int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n_modes,
+                         lmie_out_data *out, int alloc_out, const lmie_opts *opts,
+                         lmie_adj_ctx **ctx);
+int lmie_adjoint_reverse(lmie_adj_ctx *ctx, const lmie_adj_in *adj, lmie_grad *grad,
+                         double *grad_modes);
+void lmie_adjoint_free(lmie_adj_ctx *ctx);
+
+/* Both in one call: cot(out, adj, user) is called between the passes to fill
+   the cotangents from the forward outputs. */
+typedef int (*lmie_cotangent_fn)(const lmie_out_data *out, lmie_adj_in *adj, void *user);
 int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int n_modes,
                           lmie_out_data *out, int alloc_out, const lmie_opts *opts,
-                          const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes);
+                          lmie_cotangent_fn cot, void *user, lmie_grad *grad, double *grad_modes);
 
 /* (3) Text of the last error on this thread ("" if none). */
 const char *lmie_last_error(void);

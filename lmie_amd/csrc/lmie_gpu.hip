@@ -326,6 +326,8 @@ struct CoefArgs {
      int write_base;            // write A and S of the base channel
      unsigned short *A16;       // mixed path: [8 planes][Kb][Np][8] split operand
      int Kb;                    // groups of 8 orders in A16
+     double *AB;                // reverse-pass checkpoint: a, b [4 planes re/im][Kp][Np] or NULL
+     double *CB;                // reverse-pass checkpoint: c_bak series [2][Np] or NULL
 };
 
 // E = void: FP64 operand planes; E = F16 / BF16: split 16-bit operand, blocked
@@ -533,6 +535,13 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
                dchi1[t] = dchi2;
                dpsi0[t] = dpsi;
           }
+          if (p.AB) {
+               const size_t o = (size_t) (j - 1) * Np + i;
+               p.AB[o] = a.r;
+               p.AB[plane + o] = a.i;
+               p.AB[2 * plane + o] = b.r;
+               p.AB[3 * plane + o] = b.i;
+          }
           ap = a;
           bp = b;
           sgn = -sgn;
@@ -554,6 +563,10 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
 #pragma unroll
                for (int t = 0; t < NT; ++t) p.At[t][c * plane + o] = 0.;
           }
+     }
+     if (p.CB) {
+          p.CB[i] = cb.r;
+          p.CB[Np + i] = cb.i;
      }
      if (p.write_base) {
           p.S[i] = se;
@@ -620,6 +633,7 @@ struct AngArgs {
      double *part;              // [gridDim.y][nch][4][Q]
      double *save2;             // [4][N][Q] per-size matrix (save_control 1) or NULL
      int N;
+     double *sst;               // reverse-pass checkpoint [Np][Hp][8]: sp+, sm+, sp-, sm- (re, im) or NULL
 };
 
 __device__ __forceinline__ void stokes(double spr, double spi, double smr, double smi, double P[4]) {
@@ -778,6 +792,12 @@ __global__ __launch_bounds__(256) void angular_f64(AngArgs p) {
                                    ew[NW + t][b][4 + k] += wt[t] * Pm[k] + f0 * dPm[k];
                               }
                          }
+                         if (p.sst) {
+                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+                              double *o = p.sst + ((size_t) i * p.Hp + j) * 8;
+#pragma unroll
+                              for (int c = 0; c < 8; ++c) o[c] = q[c][v];
+                         }
                          if (p.save2 && i < p.N) {
                               const int j = jt0 + wn * 16 * WN + 16 * b + l16;
                               if (j < p.H) {
@@ -851,6 +871,7 @@ struct MixArgs {
      const double *W[4];            // weight channels [Np]
      int tiles_per_block;
      double *part;                  // [gridDim.y][nw][4][Q]
+     double *sst;                   // reverse-pass checkpoint [Np][Hp][8] or NULL
 };
 
 template <class E, int WM, int WN, int NW>
@@ -954,6 +975,12 @@ __global__ __launch_bounds__(256) void angular_mix(MixArgs p) {
 #pragma unroll
                     for (int b = 0; b < WN; ++b) {
                          const v4f *q = acc[a][b];
+                         if (p.sst) {
+                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+                              double *o = p.sst + ((size_t) i * p.Hp + j) * 8;
+#pragma unroll
+                              for (int c = 0; c < 8; ++c) o[c] = q[c][v] * unscale;
+                         }
                          double Pp[4], Pm[4];
                          stokes(q[0][v] * unscale, q[1][v] * unscale, q[2][v] * unscale, q[3][v] * unscale, Pp);
                          stokes(q[4][v] * unscale, q[5][v] * unscale, q[6][v] * unscale, q[7][v] * unscale, Pm);
@@ -1006,6 +1033,305 @@ __global__ __launch_bounds__(256) void angular_mix(MixArgs p) {
                }
           }
      }
+}
+
+
+// ---------------------------------------------------------------------------
+// Reverse pass.
+//
+// Transpose of the expansion (misc/rtutil_scat_int.c:63-230 read backwards):
+// from the cotangents of the raw coefficients to the cotangents of the raw
+// matrix elements F11, F33, F12, F34 at the Gauss angles.  One thread per
+// positive node j, looping over l.
+__global__ void expand_adjoint(const double *T, const double *w, const double *gbar, const double *lbar,
+                               int H, int Hp, int C, int Q, double *pfbar) {
+     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+     if (j >= H) return;
+     const size_t plane = (size_t) C * Hp;
+     double Em[6] = {0, 0, 0, 0, 0, 0}, Ep[6] = {0, 0, 0, 0, 0, 0};   // GSF rows
+     double Lm[6] = {0, 0, 0, 0, 0, 0}, Lp[6] = {0, 0, 0, 0, 0, 0};   // Legendre rows
+     for (int l = 0; l < C; ++l) {
+          const double sl = (l & 1) ? -1. : 1., fl = (2 * l + 1) / 2.;
+          const size_t o = (size_t) l * Hp + j;
+          if (gbar) {
+               double cb[6];
+               for (int k = 0; k < 6; ++k) cb[k] = gbar[(size_t) k * C + l] * fl;
+               const double e1 = (cb[1] + cb[2]) / 2., e2 = (cb[1] - cb[2]) / 2.;
+               const double p00 = T[o], p02 = T[plane + o], p22 = T[2 * plane + o], p2m = T[3 * plane + o];
+               Em[0] += sl * cb[0] * p00;  Ep[0] += cb[0] * p00;
+               Em[1] += sl * e1 * p2m;     Ep[1] += e1 * p22;
+               Em[2] += sl * e2 * p22;     Ep[2] += e2 * p2m;
+               Em[3] += sl * cb[3] * p00;  Ep[3] += cb[3] * p00;
+               Em[4] += sl * cb[4] * p02;  Ep[4] += cb[4] * p02;
+               Em[5] += sl * cb[5] * p02;  Ep[5] += cb[5] * p02;
+          }
+          if (lbar) {
+               const double pl = T[4 * plane + o];
+               for (int k = 0; k < 6; ++k) {
+                    const double c = lbar[(size_t) k * C + l] * fl * (k == 5 ? -1. : 1.);
+                    Lm[k] += sl * c * pl;
+                    Lp[k] += c * pl;
+               }
+          }
+     }
+     const int ip = H + j, im = H - 1 - j;
+     const double wq = w[ip];
+     for (int s = 0; s < 2; ++s) {
+          const double *E = s ? Ep : Em, *L = s ? Lp : Lm;
+          const int a = s ? ip : im;
+          pfbar[a] = wq * (E[0] + E[1] + E[2] + L[0] + L[1]);
+          pfbar[Q + a] = wq * (E[1] - E[2] + E[3] + L[2] + L[3]);
+          pfbar[2 * Q + a] = wq * (E[4] + L[4]);
+          pfbar[3 * Q + a] = wq * (E[5] + L[5]);
+     }
+}
+
+// Cotangents of the four complex sums at one (size, angle) from those of the
+// four matrix elements (gradient convention d/dRe + i d/dIm):
+//   sp^ = (p0 + p1) sp/2 - (p2 + i p3) sm/2,  sm^ = (p0 - p1) sm/2 - (p2 - i p3) sp/2,
+// the adjoint of P11 = (|sp|^2 + |sm|^2)/4, P33 = (|sp|^2 - |sm|^2)/4,
+// P12 = -Re(sp sm*)/2, P34 = -Im(sp sm*)/2 (src/lmie_core.c:726-744).
+__device__ __forceinline__ void stokes_adj(double spr, double spi, double smr, double smi, const double pb[4],
+                                           double out[4]) {
+     const double a = .5 * (pb[0] + pb[1]), c = .5 * (pb[0] - pb[1]), p2 = .5 * pb[2], p3 = .5 * pb[3];
+     out[0] = a * spr - (p2 * smr - p3 * smi);
+     out[1] = a * spi - (p2 * smi + p3 * smr);
+     out[2] = c * smr - (p2 * spr + p3 * spi);
+     out[3] = c * smi - (p2 * spi - p3 * spr);
+}
+
+// Angular part of the size-weight cotangent: fbar_i = sum_{j, +-, k} pfbar_k P_k(i, j),
+// one wave per size point.
+__global__ __launch_bounds__(256) void fbar_angular(const double *sst, const double *pfbar, int N, int H, int Hp,
+                                                    int Q, double *fang) {
+     const int lane = threadIdx.x & 63;
+     const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+     if (i >= N) return;
+     double acc = 0.;
+     for (int j = lane; j < H; j += 64) {
+          const double *q = sst + ((size_t) i * Hp + j) * 8;
+          double Pp[4], Pm[4];
+          stokes(q[0], q[1], q[2], q[3], Pp);
+          stokes(q[4], q[5], q[6], q[7], Pm);
+          for (int k = 0; k < 4; ++k)
+               acc += pfbar[k * Q + H + j] * Pp[k] + pfbar[k * Q + H - 1 - j] * Pm[k];
+     }
+     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+     if (lane == 0) fang[i] = acc;
+}
+
+// Transposed angular contraction on FP64 matrix cores: for every size i and
+// order n,
+//   Sbar_n = f_i sum_j [sp^+ (pi+tau)_n + s_n sp^- (pi-tau)_n],
+//   Tbar_n = f_i sum_j [sm^+ (pi-tau)_n + s_n sm^- (pi+tau)_n],
+// K = the positive angles j (16 per LDS stage), the cotangents of the sums
+// formed while staging from the checkpointed sums.  Tile 32 sizes x 32 orders.
+struct AdjArgs {
+     int Np, Kp, Hp, H, Q;
+     const double *sst;         // [Np][Hp][8]
+     const double *pfbar;       // [4][Q]
+     const double *Bp, *Bm;     // [Kp][Hp]
+     const double *f;           // [Np]
+     const int *ktile;          // orders per size tile of 32
+     double *bar;               // [4][Kp][Np]: Sbar re, im, Tbar re, im
+};
+
+__global__ __launch_bounds__(256) void angular_adj_f64(AdjArgs p) {
+     constexpr int JC = 16, TM = 32, TN = 32, SA = TM + 16, SB = TN + 16;
+     __shared__ double sA[8 * JC * SA];
+     __shared__ double sB[2 * JC * SB];
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wm = wave & 1, wn = wave >> 1;
+     const int l16 = lane & 15, lg = lane >> 4;
+     const int n0 = blockIdx.x * TN, it0 = blockIdx.y * TM;
+     if (n0 >= p.ktile[blockIdx.y]) return;          // whole tile beyond these sizes' orders
+     const size_t Np = p.Np, Hp = p.Hp, plane = (size_t) p.Kp * Np;
+     v4d acc[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+     const int ncol = wn * 16 + l16;                  // this lane's output order (B column)
+     const double sg = ((n0 + ncol) & 1) ? -1. : 1.;   // s_n, n = n0 + ncol + 1
+     for (int j0 = 0; j0 < p.H; j0 += JC) {
+          __syncthreads();
+          // stage the cotangents of the sums: 32 sizes x 16 angles
+          for (int e = threadIdx.x; e < TM * JC; e += 256) {
+               const int ii = e % TM, jj = e / TM;
+               const int i = it0 + ii, j = j0 + jj;
+               double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+               if (j < p.H) {
+                    const double *q = p.sst + ((size_t) i * Hp + j) * 8;
+                    double pb[4], pm[4];
+                    for (int k = 0; k < 4; ++k) {
+                         pb[k] = p.pfbar[k * p.Q + p.H + j];
+                         pm[k] = p.pfbar[k * p.Q + p.H - 1 - j];
+                    }
+                    stokes_adj(q[0], q[1], q[2], q[3], pb, out);      // sp^+, sm^+
+                    stokes_adj(q[4], q[5], q[6], q[7], pm, out + 4);  // sp^-, sm^-
+               }
+               for (int c = 0; c < 8; ++c) sA[(c * JC + jj) * SA + ii] = out[c];
+          }
+          // stage pi +- tau: rows = angles, columns = orders n0 .. n0 + 31
+          for (int e = threadIdx.x; e < TN * JC; e += 256) {
+               const int nn = e % TN, jj = e / TN;
+               const int j = j0 + jj;
+               const size_t o = (size_t) (n0 + nn) * Hp + j;
+               sB[jj * SB + nn] = j < p.H ? p.Bp[o] : 0.;
+               sB[(JC + jj) * SB + nn] = j < p.H ? p.Bm[o] : 0.;
+          }
+          __syncthreads();
+#pragma unroll
+          for (int ks = 0; ks < JC / 4; ++ks) {
+               const int r = ks * 4 + lg;
+               const int ic = wm * 16 + l16;
+               const double bp = sB[r * SB + ncol], bm = sB[(JC + r) * SB + ncol];
+               const double bps = sg * bp, bms = sg * bm;
+               const double spr = sA[(0 * JC + r) * SA + ic], spi = sA[(1 * JC + r) * SA + ic];
+               const double smr = sA[(2 * JC + r) * SA + ic], smi = sA[(3 * JC + r) * SA + ic];
+               const double qpr = sA[(4 * JC + r) * SA + ic], qpi = sA[(5 * JC + r) * SA + ic];
+               const double qmr = sA[(6 * JC + r) * SA + ic], qmi = sA[(7 * JC + r) * SA + ic];
+               acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(spr, bp, acc[0], 0, 0, 0);
+               acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(qpr, bms, acc[0], 0, 0, 0);
+               acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(spi, bp, acc[1], 0, 0, 0);
+               acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(qpi, bms, acc[1], 0, 0, 0);
+               acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(smr, bm, acc[2], 0, 0, 0);
+               acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(qmr, bps, acc[2], 0, 0, 0);
+               acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(smi, bm, acc[3], 0, 0, 0);
+               acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(qmi, bps, acc[3], 0, 0, 0);
+          }
+     }
+     // result rows (sizes) l/16 + 4 v, column (order) l % 16
+     for (int v = 0; v < 4; ++v) {
+          const int i = it0 + wm * 16 + lg + 4 * v;
+          const int n = n0 + ncol;            // row n of the planes holds order n + 1
+          const double fi = p.f[i];
+          for (int c = 0; c < 4; ++c) p.bar[c * plane + (size_t) n * Np + i] = fi * acc[c][v];
+     }
+}
+
+// Per-size reverse sweep (adjoint of src/lmie_core.c:298-611): cotangents of
+// a_n, b_n from the angular part (Sbar, Tbar) and from the series, through
+// a_n(u), b_n(v) to D_n and m, and through the recomputed downward recurrence
+// D_j = (j+1)/z - 1/(D_{j+1} + (j+1)/z) upward from j = 1 to n2 - 1, to z = m x.
+// Outputs per size: the cotangent of m (re, im as d/dRe + i d/dIm) and of f_i.
+struct RevArgs {
+     int N, Np, Kp, n1max;
+     const double *x, *sx, *cx, *f;
+     const int *n1, *n2;
+     double mr, mi;
+     const double *AB;          // [4][Kp][Np]
+     const double *CB;          // [2][Np]
+     const double *S;           // series [4][Np]
+     const double *bar;         // [4][Kp][Np]
+     const double *fang;        // [Np]
+     double sb0, sb1, sb2, sb3; // cotangents of the raw sums
+     double *scrD;              // [3][n2max][Np] recomputed D_j (re, im) and ratios r_j
+     int n2max;
+     double *mbar;              // [2][Np]
+     double *fbar;              // [Np]
+};
+
+__global__ __launch_bounds__(64) void mie_reverse(RevArgs p) {
+     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+     if (i >= p.Np) return;
+     const size_t Np = p.Np, plane = (size_t) p.Kp * Np, drow = (size_t) p.n2max * Np;
+     if (i >= p.N) {
+          p.mbar[i] = p.mbar[Np + i] = 0.;
+          p.fbar[i] = 0.;
+          return;
+     }
+     const double x = p.x[i], f = p.f[i];
+     const int n1 = p.n1[i], n2 = p.n2[i];
+     const cd m = cmk(p.mr, -p.mi), z = cscale(m, x), m2 = cmul(m, m);
+     // size-weight cotangent
+     p.fbar[i] = p.fang[i] + p.sb0 * p.S[i] + p.sb1 * p.S[Np + i] + p.sb2 * p.S[2 * Np + i] + p.sb3 * p.S[3 * Np + i];
+     // recompute and keep D_j, j = 1 .. n2 - 1 (D_{n2} = 0)
+     double *Dre = p.scrD, *Dim = p.scrD + drow;
+     {
+          cd D = cmk(0., 0.);
+          for (int j = n2 - 1; j >= 1; --j) {
+               const cd c1 = cdiv(cmk(j + 1., 0.), z);
+               D = csub(c1, crecip(cadd(D, c1)));
+               Dre[(size_t) j * Np + i] = D.r;
+               Dim[(size_t) j * Np + i] = D.i;
+          }
+     }
+     // psi_n recomputed as in the forward: downward ratios r_j into the third
+     // scratch plane, then psi_j = r_j psi_{j-1} upward
+     auto AB = [&](int q, int n) { return p.AB[q * plane + (size_t) (n - 1) * Np + i]; };
+     const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
+     double chi0 = p.cx[i], chi1 = (1. / x) * p.cx[i] + p.sx[i];
+     double psi0 = p.sx[i];
+     double *R = p.scrD + 2 * drow;
+     {
+          double r = 0.;
+          R[(size_t) n1 * Np + i] = 0.;
+          for (int j = n1 - 1; j >= 1; --j) {
+               r = 1. / ((2. * j + 1.) / x - r);
+               R[(size_t) j * Np + i] = r;
+          }
+     }
+     const cd im1 = crecip(m);
+     cd mb = cmk(0., 0.), carry = cmk(0., 0.), zb = cmk(0., 0.);
+     double sgn = -1.;
+     const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
+     for (int j = 1; j < n2; ++j) {
+          cd Db = carry;
+          if (j <= n1) {
+               const double psi = R[(size_t) j * Np + i] * psi0;
+               const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
+               const cd Dj = cmk(Dre[(size_t) j * Np + i], Dim[(size_t) j * Np + i]);
+               const double jx = j / x;
+               const cd a = cmk(AB(0, j), AB(1, j)), b = cmk(AB(2, j), AB(3, j));
+               const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
+               const cd Sb = cmk(p.bar[(size_t) (j - 1) * Np + i], p.bar[plane + (size_t) (j - 1) * Np + i]);
+               const cd Tb = cmk(p.bar[2 * plane + (size_t) (j - 1) * Np + i], p.bar[3 * plane + (size_t) (j - 1) * Np + i]);
+               // angular part
+               cd ab = cscale(cadd(Sb, Tb), fv4), bb = cscale(csub(Sb, Tb), fv4);
+               // series parts: ext, sca, bak, asym
+               ab.r += sb0 * fv1;
+               bb.r += sb0 * fv1;
+               ab = cadd(ab, cscale(a, 2. * sb1 * fv1));
+               bb = cadd(bb, cscale(b, 2. * sb1 * fv1));
+               ab = cadd(ab, cscale(cbv, 2. * sb2 * fv1 * sgn));
+               bb = csub(bb, cscale(cbv, 2. * sb2 * fv1 * sgn));
+               if (j < n1) {
+                    const double fv3 = (j + 2.) / ((j + 1.) / j);
+                    const cd an = cmk(AB(0, j + 1), AB(1, j + 1)), bn = cmk(AB(2, j + 1), AB(3, j + 1));
+                    ab = cadd(ab, cadd(cscale(an, sb3 * fv3), cscale(b, sb3 * fv4)));
+                    bb = cadd(bb, cadd(cscale(bn, sb3 * fv3), cscale(a, sb3 * fv4)));
+               }
+               if (j > 1) {
+                    const double fv3m = (j + 1.) / ((double) j / (j - 1.));
+                    const cd am = cmk(AB(0, j - 1), AB(1, j - 1)), bm = cmk(AB(2, j - 1), AB(3, j - 1));
+                    ab = cadd(ab, cscale(am, sb3 * fv3m));
+                    bb = cadd(bb, cscale(bm, sb3 * fv3m));
+               }
+               // a = (u psi - psi0) / (u zeta - zeta0), da/du = (psi - a zeta) / den
+               const cd u = cadd(cdiv(Dj, m), cmk(jx, 0.));
+               const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
+               const cd dua = cdiv(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
+               const cd dvb = cdiv(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
+               const cd ub = cmul(cconj(dua), ab), vb = cmul(cconj(dvb), bb);
+               // u = D/m + j/x, v = m D + j/x
+               Db = cadd(Db, cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb)));
+               mb = cadd(mb, cadd(cmul(cconj(cscale(cdiv(Dj, m2), -1.)), ub), cmul(cconj(Dj), vb)));
+               // advance psi, chi
+               const double chi2 = fv1 / x * chi1 - chi0;
+               chi0 = chi1;
+               chi1 = chi2;
+               psi0 = psi;
+               sgn = -sgn;
+          }
+          // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z
+          const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
+          const cd c1 = cdiv(cmk(j + 1., 0.), z);
+          const cd c2 = cadd(Dn, c1);
+          const cd ic22 = crecip(cmul(c2, c2));
+          carry = cmul(cconj(ic22), Db);
+          const cd dc1dz = cscale(cdiv(c1, z), -1.);
+          zb = cadd(zb, cmul(cconj(cmul(cadd(cmk(1., 0.), ic22), dc1dz)), Db));
+     }
+     mb = cadd(mb, cscale(zb, x));
+     p.mbar[i] = mb.r;
+     p.mbar[Np + i] = mb.i;
 }
 
 // Sum of the per-block partials: one thread per (pass channel, element, angle);
@@ -1213,8 +1539,11 @@ struct Workspace {
      Buf x, sx, cx, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
      Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2;
      std::vector<double> hsc;       // host staging of sin x, cos x
-     // kept for the adjoint of the same call
+     // reverse-pass checkpoint and buffers
+     Buf AB, CB, sst, pfbar, bar, fang, mbar, fbar, gbar, scrRev, ones;
      int valid = 0;
+     int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
+     double mr = 0., mi = 0.;
 };
 
 pthread_mutex_t g_pool_mu = PTHREAD_MUTEX_INITIALIZER;
@@ -1322,7 +1651,7 @@ void gauss_guesses(int n, double *g) {
 int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      const int N = J->n_size, Np = rup(N, TILE);
      const int n1max = J->n1_max, Q = J->n_qang, C = J->n_coef;
-     const int H = Q / 2, Hp = rup(H > 0 ? H : 1, TILE), Kp = rup(n1max, KC);
+     const int H = Q / 2, Hp = rup(H > 0 ? H : 1, TILE), Kp = rup(n1max, 32);
      const int nt = J->n_tan, nw = J->n_wgt, nch = 1 + nt + nw;
      if (N <= 0) return set_err("no size points");
      if (Q % 2) return set_err("odd n_qang");
@@ -1422,7 +1751,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                else
                     hipLaunchKernelGGL(pitau_table16<F16>, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kb,
                                        (const double *) w->mu.as<double>(), w->B16.as<unsigned short>());
-          } else {
+          }
+          if (!mixed || J->keep) {
                hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kp, w->mu.as<double>(),
                                   w->Bp.as<double>(), w->Bm.as<double>());
           }
@@ -1444,6 +1774,13 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
          (mixed && w->A16.get(sizeof(unsigned short) * 8 * (size_t) Kb * Np * 8)) ||
          w->scrD.get(sizeof(double) * 2 * srow * (1 + nt_pass)) || w->scrR.get(sizeof(double) * srow * (1 + nt_pass)))
           return -1;
+     const int keep = J->keep;
+     if (keep) {
+          if (nt || J->save_control == 1) return set_err("the reverse-pass checkpoint covers the value channel only");
+          if (w->AB.get(sizeof(double) * 4 * plane) || w->CB.get(sizeof(double) * 2 * Np) ||
+              (Q > 0 && w->sst.get(sizeof(double) * 8 * (size_t) Np * Hp)))
+               return -1;
+     }
      tm.mark(LMIE_T_COEFS, 0);
      for (int t0 = 0; t0 < (nt ? nt : 1); t0 += 4) {
           CoefArgs c;
@@ -1466,6 +1803,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                c.St[q] = w->S.as<double>() + (size_t) 4 * Np * (1 + t0 + q);
           }
           c.write_base = t0 == 0;
+          c.AB = keep ? w->AB.as<double>() : nullptr;
+          c.CB = keep ? w->CB.as<double>() : nullptr;
           if (mixed) {
                c.A16 = w->A16.as<unsigned short>();
                c.Kb = Kb;
@@ -1544,6 +1883,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                for (int q = 0; q < nw; ++q) m.W[1 + q] = w->dfw.as<double>() + (size_t) q * Np;
                m.tiles_per_block = 4;
                m.part = w->part.as<double>();
+               m.sst = keep ? w->sst.as<double>() : nullptr;
                dim3 grid(Hp / 64, plan[0].nblk);
 #define LMIE_MIX(E, NWV) hipLaunchKernelGGL((angular_mix<E, 2, 2, NWV>), grid, dim3(256), 0, s, m)
                if (bf16) {
@@ -1571,6 +1911,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                for (int q = 0; q < P.ntp; ++q) a.WT[q] = w->dft.as<double>() + (size_t) (P.t0 + q) * Np;
                a.part = w->part.as<double>() + P.off;
                a.save2 = (ps == 0 && J->save_control == 1) ? w->save2.as<double>() : nullptr;
+               a.sst = (ps == 0 && keep) ? w->sst.as<double>() : nullptr;
                int nblk = 0, rc;
                if (P.ntp == 0) {
                     a.ktile = w->ktile64.as<int>();
@@ -1635,6 +1976,83 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      const int ids[] = {LMIE_T_TABLES, LMIE_T_COEFS, LMIE_T_ANGULAR, LMIE_T_REDUCE, LMIE_T_EXPAND, LMIE_T_TOTAL};
      tm.collect(ids, Q > 0 ? 6 : 2);
      w->valid = 1;
+     w->N = N; w->Np = Np; w->Kp = Kp; w->Hp = Hp; w->H = H; w->Q = Q; w->C = C; w->n1max = n1max;
+     w->n2max = 0;
+     for (int i = 0; i < N; ++i) w->n2max = J->n2[i] > w->n2max ? J->n2[i] : w->n2max;
+     w->calc_gc = J->calc_gc; w->calc_lc = J->calc_lc; w->mr = J->mr; w->mi = J->mi;
+     return 0;
+}
+
+int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
+     if (!w->valid) return set_err("no forward state");
+     const int N = w->N, Np = w->Np, Kp = w->Kp, Hp = w->Hp, H = w->H, Q = w->Q, C = w->C;
+     hipStream_t s = w->stream;
+     Timer tm{w, g_timing};
+     if (w->pfbar.get(sizeof(double) * 4 * (size_t) (Q > 0 ? Q : 1)) || w->fang.get(sizeof(double) * Np) ||
+         w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * Np) ||
+         w->gbar.get(sizeof(double) * 12 * (size_t) (C > 0 ? C : 1)) ||
+         w->bar.get(sizeof(double) * 4 * (size_t) Kp * Np) ||
+         w->scrRev.get(sizeof(double) * 3 * (size_t) w->n2max * Np) || w->ones.get(sizeof(double) * Np))
+          return -1;
+     tm.mark(LMIE_T_TOTAL, 0);
+     HIPCHECK(hipMemsetAsync(w->fang.p, 0, sizeof(double) * Np, s));
+     const int use_gc = in->gc_bar && w->calc_gc, use_lc = in->lc_bar && w->calc_lc;
+     if (Q > 0 && (use_gc || use_lc)) {
+          tm.mark(LMIE_T_ADJ_ANGULAR, 0);
+          double *g = w->gbar.as<double>();
+          if (use_gc) HIPCHECK(hipMemcpyAsync(g, in->gc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
+          if (use_lc) HIPCHECK(hipMemcpyAsync(g + 6 * C, in->lc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
+          hipLaunchKernelGGL(expand_adjoint, dim3((H + 63) / 64), dim3(64), 0, s, (const double *) w->T.as<double>(),
+                             (const double *) w->w.as<double>(), use_gc ? (const double *) g : nullptr,
+                             use_lc ? (const double *) (g + 6 * C) : nullptr, H, Hp, C, Q, w->pfbar.as<double>());
+          hipLaunchKernelGGL(fbar_angular, dim3((N + 3) / 4), dim3(256), 0, s, (const double *) w->sst.as<double>(),
+                             (const double *) w->pfbar.as<double>(), N, H, Hp, Q, w->fang.as<double>());
+          AdjArgs aa;
+          aa.Np = Np; aa.Kp = Kp; aa.Hp = Hp; aa.H = H; aa.Q = Q;
+          aa.sst = w->sst.as<double>(); aa.pfbar = w->pfbar.as<double>();
+          aa.Bp = w->Bp.as<double>(); aa.Bm = w->Bm.as<double>();
+          aa.f = w->f.as<double>(); aa.ktile = w->ktile32.as<int>(); aa.bar = w->bar.as<double>();
+          hipLaunchKernelGGL(angular_adj_f64, dim3(Kp / 32, Np / 32), dim3(256), 0, s, aa);
+          HIPCHECK(hipGetLastError());
+          tm.mark(LMIE_T_ADJ_ANGULAR, 1);
+     } else {
+          HIPCHECK(hipMemsetAsync(w->bar.p, 0, sizeof(double) * 4 * (size_t) Kp * Np, s));
+     }
+     tm.mark(LMIE_T_ADJ_COEFS, 0);
+     RevArgs r;
+     r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max;
+     r.x = w->x.as<double>(); r.sx = w->sx.as<double>(); r.cx = w->cx.as<double>(); r.f = w->f.as<double>();
+     r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
+     r.AB = w->AB.as<double>(); r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
+     r.fang = w->fang.as<double>();
+     r.sb0 = in->sums_bar[0]; r.sb1 = in->sums_bar[1]; r.sb2 = in->sums_bar[2]; r.sb3 = in->sums_bar[3];
+     r.scrD = w->scrRev.as<double>(); r.n2max = w->n2max;
+     r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
+     hipLaunchKernelGGL(mie_reverse, dim3(Np / 64), dim3(64), 0, s, r);
+     HIPCHECK(hipGetLastError());
+     tm.mark(LMIE_T_ADJ_COEFS, 1);
+     // sum of the per-size m cotangents
+     {
+          std::vector<double> one(Np, 1.);
+          HIPCHECK(hipMemcpyAsync(w->ones.p, one.data(), sizeof(double) * Np, hipMemcpyHostToDevice, s));
+          DotJob jobs[2] = {{w->ones.as<double>(), w->mbar.as<double>(), nullptr, nullptr},
+                            {w->ones.as<double>(), w->mbar.as<double>() + Np, nullptr, nullptr}};
+          if (w->dotjobs.get(sizeof(jobs)) || w->dots.get(sizeof(double) * 2)) return -1;
+          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs, sizeof(jobs), hipMemcpyHostToDevice, s));
+          hipLaunchKernelGGL(weighted_dots, dim3(2), dim3(256), 0, s, (const DotJob *) w->dotjobs.as<DotJob>(), Np,
+                             w->dots.as<double>());
+          HIPCHECK(hipGetLastError());
+          double mb[2];
+          HIPCHECK(hipMemcpyAsync(mb, w->dots.p, sizeof(mb), hipMemcpyDeviceToHost, s));
+          HIPCHECK(hipMemcpyAsync(out->f_bar, w->fbar.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+          tm.mark(LMIE_T_TOTAL, 1);
+          HIPCHECK(hipStreamSynchronize(s));
+          // m = mr - i mi: dL/dmr = Re(mbar), dL/dmi = -Im(mbar)
+          out->m_bar[0] = mb[0];
+          out->m_bar[1] = -mb[1];
+     }
+     const int ids[] = {LMIE_T_ADJ_ANGULAR, LMIE_T_ADJ_COEFS, LMIE_T_TOTAL};
+     tm.collect(ids, 3);
      return 0;
 }
 
@@ -1695,15 +2113,29 @@ int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
           return set_err("unknown precision %d", job->precision);
      Workspace *w = nullptr;
      if (ws_acquire(job->device, &w)) return -1;
+     res->state = nullptr;
      int rc = forward_impl(job, res, w);
      if (rc) (void) hipStreamSynchronize(w->stream);
+     if (rc == 0 && job->keep) {
+          res->state = w;          // owned by the caller until lmie_gpu_release
+          return 0;
+     }
      ws_release(w);
      return rc;
 }
 
-int lmie_gpu_adjoint(const lmie_gpu_job *job, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
-     (void) job; (void) in; (void) out;
-     return set_err("adjoint pass not available in this build");
+int lmie_gpu_adjoint(void *state, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
+     g_err[0] = 0;
+     if (!state || !in || !out || !in->sums_bar || !out->f_bar) return set_err("null argument");
+     Workspace *w = (Workspace *) state;
+     HIPCHECK(hipSetDevice(w->device));
+     int rc = adjoint_impl(w, in, out);
+     if (rc) (void) hipStreamSynchronize(w->stream);
+     return rc;
+}
+
+void lmie_gpu_release(void *state) {
+     if (state) ws_release((Workspace *) state);
 }
 
 int lmie_gpu_phase_func(int n_set, int n_coef, int ld, const double *coefs, int n_theta, const double *theta,

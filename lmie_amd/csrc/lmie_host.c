@@ -405,6 +405,7 @@ typedef struct {
      int calc_gc, calc_lc;
      double accuracy;
      const lmie_opts *opts;
+     int keep;                      /* keep the device state for a reverse pass */
 } core_args;
 
 typedef struct {
@@ -415,6 +416,7 @@ typedef struct {
      double ***save1, ****save2;
      int n_qang, n_coef;
      double sums[4];                /* raw sums, kept for the adjoint */
+     void *state;                   /* device state when core_args.keep */
 } core_out;
 
 static void fill_job(const core_args *c, lmie_gpu_job *j, double *x, double *f, int *n1, int *n2,
@@ -453,6 +455,7 @@ static void fill_job(const core_args *c, lmie_gpu_job *j, double *x, double *f, 
      j->save_control = c->save_control;
      j->precision = c->opts ? c->opts->precision : LMIE_PRECISION_FP64;
      j->device = c->opts ? c->opts->device : 0;
+     j->keep = c->keep;
 }
 
 typedef struct {
@@ -510,6 +513,7 @@ static int core_run(const core_args *c, core_out *o) {
           fail("lmie_gpu_forward", lmie_gpu_last_error());
           goto done;
      }
+     o->state = res.state;
      /* per-channel raw tangent sums -> seed order */
      double *rs_l = vec((size_t) 4 * (nd ? nd : 1));
      for (int t = 0; t < c->nd1; ++t)
@@ -574,7 +578,7 @@ int lmie_core_solution_ex(int n_qsiz, int n_derivs1, int n_derivs2, int save_con
      c.opts = opts;
      double dummy[4];
      core_out o = {cext_, csca_, pbak_, g, gc, lc, nd ? cext_l_ : dummy, nd ? csca_l_ : dummy,
-                   nd ? pbak_l_ : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}};
+                   nd ? pbak_l_ : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}, NULL};
      return core_run(&c, &o);
 }
 
@@ -676,6 +680,7 @@ static int user_angles(int n_coef, int n_angles, int nd, int ld, double **xc, do
 typedef struct {
      int calc_gc, calc_lc, calc_pf, n_angles, save_control;
      double lambda, mr, mi, accuracy;
+     int keep;
 } run_flags;
 
 static int solve_grid(const run_flags *rf, grid_t *g, const double *lambda_l, const double *mr_l,
@@ -717,9 +722,10 @@ static int solve_grid(const run_flags *rf, grid_t *g, const double *lambda_l, co
      c.qx = g->qx; c.qw = g->qw; c.nr = g->nr; c.qx_l = g->qx_l; c.qw_l = g->qw_l; c.nr_l = g->nr_l;
      c.r2 = g->r2; c.calc_gc = rf->calc_gc; c.calc_lc = rf->calc_lc; c.accuracy = rf->accuracy;
      c.opts = opts;
+     c.keep = rf->keep;
      double dummy[4];
      core_out o = {cext, csca, cbak, g_, gc, lc, nd ? cext_l : dummy, nd ? csca_l : dummy,
-                   nd ? cbak_l : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}};
+                   nd ? cbak_l : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}, NULL};
      int rc = 0;
      if (rf->save_control == 2) rc = fail("lmie_solution", "save_control = 2 is not supported");
      else rc = core_run(&c, &o);
@@ -764,7 +770,7 @@ static int solution2_l_opts(int calc_gc, int calc_lc, int calc_pf, enum size_dis
           grid_free(&gr);
           return fail("get_particle_dist", lmie_last_error());
      }
-     run_flags rf = {calc_gc, calc_lc, calc_pf, n_angles, save_control, lambda, mr, mi, accuracy};
+     run_flags rf = {calc_gc, calc_lc, calc_pf, n_angles, save_control, lambda, mr, mi, accuracy, 0};
      int rc = solve_grid(&rf, &gr, lambda_l, mr_l, mi_l, opts, n_coef, r21, r22, norm, reff, veff,
                          gavg, vavg, ravg, rvw, cext, csca, cbak, g, gc, lc, theta, pf, r21_l, r22_l,
                          norm_l, reff_l, veff_l, gavg_l, vavg_l, ravg_l, rvw_l, cext_l, csca_l,
@@ -937,7 +943,7 @@ int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_mo
      mix_seed_column(modes, n_modes, 1, mr_l);
      mix_seed_column(modes, n_modes, 2, mi_l);
      run_flags rf = {m0->calc_gc, m0->calc_lc, m0->calc_pf, m0->n_angles, m0->save_control,
-                     m0->lambda, m0->mr, m0->mi, m0->accuracy};
+                     m0->lambda, m0->mr, m0->mi, m0->accuracy, 0};
      int rc = solve_grid(&rf, &g, lam_l, mr_l, mi_l, opts, &out->n_coef, &out->r1, &out->r2,
                          &out->norm, &out->reff, &out->veff, &out->gavg, &out->vavg, &out->ravg,
                          &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g, out->gc, out->lc,
@@ -951,30 +957,51 @@ int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_mo
 }
 
 /* ------------------------------------------------------------------------- */
-/* Forward + reverse pass.  Output cotangents are pulled back through the
-   normalisation here, through the kernels by lmie_gpu_adjoint to f_i and m,
-   and from f_i to the distribution parameters with the grid tangents. */
-int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int n_modes,
-                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
-                          const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes) {
-     if (n_modes < 1 || !modes || !modes[0] || !adj || !grad)
-          return fail("lmie_solution_adjoint", "missing argument");
+/* Reverse mode.  The forward keeps its device state; the reverse pass pulls
+   the output cotangents back through the normalisation here
+   (src/lmie_schedule.c:1158-1199 read backwards), through the kernels to
+   f_i and m, and from f_i to the distribution parameters with the number
+   density tangents of each mode. */
+struct lmie_adj_ctx {
+     void *state;
+     int n_modes, N, C, calc_gc, calc_lc;
+     double lambda, Csca, g;
+     double *gc, *lc;               /* normalised outputs [6][C] */
+     double *w;                     /* mode weights */
+     double **dn;                   /* per mode: w_k qw_i d n_k(r_i) / d a_p, [N][5] */
+};
+
+void lmie_adjoint_free(lmie_adj_ctx *ctx) {
+     if (!ctx) return;
+     lmie_gpu_release(ctx->state);
+     free(ctx->gc); free(ctx->lc); free(ctx->w);
+     if (ctx->dn)
+          for (int k = 0; k < ctx->n_modes; ++k) free(ctx->dn[k]);
+     free(ctx->dn);
+     free(ctx);
+}
+
+int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n_modes,
+                         lmie_out_data *out, int alloc_out, const lmie_opts *opts,
+                         lmie_adj_ctx **ctxp) {
+     if (n_modes < 1 || !modes || !modes[0] || !ctxp) return fail("lmie_adjoint_forward", "missing argument");
+     *ctxp = NULL;
      const lmie_in_data *m0 = modes[0];
-     if (m0->save_control != 0) return fail("lmie_solution_adjoint", "save_control must be 0");
+     if (m0->save_control != 0) return fail("lmie_adjoint_forward", "save_control must be 0");
      if (alloc_out) {
           lmie_in_data tmp = *m0;
           tmp.n_derivs = 0;
           int mc = lmie_calc_max_coef(m0->lambda, m0->dist_type, m0->a1, m0->a2, m0->a3, m0->r1, m0->r2);
           if (lmie_out_alloc(&tmp, out, mc)) return fail("lmie_out_alloc", "failed");
      }
-     /* forward on the mixture grid, no seeds */
-     lmie_in_data **nm = calloc(n_modes, sizeof(*nm));
      lmie_in_data *copies = calloc(n_modes, sizeof(*copies));
+     lmie_in_data **nm = calloc(n_modes, sizeof(*nm));
      for (int k = 0; k < n_modes; ++k) { copies[k] = *modes[k]; copies[k].n_derivs = 0; nm[k] = &copies[k]; }
      grid_t g;
      int rc = mix_grid(nm, weight, n_modes, 0, &g);
-     if (rc) { free(nm); free(copies); return -1; }
-     run_flags rf = {m0->calc_gc, m0->calc_lc, 0, 0, 0, m0->lambda, m0->mr, m0->mi, m0->accuracy};
+     free(nm); free(copies);
+     if (rc) return -1;
+     run_flags rf = {m0->calc_gc, m0->calc_lc, 0, 0, 0, m0->lambda, m0->mr, m0->mi, m0->accuracy, 1};
      core_out keep;
      double zero = 0.;
      rc = solve_grid(&rf, &g, &zero, &zero, &zero, opts, &out->n_coef, &out->r1, &out->r2,
@@ -982,89 +1009,95 @@ int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int 
                      &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g, out->gc, out->lc,
                      out->theta, out->pf, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
                      NULL, NULL, NULL, NULL, NULL, NULL, NULL, &out->save1, &out->save2, &keep);
-     if (rc) { grid_free(&g); free(nm); free(copies); return -1; }
-     if (m0->calc_pf) {
-          /* the forward of the adjoint call also fills the user-angle matrix */
-          if (user_angles(out->n_coef, m0->n_angles, 0, keep.n_coef, m0->calc_gc ? out->gc : out->lc, NULL,
-                          out->theta, out->pf, NULL, m0->calc_gc, opts ? opts->device : 0)) {
-               grid_free(&g); free(nm); free(copies); return -1;
-          }
+     if (rc) { grid_free(&g); return -1; }
+     lmie_adj_ctx *c = calloc(1, sizeof *c);
+     c->state = keep.state;
+     c->n_modes = n_modes; c->N = g.N; c->C = keep.n_coef;
+     c->calc_gc = m0->calc_gc; c->calc_lc = m0->calc_lc;
+     c->lambda = m0->lambda; c->Csca = keep.sums[1]; c->g = out->g;
+     if (c->calc_gc) { c->gc = vec((size_t) 6 * c->C); memcpy(c->gc, out->gc[0], sizeof(double) * 6 * c->C); }
+     if (c->calc_lc) { c->lc = vec((size_t) 6 * c->C); memcpy(c->lc, out->lc[0], sizeof(double) * 6 * c->C); }
+     c->w = vec(n_modes);
+     c->dn = calloc(n_modes, sizeof(double *));
+     for (int k = 0; k < n_modes; ++k) {
+          const lmie_in_data *m = modes[k];
+          c->w[k] = weight ? weight[k] : 1.;
+          double e[5][5] = {{1, 0, 0, 0, 0}, {0, 1, 0, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 1, 0}, {0, 0, 0, 0, 1}};
+          seeds sv = {5, {e[0], e[1], e[2], e[3], e[4]}, NULL, NULL};
+          double a[5] = {m->a1, m->a2, m->a3, m->a4, m->a5};
+          grid_t gk;
+          memset(&gk, 0, sizeof gk);
+          c->dn[k] = vec((size_t) 5 * g.N);
+          if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0)
+               for (int i = 0; i < gk.N; ++i)
+                    for (int q = 0; q < 5; ++q) c->dn[k][(size_t) i * 5 + q] = c->w[k] * gk.nr_l[(size_t) i * 5 + q] * gk.qw[i];
+          grid_free(&gk);
      }
-     /* pull back through the normalisation (src/lmie_schedule.c:1158-1199) */
-     int C = keep.n_coef;
-     double Csca = keep.sums[1], s = m0->lambda * m0->lambda / (2. * LM_PI);
+     grid_free(&g);
+     *ctxp = c;
+     return 0;
+}
+
+int lmie_adjoint_reverse(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes) {
+     if (!c || !adj || !grad) return fail("lmie_adjoint_reverse", "missing argument");
+     int C = c->C;
+     double Csca = c->Csca, s = c->lambda * c->lambda / (2. * LM_PI);
      double sb[4];
      sb[0] = s * adj->cext;
      sb[2] = s * adj->cbak / (2. * 4. * LM_PI);
      sb[3] = 2. * adj->g / Csca;
-     double dot = adj->g * out->g;
+     double dot = adj->g * c->g;
      double *gcb = NULL, *lcb = NULL;
-     if (adj->gc && m0->calc_gc) {
+     if (adj->gc && c->calc_gc) {
           gcb = vec((size_t) 6 * C);
-          for (int k = 0; k < 6; ++k)
-               for (int l = 0; l < C; ++l) {
-                    gcb[k * C + l] = adj->gc[k * C + l] * 2. / Csca;
-                    dot += adj->gc[k * C + l] * out->gc[k][l];
-               }
+          for (int q = 0; q < 6 * C; ++q) {
+               gcb[q] = adj->gc[q] * 2. / Csca;
+               dot += adj->gc[q] * c->gc[q];
+          }
      }
-     if (adj->lc && m0->calc_lc) {
+     if (adj->lc && c->calc_lc) {
           lcb = vec((size_t) 6 * C);
-          for (int k = 0; k < 6; ++k)
-               for (int l = 0; l < C; ++l) {
-                    lcb[k * C + l] = adj->lc[k * C + l] * 2. / Csca;
-                    dot += adj->lc[k * C + l] * out->lc[k][l];
-               }
+          for (int q = 0; q < 6 * C; ++q) {
+               lcb[q] = adj->lc[q] * 2. / Csca;
+               dot += adj->lc[q] * c->lc[q];
+          }
      }
      sb[1] = s * adj->csca - dot / Csca;
-     /* reverse pass on the GPU */
-     core_args c;
-     memset(&c, 0, sizeof c);
-     c.N = g.N; c.lambda = m0->lambda; c.mr = m0->mr; c.mi = m0->mi;
-     c.lambda_l = c.mr_l = c.mi_l = &zero;
-     c.qx = g.qx; c.qw = g.qw; c.nr = g.nr; c.r2 = g.r2;
-     c.calc_gc = m0->calc_gc; c.calc_lc = m0->calc_lc; c.accuracy = m0->accuracy; c.opts = opts;
-     job_mem jm;
-     lmie_gpu_job job;
-     job_mem_alloc(&jm, g.N, 0, 0);
-     fill_job(&c, &job, jm.x, jm.f, jm.n1, jm.n2, jm.dx, jm.dm, jm.dft, jm.dfw);
      lmie_gpu_adj_in ain = {sb, gcb, lcb};
      lmie_gpu_adj_out aout;
-     aout.f_bar = vec(g.N);
-     aout.x_bar = vec(g.N);
-     rc = lmie_gpu_adjoint(&job, &ain, &aout);
+     aout.f_bar = vec(c->N);
+     int rc = lmie_gpu_adjoint(c->state, &ain, &aout);
      if (rc) {
           fail("lmie_gpu_adjoint", lmie_gpu_last_error());
      } else {
           grad->lambda = NAN; grad->r1 = NAN; grad->r2 = NAN;
           grad->mr = aout.m_bar[0];
           grad->mi = aout.m_bar[1];
-          /* f_i = qw_i sum_k w_k n_k(r_i): parameter p of mode k enters through n_k */
-          double *gp = vec((size_t) 5 * n_modes);
-          for (int k = 0; k < n_modes; ++k) {
-               const lmie_in_data *m = modes[k];
-               double e[5][5] = {{1, 0, 0, 0, 0}, {0, 1, 0, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 1, 0}, {0, 0, 0, 0, 1}};
-               double *cols[5];
-               for (int q = 0; q < 5; ++q) cols[q] = e[q];
-               seeds sv = {5, {cols[0], cols[1], cols[2], cols[3], cols[4]}, NULL, NULL};
-               double a[5] = {m->a1, m->a2, m->a3, m->a4, m->a5};
-               grid_t gk;
-               memset(&gk, 0, sizeof gk);
-               if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0) {
-                    double w = weight ? weight[k] : 1.;
-                    for (int i = 0; i < gk.N; ++i)
-                         for (int q = 0; q < 5; ++q)
-                              gp[5 * k + q] += aout.f_bar[i] * w * gk.nr_l[(size_t) i * 5 + q] * gk.qw[i];
-               }
-               grid_free(&gk);
-          }
+          /* f_i = qw_i sum_k w_k n_k(r_i): a_p of mode k enters through n_k */
+          double *gp = vec((size_t) 5 * c->n_modes);
+          for (int k = 0; k < c->n_modes; ++k)
+               for (int i = 0; i < c->N; ++i)
+                    for (int q = 0; q < 5; ++q) gp[5 * k + q] += aout.f_bar[i] * c->dn[k][(size_t) i * 5 + q];
           grad->a1 = gp[0]; grad->a2 = gp[1]; grad->a3 = gp[2]; grad->a4 = gp[3]; grad->a5 = gp[4];
           if (grad_modes)
-               for (int q = 0; q < 5 * n_modes; ++q) grad_modes[q] = gp[q];
+               for (int q = 0; q < 5 * c->n_modes; ++q) grad_modes[q] = gp[q];
           free(gp);
      }
-     free(aout.f_bar); free(aout.x_bar); free(gcb); free(lcb);
-     job_mem_free(&jm);
-     grid_free(&g);
-     free(nm); free(copies);
+     free(aout.f_bar); free(gcb); free(lcb);
      return rc ? -1 : 0;
+}
+
+int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int n_modes,
+                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
+                          lmie_cotangent_fn cot, void *user, lmie_grad *grad, double *grad_modes) {
+     if (!cot) return fail("lmie_solution_adjoint", "no cotangent function");
+     lmie_adj_ctx *c = NULL;
+     if (lmie_adjoint_forward(modes, weight, n_modes, out, alloc_out, opts, &c)) return -1;
+     lmie_adj_in adj;
+     memset(&adj, 0, sizeof adj);
+     int rc = cot(out, &adj, user);
+     if (rc) rc = fail("lmie_solution_adjoint", "the cotangent function failed");
+     else rc = lmie_adjoint_reverse(c, &adj, grad, grad_modes);
+     lmie_adjoint_free(c);
+     return rc;
 }

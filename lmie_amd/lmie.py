@@ -44,10 +44,14 @@ def bind_all(lib):
     lib.lmie_solution_mix.argtypes = [C.POINTER(PIn), C.POINTER(C.c_double), C.c_int, POut,
                                       C.c_int, C.POINTER(LmieOpts)]
     lib.lmie_solution_mix.restype = C.c_int
-    lib.lmie_solution_adjoint.argtypes = [C.POINTER(PIn), C.POINTER(C.c_double), C.c_int, POut,
-                                          C.c_int, C.POINTER(LmieOpts), C.POINTER(LmieAdjIn),
-                                          C.POINTER(LmieGrad), C.POINTER(C.c_double)]
-    lib.lmie_solution_adjoint.restype = C.c_int
+    lib.lmie_adjoint_forward.argtypes = [C.POINTER(PIn), C.POINTER(C.c_double), C.c_int, POut,
+                                         C.c_int, C.POINTER(LmieOpts), C.POINTER(C.c_void_p)]
+    lib.lmie_adjoint_forward.restype = C.c_int
+    lib.lmie_adjoint_reverse.argtypes = [C.c_void_p, C.POINTER(LmieAdjIn), C.POINTER(LmieGrad),
+                                         C.POINTER(C.c_double)]
+    lib.lmie_adjoint_reverse.restype = C.c_int
+    lib.lmie_adjoint_free.argtypes = [C.c_void_p]
+    lib.lmie_adjoint_free.restype = None
     lib.lmie_last_error.restype = C.c_char_p
     lib.lmie_gpu_last_error.restype = C.c_char_p
     lib.lmie_set_timing.argtypes = [C.c_int]
@@ -123,47 +127,87 @@ def solve_mix(lib, modes, weights, precision="fp64", device=0):
                 lib.lmie_in_free(C.byref(x), 1)
 
 
-def adjoint(lib, modes, weights, cot, precision="fp64", device=0):
-    """Forward + reverse pass.  cot: dict with optional keys cext, csca, cbak,
-    g (floats) and gc, lc ([6][max_coef] arrays).  Returns (outputs, grad) where
-    grad maps input names to dL/d input and 'modes' to an [n_modes][5] array of
-    the a1..a5 gradients of each mode."""
-    ins, arr = _modes(lib, [(p, None) for p in modes])
-    n = len(ins)
-    w = (C.c_double * n)(*(weights if weights is not None else [1.0] * n))
-    out = LmieOut()
-    max_coef = lib.lmie_calc_max_coef(ins[0].lambda_, ins[0].dist_type, ins[0].a1, ins[0].a2,
-                                      ins[0].a3, ins[0].r1, ins[0].r2)
-    keep = []
+class Adjoint:
+    """Forward pass that keeps its device state, then reverse passes.
 
-    def arrp(key):
-        v = cot.get(key)
-        if v is None:
-            return None
-        a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(6, max_coef))
-        keep.append(a)
-        return a.ctypes.data_as(C.POINTER(C.c_double))
+    modes: list of parameter dicts on one grid (one entry for a single
+    distribution); weights: number weights (None for one mode).
+    ``forward()`` returns the outputs; ``reverse(cot)`` takes the output
+    cotangents (keys cext, csca, cbak, g: floats; gc, lc: [6][max_coef]) and
+    returns dL/d input: keys lambda, mr, mi, a1..a5, r1, r2 (mode 0 for a..),
+    and 'modes' = [n_modes][5] gradients of a1..a5 per mode.
+    """
 
-    adj = LmieAdjIn(float(cot.get("cext", 0.0)), float(cot.get("csca", 0.0)),
-                    float(cot.get("cbak", 0.0)), float(cot.get("g", 0.0)), arrp("gc"), arrp("lc"))
-    g = LmieGrad()
-    gm = (C.c_double * (5 * n))()
-    o = _opts(precision, device)
-    rc = lib.lmie_solution_adjoint(arr, w, n, C.byref(out), 1, C.byref(o), C.byref(adj),
-                                   C.byref(g), gm)
-    try:
+    def __init__(self, lib, modes, weights=None, precision="fp64", device=0):
+        self.lib, self.modes = lib, modes
+        self.weights = weights if weights is not None else [1.0] * len(modes)
+        self.opts = _opts(precision, device)
+        self.ctx = C.c_void_p()
+        self.out = None
+
+    def forward(self):
+        lib = self.lib
+        ins, arr = _modes(lib, [(p, None) for p in self.modes])
+        n = len(ins)
+        w = (C.c_double * n)(*self.weights)
+        out = LmieOut()
+        self.max_coef = lib.lmie_calc_max_coef(ins[0].lambda_, ins[0].dist_type, ins[0].a1, ins[0].a2,
+                                               ins[0].a3, ins[0].r1, ins[0].r2)
+        rc = lib.lmie_adjoint_forward(arr, w, n, C.byref(out), 1, C.byref(self.opts), C.byref(self.ctx))
         if rc != 0:
-            raise RuntimeError("lmie_solution_adjoint failed: " + last_error(lib))
-        fake = LmieIn.from_buffer_copy(ins[0])
-        fake.n_derivs = 0
-        res = abi.read_out(lib, modes[0], fake, out, max_coef)
-        grad = {"lambda": g.lambda_, "mr": g.mr, "mi": g.mi, "a1": g.a1, "a2": g.a2, "a3": g.a3,
+            raise RuntimeError("lmie_adjoint_forward failed: " + last_error(lib))
+        try:
+            fake = LmieIn.from_buffer_copy(ins[0])
+            fake.n_derivs = 0
+            self.out = abi.read_out(lib, self.modes[0], fake, out, self.max_coef)
+        finally:
+            lib.lmie_out_free(C.byref(out), 0)
+        return self.out
+
+    def reverse(self, cot):
+        keep = []
+
+        def arrp(key):
+            v = cot.get(key)
+            if v is None:
+                return None
+            a = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(6, self.max_coef))
+            keep.append(a)
+            return a.ctypes.data_as(C.POINTER(C.c_double))
+
+        adj = LmieAdjIn(float(cot.get("cext", 0.0)), float(cot.get("csca", 0.0)),
+                        float(cot.get("cbak", 0.0)), float(cot.get("g", 0.0)), arrp("gc"), arrp("lc"))
+        g = LmieGrad()
+        n = len(self.modes)
+        gm = (C.c_double * (5 * n))()
+        rc = self.lib.lmie_adjoint_reverse(self.ctx, C.byref(adj), C.byref(g), gm)
+        if rc != 0:
+            raise RuntimeError("lmie_adjoint_reverse failed: " + last_error(self.lib))
+        return {"lambda": g.lambda_, "mr": g.mr, "mi": g.mi, "a1": g.a1, "a2": g.a2, "a3": g.a3,
                 "a4": g.a4, "a5": g.a5, "r1": g.r1, "r2": g.r2,
                 "modes": np.array(list(gm)).reshape(n, 5)}
-        return res, grad
+
+    def free(self):
+        if self.ctx:
+            self.lib.lmie_adjoint_free(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def adjoint(lib, modes, weights, cot_fn, precision="fp64", device=0):
+    """Forward, cotangents from cot_fn(outputs) -> dict, one reverse pass."""
+    a = Adjoint(lib, modes, weights, precision, device)
+    try:
+        out = a.forward()
+        grad = a.reverse(cot_fn(out))
     finally:
-        if rc == 0:
-            lib.lmie_out_free(C.byref(out), 0)
+        a.free()
+    return out, grad
 
 
 def nan_to_none(v):

@@ -1,0 +1,119 @@
+"""Reverse-mode gradients of the GPU library against
+  - the reference's forward tangents contracted with dL/d gc (the fixtures'
+    loss_grad: sum_kl dL/dc_kl gc_l[seed][k][l] from the reference build), and
+  - central differences of the library's own FP64 forward.
+Loss (workloads.expansion_loss_targets): L = sum_k w_k sum_l (c_kl - t_kl)^2,
+t = c_ref (1 + z), z ~ N(0,1), w_k = 1 / ||c_ref_k||^2.
+Per-component bar: |g - g_ref| <= tol * max(|g_ref|, 1e-3 ||g_ref||)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from lmie_amd import lmie, workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-8, "mixed": 1e-4}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def loss_fn(c_ref, seed):
+    t, w = W.expansion_loss_targets(c_ref, seed)
+
+    def cot(out):
+        return {"gc": 2.0 * w * (out["gc"] - t)}
+
+    return cot
+
+
+def check(g, ref, tol, label):
+    g, ref = np.asarray(g, float), np.asarray(ref, float)
+    floor = 1e-3 * np.linalg.norm(ref)
+    err = np.abs(g - ref) / np.maximum(np.abs(ref), floor)
+    say("  %s max rel err %.2e  (%s)" % (label, err.max(), " ".join("%.3e" % v for v in g)))
+    assert err.max() <= tol, (label, g, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_adjoint_cfg2_matches_reference_tangents(product, prec):
+    say("case: adjoint cfg2 (gamma, 16384 sizes, x 914)", prec)
+    z, p, _ = load_golden("cfg2")
+    out, g = lmie.adjoint(product, [p], None, loss_fn(z["gc"], 0), prec)
+    check([g["a1"], g["a2"], g["mr"], g["mi"]], z["loss_grad"], TOL[prec], prec)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_adjoint_cfg4_two_modes_match_reference_tangents(product, prec):
+    say("case: adjoint cfg4 (two modes, 32768 sizes, x 1848)", prec)
+    z = np.load(load_golden.__globals__["GOLDEN"] + "/cfg4.npz")
+    modes = [W.cfg4_mode_params(k) for k in range(2)]
+    wts = [m["weight"] for m in W.CFG4_MODES]
+    out, g = lmie.adjoint(product, modes, wts, loss_fn(z["gc"], 0), prec)
+    gm = g["modes"]
+    ds = [W.lognormal_sigma_seed(m["sigma_g"]) for m in W.CFG4_MODES]
+    ours = [gm[0][0], gm[0][1] * ds[0], gm[1][0], gm[1][1] * ds[1], g["mr"], g["mi"]]
+    check(ours, z["loss_grad"], TOL[prec], prec)
+
+
+def test_adjoint_cfg3_cases_match_reference_tangents(product):
+    say("case: adjoint cfg3 (32 retrieval cases)")
+    z = np.load(load_golden.__globals__["GOLDEN"] + "/cfg3.npz")
+    for c, (p, seeds) in enumerate(W.cfg3_cases()):
+        c_ref = z["gc_%d" % c] if "gc_%d" % c in z.files else lmie.solve(product, p, None, "fp64")["gc"]
+        out, g = lmie.adjoint(product, [p], None, loss_fn(c_ref, 100 + c), "fp64")
+        ours = [g["a1"], g["a2"] * seeds["a2"][1], g["mr"], g["mi"]]
+        check(ours, z["loss_grad"][c], TOL["fp64"], "case %d" % c)
+
+
+FD_CASE = dict(dist_type="log_normal", n_int2=8, n_quad=64, calc_gc=1, calc_lc=1, **{"lambda": 0.61},
+               mr=1.46, mi=0.004, a1=0.9, a2=0.3, r1=0.05, r2=6.0, accuracy=-1.0)
+
+
+@pytest.mark.parametrize("part", ["cext", "csca", "cbak", "g", "gc", "lc"])
+def test_adjoint_against_central_differences(product, part):
+    say("case: adjoint vs central differences (log-normal, 512 sizes, x_max 62), cotangent on", part)
+    p = FD_CASE
+    rng = np.random.Generator(np.random.PCG64(11))
+    base = lmie.solve(product, p, None, "fp64")
+    cot = {"cext": 0.0, "csca": 0.0, "cbak": 0.0, "g": 0.0}
+    if part in ("gc", "lc"):
+        cot[part] = rng.standard_normal(base[part].shape) / np.abs(base[part]).max(axis=1, keepdims=True)
+    else:
+        cot[part] = 1.0
+
+    def L(r):
+        if part in ("gc", "lc"):
+            return float(np.sum(cot[part] * r[part]))
+        return float(r[part])
+
+    out, g = lmie.adjoint(product, [p], None, lambda o: cot, "fp64")
+    bad = []
+    for key in ("mr", "mi", "a1", "a2"):
+        h = 1e-5 * abs(p[key])
+        fd = (L(lmie.solve(product, dict(p, **{key: p[key] + h}), None, "fp64")) -
+              L(lmie.solve(product, dict(p, **{key: p[key] - h}), None, "fp64"))) / (2 * h)
+        err = abs(g[key] - fd) / max(abs(fd), 1e-12)
+        say("  %s adjoint %.10e  central difference %.10e  rel %.1e" % (key, g[key], fd, err))
+        if err > 1e-6:
+            bad.append(key)
+    assert not bad, bad
+    assert math.isnan(g["lambda"]) and math.isnan(g["r1"]) and math.isnan(g["r2"])
+
+
+def test_autograd_wrapper(product):
+    torch = pytest.importorskip("torch")
+    from lmie_amd import autograd
+    say("case: torch.autograd wrapper (cfg0-like grid)")
+    p = dict(W.CFG0)
+    theta = torch.tensor([p["mr"], p["mi"], p["a1"], p["a2"]], dtype=torch.float64, requires_grad=True)
+    gc = autograd.expansion(theta, product, [p])
+    loss = (gc ** 2).sum()
+    loss.backward()
+    out, g = lmie.adjoint(product, [p], None, lambda o: {"gc": 2.0 * o["gc"]}, "fp64")
+    ref = np.array([g["mr"], g["mi"], g["a1"], g["a2"]])
+    assert np.allclose(theta.grad.numpy(), ref, rtol=1e-12, atol=0)
