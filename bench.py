@@ -1,7 +1,7 @@
 """Benchmark of the size-integrated Mie hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--path mixed-adjoint|fp64]
-                    [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--path mixed-adjoint|fp64-adjoint|mixed|fp64] [--no-cpu-baseline]
 
 Workload: two log-normal modes (r_g 0.08 um / sigma_g 1.5 and r_g 2.5 um /
 sigma_g 2.0, number ratio 1000:1) on one grid r in [0.001, 100] um with 512 x 64
@@ -10,7 +10,11 @@ n1 1905, 3810 Gauss angles, 6 x 3810 expansion coefficients.  Inputs are a few
 scalars; the size grid, the Mie coefficients and every table are generated on
 the device inside the timed step.
 
-A step is one call of the library for that workload.  The JSON line reports:
+A step of the default path (mixed-adjoint) is one forward solution in mixed
+precision that keeps its reverse-pass state, the loss cotangent of the six
+expansion-coefficient sets (weighted L2 against seeded targets), and one
+reverse pass giving dL/d(m_r, m_i, r_g and sigma_g of both modes).  The plain
+FP64 GPU forward of the same workload is timed beside it.  The JSON line reports:
   value        sum_i n1(x_i) over the size points per second (size-pt * n-term / s)
   roofline     the angular-contraction kernel: algorithmic FLOP per launch
                (8 n_qang sum_i n1(x_i)) over its device time, measured with HIP
@@ -69,7 +73,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--path", default="mixed", choices=["fp64", "mixed", "mixed-bf16"])
+    ap.add_argument("--path", default="mixed-adjoint",
+                    choices=["mixed-adjoint", "fp64-adjoint", "fp64", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -86,44 +91,67 @@ def main():
     units = sum_n1(W.cfg4_mode_params(0))
     nq = 2 * W.calc_n1(2 * math.pi * W.CFG4_GRID["r2"] / W.CFG4_GRID["lambda"])
 
+    # loss targets from one FP64 solution (outside the timed region)
+    c_ref = lmie.solve_mix(lib, modes, weights, "fp64")["gc"]
+    targets, wts = W.expansion_loss_targets(c_ref, 0)
+
+    def step(path):
+        if path.endswith("-adjoint"):
+            adj = lmie.Adjoint(lib, [m for m, _ in modes], weights, path.split("-")[0])
+            try:
+                out = adj.forward()
+                t_fwd = lmie.timings(lib)
+                grad = adj.reverse({"gc": 2.0 * wts * (out["gc"] - targets)})
+                t_rev = lmie.timings(lib)
+            finally:
+                adj.free()
+            tm = dict(t_fwd)
+            tm["adj_angular"], tm["adj_coefs"] = t_rev["adj_angular"], t_rev["adj_coefs"]
+            tm["total"] = t_fwd["total"] + t_rev["total"]
+            return tm, grad
+        lmie.solve_mix(lib, modes, weights, path)
+        return lmie.timings(lib), None
+
     def run(path, steps, warmup):
-        for _ in range(warmup):
-            lmie.solve_mix(lib, modes, weights, path)
         lib.lmie_set_timing(1)
-        t_ang, t_all = [], []
+        for _ in range(warmup):
+            step(path)
+        t_all = []
         t0 = time.perf_counter()
         for _ in range(steps):
-            lmie.solve_mix(lib, modes, weights, path)
-            tm = lmie.timings(lib)
-            t_ang.append(tm["angular"])
+            tm, grad = step(path)
             t_all.append(tm)
         dt = (time.perf_counter() - t0) / steps
         lib.lmie_set_timing(0)
         stages = {k: float(np.mean([t[k] for t in t_all])) for k in t_all[0]}
-        return dt, float(np.mean(t_ang)), stages
+        return dt, stages["angular"], stages, grad
 
-    dt, ang_ms, stages = run(a.path, a.steps, a.warmup)
+    dt, ang_ms, stages, grad = run(a.path, a.steps, a.warmup)
     ref_dt = None
     if a.path != "fp64":
-        ref_dt, ref_ang, ref_stages = run("fp64", max(2, a.steps // 3), 1)
+        ref_dt, _, ref_stages, _ = run("fp64", max(2, a.steps // 3), 1)
 
     flop = 8.0 * nq * units
     achieved = flop / (ang_ms * 1e-3)
-    peak = PEAK["fp64"] if a.path == "fp64" else PEAK["bf16"]
+    mixed = a.path.startswith("mixed")
+    peak = PEAK["bf16"] if mixed else PEAK["fp64"]
     line = {
         "metric": METRIC, "value": units / dt, "unit": "size-pt*n-term/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if a.path == "fp64" else ("f16-split/f32-acc+f64" if a.path == "mixed" else "bf16-split/f32-acc+f64"),
+        "dtype": ("bf16-split/f32-acc+f64" if "bf16" in a.path else "f16-split/f32-acc+f64") if mixed else "f64",
         "data": "synthetic: generated size grid of two log-normal modes",
         "config": {"workload": "two-mode log-normal, 32768 sizes, x_max 1848, 3810 angles, "
                                "gc 6x3810", "path": a.path, "sum_n1": units, "n_qang": nq,
                    "stage_ms": stages,
+                   "grads_per_s": (1.0 / dt) if a.path.endswith("-adjoint") else None,
+                   "grad": None if grad is None else {k: (float(v) if np.ndim(v) == 0 else np.asarray(v).tolist())
+                                                      for k, v in grad.items()},
                    "fp64_ms_per_step": None if ref_dt is None else ref_dt * 1e3,
                    "speedup_vs_fp64": None if ref_dt is None else ref_dt / dt},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "kernel": "angular_f64" if a.path == "fp64" else "angular_mix", "kernel_ms": ang_ms},
+                     "kernel": "angular_mix" if mixed else "angular_f64", "kernel_ms": ang_ms},
         "cpu_baseline": None,
     }
     if not a.no_cpu_baseline:
