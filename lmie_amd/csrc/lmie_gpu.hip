@@ -31,6 +31,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -1036,6 +1037,189 @@ __global__ __launch_bounds__(256) void angular_mix(MixArgs p) {
 }
 
 
+// The split 16-bit contraction, one workgroup per (size tile of 64, angle tile
+// of 32 WN) pair, two LDS stages: the next 32 orders of both operands are read
+// from HBM into registers while the matrix cores work on the current stage,
+// then written to the other stage (one barrier per stage).  Four waves in a
+// 2 x 2 grid, each 32 sizes x 16 WN angles, 8 accumulators per element as
+// angular_mix.  Workgroups are numbered so that the k-th workgroup of every XCD
+// (hardware dispatches linear ids round robin over the 8 XCDs) works on the
+// size tiles order[8 q + xcd], heaviest first, with all angle tiles of one size
+// tile adjacent: the A tile is read into one XCD's L2 once.  Partials per size
+// tile: part[tile][NW][4][Q].
+struct Mix2Args {
+     int Np, Kb, Hb, H, Q, nT, nJ;
+     const unsigned short *A16;     // [8][Kb][Np][8]
+     const unsigned short *B16;     // [4][Kb][Hb][8]
+     const int *ktile;              // orders per size tile of 64 (multiple of KM)
+     const int *order;              // size tiles by decreasing ktile
+     const double *W[4];            // weight channels [Np]
+     double *part;                  // [nT][NW][4][Q]
+     double *sst;                   // reverse-pass checkpoint [Np][Hb][8] or NULL
+};
+
+template <class E, int WN, int NW>
+__global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
+     constexpr int TM = 64, TN = 32 * WN, G = KM / 8;
+     constexpr int NA = 8 * G * TM / 256, NB = 4 * G * TN / 256;   // staged v4u per thread
+     __shared__ v4u sA[2][8 * G][TM];
+     __shared__ v4u sB[2][4 * G][TN];
+     static_assert(sizeof(sA) >= sizeof(double) * 2 * NW * 8 * TN, "epilogue buffer");
+
+     const int L = blockIdx.x, xcd = L & 7, r = L >> 3;
+     const int q = r / p.nJ, jt = r - q * p.nJ, kk = 8 * q + xcd;
+     if (kk >= p.nT) return;
+     const int st = p.order[kk];
+     const int it0 = st * TM, jt0 = jt * TN, kmax = p.ktile[st], nch = kmax / KM;
+
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wm = wave & 1, wn = wave >> 1;
+     const int l16 = lane & 15, lg = lane >> 4;
+     const size_t Np = p.Np, Hb = p.Hb;
+     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hb * 8;
+     const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
+
+     v4u ra[NA], rb[NB];
+     auto load = [&](int c) {
+          const int g0 = c * G;
+#pragma unroll
+          for (int t = 0; t < NA; ++t) {
+               const int e = threadIdx.x + 256 * t, row = e / TM, col = e % TM, pl = row / G, g = row % G;
+               ra[t] = *(const v4u *) (p.A16 + pl * pA + ((size_t) (g0 + g) * Np + it0 + col) * 8);
+          }
+#pragma unroll
+          for (int t = 0; t < NB; ++t) {
+               const int e = threadIdx.x + 256 * t, row = e / TN, col = e % TN, pl = row / G, g = row % G;
+               rb[t] = *(const v4u *) (p.B16 + pl * pB + ((size_t) (g0 + g) * Hb + jt0 + col) * 8);
+          }
+     };
+     auto store = [&](int s) {
+#pragma unroll
+          for (int t = 0; t < NA; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sA[s][e / TM][e % TM] = ra[t];
+          }
+#pragma unroll
+          for (int t = 0; t < NB; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sB[s][e / TN][e % TN] = rb[t];
+          }
+     };
+
+     v4f acc[2][WN][8];
+#pragma unroll
+     for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int k = 0; k < 8; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
+
+     if (nch > 0) {
+          load(0);
+          store(0);
+     }
+     for (int c = 0; c < nch; ++c) {
+          const int s = c & 1;
+          __syncthreads();
+          if (c + 1 < nch) load(c + 1);
+          v4u bph[WN], bpl[WN], bmh[WN], bml[WN];
+#pragma unroll
+          for (int b = 0; b < WN; ++b) {
+               const int jc = wn * 16 * WN + 16 * b + l16;
+               bph[b] = sB[s][0 * G + lg][jc];
+               bpl[b] = sB[s][1 * G + lg][jc];
+               bmh[b] = sB[s][2 * G + lg][jc];
+               bml[b] = sB[s][3 * G + lg][jc];
+          }
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+               const int ic = wm * 32 + 16 * a + l16;
+               v4u x[8], y[8];
+#pragma unroll
+               for (int k = 0; k < 8; ++k) {
+                    x[k] = sA[s][k * G + lg][ic];
+                    y[k] = x[k] ^ flip;
+               }
+#pragma unroll
+               for (int b = 0; b < WN; ++b) {
+                    v4f *cc = acc[a][b];
+                    // sp+ re, im = S (pi+tau); sm+ = T (pi-tau); sp- = s S (pi-tau); sm- = s T (pi+tau)
+                    cc[0] = E::mfma(x[0], bph[b], cc[0]); cc[1] = E::mfma(x[2], bph[b], cc[1]);
+                    cc[2] = E::mfma(x[4], bmh[b], cc[2]); cc[3] = E::mfma(x[6], bmh[b], cc[3]);
+                    cc[4] = E::mfma(y[0], bmh[b], cc[4]); cc[5] = E::mfma(y[2], bmh[b], cc[5]);
+                    cc[6] = E::mfma(y[4], bph[b], cc[6]); cc[7] = E::mfma(y[6], bph[b], cc[7]);
+                    cc[0] = E::mfma(x[0], bpl[b], cc[0]); cc[1] = E::mfma(x[2], bpl[b], cc[1]);
+                    cc[2] = E::mfma(x[4], bml[b], cc[2]); cc[3] = E::mfma(x[6], bml[b], cc[3]);
+                    cc[4] = E::mfma(y[0], bml[b], cc[4]); cc[5] = E::mfma(y[2], bml[b], cc[5]);
+                    cc[6] = E::mfma(y[4], bpl[b], cc[6]); cc[7] = E::mfma(y[6], bpl[b], cc[7]);
+                    cc[0] = E::mfma(x[1], bph[b], cc[0]); cc[1] = E::mfma(x[3], bph[b], cc[1]);
+                    cc[2] = E::mfma(x[5], bmh[b], cc[2]); cc[3] = E::mfma(x[7], bmh[b], cc[3]);
+                    cc[4] = E::mfma(y[1], bmh[b], cc[4]); cc[5] = E::mfma(y[3], bmh[b], cc[5]);
+                    cc[6] = E::mfma(y[5], bph[b], cc[6]); cc[7] = E::mfma(y[7], bph[b], cc[7]);
+               }
+          }
+          if (c + 1 < nch) store(s ^ 1);
+     }
+
+     // epilogue: Stokes elements in double, weighted sums over this wave's 32
+     // sizes per angle, combined over the two size halves (wm) through LDS
+     const double unscale = 1. / (kSA * kSB);
+     double *sR = (double *) &sA[0][0][0];   // [2][NW][8][TN]
+     __syncthreads();                        // every wave is done with the stages
+#pragma unroll
+     for (int b = 0; b < WN; ++b) {
+          const int jl = wn * 16 * WN + 16 * b + l16, j = jt0 + jl;
+          double ew[NW][8];
+#pragma unroll
+          for (int c = 0; c < NW; ++c)
+#pragma unroll
+               for (int k = 0; k < 8; ++k) ew[c][k] = 0.;
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+               for (int v = 0; v < 4; ++v) {
+                    const v4f *qq = acc[a][b];
+                    const int i = it0 + wm * 32 + 16 * a + mfma16_row(lane, v);
+                    if (p.sst) {
+                         double *o = p.sst + ((size_t) i * p.Hb + j) * 8;
+#pragma unroll
+                         for (int k = 0; k < 8; ++k) o[k] = qq[k][v] * unscale;
+                    }
+                    double Pp[4], Pm[4];
+                    stokes(qq[0][v] * unscale, qq[1][v] * unscale, qq[2][v] * unscale, qq[3][v] * unscale, Pp);
+                    stokes(qq[4][v] * unscale, qq[5][v] * unscale, qq[6][v] * unscale, qq[7][v] * unscale, Pm);
+#pragma unroll
+                    for (int c = 0; c < NW; ++c) {
+                         const double wv = p.W[c][i];
+#pragma unroll
+                         for (int k = 0; k < 4; ++k) {
+                              ew[c][k] += wv * Pp[k];
+                              ew[c][4 + k] += wv * Pm[k];
+                         }
+                    }
+               }
+#pragma unroll
+          for (int c = 0; c < NW; ++c)
+#pragma unroll
+               for (int k = 0; k < 8; ++k) {
+                    double t = ew[c][k];
+                    t += __shfl_xor(t, 16);
+                    t += __shfl_xor(t, 32);
+                    if (lg == 0) sR[((wm * NW + c) * 8 + k) * TN + jl] = t;
+               }
+     }
+     __syncthreads();
+     double *out = p.part + (size_t) st * NW * 4 * p.Q;
+     for (int e = threadIdx.x; e < NW * 8 * TN; e += 256) {
+          const int jl = e % TN, ck = e / TN, c = ck / 8, k = ck % 8, j = jt0 + jl;
+          if (j >= p.H) continue;
+          const double v = sR[(c * 8 + k) * TN + jl] + sR[((NW + c) * 8 + k) * TN + jl];
+          // k < 4: direct angle H + j; k >= 4: mirrored angle H - 1 - j
+          out[((size_t) c * 4 + (k & 3)) * p.Q + (k < 4 ? p.H + j : p.H - 1 - j)] = v;
+     }
+}
+
+
 // ---------------------------------------------------------------------------
 // Reverse pass.
 //
@@ -1206,6 +1390,359 @@ __global__ __launch_bounds__(256) void angular_adj_f64(AdjArgs p) {
      }
 }
 
+// ---------------------------------------------------------------------------
+// Reverse pass of the mixed path through the order-by-order kernel matrix.
+//
+// The cotangents of the four complex sums at an angle are linear in the sums
+// (stokes_adj): at +mu_j sp^ = a_j sp - beta_j sm, sm^ = c_j sm - conj(beta_j) sp
+// with a = (p0 + p1)/2, c = (p0 - p1)/2, beta = (p2 + i p3)/2 of that angle's
+// matrix-element cotangents, and likewise (a', c', beta') at -mu_j.  Putting
+// the forward sums sp = sum_m S_m (pi+tau)_m, ... into the transposed
+// contraction gives, for every size,
+//   Sbar_n = f (sum_m Gss(n,m) S_m - Gst(n,m) T_m)
+//   Tbar_n = f (sum_m Gtt(n,m) T_m - conj(Gst(m,n)) S_m)
+// with the size-independent matrices (P+- = pi +- tau at the positive nodes,
+// s_n s_m = (-1)^(n+m) from the mirrored nodes)
+//   Gss = sum_j a_j P+_n P+_m + s_n s_m a'_j P-_n P-_m,   Gtt = sum_j c_j P-_n P-_m + s_n s_m c'_j P+_n P+_m,
+//   Gst = sum_j beta_j P+_n P-_m + s_n s_m beta'_j P-_n P+_m,
+// and the angular part of the size-weight cotangent is the quadratic form
+// sum_j pfbar . P(i, j) = 1/2 sum_n Re(conj S_n U_S,n + conj T_n U_T,n), U = Sbar / f.
+// So the reverse contraction needs no per-(size, angle) checkpoint: five
+// kernels build the weights, the split tables, G (split 16-bit, fp32
+// accumulation), its split operand, and U = G A on the matrix cores.
+struct AdjScal {
+     double wmax;               // largest |weight| (weights are stored divided by it)
+     unsigned gmax;             // bits of the largest |G| (non-negative float)
+};
+
+// Weights of the eight right-hand tables per positive node j (row q of wts):
+// a, a', c, c', beta_r, beta_i, beta'_r, beta'_i, divided by their largest
+// magnitude.  One workgroup.
+__global__ __launch_bounds__(1024) void adj_weights(const double *pfbar, int H, int Hq, int Q, double *wts,
+                                                    AdjScal *sc) {
+     __shared__ double red[1024];
+     double mx = 0.;
+     for (int j = threadIdx.x; j < H; j += blockDim.x)
+          for (int k = 0; k < 4; ++k)
+               mx = fmax(mx, fmax(fabs(pfbar[k * Q + H + j]), fabs(pfbar[k * Q + H - 1 - j])));
+     red[threadIdx.x] = mx;
+     __syncthreads();
+     for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+          if ((int) threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+          __syncthreads();
+     }
+     // |a|, |c| <= max(|p0|, |p1|) and |beta parts| <= |p| / 2: the bound of the pfbar
+     // magnitudes bounds every weight
+     const double wmax = red[0] > 0. ? red[0] : 1., iw = 1. / wmax;
+     for (int j = threadIdx.x; j < Hq; j += blockDim.x) {
+          double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          if (j < H) {
+               const double *pp = pfbar + H + j, *pm = pfbar + H - 1 - j;
+               v[0] = .5 * (pp[0] + pp[Q]);
+               v[1] = .5 * (pm[0] + pm[Q]);
+               v[2] = .5 * (pp[0] - pp[Q]);
+               v[3] = .5 * (pm[0] - pm[Q]);
+               v[4] = .5 * pp[2 * Q];
+               v[5] = .5 * pp[3 * Q];
+               v[6] = .5 * pm[2 * Q];
+               v[7] = .5 * pm[3 * Q];
+          }
+          for (int q = 0; q < 8; ++q) wts[(size_t) q * Hq + j] = v[q] * iw;
+     }
+     if (threadIdx.x == 0) {
+          sc->wmax = wmax;
+          sc->gmax = 0u;
+     }
+}
+
+// Split tables with angles along the contraction: [plane][Hq / 8][Kq][8],
+// 8 consecutive positive nodes per 16 bytes.  Left (4 planes): P+ and P- of
+// order r + 1 over r (r + 1) (r + 2) times kSB, high and low parts.  Right
+// (16 planes): weight q times P+ (q = 0, 3, 6, 7) or P- (q = 1, 2, 4, 5),
+// high and low parts.  One thread per (order, group of 8 nodes).
+template <class E>
+__global__ __launch_bounds__(64) void adj_tables(const double *Bp, const double *Bm, const double *wts, int H,
+                                                 int Hp, int Hq, int Kq, int n1max, unsigned short *L16,
+                                                 unsigned short *R16) {
+     const int r = blockIdx.x * 64 + threadIdx.x, jg = blockIdx.y;
+     if (r >= Kq) return;
+     const size_t plane = (size_t) (Hq / 8) * Kq * 8, o = ((size_t) jg * Kq + r) * 8;
+     unsigned short lv[4][8], rv[16][8];
+     const double sc = kSB / ((r + 1.) * (r + 2.));
+     for (int e = 0; e < 8; ++e) {
+          const int j = 8 * jg + e;
+          double pp = 0., pm = 0.;
+          if (j < H && r < n1max) {
+               pp = Bp[(size_t) r * Hp + j] * sc;
+               pm = Bm[(size_t) r * Hp + j] * sc;
+          }
+          split16<E>(pp, &lv[0][e], &lv[1][e]);
+          split16<E>(pm, &lv[2][e], &lv[3][e]);
+          for (int q = 0; q < 8; ++q) {
+               const bool plus = q == 0 || q == 3 || q == 6 || q == 7;
+               split16<E>(wts[(size_t) q * Hq + j] * (plus ? pp : pm), &rv[2 * q][e], &rv[2 * q + 1][e]);
+          }
+     }
+     for (int q = 0; q < 4; ++q) {
+          v4u w;
+          for (int k = 0; k < 4; ++k) w[k] = (unsigned) lv[q][2 * k] | ((unsigned) lv[q][2 * k + 1] << 16);
+          *(v4u *) (L16 + q * plane + o) = w;
+     }
+     for (int q = 0; q < 16; ++q) {
+          v4u w;
+          for (int k = 0; k < 4; ++k) w[k] = (unsigned) rv[q][2 * k] | ((unsigned) rv[q][2 * k + 1] << 16);
+          *(v4u *) (R16 + q * plane + o) = w;
+     }
+}
+
+// G on the matrix cores: one workgroup per 64 x 64 tile of (n, m), contraction
+// over the positive nodes in stages of 32, two LDS stages as angular_mix2.
+// Accumulators (left x right): 0 P+ a P+, 1 P- a' P-, 2 P- c P-, 3 P+ c' P+,
+// 4 P+ br P-, 5 P- b'r P+, 6 P+ bi P-, 7 P- b'i P+.  Output in fp32,
+// Gf[4][Kq][Kq] = Gss, Gtt, Re Gst, Im Gst (row n, column m, scaled by
+// kSB^2 / (wmax n (n+1) m (m+1))), and the largest magnitude.
+template <class E>
+__global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, const unsigned short *R16, int Hq,
+                                                  int Kq, float *Gf, AdjScal *sc) {
+     constexpr int T = 64, G = KM / 8;
+     constexpr int NL = 4 * G * T / 256, NR = 16 * G * T / 256;
+     __shared__ v4u sL[2][4 * G][T];
+     __shared__ v4u sR[2][16 * G][T];
+     const int n0 = blockIdx.x * T, m0 = blockIdx.y * T, nch = Hq / KM;
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wa = wave & 1, wb = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+     const size_t plane = (size_t) (Hq / 8) * Kq * 8;
+     v4u rl[NL], rr[NR];
+     auto load = [&](int c) {
+#pragma unroll
+          for (int t = 0; t < NL; ++t) {
+               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
+               rl[t] = *(const v4u *) (L16 + pl * plane + ((size_t) (c * G + g) * Kq + n0 + col) * 8);
+          }
+#pragma unroll
+          for (int t = 0; t < NR; ++t) {
+               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
+               rr[t] = *(const v4u *) (R16 + pl * plane + ((size_t) (c * G + g) * Kq + m0 + col) * 8);
+          }
+     };
+     auto store = [&](int s) {
+#pragma unroll
+          for (int t = 0; t < NL; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sL[s][e / T][e % T] = rl[t];
+          }
+#pragma unroll
+          for (int t = 0; t < NR; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sR[s][e / T][e % T] = rr[t];
+          }
+     };
+     v4f acc[2][2][8];
+#pragma unroll
+     for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+               for (int k = 0; k < 8; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
+     load(0);
+     store(0);
+     for (int c = 0; c < nch; ++c) {
+          const int s = c & 1;
+          __syncthreads();
+          if (c + 1 < nch) load(c + 1);
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+               const int mc = wb * 32 + 16 * b + l16;
+               v4u y[16];
+#pragma unroll
+               for (int q = 0; q < 16; ++q) y[q] = sR[s][q * G + lg][mc];
+#pragma unroll
+               for (int a = 0; a < 2; ++a) {
+                    const int nc = wa * 32 + 16 * a + l16;
+                    v4u x[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) x[q] = sL[s][q * G + lg][nc];
+                    v4f *cc = acc[a][b];
+                    // left planes: 0 P+ hi, 1 P+ lo, 2 P- hi, 3 P- lo
+#define GB(k, L, R) cc[k] = E::mfma(x[L], y[2 * (R)], cc[k]); cc[k] = E::mfma(x[L], y[2 * (R) + 1], cc[k]); \
+                    cc[k] = E::mfma(x[(L) + 1], y[2 * (R)], cc[k]);
+                    GB(0, 0, 0) GB(1, 2, 1) GB(2, 2, 2) GB(3, 0, 3)
+                    GB(4, 0, 4) GB(5, 2, 6) GB(6, 0, 5) GB(7, 2, 7)
+#undef GB
+               }
+          }
+          if (c + 1 < nch) store(s ^ 1);
+     }
+     float mx = 0.f;
+#pragma unroll
+     for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+               for (int v = 0; v < 4; ++v) {
+                    const int r = n0 + wa * 32 + 16 * a + mfma16_row(lane, v);
+                    const int cm = m0 + wb * 32 + 16 * b + l16;
+                    const float ss = ((r + cm) & 1) ? -1.f : 1.f;
+                    const v4f *q = acc[a][b];
+                    const float g0 = q[0][v] + ss * q[1][v], g1 = q[2][v] + ss * q[3][v];
+                    const float g2 = q[4][v] + ss * q[5][v], g3 = q[6][v] + ss * q[7][v];
+                    const size_t o = (size_t) r * Kq + cm, pl = (size_t) Kq * Kq;
+                    Gf[o] = g0;
+                    Gf[pl + o] = g1;
+                    Gf[2 * pl + o] = g2;
+                    Gf[3 * pl + o] = g3;
+                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(g0), fabsf(g1)), fmaxf(fabsf(g2), fabsf(g3))));
+               }
+     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+     if (lane == 0) atomicMax(&sc->gmax, __float_as_uint(mx));
+}
+
+// Exponent k with |G| * 2^(14 - k) < 2^14 for every entry (0 for G = 0).
+__device__ __forceinline__ int adj_gexp(const AdjScal *sc) {
+     const float g = __uint_as_float(sc->gmax);
+     if (!(g > 0.f)) return 0;
+     int k;
+     (void) frexpf(g, &k);
+     return k;
+}
+
+// Split operand of U = G A: [12][Kq / 8][Kq][8], contraction index m in
+// groups of 8, column n: Gss, -Re Gst, Im Gst, Gtt, -Re Gts, Im Gts (high,
+// low) with Gts(n, m) = conj Gst(m, n), all times 2^(14 - k).  One thread per
+// (group of 8 m, n).
+template <class E>
+__global__ __launch_bounds__(64) void adj_gsplit(const float *Gf, const AdjScal *sc, int Kq, unsigned short *G16) {
+     const int n = blockIdx.x * 64 + threadIdx.x, mg = blockIdx.y;
+     if (n >= Kq) return;
+     const double s = ldexp(1., 14 - adj_gexp(sc));
+     const size_t pl = (size_t) Kq * Kq, plane = (size_t) (Kq / 8) * Kq * 8, o = ((size_t) mg * Kq + n) * 8;
+     unsigned short v[12][8];
+     for (int e = 0; e < 8; ++e) {
+          const int m = 8 * mg + e;
+          const size_t nm = (size_t) n * Kq + m, mn = (size_t) m * Kq + n;
+          const double val[6] = {Gf[nm], -(double) Gf[2 * pl + nm], Gf[3 * pl + nm], Gf[pl + nm],
+                                 -(double) Gf[2 * pl + mn], -(double) Gf[3 * pl + mn]};
+          for (int q = 0; q < 6; ++q) split16<E>(val[q] * s, &v[2 * q][e], &v[2 * q + 1][e]);
+     }
+     for (int q = 0; q < 12; ++q) {
+          v4u w;
+          for (int k = 0; k < 4; ++k) w[k] = (unsigned) v[q][2 * k] | ((unsigned) v[q][2 * k + 1] << 16);
+          *(v4u *) (G16 + q * plane + o) = w;
+     }
+}
+
+// U = G A per size: one workgroup per (size tile of 64, order tile of 64)
+// task, contraction over m < the tile's depth in stages of 32, two LDS stages.
+// Matrix-core rows are orders n, columns sizes.  Accumulators: U_S re, im,
+// U_T re, im; output Uf[4][Kq][Np] in fp32, unscaled to U = Sbar / f.
+struct UArgs {
+     int Np, Kb, Kq;
+     const unsigned short *A16;     // [8][Kb][Np][8]
+     const unsigned short *G16;     // [12][Kq / 8][Kq][8]
+     const int *ktile;              // per size tile of 64
+     const int2 *task;              // (size tile, order tile), heaviest first
+     const AdjScal *sc;
+     float *Uf;                     // [4][Kq][Np]
+};
+
+template <class E>
+__global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
+     constexpr int T = 64, G = KM / 8;
+     constexpr int NA = 8 * G * T / 256, NG = 12 * G * T / 256;
+     __shared__ v4u sA[2][8 * G][T];
+     __shared__ v4u sG[2][12 * G][T];
+     const int2 tk = p.task[blockIdx.x];
+     const int it0 = tk.x * T, n0 = tk.y * T, nch = p.ktile[tk.x] / KM;
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+     const size_t Np = p.Np, Kq = p.Kq;
+     const size_t pA = (size_t) p.Kb * Np * 8, pG = (Kq / 8) * Kq * 8;
+     const v4u neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+     v4u ra[NA], rg[NG];
+     auto load = [&](int c) {
+#pragma unroll
+          for (int t = 0; t < NA; ++t) {
+               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
+               ra[t] = *(const v4u *) (p.A16 + pl * pA + ((size_t) (c * G + g) * Np + it0 + col) * 8);
+          }
+#pragma unroll
+          for (int t = 0; t < NG; ++t) {
+               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
+               rg[t] = *(const v4u *) (p.G16 + pl * pG + ((size_t) (c * G + g) * Kq + n0 + col) * 8);
+          }
+     };
+     auto store = [&](int s) {
+#pragma unroll
+          for (int t = 0; t < NA; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sA[s][e / T][e % T] = ra[t];
+          }
+#pragma unroll
+          for (int t = 0; t < NG; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sG[s][e / T][e % T] = rg[t];
+          }
+     };
+     v4f acc[2][2][4];
+#pragma unroll
+     for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+               for (int k = 0; k < 4; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
+     if (nch > 0) {
+          load(0);
+          store(0);
+     }
+     for (int c = 0; c < nch; ++c) {
+          const int s = c & 1;
+          __syncthreads();
+          if (c + 1 < nch) load(c + 1);
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+               const int nc = wn * 32 + 16 * b + l16;
+               v4u g[12];
+#pragma unroll
+               for (int q = 0; q < 12; ++q) g[q] = sG[s][q * G + lg][nc];
+#pragma unroll
+               for (int a = 0; a < 2; ++a) {
+                    const int ic = wm * 32 + 16 * a + l16;
+                    v4u x[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) x[q] = sA[s][q * G + lg][ic];
+                    const v4u nx0 = x[0] ^ neg, nx1 = x[1] ^ neg, nx4 = x[4] ^ neg, nx5 = x[5] ^ neg;
+                    v4f *cc = acc[a][b];
+#define UG(k, XH, XL, GP) cc[k] = E::mfma(g[2 * (GP)], XH, cc[k]); cc[k] = E::mfma(g[2 * (GP) + 1], XH, cc[k]); \
+                    cc[k] = E::mfma(g[2 * (GP)], XL, cc[k]);
+                    // U_S re = Gss A+r - Re Gst A-r + Im Gst A-i
+                    UG(0, x[0], x[1], 0) UG(0, x[4], x[5], 1) UG(0, x[6], x[7], 2)
+                    // U_S im = Gss A+i - Re Gst A-i - Im Gst A-r
+                    UG(1, x[2], x[3], 0) UG(1, x[6], x[7], 1) UG(1, nx4, nx5, 2)
+                    // U_T re = Gtt A-r - Re Gts A+r + Im Gts A+i
+                    UG(2, x[4], x[5], 3) UG(2, x[0], x[1], 4) UG(2, x[2], x[3], 5)
+                    // U_T im = Gtt A-i - Re Gts A+i - Im Gts A+r
+                    UG(3, x[6], x[7], 3) UG(3, x[2], x[3], 4) UG(3, nx0, nx1, 5)
+#undef UG
+               }
+          }
+          if (c + 1 < nch) store(s ^ 1);
+     }
+     // U = acc n (n+1) wmax / (2^(14 - k) kSB^2 kSA)
+     const double base = p.sc->wmax * ldexp(1., adj_gexp(p.sc) - 14) / (kSB * kSB * kSA);
+#pragma unroll
+     for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+#pragma unroll
+               for (int v = 0; v < 4; ++v) {
+                    const int r = n0 + wn * 32 + 16 * b + mfma16_row(lane, v);
+                    const int i = it0 + wm * 32 + 16 * a + l16;
+                    const double f = base * (r + 1.) * (r + 2.);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) p.Uf[((size_t) k * Kq + r) * Np + i] = (float) (acc[a][b][k][v] * f);
+               }
+}
+
 // Per-size reverse sweep (adjoint of src/lmie_core.c:298-611): cotangents of
 // a_n, b_n from the angular part (Sbar, Tbar) and from the series, through
 // a_n(u), b_n(v) to D_n and m, and through the recomputed downward recurrence
@@ -1219,7 +1756,9 @@ struct RevArgs {
      const double *AB;          // [4][Kp][Np]
      const double *CB;          // [2][Np]
      const double *S;           // series [4][Np]
-     const double *bar;         // [4][Kp][Np]
+     const double *bar;         // [4][Kp][Np] (FP64 path), or
+     const float *Uf;           // [4][Kq][Np] U = Sbar / f (mixed path; fang from U)
+     int Kq;
      const double *fang;        // [Np]
      double sb0, sb1, sb2, sb3; // cotangents of the raw sums
      double *scrD;              // [3][n2max][Np] recomputed D_j (re, im) and ratios r_j
@@ -1240,8 +1779,9 @@ __global__ __launch_bounds__(64) void mie_reverse(RevArgs p) {
      const double x = p.x[i], f = p.f[i];
      const int n1 = p.n1[i], n2 = p.n2[i];
      const cd m = cmk(p.mr, -p.mi), z = cscale(m, x), m2 = cmul(m, m);
-     // size-weight cotangent
-     p.fbar[i] = p.fang[i] + p.sb0 * p.S[i] + p.sb1 * p.S[Np + i] + p.sb2 * p.S[2 * Np + i] + p.sb3 * p.S[3 * Np + i];
+     // size-weight cotangent: series part (angular part below when it comes from U)
+     const double fser = p.sb0 * p.S[i] + p.sb1 * p.S[Np + i] + p.sb2 * p.S[2 * Np + i] + p.sb3 * p.S[3 * Np + i];
+     double fq = 0.;
      // recompute and keep D_j, j = 1 .. n2 - 1 (D_{n2} = 0)
      double *Dre = p.scrD, *Dim = p.scrD + drow;
      {
@@ -1281,8 +1821,18 @@ __global__ __launch_bounds__(64) void mie_reverse(RevArgs p) {
                const double jx = j / x;
                const cd a = cmk(AB(0, j), AB(1, j)), b = cmk(AB(2, j), AB(3, j));
                const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
-               const cd Sb = cmk(p.bar[(size_t) (j - 1) * Np + i], p.bar[plane + (size_t) (j - 1) * Np + i]);
-               const cd Tb = cmk(p.bar[2 * plane + (size_t) (j - 1) * Np + i], p.bar[3 * plane + (size_t) (j - 1) * Np + i]);
+               cd Sb, Tb;
+               if (p.Uf) {
+                    const size_t uq = (size_t) p.Kq * Np, o = (size_t) (j - 1) * Np + i;
+                    const cd us = cmk(p.Uf[o], p.Uf[uq + o]), ut = cmk(p.Uf[2 * uq + o], p.Uf[3 * uq + o]);
+                    // 1/2 Re(conj S U_S + conj T U_T), S = fv4 (a + b), T = fv4 (a - b)
+                    fq += .5 * fv4 * ((a.r + b.r) * us.r + (a.i + b.i) * us.i + (a.r - b.r) * ut.r + (a.i - b.i) * ut.i);
+                    Sb = cscale(us, f);
+                    Tb = cscale(ut, f);
+               } else {
+                    Sb = cmk(p.bar[(size_t) (j - 1) * Np + i], p.bar[plane + (size_t) (j - 1) * Np + i]);
+                    Tb = cmk(p.bar[2 * plane + (size_t) (j - 1) * Np + i], p.bar[3 * plane + (size_t) (j - 1) * Np + i]);
+               }
                // angular part
                cd ab = cscale(cadd(Sb, Tb), fv4), bb = cscale(csub(Sb, Tb), fv4);
                // series parts: ext, sca, bak, asym
@@ -1332,6 +1882,7 @@ __global__ __launch_bounds__(64) void mie_reverse(RevArgs p) {
      mb = cadd(mb, cscale(zb, x));
      p.mbar[i] = mb.r;
      p.mbar[Np + i] = mb.i;
+     p.fbar[i] = (p.Uf ? fq : p.fang[i]) + fser;
 }
 
 // Sum of the per-block partials: one thread per (pass channel, element, angle);
@@ -1537,12 +2088,15 @@ struct Workspace {
      hipStream_t stream = nullptr;
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
      Buf x, sx, cx, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
-     Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2;
+     Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2, order;
      std::vector<double> hsc;       // host staging of sin x, cos x
      // reverse-pass checkpoint and buffers
      Buf AB, CB, sst, pfbar, bar, fang, mbar, fbar, gbar, scrRev, ones;
+     Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
+     int mixed = 0, bf16 = 0, Kb = 0;
+     std::vector<int> kt64;         // contraction depth per size tile of 64
      double mr = 0., mi = 0.;
 };
 
@@ -1668,6 +2222,9 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      const int bf16 = J->precision == LMIE_PRECISION_MIXED_BF16;
      const int kgran = mixed ? KM : KC;
      const int Kb = rup(n1max, KM) / 8;
+     // angle padding of the split tables: tiles of 128 angles (64 when the
+     // checkpoint for the reverse pass is kept, whose rows are Hp long)
+     const int mixWN = (J->keep && !mixed) ? 2 : 4, Hb = rup(H > 0 ? H : 1, 32 * mixWN);
 
      // host-side tables: per-tile contraction depths
      const int nt32 = Np / 32, nt64 = Np / 64;
@@ -1744,12 +2301,12 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           hipLaunchKernelGGL(gauss_nodes, dim3((H + 63) / 64), dim3(64), 0, s, Q, (const double *) w->guess.as<double>(),
                              w->mu.as<double>(), w->w.as<double>());
           if (mixed) {
-               if (w->B16.get(sizeof(unsigned short) * 4 * (size_t) Kb * Hp * 8)) return -1;
+               if (w->B16.get(sizeof(unsigned short) * 4 * (size_t) Kb * Hb * 8)) return -1;
                if (bf16)
-                    hipLaunchKernelGGL(pitau_table16<BF16>, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kb,
+                    hipLaunchKernelGGL(pitau_table16<BF16>, dim3(Hb / 64), dim3(64), 0, s, H, Hb, n1max, Kb,
                                        (const double *) w->mu.as<double>(), w->B16.as<unsigned short>());
                else
-                    hipLaunchKernelGGL(pitau_table16<F16>, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kb,
+                    hipLaunchKernelGGL(pitau_table16<F16>, dim3(Hb / 64), dim3(64), 0, s, H, Hb, n1max, Kb,
                                        (const double *) w->mu.as<double>(), w->B16.as<unsigned short>());
           }
           if (!mixed || J->keep) {
@@ -1778,7 +2335,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      if (keep) {
           if (nt || J->save_control == 1) return set_err("the reverse-pass checkpoint covers the value channel only");
           if (w->AB.get(sizeof(double) * 4 * plane) || w->CB.get(sizeof(double) * 2 * Np) ||
-              (Q > 0 && w->sst.get(sizeof(double) * 8 * (size_t) Np * Hp)))
+              (Q > 0 && !mixed && w->sst.get(sizeof(double) * 8 * (size_t) Np * Hp)))
                return -1;
      }
      tm.mark(LMIE_T_COEFS, 0);
@@ -1866,33 +2423,44 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                plan[0].ntp = 0;
                plan[0].nwp = 1 + nw;
                plan[0].t0 = plan[0].w0 = 0;
-               plan[0].nblk = ((Np / 64) + 3) / 4;
+               plan[0].nblk = nt64;
                plan[0].off = 0;
                tot = (size_t) plan[0].nblk * plan[0].nwp * 4 * Q;
           }
           if (w->part.get(sizeof(double) * tot)) return -1;
           tm.mark(LMIE_T_ANGULAR, 0);
           if (mixed) {
-               MixArgs m;
+               // size tiles by decreasing contraction depth (ties by index)
+               std::vector<int> ord(nt64);
+               for (int t = 0; t < nt64; ++t) ord[t] = t;
+               std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return kt64[a] > kt64[b]; });
+               if (w->order.get(sizeof(int) * nt64)) return -1;
+               HIPCHECK(hipMemcpyAsync(w->order.p, ord.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
+               Mix2Args m;
                memset(&m, 0, sizeof m);
-               m.Np = Np; m.Kb = Kb; m.Hp = Hp; m.H = H; m.Q = Q; m.N = N;
+               m.Np = Np; m.Kb = Kb; m.Hb = Hb; m.H = H; m.Q = Q;
+               m.nT = nt64; m.nJ = Hb / (32 * mixWN);
                m.A16 = w->A16.as<unsigned short>();
                m.B16 = w->B16.as<unsigned short>();
                m.ktile = w->ktile64.as<int>();
+               m.order = w->order.as<int>();
                m.W[0] = w->f.as<double>();
                for (int q = 0; q < nw; ++q) m.W[1 + q] = w->dfw.as<double>() + (size_t) q * Np;
-               m.tiles_per_block = 4;
                m.part = w->part.as<double>();
-               m.sst = keep ? w->sst.as<double>() : nullptr;
-               dim3 grid(Hp / 64, plan[0].nblk);
-#define LMIE_MIX(E, NWV) hipLaunchKernelGGL((angular_mix<E, 2, 2, NWV>), grid, dim3(256), 0, s, m)
+               m.sst = nullptr;
+               const dim3 grid(8 * ((nt64 + 7) / 8) * m.nJ);
+#define LMIE_MIX(E, WNV, NWV) hipLaunchKernelGGL((angular_mix2<E, WNV, NWV>), grid, dim3(256), 0, s, m)
+#define LMIE_MIX_NW(E, WNV)                                                                \
+               do {                                                                        \
+                    if (nw == 0) LMIE_MIX(E, WNV, 1); else if (nw == 1) LMIE_MIX(E, WNV, 2);  \
+                    else if (nw == 2) LMIE_MIX(E, WNV, 3); else LMIE_MIX(E, WNV, 4);          \
+               } while (0)
                if (bf16) {
-                    if (nw == 0) LMIE_MIX(BF16, 1); else if (nw == 1) LMIE_MIX(BF16, 2);
-                    else if (nw == 2) LMIE_MIX(BF16, 3); else LMIE_MIX(BF16, 4);
+                    if (mixWN == 4) LMIE_MIX_NW(BF16, 4); else LMIE_MIX_NW(BF16, 2);
                } else {
-                    if (nw == 0) LMIE_MIX(F16, 1); else if (nw == 1) LMIE_MIX(F16, 2);
-                    else if (nw == 2) LMIE_MIX(F16, 3); else LMIE_MIX(F16, 4);
+                    if (mixWN == 4) LMIE_MIX_NW(F16, 4); else LMIE_MIX_NW(F16, 2);
                }
+#undef LMIE_MIX_NW
 #undef LMIE_MIX
                HIPCHECK(hipGetLastError());
           }
@@ -1980,6 +2548,58 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      w->n2max = 0;
      for (int i = 0; i < N; ++i) w->n2max = J->n2[i] > w->n2max ? J->n2[i] : w->n2max;
      w->calc_gc = J->calc_gc; w->calc_lc = J->calc_lc; w->mr = J->mr; w->mi = J->mi;
+     w->mixed = mixed; w->bf16 = bf16; w->Kb = Kb; w->kt64 = kt64;
+     return 0;
+}
+
+// Mixed reverse contraction: weights, split tables, G, its split operand and
+// U = G A (Uf, [4][Kq][Np]).
+int adjoint_mixed_angular(Workspace *w, hipStream_t s) {
+     const int Np = w->Np, Hp = w->Hp, H = w->H, Q = w->Q, n1max = w->n1max;
+     const int Kq = rup(n1max, 64), Hq = rup(H, KM), nt64 = Np / 64;
+     if (rup(n1max, KM) > w->Kb * 8) return set_err("internal: split operand depth");
+     if (w->wts.get(sizeof(double) * 8 * (size_t) Hq) || w->scal.get(sizeof(AdjScal)) ||
+         w->L16.get(sizeof(unsigned short) * 4 * (size_t) Hq * Kq) ||
+         w->R16.get(sizeof(unsigned short) * 16 * (size_t) Hq * Kq) ||
+         w->Gf.get(sizeof(float) * 4 * (size_t) Kq * Kq) || w->G16.get(sizeof(unsigned short) * 12 * (size_t) Kq * Kq) ||
+         w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np))
+          return -1;
+     // (size tile, order tile) tasks, deepest size tiles first
+     std::vector<int> ord(nt64);
+     for (int t = 0; t < nt64; ++t) ord[t] = t;
+     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return w->kt64[a] > w->kt64[b]; });
+     std::vector<int2> task;
+     for (int t : ord)
+          for (int n0 = 0; n0 < w->kt64[t]; n0 += 64) task.push_back(make_int2(t, n0 / 64));
+     if (task.empty()) {
+          HIPCHECK(hipMemsetAsync(w->Uf.p, 0, sizeof(float) * 4 * (size_t) Kq * Np, s));
+          return 0;
+     }
+     if (w->tasks.get(sizeof(int2) * task.size())) return -1;
+     HIPCHECK(hipMemcpyAsync(w->tasks.p, task.data(), sizeof(int2) * task.size(), hipMemcpyHostToDevice, s));
+     AdjScal *sc = w->scal.as<AdjScal>();
+     hipLaunchKernelGGL(adj_weights, dim3(1), dim3(1024), 0, s, (const double *) w->pfbar.as<double>(), H, Hq, Q,
+                        w->wts.as<double>(), sc);
+     UArgs u;
+     u.Np = Np; u.Kb = w->Kb; u.Kq = Kq;
+     u.A16 = w->A16.as<unsigned short>(); u.G16 = w->G16.as<unsigned short>();
+     u.ktile = w->ktile64.as<int>(); u.task = w->tasks.as<int2>(); u.sc = sc; u.Uf = w->Uf.as<float>();
+#define LMIE_ADJ(E)                                                                                            \
+     do {                                                                                                     \
+          hipLaunchKernelGGL(adj_tables<E>, dim3(Kq / 64, Hq / 8), dim3(64), 0, s, (const double *) w->Bp.as<double>(), \
+                             (const double *) w->Bm.as<double>(), (const double *) w->wts.as<double>(), H, Hp, Hq, Kq, \
+                             n1max, w->L16.as<unsigned short>(), w->R16.as<unsigned short>());                      \
+          hipLaunchKernelGGL(adj_gbuild<E>, dim3(Kq / 64, Kq / 64), dim3(256), 0, s,                            \
+                             (const unsigned short *) w->L16.as<unsigned short>(),                             \
+                             (const unsigned short *) w->R16.as<unsigned short>(), Hq, Kq, w->Gf.as<float>(), sc); \
+          hipLaunchKernelGGL(adj_gsplit<E>, dim3(Kq / 64, Kq / 8), dim3(64), 0, s, (const float *) w->Gf.as<float>(), \
+                             (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
+          hipLaunchKernelGGL(adj_ugemm<E>, dim3((unsigned) task.size()), dim3(256), 0, s, u);                   \
+     } while (0)
+     if (w->bf16) LMIE_ADJ(BF16);
+     else LMIE_ADJ(F16);
+#undef LMIE_ADJ
+     HIPCHECK(hipGetLastError());
      return 0;
 }
 
@@ -2005,6 +2625,11 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           hipLaunchKernelGGL(expand_adjoint, dim3((H + 63) / 64), dim3(64), 0, s, (const double *) w->T.as<double>(),
                              (const double *) w->w.as<double>(), use_gc ? (const double *) g : nullptr,
                              use_lc ? (const double *) (g + 6 * C) : nullptr, H, Hp, C, Q, w->pfbar.as<double>());
+     }
+     if (w->mixed && Q > 0 && (use_gc || use_lc)) {
+          if (adjoint_mixed_angular(w, s)) return -1;
+          tm.mark(LMIE_T_ADJ_ANGULAR, 1);
+     } else if (Q > 0 && (use_gc || use_lc)) {
           hipLaunchKernelGGL(fbar_angular, dim3((N + 3) / 4), dim3(256), 0, s, (const double *) w->sst.as<double>(),
                              (const double *) w->pfbar.as<double>(), N, H, Hp, Q, w->fang.as<double>());
           AdjArgs aa;
@@ -2016,14 +2641,18 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           HIPCHECK(hipGetLastError());
           tm.mark(LMIE_T_ADJ_ANGULAR, 1);
      } else {
+          tm.mark(LMIE_T_ADJ_ANGULAR, 0);
           HIPCHECK(hipMemsetAsync(w->bar.p, 0, sizeof(double) * 4 * (size_t) Kp * Np, s));
+          tm.mark(LMIE_T_ADJ_ANGULAR, 1);
      }
+     const int use_u = w->mixed && Q > 0 && (use_gc || use_lc);
      tm.mark(LMIE_T_ADJ_COEFS, 0);
      RevArgs r;
      r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max;
      r.x = w->x.as<double>(); r.sx = w->sx.as<double>(); r.cx = w->cx.as<double>(); r.f = w->f.as<double>();
      r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
      r.AB = w->AB.as<double>(); r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
+     r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
      r.fang = w->fang.as<double>();
      r.sb0 = in->sums_bar[0]; r.sb1 = in->sums_bar[1]; r.sb2 = in->sums_bar[2]; r.sb3 = in->sums_bar[3];
      r.scrD = w->scrRev.as<double>(); r.n2max = w->n2max;

@@ -111,6 +111,25 @@ def test_adjoint_against_central_differences(product, part):
     assert math.isnan(g["lambda"]) and math.isnan(g["r1"]) and math.isnan(g["r2"])
 
 
+@pytest.mark.parametrize("part", ["cext", "csca", "cbak", "g", "gc", "lc"])
+def test_mixed_adjoint_matches_fp64_adjoint(product, part):
+    """The mixed reverse pass (split 16-bit kernel matrix) against the FP64
+    reverse pass (stored sums, FP64 transposed contraction), per cotangent."""
+    say("case: mixed vs fp64 adjoint (log-normal, 512 sizes, x_max 62), cotangent on", part)
+    p = FD_CASE
+    rng = np.random.Generator(np.random.PCG64(12))
+    base = lmie.solve(product, p, None, "fp64")
+    cot = {"cext": 0.0, "csca": 0.0, "cbak": 0.0, "g": 0.0}
+    if part in ("gc", "lc"):
+        cot[part] = rng.standard_normal(base[part].shape) / np.abs(base[part]).max(axis=1, keepdims=True)
+    else:
+        cot[part] = 1.0
+    _, g64 = lmie.adjoint(product, [p], None, lambda o: cot, "fp64")
+    _, gmx = lmie.adjoint(product, [p], None, lambda o: cot, "mixed")
+    keys = ("mr", "mi", "a1", "a2")
+    check([gmx[k] for k in keys], [g64[k] for k in keys], TOL["mixed"], "mixed " + part)
+
+
 def test_autograd_wrapper(product):
     torch = pytest.importorskip("torch")
     from lmie_amd import autograd
