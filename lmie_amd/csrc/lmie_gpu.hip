@@ -325,39 +325,23 @@ struct CoefArgs {
      double *S;                 // series [4][Np]: ext, sca, |c_bak|^2, asym
      double *St[4];             // tangent series of this pass, each [4][Np]
      int write_base;            // write A and S of the base channel
-     unsigned short *A16;       // mixed path: [8 planes][Kb][Np][8] split operand
-     int Kb;                    // groups of 8 orders in A16
      double *AB;                // reverse-pass checkpoint: a, b [4 planes re/im][Kp][Np] or NULL
      double *CB;                // reverse-pass checkpoint: c_bak series [2][Np] or NULL
 };
 
-// E = void: FP64 operand planes; E = F16 / BF16: split 16-bit operand, blocked
-// by 8 orders ([8 planes: S re hi, lo, S im hi, lo, T re hi, lo, T im hi, lo]
-// [Kb][Np][8], value (2n+1)(a_n +- b_n) * kSA), so every lane stores 16
-// contiguous bytes per plane and the contraction loads its operand fragments
-// without a transpose.
-template <int NT, class E = void>
+// Used for passes that carry tangent channels (NT >= 1); the value channel
+// alone goes through mie_chains / mie_terms / mie_sums.
+template <int NT>
 __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
      // separately rounded operations in the reference's order and association
      // (src/lmie_core.c:298-611): the real recurrences reproduce the
      // reference's bits; complex divisions use Smith's scaling (ulp-level
      // differences that the downward recurrences damp)
 #pragma clang fp contract(off)
-     constexpr bool MIX = !std::is_void<E>::value;
      int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= p.Np) return;
      const size_t Np = p.Np, plane = (size_t) p.Kp * Np, srow = (size_t) (p.n1max + 1) * Np;
      int kf = p.kfill[i / TILE];
-     const size_t p16 = (size_t) p.Kb * Np * 8;
-     if constexpr (MIX) {
-          if (i >= p.N) {
-               const v4u z = {0u, 0u, 0u, 0u};
-               for (int g = 0; g < kf / 8; ++g)
-                    for (int q = 0; q < 8; ++q) *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = z;
-               for (int q = 0; q < 4; ++q) p.S[q * Np + i] = 0.;
-               return;
-          }
-     }
      if (i >= p.N) {
           // padded columns: zero operand rows and series
           for (int r = 0; r < kf; ++r)
@@ -452,7 +436,6 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           dcb[t] = dap[t] = dbp[t] = cmk(0., 0.);
      }
      double sgn = -1.;
-     unsigned short buf[8][8];      // mixed path: 8 planes x 8 orders of the current group
      for (int j = 1; j <= n1; ++j) {
           const double r = R[j * Np + i];
           const double psi = r * psi0;
@@ -474,24 +457,7 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           cb = cadd(cb, cscale(csub(a, b), fv1 * sgn));
           if (j > 1)
                sg += fv3m * ((ap.r * a.r + ap.i * a.i) + (bp.r * b.r + bp.i * b.i)) + fv4m * (ap.r * bp.r + ap.i * bp.i);
-          if constexpr (MIX) {
-               const int e = (j - 1) & 7;
-               const double sc = fv1 * kSA;
-               split16<E>(sc * (a.r + b.r), &buf[0][e], &buf[1][e]);
-               split16<E>(sc * (a.i + b.i), &buf[2][e], &buf[3][e]);
-               split16<E>(sc * (a.r - b.r), &buf[4][e], &buf[5][e]);
-               split16<E>(sc * (a.i - b.i), &buf[6][e], &buf[7][e]);
-               if (e == 7 || j == n1) {
-                    for (int e2 = e + 1; e2 < 8; ++e2)
-                         for (int q = 0; q < 8; ++q) buf[q][e2] = 0;
-                    const int g = (j - 1) >> 3;
-                    for (int q = 0; q < 8; ++q) {
-                         v4u w;
-                         for (int k = 0; k < 4; ++k) w[k] = (unsigned) buf[q][2 * k] | ((unsigned) buf[q][2 * k + 1] << 16);
-                         *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = w;
-                    }
-               }
-          } else if (p.write_base) {
+          if (p.write_base) {
                const size_t o = (size_t) (j - 1) * Np + i;
                p.A[o] = fv4 * (a.r + b.r);
                p.A[plane + o] = fv4 * (a.i + b.i);
@@ -552,12 +518,7 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           psi0 = psi;
      }
      // zero rows n1 .. kf-1 so the contraction can run whole chunks
-     if constexpr (MIX) {
-          const v4u z = {0u, 0u, 0u, 0u};
-          for (int g = (n1 + 7) / 8; g < kf / 8; ++g)
-               for (int q = 0; q < 8; ++q) *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = z;
-     }
-     for (int r = n1; r < kf && !MIX; ++r) {
+     for (int r = n1; r < kf; ++r) {
           const size_t o = (size_t) r * Np + i;
           for (int c = 0; c < 4; ++c) {
                if (p.write_base) p.A[c * plane + o] = 0.;
@@ -581,6 +542,206 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
           p.St[t][Np + i] = dss[t];
           p.St[t][2 * Np + i] = 2. * (cb.r * dcb[t].r + cb.i * dcb[t].i);
           p.St[t][3 * Np + i] = dsg[t];
+     }
+}
+
+// ---------------------------------------------------------------------------
+// Value channel in three kernels (the same operations as mie_coefs<0>, so the
+// same bits): mie_chains runs the three recurrences that are serial in n, one
+// thread per size; mie_terms forms a_n, b_n, the contraction operand and the
+// partial series sums in parallel, one thread per (size, group of 8 orders);
+// mie_sums adds the partial sums per size.  The serial part is a few
+// operations per order, the parallel part fills the GPU.
+struct ChainArgs {
+     int N, Np, n1max, n2rows;
+     const double *x, *sx, *cx;
+     const int *n1, *n2;
+     double mr, mi;
+     int d_all;                 // store D_j up to n2 - 1 (reverse pass) instead of n1
+     double *D;                 // [2][n2rows][Np]: D_j re, im at row j
+     double *PC;                // [2][n1max + 1][Np]: psi_j, chi_j at row j
+};
+
+__global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
+#pragma clang fp contract(off)
+     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+     if (i >= p.N) return;
+     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
+     const double x = p.x[i];
+     const int n1 = p.n1[i], n2 = p.n2[i], dtop = p.d_all ? n2 - 1 : n1;
+     const cd z = cscale(cmk(p.mr, -p.mi), x);
+     double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
+     // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0
+     cd D = cmk(0., 0.);
+     for (int j = n2 - 1; j >= 1; --j) {
+          const cd c1 = cdiv(cmk(j + 1., 0.), z);
+          const cd c2 = cadd(D, c1);
+          D = csub(c1, crecip(c2));
+          if (j <= dtop) {
+               Dre[j * Np + i] = D.r;
+               Dim[j * Np + i] = D.i;
+          }
+     }
+     // downward ratio r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows
+     {
+          double r = 0.;
+          Psi[n1 * Np + i] = 0.;
+          for (int j = n1 - 1; j >= 1; --j) {
+               const double c = (2. * j + 1.) / x;
+               r = 1. / (c - r);
+               Psi[j * Np + i] = r;
+          }
+     }
+     // upward psi_j = r_j psi_{j-1}, chi_{j+1} = (2j+1)/x chi_j - chi_{j-1}
+     const double sx = p.sx[i], cx = p.cx[i], ax = 1. / x;
+     double psi0 = sx, chi0 = cx, chi1 = ax * cx + sx;
+     Psi[i] = psi0;
+     Chi[i] = chi0;
+     for (int j = 1; j <= n1; ++j) {
+          const double psi = Psi[j * Np + i] * psi0;
+          Psi[j * Np + i] = psi;
+          Chi[j * Np + i] = chi1;
+          const double fv1 = 2. * j + 1.;
+          const double chi2 = fv1 / x * chi1 - chi0;
+          chi0 = chi1;
+          chi1 = chi2;
+          psi0 = psi;
+     }
+}
+
+// a_j, b_j (src/lmie_core.c:557-583), operations as mie_coefs
+__device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd Dj, double psi, double psi0, double chi1,
+                                          double chi0, cd *a, cd *b) {
+#pragma clang fp contract(off)
+     const double jx = j / x;
+     const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
+     const cd u = cadd(cdiv(Dj, m), cmk(jx, 0.));
+     const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
+     const cd na = csub(cscale(u, psi), cmk(psi0, 0.)), da = csub(cmul(u, zj), zjm);
+     const cd nb = csub(cscale(v, psi), cmk(psi0, 0.)), db = csub(cmul(v, zj), zjm);
+     *a = cdiv(na, da);
+     *b = cdiv(nb, db);
+}
+
+struct TermArgs {
+     int N, Np, n1max, n2rows, Kp, Kb, ng;
+     const double *x;
+     const int *n1;
+     const int *kfill;          // [Np / TILE]
+     double mr, mi;
+     const double *D, *PC;      // from mie_chains
+     double *A;                 // FP64 planes [4][Kp][Np] (E = void)
+     unsigned short *A16;       // split planes [8][Kb][Np][8] (E = F16 / BF16)
+     double *AB;                // a, b [4][Kp][Np] or NULL
+     double *part;              // [5][ng][Np]: se, ss, cb re, cb im, sg of each group
+};
+
+template <class E>
+__global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
+#pragma clang fp contract(off)
+     constexpr bool MIX = !std::is_void<E>::value;
+     const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+     if (i >= p.Np || g >= p.ng) return;
+     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
+     const size_t plane = (size_t) p.Kp * Np, p16 = (size_t) p.Kb * Np * 8, gp = (size_t) p.ng * Np;
+     const int kf = p.kfill[i / TILE];
+     const int n1 = i < p.N ? p.n1[i] : 0;
+     const int j0 = 8 * g + 1, j1 = min(8 * g + 8, n1);
+     double se = 0., ss = 0., sg = 0.;
+     cd cb = cmk(0., 0.);
+     unsigned short buf[8][8];
+     if (j0 <= j1) {
+          const double x = p.x[i];
+          const cd m = cmk(p.mr, -p.mi);
+          const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
+          cd ap = cmk(0., 0.), bp = cmk(0., 0.);
+          if (j0 > 1)
+               mie_ab_at(j0 - 1, x, m, cmk(Dre[(j0 - 1) * Np + i], Dim[(j0 - 1) * Np + i]), Psi[(j0 - 1) * Np + i],
+                         Psi[(j0 - 2) * Np + i], Chi[(j0 - 1) * Np + i], Chi[(j0 - 2) * Np + i], &ap, &bp);
+          double sgn = (j0 & 1) ? -1. : 1.;
+          for (int j = j0; j <= j1; ++j) {
+               cd a, b;
+               mie_ab_at(j, x, m, cmk(Dre[j * Np + i], Dim[j * Np + i]), Psi[j * Np + i], Psi[(j - 1) * Np + i],
+                         Chi[j * Np + i], Chi[(j - 1) * Np + i], &a, &b);
+               const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
+               const double fv3m = j > 1 ? (j + 1.) / ((double) j / (j - 1.)) : 0.;
+               const double fv4m = j > 1 ? (2. * j - 1.) / ((j - 1.) * j) : 0.;
+               se += fv1 * (a.r + b.r);
+               ss += fv1 * (cabs2(a) + cabs2(b));
+               cb = cadd(cb, cscale(csub(a, b), fv1 * sgn));
+               if (j > 1)
+                    sg += fv3m * ((ap.r * a.r + ap.i * a.i) + (bp.r * b.r + bp.i * b.i)) + fv4m * (ap.r * bp.r + ap.i * bp.i);
+               const size_t o = (size_t) (j - 1) * Np + i;
+               if constexpr (MIX) {
+                    const int e = j - j0;
+                    const double sc = fv1 * kSA;
+                    split16<E>(sc * (a.r + b.r), &buf[0][e], &buf[1][e]);
+                    split16<E>(sc * (a.i + b.i), &buf[2][e], &buf[3][e]);
+                    split16<E>(sc * (a.r - b.r), &buf[4][e], &buf[5][e]);
+                    split16<E>(sc * (a.i - b.i), &buf[6][e], &buf[7][e]);
+               } else {
+                    p.A[o] = fv4 * (a.r + b.r);
+                    p.A[plane + o] = fv4 * (a.i + b.i);
+                    p.A[2 * plane + o] = fv4 * (a.r - b.r);
+                    p.A[3 * plane + o] = fv4 * (a.i - b.i);
+               }
+               if (p.AB) {
+                    p.AB[o] = a.r;
+                    p.AB[plane + o] = a.i;
+                    p.AB[2 * plane + o] = b.r;
+                    p.AB[3 * plane + o] = b.i;
+               }
+               ap = a;
+               bp = b;
+               sgn = -sgn;
+          }
+     }
+     // operand rows of orders beyond n1 (up to the tile's depth) are zero
+     if constexpr (MIX) {
+          if (8 * g < kf) {
+               for (int e = (j0 <= j1 ? j1 - j0 + 1 : 0); e < 8; ++e)
+                    for (int q = 0; q < 8; ++q) buf[q][e] = 0;
+               for (int q = 0; q < 8; ++q) {
+                    v4u w;
+                    for (int k = 0; k < 4; ++k) w[k] = (unsigned) buf[q][2 * k] | ((unsigned) buf[q][2 * k + 1] << 16);
+                    *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = w;
+               }
+          }
+     } else {
+          for (int r = max(8 * g, n1); r < min(8 * g + 8, kf); ++r)
+               for (int c = 0; c < 4; ++c) p.A[c * plane + (size_t) r * Np + i] = 0.;
+     }
+     double *pt = p.part + (size_t) g * Np + i;
+     pt[0] = se;
+     pt[gp] = ss;
+     pt[2 * gp] = cb.r;
+     pt[3 * gp] = cb.i;
+     pt[4 * gp] = sg;
+}
+
+// Series per size from the group partials (src/lmie_core.c:586-596).
+__global__ __launch_bounds__(256) void mie_sums(const double *part, int N, int Np, int ng, const int *n1, double *S,
+                                                double *CB) {
+     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+     if (i >= Np) return;
+     const size_t gp = (size_t) ng * Np;
+     double se = 0., ss = 0., sg = 0.;
+     cd cb = cmk(0., 0.);
+     const int gn = i < N ? (n1[i] + 7) / 8 : 0;
+     for (int g = 0; g < gn; ++g) {
+          const double *pt = part + (size_t) g * Np + i;
+          se += pt[0];
+          ss += pt[gp];
+          cb = cadd(cb, cmk(pt[2 * gp], pt[3 * gp]));
+          sg += pt[4 * gp];
+     }
+     S[i] = se;
+     S[Np + i] = ss;
+     S[2 * Np + i] = cabs2(cb);
+     S[3 * Np + i] = sg;
+     if (CB) {
+          CB[i] = cb.r;
+          CB[Np + i] = cb.i;
      }
 }
 
@@ -1225,16 +1386,20 @@ __global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
 //
 // Transpose of the expansion (misc/rtutil_scat_int.c:63-230 read backwards):
 // from the cotangents of the raw coefficients to the cotangents of the raw
-// matrix elements F11, F33, F12, F34 at the Gauss angles.  One thread per
-// positive node j, looping over l.
-__global__ void expand_adjoint(const double *T, const double *w, const double *gbar, const double *lbar,
-                               int H, int Hp, int C, int Q, double *pfbar) {
-     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-     if (j >= H) return;
+// matrix elements F11, F33, F12, F34 at the Gauss angles.  Two kernels: the
+// sums over l in chunks of EXA_L (one thread per positive node j and chunk,
+// partials [chunk][24][Hp]), then the fixed-order sum over chunks.
+constexpr int EXA_L = 128;
+
+__global__ __launch_bounds__(64) void expand_adjoint_part(const double *T, const double *gbar, const double *lbar,
+                                                          int H, int Hp, int C, double *part) {
+     const int j = blockIdx.x * blockDim.x + threadIdx.x, ck = blockIdx.y;
+     if (j >= Hp) return;
      const size_t plane = (size_t) C * Hp;
      double Em[6] = {0, 0, 0, 0, 0, 0}, Ep[6] = {0, 0, 0, 0, 0, 0};   // GSF rows
      double Lm[6] = {0, 0, 0, 0, 0, 0}, Lp[6] = {0, 0, 0, 0, 0, 0};   // Legendre rows
-     for (int l = 0; l < C; ++l) {
+     const int l1 = min(C, (ck + 1) * EXA_L);
+     for (int l = ck * EXA_L; l < l1 && j < H; ++l) {
           const double sl = (l & 1) ? -1. : 1., fl = (2 * l + 1) / 2.;
           const size_t o = (size_t) l * Hp + j;
           if (gbar) {
@@ -1258,10 +1423,26 @@ __global__ void expand_adjoint(const double *T, const double *w, const double *g
                }
           }
      }
+     double *out = part + (size_t) ck * 24 * Hp + j;
+     for (int k = 0; k < 6; ++k) {
+          out[(size_t) k * Hp] = Em[k];
+          out[(size_t) (6 + k) * Hp] = Ep[k];
+          out[(size_t) (12 + k) * Hp] = Lm[k];
+          out[(size_t) (18 + k) * Hp] = Lp[k];
+     }
+}
+
+__global__ void expand_adjoint_fin(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) {
+     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+     if (j >= H) return;
+     double v[24];
+     for (int k = 0; k < 24; ++k) v[k] = 0.;
+     for (int ck = 0; ck < nck; ++ck)
+          for (int k = 0; k < 24; ++k) v[k] += part[((size_t) ck * 24 + k) * Hp + j];
      const int ip = H + j, im = H - 1 - j;
      const double wq = w[ip];
      for (int s = 0; s < 2; ++s) {
-          const double *E = s ? Ep : Em, *L = s ? Lp : Lm;
+          const double *E = v + (s ? 6 : 0), *L = v + (s ? 18 : 12);
           const int a = s ? ip : im;
           pfbar[a] = wq * (E[0] + E[1] + E[2] + L[0] + L[1]);
           pfbar[Q + a] = wq * (E[1] - E[2] + E[3] + L[2] + L[3]);
@@ -1743,81 +1924,60 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
                }
 }
 
-// Per-size reverse sweep (adjoint of src/lmie_core.c:298-611): cotangents of
-// a_n, b_n from the angular part (Sbar, Tbar) and from the series, through
-// a_n(u), b_n(v) to D_n and m, and through the recomputed downward recurrence
-// D_j = (j+1)/z - 1/(D_{j+1} + (j+1)/z) upward from j = 1 to n2 - 1, to z = m x.
+// Per-size reverse sweep (adjoint of src/lmie_core.c:298-611), in two
+// kernels over the forward's kept recurrences (D_j up to n2 - 1, psi_j,
+// chi_j) and a_j, b_j:
+//   rev_terms  one thread per (size, group of 8 orders): cotangents of a_j, b_j
+//              from the angular part (Sbar, Tbar) and the series, through
+//              a = (u psi - psi0) / (u zeta - zeta0), u = D/m + j/x and
+//              b (v = m D + j/x) to the local cotangent of D_j and the direct
+//              part of the cotangent of m;
+//   rev_carry  one thread per size: the adjoint of the downward recurrence
+//              D_j = (j+1)/z - 1/(D_{j+1} + (j+1)/z), a first-order linear
+//              recurrence upward from j = 1 to n2 - 1 carrying the D cotangent
+//              to z = m x, then the per-size totals.
 // Outputs per size: the cotangent of m (re, im as d/dRe + i d/dIm) and of f_i.
 struct RevArgs {
-     int N, Np, Kp, n1max;
-     const double *x, *sx, *cx, *f;
+     int N, Np, Kp, n1max, n2rows, ng;
+     const double *x, *f;
      const int *n1, *n2;
      double mr, mi;
+     const double *D, *PC;      // forward recurrences (mie_chains, D up to n2 - 1)
      const double *AB;          // [4][Kp][Np]
      const double *CB;          // [2][Np]
      const double *S;           // series [4][Np]
-     const double *bar;         // [4][Kp][Np] (FP64 path), or
+     const double *bar;         // [4][Kp][Np] f-weighted cotangents (FP64 path), or
      const float *Uf;           // [4][Kq][Np] U = Sbar / f (mixed path; fang from U)
      int Kq;
-     const double *fang;        // [Np]
+     const double *fang;        // [Np] (FP64 path)
      double sb0, sb1, sb2, sb3; // cotangents of the raw sums
-     double *scrD;              // [3][n2max][Np] recomputed D_j (re, im) and ratios r_j
-     int n2max;
+     double *Lc;                // [2][n1max + 1][Np] local D cotangents
+     double *rpart;             // [3][ng][Np] partial m cotangent (re, im) and quadratic form
      double *mbar;              // [2][Np]
      double *fbar;              // [Np]
 };
 
-__global__ __launch_bounds__(64) void mie_reverse(RevArgs p) {
-     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-     if (i >= p.Np) return;
-     const size_t Np = p.Np, plane = (size_t) p.Kp * Np, drow = (size_t) p.n2max * Np;
-     if (i >= p.N) {
-          p.mbar[i] = p.mbar[Np + i] = 0.;
-          p.fbar[i] = 0.;
-          return;
-     }
-     const double x = p.x[i], f = p.f[i];
-     const int n1 = p.n1[i], n2 = p.n2[i];
-     const cd m = cmk(p.mr, -p.mi), z = cscale(m, x), m2 = cmul(m, m);
-     // size-weight cotangent: series part (angular part below when it comes from U)
-     const double fser = p.sb0 * p.S[i] + p.sb1 * p.S[Np + i] + p.sb2 * p.S[2 * Np + i] + p.sb3 * p.S[3 * Np + i];
+__global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
+     const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
+     if (i >= p.Np || g >= p.ng) return;
+     const size_t Np = p.Np, plane = (size_t) p.Kp * Np, dpl = (size_t) p.n2rows * Np;
+     const size_t ppl = (size_t) (p.n1max + 1) * Np, gp = (size_t) p.ng * Np;
+     const int n1 = i < p.N ? p.n1[i] : 0;
+     const int j0 = 8 * g + 1, j1 = min(8 * g + 8, n1);
+     cd mb = cmk(0., 0.);
      double fq = 0.;
-     // recompute and keep D_j, j = 1 .. n2 - 1 (D_{n2} = 0)
-     double *Dre = p.scrD, *Dim = p.scrD + drow;
-     {
-          cd D = cmk(0., 0.);
-          for (int j = n2 - 1; j >= 1; --j) {
-               const cd c1 = cdiv(cmk(j + 1., 0.), z);
-               D = csub(c1, crecip(cadd(D, c1)));
-               Dre[(size_t) j * Np + i] = D.r;
-               Dim[(size_t) j * Np + i] = D.i;
-          }
-     }
-     // psi_n recomputed as in the forward: downward ratios r_j into the third
-     // scratch plane, then psi_j = r_j psi_{j-1} upward
-     auto AB = [&](int q, int n) { return p.AB[q * plane + (size_t) (n - 1) * Np + i]; };
-     const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
-     double chi0 = p.cx[i], chi1 = (1. / x) * p.cx[i] + p.sx[i];
-     double psi0 = p.sx[i];
-     double *R = p.scrD + 2 * drow;
-     {
-          double r = 0.;
-          R[(size_t) n1 * Np + i] = 0.;
-          for (int j = n1 - 1; j >= 1; --j) {
-               r = 1. / ((2. * j + 1.) / x - r);
-               R[(size_t) j * Np + i] = r;
-          }
-     }
-     const cd im1 = crecip(m);
-     cd mb = cmk(0., 0.), carry = cmk(0., 0.), zb = cmk(0., 0.);
-     double sgn = -1.;
-     const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
-     for (int j = 1; j < n2; ++j) {
-          cd Db = carry;
-          if (j <= n1) {
-               const double psi = R[(size_t) j * Np + i] * psi0;
-               const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
-               const cd Dj = cmk(Dre[(size_t) j * Np + i], Dim[(size_t) j * Np + i]);
+     if (j0 <= j1) {
+          const double x = p.x[i], f = p.f[i];
+          const cd m = cmk(p.mr, -p.mi), m2 = cmul(m, m), im1 = crecip(m);
+          const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
+          auto AB = [&](int q, int n) { return p.AB[q * plane + (size_t) (n - 1) * Np + i]; };
+          const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
+          const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
+          double sgn = (j0 & 1) ? -1. : 1.;
+          for (int j = j0; j <= j1; ++j) {
+               const double psi = Psi[j * Np + i], psi0 = Psi[(j - 1) * Np + i];
+               const cd zj = cmk(psi, Chi[j * Np + i]), zjm = cmk(psi0, Chi[(j - 1) * Np + i]);
+               const cd Dj = cmk(Dre[j * Np + i], Dim[j * Np + i]);
                const double jx = j / x;
                const cd a = cmk(AB(0, j), AB(1, j)), b = cmk(AB(2, j), AB(3, j));
                const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
@@ -1861,15 +2021,36 @@ __global__ __launch_bounds__(64) void mie_reverse(RevArgs p) {
                const cd dvb = cdiv(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
                const cd ub = cmul(cconj(dua), ab), vb = cmul(cconj(dvb), bb);
                // u = D/m + j/x, v = m D + j/x
-               Db = cadd(Db, cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb)));
+               const cd lc = cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb));
+               p.Lc[(size_t) j * Np + i] = lc.r;
+               p.Lc[ppl + (size_t) j * Np + i] = lc.i;
                mb = cadd(mb, cadd(cmul(cconj(cscale(cdiv(Dj, m2), -1.)), ub), cmul(cconj(Dj), vb)));
-               // advance psi, chi
-               const double chi2 = fv1 / x * chi1 - chi0;
-               chi0 = chi1;
-               chi1 = chi2;
-               psi0 = psi;
                sgn = -sgn;
           }
+     }
+     double *pt = p.rpart + (size_t) g * Np + i;
+     pt[0] = mb.r;
+     pt[gp] = mb.i;
+     pt[2 * gp] = fq;
+}
+
+__global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
+     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+     if (i >= p.Np) return;
+     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np, gp = (size_t) p.ng * Np;
+     if (i >= p.N) {
+          p.mbar[i] = p.mbar[Np + i] = 0.;
+          p.fbar[i] = 0.;
+          return;
+     }
+     const double x = p.x[i];
+     const int n1 = p.n1[i], n2 = p.n2[i];
+     const cd z = cscale(cmk(p.mr, -p.mi), x);
+     const double *Dre = p.D, *Dim = p.D + dpl;
+     cd carry = cmk(0., 0.), zb = cmk(0., 0.);
+     for (int j = 1; j < n2; ++j) {
+          cd Db = carry;
+          if (j <= n1) Db = cadd(Db, cmk(p.Lc[(size_t) j * Np + i], p.Lc[ppl + (size_t) j * Np + i]));
           // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z
           const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
           const cd c1 = cdiv(cmk(j + 1., 0.), z);
@@ -1879,9 +2060,18 @@ __global__ __launch_bounds__(64) void mie_reverse(RevArgs p) {
           const cd dc1dz = cscale(cdiv(c1, z), -1.);
           zb = cadd(zb, cmul(cconj(cmul(cadd(cmk(1., 0.), ic22), dc1dz)), Db));
      }
+     cd mb = cmk(0., 0.);
+     double fq = 0.;
+     for (int g = 0; g < (n1 + 7) / 8; ++g) {
+          const double *pt = p.rpart + (size_t) g * Np + i;
+          mb = cadd(mb, cmk(pt[0], pt[gp]));
+          fq += pt[2 * gp];
+     }
      mb = cadd(mb, cscale(zb, x));
      p.mbar[i] = mb.r;
      p.mbar[Np + i] = mb.i;
+     // size-weight cotangent: angular part plus the series part
+     const double fser = p.sb0 * p.S[i] + p.sb1 * p.S[Np + i] + p.sb2 * p.S[2 * Np + i] + p.sb3 * p.S[3 * Np + i];
      p.fbar[i] = (p.Uf ? fq : p.fang[i]) + fser;
 }
 
@@ -2086,13 +2276,18 @@ struct Buf {
 struct Workspace {
      int device = -1;
      hipStream_t stream = nullptr;
+     hipStream_t side = nullptr;     // angle tables, concurrent with the per-size recurrences
+     hipEvent_t fork = nullptr, join = nullptr;
+     std::vector<double> hguess;     // host staging of the Gauss root guesses
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
      Buf x, sx, cx, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
      Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2, order;
      std::vector<double> hsc;       // host staging of sin x, cos x
      // reverse-pass checkpoint and buffers
-     Buf AB, CB, sst, pfbar, bar, fang, mbar, fbar, gbar, scrRev, ones;
+     Buf AB, CB, sst, pfbar, bar, fang, mbar, fbar, gbar, ones, expart;
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
+     Buf chD, chPC, chPart, revLc, revPart;         // per-size recurrences of the value channel
+     int n2rows = 0, ng = 0;
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
      int mixed = 0, bf16 = 0, Kb = 0;
@@ -2119,6 +2314,9 @@ int ws_acquire(int device, Workspace **out) {
           w = new Workspace();
           w->device = device;
           HIPCHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+          HIPCHECK(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
+          HIPCHECK(hipEventCreateWithFlags(&w->fork, hipEventDisableTiming));
+          HIPCHECK(hipEventCreateWithFlags(&w->join, hipEventDisableTiming));
           for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
      }
      *out = w;
@@ -2134,8 +2332,8 @@ void ws_release(Workspace *w) {
 struct Timer {
      Workspace *w;
      int on;
-     void mark(int id, int end) {
-          if (on) (void) hipEventRecord(w->ev[2 * id + end], w->stream);
+     void mark(int id, int end, hipStream_t st = nullptr) {
+          if (on) (void) hipEventRecord(w->ev[2 * id + end], st ? st : w->stream);
      }
      void collect(const int *ids, int n) {
           for (int k = 0; k < LMIE_N_TIMERS; ++k) g_times[k] = 0.;
@@ -2181,7 +2379,6 @@ int launch_angular_nw(int nw, const AngArgs &a, int n_tiles, hipStream_t s, int 
 int launch_coefs(int nt, const CoefArgs &c, hipStream_t s) {
      dim3 grid(c.Np / 64);
      switch (nt) {
-     case 0: hipLaunchKernelGGL(mie_coefs<0>, grid, dim3(64), 0, s, c); break;
      case 1: hipLaunchKernelGGL(mie_coefs<1>, grid, dim3(64), 0, s, c); break;
      case 2: hipLaunchKernelGGL(mie_coefs<2>, grid, dim3(64), 0, s, c); break;
      case 3: hipLaunchKernelGGL(mie_coefs<3>, grid, dim3(64), 0, s, c); break;
@@ -2285,18 +2482,21 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      }
      tm.mark(LMIE_T_TOTAL, 0);
 
-     // angle tables
-     tm.mark(LMIE_T_TABLES, 0);
+     // angle tables, on the side stream while the per-size recurrences run
+     hipStream_t ts = w->side;
+     HIPCHECK(hipEventRecord(w->fork, s));
+     HIPCHECK(hipStreamWaitEvent(ts, w->fork, 0));
+     tm.mark(LMIE_T_TABLES, 0, ts);
      if (Q > 0) {
+          hipStream_t s = ts;
           if (w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) ||
               w->Bp.get(sizeof(double) * (size_t) Kp * Hp) || w->Bm.get(sizeof(double) * (size_t) Kp * Hp))
                return -1;
           {
-               std::vector<double> guess(H);
-               gauss_guesses(Q, guess.data());
+               w->hguess.resize(H > 0 ? H : 1);
+               gauss_guesses(Q, w->hguess.data());
                if (w->guess.get(sizeof(double) * H)) return -1;
-               HIPCHECK(hipMemcpyAsync(w->guess.p, guess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
-               HIPCHECK(hipStreamSynchronize(s));   // guess lives on this stack frame
+               HIPCHECK(hipMemcpyAsync(w->guess.p, w->hguess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
           }
           hipLaunchKernelGGL(gauss_nodes, dim3((H + 63) / 64), dim3(64), 0, s, Q, (const double *) w->guess.as<double>(),
                              w->mu.as<double>(), w->w.as<double>());
@@ -2322,14 +2522,16 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           }
           HIPCHECK(hipGetLastError());
      }
-     tm.mark(LMIE_T_TABLES, 1);
+     tm.mark(LMIE_T_TABLES, 1, ts);
+     HIPCHECK(hipEventRecord(w->join, ts));
 
      // per-size coefficients (passes of up to 4 tangent channels)
      const size_t plane = (size_t) Kp * Np, srow = (size_t) (n1max + 1) * Np;
      const int nt_pass = nt < 4 ? nt : 4;
      if ((!mixed && w->A.get(sizeof(double) * 4 * plane * (1 + nt))) || w->S.get(sizeof(double) * 4 * Np * (1 + nt)) ||
          (mixed && w->A16.get(sizeof(unsigned short) * 8 * (size_t) Kb * Np * 8)) ||
-         w->scrD.get(sizeof(double) * 2 * srow * (1 + nt_pass)) || w->scrR.get(sizeof(double) * srow * (1 + nt_pass)))
+         (nt && (w->scrD.get(sizeof(double) * 2 * srow * (1 + nt_pass)) ||
+                 w->scrR.get(sizeof(double) * srow * (1 + nt_pass)))))
           return -1;
      const int keep = J->keep;
      if (keep) {
@@ -2339,7 +2541,43 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                return -1;
      }
      tm.mark(LMIE_T_COEFS, 0);
-     for (int t0 = 0; t0 < (nt ? nt : 1); t0 += 4) {
+     if (nt == 0) {
+          // value channel: serial recurrences, parallel terms, per-size sums
+          int n2max = 0, kfmax = 0;
+          for (int i = 0; i < N; ++i) n2max = J->n2[i] > n2max ? J->n2[i] : n2max;
+          for (int t = 0; t < nt64; ++t) kfmax = kfill[t] > kfmax ? kfill[t] : kfmax;
+          const int n2rows = n2max + 1, ng = kfmax / 8;
+          if (w->chD.get(sizeof(double) * 2 * (size_t) n2rows * Np) ||
+              w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) ||
+              w->chPart.get(sizeof(double) * 5 * (size_t) (ng > 0 ? ng : 1) * Np))
+               return -1;
+          ChainArgs ca;
+          ca.N = N; ca.Np = Np; ca.n1max = n1max; ca.n2rows = n2rows;
+          ca.x = w->x.as<double>(); ca.sx = w->sx.as<double>(); ca.cx = w->cx.as<double>();
+          ca.n1 = w->n1.as<int>(); ca.n2 = w->n2.as<int>(); ca.mr = J->mr; ca.mi = J->mi;
+          ca.d_all = keep; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
+          hipLaunchKernelGGL(mie_chains, dim3(Np / 64), dim3(64), 0, s, ca);
+          TermArgs ta;
+          ta.N = N; ta.Np = Np; ta.n1max = n1max; ta.n2rows = n2rows; ta.Kp = Kp; ta.Kb = Kb; ta.ng = ng;
+          ta.x = w->x.as<double>(); ta.n1 = w->n1.as<int>(); ta.kfill = w->kfill.as<int>();
+          ta.mr = J->mr; ta.mi = J->mi; ta.D = w->chD.as<double>(); ta.PC = w->chPC.as<double>();
+          ta.A = mixed ? nullptr : w->A.as<double>();
+          ta.A16 = mixed ? w->A16.as<unsigned short>() : nullptr;
+          ta.AB = keep ? w->AB.as<double>() : nullptr;
+          ta.part = w->chPart.as<double>();
+          if (ng > 0) {
+               const dim3 grid(Np / 64, ng);
+               if (!mixed) hipLaunchKernelGGL(mie_terms<void>, grid, dim3(64), 0, s, ta);
+               else if (bf16) hipLaunchKernelGGL(mie_terms<BF16>, grid, dim3(64), 0, s, ta);
+               else hipLaunchKernelGGL(mie_terms<F16>, grid, dim3(64), 0, s, ta);
+          }
+          hipLaunchKernelGGL(mie_sums, dim3(Np / 256 + 1), dim3(256), 0, s, (const double *) w->chPart.as<double>(), N, Np,
+                             ng, (const int *) w->n1.as<int>(), w->S.as<double>(), keep ? w->CB.as<double>() : nullptr);
+          HIPCHECK(hipGetLastError());
+          w->n2rows = n2rows;
+          w->ng = ng;
+     }
+     for (int t0 = 0; nt > 0 && t0 < nt; t0 += 4) {
           CoefArgs c;
           memset(&c, 0, sizeof c);
           c.N = N; c.Np = Np; c.n1max = n1max;
@@ -2360,17 +2598,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                c.St[q] = w->S.as<double>() + (size_t) 4 * Np * (1 + t0 + q);
           }
           c.write_base = t0 == 0;
-          c.AB = keep ? w->AB.as<double>() : nullptr;
-          c.CB = keep ? w->CB.as<double>() : nullptr;
-          if (mixed) {
-               c.A16 = w->A16.as<unsigned short>();
-               c.Kb = Kb;
-               if (bf16) hipLaunchKernelGGL((mie_coefs<0, BF16>), dim3(Np / 64), dim3(64), 0, s, c);
-               else hipLaunchKernelGGL((mie_coefs<0, F16>), dim3(Np / 64), dim3(64), 0, s, c);
-               HIPCHECK(hipGetLastError());
-          } else if (launch_coefs(k, c, s)) {
-               return -1;
-          }
+          if (launch_coefs(k, c, s)) return -1;
      }
      tm.mark(LMIE_T_COEFS, 1);
 
@@ -2394,7 +2622,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           HIPCHECK(hipGetLastError());
      }
 
-     // angular contraction and reduction
+     // angular contraction and reduction (after the angle tables)
+     HIPCHECK(hipStreamWaitEvent(s, w->join, 0));
      if (Q > 0) {
           if (w->pf.get(sizeof(double) * (size_t) nch * 4 * Q)) return -1;
           if (J->save_control == 1 && w->save2.get(sizeof(double) * (size_t) 4 * N * Q)) return -1;
@@ -2611,8 +2840,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      if (w->pfbar.get(sizeof(double) * 4 * (size_t) (Q > 0 ? Q : 1)) || w->fang.get(sizeof(double) * Np) ||
          w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * Np) ||
          w->gbar.get(sizeof(double) * 12 * (size_t) (C > 0 ? C : 1)) ||
-         w->bar.get(sizeof(double) * 4 * (size_t) Kp * Np) ||
-         w->scrRev.get(sizeof(double) * 3 * (size_t) w->n2max * Np) || w->ones.get(sizeof(double) * Np))
+         w->bar.get(sizeof(double) * 4 * (size_t) Kp * Np) || w->ones.get(sizeof(double) * Np))
           return -1;
      tm.mark(LMIE_T_TOTAL, 0);
      HIPCHECK(hipMemsetAsync(w->fang.p, 0, sizeof(double) * Np, s));
@@ -2622,9 +2850,13 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           double *g = w->gbar.as<double>();
           if (use_gc) HIPCHECK(hipMemcpyAsync(g, in->gc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
           if (use_lc) HIPCHECK(hipMemcpyAsync(g + 6 * C, in->lc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
-          hipLaunchKernelGGL(expand_adjoint, dim3((H + 63) / 64), dim3(64), 0, s, (const double *) w->T.as<double>(),
-                             (const double *) w->w.as<double>(), use_gc ? (const double *) g : nullptr,
-                             use_lc ? (const double *) (g + 6 * C) : nullptr, H, Hp, C, Q, w->pfbar.as<double>());
+          const int nck = (C + EXA_L - 1) / EXA_L;
+          if (w->expart.get(sizeof(double) * 24 * (size_t) nck * Hp)) return -1;
+          hipLaunchKernelGGL(expand_adjoint_part, dim3(Hp / 64, nck), dim3(64), 0, s, (const double *) w->T.as<double>(),
+                             use_gc ? (const double *) g : nullptr, use_lc ? (const double *) (g + 6 * C) : nullptr,
+                             H, Hp, C, w->expart.as<double>());
+          hipLaunchKernelGGL(expand_adjoint_fin, dim3((H + 63) / 64), dim3(64), 0, s, (const double *) w->expart.as<double>(),
+                             nck, (const double *) w->w.as<double>(), H, Hp, Q, w->pfbar.as<double>());
      }
      if (w->mixed && Q > 0 && (use_gc || use_lc)) {
           if (adjoint_mixed_angular(w, s)) return -1;
@@ -2647,17 +2879,23 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      }
      const int use_u = w->mixed && Q > 0 && (use_gc || use_lc);
      tm.mark(LMIE_T_ADJ_COEFS, 0);
+     const int ng = w->ng;
+     if (w->revLc.get(sizeof(double) * 2 * (size_t) (w->n1max + 1) * Np) ||
+         w->revPart.get(sizeof(double) * 3 * (size_t) (ng > 0 ? ng : 1) * Np))
+          return -1;
      RevArgs r;
-     r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max;
-     r.x = w->x.as<double>(); r.sx = w->sx.as<double>(); r.cx = w->cx.as<double>(); r.f = w->f.as<double>();
+     r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.ng = ng;
+     r.x = w->x.as<double>(); r.f = w->f.as<double>();
      r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
+     r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>();
      r.AB = w->AB.as<double>(); r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
      r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
      r.fang = w->fang.as<double>();
      r.sb0 = in->sums_bar[0]; r.sb1 = in->sums_bar[1]; r.sb2 = in->sums_bar[2]; r.sb3 = in->sums_bar[3];
-     r.scrD = w->scrRev.as<double>(); r.n2max = w->n2max;
+     r.Lc = w->revLc.as<double>(); r.rpart = w->revPart.as<double>();
      r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
-     hipLaunchKernelGGL(mie_reverse, dim3(Np / 64), dim3(64), 0, s, r);
+     if (ng > 0) hipLaunchKernelGGL(rev_terms, dim3(Np / 64, ng), dim3(64), 0, s, r);
+     hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r);
      HIPCHECK(hipGetLastError());
      tm.mark(LMIE_T_ADJ_COEFS, 1);
      // sum of the per-size m cotangents
@@ -2744,7 +2982,10 @@ int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
      if (ws_acquire(job->device, &w)) return -1;
      res->state = nullptr;
      int rc = forward_impl(job, res, w);
-     if (rc) (void) hipStreamSynchronize(w->stream);
+     if (rc) {
+          (void) hipStreamSynchronize(w->side);
+          (void) hipStreamSynchronize(w->stream);
+     }
      if (rc == 0 && job->keep) {
           res->state = w;          // owned by the caller until lmie_gpu_release
           return 0;
