@@ -21,6 +21,7 @@ typedef struct {
      const double *f;         /* [N] number density x quadrature weight          */
      const int *n1;           /* [N] per-size order n1(x_i), src/lmie_core.c:18-26 */
      const int *n2;           /* [N] per-size D_n start,   src/lmie_core.c:30-38 */
+     const double *sx, *cx;   /* [N] sin x_i, cos x_i from the C library, or NULL */
      double mr, mi;           /* m = mr - i mi (src/lmie_schedule.c:1048)         */
      int n1_max;              /* order of the largest radius r2                   */
      int n_qang;              /* Gauss angles (2 n1_max), 0 = no angular work     */

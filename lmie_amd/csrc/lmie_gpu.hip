@@ -141,6 +141,12 @@ __device__ __forceinline__ cd cdiv(cd a, cd b) {
      return cd{(a.r * t + a.i) / d, (a.i * t - a.r) / d};
 }
 __device__ __forceinline__ cd crecip(cd b) { return cdiv(cd{1., 0.}, b); }
+// 1/b = conj(b) / |b|^2 with one division, for divisors well inside the
+// range (|b|^2 neither overflows nor underflows)
+__device__ __forceinline__ cd crecip_fast(cd b) {
+     const double q = 1. / (b.r * b.r + b.i * b.i);
+     return cd{b.r * q, -b.i * q};
+}
 
 // ---------------------------------------------------------------------------
 // Gauss-Legendre angles: one thread per root.  Newton steps with the
@@ -569,14 +575,15 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
      const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
      const double x = p.x[i];
      const int n1 = p.n1[i], n2 = p.n2[i], dtop = p.d_all ? n2 - 1 : n1;
-     const cd z = cscale(cmk(p.mr, -p.mi), x);
+     const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
      double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
-     // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0
+     // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0; one
+     // division per step (|D_{j+1} + (j+1)/z| stays between about 1/|z| and n2/|z|)
      cd D = cmk(0., 0.);
      for (int j = n2 - 1; j >= 1; --j) {
-          const cd c1 = cdiv(cmk(j + 1., 0.), z);
+          const cd c1 = cscale(iz, j + 1.);
           const cd c2 = cadd(D, c1);
-          D = csub(c1, crecip(c2));
+          D = csub(c1, crecip_fast(c2));
           if (j <= dtop) {
                Dre[j * Np + i] = D.r;
                Dim[j * Np + i] = D.i;
@@ -597,15 +604,25 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
      double psi0 = sx, chi0 = cx, chi1 = ax * cx + sx;
      Psi[i] = psi0;
      Chi[i] = chi0;
-     for (int j = 1; j <= n1; ++j) {
-          const double psi = Psi[j * Np + i] * psi0;
-          Psi[j * Np + i] = psi;
-          Chi[j * Np + i] = chi1;
-          const double fv1 = 2. * j + 1.;
-          const double chi2 = fv1 / x * chi1 - chi0;
-          chi0 = chi1;
-          chi1 = chi2;
-          psi0 = psi;
+     // ratios read 16 rows ahead so the loads are off the dependence chain
+     for (int jb = 1; jb <= n1; jb += 16) {
+          double rr[16];
+#pragma unroll
+          for (int e = 0; e < 16; ++e) rr[e] = jb + e <= n1 ? Psi[(size_t) (jb + e) * Np + i] : 0.;
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+               const int j = jb + e;
+               if (j <= n1) {
+                    const double psi = rr[e] * psi0;
+                    Psi[(size_t) j * Np + i] = psi;
+                    Chi[(size_t) j * Np + i] = chi1;
+                    const double fv1 = 2. * j + 1.;
+                    const double chi2 = fv1 / x * chi1 - chi0;
+                    chi0 = chi1;
+                    chi1 = chi2;
+                    psi0 = psi;
+               }
+          }
      }
 }
 
@@ -2045,20 +2062,35 @@ __global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
      }
      const double x = p.x[i];
      const int n1 = p.n1[i], n2 = p.n2[i];
-     const cd z = cscale(cmk(p.mr, -p.mi), x);
+     const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
      const double *Dre = p.D, *Dim = p.D + dpl;
      cd carry = cmk(0., 0.), zb = cmk(0., 0.);
-     for (int j = 1; j < n2; ++j) {
-          cd Db = carry;
-          if (j <= n1) Db = cadd(Db, cmk(p.Lc[(size_t) j * Np + i], p.Lc[ppl + (size_t) j * Np + i]));
-          // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z
-          const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
-          const cd c1 = cdiv(cmk(j + 1., 0.), z);
-          const cd c2 = cadd(Dn, c1);
-          const cd ic22 = crecip(cmul(c2, c2));
-          carry = cmul(cconj(ic22), Db);
-          const cd dc1dz = cscale(cdiv(c1, z), -1.);
-          zb = cadd(zb, cmul(cconj(cmul(cadd(cmk(1., 0.), ic22), dc1dz)), Db));
+     // blocks of 8 steps: loads and the factors that do not depend on the
+     // carried cotangent first, then the first-order recurrence itself
+     for (int jb = 1; jb < n2; jb += 8) {
+          cd lc[8], g[8], h[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+               const int j = jb + e;
+               lc[e] = j <= n1 ? cmk(p.Lc[(size_t) j * Np + i], p.Lc[ppl + (size_t) j * Np + i]) : cmk(0., 0.);
+               // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z
+               const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
+               const cd c1 = cscale(iz, j + 1.);
+               const cd c2 = cadd(Dn, c1);
+               const cd rc = crecip_fast(c2);
+               const cd ic22 = cmul(rc, rc);
+               const cd dc1dz = cscale(cmul(c1, iz), -1.);
+               g[e] = cconj(ic22);
+               h[e] = cconj(cmul(cadd(cmk(1., 0.), ic22), dc1dz));
+          }
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+               if (jb + e < n2) {
+                    const cd Db = cadd(carry, lc[e]);
+                    carry = cmul(g[e], Db);
+                    zb = cadd(zb, cmul(h[e], Db));
+               }
+          }
      }
      cd mb = cmk(0., 0.);
      double fq = 0.;
@@ -2277,7 +2309,7 @@ struct Workspace {
      int device = -1;
      hipStream_t stream = nullptr;
      hipStream_t side = nullptr;     // angle tables, concurrent with the per-size recurrences
-     hipEvent_t fork = nullptr, join = nullptr;
+     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
      std::vector<double> hguess;     // host staging of the Gauss root guesses
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
      Buf x, sx, cx, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
@@ -2317,6 +2349,7 @@ int ws_acquire(int device, Workspace **out) {
           HIPCHECK(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
           HIPCHECK(hipEventCreateWithFlags(&w->fork, hipEventDisableTiming));
           HIPCHECK(hipEventCreateWithFlags(&w->join, hipEventDisableTiming));
+          HIPCHECK(hipEventCreateWithFlags(&w->join2, hipEventDisableTiming));
           for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
      }
      *out = w;
@@ -2451,8 +2484,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           if (w->sx.get(sizeof(double) * Np) || w->cx.get(sizeof(double) * Np)) return -1;
           w->hsc.resize(2 * (size_t) Np);
           for (int i = 0; i < Np; ++i) {
-               w->hsc[i] = i < N ? sin(J->x[i]) : 0.;
-               w->hsc[Np + i] = i < N ? cos(J->x[i]) : 1.;
+               w->hsc[i] = i < N ? (J->sx ? J->sx[i] : sin(J->x[i])) : 0.;
+               w->hsc[Np + i] = i < N ? (J->cx ? J->cx[i] : cos(J->x[i])) : 1.;
           }
           HIPCHECK(hipMemcpyAsync(w->sx.p, w->hsc.data(), sizeof(double) * Np, hipMemcpyHostToDevice, s));
           HIPCHECK(hipMemcpyAsync(w->cx.p, w->hsc.data() + Np, sizeof(double) * Np, hipMemcpyHostToDevice, s));
@@ -2513,6 +2546,9 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kp, w->mu.as<double>(),
                                   w->Bp.as<double>(), w->Bm.as<double>());
           }
+          // the contraction needs the tables above; the expansion tables below
+          // only the expansion
+          HIPCHECK(hipEventRecord(w->join, s));
           if (J->calc_gc || J->calc_lc) {
                const int planes = J->calc_lc ? 5 : 4;
                if (w->T.get(sizeof(double) * planes * (size_t) C * Hp)) return -1;
@@ -2523,7 +2559,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           HIPCHECK(hipGetLastError());
      }
      tm.mark(LMIE_T_TABLES, 1, ts);
-     HIPCHECK(hipEventRecord(w->join, ts));
+     if (Q <= 0) HIPCHECK(hipEventRecord(w->join, ts));
+     HIPCHECK(hipEventRecord(w->join2, ts));
 
      // per-size coefficients (passes of up to 4 tangent channels)
      const size_t plane = (size_t) Kp * Np, srow = (size_t) (n1max + 1) * Np;
@@ -2738,7 +2775,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           HIPCHECK(hipGetLastError());
           tm.mark(LMIE_T_REDUCE, 1);
 
-          // expansion
+          // expansion (after the expansion tables)
+          HIPCHECK(hipStreamWaitEvent(s, w->join2, 0));
           tm.mark(LMIE_T_EXPAND, 0);
           if (w->E.get(sizeof(double) * (size_t) nch * 12 * Hp) || w->coefs.get(sizeof(double) * (size_t) nch * 6 * C))
                return -1;

@@ -14,15 +14,91 @@
 #include <complex.h>
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "../../include/lmie_interface.h"
 #include "../../include/lmie_hip.h"
 #include "../../include/lmie_gpu.h"
 
 #define LM_PI 3.14159265358979323846264338327950288
+
+/* ------------------------------------------------------------------------- */
+/* Per-size host loops on a small persistent thread pool (up to 16 threads,
+   one parallel region at a time; a caller that finds the pool busy runs its
+   loop alone).  Chunks are static, so every element is computed by the same
+   code as a serial loop would: the results do not depend on the threads. */
+typedef void (*par_fn)(void *ctx, int i0, int i1);
+static struct {
+     pthread_mutex_t region, lk;
+     pthread_cond_t go, done;
+     int started, nthr, gen, pending, n;
+     par_fn fn;
+     void *ctx;
+} g_pool = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER,
+            PTHREAD_COND_INITIALIZER, 0, 0, 0, 0, 0, NULL, NULL};
+
+static void pool_chunk(int t, int T, int n, int *i0, int *i1) {
+     *i0 = (int) ((long long) n * t / T);
+     *i1 = (int) ((long long) n * (t + 1) / T);
+}
+
+static void *pool_worker(void *arg) {
+     const int t = (int) (intptr_t) arg;
+     int seen = 0;
+     pthread_mutex_lock(&g_pool.lk);
+     for (;;) {
+          while (g_pool.gen == seen) pthread_cond_wait(&g_pool.go, &g_pool.lk);
+          seen = g_pool.gen;
+          par_fn fn = g_pool.fn;
+          void *ctx = g_pool.ctx;
+          int n = g_pool.n, T = g_pool.nthr + 1, i0, i1;
+          pthread_mutex_unlock(&g_pool.lk);
+          pool_chunk(t, T, n, &i0, &i1);
+          if (i1 > i0) fn(ctx, i0, i1);
+          pthread_mutex_lock(&g_pool.lk);
+          if (--g_pool.pending == 0) pthread_cond_signal(&g_pool.done);
+     }
+     return NULL;
+}
+
+static void par_for(int n, par_fn fn, void *ctx) {
+     if (n < 2048 || pthread_mutex_trylock(&g_pool.region) != 0) {
+          fn(ctx, 0, n);
+          return;
+     }
+     pthread_mutex_lock(&g_pool.lk);
+     if (!g_pool.started) {
+          g_pool.started = 1;
+          long nc = sysconf(_SC_NPROCESSORS_ONLN);
+          int T = nc > 16 ? 16 : (nc < 1 ? 1 : (int) nc);
+          for (int t = 1; t < T; ++t) {
+               pthread_t th;
+               if (pthread_create(&th, NULL, pool_worker, (void *) (intptr_t) t) != 0) break;
+               pthread_detach(th);
+               g_pool.nthr++;
+          }
+     }
+     const int T = g_pool.nthr + 1;
+     g_pool.fn = fn;
+     g_pool.ctx = ctx;
+     g_pool.n = n;
+     g_pool.pending = g_pool.nthr;
+     g_pool.gen++;
+     pthread_cond_broadcast(&g_pool.go);
+     pthread_mutex_unlock(&g_pool.lk);
+     int i0, i1;
+     pool_chunk(0, T, n, &i0, &i1);
+     fn(ctx, i0, i1);
+     pthread_mutex_lock(&g_pool.lk);
+     while (g_pool.pending > 0) pthread_cond_wait(&g_pool.done, &g_pool.lk);
+     pthread_mutex_unlock(&g_pool.lk);
+     pthread_mutex_unlock(&g_pool.region);
+}
 
 /* ------------------------------------------------------------------------- */
 /* errors: one message per thread; -1 returns leave a breadcrumb like the
@@ -142,7 +218,7 @@ static void grid_intervals(int n_int, int n_quad, int nd, double r1, double r2,
                qx[a] = mid - half * tx[k];
                qx[b] = mid + half * tx[k];
                qw[a] = qw[b] = tw[i] * half;
-               for (int d = 0; d < nd; ++d) {
+               for (int d = 0; d < nd && (r1_l || r2_l); ++d) {   /* else the tangents stay zero */
                     double st_l = (sd(r2_l, d) - sd(r1_l, d)) / n_int;
                     double lo_l = sd(r1_l, d) + s * st_l, hi_l = lo_l + st_l;
                     double mid_l = (lo_l + hi_l) / 2., half_l = (hi_l - lo_l) / 2.;
@@ -214,49 +290,23 @@ static void grid_free(grid_t *g) {
 
 /* Grid, number density normalised to unit integral, and tangents along each
    seed column (misc/rtutil_size_dist.c:200-646). */
-static int grid_build(int dist, int n_int1, int n_int2, int n_quad, const double a[5],
-                      double r1, double r2, const seeds *sv, grid_t *g) {
-     int nd = sv->nd, N = grid_points(dist, n_int1, n_int2, n_quad);
-     if (N <= 0) return fail("size distribution", "invalid distribution type");
-     size_t L = (size_t) N * (nd > 0 ? nd : 1);
-     g->N = N; g->nd = nd;
-     g->qx = vec(N); g->qw = vec(N); g->nr = vec(N);
-     g->qx_l = vec(L); g->qw_l = vec(L); g->nr_l = vec(L);
-     g->r1_l = vec(nd); g->r2_l = vec(nd); g->norm_l = vec(nd);
-     double *tot_l = vec(nd), tot = 0.;
-     g->r1 = r1; g->r2 = r2;
-     for (int d = 0; d < nd; ++d) { g->r1_l[d] = sd(sv->r1, d); g->r2_l[d] = sd(sv->r2, d); }
-     double a1 = a[0], a2 = a[1], a3 = a[2], a4 = a[3], a5 = a[4];
+/* Number density and its tangents at the nodes i0 .. i1-1
+   (misc/rtutil_size_dist.c:20-646). */
+typedef struct {
+     int dist, nd, N1;
+     const seeds *sv;
+     grid_t *g;
+     double a[5];
+} dens_ctx;
 
-     if (dist == SIZE_DIST_MONO) {
-          g->r1 = g->r2 = a1;
-          for (int d = 0; d < nd; ++d) { g->r1_l[d] = g->r2_l[d] = 0.; g->qx_l[d] = sd(sv->a[0], d); }
-          g->qx[0] = a1; g->qw[0] = 1.; g->nr[0] = 1.;
-          g->norm = 1.;
-          free(tot_l);
-          return 0;
-     }
-     if (dist == SIZE_DIST_POWER_LAW) {
-          pl_range(a1, a2, &g->r1, &g->r2);
-          double Lg = log(g->r2) - log(g->r1);
-          double p = -Lg + (g->r2 - g->r1) / g->r1, q = Lg - (g->r2 - g->r1) / g->r2;
-          for (int d = 0; d < nd; ++d) {
-               double u = sd(sv->a[0], d) * Lg * Lg;
-               double v = sd(sv->a[1], d) * 2. * a1 + (1. + a2) * 2. * sd(sv->a[0], d);
-               g->r1_l[d] = (q * v - u) / (q - p);
-               g->r2_l[d] = (p * v - u) / (p - q);
-          }
-     }
-     int N1 = 0;
-     if (dist == SIZE_DIST_MODIFIED_POWER_LAW) {
-          N1 = n_int1 * n_quad;
-          grid_intervals(n_int1, n_quad, nd, 0., g->r1, NULL, g->r1_l, g->qx, g->qw, g->qx_l, g->qw_l);
-          grid_intervals(n_int2, n_quad, nd, g->r1, g->r2, g->r1_l, g->r2_l, g->qx + N1, g->qw + N1,
-                         g->qx_l + (size_t) N1 * nd, g->qw_l + (size_t) N1 * nd);
-     } else
-          grid_intervals(n_int2, n_quad, nd, g->r1, g->r2, g->r1_l, g->r2_l, g->qx, g->qw, g->qx_l, g->qw_l);
-
-     for (int i = 0; i < N; ++i) {
+static void dens_range(void *vp, int i0, int i1) {
+     const dens_ctx *dc = vp;
+     const int dist = dc->dist, nd = dc->nd, N1 = dc->N1;
+     const seeds *sv = dc->sv;
+     grid_t *g = dc->g;
+     const double a1 = dc->a[0], a2 = dc->a[1], a3 = dc->a[2], a4 = dc->a[3], a5 = dc->a[4];
+     (void) N1;
+     for (int i = i0; i < i1; ++i) {
           double r = g->qx[i], lr = log(r), *nl = g->nr_l + (size_t) i * nd;
           const double *xl = g->qx_l + (size_t) i * nd;
           double n = 0.;
@@ -324,6 +374,54 @@ static int grid_build(int dist, int n_int1, int n_int2, int n_quad, const double
                break;
           }
           g->nr[i] = n;
+     }
+}
+
+static int grid_build(int dist, int n_int1, int n_int2, int n_quad, const double a[5],
+                      double r1, double r2, const seeds *sv, grid_t *g) {
+     int nd = sv->nd, N = grid_points(dist, n_int1, n_int2, n_quad);
+     if (N <= 0) return fail("size distribution", "invalid distribution type");
+     size_t L = (size_t) N * (nd > 0 ? nd : 1);
+     g->N = N; g->nd = nd;
+     g->qx = vec(N); g->qw = vec(N); g->nr = vec(N);
+     g->qx_l = vec(L); g->qw_l = vec(L); g->nr_l = vec(L);
+     g->r1_l = vec(nd); g->r2_l = vec(nd); g->norm_l = vec(nd);
+     double *tot_l = vec(nd), tot = 0.;
+     g->r1 = r1; g->r2 = r2;
+     for (int d = 0; d < nd; ++d) { g->r1_l[d] = sd(sv->r1, d); g->r2_l[d] = sd(sv->r2, d); }
+     double a1 = a[0], a2 = a[1], a3 = a[2], a4 = a[3], a5 = a[4];
+
+     if (dist == SIZE_DIST_MONO) {
+          g->r1 = g->r2 = a1;
+          for (int d = 0; d < nd; ++d) { g->r1_l[d] = g->r2_l[d] = 0.; g->qx_l[d] = sd(sv->a[0], d); }
+          g->qx[0] = a1; g->qw[0] = 1.; g->nr[0] = 1.;
+          g->norm = 1.;
+          free(tot_l);
+          return 0;
+     }
+     if (dist == SIZE_DIST_POWER_LAW) {
+          pl_range(a1, a2, &g->r1, &g->r2);
+          double Lg = log(g->r2) - log(g->r1);
+          double p = -Lg + (g->r2 - g->r1) / g->r1, q = Lg - (g->r2 - g->r1) / g->r2;
+          for (int d = 0; d < nd; ++d) {
+               double u = sd(sv->a[0], d) * Lg * Lg;
+               double v = sd(sv->a[1], d) * 2. * a1 + (1. + a2) * 2. * sd(sv->a[0], d);
+               g->r1_l[d] = (q * v - u) / (q - p);
+               g->r2_l[d] = (p * v - u) / (p - q);
+          }
+     }
+     int N1 = 0;
+     if (dist == SIZE_DIST_MODIFIED_POWER_LAW) {
+          N1 = n_int1 * n_quad;
+          grid_intervals(n_int1, n_quad, nd, 0., g->r1, NULL, g->r1_l, g->qx, g->qw, g->qx_l, g->qw_l);
+          grid_intervals(n_int2, n_quad, nd, g->r1, g->r2, g->r1_l, g->r2_l, g->qx + N1, g->qw + N1,
+                         g->qx_l + (size_t) N1 * nd, g->qw_l + (size_t) N1 * nd);
+     } else
+          grid_intervals(n_int2, n_quad, nd, g->r1, g->r2, g->r1_l, g->r2_l, g->qx, g->qw, g->qx_l, g->qw_l);
+
+     {
+          dens_ctx dc = {dist, nd, N1, sv, g, {a1, a2, a3, a4, a5}};
+          par_for(N, dens_range, &dc);
      }
      for (int i = 0; i < N; ++i) {
           tot += g->nr[i] * g->qw[i];
@@ -419,32 +517,51 @@ typedef struct {
      void *state;                   /* device state when core_args.keep */
 } core_out;
 
-static void fill_job(const core_args *c, lmie_gpu_job *j, double *x, double *f, int *n1, int *n2,
-                     double *dx, double *dm, double *dft, double *dfw) {
-     double x2 = 2. * LM_PI * c->r2 / c->lambda;
-     int n1m = mie_n1(x2);
-     memset(j, 0, sizeof *j);
-     for (int i = 0; i < c->N; ++i) {
+typedef struct {
+     const core_args *c;
+     double *x, *f, *dx, *dft, *dfw, *sx, *cx;
+     int *n1, *n2;
+} job_ctx;
+
+/* Per-size inputs of the kernels at the nodes i0 .. i1-1: size parameter,
+   weight, orders (src/lmie_core.c:18-38), sin x and cos x from the C library
+   (as the reference's recurrences start from them), and the seed columns. */
+static void job_range(void *vp, int i0, int i1) {
+     const job_ctx *jc = vp;
+     const core_args *c = jc->c;
+     for (int i = i0; i < i1; ++i) {
+          double *x = jc->x;
           x[i] = 2. * LM_PI * c->qx[i] / c->lambda;
-          f[i] = c->nr[i] * c->qw[i];
-          n1[i] = mie_n1(x[i]);
-          n2[i] = mie_n2(n1[i], x[i], c->mr, c->mi);
+          jc->f[i] = c->nr[i] * c->qw[i];
+          jc->n1[i] = mie_n1(x[i]);
+          jc->n2[i] = mie_n2(jc->n1[i], x[i], c->mr, c->mi);
+          jc->sx[i] = sin(x[i]);
+          jc->cx[i] = cos(x[i]);
           for (int t = 0; t < c->nd1; ++t) {
                int d = c->idx1[t];
                size_t o = (size_t) i * c->nd + d;
-               dx[(size_t) t * c->N + i] = 2. * LM_PI * (c->qx_l[o] - c->qx[i] * c->lambda_l[d] / c->lambda) / c->lambda;
-               dft[(size_t) t * c->N + i] = c->nr_l[o] * c->qw[i] + c->nr[i] * c->qw_l[o];
+               jc->dx[(size_t) t * c->N + i] = 2. * LM_PI * (c->qx_l[o] - c->qx[i] * c->lambda_l[d] / c->lambda) / c->lambda;
+               jc->dft[(size_t) t * c->N + i] = c->nr_l[o] * c->qw[i] + c->nr[i] * c->qw_l[o];
           }
           for (int t = 0; t < c->nd2; ++t) {
                size_t o = (size_t) i * c->nd + c->idx2[t];
-               dfw[(size_t) t * c->N + i] = c->nr_l[o] * c->qw[i];
+               jc->dfw[(size_t) t * c->N + i] = c->nr_l[o] * c->qw[i];
           }
      }
+}
+
+static void fill_job(const core_args *c, lmie_gpu_job *j, double *x, double *f, int *n1, int *n2,
+                     double *dx, double *dm, double *dft, double *dfw, double *sx, double *cx) {
+     double x2 = 2. * LM_PI * c->r2 / c->lambda;
+     int n1m = mie_n1(x2);
+     memset(j, 0, sizeof *j);
+     job_ctx jc = {c, x, f, dx, dft, dfw, sx, cx, n1, n2};
+     par_for(c->N, job_range, &jc);
      for (int t = 0; t < c->nd1; ++t) {
           dm[2 * t] = c->mr_l[c->idx1[t]];
           dm[2 * t + 1] = -c->mi_l[c->idx1[t]];
      }
-     j->n_size = c->N; j->x = x; j->f = f; j->n1 = n1; j->n2 = n2;
+     j->n_size = c->N; j->x = x; j->f = f; j->n1 = n1; j->n2 = n2; j->sx = sx; j->cx = cx;
      j->mr = c->mr; j->mi = c->mi;
      j->n1_max = n1m;
      j->n_qang = (c->calc_gc || c->calc_lc) ? 2 * n1m : 0;
@@ -459,12 +576,12 @@ static void fill_job(const core_args *c, lmie_gpu_job *j, double *x, double *f, 
 }
 
 typedef struct {
-     double *x, *f, *dx, *dm, *dft, *dfw;
+     double *x, *f, *dx, *dm, *dft, *dfw, *sx, *cx;
      int *n1, *n2;
 } job_mem;
 
 static void job_mem_alloc(job_mem *m, int N, int nd1, int nd2) {
-     m->x = vec(N); m->f = vec(N);
+     m->x = vec(N); m->f = vec(N); m->sx = vec(N); m->cx = vec(N);
      m->n1 = calloc(N, sizeof(int)); m->n2 = calloc(N, sizeof(int));
      m->dx = vec((size_t) nd1 * N); m->dm = vec(2 * (size_t) nd1); m->dft = vec((size_t) nd1 * N);
      m->dfw = vec((size_t) nd2 * N);
@@ -472,6 +589,7 @@ static void job_mem_alloc(job_mem *m, int N, int nd1, int nd2) {
 
 static void job_mem_free(job_mem *m) {
      free(m->x); free(m->f); free(m->n1); free(m->n2); free(m->dx); free(m->dm); free(m->dft); free(m->dfw);
+     free(m->sx); free(m->cx);
 }
 
 /* Normalise the expansion of channel c of raw into out (one 6 x n_coef set):
@@ -495,7 +613,7 @@ static int core_run(const core_args *c, core_out *o) {
      lmie_gpu_job job;
      lmie_gpu_result res;
      job_mem_alloc(&jm, c->N, c->nd1, c->nd2);
-     fill_job(c, &job, jm.x, jm.f, jm.n1, jm.n2, jm.dx, jm.dm, jm.dft, jm.dfw);
+     fill_job(c, &job, jm.x, jm.f, jm.n1, jm.n2, jm.dx, jm.dm, jm.dft, jm.dfw, jm.sx, jm.cx);
      int C = job.n_coef, Q = job.n_qang;
      o->n_qang = Q; o->n_coef = C;
      memset(&res, 0, sizeof res);
@@ -981,6 +1099,35 @@ void lmie_adjoint_free(lmie_adj_ctx *ctx) {
      free(ctx);
 }
 
+/* Per mode k: w_k qw_i d n_k(r_i) / d a_p, [N][5] (unit seeds on a1..a5). */
+typedef struct {
+     lmie_in_data *const *modes;
+     const double *weight;
+     int n_modes, N;
+     double **dn;
+} dn_job;
+
+static void *dn_build(void *vp) {
+     dn_job *j = vp;
+     const lmie_in_data *m0 = j->modes[0];
+     j->dn = calloc(j->n_modes, sizeof(double *));
+     for (int k = 0; k < j->n_modes; ++k) {
+          const lmie_in_data *m = j->modes[k];
+          const double w = j->weight ? j->weight[k] : 1.;
+          double e[5][5] = {{1, 0, 0, 0, 0}, {0, 1, 0, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 1, 0}, {0, 0, 0, 0, 1}};
+          seeds sv = {5, {e[0], e[1], e[2], e[3], e[4]}, NULL, NULL};
+          double a[5] = {m->a1, m->a2, m->a3, m->a4, m->a5};
+          grid_t gk;
+          memset(&gk, 0, sizeof gk);
+          j->dn[k] = vec((size_t) 5 * j->N);
+          if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0)
+               for (int i = 0; i < gk.N; ++i)
+                    for (int q = 0; q < 5; ++q) j->dn[k][(size_t) i * 5 + q] = w * gk.nr_l[(size_t) i * 5 + q] * gk.qw[i];
+          grid_free(&gk);
+     }
+     return NULL;
+}
+
 int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n_modes,
                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
                          lmie_adj_ctx **ctxp) {
@@ -1001,6 +1148,12 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      int rc = mix_grid(nm, weight, n_modes, 0, &g);
      free(nm); free(copies);
      if (rc) return -1;
+     /* the number density tangents of the modes (for the reverse pass) on a
+        helper thread while the forward runs on the GPU */
+     dn_job dj = {modes, weight, n_modes, g.N, NULL};
+     pthread_t dth;
+     const int dthr = pthread_create(&dth, NULL, dn_build, &dj) == 0;
+     if (!dthr) dn_build(&dj);
      run_flags rf = {m0->calc_gc, m0->calc_lc, 0, 0, 0, m0->lambda, m0->mr, m0->mi, m0->accuracy, 1};
      core_out keep;
      double zero = 0.;
@@ -1009,7 +1162,13 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
                      &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g, out->gc, out->lc,
                      out->theta, out->pf, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
                      NULL, NULL, NULL, NULL, NULL, NULL, NULL, &out->save1, &out->save2, &keep);
-     if (rc) { grid_free(&g); return -1; }
+     if (dthr) pthread_join(dth, NULL);
+     if (rc) {
+          for (int k = 0; k < n_modes; ++k) free(dj.dn[k]);
+          free(dj.dn);
+          grid_free(&g);
+          return -1;
+     }
      lmie_adj_ctx *c = calloc(1, sizeof *c);
      c->state = keep.state;
      c->n_modes = n_modes; c->N = g.N; c->C = keep.n_coef;
@@ -1018,21 +1177,8 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      if (c->calc_gc) { c->gc = vec((size_t) 6 * c->C); memcpy(c->gc, out->gc[0], sizeof(double) * 6 * c->C); }
      if (c->calc_lc) { c->lc = vec((size_t) 6 * c->C); memcpy(c->lc, out->lc[0], sizeof(double) * 6 * c->C); }
      c->w = vec(n_modes);
-     c->dn = calloc(n_modes, sizeof(double *));
-     for (int k = 0; k < n_modes; ++k) {
-          const lmie_in_data *m = modes[k];
-          c->w[k] = weight ? weight[k] : 1.;
-          double e[5][5] = {{1, 0, 0, 0, 0}, {0, 1, 0, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 1, 0}, {0, 0, 0, 0, 1}};
-          seeds sv = {5, {e[0], e[1], e[2], e[3], e[4]}, NULL, NULL};
-          double a[5] = {m->a1, m->a2, m->a3, m->a4, m->a5};
-          grid_t gk;
-          memset(&gk, 0, sizeof gk);
-          c->dn[k] = vec((size_t) 5 * g.N);
-          if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0)
-               for (int i = 0; i < gk.N; ++i)
-                    for (int q = 0; q < 5; ++q) c->dn[k][(size_t) i * 5 + q] = c->w[k] * gk.nr_l[(size_t) i * 5 + q] * gk.qw[i];
-          grid_free(&gk);
-     }
+     for (int k = 0; k < n_modes; ++k) c->w[k] = weight ? weight[k] : 1.;
+     c->dn = dj.dn;
      grid_free(&g);
      *ctxp = c;
      return 0;
