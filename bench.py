@@ -16,11 +16,18 @@ expansion-coefficient sets (weighted L2 against seeded targets), and one
 reverse pass giving dL/d(m_r, m_i, r_g and sigma_g of both modes).  The plain
 FP64 GPU forward of the same workload is timed beside it.  The JSON line reports:
   value        sum_i n1(x_i) over the size points per second (size-pt * n-term / s)
-  roofline     the angular-contraction kernel: algorithmic FLOP per launch
-               (8 n_qang sum_i n1(x_i)) over its device time, measured with HIP
-               events on the library's stream, against the MI355X peak
+  roofline     the dominant kernel of the step, algorithmic FLOP per launch over
+               its device time (HIP events on the library's stream) against the
+               MI355X peak of its arithmetic: the forward angular contraction
+               (8 n_qang sum_i n1(x_i)) or the reverse contraction U = G A of the
+               mixed path (24 sum_i n1(x_i)^2); traffic = HBM bytes per launch
+               from the committed counter summary profiles/traffic.json
+               (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), if present
   cpu_baseline the reference's pthreads CPU path (oracle/_ref) on a bounded
                sample of the same workload, on this host's cores
+
+    python bench.py --calibrate-copy BYTES   one plain device copy of BYTES
+               (calibration point of the traffic counters), then exit
 """
 import argparse
 import ctypes as C
@@ -39,13 +46,18 @@ METRIC = "size-pts*n-terms/s + adjoint grads/s vs plain FP64 GPU at rel err<=1e-
 PEAK = {"fp64": 78.6e12, "bf16": 2.5e15}
 
 
-def sum_n1(p):
-    """sum_i n1(x_i) over the grid of p (Gauss nodes per interval)."""
+def grid_n1(p):
+    """n1(x_i) over the size grid of p (Gauss nodes per interval)."""
     t, _ = np.polynomial.legendre.leggauss(p["n_quad"])
     h = (p["r2"] - p["r1"]) / p["n_int2"]
     r = (p["r1"] + h * (np.arange(p["n_int2"])[:, None] + 0.5) + 0.5 * h * t[None, :]).ravel()
     x = 2 * math.pi * r / p["lambda"]
-    return int(np.sum(np.floor(x + 4.05 * x ** (1.0 / 3.0) + 8)))
+    return np.floor(x + 4.05 * x ** (1.0 / 3.0) + 8)
+
+
+def sum_n1(p):
+    """sum_i n1(x_i) over the grid of p."""
+    return int(np.sum(grid_n1(p)))
 
 
 def cpu_baseline(threads):
@@ -76,6 +88,9 @@ def main():
     ap.add_argument("--path", default="mixed-adjoint",
                     choices=["mixed-adjoint", "fp64-adjoint", "fp64", "mixed", "mixed-bf16"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-compare", action="store_true",
+                    help="omit the separate plain-FP64 run (counter passes)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the size points "
@@ -86,9 +101,19 @@ def main():
     import lmie_amd
     from lmie_amd import lmie, workloads as W
     lib = lmie_amd.load()
+    if a.calibrate_copy:
+        ms = C.c_double()
+        f = lib.lmie_gpu_selftest_copy
+        f.argtypes = [C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_double)]
+        if f(a.calibrate_copy, 5, 0, C.byref(ms)) != 0:
+            raise SystemExit("copy failed: " + lmie.last_error(lib))
+        print(json.dumps({"copy_bytes": a.calibrate_copy, "ms": ms.value,
+                          "GB/s": 2 * a.calibrate_copy / ms.value / 1e6}), flush=True)
+        return
     modes = [(W.cfg4_mode_params(k), None) for k in range(2)]
     weights = [m["weight"] for m in W.CFG4_MODES]
-    units = sum_n1(W.cfg4_mode_params(0))
+    n1s = grid_n1(W.cfg4_mode_params(0))
+    units = int(np.sum(n1s))
     nq = 2 * W.calc_n1(2 * math.pi * W.CFG4_GRID["r2"] / W.CFG4_GRID["lambda"])
 
     # loss targets from one FP64 solution (outside the timed region)
@@ -106,7 +131,8 @@ def main():
             finally:
                 adj.free()
             tm = dict(t_fwd)
-            tm["adj_angular"], tm["adj_coefs"] = t_rev["adj_angular"], t_rev["adj_coefs"]
+            for k in ("adj_angular", "adj_coefs", "adj_gemm"):
+                tm[k] = t_rev[k]
             tm["total"] = t_fwd["total"] + t_rev["total"]
             return tm, grad
         lmie.solve_mix(lib, modes, weights, path)
@@ -128,13 +154,23 @@ def main():
 
     dt, ang_ms, stages, grad = run(a.path, a.steps, a.warmup)
     ref_dt = None
-    if a.path != "fp64":
+    if a.path != "fp64" and not a.no_fp64_compare:
         ref_dt, _, ref_stages, _ = run("fp64", max(2, a.steps // 3), 1)
 
-    flop = 8.0 * nq * units
-    achieved = flop / (ang_ms * 1e-3)
     mixed = a.path.startswith("mixed")
     peak = PEAK["bf16"] if mixed else PEAK["fp64"]
+    # dominant kernel: forward contraction, or the reverse contraction U = G A
+    kern = {"angular_mix2" if mixed else "angular_f64": (8.0 * nq * units, ang_ms)}
+    if mixed and a.path.endswith("-adjoint") and stages.get("adj_gemm", 0) > 0:
+        kern["adj_ugemm"] = (24.0 * float(np.sum(n1s * n1s)), stages["adj_gemm"])
+    kname = max(kern, key=lambda k: kern[k][1])
+    flop, kms = kern[kname]
+    achieved = flop / (kms * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(kname, {}).get("bytes_per_launch")
     line = {
         "metric": METRIC, "value": units / dt, "unit": "size-pt*n-term/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
@@ -150,8 +186,9 @@ def main():
                    "fp64_ms_per_step": None if ref_dt is None else ref_dt * 1e3,
                    "speedup_vs_fp64": None if ref_dt is None else ref_dt / dt},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "kernel": "angular_mix" if mixed else "angular_f64", "kernel_ms": ang_ms},
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                     "kernel": kname, "kernel_ms": kms, "flop_per_launch": flop,
+                     "kernels": {k: {"ms": v[1], "TFLOP/s": v[0] / (v[1] * 1e-3) / 1e12} for k, v in kern.items()}},
         "cpu_baseline": None,
     }
     if not a.no_cpu_baseline:

@@ -162,6 +162,7 @@ enum lmie_timer {
      LMIE_T_ADJ_ANGULAR,   /* adjoint angular contraction                  */
      LMIE_T_ADJ_COEFS,     /* adjoint per-size recurrences                 */
      LMIE_T_TOTAL,         /* whole device pipeline of the call            */
+     LMIE_T_ADJ_GEMM,      /* reverse contraction kernel alone (mixed)     */
      LMIE_N_TIMERS
 };
 int lmie_set_timing(int on);

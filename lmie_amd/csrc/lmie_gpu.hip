@@ -2311,6 +2311,7 @@ struct Workspace {
      hipStream_t side = nullptr;     // angle tables, concurrent with the per-size recurrences
      hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
      std::vector<double> hguess;     // host staging of the Gauss root guesses
+     std::vector<int> hord;          // host staging of the size-tile order
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
      Buf x, sx, cx, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
      Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2, order;
@@ -2694,14 +2695,16 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                tot = (size_t) plan[0].nblk * plan[0].nwp * 4 * Q;
           }
           if (w->part.get(sizeof(double) * tot)) return -1;
-          tm.mark(LMIE_T_ANGULAR, 0);
           if (mixed) {
                // size tiles by decreasing contraction depth (ties by index)
-               std::vector<int> ord(nt64);
-               for (int t = 0; t < nt64; ++t) ord[t] = t;
-               std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return kt64[a] > kt64[b]; });
+               w->hord.resize(nt64);
+               for (int t = 0; t < nt64; ++t) w->hord[t] = t;
+               std::stable_sort(w->hord.begin(), w->hord.end(), [&](int a, int b) { return kt64[a] > kt64[b]; });
                if (w->order.get(sizeof(int) * nt64)) return -1;
-               HIPCHECK(hipMemcpyAsync(w->order.p, ord.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
+               HIPCHECK(hipMemcpyAsync(w->order.p, w->hord.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
+          }
+          tm.mark(LMIE_T_ANGULAR, 0);
+          if (mixed) {
                Mix2Args m;
                memset(&m, 0, sizeof m);
                m.Np = Np; m.Kb = Kb; m.Hb = Hb; m.H = H; m.Q = Q;
@@ -2821,7 +2824,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
 
 // Mixed reverse contraction: weights, split tables, G, its split operand and
 // U = G A (Uf, [4][Kq][Np]).
-int adjoint_mixed_angular(Workspace *w, hipStream_t s) {
+int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
      const int Np = w->Np, Hp = w->Hp, H = w->H, Q = w->Q, n1max = w->n1max;
      const int Kq = rup(n1max, 64), Hq = rup(H, KM), nt64 = Np / 64;
      if (rup(n1max, KM) > w->Kb * 8) return set_err("internal: split operand depth");
@@ -2861,7 +2864,9 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s) {
                              (const unsigned short *) w->R16.as<unsigned short>(), Hq, Kq, w->Gf.as<float>(), sc); \
           hipLaunchKernelGGL(adj_gsplit<E>, dim3(Kq / 64, Kq / 8), dim3(64), 0, s, (const float *) w->Gf.as<float>(), \
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
+          tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
           hipLaunchKernelGGL(adj_ugemm<E>, dim3((unsigned) task.size()), dim3(256), 0, s, u);                   \
+          tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);
      else LMIE_ADJ(F16);
@@ -2897,7 +2902,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
                              nck, (const double *) w->w.as<double>(), H, Hp, Q, w->pfbar.as<double>());
      }
      if (w->mixed && Q > 0 && (use_gc || use_lc)) {
-          if (adjoint_mixed_angular(w, s)) return -1;
+          if (adjoint_mixed_angular(w, s, tm)) return -1;
           tm.mark(LMIE_T_ADJ_ANGULAR, 1);
      } else if (Q > 0 && (use_gc || use_lc)) {
           hipLaunchKernelGGL(fbar_angular, dim3((N + 3) / 4), dim3(256), 0, s, (const double *) w->sst.as<double>(),
@@ -2956,9 +2961,16 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           out->m_bar[0] = mb[0];
           out->m_bar[1] = -mb[1];
      }
-     const int ids[] = {LMIE_T_ADJ_ANGULAR, LMIE_T_ADJ_COEFS, LMIE_T_TOTAL};
-     tm.collect(ids, 3);
+     const int ids[] = {LMIE_T_ADJ_ANGULAR, LMIE_T_ADJ_COEFS, LMIE_T_TOTAL, LMIE_T_ADJ_GEMM};
+     tm.collect(ids, use_u ? 4 : 3);
      return 0;
+}
+
+// Plain 16-byte-per-lane copy: the calibration point for HBM traffic counters
+// (n16 elements of 16 bytes read and written once).
+__global__ __launch_bounds__(256) void copy16(const v4u *src, v4u *dst, size_t n16) {
+     for (size_t e = (size_t) blockIdx.x * blockDim.x + threadIdx.x; e < n16; e += (size_t) gridDim.x * blockDim.x)
+          dst[e] = src[e];
 }
 
 // Lane-layout check of the FP64 matrix instruction used by the contraction:
@@ -3100,6 +3112,35 @@ int lmie_gpu_selftest_mfma16(int bf16, const float *A, const float *B, float *D,
      HIPCHECK(hipGetLastError());
      HIPCHECK(hipMemcpy(D, d + 1024, sizeof(float) * 256, hipMemcpyDeviceToHost));
      HIPCHECK(hipFree(d));
+     return 0;
+}
+
+// Copies n_bytes (a multiple of 16) device to device `reps` times with copy16
+// and returns the average kernel time in ms through *ms.
+int lmie_gpu_selftest_copy(size_t n_bytes, int reps, int device, double *ms) {
+     if (n_bytes == 0 || n_bytes % 16 || reps < 1) return set_err("copy size must be a positive multiple of 16");
+     HIPCHECK(hipSetDevice(device));
+     void *a = nullptr, *b = nullptr;
+     hipEvent_t e0, e1;
+     HIPCHECK(hipMalloc(&a, n_bytes));
+     HIPCHECK(hipMalloc(&b, n_bytes));
+     HIPCHECK(hipMemset(a, 1, n_bytes));
+     HIPCHECK(hipEventCreate(&e0));
+     HIPCHECK(hipEventCreate(&e1));
+     const size_t n16 = n_bytes / 16;
+     HIPCHECK(hipEventRecord(e0, 0));
+     for (int r = 0; r < reps; ++r)
+          hipLaunchKernelGGL(copy16, dim3(2048), dim3(256), 0, 0, (const v4u *) a, (v4u *) b, n16);
+     HIPCHECK(hipGetLastError());
+     HIPCHECK(hipEventRecord(e1, 0));
+     HIPCHECK(hipEventSynchronize(e1));
+     float t = 0.f;
+     HIPCHECK(hipEventElapsedTime(&t, e0, e1));
+     if (ms) *ms = t / reps;
+     (void) hipEventDestroy(e0);
+     (void) hipEventDestroy(e1);
+     HIPCHECK(hipFree(a));
+     HIPCHECK(hipFree(b));
      return 0;
 }
 

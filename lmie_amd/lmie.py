@@ -16,7 +16,7 @@ from .abi import LmieIn, LmieOut
 
 PRECISION = {"fp64": 0, "mixed": 1, "mixed-bf16": 2}
 
-TIMERS = ("tables", "coefs", "angular", "reduce", "expand", "adj_angular", "adj_coefs", "total")
+TIMERS = ("tables", "coefs", "angular", "reduce", "expand", "adj_angular", "adj_coefs", "total", "adj_gemm")
 
 
 class LmieOpts(C.Structure):
