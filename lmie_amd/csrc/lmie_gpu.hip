@@ -147,6 +147,19 @@ __device__ __forceinline__ cd crecip_fast(cd b) {
      const double q = 1. / (b.r * b.r + b.i * b.i);
      return cd{b.r * q, -b.i * q};
 }
+// 1/q from the hardware reciprocal and two Newton steps (about 1 ulp; no
+// special-value handling), for the dependence chains of the recurrences
+__device__ __forceinline__ double rcp_nr(double q) {
+     double r = __builtin_amdgcn_rcp(q);
+     double e = __builtin_fma(-q, r, 1.);
+     r = __builtin_fma(r, e, r);
+     e = __builtin_fma(-q, r, 1.);
+     return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ cd crecip_nr(cd b) {
+     const double q = rcp_nr(b.r * b.r + b.i * b.i);
+     return cd{b.r * q, -b.i * q};
+}
 
 // ---------------------------------------------------------------------------
 // Gauss-Legendre angles: one thread per root.  Newton steps with the
@@ -331,7 +344,6 @@ struct CoefArgs {
      double *S;                 // series [4][Np]: ext, sca, |c_bak|^2, asym
      double *St[4];             // tangent series of this pass, each [4][Np]
      int write_base;            // write A and S of the base channel
-     double *AB;                // reverse-pass checkpoint: a, b [4 planes re/im][Kp][Np] or NULL
      double *CB;                // reverse-pass checkpoint: c_bak series [2][Np] or NULL
 };
 
@@ -508,13 +520,6 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
                dchi1[t] = dchi2;
                dpsi0[t] = dpsi;
           }
-          if (p.AB) {
-               const size_t o = (size_t) (j - 1) * Np + i;
-               p.AB[o] = a.r;
-               p.AB[plane + o] = a.i;
-               p.AB[2 * plane + o] = b.r;
-               p.AB[3 * plane + o] = b.i;
-          }
           ap = a;
           bp = b;
           sgn = -sgn;
@@ -578,12 +583,13 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
      const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
      double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
      // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0; one
-     // division per step (|D_{j+1} + (j+1)/z| stays between about 1/|z| and n2/|z|)
+     // reciprocal per step (|D_{j+1} + (j+1)/z| stays between about 1/|z| and
+     // n2/|z|), the recurrence damps its last-bit differences
      cd D = cmk(0., 0.);
      for (int j = n2 - 1; j >= 1; --j) {
           const cd c1 = cscale(iz, j + 1.);
           const cd c2 = cadd(D, c1);
-          D = csub(c1, crecip_fast(c2));
+          D = csub(c1, crecip_nr(c2));
           if (j <= dtop) {
                Dre[j * Np + i] = D.r;
                Dim[j * Np + i] = D.i;
@@ -627,12 +633,12 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
 }
 
 // a_j, b_j (src/lmie_core.c:557-583), operations as mie_coefs
-__device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd Dj, double psi, double psi0, double chi1,
+__device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, double psi, double psi0, double chi1,
                                           double chi0, cd *a, cd *b) {
 #pragma clang fp contract(off)
      const double jx = j / x;
      const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
-     const cd u = cadd(cdiv(Dj, m), cmk(jx, 0.));
+     const cd u = cadd(cmul(Dj, im), cmk(jx, 0.));   // D/m with im = 1/m
      const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
      const cd na = csub(cscale(u, psi), cmk(psi0, 0.)), da = csub(cmul(u, zj), zjm);
      const cd nb = csub(cscale(v, psi), cmk(psi0, 0.)), db = csub(cmul(v, zj), zjm);
@@ -649,7 +655,6 @@ struct TermArgs {
      const double *D, *PC;      // from mie_chains
      double *A;                 // FP64 planes [4][Kp][Np] (E = void)
      unsigned short *A16;       // split planes [8][Kb][Np][8] (E = F16 / BF16)
-     double *AB;                // a, b [4][Kp][Np] or NULL
      double *part;              // [5][ng][Np]: se, ss, cb re, cb im, sg of each group
 };
 
@@ -669,16 +674,16 @@ __global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
      unsigned short buf[8][8];
      if (j0 <= j1) {
           const double x = p.x[i];
-          const cd m = cmk(p.mr, -p.mi);
+          const cd m = cmk(p.mr, -p.mi), im = crecip(m);
           const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
           cd ap = cmk(0., 0.), bp = cmk(0., 0.);
           if (j0 > 1)
-               mie_ab_at(j0 - 1, x, m, cmk(Dre[(j0 - 1) * Np + i], Dim[(j0 - 1) * Np + i]), Psi[(j0 - 1) * Np + i],
+               mie_ab_at(j0 - 1, x, m, im, cmk(Dre[(j0 - 1) * Np + i], Dim[(j0 - 1) * Np + i]), Psi[(j0 - 1) * Np + i],
                          Psi[(j0 - 2) * Np + i], Chi[(j0 - 1) * Np + i], Chi[(j0 - 2) * Np + i], &ap, &bp);
           double sgn = (j0 & 1) ? -1. : 1.;
           for (int j = j0; j <= j1; ++j) {
                cd a, b;
-               mie_ab_at(j, x, m, cmk(Dre[j * Np + i], Dim[j * Np + i]), Psi[j * Np + i], Psi[(j - 1) * Np + i],
+               mie_ab_at(j, x, m, im, cmk(Dre[j * Np + i], Dim[j * Np + i]), Psi[j * Np + i], Psi[(j - 1) * Np + i],
                          Chi[j * Np + i], Chi[(j - 1) * Np + i], &a, &b);
                const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
                const double fv3m = j > 1 ? (j + 1.) / ((double) j / (j - 1.)) : 0.;
@@ -701,12 +706,6 @@ __global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
                     p.A[plane + o] = fv4 * (a.i + b.i);
                     p.A[2 * plane + o] = fv4 * (a.r - b.r);
                     p.A[3 * plane + o] = fv4 * (a.i - b.i);
-               }
-               if (p.AB) {
-                    p.AB[o] = a.r;
-                    p.AB[plane + o] = a.i;
-                    p.AB[2 * plane + o] = b.r;
-                    p.AB[3 * plane + o] = b.i;
                }
                ap = a;
                bp = b;
@@ -1215,39 +1214,60 @@ __global__ __launch_bounds__(256) void angular_mix(MixArgs p) {
 }
 
 
+// Staging of one LDS stage of a blocked split operand: plane-major rows of
+// `cols` 16-byte columns, [plane][group of 8][column], `rows` = planes x G rows
+// per stage.  Thread tid of NT copies elements tid + NT t; with R = NT / cols
+// rows per step, element t of thread tid sits at the thread's base address
+// plus a compile-time offset in (plane, group), and each stage adds one
+// uniform stride, so no address arithmetic runs inside the stage loop.
+template <int NT, int COLS, int G>
+struct Stager {
+     static constexpr int R = NT / COLS;
+     static_assert(NT % COLS == 0 && (R % G == 0 || G % R == 0), "staging shape");
+     const unsigned short *base;    // thread's element 0 at stage 0
+     size_t plane, row;             // strides in unsigned shorts: plane, group of 8
+     __device__ Stager(const unsigned short *src, size_t plane_, size_t row_, int col0, int tid)
+         : plane(plane_), row(row_) {
+          const int r = tid / COLS, col = tid % COLS;
+          base = src + (size_t) (r / G) * plane + (size_t) (r % G) * row + (size_t) (col0 + col) * 8;
+     }
+     __device__ __forceinline__ const v4u *at(int t, int stage) const {
+          return (const v4u *) (base + (size_t) ((R * t) / G) * plane + (size_t) ((R * t) % G) * row +
+                                (size_t) stage * G * row);
+     }
+};
+
 // The split 16-bit contraction, one workgroup per (size tile of 64, angle tile
 // of 32 WN) pair, two LDS stages: the next 32 orders of both operands are read
 // from HBM into registers while the matrix cores work on the current stage,
 // then written to the other stage (one barrier per stage).  Four waves in a
 // 2 x 2 grid, each 32 sizes x 16 WN angles, 8 accumulators per element as
-// angular_mix.  Workgroups are numbered so that the k-th workgroup of every XCD
-// (hardware dispatches linear ids round robin over the 8 XCDs) works on the
-// size tiles order[8 q + xcd], heaviest first, with all angle tiles of one size
-// tile adjacent: the A tile is read into one XCD's L2 once.  Partials per size
-// tile: part[tile][NW][4][Q].
+// angular_mix.  The (size tile, angle tile) of each workgroup comes from a
+// host-built list dealt over the XCDs (xcd_tasks).  Partials per size tile:
+// part[tile][NW][4][Q].
 struct Mix2Args {
-     int Np, Kb, Hb, H, Q, nT, nJ;
+     int Np, Kb, Hb, H, Q;
      const unsigned short *A16;     // [8][Kb][Np][8]
      const unsigned short *B16;     // [4][Kb][Hb][8]
      const int *ktile;              // orders per size tile of 64 (multiple of KM)
-     const int *order;              // size tiles by decreasing ktile
+     const int2 *task;              // (size tile, angle tile) per workgroup, (-1, -1) = none
      const double *W[4];            // weight channels [Np]
      double *part;                  // [nT][NW][4][Q]
      double *sst;                   // reverse-pass checkpoint [Np][Hb][8] or NULL
 };
 
-template <class E, int WN, int NW>
-__global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
-     constexpr int TM = 64, TN = 32 * WN, G = KM / 8;
-     constexpr int NA = 8 * G * TM / 256, NB = 4 * G * TN / 256;   // staged v4u per thread
+template <class E, int WB, int NWV, int NW>
+__global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
+     constexpr int NT = 128 * NWV, TM = 64, TN = 16 * WB * NWV, G = KM / 8;
+     constexpr int NA = 8 * G * TM / NT, NB = 4 * G * TN / NT;   // staged v4u per thread
+     static_assert(NA * NT == 8 * G * TM && NB * NT == 4 * G * TN, "staging split");
      __shared__ v4u sA[2][8 * G][TM];
      __shared__ v4u sB[2][4 * G][TN];
      static_assert(sizeof(sA) >= sizeof(double) * 2 * NW * 8 * TN, "epilogue buffer");
 
-     const int L = blockIdx.x, xcd = L & 7, r = L >> 3;
-     const int q = r / p.nJ, jt = r - q * p.nJ, kk = 8 * q + xcd;
-     if (kk >= p.nT) return;
-     const int st = p.order[kk];
+     const int2 tk = p.task[blockIdx.x];
+     if (tk.x < 0) return;
+     const int st = tk.x, jt = tk.y;
      const int it0 = st * TM, jt0 = jt * TN, kmax = p.ktile[st], nch = kmax / KM;
 
      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1258,37 +1278,32 @@ __global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
      const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
 
      v4u ra[NA], rb[NB];
+     const Stager<NT, TM, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
+     const Stager<NT, TN, G> stB(p.B16, pB, Hb * 8, jt0, threadIdx.x);
      auto load = [&](int c) {
-          const int g0 = c * G;
 #pragma unroll
-          for (int t = 0; t < NA; ++t) {
-               const int e = threadIdx.x + 256 * t, row = e / TM, col = e % TM, pl = row / G, g = row % G;
-               ra[t] = *(const v4u *) (p.A16 + pl * pA + ((size_t) (g0 + g) * Np + it0 + col) * 8);
-          }
+          for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
 #pragma unroll
-          for (int t = 0; t < NB; ++t) {
-               const int e = threadIdx.x + 256 * t, row = e / TN, col = e % TN, pl = row / G, g = row % G;
-               rb[t] = *(const v4u *) (p.B16 + pl * pB + ((size_t) (g0 + g) * Hb + jt0 + col) * 8);
-          }
+          for (int t = 0; t < NB; ++t) rb[t] = *stB.at(t, c);
      };
      auto store = [&](int s) {
 #pragma unroll
           for (int t = 0; t < NA; ++t) {
-               const int e = threadIdx.x + 256 * t;
+               const int e = threadIdx.x + NT * t;
                sA[s][e / TM][e % TM] = ra[t];
           }
 #pragma unroll
           for (int t = 0; t < NB; ++t) {
-               const int e = threadIdx.x + 256 * t;
+               const int e = threadIdx.x + NT * t;
                sB[s][e / TN][e % TN] = rb[t];
           }
      };
 
-     v4f acc[2][WN][8];
+     v4f acc[2][WB][8];
 #pragma unroll
      for (int a = 0; a < 2; ++a)
 #pragma unroll
-          for (int b = 0; b < WN; ++b)
+          for (int b = 0; b < WB; ++b)
 #pragma unroll
                for (int k = 0; k < 8; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
 
@@ -1300,10 +1315,10 @@ __global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
           const int s = c & 1;
           __syncthreads();
           if (c + 1 < nch) load(c + 1);
-          v4u bph[WN], bpl[WN], bmh[WN], bml[WN];
+          v4u bph[WB], bpl[WB], bmh[WB], bml[WB];
 #pragma unroll
-          for (int b = 0; b < WN; ++b) {
-               const int jc = wn * 16 * WN + 16 * b + l16;
+          for (int b = 0; b < WB; ++b) {
+               const int jc = wn * 16 * WB + 16 * b + l16;
                bph[b] = sB[s][0 * G + lg][jc];
                bpl[b] = sB[s][1 * G + lg][jc];
                bmh[b] = sB[s][2 * G + lg][jc];
@@ -1319,7 +1334,7 @@ __global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
                     y[k] = x[k] ^ flip;
                }
 #pragma unroll
-               for (int b = 0; b < WN; ++b) {
+               for (int b = 0; b < WB; ++b) {
                     v4f *cc = acc[a][b];
                     // sp+ re, im = S (pi+tau); sm+ = T (pi-tau); sp- = s S (pi-tau); sm- = s T (pi+tau)
                     cc[0] = E::mfma(x[0], bph[b], cc[0]); cc[1] = E::mfma(x[2], bph[b], cc[1]);
@@ -1345,8 +1360,8 @@ __global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
      double *sR = (double *) &sA[0][0][0];   // [2][NW][8][TN]
      __syncthreads();                        // every wave is done with the stages
 #pragma unroll
-     for (int b = 0; b < WN; ++b) {
-          const int jl = wn * 16 * WN + 16 * b + l16, j = jt0 + jl;
+     for (int b = 0; b < WB; ++b) {
+          const int jl = wn * 16 * WB + 16 * b + l16, j = jt0 + jl;
           double ew[NW][8];
 #pragma unroll
           for (int c = 0; c < NW; ++c)
@@ -1388,7 +1403,7 @@ __global__ __launch_bounds__(256) void angular_mix2(Mix2Args p) {
      }
      __syncthreads();
      double *out = p.part + (size_t) st * NW * 4 * p.Q;
-     for (int e = threadIdx.x; e < NW * 8 * TN; e += 256) {
+     for (int e = threadIdx.x; e < NW * 8 * TN; e += NT) {
           const int jl = e % TN, ck = e / TN, c = ck / 8, k = ck % 8, j = jt0 + jl;
           if (j >= p.H) continue;
           const double v = sR[(c * 8 + k) * TN + jl] + sR[((NW + c) * 8 + k) * TN + jl];
@@ -1711,17 +1726,13 @@ __global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, con
      const int wa = wave & 1, wb = wave >> 1, l16 = lane & 15, lg = lane >> 4;
      const size_t plane = (size_t) (Hq / 8) * Kq * 8;
      v4u rl[NL], rr[NR];
+     const Stager<256, T, G> stL(L16, plane, (size_t) Kq * 8, n0, threadIdx.x);
+     const Stager<256, T, G> stR(R16, plane, (size_t) Kq * 8, m0, threadIdx.x);
      auto load = [&](int c) {
 #pragma unroll
-          for (int t = 0; t < NL; ++t) {
-               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
-               rl[t] = *(const v4u *) (L16 + pl * plane + ((size_t) (c * G + g) * Kq + n0 + col) * 8);
-          }
+          for (int t = 0; t < NL; ++t) rl[t] = *stL.at(t, c);
 #pragma unroll
-          for (int t = 0; t < NR; ++t) {
-               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
-               rr[t] = *(const v4u *) (R16 + pl * plane + ((size_t) (c * G + g) * Kq + m0 + col) * 8);
-          }
+          for (int t = 0; t < NR; ++t) rr[t] = *stR.at(t, c);
      };
      auto store = [&](int s) {
 #pragma unroll
@@ -1850,6 +1861,7 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
      __shared__ v4u sA[2][8 * G][T];
      __shared__ v4u sG[2][12 * G][T];
      const int2 tk = p.task[blockIdx.x];
+     if (tk.x < 0) return;
      const int it0 = tk.x * T, n0 = tk.y * T, nch = p.ktile[tk.x] / KM;
      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
      const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
@@ -1857,17 +1869,13 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
      const size_t pA = (size_t) p.Kb * Np * 8, pG = (Kq / 8) * Kq * 8;
      const v4u neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
      v4u ra[NA], rg[NG];
+     const Stager<256, T, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
+     const Stager<256, T, G> stG(p.G16, pG, Kq * 8, n0, threadIdx.x);
      auto load = [&](int c) {
 #pragma unroll
-          for (int t = 0; t < NA; ++t) {
-               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
-               ra[t] = *(const v4u *) (p.A16 + pl * pA + ((size_t) (c * G + g) * Np + it0 + col) * 8);
-          }
+          for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
 #pragma unroll
-          for (int t = 0; t < NG; ++t) {
-               const int e = threadIdx.x + 256 * t, row = e / T, col = e % T, pl = row / G, g = row % G;
-               rg[t] = *(const v4u *) (p.G16 + pl * pG + ((size_t) (c * G + g) * Kq + n0 + col) * 8);
-          }
+          for (int t = 0; t < NG; ++t) rg[t] = *stG.at(t, c);
      };
      auto store = [&](int s) {
 #pragma unroll
@@ -1960,7 +1968,6 @@ struct RevArgs {
      const int *n1, *n2;
      double mr, mi;
      const double *D, *PC;      // forward recurrences (mie_chains, D up to n2 - 1)
-     const double *AB;          // [4][Kp][Np]
      const double *CB;          // [2][Np]
      const double *S;           // series [4][Np]
      const double *bar;         // [4][Kp][Np] f-weighted cotangents (FP64 path), or
@@ -1985,18 +1992,26 @@ __global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
      double fq = 0.;
      if (j0 <= j1) {
           const double x = p.x[i], f = p.f[i];
-          const cd m = cmk(p.mr, -p.mi), m2 = cmul(m, m), im1 = crecip(m);
+          const cd m = cmk(p.mr, -p.mi), m2 = cmul(m, m), im1 = crecip(m), im2 = crecip(m2);
           const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
-          auto AB = [&](int q, int n) { return p.AB[q * plane + (size_t) (n - 1) * Np + i]; };
+          // a_n, b_n recomputed from the kept recurrences by the forward's own
+          // code (the same bits), one order ahead for the asymmetry series
+          auto ab_at = [&](int n, cd *a, cd *b) {
+               mie_ab_at(n, x, m, im1, cmk(Dre[(size_t) n * Np + i], Dim[(size_t) n * Np + i]), Psi[(size_t) n * Np + i],
+                         Psi[(size_t) (n - 1) * Np + i], Chi[(size_t) n * Np + i], Chi[(size_t) (n - 1) * Np + i], a, b);
+          };
+          cd am = cmk(0., 0.), bm = cmk(0., 0.), a, b, an = cmk(0., 0.), bn = cmk(0., 0.);
+          if (j0 > 1) ab_at(j0 - 1, &am, &bm);
+          ab_at(j0, &a, &b);
           const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
           const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
           double sgn = (j0 & 1) ? -1. : 1.;
           for (int j = j0; j <= j1; ++j) {
+               if (j < n1) ab_at(j + 1, &an, &bn);
                const double psi = Psi[j * Np + i], psi0 = Psi[(j - 1) * Np + i];
                const cd zj = cmk(psi, Chi[j * Np + i]), zjm = cmk(psi0, Chi[(j - 1) * Np + i]);
                const cd Dj = cmk(Dre[j * Np + i], Dim[j * Np + i]);
                const double jx = j / x;
-               const cd a = cmk(AB(0, j), AB(1, j)), b = cmk(AB(2, j), AB(3, j));
                const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
                cd Sb, Tb;
                if (p.Uf) {
@@ -2021,18 +2036,16 @@ __global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
                bb = csub(bb, cscale(cbv, 2. * sb2 * fv1 * sgn));
                if (j < n1) {
                     const double fv3 = (j + 2.) / ((j + 1.) / j);
-                    const cd an = cmk(AB(0, j + 1), AB(1, j + 1)), bn = cmk(AB(2, j + 1), AB(3, j + 1));
                     ab = cadd(ab, cadd(cscale(an, sb3 * fv3), cscale(b, sb3 * fv4)));
                     bb = cadd(bb, cadd(cscale(bn, sb3 * fv3), cscale(a, sb3 * fv4)));
                }
                if (j > 1) {
                     const double fv3m = (j + 1.) / ((double) j / (j - 1.));
-                    const cd am = cmk(AB(0, j - 1), AB(1, j - 1)), bm = cmk(AB(2, j - 1), AB(3, j - 1));
                     ab = cadd(ab, cscale(am, sb3 * fv3m));
                     bb = cadd(bb, cscale(bm, sb3 * fv3m));
                }
                // a = (u psi - psi0) / (u zeta - zeta0), da/du = (psi - a zeta) / den
-               const cd u = cadd(cdiv(Dj, m), cmk(jx, 0.));
+               const cd u = cadd(cmul(Dj, im1), cmk(jx, 0.));
                const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
                const cd dua = cdiv(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
                const cd dvb = cdiv(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
@@ -2041,8 +2054,12 @@ __global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
                const cd lc = cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb));
                p.Lc[(size_t) j * Np + i] = lc.r;
                p.Lc[ppl + (size_t) j * Np + i] = lc.i;
-               mb = cadd(mb, cadd(cmul(cconj(cscale(cdiv(Dj, m2), -1.)), ub), cmul(cconj(Dj), vb)));
+               mb = cadd(mb, cadd(cmul(cconj(cscale(cmul(Dj, im2), -1.)), ub), cmul(cconj(Dj), vb)));
                sgn = -sgn;
+               am = a;
+               bm = b;
+               a = an;
+               b = bn;
           }
      }
      double *pt = p.rpart + (size_t) g * Np + i;
@@ -2077,7 +2094,7 @@ __global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
                const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
                const cd c1 = cscale(iz, j + 1.);
                const cd c2 = cadd(Dn, c1);
-               const cd rc = crecip_fast(c2);
+               const cd rc = crecip_nr(c2);
                const cd ic22 = cmul(rc, rc);
                const cd dc1dz = cscale(cmul(c1, iz), -1.);
                g[e] = cconj(ic22);
@@ -2311,13 +2328,13 @@ struct Workspace {
      hipStream_t side = nullptr;     // angle tables, concurrent with the per-size recurrences
      hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
      std::vector<double> hguess;     // host staging of the Gauss root guesses
-     std::vector<int> hord;          // host staging of the size-tile order
+     std::vector<int2> htask;        // host staging of the contraction task list
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
      Buf x, sx, cx, f, n1, n2, kfill, ktile32, ktile64, dx, dm, dft, dfw;
      Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2, order;
      std::vector<double> hsc;       // host staging of sin x, cos x
      // reverse-pass checkpoint and buffers
-     Buf AB, CB, sst, pfbar, bar, fang, mbar, fbar, gbar, ones, expart;
+     Buf CB, sst, pfbar, bar, fang, mbar, fbar, gbar, ones, expart;
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
      Buf chD, chPC, chPart, revLc, revPart;         // per-size recurrences of the value channel
      int n2rows = 0, ng = 0;
@@ -2422,6 +2439,63 @@ int launch_coefs(int nt, const CoefArgs &c, hipStream_t s) {
      return 0;
 }
 
+// Task list of a tiled contraction over (size tile, column tile) pairs for
+// the XCD layout of the chip: the hardware deals workgroup ids round robin
+// over the 8 XCDs, so id 8k + x is the k-th workgroup of XCD x.  Size tiles go
+// heaviest first (ktile descending) to the XCDs in turn; inside an XCD they
+// are taken in groups of `group`, and a group runs column tile by column tile.
+// The workgroups in flight on one XCD then share a column tile's operand
+// (angle or order tables) across the group and a size tile's operand across
+// the column tiles, both from that XCD's L2.  ncol(s) = column tiles of size
+// tile s.  Padding entries are (-1, -1).
+template <class F>
+void xcd_tasks(const std::vector<int> &kt, F ncol, int group, std::vector<int2> &out) {
+     const int nT = (int) kt.size();
+     std::vector<int> ord(nT);
+     for (int t = 0; t < nT; ++t) ord[t] = t;
+     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return kt[a] > kt[b]; });
+     std::vector<std::vector<int2>> per(8);
+     for (int x = 0; x < 8; ++x) {
+          std::vector<int> mine;
+          for (int k = x; k < nT; k += 8) mine.push_back(ord[k]);
+          for (size_t g0 = 0; g0 < mine.size(); g0 += group) {
+               const size_t g1 = std::min(mine.size(), g0 + group);
+               int cmax = 0;
+               for (size_t k = g0; k < g1; ++k) cmax = std::max(cmax, ncol(mine[k]));
+               for (int c = 0; c < cmax; ++c)
+                    for (size_t k = g0; k < g1; ++k)
+                         if (c < ncol(mine[k])) per[x].push_back(make_int2(mine[k], c));
+          }
+     }
+     size_t len = 0;
+     for (auto &v : per) len = std::max(len, v.size());
+     out.assign(8 * len, make_int2(-1, -1));
+     for (int x = 0; x < 8; ++x)
+          for (size_t k = 0; k < per[x].size(); ++k) out[8 * k + x] = per[x][k];
+}
+
+// Waves per workgroup of the mixed contraction (environment LMIE_MIX_WAVES: 4
+// or 8, default 4; the 8-wave form spills at two waves per SIMD).
+int mix_waves() {
+     static int v = 0;
+     if (!v) {
+          const char *e = getenv("LMIE_MIX_WAVES");
+          v = (e && atoi(e) == 8) ? 8 : 4;
+     }
+     return v;
+}
+
+// Size tiles per group of xcd_tasks (environment LMIE_TILE_GROUP, default 4).
+int tile_group() {
+     static int g = 0;
+     if (!g) {
+          const char *e = getenv("LMIE_TILE_GROUP");
+          g = e ? atoi(e) : 4;
+          if (g < 1 || g > 64) g = 4;
+     }
+     return g;
+}
+
 // Initial guesses of the Gauss-Legendre roots, cos(t + (n-1)/(8 n^3) / tan t)
 // with t = (4 i + 3) pi / (4 n + 2) (misc/rtutil_math.c:93-97), on the host.
 void gauss_guesses(int n, double *g) {
@@ -2453,9 +2527,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      const int bf16 = J->precision == LMIE_PRECISION_MIXED_BF16;
      const int kgran = mixed ? KM : KC;
      const int Kb = rup(n1max, KM) / 8;
-     // angle padding of the split tables: tiles of 128 angles (64 when the
-     // checkpoint for the reverse pass is kept, whose rows are Hp long)
-     const int mixWN = (J->keep && !mixed) ? 2 : 4, Hb = rup(H > 0 ? H : 1, 32 * mixWN);
+     // angle padding of the split tables: tiles of 128 angles
+     const int Hb = rup(H > 0 ? H : 1, 128);
 
      // host-side tables: per-tile contraction depths
      const int nt32 = Np / 32, nt64 = Np / 64;
@@ -2574,7 +2647,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      const int keep = J->keep;
      if (keep) {
           if (nt || J->save_control == 1) return set_err("the reverse-pass checkpoint covers the value channel only");
-          if (w->AB.get(sizeof(double) * 4 * plane) || w->CB.get(sizeof(double) * 2 * Np) ||
+          if (w->CB.get(sizeof(double) * 2 * Np) ||
               (Q > 0 && !mixed && w->sst.get(sizeof(double) * 8 * (size_t) Np * Hp)))
                return -1;
      }
@@ -2601,7 +2674,6 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           ta.mr = J->mr; ta.mi = J->mi; ta.D = w->chD.as<double>(); ta.PC = w->chPC.as<double>();
           ta.A = mixed ? nullptr : w->A.as<double>();
           ta.A16 = mixed ? w->A16.as<unsigned short>() : nullptr;
-          ta.AB = keep ? w->AB.as<double>() : nullptr;
           ta.part = w->chPart.as<double>();
           if (ng > 0) {
                const dim3 grid(Np / 64, ng);
@@ -2695,39 +2767,39 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                tot = (size_t) plan[0].nblk * plan[0].nwp * 4 * Q;
           }
           if (w->part.get(sizeof(double) * tot)) return -1;
+          const int nJ = Hb / 128;
           if (mixed) {
-               // size tiles by decreasing contraction depth (ties by index)
-               w->hord.resize(nt64);
-               for (int t = 0; t < nt64; ++t) w->hord[t] = t;
-               std::stable_sort(w->hord.begin(), w->hord.end(), [&](int a, int b) { return kt64[a] > kt64[b]; });
-               if (w->order.get(sizeof(int) * nt64)) return -1;
-               HIPCHECK(hipMemcpyAsync(w->order.p, w->hord.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
+               xcd_tasks(kt64, [&](int) { return nJ; }, tile_group(), w->htask);
+               if (w->order.get(sizeof(int2) * w->htask.size())) return -1;
+               HIPCHECK(hipMemcpyAsync(w->order.p, w->htask.data(), sizeof(int2) * w->htask.size(),
+                                       hipMemcpyHostToDevice, s));
           }
           tm.mark(LMIE_T_ANGULAR, 0);
           if (mixed) {
                Mix2Args m;
                memset(&m, 0, sizeof m);
                m.Np = Np; m.Kb = Kb; m.Hb = Hb; m.H = H; m.Q = Q;
-               m.nT = nt64; m.nJ = Hb / (32 * mixWN);
                m.A16 = w->A16.as<unsigned short>();
                m.B16 = w->B16.as<unsigned short>();
                m.ktile = w->ktile64.as<int>();
-               m.order = w->order.as<int>();
+               m.task = w->order.as<int2>();
                m.W[0] = w->f.as<double>();
                for (int q = 0; q < nw; ++q) m.W[1 + q] = w->dfw.as<double>() + (size_t) q * Np;
                m.part = w->part.as<double>();
                m.sst = nullptr;
-               const dim3 grid(8 * ((nt64 + 7) / 8) * m.nJ);
-#define LMIE_MIX(E, WNV, NWV) hipLaunchKernelGGL((angular_mix2<E, WNV, NWV>), grid, dim3(256), 0, s, m)
-#define LMIE_MIX_NW(E, WNV)                                                                \
-               do {                                                                        \
-                    if (nw == 0) LMIE_MIX(E, WNV, 1); else if (nw == 1) LMIE_MIX(E, WNV, 2);  \
-                    else if (nw == 2) LMIE_MIX(E, WNV, 3); else LMIE_MIX(E, WNV, 4);          \
+               const dim3 grid((unsigned) w->htask.size());
+               // tile 64 sizes x 128 angles: 4 waves of 32 x 64, or 8 waves of
+               // 32 x 32 (LMIE_MIX_WAVES=8)
+#define LMIE_MIX(E, WB, NWV, NWC) hipLaunchKernelGGL((angular_mix2<E, WB, NWV, NWC>), grid, dim3(128 * NWV), 0, s, m)
+#define LMIE_MIX_NW(E, WB, NWV)                                                                          \
+               do {                                                                                      \
+                    if (nw == 0) LMIE_MIX(E, WB, NWV, 1); else if (nw == 1) LMIE_MIX(E, WB, NWV, 2);    \
+                    else if (nw == 2) LMIE_MIX(E, WB, NWV, 3); else LMIE_MIX(E, WB, NWV, 4);            \
                } while (0)
-               if (bf16) {
-                    if (mixWN == 4) LMIE_MIX_NW(BF16, 4); else LMIE_MIX_NW(BF16, 2);
+               if (mix_waves() == 4) {
+                    if (bf16) LMIE_MIX_NW(BF16, 4, 2); else LMIE_MIX_NW(F16, 4, 2);
                } else {
-                    if (mixWN == 4) LMIE_MIX_NW(F16, 4); else LMIE_MIX_NW(F16, 2);
+                    if (bf16) LMIE_MIX_NW(BF16, 2, 4); else LMIE_MIX_NW(F16, 2, 4);
                }
 #undef LMIE_MIX_NW
 #undef LMIE_MIX
@@ -2834,13 +2906,9 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
          w->Gf.get(sizeof(float) * 4 * (size_t) Kq * Kq) || w->G16.get(sizeof(unsigned short) * 12 * (size_t) Kq * Kq) ||
          w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np))
           return -1;
-     // (size tile, order tile) tasks, deepest size tiles first
-     std::vector<int> ord(nt64);
-     for (int t = 0; t < nt64; ++t) ord[t] = t;
-     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return w->kt64[a] > w->kt64[b]; });
+     // (size tile, order tile of 64) tasks over the XCDs
      std::vector<int2> task;
-     for (int t : ord)
-          for (int n0 = 0; n0 < w->kt64[t]; n0 += 64) task.push_back(make_int2(t, n0 / 64));
+     xcd_tasks(w->kt64, [&](int t) { return (w->kt64[t] + 63) / 64; }, tile_group(), task);
      if (task.empty()) {
           HIPCHECK(hipMemsetAsync(w->Uf.p, 0, sizeof(float) * 4 * (size_t) Kq * Np, s));
           return 0;
@@ -2931,7 +2999,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.x = w->x.as<double>(); r.f = w->f.as<double>();
      r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
      r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>();
-     r.AB = w->AB.as<double>(); r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
+     r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
      r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
      r.fang = w->fang.as<double>();
      r.sb0 = in->sums_bar[0]; r.sb1 = in->sums_bar[1]; r.sb2 = in->sums_bar[2]; r.sb3 = in->sums_bar[3];
