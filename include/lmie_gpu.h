@@ -94,12 +94,15 @@ const char *lmie_gpu_last_error(void);
      lmie_gpu_selftest_mfma16   one 16x16x32 16-bit matrix instruction on row-major
                                 float inputs A (16x32), B (32x16) -> D (16x16)
                                 (bf16 = 1: bf16 parts, else fp16)
+     lmie_gpu_selftest_mfma32   one 32x32x16 fp16 matrix instruction, A (32x16), B (16x32)
+                                -> D (32x32), row-major float inputs
      lmie_gpu_selftest_mfma_f64 one 16x16x4 FP64 matrix instruction, A (16x4), B (4x16)
      lmie_gpu_selftest_copy     a plain device copy of n_bytes, reps times: the
                                 calibration point of the HBM traffic counters;
                                 *ms = average kernel time */
 int lmie_gpu_gauss_rule(int n, double *x, double *wt, int device);
 int lmie_gpu_selftest_mfma16(int bf16, const float *A, const float *B, float *D, int device);
+int lmie_gpu_selftest_mfma32(const float *A, const float *B, float *D, int device);
 int lmie_gpu_selftest_mfma_f64(const double *A, const double *B, double *D, int device);
 int lmie_gpu_selftest_copy(size_t n_bytes, int reps, int device, double *ms);
 

@@ -49,6 +49,7 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef _Float16 v8h __attribute__((ext_vector_type(8)));
 typedef __bf16 v8b __attribute__((ext_vector_type(8)));
 typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
 
 // Mixed path scaling: the split operands hold (2n+1)(a_n +- b_n) * kSA and
 // (pi_n +- tau_n) / (n(n+1)) * kSB, so that their product is
@@ -70,6 +71,9 @@ struct F16 {
      static __device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c) {
           return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0, 0);
      }
+     static __device__ __forceinline__ v16f mfma32(v4u a, v4u b, v16f c) {
+          return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0, 0);
+     }
 };
 struct BF16 {
      typedef __bf16 T;
@@ -81,6 +85,9 @@ struct BF16 {
      }
      static __device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c) {
           return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8b, a), __builtin_bit_cast(v8b, b), c, 0, 0, 0);
+     }
+     static __device__ __forceinline__ v16f mfma32(v4u a, v4u b, v16f c) {
+          return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(v8b, a), __builtin_bit_cast(v8b, b), c, 0, 0, 0);
      }
 };
 
@@ -95,6 +102,9 @@ __device__ __forceinline__ void split16(double v, unsigned short *hi, unsigned s
 // Row of the 16x16 f32 result held in lane l, element v, for the 16-bit
 // matrix instructions (checked by lmie_gpu_selftest_mfma16).
 __device__ __forceinline__ int mfma16_row(int l, int v) { return 4 * (l >> 4) + v; }
+// Row of the 32x32 f32 result held in lane l, value v, for the 32x32x16
+// 16-bit instructions (column l % 32; checked by lmie_gpu_selftest_mfma32).
+__device__ __forceinline__ int mfma32_row(int l, int v) { return 8 * (v >> 2) + 4 * (l >> 5) + (v & 3); }
 
 thread_local char g_err[512];
 thread_local double g_times[LMIE_N_TIMERS];
@@ -1307,14 +1317,16 @@ __global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
 #pragma unroll
                for (int k = 0; k < 8; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
 
+     // stage c+1 goes to LDS in the middle of stage c's products and the
+     // loads of stage c+2 follow it, so both overlap the matrix cores
      if (nch > 0) {
           load(0);
           store(0);
+          if (nch > 1) load(1);
      }
      for (int c = 0; c < nch; ++c) {
           const int s = c & 1;
           __syncthreads();
-          if (c + 1 < nch) load(c + 1);
           v4u bph[WB], bpl[WB], bmh[WB], bml[WB];
 #pragma unroll
           for (int b = 0; b < WB; ++b) {
@@ -1327,31 +1339,38 @@ __global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
 #pragma unroll
           for (int a = 0; a < 2; ++a) {
                const int ic = wm * 32 + 16 * a + l16;
-               v4u x[8], y[8];
+               v4u x[8];
 #pragma unroll
-               for (int k = 0; k < 8; ++k) {
-                    x[k] = sA[s][k * G + lg][ic];
-                    y[k] = x[k] ^ flip;
-               }
+               for (int k = 0; k < 8; ++k) x[k] = sA[s][k * G + lg][ic];
+               // +mu: sp = S (pi+tau), sm = T (pi-tau)
 #pragma unroll
                for (int b = 0; b < WB; ++b) {
                     v4f *cc = acc[a][b];
-                    // sp+ re, im = S (pi+tau); sm+ = T (pi-tau); sp- = s S (pi-tau); sm- = s T (pi+tau)
                     cc[0] = E::mfma(x[0], bph[b], cc[0]); cc[1] = E::mfma(x[2], bph[b], cc[1]);
                     cc[2] = E::mfma(x[4], bmh[b], cc[2]); cc[3] = E::mfma(x[6], bmh[b], cc[3]);
-                    cc[4] = E::mfma(y[0], bmh[b], cc[4]); cc[5] = E::mfma(y[2], bmh[b], cc[5]);
-                    cc[6] = E::mfma(y[4], bph[b], cc[6]); cc[7] = E::mfma(y[6], bph[b], cc[7]);
                     cc[0] = E::mfma(x[0], bpl[b], cc[0]); cc[1] = E::mfma(x[2], bpl[b], cc[1]);
                     cc[2] = E::mfma(x[4], bml[b], cc[2]); cc[3] = E::mfma(x[6], bml[b], cc[3]);
-                    cc[4] = E::mfma(y[0], bml[b], cc[4]); cc[5] = E::mfma(y[2], bml[b], cc[5]);
-                    cc[6] = E::mfma(y[4], bpl[b], cc[6]); cc[7] = E::mfma(y[6], bpl[b], cc[7]);
                     cc[0] = E::mfma(x[1], bph[b], cc[0]); cc[1] = E::mfma(x[3], bph[b], cc[1]);
                     cc[2] = E::mfma(x[5], bmh[b], cc[2]); cc[3] = E::mfma(x[7], bmh[b], cc[3]);
-                    cc[4] = E::mfma(y[1], bmh[b], cc[4]); cc[5] = E::mfma(y[3], bmh[b], cc[5]);
-                    cc[6] = E::mfma(y[5], bph[b], cc[6]); cc[7] = E::mfma(y[7], bph[b], cc[7]);
+               }
+               // -mu: the odd orders change sign, sp = s S (pi-tau), sm = s T (pi+tau)
+#pragma unroll
+               for (int k = 0; k < 8; ++k) x[k] ^= flip;
+#pragma unroll
+               for (int b = 0; b < WB; ++b) {
+                    v4f *cc = acc[a][b];
+                    cc[4] = E::mfma(x[0], bmh[b], cc[4]); cc[5] = E::mfma(x[2], bmh[b], cc[5]);
+                    cc[6] = E::mfma(x[4], bph[b], cc[6]); cc[7] = E::mfma(x[6], bph[b], cc[7]);
+                    cc[4] = E::mfma(x[0], bml[b], cc[4]); cc[5] = E::mfma(x[2], bml[b], cc[5]);
+                    cc[6] = E::mfma(x[4], bpl[b], cc[6]); cc[7] = E::mfma(x[6], bpl[b], cc[7]);
+                    cc[4] = E::mfma(x[1], bmh[b], cc[4]); cc[5] = E::mfma(x[3], bmh[b], cc[5]);
+                    cc[6] = E::mfma(x[5], bph[b], cc[6]); cc[7] = E::mfma(x[7], bph[b], cc[7]);
+               }
+               if (a == 0) {
+                    if (c + 1 < nch) store(s ^ 1);
+                    if (c + 2 < nch) load(c + 2);
                }
           }
-          if (c + 1 < nch) store(s ^ 1);
      }
 
      // epilogue: Stokes elements in double, weighted sums over this wave's 32
@@ -1755,10 +1774,10 @@ __global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, con
                for (int k = 0; k < 8; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
      load(0);
      store(0);
+     if (nch > 1) load(1);
      for (int c = 0; c < nch; ++c) {
           const int s = c & 1;
           __syncthreads();
-          if (c + 1 < nch) load(c + 1);
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
                const int mc = wb * 32 + 16 * b + l16;
@@ -1779,8 +1798,11 @@ __global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, con
                     GB(4, 0, 4) GB(5, 2, 6) GB(6, 0, 5) GB(7, 2, 7)
 #undef GB
                }
+               if (b == 0) {
+                    if (c + 1 < nch) store(s ^ 1);
+                    if (c + 2 < nch) load(c + 2);
+               }
           }
-          if (c + 1 < nch) store(s ^ 1);
      }
      float mx = 0.f;
 #pragma unroll
@@ -1899,11 +1921,11 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
      if (nch > 0) {
           load(0);
           store(0);
+          if (nch > 1) load(1);
      }
      for (int c = 0; c < nch; ++c) {
           const int s = c & 1;
           __syncthreads();
-          if (c + 1 < nch) load(c + 1);
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
                const int nc = wn * 32 + 16 * b + l16;
@@ -1930,8 +1952,11 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
                     UG(3, x[6], x[7], 3) UG(3, x[2], x[3], 4) UG(3, nx0, nx1, 5)
 #undef UG
                }
+               if (b == 0) {
+                    if (c + 1 < nch) store(s ^ 1);
+                    if (c + 2 < nch) load(c + 2);
+               }
           }
-          if (c + 1 < nch) store(s ^ 1);
      }
      // U = acc n (n+1) wmax / (2^(14 - k) kSB^2 kSA)
      const double base = p.sc->wmax * ldexp(1., adj_gexp(p.sc) - 14) / (kSB * kSB * kSA);
@@ -1947,6 +1972,116 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
 #pragma unroll
                     for (int k = 0; k < 4; ++k) p.Uf[((size_t) k * Kq + r) * Np + i] = (float) (acc[a][b][k][v] * f);
                }
+}
+
+// U = G A with the 32x32x16 matrix instruction: one workgroup per (size tile
+// of 128, order tile of 64) task, stages of 16 orders in two LDS buffers
+// (112 KiB).  Four waves in a 2 x 2 grid, each 32 orders (matrix rows) x 64
+// sizes (two 32-column blocks), 4 accumulators per element as adj_ugemm; the
+// larger wave tile reads each operand fragment for twice the products.
+template <class E>
+__global__ __launch_bounds__(256) void adj_ugemm32(UArgs p) {
+     constexpr int TS = 128, TO = 64, G = 2;                 // sizes, orders, groups of 8 per stage
+     constexpr int NA = 8 * G * TS / 256, NG = 12 * G * TO / 256;
+     __shared__ v4u sA[2][8 * G][TS];
+     __shared__ v4u sG[2][12 * G][TO];
+     const int2 tk = p.task[blockIdx.x];
+     if (tk.x < 0) return;
+     const int it0 = tk.x * TS, n0 = tk.y * TO, nch = p.ktile[tk.x] / (8 * G);
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wm = wave & 1, wn = wave >> 1, l32 = lane & 31, lh = lane >> 5;
+     const size_t Np = p.Np, Kq = p.Kq;
+     const size_t pA = (size_t) p.Kb * Np * 8, pG = (Kq / 8) * Kq * 8;
+     const v4u neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
+     v4u ra[NA], rg[NG];
+     const Stager<256, TS, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
+     const Stager<256, TO, G> stG(p.G16, pG, Kq * 8, n0, threadIdx.x);
+     auto load = [&](int c) {
+#pragma unroll
+          for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
+#pragma unroll
+          for (int t = 0; t < NG; ++t) rg[t] = *stG.at(t, c);
+     };
+     auto store = [&](int s) {
+#pragma unroll
+          for (int t = 0; t < NA; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sA[s][e / TS][e % TS] = ra[t];
+          }
+#pragma unroll
+          for (int t = 0; t < NG; ++t) {
+               const int e = threadIdx.x + 256 * t;
+               sG[s][e / TO][e % TO] = rg[t];
+          }
+     };
+     v16f acc[2][4];
+#pragma unroll
+     for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+#pragma unroll
+               for (int v = 0; v < 16; ++v) acc[a][k][v] = 0.f;
+     if (nch > 0) {
+          load(0);
+          store(0);
+          if (nch > 1) load(1);
+     }
+     // terms per accumulator: (G plane pair, A plane hi, A plane lo); A planes
+     // 0..7 as A16, 8 = -A+r hi, 9 = -A+r lo, 10 = -A-r hi, 11 = -A-r lo
+     constexpr int term[4][3][3] = {{{0, 0, 1}, {1, 4, 5}, {2, 6, 7}},     // U_S re
+                                    {{0, 2, 3}, {1, 6, 7}, {2, 10, 11}},   // U_S im
+                                    {{3, 4, 5}, {4, 0, 1}, {5, 2, 3}},     // U_T re
+                                    {{3, 6, 7}, {4, 2, 3}, {5, 8, 9}}};    // U_T im
+     for (int c = 0; c < nch; ++c) {
+          const int s = c & 1;
+          __syncthreads();
+          v4u g[12];
+#pragma unroll
+          for (int q = 0; q < 12; ++q) g[q] = sG[s][q * G + lh][wn * 32 + l32];
+          v4u x[2][12];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+#pragma unroll
+               for (int q = 0; q < 8; ++q) x[a][q] = sA[s][q * G + lh][wm * 64 + 32 * a + l32];
+               x[a][8] = x[a][0] ^ neg;
+               x[a][9] = x[a][1] ^ neg;
+               x[a][10] = x[a][4] ^ neg;
+               x[a][11] = x[a][5] ^ neg;
+          }
+          // hi x hi, lo(G) x hi(A), hi(G) x lo(A) for every (term, size block,
+          // accumulator): eight independent accumulators between dependent products
+#pragma unroll
+          for (int t = 0; t < 3; ++t) {
+#pragma unroll
+               for (int pr = 0; pr < 3; ++pr) {
+#pragma unroll
+                    for (int a = 0; a < 2; ++a)
+#pragma unroll
+                         for (int k = 0; k < 4; ++k) {
+                              const int gp = term[k][t][0], xh = term[k][t][1], xl = term[k][t][2];
+                              const v4u gv = g[2 * gp + (pr == 1 ? 1 : 0)];
+                              const v4u xv = x[a][pr == 2 ? xl : xh];
+                              acc[a][k] = E::mfma32(gv, xv, acc[a][k]);
+                         }
+               }
+               if (t == 0) {
+                    if (c + 1 < nch) store(s ^ 1);
+                    if (c + 2 < nch) load(c + 2);
+               }
+          }
+     }
+     // U = acc n (n+1) wmax / (2^(14 - k) kSB^2 kSA); rows are orders
+     const double base = p.sc->wmax * ldexp(1., adj_gexp(p.sc) - 14) / (kSB * kSB * kSA);
+#pragma unroll
+     for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+               const int r = n0 + wn * 32 + mfma32_row(lane, v);
+               const int i = it0 + wm * 64 + 32 * a + l32;
+               const double f = base * (r + 1.) * (r + 2.);
+#pragma unroll
+               for (int k = 0; k < 4; ++k) p.Uf[((size_t) k * Kq + r) * Np + i] = (float) (acc[a][k][v] * f);
+          }
 }
 
 // Per-size reverse sweep (adjoint of src/lmie_core.c:298-611), in two
@@ -2341,7 +2476,8 @@ struct Workspace {
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
      int mixed = 0, bf16 = 0, Kb = 0;
-     std::vector<int> kt64;         // contraction depth per size tile of 64
+     std::vector<int> kt64, kt128;  // contraction depth per size tile of 64, 128
+     Buf ktile128;
      double mr = 0., mi = 0.;
 };
 
@@ -2475,12 +2611,12 @@ void xcd_tasks(const std::vector<int> &kt, F ncol, int group, std::vector<int2> 
 }
 
 // Waves per workgroup of the mixed contraction (environment LMIE_MIX_WAVES: 4
-// or 8, default 4; the 8-wave form spills at two waves per SIMD).
+// or 8, default 8: two waves per SIMD hide each other's LDS and barrier waits).
 int mix_waves() {
      static int v = 0;
      if (!v) {
           const char *e = getenv("LMIE_MIX_WAVES");
-          v = (e && atoi(e) == 8) ? 8 : 4;
+          v = (e && atoi(e) == 4) ? 4 : 8;
      }
      return v;
 }
@@ -2508,7 +2644,7 @@ void gauss_guesses(int n, double *g) {
 }
 
 int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
-     const int N = J->n_size, Np = rup(N, TILE);
+     const int N = J->n_size, Np = rup(N, 2 * TILE);   // whole tiles of 128 sizes
      const int n1max = J->n1_max, Q = J->n_qang, C = J->n_coef;
      const int H = Q / 2, Hp = rup(H > 0 ? H : 1, TILE), Kp = rup(n1max, 32);
      const int nt = J->n_tan, nw = J->n_wgt, nch = 1 + nt + nw;
@@ -2541,8 +2677,13 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      }
      for (int t = 0; t < nt64; ++t) {
           kt64[t] = kt32[2 * t] > kt32[2 * t + 1] ? kt32[2 * t] : kt32[2 * t + 1];
-          kfill[t] = kt64[t];
+          kfill[t] = 0;
      }
+     // operand rows are zero up to the depth of the 128-size tile (the
+     // reverse contraction runs whole 128-size tiles)
+     std::vector<int> kt128(nt64 / 2);
+     for (int t = 0; t < nt64 / 2; ++t) kt128[t] = std::max(kt64[2 * t], kt64[2 * t + 1]);
+     for (int t = 0; t < nt64; ++t) kfill[t] = kt128[t / 2];
      // uploads
      if (w->x.get(sizeof(double) * Np) || w->f.get(sizeof(double) * Np) || w->n1.get(sizeof(int) * Np) ||
          w->n2.get(sizeof(int) * Np) || w->kfill.get(sizeof(int) * nt64) || w->ktile32.get(sizeof(int) * nt32) ||
@@ -2788,8 +2929,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                m.part = w->part.as<double>();
                m.sst = nullptr;
                const dim3 grid((unsigned) w->htask.size());
-               // tile 64 sizes x 128 angles: 4 waves of 32 x 64, or 8 waves of
-               // 32 x 32 (LMIE_MIX_WAVES=8)
+               // tile 64 sizes x 128 angles: 8 waves of 32 x 32, or 4 waves of
+               // 32 x 64 (LMIE_MIX_WAVES=4)
 #define LMIE_MIX(E, WB, NWV, NWC) hipLaunchKernelGGL((angular_mix2<E, WB, NWV, NWC>), grid, dim3(128 * NWV), 0, s, m)
 #define LMIE_MIX_NW(E, WB, NWV)                                                                          \
                do {                                                                                      \
@@ -2890,7 +3031,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      w->n2max = 0;
      for (int i = 0; i < N; ++i) w->n2max = J->n2[i] > w->n2max ? J->n2[i] : w->n2max;
      w->calc_gc = J->calc_gc; w->calc_lc = J->calc_lc; w->mr = J->mr; w->mi = J->mi;
-     w->mixed = mixed; w->bf16 = bf16; w->Kb = Kb; w->kt64 = kt64;
+     w->mixed = mixed; w->bf16 = bf16; w->Kb = Kb; w->kt64 = kt64; w->kt128 = kt128;
      return 0;
 }
 
@@ -2906,9 +3047,11 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
          w->Gf.get(sizeof(float) * 4 * (size_t) Kq * Kq) || w->G16.get(sizeof(unsigned short) * 12 * (size_t) Kq * Kq) ||
          w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np))
           return -1;
-     // (size tile, order tile of 64) tasks over the XCDs
+     // (size tile of 128, order tile of 64) tasks over the XCDs
      std::vector<int2> task;
-     xcd_tasks(w->kt64, [&](int t) { return (w->kt64[t] + 63) / 64; }, tile_group(), task);
+     xcd_tasks(w->kt128, [&](int t) { return (w->kt128[t] + 63) / 64; }, tile_group(), task);
+     if (w->ktile128.get(sizeof(int) * w->kt128.size())) return -1;
+     HIPCHECK(hipMemcpyAsync(w->ktile128.p, w->kt128.data(), sizeof(int) * w->kt128.size(), hipMemcpyHostToDevice, s));
      if (task.empty()) {
           HIPCHECK(hipMemsetAsync(w->Uf.p, 0, sizeof(float) * 4 * (size_t) Kq * Np, s));
           return 0;
@@ -2921,7 +3064,7 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
      UArgs u;
      u.Np = Np; u.Kb = w->Kb; u.Kq = Kq;
      u.A16 = w->A16.as<unsigned short>(); u.G16 = w->G16.as<unsigned short>();
-     u.ktile = w->ktile64.as<int>(); u.task = w->tasks.as<int2>(); u.sc = sc; u.Uf = w->Uf.as<float>();
+     u.ktile = w->ktile128.as<int>(); u.task = w->tasks.as<int2>(); u.sc = sc; u.Uf = w->Uf.as<float>();
 #define LMIE_ADJ(E)                                                                                            \
      do {                                                                                                     \
           hipLaunchKernelGGL(adj_tables<E>, dim3(Kq / 64, Hq / 8), dim3(64), 0, s, (const double *) w->Bp.as<double>(), \
@@ -2933,7 +3076,7 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
           hipLaunchKernelGGL(adj_gsplit<E>, dim3(Kq / 64, Kq / 8), dim3(64), 0, s, (const float *) w->Gf.as<float>(), \
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
           tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
-          hipLaunchKernelGGL(adj_ugemm<E>, dim3((unsigned) task.size()), dim3(256), 0, s, u);                   \
+          hipLaunchKernelGGL(adj_ugemm32<E>, dim3((unsigned) task.size()), dim3(256), 0, s, u);                 \
           tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);
@@ -3050,6 +3193,31 @@ __global__ void mfma_f64_probe(const double *A, const double *B, double *D) {
      v4d c = {0., 0., 0., 0.};
      c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
      for (int v = 0; v < 4; ++v) D[((l >> 4) + 4 * v) * 16 + (l & 15)] = c[v];
+}
+
+// Lane-layout check of the 32x32x16 fp16 matrix instruction: D = A (32x16)
+// B (16x32), row-major float inputs, one wave.  Operand lane l: row (column)
+// l % 32, elements 8 (l / 32) .. +7 of the contraction; result lane l, value v:
+// column l % 32, row given by mfma32_row (checked by lmie_gpu_selftest_mfma32).
+
+
+__global__ void mfma32_probe(const float *A, const float *B, float *D) {
+     const int l = threadIdx.x;
+     unsigned short av[8], bv[8];
+     double r;
+     for (int e = 0; e < 8; ++e) {
+          av[e] = F16::bits(A[(l & 31) * 16 + 8 * (l >> 5) + e], &r);
+          bv[e] = F16::bits(B[(8 * (l >> 5) + e) * 32 + (l & 31)], &r);
+     }
+     v4u a, b;
+     for (int k = 0; k < 4; ++k) {
+          a[k] = (unsigned) av[2 * k] | ((unsigned) av[2 * k + 1] << 16);
+          b[k] = (unsigned) bv[2 * k] | ((unsigned) bv[2 * k + 1] << 16);
+     }
+     v16f c;
+     for (int v = 0; v < 16; ++v) c[v] = 0.f;
+     c = F16::mfma32(a, b, c);
+     for (int v = 0; v < 16; ++v) D[mfma32_row(l, v) * 32 + (l & 31)] = c[v];
 }
 
 // Lane-layout check of the 16-bit matrix instructions used by the mixed
@@ -3209,6 +3377,19 @@ int lmie_gpu_selftest_copy(size_t n_bytes, int reps, int device, double *ms) {
      (void) hipEventDestroy(e1);
      HIPCHECK(hipFree(a));
      HIPCHECK(hipFree(b));
+     return 0;
+}
+
+int lmie_gpu_selftest_mfma32(const float *A, const float *B, float *D, int device) {
+     HIPCHECK(hipSetDevice(device));
+     float *d = nullptr;
+     HIPCHECK(hipMalloc(&d, sizeof(float) * (512 + 512 + 1024)));
+     HIPCHECK(hipMemcpy(d, A, sizeof(float) * 512, hipMemcpyHostToDevice));
+     HIPCHECK(hipMemcpy(d + 512, B, sizeof(float) * 512, hipMemcpyHostToDevice));
+     hipLaunchKernelGGL(mfma32_probe, dim3(1), dim3(64), 0, 0, (const float *) d, (const float *) (d + 512), d + 1024);
+     HIPCHECK(hipGetLastError());
+     HIPCHECK(hipMemcpy(D, d + 1024, sizeof(float) * 1024, hipMemcpyDeviceToHost));
+     HIPCHECK(hipFree(d));
      return 0;
 }
 

@@ -45,6 +45,25 @@ def test_mfma16_lane_layout(product, bf16):
     assert np.array_equal(D, E)
 
 
+def test_mfma32_lane_layout(product):
+    say("case: mfma 32x32x16 f16 lane layout")
+    rng = np.random.Generator(np.random.PCG64(6))
+    A = rng.integers(-8, 9, (32, 16)).astype(np.float32)
+    B = rng.integers(-8, 9, (16, 32)).astype(np.float32)
+    D = np.zeros((32, 32), np.float32)
+    f = product.lmie_gpu_selftest_mfma32
+    f.argtypes = [C.POINTER(C.c_float)] * 3 + [C.c_int]
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    assert f(fp(np.ascontiguousarray(A)), fp(np.ascontiguousarray(B)), fp(D), 0) == 0
+    E = A.astype(np.float64) @ B.astype(np.float64)
+    if not np.array_equal(D, E):
+        # report which row of the result each (lane group, value) holds
+        for r0 in range(32):
+            hits = [r for r in range(32) if np.array_equal(D[r0], E[r])]
+            say("  stored row %d holds rows %s" % (r0, hits))
+    assert np.array_equal(D, E)
+
+
 def errors(mix, ref):
     out = {}
     for k in range(6):
