@@ -170,6 +170,15 @@ __device__ __forceinline__ cd crecip_nr(cd b) {
      const double q = rcp_nr(b.r * b.r + b.i * b.i);
      return cd{b.r * q, -b.i * q};
 }
+// Smith's division with the reciprocals of rcp_nr (about 1 ulp)
+__device__ __forceinline__ cd cdiv_nr(cd a, cd b) {
+     if (fabs(b.r) >= fabs(b.i)) {
+          const double t = b.i * rcp_nr(b.r), rd = rcp_nr(b.r + b.i * t);
+          return cd{(a.r + a.i * t) * rd, (a.i - a.r * t) * rd};
+     }
+     const double t = b.r * rcp_nr(b.i), rd = rcp_nr(b.r * t + b.i);
+     return cd{(a.r * t + a.i) * rd, (a.i * t - a.r) * rd};
+}
 
 // ---------------------------------------------------------------------------
 // Gauss-Legendre angles: one thread per root.  Newton steps with the
@@ -652,8 +661,8 @@ __device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, d
      const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
      const cd na = csub(cscale(u, psi), cmk(psi0, 0.)), da = csub(cmul(u, zj), zjm);
      const cd nb = csub(cscale(v, psi), cmk(psi0, 0.)), db = csub(cmul(v, zj), zjm);
-     *a = cdiv(na, da);
-     *b = cdiv(nb, db);
+     *a = cdiv_nr(na, da);
+     *b = cdiv_nr(nb, db);
 }
 
 struct TermArgs {
@@ -2182,8 +2191,8 @@ __global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
                // a = (u psi - psi0) / (u zeta - zeta0), da/du = (psi - a zeta) / den
                const cd u = cadd(cmul(Dj, im1), cmk(jx, 0.));
                const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
-               const cd dua = cdiv(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
-               const cd dvb = cdiv(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
+               const cd dua = cdiv_nr(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
+               const cd dvb = cdiv_nr(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
                const cd ub = cmul(cconj(dua), ab), vb = cmul(cconj(dvb), bb);
                // u = D/m + j/x, v = m D + j/x
                const cd lc = cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb));
