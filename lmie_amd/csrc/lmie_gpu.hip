@@ -590,6 +590,7 @@ struct ChainArgs {
      int d_all;                 // store D_j up to n2 - 1 (reverse pass) instead of n1
      double *D;                 // [2][n2rows][Np]: D_j re, im at row j
      double *PC;                // [2][n1max + 1][Np]: psi_j, chi_j at row j
+     double *RC;                // [2][n2rows][Np]: 1 / (D_{j+1} + (j+1)/z) at row j (reverse pass) or NULL
 };
 
 __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
@@ -603,22 +604,26 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
      double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
      // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0; one
      // reciprocal per step (|D_{j+1} + (j+1)/z| stays between about 1/|z| and
-     // n2/|z|), the recurrence damps its last-bit differences
+     // n2/|z|), the recurrence damps its last-bit differences.  In the same
+     // loop, as an independent chain, the downward ratio
+     // r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows.
      cd D = cmk(0., 0.);
+     double r = 0.;
+     Psi[n1 * Np + i] = 0.;
      for (int j = n2 - 1; j >= 1; --j) {
           const cd c1 = cscale(iz, j + 1.);
           const cd c2 = cadd(D, c1);
-          D = csub(c1, crecip_nr(c2));
+          const cd rc = crecip_nr(c2);
+          D = csub(c1, rc);
           if (j <= dtop) {
                Dre[j * Np + i] = D.r;
                Dim[j * Np + i] = D.i;
           }
-     }
-     // downward ratio r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows
-     {
-          double r = 0.;
-          Psi[n1 * Np + i] = 0.;
-          for (int j = n1 - 1; j >= 1; --j) {
+          if (p.RC) {
+               p.RC[(size_t) j * Np + i] = rc.r;
+               p.RC[dpl + (size_t) j * Np + i] = rc.i;
+          }
+          if (j < n1) {
                const double c = (2. * j + 1.) / x;
                r = 1. / (c - r);
                Psi[j * Np + i] = r;
@@ -2112,6 +2117,7 @@ struct RevArgs {
      const int *n1, *n2;
      double mr, mi;
      const double *D, *PC;      // forward recurrences (mie_chains, D up to n2 - 1)
+     const double *RC;          // forward reciprocals 1 / (D_{j+1} + (j+1)/z), rows 1 .. n2 - 1
      const double *CB;          // [2][Np]
      const double *S;           // series [4][Np]
      const double *bar;         // [4][Kp][Np] f-weighted cotangents (FP64 path), or
@@ -2224,7 +2230,6 @@ __global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
      const double x = p.x[i];
      const int n1 = p.n1[i], n2 = p.n2[i];
      const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
-     const double *Dre = p.D, *Dim = p.D + dpl;
      cd carry = cmk(0., 0.), zb = cmk(0., 0.);
      // blocks of 8 steps: loads and the factors that do not depend on the
      // carried cotangent first, then the first-order recurrence itself
@@ -2234,11 +2239,9 @@ __global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
           for (int e = 0; e < 8; ++e) {
                const int j = jb + e;
                lc[e] = j <= n1 ? cmk(p.Lc[(size_t) j * Np + i], p.Lc[ppl + (size_t) j * Np + i]) : cmk(0., 0.);
-               // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z
-               const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
+               // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z; 1/c2 from the forward
                const cd c1 = cscale(iz, j + 1.);
-               const cd c2 = cadd(Dn, c1);
-               const cd rc = crecip_nr(c2);
+               const cd rc = j < n2 ? cmk(p.RC[(size_t) j * Np + i], p.RC[dpl + (size_t) j * Np + i]) : cmk(0., 0.);
                const cd ic22 = cmul(rc, rc);
                const cd dc1dz = cscale(cmul(c1, iz), -1.);
                g[e] = cconj(ic22);
@@ -2480,7 +2483,7 @@ struct Workspace {
      // reverse-pass checkpoint and buffers
      Buf CB, sst, pfbar, bar, fang, mbar, fbar, gbar, ones, expart;
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
-     Buf chD, chPC, chPart, revLc, revPart;         // per-size recurrences of the value channel
+     Buf chD, chPC, chPart, chRC, revLc, revPart;   // per-size recurrences of the value channel
      int n2rows = 0, ng = 0;
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
@@ -2809,6 +2812,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           for (int t = 0; t < nt64; ++t) kfmax = kfill[t] > kfmax ? kfill[t] : kfmax;
           const int n2rows = n2max + 1, ng = kfmax / 8;
           if (w->chD.get(sizeof(double) * 2 * (size_t) n2rows * Np) ||
+              (keep && w->chRC.get(sizeof(double) * 2 * (size_t) n2rows * Np)) ||
               w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) ||
               w->chPart.get(sizeof(double) * 5 * (size_t) (ng > 0 ? ng : 1) * Np))
                return -1;
@@ -2816,7 +2820,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           ca.N = N; ca.Np = Np; ca.n1max = n1max; ca.n2rows = n2rows;
           ca.x = w->x.as<double>(); ca.sx = w->sx.as<double>(); ca.cx = w->cx.as<double>();
           ca.n1 = w->n1.as<int>(); ca.n2 = w->n2.as<int>(); ca.mr = J->mr; ca.mi = J->mi;
-          ca.d_all = keep; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
+          ca.d_all = 0; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
+          ca.RC = keep ? w->chRC.as<double>() : nullptr;
           hipLaunchKernelGGL(mie_chains, dim3(Np / 64), dim3(64), 0, s, ca);
           TermArgs ta;
           ta.N = N; ta.Np = Np; ta.n1max = n1max; ta.n2rows = n2rows; ta.Kp = Kp; ta.Kb = Kb; ta.ng = ng;
@@ -3048,7 +3053,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
 // U = G A (Uf, [4][Kq][Np]).
 int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
      const int Np = w->Np, Hp = w->Hp, H = w->H, Q = w->Q, n1max = w->n1max;
-     const int Kq = rup(n1max, 64), Hq = rup(H, KM), nt64 = Np / 64;
+     const int Kq = rup(n1max, 64), Hq = rup(H, KM);
      if (rup(n1max, KM) > w->Kb * 8) return set_err("internal: split operand depth");
      if (w->wts.get(sizeof(double) * 8 * (size_t) Hq) || w->scal.get(sizeof(AdjScal)) ||
          w->L16.get(sizeof(unsigned short) * 4 * (size_t) Hq * Kq) ||
@@ -3150,7 +3155,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.ng = ng;
      r.x = w->x.as<double>(); r.f = w->f.as<double>();
      r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
-     r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>();
+     r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>(); r.RC = w->chRC.as<double>();
      r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
      r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
      r.fang = w->fang.as<double>();
