@@ -285,22 +285,26 @@ __global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu
 
 // Generalized spherical functions at the positive angles, l = 0 .. C-1, by
 // their three-term recurrences in l (misc/rtutil_math.c:956-1016).  Layout
-// [4][C][Hp]: lanes (angles) store contiguously for every l.
-__global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T, int legendre) {
+// [4][C][Hp] (+ plane 4: P_l for the Legendre expansion,
+// misc/rtutil_math.c:506-513): lanes (angles) store contiguously for every l.
+// The five recurrences are independent: blockIdx.y picks one (0: P00, 1: P02,
+// 2: P22, 3: P2-2, 4: P_l), so five times as many threads run one fifth of the
+// steps.  Columns H .. Hp-1 are written as zeros.
+__global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) {
      // separately rounded operations in the reference's order (bitwise the
-     // reference's functions); plane 4 (if legendre) holds P_l by the
-     // recurrence of misc/rtutil_math.c:506-513 for the Legendre expansion
+     // reference's functions)
 #pragma clang fp contract(off)
-     int j = blockIdx.x * blockDim.x + threadIdx.x;
-     if (j >= H) return;
-     double m = mu[H + j], m2 = m * m;
-     size_t plane = (size_t) C * Hp;
-     double *P00 = T, *P02 = T + plane, *P22 = T + 2 * plane, *P2m = T + 3 * plane;
-     double a00 = 1., b00 = m, a02 = 0., b02 = 0., a22 = 0., b22 = 0., a2m = 0., b2m = 0.;
-     double *PL = T + 4 * plane, lp0 = 1., lp1 = m;
-     for (int l = 0; l < C; ++l) {
-          double v00, v02, v22, v2m;
-          if (legendre) {
+     const int j = blockIdx.x * blockDim.x + threadIdx.x, fn = blockIdx.y;
+     if (j >= Hp) return;
+     double *P = T + (size_t) fn * C * Hp;
+     if (j >= H) {
+          for (int l = 0; l < C; ++l) P[(size_t) l * Hp + j] = 0.;
+          return;
+     }
+     const double m = mu[H + j], m2 = m * m;
+     if (fn == 4) {
+          double lp0 = 1., lp1 = m;
+          for (int l = 0; l < C; ++l) {
                double v;
                if (l == 0) v = 1.;
                else if (l == 1) v = m;
@@ -310,31 +314,38 @@ __global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T, int
                     lp0 = lp1;
                     lp1 = v;
                }
-               PL[(size_t) l * Hp + j] = v;
+               P[(size_t) l * Hp + j] = v;
           }
-          if (l == 0) { v00 = 1.; v02 = v22 = v2m = 0.; }
-          else if (l == 1) { v00 = m; v02 = v22 = v2m = 0.; }
+          return;
+     }
+     double a0 = fn == 0 ? 1. : 0., b0 = fn == 0 ? m : 0.;
+     for (int l = 0; l < C; ++l) {
+          double v;
+          if (l == 0) v = fn == 0 ? 1. : 0.;
+          else if (l == 1) v = fn == 0 ? m : 0.;
           else if (l == 2) {
-               v00 = (3. * m2 - 1.) / 2.;
-               v02 = .25 * sqrt(6.) * (m2 - 1.);
-               v22 = .25 * (1. + m) * (1. + m);
-               v2m = .25 * (1. - m) * (1. - m);
+               if (fn == 0) v = (3. * m2 - 1.) / 2.;
+               else if (fn == 1) v = .25 * sqrt(6.) * (m2 - 1.);
+               else if (fn == 2) v = .25 * (1. + m) * (1. + m);
+               else v = .25 * (1. - m) * (1. - m);
           } else {
                const double L = l - 1, L1 = l, t = 2. * L + 1.;
-               const double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
-               const double k4 = t * 4, kb = L1 * (L * L - 4), kd = L * (L1 * L1 - 4);
-               const double da = t * m;
-               v00 = (da * b00 - L * a00) / L1;
-               v02 = (da * b02 - sa * a02) / sb;
-               const double dc = t * (L * L1) * m;
-               v22 = ((dc - k4) * b22 - kb * a22) / kd;
-               v2m = ((dc + k4) * b2m - kb * a2m) / kd;
+               if (fn == 0) {
+                    const double da = t * m;
+                    v = (da * b0 - L * a0) / L1;
+               } else if (fn == 1) {
+                    const double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
+                    const double da = t * m;
+                    v = (da * b0 - sa * a0) / sb;
+               } else {
+                    const double k4 = t * 4, kb = L1 * (L * L - 4), kd = L * (L1 * L1 - 4);
+                    const double dc = t * (L * L1) * m;
+                    v = fn == 2 ? ((dc - k4) * b0 - kb * a0) / kd : ((dc + k4) * b0 - kb * a0) / kd;
+               }
           }
-          P00[(size_t) l * Hp + j] = v00;
-          P02[(size_t) l * Hp + j] = v02;
-          P22[(size_t) l * Hp + j] = v22;
-          P2m[(size_t) l * Hp + j] = v2m;
-          a00 = b00; b00 = v00; a02 = b02; b02 = v02; a22 = b22; b22 = v22; a2m = b2m; b2m = v2m;
+          P[(size_t) l * Hp + j] = v;
+          a0 = b0;
+          b0 = v;
      }
 }
 
@@ -1990,26 +2001,30 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
 
 // U = G A with the 32x32x16 matrix instruction: one workgroup per (size tile
 // of 128, order tile of 64) task, stages of 16 orders in two LDS buffers
-// (112 KiB).  Four waves in a 2 x 2 grid, each 32 orders (matrix rows) x 64
-// sizes (two 32-column blocks), 4 accumulators per element as adj_ugemm; the
-// larger wave tile reads each operand fragment for twice the products.
-template <class E>
-__global__ __launch_bounds__(256) void adj_ugemm32(UArgs p) {
+// (112 KiB).  Waves in a grid of (4 / WA) size blocks x 2 order blocks, each
+// wave 32 orders (matrix rows) x 32 WA sizes, 4 accumulators per element as
+// adj_ugemm.  WA = 2: four waves, each operand fragment read for twice the
+// products; WA = 1: eight waves (two per SIMD), which hide each other's LDS
+// and barrier waits.
+template <class E, int WA>
+__global__ __launch_bounds__(512 / WA) void adj_ugemm32(UArgs p) {
      constexpr int TS = 128, TO = 64, G = 2;                 // sizes, orders, groups of 8 per stage
-     constexpr int NA = 8 * G * TS / 256, NG = 12 * G * TO / 256;
+     constexpr int NT = 512 / WA, WS = 4 / WA;               // threads, size-block waves
+     constexpr int NA = 8 * G * TS / NT, NG = 12 * G * TO / NT;
+     static_assert(NA * NT == 8 * G * TS && NG * NT == 12 * G * TO, "staging split");
      __shared__ v4u sA[2][8 * G][TS];
      __shared__ v4u sG[2][12 * G][TO];
      const int2 tk = p.task[blockIdx.x];
      if (tk.x < 0) return;
      const int it0 = tk.x * TS, n0 = tk.y * TO, nch = p.ktile[tk.x] / (8 * G);
      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-     const int wm = wave & 1, wn = wave >> 1, l32 = lane & 31, lh = lane >> 5;
+     const int wm = wave % WS, wn = wave / WS, l32 = lane & 31, lh = lane >> 5;
      const size_t Np = p.Np, Kq = p.Kq;
      const size_t pA = (size_t) p.Kb * Np * 8, pG = (Kq / 8) * Kq * 8;
      const v4u neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
      v4u ra[NA], rg[NG];
-     const Stager<256, TS, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
-     const Stager<256, TO, G> stG(p.G16, pG, Kq * 8, n0, threadIdx.x);
+     const Stager<NT, TS, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
+     const Stager<NT, TO, G> stG(p.G16, pG, Kq * 8, n0, threadIdx.x);
      auto load = [&](int c) {
 #pragma unroll
           for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
@@ -2019,18 +2034,18 @@ __global__ __launch_bounds__(256) void adj_ugemm32(UArgs p) {
      auto store = [&](int s) {
 #pragma unroll
           for (int t = 0; t < NA; ++t) {
-               const int e = threadIdx.x + 256 * t;
+               const int e = threadIdx.x + NT * t;
                sA[s][e / TS][e % TS] = ra[t];
           }
 #pragma unroll
           for (int t = 0; t < NG; ++t) {
-               const int e = threadIdx.x + 256 * t;
+               const int e = threadIdx.x + NT * t;
                sG[s][e / TO][e % TO] = rg[t];
           }
      };
-     v16f acc[2][4];
+     v16f acc[WA][4];
 #pragma unroll
-     for (int a = 0; a < 2; ++a)
+     for (int a = 0; a < WA; ++a)
 #pragma unroll
           for (int k = 0; k < 4; ++k)
 #pragma unroll
@@ -2052,24 +2067,24 @@ __global__ __launch_bounds__(256) void adj_ugemm32(UArgs p) {
           v4u g[12];
 #pragma unroll
           for (int q = 0; q < 12; ++q) g[q] = sG[s][q * G + lh][wn * 32 + l32];
-          v4u x[2][12];
+          v4u x[WA][12];
 #pragma unroll
-          for (int a = 0; a < 2; ++a) {
+          for (int a = 0; a < WA; ++a) {
 #pragma unroll
-               for (int q = 0; q < 8; ++q) x[a][q] = sA[s][q * G + lh][wm * 64 + 32 * a + l32];
+               for (int q = 0; q < 8; ++q) x[a][q] = sA[s][q * G + lh][(wm * WA + a) * 32 + l32];
                x[a][8] = x[a][0] ^ neg;
                x[a][9] = x[a][1] ^ neg;
                x[a][10] = x[a][4] ^ neg;
                x[a][11] = x[a][5] ^ neg;
           }
           // hi x hi, lo(G) x hi(A), hi(G) x lo(A) for every (term, size block,
-          // accumulator): eight independent accumulators between dependent products
+          // accumulator): 4 WA independent accumulators between dependent products
 #pragma unroll
           for (int t = 0; t < 3; ++t) {
 #pragma unroll
                for (int pr = 0; pr < 3; ++pr) {
 #pragma unroll
-                    for (int a = 0; a < 2; ++a)
+                    for (int a = 0; a < WA; ++a)
 #pragma unroll
                          for (int k = 0; k < 4; ++k) {
                               const int gp = term[k][t][0], xh = term[k][t][1], xl = term[k][t][2];
@@ -2087,15 +2102,26 @@ __global__ __launch_bounds__(256) void adj_ugemm32(UArgs p) {
      // U = acc n (n+1) wmax / (2^(14 - k) kSB^2 kSA); rows are orders
      const double base = p.sc->wmax * ldexp(1., adj_gexp(p.sc) - 14) / (kSB * kSB * kSA);
 #pragma unroll
-     for (int a = 0; a < 2; ++a)
+     for (int a = 0; a < WA; ++a)
 #pragma unroll
           for (int v = 0; v < 16; ++v) {
                const int r = n0 + wn * 32 + mfma32_row(lane, v);
-               const int i = it0 + wm * 64 + 32 * a + l32;
+               const int i = it0 + (wm * WA + a) * 32 + l32;
                const double f = base * (r + 1.) * (r + 2.);
 #pragma unroll
                for (int k = 0; k < 4; ++k) p.Uf[((size_t) k * Kq + r) * Np + i] = (float) (acc[a][k][v] * f);
           }
+}
+
+// Waves per workgroup of the reverse contraction (environment
+// LMIE_UGEMM_WAVES: 4 or 8, default 8).
+int ugemm_waves() {
+     static int v = 0;
+     if (!v) {
+          const char *e = getenv("LMIE_UGEMM_WAVES");
+          v = (e && atoi(e) == 4) ? 4 : 8;
+     }
+     return v;
 }
 
 // Per-size reverse sweep (adjoint of src/lmie_core.c:298-611), in two
@@ -2779,9 +2805,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           if (J->calc_gc || J->calc_lc) {
                const int planes = J->calc_lc ? 5 : 4;
                if (w->T.get(sizeof(double) * planes * (size_t) C * Hp)) return -1;
-               HIPCHECK(hipMemsetAsync(w->T.p, 0, sizeof(double) * planes * (size_t) C * Hp, s));
-               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, C, (const double *) w->mu.as<double>(),
-                                  w->T.as<double>(), J->calc_lc);
+               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64, planes), dim3(64), 0, s, H, Hp, C,
+                                  (const double *) w->mu.as<double>(), w->T.as<double>());
           }
           HIPCHECK(hipGetLastError());
      }
@@ -3090,7 +3115,10 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
           hipLaunchKernelGGL(adj_gsplit<E>, dim3(Kq / 64, Kq / 8), dim3(64), 0, s, (const float *) w->Gf.as<float>(), \
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
           tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
-          hipLaunchKernelGGL(adj_ugemm32<E>, dim3((unsigned) task.size()), dim3(256), 0, s, u);                 \
+          if (ugemm_waves() == 4)                                                                               \
+               hipLaunchKernelGGL((adj_ugemm32<E, 2>), dim3((unsigned) task.size()), dim3(256), 0, s, u);       \
+          else                                                                                                  \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1>), dim3((unsigned) task.size()), dim3(512), 0, s, u);       \
           tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);
