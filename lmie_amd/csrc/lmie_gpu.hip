@@ -588,8 +588,9 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
 
 // ---------------------------------------------------------------------------
 // Value channel in three kernels (the same operations as mie_coefs<0>, so the
-// same bits): mie_chains runs the three recurrences that are serial in n, one
-// thread per size; mie_terms forms a_n, b_n, the contraction operand and the
+// same bits): mie_chains runs the three recurrences that are serial in n, two
+// threads per size in different waves (the D chain; the ratio chain and then
+// the psi, chi chain); mie_terms forms a_n, b_n, the contraction operand and the
 // partial series sums in parallel, one thread per (size, group of 8 orders);
 // mie_sums adds the partial sums per size.  The serial part is a few
 // operations per order, the parallel part fills the GPU.
@@ -611,34 +612,37 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
      const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
      const double x = p.x[i];
      const int n1 = p.n1[i], n2 = p.n2[i], dtop = p.d_all ? n2 - 1 : n1;
-     const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
      double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
-     // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0; one
-     // reciprocal per step (|D_{j+1} + (j+1)/z| stays between about 1/|z| and
-     // n2/|z|), the recurrence damps its last-bit differences.  In the same
-     // loop, as an independent chain, the downward ratio
-     // r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows.
-     cd D = cmk(0., 0.);
+     if (blockIdx.y == 0) {
+          // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0; one
+          // reciprocal per step (|D_{j+1} + (j+1)/z| stays between about 1/|z|
+          // and n2/|z|), the recurrence damps its last-bit differences
+          const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
+          cd D = cmk(0., 0.);
+          for (int j = n2 - 1; j >= 1; --j) {
+               const cd c1 = cscale(iz, j + 1.);
+               const cd c2 = cadd(D, c1);
+               const cd rc = crecip_nr(c2);
+               D = csub(c1, rc);
+               if (j <= dtop) {
+                    Dre[j * Np + i] = D.r;
+                    Dim[j * Np + i] = D.i;
+               }
+               if (p.RC) {
+                    p.RC[(size_t) j * Np + i] = rc.r;
+                    p.RC[dpl + (size_t) j * Np + i] = rc.i;
+               }
+          }
+          return;
+     }
+     // the other chains in other waves (blockIdx.y = 1): the downward ratio
+     // r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows
      double r = 0.;
      Psi[n1 * Np + i] = 0.;
-     for (int j = n2 - 1; j >= 1; --j) {
-          const cd c1 = cscale(iz, j + 1.);
-          const cd c2 = cadd(D, c1);
-          const cd rc = crecip_nr(c2);
-          D = csub(c1, rc);
-          if (j <= dtop) {
-               Dre[j * Np + i] = D.r;
-               Dim[j * Np + i] = D.i;
-          }
-          if (p.RC) {
-               p.RC[(size_t) j * Np + i] = rc.r;
-               p.RC[dpl + (size_t) j * Np + i] = rc.i;
-          }
-          if (j < n1) {
-               const double c = (2. * j + 1.) / x;
-               r = 1. / (c - r);
-               Psi[j * Np + i] = r;
-          }
+     for (int j = n1 - 1; j >= 1; --j) {
+          const double c = (2. * j + 1.) / x;
+          r = 1. / (c - r);
+          Psi[j * Np + i] = r;
      }
      // upward psi_j = r_j psi_{j-1}, chi_{j+1} = (2j+1)/x chi_j - chi_{j-1}
      const double sx = p.sx[i], cx = p.cx[i], ax = 1. / x;
@@ -1457,6 +1461,146 @@ __global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
 }
 
 
+// The split 16-bit contraction on the 32x32x16 matrix instruction: one
+// workgroup of eight waves (two per SIMD) per (size tile of 64, angle tile of
+// 128) task, stages of 32 orders (two K-steps of 16) in two LDS buffers.
+// Waves in a 2 x 4 grid, each one 32 x 32 block (rows sizes, columns angles),
+// 8 accumulators per element as angular_mix2.  The -mu products flip the sign
+// of the odd orders on the angle operand (the same products as flipping the
+// size operand).  Partials per size tile: part[tile][NW][4][Q].
+template <class E, int NW>
+__global__ __launch_bounds__(512) void angular_mix32(Mix2Args p) {
+     constexpr int NT = 512, TM = 64, TN = 128, G = KM / 8;
+     static_assert(G == 4, "two K-steps of 16 orders per stage");
+     constexpr int NA = 8 * G * TM / NT, NB = 4 * G * TN / NT;   // staged v4u per thread
+     static_assert(NA * NT == 8 * G * TM && NB * NT == 4 * G * TN, "staging split");
+     __shared__ v4u sA[2][8 * G][TM];
+     __shared__ v4u sB[2][4 * G][TN];
+     static_assert(sizeof(sA) >= sizeof(double) * 2 * NW * 8 * TN, "epilogue buffer");
+
+     const int2 tk = p.task[blockIdx.x];
+     if (tk.x < 0) return;
+     const int st = tk.x, jt = tk.y;
+     const int it0 = st * TM, jt0 = jt * TN, nch = p.ktile[st] / KM;
+     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+     const int wm = wave & 1, wn = wave >> 1, l32 = lane & 31, lh = lane >> 5;
+     const size_t Np = p.Np, Hb = p.Hb;
+     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hb * 8;
+     const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
+
+     v4u ra[NA], rb[NB];
+     const Stager<NT, TM, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
+     const Stager<NT, TN, G> stB(p.B16, pB, Hb * 8, jt0, threadIdx.x);
+     auto load = [&](int c) {
+#pragma unroll
+          for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
+#pragma unroll
+          for (int t = 0; t < NB; ++t) rb[t] = *stB.at(t, c);
+     };
+     auto store = [&](int s) {
+#pragma unroll
+          for (int t = 0; t < NA; ++t) {
+               const int e = threadIdx.x + NT * t;
+               sA[s][e / TM][e % TM] = ra[t];
+          }
+#pragma unroll
+          for (int t = 0; t < NB; ++t) {
+               const int e = threadIdx.x + NT * t;
+               sB[s][e / TN][e % TN] = rb[t];
+          }
+     };
+
+     v16f acc[8];
+#pragma unroll
+     for (int k = 0; k < 8; ++k)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
+     if (nch > 0) {
+          load(0);
+          store(0);
+          if (nch > 1) load(1);
+     }
+     for (int c = 0; c < nch; ++c) {
+          const int s = c & 1;
+          __syncthreads();
+#pragma unroll
+          for (int ks = 0; ks < 2; ++ks) {
+               // lane l: row (size or angle) l % 32, orders 8 (2 ks + l / 32) + e
+               const int row = 2 * ks + lh;
+               v4u x[8], b[4];
+#pragma unroll
+               for (int q = 0; q < 8; ++q) x[q] = sA[s][q * G + row][wm * 32 + l32];
+#pragma unroll
+               for (int q = 0; q < 4; ++q) b[q] = sB[s][q * G + row][wn * 32 + l32];
+               // +mu: sp = S (pi+tau), sm = T (pi-tau); b = (pi+tau hi, lo, pi-tau hi, lo)
+               acc[0] = E::mfma32(x[0], b[0], acc[0]); acc[1] = E::mfma32(x[2], b[0], acc[1]);
+               acc[2] = E::mfma32(x[4], b[2], acc[2]); acc[3] = E::mfma32(x[6], b[2], acc[3]);
+               acc[0] = E::mfma32(x[0], b[1], acc[0]); acc[1] = E::mfma32(x[2], b[1], acc[1]);
+               acc[2] = E::mfma32(x[4], b[3], acc[2]); acc[3] = E::mfma32(x[6], b[3], acc[3]);
+               acc[0] = E::mfma32(x[1], b[0], acc[0]); acc[1] = E::mfma32(x[3], b[0], acc[1]);
+               acc[2] = E::mfma32(x[5], b[2], acc[2]); acc[3] = E::mfma32(x[7], b[2], acc[3]);
+               // -mu: the odd orders change sign, sp = s S (pi-tau), sm = s T (pi+tau)
+#pragma unroll
+               for (int q = 0; q < 4; ++q) b[q] ^= flip;
+               acc[4] = E::mfma32(x[0], b[2], acc[4]); acc[5] = E::mfma32(x[2], b[2], acc[5]);
+               acc[6] = E::mfma32(x[4], b[0], acc[6]); acc[7] = E::mfma32(x[6], b[0], acc[7]);
+               acc[4] = E::mfma32(x[0], b[3], acc[4]); acc[5] = E::mfma32(x[2], b[3], acc[5]);
+               acc[6] = E::mfma32(x[4], b[1], acc[6]); acc[7] = E::mfma32(x[6], b[1], acc[7]);
+               acc[4] = E::mfma32(x[1], b[2], acc[4]); acc[5] = E::mfma32(x[3], b[2], acc[5]);
+               acc[6] = E::mfma32(x[5], b[0], acc[6]); acc[7] = E::mfma32(x[7], b[0], acc[7]);
+               if (ks == 0) {
+                    if (c + 1 < nch) store(s ^ 1);
+                    if (c + 2 < nch) load(c + 2);
+               }
+          }
+     }
+
+     // epilogue: Stokes elements in double, weighted sums over this wave's 32
+     // sizes per angle (16 rows per lane, the other 16 in lane l ^ 32),
+     // combined over the two size halves (wm) through LDS
+     const double unscale = 1. / (kSA * kSB);
+     double *sR = (double *) &sA[0][0][0];   // [2][NW][8][TN]
+     __syncthreads();                        // every wave is done with the stages
+     const int jl = wn * 32 + l32;
+     // one weight channel at a time (the Stokes elements are recomputed per
+     // channel), so few doubles are live beside the accumulators
+#pragma unroll
+     for (int c = 0; c < NW; ++c) {
+          double ew[8];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) ew[k] = 0.;
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+               const int i = it0 + wm * 32 + mfma32_row(lane, v);
+               double Pp[4], Pm[4];
+               stokes(acc[0][v] * unscale, acc[1][v] * unscale, acc[2][v] * unscale, acc[3][v] * unscale, Pp);
+               stokes(acc[4][v] * unscale, acc[5][v] * unscale, acc[6][v] * unscale, acc[7][v] * unscale, Pm);
+               const double wv = p.W[c][i];
+#pragma unroll
+               for (int k = 0; k < 4; ++k) {
+                    ew[k] += wv * Pp[k];
+                    ew[4 + k] += wv * Pm[k];
+               }
+          }
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+               double t = ew[k];
+               t += __shfl_xor(t, 32);
+               if (lh == 0) sR[((wm * NW + c) * 8 + k) * TN + jl] = t;
+          }
+     }
+     __syncthreads();
+     double *out = p.part + (size_t) st * NW * 4 * p.Q;
+     for (int e = threadIdx.x; e < NW * 8 * TN; e += NT) {
+          const int jl2 = e % TN, ck = e / TN, c = ck / 8, k = ck % 8, j2 = jt0 + jl2;
+          if (j2 >= p.H) continue;
+          const double v = sR[(c * 8 + k) * TN + jl2] + sR[((NW + c) * 8 + k) * TN + jl2];
+          // k < 4: direct angle H + j; k >= 4: mirrored angle H - 1 - j
+          out[((size_t) c * 4 + (k & 3)) * p.Q + (k < 4 ? p.H + j2 : p.H - 1 - j2)] = v;
+     }
+}
+
+
 // ---------------------------------------------------------------------------
 // Reverse pass.
 //
@@ -2006,7 +2150,7 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
 // adj_ugemm.  WA = 2: four waves, each operand fragment read for twice the
 // products; WA = 1: eight waves (two per SIMD), which hide each other's LDS
 // and barrier waits.
-template <class E, int WA>
+template <class E, int WA, int SCH>
 __global__ __launch_bounds__(512 / WA) void adj_ugemm32(UArgs p) {
      constexpr int TS = 128, TO = 64, G = 2;                 // sizes, orders, groups of 8 per stage
      constexpr int NT = 512 / WA, WS = 4 / WA;               // threads, size-block waves
@@ -2053,7 +2197,7 @@ __global__ __launch_bounds__(512 / WA) void adj_ugemm32(UArgs p) {
      if (nch > 0) {
           load(0);
           store(0);
-          if (nch > 1) load(1);
+          load(min(1, nch - 1));
      }
      // terms per accumulator: (G plane pair, A plane hi, A plane lo); A planes
      // 0..7 as A16, 8 = -A+r hi, 9 = -A+r lo, 10 = -A-r hi, 11 = -A-r lo
@@ -2093,10 +2237,37 @@ __global__ __launch_bounds__(512 / WA) void adj_ugemm32(UArgs p) {
                               acc[a][k] = E::mfma32(gv, xv, acc[a][k]);
                          }
                }
+               // the stage body is one basic block: the last stages store and
+               // load a repeat of the last stage (the other LDS buffer is not
+               // read again, the repeated rows are in bounds)
                if (t == 0) {
-                    if (c + 1 < nch) store(s ^ 1);
-                    if (c + 2 < nch) load(c + 2);
+                    store(s ^ 1);
+                    load(min(c + 2, nch - 1));
                }
+          }
+          if constexpr (SCH == 1) {
+               // issue order of the stage: the first fragments, then one
+               // matrix product between consecutive LDS reads, LDS writes and
+               // HBM loads, so no run of memory instructions stalls the
+               // in-order issue of the products
+               constexpr int NR = 8 * WA + 12, NM = 36 * WA;
+               __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+#pragma unroll
+               for (int q = 6; q < NR; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+               }
+#pragma unroll
+               for (int q = 0; q < NA + NG; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+               }
+#pragma unroll
+               for (int q = 0; q < NA + NG; ++q) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+               }
+               __builtin_amdgcn_sched_group_barrier(0x008, NM - (NR - 6) - 2 * (NA + NG), 0);
           }
      }
      // U = acc n (n+1) wmax / (2^(14 - k) kSB^2 kSA); rows are orders
@@ -2114,7 +2285,8 @@ __global__ __launch_bounds__(512 / WA) void adj_ugemm32(UArgs p) {
 }
 
 // Waves per workgroup of the reverse contraction (environment
-// LMIE_UGEMM_WAVES: 4 or 8, default 8).
+// LMIE_UGEMM_WAVES: 4 or 8, default 8) and its issue schedule
+// (LMIE_UGEMM_SCHED: 0 compiler order, 1 interleaved, default 0).
 int ugemm_waves() {
      static int v = 0;
      if (!v) {
@@ -2124,7 +2296,16 @@ int ugemm_waves() {
      return v;
 }
 
-// Per-size reverse sweep (adjoint of src/lmie_core.c:298-611), in two
+int ugemm_sched() {
+     static int v = -1;
+     if (v < 0) {
+          const char *e = getenv("LMIE_UGEMM_SCHED");
+          v = (e && atoi(e) == 1) ? 1 : 0;
+     }
+     return v;
+}
+
+// Per-size reverse sweep (adjoint of src/lmie_core.c:298-611), in three
 // kernels over the forward's kept recurrences (D_j up to n2 - 1, psi_j,
 // chi_j) and a_j, b_j:
 //   rev_terms  one thread per (size, group of 8 orders): cotangents of a_j, b_j
@@ -2132,10 +2313,12 @@ int ugemm_waves() {
 //              a = (u psi - psi0) / (u zeta - zeta0), u = D/m + j/x and
 //              b (v = m D + j/x) to the local cotangent of D_j and the direct
 //              part of the cotangent of m;
-//   rev_carry  one thread per size: the adjoint of the downward recurrence
-//              D_j = (j+1)/z - 1/(D_{j+1} + (j+1)/z), a first-order linear
-//              recurrence upward from j = 1 to n2 - 1 carrying the D cotangent
-//              to z = m x, then the per-size totals.
+//   rev_chunk  one thread per (size, chunk of the order range): the adjoint
+//              of the downward recurrence D_j = (j+1)/z - 1/(D_{j+1} + (j+1)/z)
+//              is a first-order linear recurrence upward from j = 1 to n2 - 1
+//              carrying the D cotangent to z = m x; each chunk returns it as an
+//              affine function of the carry entering the chunk;
+//   rev_carry  one thread per size: chains the chunks, then the per-size totals.
 // Outputs per size: the cotangent of m (re, im as d/dRe + i d/dIm) and of f_i.
 struct RevArgs {
      int N, Np, Kp, n1max, n2rows, ng;
@@ -2244,43 +2427,77 @@ __global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
      pt[2 * gp] = fq;
 }
 
-__global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
+// The D-cotangent recurrence of rev_carry in RCK chunks of the order range
+// per size, one thread per (size, chunk): with the carry entering the chunk
+// as c, every carry and D cotangent inside it is affine in c, so the thread
+// runs the chunk once for the affine coefficients and returns
+//   carry out = P c + C0,   z cotangent of the chunk = Q c + Z0
+// (scan[k][8][Np]: P, C0, Q, Z0 as re, im).  rev_carry chains the chunks.
+constexpr int RCK = 8;
+
+__global__ __launch_bounds__(64) void rev_chunk(RevArgs p, double *scan) {
+     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+     if (i >= p.Np) return;
+     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
+     cd P = cmk(1., 0.), C0 = cmk(0., 0.), Q = cmk(0., 0.), Z0 = cmk(0., 0.);
+     if (i < p.N) {
+          const double x = p.x[i];
+          const int n1 = p.n1[i], n2 = p.n2[i];
+          const int cs = (n2 - 1 + RCK - 1) / RCK, lo = 1 + k * cs, hi = min(n2, lo + cs);
+          const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
+          // blocks of 8 steps: loads and the factors that do not depend on
+          // the carried terms first, then the affine recurrence
+          for (int jb = lo; jb < hi; jb += 8) {
+               cd lc[8], g[8], h[8];
+#pragma unroll
+               for (int e = 0; e < 8; ++e) {
+                    const int j = jb + e;
+                    lc[e] = j <= n1 ? cmk(p.Lc[(size_t) j * Np + i], p.Lc[ppl + (size_t) j * Np + i]) : cmk(0., 0.);
+                    // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z; 1/c2 from the forward
+                    const cd c1 = cscale(iz, j + 1.);
+                    const cd rc = j < hi ? cmk(p.RC[(size_t) j * Np + i], p.RC[dpl + (size_t) j * Np + i]) : cmk(0., 0.);
+                    const cd ic22 = cmul(rc, rc);
+                    const cd dc1dz = cscale(cmul(c1, iz), -1.);
+                    g[e] = cconj(ic22);
+                    h[e] = cconj(cmul(cadd(cmk(1., 0.), ic22), dc1dz));
+               }
+#pragma unroll
+               for (int e = 0; e < 8; ++e) {
+                    if (jb + e < hi) {
+                         // D cotangent = P c + (C0 + lc); carry = g D cotangent
+                         const cd t = cadd(C0, lc[e]);
+                         Q = cadd(Q, cmul(h[e], P));
+                         Z0 = cadd(Z0, cmul(h[e], t));
+                         P = cmul(g[e], P);
+                         C0 = cmul(g[e], t);
+                    }
+               }
+          }
+     }
+     double *o = scan + (size_t) k * 8 * Np + i;
+     o[0] = P.r; o[Np] = P.i; o[2 * Np] = C0.r; o[3 * Np] = C0.i;
+     o[4 * Np] = Q.r; o[5 * Np] = Q.i; o[6 * Np] = Z0.r; o[7 * Np] = Z0.i;
+}
+
+__global__ __launch_bounds__(64) void rev_carry(RevArgs p, const double *scan) {
      const int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= p.Np) return;
-     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np, gp = (size_t) p.ng * Np;
+     const size_t Np = p.Np, gp = (size_t) p.ng * Np;
      if (i >= p.N) {
           p.mbar[i] = p.mbar[Np + i] = 0.;
           p.fbar[i] = 0.;
           return;
      }
      const double x = p.x[i];
-     const int n1 = p.n1[i], n2 = p.n2[i];
-     const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
+     const int n1 = p.n1[i];
+     // chain the chunks upward from j = 1 with carry 0
      cd carry = cmk(0., 0.), zb = cmk(0., 0.);
-     // blocks of 8 steps: loads and the factors that do not depend on the
-     // carried cotangent first, then the first-order recurrence itself
-     for (int jb = 1; jb < n2; jb += 8) {
-          cd lc[8], g[8], h[8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-               const int j = jb + e;
-               lc[e] = j <= n1 ? cmk(p.Lc[(size_t) j * Np + i], p.Lc[ppl + (size_t) j * Np + i]) : cmk(0., 0.);
-               // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z; 1/c2 from the forward
-               const cd c1 = cscale(iz, j + 1.);
-               const cd rc = j < n2 ? cmk(p.RC[(size_t) j * Np + i], p.RC[dpl + (size_t) j * Np + i]) : cmk(0., 0.);
-               const cd ic22 = cmul(rc, rc);
-               const cd dc1dz = cscale(cmul(c1, iz), -1.);
-               g[e] = cconj(ic22);
-               h[e] = cconj(cmul(cadd(cmk(1., 0.), ic22), dc1dz));
-          }
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-               if (jb + e < n2) {
-                    const cd Db = cadd(carry, lc[e]);
-                    carry = cmul(g[e], Db);
-                    zb = cadd(zb, cmul(h[e], Db));
-               }
-          }
+     for (int k = 0; k < RCK; ++k) {
+          const double *q = scan + (size_t) k * 8 * Np + i;
+          const cd P = cmk(q[0], q[Np]), C0 = cmk(q[2 * Np], q[3 * Np]);
+          const cd Q = cmk(q[4 * Np], q[5 * Np]), Z0 = cmk(q[6 * Np], q[7 * Np]);
+          zb = cadd(zb, cadd(cmul(Q, carry), Z0));
+          carry = cadd(cmul(P, carry), C0);
      }
      cd mb = cmk(0., 0.);
      double fq = 0.;
@@ -2509,7 +2726,7 @@ struct Workspace {
      // reverse-pass checkpoint and buffers
      Buf CB, sst, pfbar, bar, fang, mbar, fbar, gbar, ones, expart;
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
-     Buf chD, chPC, chPart, chRC, revLc, revPart;   // per-size recurrences of the value channel
+     Buf chD, chPC, chPart, chRC, revLc, revPart, revScan;   // per-size recurrences of the value channel
      int n2rows = 0, ng = 0;
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
@@ -2648,13 +2865,15 @@ void xcd_tasks(const std::vector<int> &kt, F ncol, int group, std::vector<int2> 
           for (size_t k = 0; k < per[x].size(); ++k) out[8 * k + x] = per[x][k];
 }
 
-// Waves per workgroup of the mixed contraction (environment LMIE_MIX_WAVES: 4
-// or 8, default 8: two waves per SIMD hide each other's LDS and barrier waits).
+// Kernel of the mixed contraction (environment LMIE_MIX_WAVES): 8 (default)
+// or 4 the 16x16x32 instruction with eight or four waves, 32 the 32x32x16
+// instruction with eight waves.
 int mix_waves() {
      static int v = 0;
      if (!v) {
           const char *e = getenv("LMIE_MIX_WAVES");
-          v = (e && atoi(e) == 4) ? 4 : 8;
+          const int r = e ? atoi(e) : 8;
+          v = (r == 4 || r == 32) ? r : 8;
      }
      return v;
 }
@@ -2847,7 +3066,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           ca.n1 = w->n1.as<int>(); ca.n2 = w->n2.as<int>(); ca.mr = J->mr; ca.mi = J->mi;
           ca.d_all = 0; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
           ca.RC = keep ? w->chRC.as<double>() : nullptr;
-          hipLaunchKernelGGL(mie_chains, dim3(Np / 64), dim3(64), 0, s, ca);
+          hipLaunchKernelGGL(mie_chains, dim3(Np / 64, 2), dim3(64), 0, s, ca);
           TermArgs ta;
           ta.N = N; ta.Np = Np; ta.n1max = n1max; ta.n2rows = n2rows; ta.Kp = Kp; ta.Kb = Kb; ta.ng = ng;
           ta.x = w->x.as<double>(); ta.n1 = w->n1.as<int>(); ta.kfill = w->kfill.as<int>();
@@ -2978,8 +3197,18 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                } while (0)
                if (mix_waves() == 4) {
                     if (bf16) LMIE_MIX_NW(BF16, 4, 2); else LMIE_MIX_NW(F16, 4, 2);
-               } else {
+               } else if (mix_waves() == 8) {
                     if (bf16) LMIE_MIX_NW(BF16, 2, 4); else LMIE_MIX_NW(F16, 2, 4);
+               } else {
+#define LMIE_MIX32(E)                                                                                    \
+                    do {                                                                                 \
+                         if (nw == 0) hipLaunchKernelGGL((angular_mix32<E, 1>), grid, dim3(512), 0, s, m); \
+                         else if (nw == 1) hipLaunchKernelGGL((angular_mix32<E, 2>), grid, dim3(512), 0, s, m); \
+                         else if (nw == 2) hipLaunchKernelGGL((angular_mix32<E, 3>), grid, dim3(512), 0, s, m); \
+                         else hipLaunchKernelGGL((angular_mix32<E, 4>), grid, dim3(512), 0, s, m);        \
+                    } while (0)
+                    if (bf16) LMIE_MIX32(BF16); else LMIE_MIX32(F16);
+#undef LMIE_MIX32
                }
 #undef LMIE_MIX_NW
 #undef LMIE_MIX
@@ -3116,9 +3345,11 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
           tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
           if (ugemm_waves() == 4)                                                                               \
-               hipLaunchKernelGGL((adj_ugemm32<E, 2>), dim3((unsigned) task.size()), dim3(256), 0, s, u);       \
+               hipLaunchKernelGGL((adj_ugemm32<E, 2, 0>), dim3((unsigned) task.size()), dim3(256), 0, s, u);    \
+          else if (ugemm_sched() == 1)                                                                          \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 1>), dim3((unsigned) task.size()), dim3(512), 0, s, u);    \
           else                                                                                                  \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1>), dim3((unsigned) task.size()), dim3(512), 0, s, u);       \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 0>), dim3((unsigned) task.size()), dim3(512), 0, s, u);    \
           tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);
@@ -3177,7 +3408,8 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      tm.mark(LMIE_T_ADJ_COEFS, 0);
      const int ng = w->ng;
      if (w->revLc.get(sizeof(double) * 2 * (size_t) (w->n1max + 1) * Np) ||
-         w->revPart.get(sizeof(double) * 3 * (size_t) (ng > 0 ? ng : 1) * Np))
+         w->revPart.get(sizeof(double) * 3 * (size_t) (ng > 0 ? ng : 1) * Np) ||
+         w->revScan.get(sizeof(double) * 8 * RCK * (size_t) Np))
           return -1;
      RevArgs r;
      r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.ng = ng;
@@ -3191,7 +3423,8 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.Lc = w->revLc.as<double>(); r.rpart = w->revPart.as<double>();
      r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
      if (ng > 0) hipLaunchKernelGGL(rev_terms, dim3(Np / 64, ng), dim3(64), 0, s, r);
-     hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r);
+     hipLaunchKernelGGL(rev_chunk, dim3(Np / 64, RCK), dim3(64), 0, s, r, w->revScan.as<double>());
+     hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r, (const double *) w->revScan.as<double>());
      HIPCHECK(hipGetLastError());
      tm.mark(LMIE_T_ADJ_COEFS, 1);
      // sum of the per-size m cotangents
