@@ -685,8 +685,28 @@ __device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, d
      *b = cdiv_nr(nb, db);
 }
 
+// Size-independent factors of order j (the reference's expressions, so the
+// same bits), one row each over j = 0 .. n: fv4 = (2j+1) / (j (j+1)),
+// fv3m = (j+1) / (j / (j-1)) and fv4m = (2j-1) / ((j-1) j) for j > 1 (else 0),
+// fv3 = (j+2) / ((j+1) / j).  The per-order kernels read them as uniform loads
+// instead of dividing in every lane.
+constexpr int FT_FV4 = 0, FT_FV3M = 1, FT_FV4M = 2, FT_FV3 = 3, FT_ROWS = 4;
+
+__global__ void order_factors(int n, double *ft) {
+#pragma clang fp contract(off)
+     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+     if (j > n) return;
+     const size_t ld = (size_t) n + 1;
+     const double fv1 = 2. * j + 1.;
+     ft[FT_FV4 * ld + j] = j > 0 ? fv1 / (j * (j + 1.)) : 0.;
+     ft[FT_FV3M * ld + j] = j > 1 ? (j + 1.) / ((double) j / (j - 1.)) : 0.;
+     ft[FT_FV4M * ld + j] = j > 1 ? (2. * j - 1.) / ((j - 1.) * j) : 0.;
+     ft[FT_FV3 * ld + j] = j > 0 ? (j + 2.) / ((j + 1.) / j) : 0.;
+}
+
 struct TermArgs {
      int N, Np, n1max, n2rows, Kp, Kb, ng;
+     const double *ft;          // order_factors rows over 0 .. n1max + 1
      const double *x;
      const int *n1;
      const int *kfill;          // [Np / TILE]
@@ -705,60 +725,74 @@ __global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
      if (i >= p.Np || g >= p.ng) return;
      const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
      const size_t plane = (size_t) p.Kp * Np, p16 = (size_t) p.Kb * Np * 8, gp = (size_t) p.ng * Np;
+     const size_t ld = (size_t) p.n1max + 2;
      const int kf = p.kfill[i / TILE];
      const int n1 = i < p.N ? p.n1[i] : 0;
      const int j0 = 8 * g + 1, j1 = min(8 * g + 8, n1);
      double se = 0., ss = 0., sg = 0.;
      cd cb = cmk(0., 0.);
-     unsigned short buf[8][8];
+     // split operand of the 8 orders, [plane][order pair]; orders past n1 stay 0
+     unsigned buf[8][4];
+#pragma unroll
+     for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) buf[q][k] = 0u;
      if (j0 <= j1) {
           const double x = p.x[i];
           const cd m = cmk(p.mr, -p.mi), im = crecip(m);
           const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
           cd ap = cmk(0., 0.), bp = cmk(0., 0.);
+          double psi0 = Psi[(j0 - 1) * Np + i], chi0 = Chi[(j0 - 1) * Np + i];
           if (j0 > 1)
-               mie_ab_at(j0 - 1, x, m, im, cmk(Dre[(j0 - 1) * Np + i], Dim[(j0 - 1) * Np + i]), Psi[(j0 - 1) * Np + i],
-                         Psi[(j0 - 2) * Np + i], Chi[(j0 - 1) * Np + i], Chi[(j0 - 2) * Np + i], &ap, &bp);
+               mie_ab_at(j0 - 1, x, m, im, cmk(Dre[(j0 - 1) * Np + i], Dim[(j0 - 1) * Np + i]), psi0,
+                         Psi[(j0 - 2) * Np + i], chi0, Chi[(j0 - 2) * Np + i], &ap, &bp);
           double sgn = (j0 & 1) ? -1. : 1.;
-          for (int j = j0; j <= j1; ++j) {
-               cd a, b;
-               mie_ab_at(j, x, m, im, cmk(Dre[j * Np + i], Dim[j * Np + i]), Psi[j * Np + i], Psi[(j - 1) * Np + i],
-                         Chi[j * Np + i], Chi[(j - 1) * Np + i], &a, &b);
-               const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
-               const double fv3m = j > 1 ? (j + 1.) / ((double) j / (j - 1.)) : 0.;
-               const double fv4m = j > 1 ? (2. * j - 1.) / ((j - 1.) * j) : 0.;
-               se += fv1 * (a.r + b.r);
-               ss += fv1 * (cabs2(a) + cabs2(b));
-               cb = cadd(cb, cscale(csub(a, b), fv1 * sgn));
-               if (j > 1)
-                    sg += fv3m * ((ap.r * a.r + ap.i * a.i) + (bp.r * b.r + bp.i * b.i)) + fv4m * (ap.r * bp.r + ap.i * bp.i);
-               const size_t o = (size_t) (j - 1) * Np + i;
-               if constexpr (MIX) {
-                    const int e = j - j0;
-                    const double sc = fv1 * kSA;
-                    split16<E>(sc * (a.r + b.r), &buf[0][e], &buf[1][e]);
-                    split16<E>(sc * (a.i + b.i), &buf[2][e], &buf[3][e]);
-                    split16<E>(sc * (a.r - b.r), &buf[4][e], &buf[5][e]);
-                    split16<E>(sc * (a.i - b.i), &buf[6][e], &buf[7][e]);
-               } else {
-                    p.A[o] = fv4 * (a.r + b.r);
-                    p.A[plane + o] = fv4 * (a.i + b.i);
-                    p.A[2 * plane + o] = fv4 * (a.r - b.r);
-                    p.A[3 * plane + o] = fv4 * (a.i - b.i);
+          // the 8 orders unrolled: the split halves land in registers
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+               const int j = j0 + e;
+               if (j <= j1) {
+                    const double psi = Psi[j * Np + i], chi = Chi[j * Np + i];
+                    cd a, b;
+                    mie_ab_at(j, x, m, im, cmk(Dre[j * Np + i], Dim[j * Np + i]), psi, psi0, chi, chi0, &a, &b);
+                    const double fv1 = 2. * j + 1.;
+                    se += fv1 * (a.r + b.r);
+                    ss += fv1 * (cabs2(a) + cabs2(b));
+                    cb = cadd(cb, cscale(csub(a, b), fv1 * sgn));
+                    if (j > 1)
+                         sg += p.ft[FT_FV3M * ld + j] * ((ap.r * a.r + ap.i * a.i) + (bp.r * b.r + bp.i * b.i)) +
+                               p.ft[FT_FV4M * ld + j] * (ap.r * bp.r + ap.i * bp.i);
+                    if constexpr (MIX) {
+                         const double sc = fv1 * kSA;
+                         unsigned short h[8];
+                         split16<E>(sc * (a.r + b.r), &h[0], &h[1]);
+                         split16<E>(sc * (a.i + b.i), &h[2], &h[3]);
+                         split16<E>(sc * (a.r - b.r), &h[4], &h[5]);
+                         split16<E>(sc * (a.i - b.i), &h[6], &h[7]);
+#pragma unroll
+                         for (int q = 0; q < 8; ++q) buf[q][e / 2] |= (unsigned) h[q] << (16 * (e & 1));
+                    } else {
+                         const size_t o = (size_t) (j - 1) * Np + i;
+                         const double fv4 = p.ft[FT_FV4 * ld + j];
+                         p.A[o] = fv4 * (a.r + b.r);
+                         p.A[plane + o] = fv4 * (a.i + b.i);
+                         p.A[2 * plane + o] = fv4 * (a.r - b.r);
+                         p.A[3 * plane + o] = fv4 * (a.i - b.i);
+                    }
+                    ap = a;
+                    bp = b;
+                    psi0 = psi;
+                    chi0 = chi;
+                    sgn = -sgn;
                }
-               ap = a;
-               bp = b;
-               sgn = -sgn;
           }
      }
      // operand rows of orders beyond n1 (up to the tile's depth) are zero
      if constexpr (MIX) {
           if (8 * g < kf) {
-               for (int e = (j0 <= j1 ? j1 - j0 + 1 : 0); e < 8; ++e)
-                    for (int q = 0; q < 8; ++q) buf[q][e] = 0;
+#pragma unroll
                for (int q = 0; q < 8; ++q) {
-                    v4u w;
-                    for (int k = 0; k < 4; ++k) w[k] = (unsigned) buf[q][2 * k] | ((unsigned) buf[q][2 * k + 1] << 16);
+                    const v4u w = {buf[q][0], buf[q][1], buf[q][2], buf[q][3]};
                     *(v4u *) (p.A16 + q * p16 + ((size_t) g * Np + i) * 8) = w;
                }
           }
@@ -812,7 +846,7 @@ __global__ __launch_bounds__(256) void weighted_dots(const DotJob *jobs, int n, 
      const DotJob jb = jobs[blockIdx.x];
      double s = 0.;
      for (int i = threadIdx.x; i < n; i += 256) {
-          s += jb.u[i] * jb.v[i];
+          s += jb.u ? jb.u[i] * jb.v[i] : jb.v[i];
           if (jb.u2) s += jb.u2[i] * jb.v2[i];
      }
      red[threadIdx.x] = s;
@@ -2305,23 +2339,26 @@ int ugemm_sched() {
      return v;
 }
 
-// Per-size reverse sweep (adjoint of src/lmie_core.c:298-611), in three
-// kernels over the forward's kept recurrences (D_j up to n2 - 1, psi_j,
-// chi_j) and a_j, b_j:
-//   rev_terms  one thread per (size, group of 8 orders): cotangents of a_j, b_j
-//              from the angular part (Sbar, Tbar) and the series, through
-//              a = (u psi - psi0) / (u zeta - zeta0), u = D/m + j/x and
-//              b (v = m D + j/x) to the local cotangent of D_j and the direct
-//              part of the cotangent of m;
-//   rev_chunk  one thread per (size, chunk of the order range): the adjoint
-//              of the downward recurrence D_j = (j+1)/z - 1/(D_{j+1} + (j+1)/z)
-//              is a first-order linear recurrence upward from j = 1 to n2 - 1
-//              carrying the D cotangent to z = m x; each chunk returns it as an
-//              affine function of the carry entering the chunk;
+// Per-size reverse sweep (adjoint of src/lmie_core.c:298-611) over the
+// forward's kept recurrences (D_j up to n2 - 1, psi_j, chi_j, the reciprocals
+// 1 / (D_{j+1} + (j+1)/z)), in two kernels:
+//   rev_orders one thread per (size, chunk of RCH orders j in [1, n2)):
+//              for j <= n1 the cotangents of a_j, b_j from the angular part
+//              (Sbar, Tbar) and the series, through a = (u psi - psi0) /
+//              (u zeta - zeta0), u = D/m + j/x and b (v = m D + j/x), to the
+//              local cotangent of D_j and the direct part of the cotangent of
+//              m; and the adjoint of the downward recurrence
+//              D_j = (j+1)/z - 1/(D_{j+1} + (j+1)/z), a first-order linear
+//              recurrence upward in j carrying the D cotangent to z = m x,
+//              which the chunk returns as an affine function of the carry
+//              entering it;
 //   rev_carry  one thread per size: chains the chunks, then the per-size totals.
 // Outputs per size: the cotangent of m (re, im as d/dRe + i d/dIm) and of f_i.
+constexpr int RCH = 64;
+
 struct RevArgs {
-     int N, Np, Kp, n1max, n2rows, ng;
+     int N, Np, Kp, n1max, n2rows, nch;
+     const double *ft;          // order_factors rows over 0 .. n1max + 1
      const double *x, *f;
      const int *n1, *n2;
      double mr, mi;
@@ -2334,24 +2371,26 @@ struct RevArgs {
      int Kq;
      const double *fang;        // [Np] (FP64 path)
      double sb0, sb1, sb2, sb3; // cotangents of the raw sums
-     double *Lc;                // [2][n1max + 1][Np] local D cotangents
-     double *rpart;             // [3][ng][Np] partial m cotangent (re, im) and quadratic form
+     double *scan;              // [nch][11][Np]: P, C0, Q, Z0 (re, im), m cotangent (re, im), f quadratic form
      double *mbar;              // [2][Np]
      double *fbar;              // [Np]
 };
 
-__global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
-     const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
-     if (i >= p.Np || g >= p.ng) return;
+// Per chunk, with the carry entering it as c, every carry and D cotangent
+// inside is affine in c: carry out = P c + C0, z cotangent = Q c + Z0.
+__global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
+     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+     if (i >= p.Np) return;
      const size_t Np = p.Np, plane = (size_t) p.Kp * Np, dpl = (size_t) p.n2rows * Np;
-     const size_t ppl = (size_t) (p.n1max + 1) * Np, gp = (size_t) p.ng * Np;
-     const int n1 = i < p.N ? p.n1[i] : 0;
-     const int j0 = 8 * g + 1, j1 = min(8 * g + 8, n1);
-     cd mb = cmk(0., 0.);
+     const size_t ppl = (size_t) (p.n1max + 1) * Np, ld = (size_t) p.n1max + 2;
+     cd P = cmk(1., 0.), C0 = cmk(0., 0.), Q = cmk(0., 0.), Z0 = cmk(0., 0.), mb = cmk(0., 0.);
      double fq = 0.;
-     if (j0 <= j1) {
+     const int n1 = i < p.N ? p.n1[i] : 0, n2 = i < p.N ? p.n2[i] : 0;
+     const int lo = 1 + k * RCH, hi = min(n2, lo + RCH);
+     if (lo < hi) {
           const double x = p.x[i], f = p.f[i];
           const cd m = cmk(p.mr, -p.mi), m2 = cmul(m, m), im1 = crecip(m), im2 = crecip(m2);
+          const cd z = cscale(m, x), iz = crecip(z);
           const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
           // a_n, b_n recomputed from the kept recurrences by the forward's own
           // code (the same bits), one order ahead for the asymmetry series
@@ -2359,152 +2398,111 @@ __global__ __launch_bounds__(64) void rev_terms(RevArgs p) {
                mie_ab_at(n, x, m, im1, cmk(Dre[(size_t) n * Np + i], Dim[(size_t) n * Np + i]), Psi[(size_t) n * Np + i],
                          Psi[(size_t) (n - 1) * Np + i], Chi[(size_t) n * Np + i], Chi[(size_t) (n - 1) * Np + i], a, b);
           };
-          cd am = cmk(0., 0.), bm = cmk(0., 0.), a, b, an = cmk(0., 0.), bn = cmk(0., 0.);
-          if (j0 > 1) ab_at(j0 - 1, &am, &bm);
-          ab_at(j0, &a, &b);
+          cd am = cmk(0., 0.), bm = cmk(0., 0.), a = cmk(0., 0.), b = cmk(0., 0.), an = cmk(0., 0.), bn = cmk(0., 0.);
+          if (lo <= n1) {
+               if (lo > 1) ab_at(lo - 1, &am, &bm);
+               ab_at(lo, &a, &b);
+          }
           const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
           const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
-          double sgn = (j0 & 1) ? -1. : 1.;
-          for (int j = j0; j <= j1; ++j) {
-               if (j < n1) ab_at(j + 1, &an, &bn);
-               const double psi = Psi[j * Np + i], psi0 = Psi[(j - 1) * Np + i];
-               const cd zj = cmk(psi, Chi[j * Np + i]), zjm = cmk(psi0, Chi[(j - 1) * Np + i]);
-               const cd Dj = cmk(Dre[j * Np + i], Dim[j * Np + i]);
-               const double jx = j / x;
-               const double fv1 = 2. * j + 1., fv4 = fv1 / (j * (j + 1.));
-               cd Sb, Tb;
-               if (p.Uf) {
-                    const size_t uq = (size_t) p.Kq * Np, o = (size_t) (j - 1) * Np + i;
-                    const cd us = cmk(p.Uf[o], p.Uf[uq + o]), ut = cmk(p.Uf[2 * uq + o], p.Uf[3 * uq + o]);
-                    // 1/2 Re(conj S U_S + conj T U_T), S = fv4 (a + b), T = fv4 (a - b)
-                    fq += .5 * fv4 * ((a.r + b.r) * us.r + (a.i + b.i) * us.i + (a.r - b.r) * ut.r + (a.i - b.i) * ut.i);
-                    Sb = cscale(us, f);
-                    Tb = cscale(ut, f);
-               } else {
-                    Sb = cmk(p.bar[(size_t) (j - 1) * Np + i], p.bar[plane + (size_t) (j - 1) * Np + i]);
-                    Tb = cmk(p.bar[2 * plane + (size_t) (j - 1) * Np + i], p.bar[3 * plane + (size_t) (j - 1) * Np + i]);
-               }
-               // angular part
-               cd ab = cscale(cadd(Sb, Tb), fv4), bb = cscale(csub(Sb, Tb), fv4);
-               // series parts: ext, sca, bak, asym
-               ab.r += sb0 * fv1;
-               bb.r += sb0 * fv1;
-               ab = cadd(ab, cscale(a, 2. * sb1 * fv1));
-               bb = cadd(bb, cscale(b, 2. * sb1 * fv1));
-               ab = cadd(ab, cscale(cbv, 2. * sb2 * fv1 * sgn));
-               bb = csub(bb, cscale(cbv, 2. * sb2 * fv1 * sgn));
-               if (j < n1) {
-                    const double fv3 = (j + 2.) / ((j + 1.) / j);
-                    ab = cadd(ab, cadd(cscale(an, sb3 * fv3), cscale(b, sb3 * fv4)));
-                    bb = cadd(bb, cadd(cscale(bn, sb3 * fv3), cscale(a, sb3 * fv4)));
-               }
-               if (j > 1) {
-                    const double fv3m = (j + 1.) / ((double) j / (j - 1.));
-                    ab = cadd(ab, cscale(am, sb3 * fv3m));
-                    bb = cadd(bb, cscale(bm, sb3 * fv3m));
-               }
-               // a = (u psi - psi0) / (u zeta - zeta0), da/du = (psi - a zeta) / den
-               const cd u = cadd(cmul(Dj, im1), cmk(jx, 0.));
-               const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
-               const cd dua = cdiv_nr(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
-               const cd dvb = cdiv_nr(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
-               const cd ub = cmul(cconj(dua), ab), vb = cmul(cconj(dvb), bb);
-               // u = D/m + j/x, v = m D + j/x
-               const cd lc = cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb));
-               p.Lc[(size_t) j * Np + i] = lc.r;
-               p.Lc[ppl + (size_t) j * Np + i] = lc.i;
-               mb = cadd(mb, cadd(cmul(cconj(cscale(cmul(Dj, im2), -1.)), ub), cmul(cconj(Dj), vb)));
-               sgn = -sgn;
-               am = a;
-               bm = b;
-               a = an;
-               b = bn;
-          }
-     }
-     double *pt = p.rpart + (size_t) g * Np + i;
-     pt[0] = mb.r;
-     pt[gp] = mb.i;
-     pt[2 * gp] = fq;
-}
-
-// The D-cotangent recurrence of rev_carry in RCK chunks of the order range
-// per size, one thread per (size, chunk): with the carry entering the chunk
-// as c, every carry and D cotangent inside it is affine in c, so the thread
-// runs the chunk once for the affine coefficients and returns
-//   carry out = P c + C0,   z cotangent of the chunk = Q c + Z0
-// (scan[k][8][Np]: P, C0, Q, Z0 as re, im).  rev_carry chains the chunks.
-constexpr int RCK = 8;
-
-__global__ __launch_bounds__(64) void rev_chunk(RevArgs p, double *scan) {
-     const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-     if (i >= p.Np) return;
-     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
-     cd P = cmk(1., 0.), C0 = cmk(0., 0.), Q = cmk(0., 0.), Z0 = cmk(0., 0.);
-     if (i < p.N) {
-          const double x = p.x[i];
-          const int n1 = p.n1[i], n2 = p.n2[i];
-          const int cs = (n2 - 1 + RCK - 1) / RCK, lo = 1 + k * cs, hi = min(n2, lo + cs);
-          const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
-          // blocks of 8 steps: loads and the factors that do not depend on
-          // the carried terms first, then the affine recurrence
-          for (int jb = lo; jb < hi; jb += 8) {
-               cd lc[8], g[8], h[8];
-#pragma unroll
-               for (int e = 0; e < 8; ++e) {
-                    const int j = jb + e;
-                    lc[e] = j <= n1 ? cmk(p.Lc[(size_t) j * Np + i], p.Lc[ppl + (size_t) j * Np + i]) : cmk(0., 0.);
-                    // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z; 1/c2 from the forward
-                    const cd c1 = cscale(iz, j + 1.);
-                    const cd rc = j < hi ? cmk(p.RC[(size_t) j * Np + i], p.RC[dpl + (size_t) j * Np + i]) : cmk(0., 0.);
-                    const cd ic22 = cmul(rc, rc);
-                    const cd dc1dz = cscale(cmul(c1, iz), -1.);
-                    g[e] = cconj(ic22);
-                    h[e] = cconj(cmul(cadd(cmk(1., 0.), ic22), dc1dz));
-               }
-#pragma unroll
-               for (int e = 0; e < 8; ++e) {
-                    if (jb + e < hi) {
-                         // D cotangent = P c + (C0 + lc); carry = g D cotangent
-                         const cd t = cadd(C0, lc[e]);
-                         Q = cadd(Q, cmul(h[e], P));
-                         Z0 = cadd(Z0, cmul(h[e], t));
-                         P = cmul(g[e], P);
-                         C0 = cmul(g[e], t);
+          double sgn = (lo & 1) ? -1. : 1.;
+          for (int j = lo; j < hi; ++j) {
+               cd lc = cmk(0., 0.);
+               if (j <= n1) {
+                    if (j < n1) ab_at(j + 1, &an, &bn);
+                    const double psi = Psi[j * Np + i], psi0 = Psi[(j - 1) * Np + i];
+                    const cd zj = cmk(psi, Chi[j * Np + i]), zjm = cmk(psi0, Chi[(j - 1) * Np + i]);
+                    const cd Dj = cmk(Dre[j * Np + i], Dim[j * Np + i]);
+                    const double jx = j / x;
+                    const double fv1 = 2. * j + 1., fv4 = p.ft[FT_FV4 * ld + j];
+                    cd Sb, Tb;
+                    if (p.Uf) {
+                         const size_t uq = (size_t) p.Kq * Np, o = (size_t) (j - 1) * Np + i;
+                         const cd us = cmk(p.Uf[o], p.Uf[uq + o]), ut = cmk(p.Uf[2 * uq + o], p.Uf[3 * uq + o]);
+                         // 1/2 Re(conj S U_S + conj T U_T), S = fv4 (a + b), T = fv4 (a - b)
+                         fq += .5 * fv4 * ((a.r + b.r) * us.r + (a.i + b.i) * us.i + (a.r - b.r) * ut.r + (a.i - b.i) * ut.i);
+                         Sb = cscale(us, f);
+                         Tb = cscale(ut, f);
+                    } else {
+                         Sb = cmk(p.bar[(size_t) (j - 1) * Np + i], p.bar[plane + (size_t) (j - 1) * Np + i]);
+                         Tb = cmk(p.bar[2 * plane + (size_t) (j - 1) * Np + i], p.bar[3 * plane + (size_t) (j - 1) * Np + i]);
                     }
+                    // angular part
+                    cd ab = cscale(cadd(Sb, Tb), fv4), bb = cscale(csub(Sb, Tb), fv4);
+                    // series parts: ext, sca, bak, asym
+                    ab.r += sb0 * fv1;
+                    bb.r += sb0 * fv1;
+                    ab = cadd(ab, cscale(a, 2. * sb1 * fv1));
+                    bb = cadd(bb, cscale(b, 2. * sb1 * fv1));
+                    ab = cadd(ab, cscale(cbv, 2. * sb2 * fv1 * sgn));
+                    bb = csub(bb, cscale(cbv, 2. * sb2 * fv1 * sgn));
+                    if (j < n1) {
+                         const double fv3 = p.ft[FT_FV3 * ld + j];
+                         ab = cadd(ab, cadd(cscale(an, sb3 * fv3), cscale(b, sb3 * fv4)));
+                         bb = cadd(bb, cadd(cscale(bn, sb3 * fv3), cscale(a, sb3 * fv4)));
+                    }
+                    if (j > 1) {
+                         const double fv3m = p.ft[FT_FV3M * ld + j];
+                         ab = cadd(ab, cscale(am, sb3 * fv3m));
+                         bb = cadd(bb, cscale(bm, sb3 * fv3m));
+                    }
+                    // a = (u psi - psi0) / (u zeta - zeta0), da/du = (psi - a zeta) / den
+                    const cd u = cadd(cmul(Dj, im1), cmk(jx, 0.));
+                    const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
+                    const cd dua = cdiv_nr(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
+                    const cd dvb = cdiv_nr(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
+                    const cd ub = cmul(cconj(dua), ab), vb = cmul(cconj(dvb), bb);
+                    // u = D/m + j/x, v = m D + j/x
+                    lc = cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb));
+                    mb = cadd(mb, cadd(cmul(cconj(cscale(cmul(Dj, im2), -1.)), ub), cmul(cconj(Dj), vb)));
+                    sgn = -sgn;
+                    am = a;
+                    bm = b;
+                    a = an;
+                    b = bn;
                }
+               // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z; 1/c2 from the
+               // forward.  D cotangent = P c + (C0 + lc); carry = g D cotangent
+               const cd c1 = cscale(iz, j + 1.);
+               const cd rc = cmk(p.RC[(size_t) j * Np + i], p.RC[dpl + (size_t) j * Np + i]);
+               const cd ic22 = cmul(rc, rc);
+               const cd g = cconj(ic22);
+               const cd h = cconj(cmul(cadd(cmk(1., 0.), ic22), cscale(cmul(c1, iz), -1.)));
+               const cd t = cadd(C0, lc);
+               Q = cadd(Q, cmul(h, P));
+               Z0 = cadd(Z0, cmul(h, t));
+               P = cmul(g, P);
+               C0 = cmul(g, t);
           }
      }
-     double *o = scan + (size_t) k * 8 * Np + i;
+     double *o = p.scan + (size_t) k * 11 * Np + i;
      o[0] = P.r; o[Np] = P.i; o[2 * Np] = C0.r; o[3 * Np] = C0.i;
      o[4 * Np] = Q.r; o[5 * Np] = Q.i; o[6 * Np] = Z0.r; o[7 * Np] = Z0.i;
+     o[8 * Np] = mb.r; o[9 * Np] = mb.i; o[10 * Np] = fq;
 }
 
-__global__ __launch_bounds__(64) void rev_carry(RevArgs p, const double *scan) {
+__global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
      const int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= p.Np) return;
-     const size_t Np = p.Np, gp = (size_t) p.ng * Np;
+     const size_t Np = p.Np;
      if (i >= p.N) {
           p.mbar[i] = p.mbar[Np + i] = 0.;
           p.fbar[i] = 0.;
           return;
      }
      const double x = p.x[i];
-     const int n1 = p.n1[i];
+     const int nk = (p.n2[i] - 1 + RCH - 1) / RCH;
      // chain the chunks upward from j = 1 with carry 0
-     cd carry = cmk(0., 0.), zb = cmk(0., 0.);
-     for (int k = 0; k < RCK; ++k) {
-          const double *q = scan + (size_t) k * 8 * Np + i;
+     cd carry = cmk(0., 0.), zb = cmk(0., 0.), mb = cmk(0., 0.);
+     double fq = 0.;
+     for (int k = 0; k < nk; ++k) {
+          const double *q = p.scan + (size_t) k * 11 * Np + i;
           const cd P = cmk(q[0], q[Np]), C0 = cmk(q[2 * Np], q[3 * Np]);
           const cd Q = cmk(q[4 * Np], q[5 * Np]), Z0 = cmk(q[6 * Np], q[7 * Np]);
           zb = cadd(zb, cadd(cmul(Q, carry), Z0));
           carry = cadd(cmul(P, carry), C0);
-     }
-     cd mb = cmk(0., 0.);
-     double fq = 0.;
-     for (int g = 0; g < (n1 + 7) / 8; ++g) {
-          const double *pt = p.rpart + (size_t) g * Np + i;
-          mb = cadd(mb, cmk(pt[0], pt[gp]));
-          fq += pt[2 * gp];
+          mb = cadd(mb, cmk(q[8 * Np], q[9 * Np]));
+          fq += q[10 * Np];
      }
      mb = cadd(mb, cscale(zb, x));
      p.mbar[i] = mb.r;
@@ -2514,18 +2512,26 @@ __global__ __launch_bounds__(64) void rev_carry(RevArgs p, const double *scan) {
      p.fbar[i] = (p.Uf ? fq : p.fang[i]) + fser;
 }
 
-// Sum of the per-block partials: one thread per (pass channel, element, angle);
-// channel c of the pass goes to channel map.g[c] of the output.
+// Sum of the per-block partials per (pass channel, element, angle); channel c
+// of the pass goes to channel map.g[c] of the output.
 struct ChMap {
      int g[6];
 };
 
-__global__ void reduce_parts(const double *part, int nblk, int nchp, int Q, ChMap map, double *out) {
-     const int e = blockIdx.x * blockDim.x + threadIdx.x;
+// Eight lanes per output element, each summing every eighth block, then the
+// eight partials in a fixed order: deterministic, and eight times the
+// threads of a plain per-element loop.
+__global__ __launch_bounds__(256) void reduce_parts(const double *part, int nblk, int nchp, int Q, ChMap map,
+                                                    double *out) {
      const int per = nchp * 4 * Q;
-     if (e >= per) return;
+     const int e = blockIdx.x * 32 + (threadIdx.x >> 3), l = threadIdx.x & 7;
      double s = 0.;
-     for (int b = 0; b < nblk; ++b) s += part[(size_t) b * per + e];
+     if (e < per)
+          for (int b = l; b < nblk; b += 8) s += part[(size_t) b * per + e];
+     s += __shfl_xor(s, 1);
+     s += __shfl_xor(s, 2);
+     s += __shfl_xor(s, 4);
+     if (e >= per || l) return;
      const int c = e / (4 * Q), rem = e % (4 * Q);
      out[(size_t) map.g[c] * 4 * Q + rem] = s;
 }
@@ -2724,9 +2730,10 @@ struct Workspace {
      Buf guess, mu, w, Bp, Bm, B16, A16, T, scrD, scrR, A, S, part, pf, E, coefs, dots, dotjobs, save2, order;
      std::vector<double> hsc;       // host staging of sin x, cos x
      // reverse-pass checkpoint and buffers
-     Buf CB, sst, pfbar, bar, fang, mbar, fbar, gbar, ones, expart;
+     Buf CB, sst, pfbar, bar, fang, mbar, fbar, gbar, expart;
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
-     Buf chD, chPC, chPart, chRC, revLc, revPart, revScan;   // per-size recurrences of the value channel
+     Buf chD, chPC, chPart, chRC, revScan;   // per-size recurrences of the value channel
+     Buf ftab;                      // order_factors of the value channel
      int n2rows = 0, ng = 0;
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
@@ -3067,8 +3074,12 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           ca.d_all = 0; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
           ca.RC = keep ? w->chRC.as<double>() : nullptr;
           hipLaunchKernelGGL(mie_chains, dim3(Np / 64, 2), dim3(64), 0, s, ca);
+          if (w->ftab.get(sizeof(double) * FT_ROWS * (size_t) (n1max + 2))) return -1;
+          hipLaunchKernelGGL(order_factors, dim3((n1max + 2 + 255) / 256), dim3(256), 0, s, n1max + 1,
+                             w->ftab.as<double>());
           TermArgs ta;
           ta.N = N; ta.Np = Np; ta.n1max = n1max; ta.n2rows = n2rows; ta.Kp = Kp; ta.Kb = Kb; ta.ng = ng;
+          ta.ft = w->ftab.as<double>();
           ta.x = w->x.as<double>(); ta.n1 = w->n1.as<int>(); ta.kfill = w->kfill.as<int>();
           ta.mr = J->mr; ta.mi = J->mi; ta.D = w->chD.as<double>(); ta.PC = w->chPC.as<double>();
           ta.A = mixed ? nullptr : w->A.as<double>();
@@ -3252,7 +3263,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                for (int q = 0; c < P.nwp; ++q) map.g[c++] = 1 + nt + P.w0 + q;
                for (int q = 0; q < P.ntp; ++q) map.g[c++] = 1 + P.t0 + q;
                const int per = (P.nwp + P.ntp) * 4 * Q;
-               hipLaunchKernelGGL(reduce_parts, dim3((per + 255) / 256), dim3(256), 0, s,
+               hipLaunchKernelGGL(reduce_parts, dim3((per + 31) / 32), dim3(256), 0, s,
                                   (const double *) (w->part.as<double>() + P.off), P.nblk, P.nwp + P.ntp, Q, map,
                                   w->pf.as<double>());
           }
@@ -3367,16 +3378,18 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      if (w->pfbar.get(sizeof(double) * 4 * (size_t) (Q > 0 ? Q : 1)) || w->fang.get(sizeof(double) * Np) ||
          w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * Np) ||
          w->gbar.get(sizeof(double) * 12 * (size_t) (C > 0 ? C : 1)) ||
-         w->bar.get(sizeof(double) * 4 * (size_t) Kp * Np) || w->ones.get(sizeof(double) * Np))
+         w->bar.get(sizeof(double) * 4 * (size_t) Kp * Np))
           return -1;
+     const int use_gc = in->gc_bar && w->calc_gc, use_lc = in->lc_bar && w->calc_lc;
+     // the cotangent inputs go to the device before the timed pipeline
+     double *g = w->gbar.as<double>();
+     if (Q > 0 && use_gc) HIPCHECK(hipMemcpyAsync(g, in->gc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
+     if (Q > 0 && use_lc)
+          HIPCHECK(hipMemcpyAsync(g + 6 * C, in->lc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
      tm.mark(LMIE_T_TOTAL, 0);
      HIPCHECK(hipMemsetAsync(w->fang.p, 0, sizeof(double) * Np, s));
-     const int use_gc = in->gc_bar && w->calc_gc, use_lc = in->lc_bar && w->calc_lc;
      if (Q > 0 && (use_gc || use_lc)) {
           tm.mark(LMIE_T_ADJ_ANGULAR, 0);
-          double *g = w->gbar.as<double>();
-          if (use_gc) HIPCHECK(hipMemcpyAsync(g, in->gc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
-          if (use_lc) HIPCHECK(hipMemcpyAsync(g + 6 * C, in->lc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
           const int nck = (C + EXA_L - 1) / EXA_L;
           if (w->expart.get(sizeof(double) * 24 * (size_t) nck * Hp)) return -1;
           hipLaunchKernelGGL(expand_adjoint_part, dim3(Hp / 64, nck), dim3(64), 0, s, (const double *) w->T.as<double>(),
@@ -3406,13 +3419,11 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      }
      const int use_u = w->mixed && Q > 0 && (use_gc || use_lc);
      tm.mark(LMIE_T_ADJ_COEFS, 0);
-     const int ng = w->ng;
-     if (w->revLc.get(sizeof(double) * 2 * (size_t) (w->n1max + 1) * Np) ||
-         w->revPart.get(sizeof(double) * 3 * (size_t) (ng > 0 ? ng : 1) * Np) ||
-         w->revScan.get(sizeof(double) * 8 * RCK * (size_t) Np))
-          return -1;
+     const int nch = (w->n2rows - 1 + RCH - 1) / RCH;
+     if (w->revScan.get(sizeof(double) * 11 * (size_t) (nch > 0 ? nch : 1) * Np)) return -1;
      RevArgs r;
-     r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.ng = ng;
+     r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.nch = nch;
+     r.ft = w->ftab.as<double>();
      r.x = w->x.as<double>(); r.f = w->f.as<double>();
      r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
      r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>(); r.RC = w->chRC.as<double>();
@@ -3420,19 +3431,16 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
      r.fang = w->fang.as<double>();
      r.sb0 = in->sums_bar[0]; r.sb1 = in->sums_bar[1]; r.sb2 = in->sums_bar[2]; r.sb3 = in->sums_bar[3];
-     r.Lc = w->revLc.as<double>(); r.rpart = w->revPart.as<double>();
+     r.scan = w->revScan.as<double>();
      r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
-     if (ng > 0) hipLaunchKernelGGL(rev_terms, dim3(Np / 64, ng), dim3(64), 0, s, r);
-     hipLaunchKernelGGL(rev_chunk, dim3(Np / 64, RCK), dim3(64), 0, s, r, w->revScan.as<double>());
-     hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r, (const double *) w->revScan.as<double>());
+     if (nch > 0) hipLaunchKernelGGL(rev_orders, dim3(Np / 64, nch), dim3(64), 0, s, r);
+     hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r);
      HIPCHECK(hipGetLastError());
      tm.mark(LMIE_T_ADJ_COEFS, 1);
      // sum of the per-size m cotangents
      {
-          std::vector<double> one(Np, 1.);
-          HIPCHECK(hipMemcpyAsync(w->ones.p, one.data(), sizeof(double) * Np, hipMemcpyHostToDevice, s));
-          DotJob jobs[2] = {{w->ones.as<double>(), w->mbar.as<double>(), nullptr, nullptr},
-                            {w->ones.as<double>(), w->mbar.as<double>() + Np, nullptr, nullptr}};
+          DotJob jobs[2] = {{nullptr, w->mbar.as<double>(), nullptr, nullptr},
+                            {nullptr, w->mbar.as<double>() + Np, nullptr, nullptr}};
           if (w->dotjobs.get(sizeof(jobs)) || w->dots.get(sizeof(double) * 2)) return -1;
           HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs, sizeof(jobs), hipMemcpyHostToDevice, s));
           hipLaunchKernelGGL(weighted_dots, dim3(2), dim3(256), 0, s, (const DotJob *) w->dotjobs.as<DotJob>(), Np,
