@@ -808,21 +808,34 @@ __global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
      pt[4 * gp] = sg;
 }
 
-// Series per size from the group partials (src/lmie_core.c:586-596).
+// Series per size from the group partials (src/lmie_core.c:586-596): a block
+// of 32 sizes x 8 lanes, lane l adds the groups g = l mod 8, then the eight
+// lane sums are added in a fixed order (deterministic).
 __global__ __launch_bounds__(256) void mie_sums(const double *part, int N, int Np, int ng, const int *n1, double *S,
                                                 double *CB) {
-     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-     if (i >= Np) return;
+     __shared__ double red[5][8][32];
+     const int il = threadIdx.x & 31, l = threadIdx.x >> 5, i = blockIdx.x * 32 + il;
      const size_t gp = (size_t) ng * Np;
+     double v[5] = {0., 0., 0., 0., 0.};
+     if (i < Np) {
+          const int gn = i < N ? (n1[i] + 7) / 8 : 0;
+          for (int g = l; g < gn; g += 8) {
+               const double *pt = part + (size_t) g * Np + i;
+#pragma unroll
+               for (int q = 0; q < 5; ++q) v[q] += pt[q * gp];
+          }
+     }
+#pragma unroll
+     for (int q = 0; q < 5; ++q) red[q][l][il] = v[q];
+     __syncthreads();
+     if (l || i >= Np) return;
      double se = 0., ss = 0., sg = 0.;
      cd cb = cmk(0., 0.);
-     const int gn = i < N ? (n1[i] + 7) / 8 : 0;
-     for (int g = 0; g < gn; ++g) {
-          const double *pt = part + (size_t) g * Np + i;
-          se += pt[0];
-          ss += pt[gp];
-          cb = cadd(cb, cmk(pt[2 * gp], pt[3 * gp]));
-          sg += pt[4 * gp];
+     for (int k = 0; k < 8; ++k) {
+          se += red[0][k][il];
+          ss += red[1][k][il];
+          cb = cadd(cb, cmk(red[2][k][il], red[3][k][il]));
+          sg += red[4][k][il];
      }
      S[i] = se;
      S[Np + i] = ss;
@@ -2722,7 +2735,7 @@ struct Workspace {
      int device = -1;
      hipStream_t stream = nullptr;
      hipStream_t side = nullptr;     // angle tables, concurrent with the per-size recurrences
-     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
+     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, sums = nullptr;
      std::vector<double> hguess;     // host staging of the Gauss root guesses
      std::vector<int2> htask;        // host staging of the contraction task list
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
@@ -2766,6 +2779,7 @@ int ws_acquire(int device, Workspace **out) {
           HIPCHECK(hipEventCreateWithFlags(&w->fork, hipEventDisableTiming));
           HIPCHECK(hipEventCreateWithFlags(&w->join, hipEventDisableTiming));
           HIPCHECK(hipEventCreateWithFlags(&w->join2, hipEventDisableTiming));
+          HIPCHECK(hipEventCreateWithFlags(&w->sums, hipEventDisableTiming));
           for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
      }
      *out = w;
@@ -2994,11 +3008,14 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      }
      tm.mark(LMIE_T_TOTAL, 0);
 
-     // angle tables, on the side stream while the per-size recurrences run
+     // angle tables, on the side stream while the per-size recurrences run;
+     // the side stream forks here, its host preparation and launches follow
+     // the first per-size launch (side_tables) so the device starts at once
      hipStream_t ts = w->side;
      HIPCHECK(hipEventRecord(w->fork, s));
      HIPCHECK(hipStreamWaitEvent(ts, w->fork, 0));
      tm.mark(LMIE_T_TABLES, 0, ts);
+     auto side_tables = [&]() -> int {
      if (Q > 0) {
           hipStream_t s = ts;
           if (w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) ||
@@ -3038,7 +3055,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      }
      tm.mark(LMIE_T_TABLES, 1, ts);
      if (Q <= 0) HIPCHECK(hipEventRecord(w->join, ts));
-     HIPCHECK(hipEventRecord(w->join2, ts));
+     return 0;
+     };
 
      // per-size coefficients (passes of up to 4 tangent channels)
      const size_t plane = (size_t) Kp * Np, srow = (size_t) (n1max + 1) * Np;
@@ -3074,6 +3092,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           ca.d_all = 0; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
           ca.RC = keep ? w->chRC.as<double>() : nullptr;
           hipLaunchKernelGGL(mie_chains, dim3(Np / 64, 2), dim3(64), 0, s, ca);
+          if (side_tables()) return -1;
           if (w->ftab.get(sizeof(double) * FT_ROWS * (size_t) (n1max + 2))) return -1;
           hipLaunchKernelGGL(order_factors, dim3((n1max + 2 + 255) / 256), dim3(256), 0, s, n1max + 1,
                              w->ftab.as<double>());
@@ -3091,12 +3110,13 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                else if (bf16) hipLaunchKernelGGL(mie_terms<BF16>, grid, dim3(64), 0, s, ta);
                else hipLaunchKernelGGL(mie_terms<F16>, grid, dim3(64), 0, s, ta);
           }
-          hipLaunchKernelGGL(mie_sums, dim3(Np / 256 + 1), dim3(256), 0, s, (const double *) w->chPart.as<double>(), N, Np,
+          hipLaunchKernelGGL(mie_sums, dim3((Np + 31) / 32), dim3(256), 0, s, (const double *) w->chPart.as<double>(), N, Np,
                              ng, (const int *) w->n1.as<int>(), w->S.as<double>(), keep ? w->CB.as<double>() : nullptr);
           HIPCHECK(hipGetLastError());
           w->n2rows = n2rows;
           w->ng = ng;
      }
+     if (nt > 0 && side_tables()) return -1;
      for (int t0 = 0; nt > 0 && t0 < nt; t0 += 4) {
           CoefArgs c;
           memset(&c, 0, sizeof c);
@@ -3135,11 +3155,16 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                for (int q = 0; q < 4; ++q)
                     jobs.push_back(DotJob{w->dfw.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, nullptr, nullptr});
           if (w->dotjobs.get(sizeof(DotJob) * jobs.size()) || w->dots.get(sizeof(double) * jobs.size())) return -1;
-          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs.data(), sizeof(DotJob) * jobs.size(), hipMemcpyHostToDevice, s));
+          // on the side stream after the angle tables (beside the contraction);
+          // the main stream joins it (join2) before the expansion
+          HIPCHECK(hipEventRecord(w->sums, s));
+          HIPCHECK(hipStreamWaitEvent(ts, w->sums, 0));
+          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs.data(), sizeof(DotJob) * jobs.size(), hipMemcpyHostToDevice, ts));
           tm.mark(LMIE_T_REDUCE, 0);
-          hipLaunchKernelGGL(weighted_dots, dim3(jobs.size()), dim3(256), 0, s, w->dotjobs.as<DotJob>(), Np,
+          hipLaunchKernelGGL(weighted_dots, dim3(jobs.size()), dim3(256), 0, ts, w->dotjobs.as<DotJob>(), Np,
                              w->dots.as<double>());
           HIPCHECK(hipGetLastError());
+          HIPCHECK(hipEventRecord(w->join2, ts));
      }
 
      // angular contraction and reduction (after the angle tables)
@@ -3291,6 +3316,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      } else {
           tm.mark(LMIE_T_REDUCE, 1);
      }
+     HIPCHECK(hipStreamWaitEvent(s, w->join2, 0));   // the series dots
      tm.mark(LMIE_T_TOTAL, 1);
      // results
      std::vector<double> dots((size_t) 4 * nch);
