@@ -3037,14 +3037,17 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                else
                     hipLaunchKernelGGL(pitau_table16<F16>, dim3(Hb / 64), dim3(64), 0, s, H, Hb, n1max, Kb,
                                        (const double *) w->mu.as<double>(), w->B16.as<unsigned short>());
+               // the mixed contraction needs only the split tables; the FP64
+               // tables below serve the reverse pass, which runs after join2
+               HIPCHECK(hipEventRecord(w->join, s));
           }
           if (!mixed || J->keep) {
                hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kp, w->mu.as<double>(),
                                   w->Bp.as<double>(), w->Bm.as<double>());
           }
-          // the contraction needs the tables above; the expansion tables below
-          // only the expansion
-          HIPCHECK(hipEventRecord(w->join, s));
+          // the FP64 contraction needs the tables above; the expansion tables
+          // below only the expansion
+          if (!mixed) HIPCHECK(hipEventRecord(w->join, s));
           if (J->calc_gc || J->calc_lc) {
                const int planes = J->calc_lc ? 5 : 4;
                if (w->T.get(sizeof(double) * planes * (size_t) C * Hp)) return -1;
