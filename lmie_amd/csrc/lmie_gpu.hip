@@ -2735,7 +2735,8 @@ struct Workspace {
      int device = -1;
      hipStream_t stream = nullptr;
      hipStream_t side = nullptr;     // angle tables, concurrent with the per-size recurrences
-     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, sums = nullptr;
+     hipStream_t side2 = nullptr;    // expansion tables, from the Gauss rule on
+     hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, sums = nullptr, nodes = nullptr, join3 = nullptr;
      std::vector<double> hguess;     // host staging of the Gauss root guesses
      std::vector<int2> htask;        // host staging of the contraction task list
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
@@ -2776,10 +2777,13 @@ int ws_acquire(int device, Workspace **out) {
           w->device = device;
           HIPCHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
           HIPCHECK(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
+          HIPCHECK(hipStreamCreateWithFlags(&w->side2, hipStreamNonBlocking));
           HIPCHECK(hipEventCreateWithFlags(&w->fork, hipEventDisableTiming));
           HIPCHECK(hipEventCreateWithFlags(&w->join, hipEventDisableTiming));
           HIPCHECK(hipEventCreateWithFlags(&w->join2, hipEventDisableTiming));
           HIPCHECK(hipEventCreateWithFlags(&w->sums, hipEventDisableTiming));
+          HIPCHECK(hipEventCreateWithFlags(&w->nodes, hipEventDisableTiming));
+          HIPCHECK(hipEventCreateWithFlags(&w->join3, hipEventDisableTiming));
           for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
      }
      *out = w;
@@ -3029,6 +3033,17 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           }
           hipLaunchKernelGGL(gauss_nodes, dim3((H + 63) / 64), dim3(64), 0, s, Q, (const double *) w->guess.as<double>(),
                              w->mu.as<double>(), w->w.as<double>());
+          // the expansion tables on a third stream from here: they run beside
+          // the per-size recurrences rather than beside the contraction
+          if (J->calc_gc || J->calc_lc) {
+               const int planes = J->calc_lc ? 5 : 4;
+               if (w->T.get(sizeof(double) * planes * (size_t) C * Hp)) return -1;
+               HIPCHECK(hipEventRecord(w->nodes, s));
+               HIPCHECK(hipStreamWaitEvent(w->side2, w->nodes, 0));
+               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64, planes), dim3(64), 0, w->side2, H, Hp, C,
+                                  (const double *) w->mu.as<double>(), w->T.as<double>());
+          }
+          HIPCHECK(hipEventRecord(w->join3, w->side2));
           if (mixed) {
                if (w->B16.get(sizeof(unsigned short) * 4 * (size_t) Kb * Hb * 8)) return -1;
                if (bf16)
@@ -3048,12 +3063,6 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           // the FP64 contraction needs the tables above; the expansion tables
           // below only the expansion
           if (!mixed) HIPCHECK(hipEventRecord(w->join, s));
-          if (J->calc_gc || J->calc_lc) {
-               const int planes = J->calc_lc ? 5 : 4;
-               if (w->T.get(sizeof(double) * planes * (size_t) C * Hp)) return -1;
-               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64, planes), dim3(64), 0, s, H, Hp, C,
-                                  (const double *) w->mu.as<double>(), w->T.as<double>());
-          }
           HIPCHECK(hipGetLastError());
      }
      tm.mark(LMIE_T_TABLES, 1, ts);
@@ -3300,6 +3309,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
 
           // expansion (after the expansion tables)
           HIPCHECK(hipStreamWaitEvent(s, w->join2, 0));
+          HIPCHECK(hipStreamWaitEvent(s, w->join3, 0));
           tm.mark(LMIE_T_EXPAND, 0);
           if (w->E.get(sizeof(double) * (size_t) nch * 12 * Hp) || w->coefs.get(sizeof(double) * (size_t) nch * 6 * C))
                return -1;
@@ -3320,6 +3330,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           tm.mark(LMIE_T_REDUCE, 1);
      }
      HIPCHECK(hipStreamWaitEvent(s, w->join2, 0));   // the series dots
+     HIPCHECK(hipStreamWaitEvent(s, w->join3, 0));   // the expansion tables
      tm.mark(LMIE_T_TOTAL, 1);
      // results
      std::vector<double> dots((size_t) 4 * nch);
@@ -3582,6 +3593,7 @@ int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
      int rc = forward_impl(job, res, w);
      if (rc) {
           (void) hipStreamSynchronize(w->side);
+          (void) hipStreamSynchronize(w->side2);
           (void) hipStreamSynchronize(w->stream);
      }
      if (rc == 0 && job->keep) {
