@@ -602,7 +602,6 @@ struct ChainArgs {
      int d_all;                 // store D_j up to n2 - 1 (reverse pass) instead of n1
      double *D;                 // [2][n2rows][Np]: D_j re, im at row j
      double *PC;                // [2][n1max + 1][Np]: psi_j, chi_j at row j
-     double *RC;                // [2][n2rows][Np]: 1 / (D_{j+1} + (j+1)/z) at row j (reverse pass) or NULL
 };
 
 __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
@@ -627,10 +626,6 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
                if (j <= dtop) {
                     Dre[j * Np + i] = D.r;
                     Dim[j * Np + i] = D.i;
-               }
-               if (p.RC) {
-                    p.RC[(size_t) j * Np + i] = rc.r;
-                    p.RC[dpl + (size_t) j * Np + i] = rc.i;
                }
           }
           return;
@@ -2376,7 +2371,6 @@ struct RevArgs {
      const int *n1, *n2;
      double mr, mi;
      const double *D, *PC;      // forward recurrences (mie_chains, D up to n2 - 1)
-     const double *RC;          // forward reciprocals 1 / (D_{j+1} + (j+1)/z), rows 1 .. n2 - 1
      const double *CB;          // [2][Np]
      const double *S;           // series [4][Np]
      const double *bar;         // [4][Kp][Np] f-weighted cotangents (FP64 path), or
@@ -2474,10 +2468,12 @@ __global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
                     a = an;
                     b = bn;
                }
-               // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z; 1/c2 from the
-               // forward.  D cotangent = P c + (C0 + lc); carry = g D cotangent
+               // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z, D_{n2} = 0; 1/c2
+               // by the forward's own operations (the same bits).  D cotangent
+               // = P c + (C0 + lc); carry = g D cotangent
                const cd c1 = cscale(iz, j + 1.);
-               const cd rc = cmk(p.RC[(size_t) j * Np + i], p.RC[dpl + (size_t) j * Np + i]);
+               const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
+               const cd rc = crecip_nr(cadd(Dn, c1));
                const cd ic22 = cmul(rc, rc);
                const cd g = cconj(ic22);
                const cd h = cconj(cmul(cadd(cmk(1., 0.), ic22), cscale(cmul(c1, iz), -1.)));
@@ -2746,7 +2742,7 @@ struct Workspace {
      // reverse-pass checkpoint and buffers
      Buf CB, sst, pfbar, bar, fang, mbar, fbar, gbar, expart;
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
-     Buf chD, chPC, chPart, chRC, revScan;   // per-size recurrences of the value channel
+     Buf chD, chPC, chPart, revScan;   // per-size recurrences of the value channel
      Buf ftab;                      // order_factors of the value channel
      int n2rows = 0, ng = 0;
      int valid = 0;
@@ -3093,7 +3089,6 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           for (int t = 0; t < nt64; ++t) kfmax = kfill[t] > kfmax ? kfill[t] : kfmax;
           const int n2rows = n2max + 1, ng = kfmax / 8;
           if (w->chD.get(sizeof(double) * 2 * (size_t) n2rows * Np) ||
-              (keep && w->chRC.get(sizeof(double) * 2 * (size_t) n2rows * Np)) ||
               w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) ||
               w->chPart.get(sizeof(double) * 5 * (size_t) (ng > 0 ? ng : 1) * Np))
                return -1;
@@ -3101,8 +3096,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           ca.N = N; ca.Np = Np; ca.n1max = n1max; ca.n2rows = n2rows;
           ca.x = w->x.as<double>(); ca.sx = w->sx.as<double>(); ca.cx = w->cx.as<double>();
           ca.n1 = w->n1.as<int>(); ca.n2 = w->n2.as<int>(); ca.mr = J->mr; ca.mi = J->mi;
-          ca.d_all = 0; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
-          ca.RC = keep ? w->chRC.as<double>() : nullptr;
+          ca.d_all = keep; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
           hipLaunchKernelGGL(mie_chains, dim3(Np / 64, 2), dim3(64), 0, s, ca);
           if (side_tables()) return -1;
           if (w->ftab.get(sizeof(double) * FT_ROWS * (size_t) (n1max + 2))) return -1;
@@ -3466,7 +3460,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.ft = w->ftab.as<double>();
      r.x = w->x.as<double>(); r.f = w->f.as<double>();
      r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
-     r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>(); r.RC = w->chRC.as<double>();
+     r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>();
      r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
      r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
      r.fang = w->fang.as<double>();
