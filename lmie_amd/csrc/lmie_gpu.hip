@@ -1923,7 +1923,11 @@ __global__ __launch_bounds__(64) void adj_tables(const double *Bp, const double 
           split16<E>(pm, &lv[2][e], &lv[3][e]);
           for (int q = 0; q < 8; ++q) {
                const bool plus = q == 0 || q == 3 || q == 6 || q == 7;
-               split16<E>(wts[(size_t) q * Hq + j] * (plus ? pp : pm), &rv[2 * q][e], &rv[2 * q + 1][e]);
+               // the planes of the sigma-sigma products (q = 1, 3, 6, 7) carry
+               // sigma_m = (-1)^r of their order (an exact sign change)
+               const bool odd = (q == 1 || q == 3 || q == 6 || q == 7) && (r & 1);
+               const double v = wts[(size_t) q * Hq + j] * (plus ? pp : pm);
+               split16<E>(odd ? -v : v, &rv[2 * q][e], &rv[2 * q + 1][e]);
           }
      }
      for (int q = 0; q < 4; ++q) {
@@ -1938,23 +1942,31 @@ __global__ __launch_bounds__(64) void adj_tables(const double *Bp, const double 
      }
 }
 
-// G on the matrix cores: one workgroup per 64 x 64 tile of (n, m), contraction
-// over the positive nodes in stages of 32, two LDS stages as angular_mix2.
-// Accumulators (left x right): 0 P+ a P+, 1 P- a' P-, 2 P- c P-, 3 P+ c' P+,
-// 4 P+ br P-, 5 P- b'r P+, 6 P+ bi P-, 7 P- b'i P+.  Output in fp32,
-// Gf[4][Kq][Kq] = Gss, Gtt, Re Gst, Im Gst (row n, column m, scaled by
-// kSB^2 / (wmax n (n+1) m (m+1))), and the largest magnitude.
+// G on the 32x32x16 matrix instruction: one workgroup per 64 x 64 tile of
+// (n, m), four waves of 32 x 32, contraction over the positive nodes in
+// stages of 16 (two LDS stages, 80 KiB: two workgroups per CU).  Each G
+// element is a sum of two products, the second with the factor
+// sigma_n sigma_m = (-1)^(n+m): the right planes of those products carry
+// sigma_m (adj_tables) and the left fragment is negated on the lanes of odd
+// rows, so both products accumulate into one accumulator:
+// 0 P+ a P+ + P- a' P-, 1 P- c P- + P+ c' P+, 2 P+ br P- + P- b'r P+,
+// 3 P+ bi P- + P- b'i P+.  Output in fp32, Gf[4][Kq][Kq] = Gss, Gtt, Re Gst,
+// Im Gst (row n, column m, scaled by kSB^2 / (wmax n (n+1) m (m+1))), and the
+// largest magnitude.
 template <class E>
 __global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, const unsigned short *R16, int Hq,
                                                   int Kq, float *Gf, AdjScal *sc) {
-     constexpr int T = 64, G = KM / 8;
+     constexpr int T = 64, G = 2, KS = 8 * G;
      constexpr int NL = 4 * G * T / 256, NR = 16 * G * T / 256;
      __shared__ v4u sL[2][4 * G][T];
      __shared__ v4u sR[2][16 * G][T];
-     const int n0 = blockIdx.x * T, m0 = blockIdx.y * T, nch = Hq / KM;
+     const int n0 = blockIdx.x * T, m0 = blockIdx.y * T, nch = Hq / KS;
      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-     const int wa = wave & 1, wb = wave >> 1, l16 = lane & 15, lg = lane >> 4;
+     const int wa = wave & 1, wb = wave >> 1, l32 = lane & 31, lh = lane >> 5;
      const size_t plane = (size_t) (Hq / 8) * Kq * 8;
+     // odd rows n (lane parity) negate the left fragment of the sigma-sigma products
+     const unsigned sg = (lane & 1) ? 0x80008000u : 0u;
+     const v4u neg = {sg, sg, sg, sg};
      v4u rl[NL], rr[NR];
      const Stager<256, T, G> stL(L16, plane, (size_t) Kq * 8, n0, threadIdx.x);
      const Stager<256, T, G> stR(R16, plane, (size_t) Kq * 8, m0, threadIdx.x);
@@ -1976,65 +1988,48 @@ __global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, con
                sR[s][e / T][e % T] = rr[t];
           }
      };
-     v4f acc[2][2][8];
+     v16f acc[4];
 #pragma unroll
-     for (int a = 0; a < 2; ++a)
+     for (int k = 0; k < 4; ++k)
 #pragma unroll
-          for (int b = 0; b < 2; ++b)
-#pragma unroll
-               for (int k = 0; k < 8; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
+          for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
      load(0);
      store(0);
      if (nch > 1) load(1);
      for (int c = 0; c < nch; ++c) {
           const int s = c & 1;
           __syncthreads();
+          // lane l: row (n or m) l % 32, nodes 8 (l / 32) + e of the stage
+          v4u y[16], x[8];
 #pragma unroll
-          for (int b = 0; b < 2; ++b) {
-               const int mc = wb * 32 + 16 * b + l16;
-               v4u y[16];
+          for (int q = 0; q < 16; ++q) y[q] = sR[s][q * G + lh][wb * 32 + l32];
+          // left planes: 0 P+ hi, 1 P+ lo, 2 P- hi, 3 P- lo; 4..7 the same
+          // negated on odd rows
 #pragma unroll
-               for (int q = 0; q < 16; ++q) y[q] = sR[s][q * G + lg][mc];
+          for (int q = 0; q < 4; ++q) x[q] = sL[s][q * G + lh][wa * 32 + l32];
 #pragma unroll
-               for (int a = 0; a < 2; ++a) {
-                    const int nc = wa * 32 + 16 * a + l16;
-                    v4u x[4];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) x[q] = sL[s][q * G + lg][nc];
-                    v4f *cc = acc[a][b];
-                    // left planes: 0 P+ hi, 1 P+ lo, 2 P- hi, 3 P- lo
-#define GB(k, L, R) cc[k] = E::mfma(x[L], y[2 * (R)], cc[k]); cc[k] = E::mfma(x[L], y[2 * (R) + 1], cc[k]); \
-                    cc[k] = E::mfma(x[(L) + 1], y[2 * (R)], cc[k]);
-                    GB(0, 0, 0) GB(1, 2, 1) GB(2, 2, 2) GB(3, 0, 3)
-                    GB(4, 0, 4) GB(5, 2, 6) GB(6, 0, 5) GB(7, 2, 7)
+          for (int q = 0; q < 4; ++q) x[4 + q] = x[q] ^ neg;
+          if (c + 1 < nch) store(s ^ 1);
+          if (c + 2 < nch) load(c + 2);
+#define GB(k, L, R) acc[k] = E::mfma32(x[L], y[2 * (R)], acc[k]); acc[k] = E::mfma32(x[L], y[2 * (R) + 1], acc[k]); \
+          acc[k] = E::mfma32(x[(L) + 1], y[2 * (R)], acc[k]);
+          GB(0, 0, 0) GB(1, 2, 2) GB(2, 0, 4) GB(3, 0, 5)
+          GB(0, 6, 1) GB(1, 4, 3) GB(2, 6, 6) GB(3, 6, 7)
 #undef GB
-               }
-               if (b == 0) {
-                    if (c + 1 < nch) store(s ^ 1);
-                    if (c + 2 < nch) load(c + 2);
-               }
-          }
      }
      float mx = 0.f;
 #pragma unroll
-     for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b)
-#pragma unroll
-               for (int v = 0; v < 4; ++v) {
-                    const int r = n0 + wa * 32 + 16 * a + mfma16_row(lane, v);
-                    const int cm = m0 + wb * 32 + 16 * b + l16;
-                    const float ss = ((r + cm) & 1) ? -1.f : 1.f;
-                    const v4f *q = acc[a][b];
-                    const float g0 = q[0][v] + ss * q[1][v], g1 = q[2][v] + ss * q[3][v];
-                    const float g2 = q[4][v] + ss * q[5][v], g3 = q[6][v] + ss * q[7][v];
-                    const size_t o = (size_t) r * Kq + cm, pl = (size_t) Kq * Kq;
-                    Gf[o] = g0;
-                    Gf[pl + o] = g1;
-                    Gf[2 * pl + o] = g2;
-                    Gf[3 * pl + o] = g3;
-                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(g0), fabsf(g1)), fmaxf(fabsf(g2), fabsf(g3))));
-               }
+     for (int v = 0; v < 16; ++v) {
+          const int r = n0 + wa * 32 + mfma32_row(lane, v);
+          const int cm = m0 + wb * 32 + l32;
+          const float g0 = acc[0][v], g1 = acc[1][v], g2 = acc[2][v], g3 = acc[3][v];
+          const size_t o = (size_t) r * Kq + cm, pl = (size_t) Kq * Kq;
+          Gf[o] = g0;
+          Gf[pl + o] = g1;
+          Gf[2 * pl + o] = g2;
+          Gf[3 * pl + o] = g3;
+          mx = fmaxf(mx, fmaxf(fmaxf(fabsf(g0), fabsf(g1)), fmaxf(fabsf(g2), fabsf(g3))));
+     }
      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
      if (lane == 0) atomicMax(&sc->gmax, __float_as_uint(mx));
 }
