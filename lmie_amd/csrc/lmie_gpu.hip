@@ -68,6 +68,9 @@ struct F16 {
           *rounded = (double) (float) h;
           return __builtin_bit_cast(unsigned short, h);
      }
+     static __device__ __forceinline__ double value(unsigned short b) {
+          return (double) (float) __builtin_bit_cast(_Float16, b);
+     }
      static __device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c) {
           return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(v8h, a), __builtin_bit_cast(v8h, b), c, 0, 0, 0);
      }
@@ -82,6 +85,9 @@ struct BF16 {
           __bf16 h = (__bf16) (float) v;
           *rounded = (double) (float) h;
           return __builtin_bit_cast(unsigned short, h);
+     }
+     static __device__ __forceinline__ double value(unsigned short b) {
+          return (double) __builtin_bit_cast(float, (unsigned) b << 16);
      }
      static __device__ __forceinline__ v4f mfma(v4u a, v4u b, v4f c) {
           return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8b, a), __builtin_bit_cast(v8b, b), c, 0, 0, 0);
@@ -1899,28 +1905,32 @@ __global__ __launch_bounds__(1024) void adj_weights(const double *pfbar, int H, 
 }
 
 // Split tables with angles along the contraction: [plane][Hq / 8][Kq][8],
-// 8 consecutive positive nodes per 16 bytes.  Left (4 planes): P+ and P- of
-// order r + 1 over r (r + 1) (r + 2) times kSB, high and low parts.  Right
-// (16 planes): weight q times P+ (q = 0, 3, 6, 7) or P- (q = 1, 2, 4, 5),
+// 8 consecutive positive nodes per 16 bytes, from the forward's split tables
+// B16 ([4][Kb][Hb][8], orders along the contraction): left (4 planes) P+ and
+// P- of order r + 1 over r (r + 1) (r + 2) times kSB, high and low parts (the
+// B16 halves, transposed); right (16 planes) weight q times P+ (q = 0, 3, 6,
+// 7) or P- (q = 1, 2, 4, 5), from P = high + low (relative error about 2^-22),
 // high and low parts.  One thread per (order, group of 8 nodes).
 template <class E>
-__global__ __launch_bounds__(64) void adj_tables(const double *Bp, const double *Bm, const double *wts, int H,
-                                                 int Hp, int Hq, int Kq, int n1max, unsigned short *L16,
-                                                 unsigned short *R16) {
+__global__ __launch_bounds__(64) void adj_tables(const unsigned short *B16, int Kb, int Hb, const double *wts, int H,
+                                                 int Hq, int Kq, int n1max, unsigned short *L16, unsigned short *R16) {
      const int r = blockIdx.x * 64 + threadIdx.x, jg = blockIdx.y;
      if (r >= Kq) return;
-     const size_t plane = (size_t) (Hq / 8) * Kq * 8, o = ((size_t) jg * Kq + r) * 8;
+     const size_t plane = (size_t) (Hq / 8) * Kq * 8, o = ((size_t) jg * Kq + r) * 8, bpl = (size_t) Kb * Hb * 8;
      unsigned short lv[4][8], rv[16][8];
-     const double sc = kSB / ((r + 1.) * (r + 2.));
      for (int e = 0; e < 8; ++e) {
           const int j = 8 * jg + e;
           double pp = 0., pm = 0.;
           if (j < H && r < n1max) {
-               pp = Bp[(size_t) r * Hp + j] * sc;
-               pm = Bm[(size_t) r * Hp + j] * sc;
+               const size_t bo = ((size_t) (r / 8) * Hb + j) * 8 + (r % 8);
+#pragma unroll
+               for (int q = 0; q < 4; ++q) lv[q][e] = B16[q * bpl + bo];
+               pp = E::value(lv[0][e]) + E::value(lv[1][e]);
+               pm = E::value(lv[2][e]) + E::value(lv[3][e]);
+          } else {
+#pragma unroll
+               for (int q = 0; q < 4; ++q) lv[q][e] = 0;
           }
-          split16<E>(pp, &lv[0][e], &lv[1][e]);
-          split16<E>(pm, &lv[2][e], &lv[3][e]);
           for (int q = 0; q < 8; ++q) {
                const bool plus = q == 0 || q == 3 || q == 6 || q == 7;
                // the planes of the sigma-sigma products (q = 1, 3, 6, 7) carry
@@ -2187,10 +2197,11 @@ __global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
 // adj_ugemm.  WA = 2: four waves, each operand fragment read for twice the
 // products; WA = 1: eight waves (two per SIMD), which hide each other's LDS
 // and barrier waits.
-template <class E, int WA, int SCH>
-__global__ __launch_bounds__(512 / WA) void adj_ugemm32(UArgs p) {
-     constexpr int TS = 128, TO = 64, G = 2;                 // sizes, orders, groups of 8 per stage
-     constexpr int NT = 512 / WA, WS = 4 / WA;               // threads, size-block waves
+template <class E, int WA, int SCH, int TS>
+__global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
+void adj_ugemm32(UArgs p) {
+     constexpr int TO = 64, G = 2;                           // orders, groups of 8 per stage
+     constexpr int NT = TS * 4 / WA, WS = TS / 32 / WA;      // threads, size-block waves
      constexpr int NA = 8 * G * TS / NT, NG = 12 * G * TO / NT;
      static_assert(NA * NT == 8 * G * TS && NG * NT == 12 * G * TO, "staging split");
      __shared__ v4u sA[2][8 * G][TS];
@@ -2321,14 +2332,16 @@ __global__ __launch_bounds__(512 / WA) void adj_ugemm32(UArgs p) {
           }
 }
 
-// Waves per workgroup of the reverse contraction (environment
-// LMIE_UGEMM_WAVES: 4 or 8, default 8) and its issue schedule
+// Shape of the reverse contraction (environment LMIE_UGEMM_WAVES): 8
+// (default) eight waves on 128 sizes, 4 four waves on 128 sizes, 2 four waves
+// on 64 sizes with two workgroups per CU; and its issue schedule
 // (LMIE_UGEMM_SCHED: 0 compiler order, 1 interleaved, default 0).
 int ugemm_waves() {
      static int v = 0;
      if (!v) {
           const char *e = getenv("LMIE_UGEMM_WAVES");
-          v = (e && atoi(e) == 4) ? 4 : 8;
+          const int r = e ? atoi(e) : 8;
+          v = (r == 4 || r == 2) ? r : 8;
      }
      return v;
 }
@@ -2742,7 +2755,7 @@ struct Workspace {
      int n2rows = 0, ng = 0;
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
-     int mixed = 0, bf16 = 0, Kb = 0;
+     int mixed = 0, bf16 = 0, Kb = 0, Hb = 0;
      std::vector<int> kt64, kt128;  // contraction depth per size tile of 64, 128
      Buf ktile128;
      double mr = 0., mi = 0.;
@@ -3014,7 +3027,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      if (Q > 0) {
           hipStream_t s = ts;
           if (w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) ||
-              w->Bp.get(sizeof(double) * (size_t) Kp * Hp) || w->Bm.get(sizeof(double) * (size_t) Kp * Hp))
+              (!mixed && (w->Bp.get(sizeof(double) * (size_t) Kp * Hp) || w->Bm.get(sizeof(double) * (size_t) Kp * Hp))))
                return -1;
           {
                w->hguess.resize(H > 0 ? H : 1);
@@ -3047,7 +3060,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                // tables below serve the reverse pass, which runs after join2
                HIPCHECK(hipEventRecord(w->join, s));
           }
-          if (!mixed || J->keep) {
+          if (!mixed) {
                hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, s, H, Hp, n1max, Kp, w->mu.as<double>(),
                                   w->Bp.as<double>(), w->Bm.as<double>());
           }
@@ -3339,7 +3352,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      w->n2max = 0;
      for (int i = 0; i < N; ++i) w->n2max = J->n2[i] > w->n2max ? J->n2[i] : w->n2max;
      w->calc_gc = J->calc_gc; w->calc_lc = J->calc_lc; w->mr = J->mr; w->mi = J->mi;
-     w->mixed = mixed; w->bf16 = bf16; w->Kb = Kb; w->kt64 = kt64; w->kt128 = kt128;
+     w->mixed = mixed; w->bf16 = bf16; w->Kb = Kb; w->Hb = Hb; w->kt64 = kt64; w->kt128 = kt128;
      return 0;
 }
 
@@ -3355,11 +3368,12 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
          w->Gf.get(sizeof(float) * 4 * (size_t) Kq * Kq) || w->G16.get(sizeof(unsigned short) * 12 * (size_t) Kq * Kq) ||
          w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np))
           return -1;
-     // (size tile of 128, order tile of 64) tasks over the XCDs
+     // (size tile of 128 or 64, order tile of 64) tasks over the XCDs
+     const std::vector<int> &kts = ugemm_waves() == 2 ? w->kt64 : w->kt128;
      std::vector<int2> task;
-     xcd_tasks(w->kt128, [&](int t) { return (w->kt128[t] + 63) / 64; }, tile_group(), task);
-     if (w->ktile128.get(sizeof(int) * w->kt128.size())) return -1;
-     HIPCHECK(hipMemcpyAsync(w->ktile128.p, w->kt128.data(), sizeof(int) * w->kt128.size(), hipMemcpyHostToDevice, s));
+     xcd_tasks(kts, [&](int t) { return (kts[t] + 63) / 64; }, tile_group(), task);
+     if (w->ktile128.get(sizeof(int) * kts.size())) return -1;
+     HIPCHECK(hipMemcpyAsync(w->ktile128.p, kts.data(), sizeof(int) * kts.size(), hipMemcpyHostToDevice, s));
      if (task.empty()) {
           HIPCHECK(hipMemsetAsync(w->Uf.p, 0, sizeof(float) * 4 * (size_t) Kq * Np, s));
           return 0;
@@ -3375,9 +3389,10 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
      u.ktile = w->ktile128.as<int>(); u.task = w->tasks.as<int2>(); u.sc = sc; u.Uf = w->Uf.as<float>();
 #define LMIE_ADJ(E)                                                                                            \
      do {                                                                                                     \
-          hipLaunchKernelGGL(adj_tables<E>, dim3(Kq / 64, Hq / 8), dim3(64), 0, s, (const double *) w->Bp.as<double>(), \
-                             (const double *) w->Bm.as<double>(), (const double *) w->wts.as<double>(), H, Hp, Hq, Kq, \
-                             n1max, w->L16.as<unsigned short>(), w->R16.as<unsigned short>());                      \
+          hipLaunchKernelGGL(adj_tables<E>, dim3(Kq / 64, Hq / 8), dim3(64), 0, s,                              \
+                             (const unsigned short *) w->B16.as<unsigned short>(), w->Kb, w->Hb,                      \
+                             (const double *) w->wts.as<double>(), H, Hq, Kq, n1max, w->L16.as<unsigned short>(),    \
+                             w->R16.as<unsigned short>());                                                            \
           hipLaunchKernelGGL(adj_gbuild<E>, dim3(Kq / 64, Kq / 64), dim3(256), 0, s,                            \
                              (const unsigned short *) w->L16.as<unsigned short>(),                             \
                              (const unsigned short *) w->R16.as<unsigned short>(), Hq, Kq, w->Gf.as<float>(), sc); \
@@ -3385,11 +3400,13 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
           tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
           if (ugemm_waves() == 4)                                                                               \
-               hipLaunchKernelGGL((adj_ugemm32<E, 2, 0>), dim3((unsigned) task.size()), dim3(256), 0, s, u);    \
+               hipLaunchKernelGGL((adj_ugemm32<E, 2, 0, 128>), dim3((unsigned) task.size()), dim3(256), 0, s, u); \
+          else if (ugemm_waves() == 2)                                                                          \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 0, 64>), dim3((unsigned) task.size()), dim3(256), 0, s, u); \
           else if (ugemm_sched() == 1)                                                                          \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 1>), dim3((unsigned) task.size()), dim3(512), 0, s, u);    \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 1, 128>), dim3((unsigned) task.size()), dim3(512), 0, s, u); \
           else                                                                                                  \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 0>), dim3((unsigned) task.size()), dim3(512), 0, s, u);    \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 0, 128>), dim3((unsigned) task.size()), dim3(512), 0, s, u); \
           tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);
