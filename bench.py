@@ -61,8 +61,8 @@ def sum_n1(p):
 
 
 def cpu_baseline(threads):
-    """The reference build on a bounded sample: both modes, 64 x 64 = 4096 size
-    points each (1/8 of the grid, same radius range, same 3810 angles)."""
+    """The reference build on a bounded sample: both modes, 256 x 64 = 16384 size
+    points each (1/2 of the grid, same radius range, same 3810 angles)."""
     from lmie_amd import abi, workloads as W
     so = os.path.join(ROOT, "oracle", "_ref", "liblmie_ref.so")
     if not os.path.exists(so):
@@ -70,14 +70,14 @@ def cpu_baseline(threads):
     ref = abi.bind(C.CDLL(so))
     tot_units, t0 = 0, time.perf_counter()
     for k in range(2):
-        p = dict(W.cfg4_mode_params(k), n_int2=64)
+        p = dict(W.cfg4_mode_params(k), n_int2=256)
         abi.solve(ref, p, None, threads)
         tot_units += sum_n1(p)
     dt = time.perf_counter() - t0
     return {"value": tot_units / dt, "unit": "size-pt*n-term/s", "cores": threads,
             "kind": "reference",
-            "sample": "reference lmie_solution (pthreads) on both modes at 64x64 size points each "
-                      "(1/8 of the grid, same r range and 3810 angles): %.1f s" % dt}
+            "sample": "reference lmie_solution (pthreads) on both modes at 256x64 size points each "
+                      "(1/2 of the grid, same r range and 3810 angles): %.1f s" % dt}
 
 
 def main():

@@ -1455,47 +1455,40 @@ __global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
      const double unscale = 1. / (kSA * kSB);
      double *sR = (double *) &sA[0][0][0];   // [2][NW][8][TN]
      __syncthreads();                        // every wave is done with the stages
+     // one weight channel at a time (the Stokes elements are recomputed per
+     // channel), so few doubles are live beside the accumulators
 #pragma unroll
      for (int b = 0; b < WB; ++b) {
-          const int jl = wn * 16 * WB + 16 * b + l16, j = jt0 + jl;
-          double ew[NW][8];
+          const int jl = wn * 16 * WB + 16 * b + l16;
 #pragma unroll
-          for (int c = 0; c < NW; ++c)
+          for (int c = 0; c < NW; ++c) {
+               double ew[8];
 #pragma unroll
-               for (int k = 0; k < 8; ++k) ew[c][k] = 0.;
+               for (int k = 0; k < 8; ++k) ew[k] = 0.;
 #pragma unroll
-          for (int a = 0; a < 2; ++a)
+               for (int a = 0; a < 2; ++a)
 #pragma unroll
-               for (int v = 0; v < 4; ++v) {
-                    const v4f *qq = acc[a][b];
-                    const int i = it0 + wm * 32 + 16 * a + mfma16_row(lane, v);
-                    if (p.sst) {
-                         double *o = p.sst + ((size_t) i * p.Hb + j) * 8;
-#pragma unroll
-                         for (int k = 0; k < 8; ++k) o[k] = qq[k][v] * unscale;
-                    }
-                    double Pp[4], Pm[4];
-                    stokes(qq[0][v] * unscale, qq[1][v] * unscale, qq[2][v] * unscale, qq[3][v] * unscale, Pp);
-                    stokes(qq[4][v] * unscale, qq[5][v] * unscale, qq[6][v] * unscale, qq[7][v] * unscale, Pm);
-#pragma unroll
-                    for (int c = 0; c < NW; ++c) {
+                    for (int v = 0; v < 4; ++v) {
+                         const v4f *qq = acc[a][b];
+                         const int i = it0 + wm * 32 + 16 * a + mfma16_row(lane, v);
+                         double Pp[4], Pm[4];
+                         stokes(qq[0][v] * unscale, qq[1][v] * unscale, qq[2][v] * unscale, qq[3][v] * unscale, Pp);
+                         stokes(qq[4][v] * unscale, qq[5][v] * unscale, qq[6][v] * unscale, qq[7][v] * unscale, Pm);
                          const double wv = p.W[c][i];
 #pragma unroll
                          for (int k = 0; k < 4; ++k) {
-                              ew[c][k] += wv * Pp[k];
-                              ew[c][4 + k] += wv * Pm[k];
+                              ew[k] += wv * Pp[k];
+                              ew[4 + k] += wv * Pm[k];
                          }
                     }
-               }
-#pragma unroll
-          for (int c = 0; c < NW; ++c)
 #pragma unroll
                for (int k = 0; k < 8; ++k) {
-                    double t = ew[c][k];
+                    double t = ew[k];
                     t += __shfl_xor(t, 16);
                     t += __shfl_xor(t, 32);
                     if (lg == 0) sR[((wm * NW + c) * 8 + k) * TN + jl] = t;
                }
+          }
      }
      __syncthreads();
      double *out = p.part + (size_t) st * NW * 4 * p.Q;
