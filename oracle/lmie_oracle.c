@@ -10,7 +10,7 @@
  * product alike, plus a few oracle_* probes of intermediate quantities.
  *
  * Each function restates, in plain double / double complex C, the algorithm of
- * the reference file cited beside it.  Pins (tests/test_oracle_*.py):
+ * the reference file cited beside it.  Pins (tests/test_oracle.py):
  *   - every output against the reference's own printed example
  *     (examples/example.out) and against full-precision fixtures produced by
  *     the reference build (the .npz files under tests/golden);
