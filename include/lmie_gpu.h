@@ -37,7 +37,13 @@ typedef struct {
      /* weight-only channels: seeds that move only the number density */
      int n_wgt;
      const double *df_wgt;    /* [n_wgt][N]                                       */
-     int save_control;        /* 1: return per-size series and matrix elements    */
+     int save_control;        /* 1: return per-size series and matrix elements;
+                                 2: re-weight those of an earlier call instead of
+                                 solving (src/lmie_schedule.c:1087-1124): only
+                                 f and the weight-only channels are used, the Mie
+                                 tangent channels return zero sums            */
+     const double *saved1;    /* [4][N]          if save_control == 2            */
+     const double *saved2;    /* [4][N][n_qang]  if save_control == 2            */
      int precision;           /* enum lmie_precision                              */
      int device;
      int keep;                /* 1: keep the device state for lmie_gpu_adjoint    */

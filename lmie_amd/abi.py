@@ -154,6 +154,45 @@ def read_out(lib, p, inp, out, max_coef):
     return r
 
 
+def solve_resaved(lib, p1, p2, seeds2=None, n_threads=1):
+    """The save/re-weight pair of lmie_solution (src/lmie_schedule.c:1087-1124):
+    a save_control=1 call with p1 keeps the per-size arrays, then a
+    save_control=2 call with p2 (same size grid and angles, new distribution
+    parameters and weight-only seeds) re-weights them.  Returns the outputs of
+    the second call.  The saved blocks are single allocations of the library,
+    released with the C library's free (lmie_out_free leaves them, as the
+    reference's does)."""
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    in1 = make_in(lib, dict(p1, save_control=1), None)
+    out1 = LmieOut()
+    rc = lib.lmie_solution(C.byref(in1), C.byref(out1), 1, 0, int(n_threads), 0)
+    if rc != 0:
+        raise RuntimeError("lmie_solution (save_control 1) returned %d" % rc)
+    s1, s2 = out1.save1, out1.save2
+    lib.lmie_out_free(C.byref(out1), 0)
+    try:
+        in2 = make_in(lib, dict(p2, save_control=2), seeds2)
+        out2 = LmieOut()
+        out2.save1, out2.save2 = s1, s2
+        max_coef = lib.lmie_calc_max_coef(in2.lambda_, in2.dist_type, in2.a1, in2.a2,
+                                          in2.a3, in2.r1, in2.r2)
+        rc = lib.lmie_solution(C.byref(in2), C.byref(out2), 1, 0, int(n_threads), 0)
+        if rc != 0:
+            if in2.n_derivs > 0:
+                lib.lmie_in_free(C.byref(in2), 1)
+            raise RuntimeError("lmie_solution (save_control 2) returned %d" % rc)
+        try:
+            return read_out(lib, p2, in2, out2, max_coef)
+        finally:
+            lib.lmie_out_free(C.byref(out2), 1 if in2.n_derivs > 0 else 0)
+            if in2.n_derivs > 0:
+                lib.lmie_in_free(C.byref(in2), 1)
+    finally:
+        libc.free(C.cast(s1, C.c_void_p))
+        libc.free(C.cast(s2, C.c_void_p))
+
+
 def solve(lib, p, seeds=None, n_threads=1):
     """Run lmie_solution(alloc_out=1) and return the outputs as numpy arrays.
 

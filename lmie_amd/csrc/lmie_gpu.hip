@@ -2745,6 +2745,7 @@ struct Workspace {
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
      Buf chD, chPC, chPart, revScan;   // per-size recurrences of the value channel
      Buf ftab;                      // order_factors of the value channel
+     Buf sw, s1, s2;                // save_control 2: channel weights, saved per-size arrays
      int n2rows = 0, ng = 0;
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
@@ -3349,10 +3350,109 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      return 0;
 }
 
+// Re-weighting of the per-size series and matrix elements of an earlier call
+// (save_control = 2, src/lmie_schedule.c:1087-1124): per channel c with
+// weights W[c][i], pf[c][q][j] = sum_i W[c][i] s2[q][i][j] and
+// dots[c][q] = sum_i W[c][i] s1[q][i].  Eight lanes per output, each adding
+// every eighth size, then the eight partials in a fixed order.
+__global__ __launch_bounds__(256) void saved_sums(const double *W, const double *s1, const double *s2, int N, int Q,
+                                                  double *pf, double *dots) {
+     const int per = 4 * (Q + 1), c = blockIdx.y;
+     const int e = blockIdx.x * 32 + (threadIdx.x >> 3), l = threadIdx.x & 7;
+     const double *w = W + (size_t) c * N;
+     double s = 0.;
+     if (e < 4 * Q) {
+          const int q = e / Q, j = e % Q;
+          for (int i = l; i < N; i += 8) s += w[i] * s2[((size_t) q * N + i) * Q + j];
+     } else if (e < per) {
+          const int q = e - 4 * Q;
+          for (int i = l; i < N; i += 8) s += w[i] * s1[(size_t) q * N + i];
+     }
+     s += __shfl_xor(s, 1);
+     s += __shfl_xor(s, 2);
+     s += __shfl_xor(s, 4);
+     if (l || e >= per) return;
+     if (e < 4 * Q) pf[(size_t) c * 4 * Q + e] = s;
+     else dots[4 * c + e - 4 * Q] = s;
+}
+
+// save_control = 2: the raw sums and expansions from the saved per-size
+// arrays and new weights (FP64), on the main stream.  Channels as the
+// forward: value, Mie tangents (zero: the saved arrays carry none), weights.
+int saved_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
+     const int N = J->n_size, Q = J->n_qang, C = J->n_coef;
+     const int nt = J->n_tan, nw = J->n_wgt, nch = 1 + nt + nw;
+     const int H = Q / 2, Hp = rup(H > 0 ? H : 1, TILE);
+     if (!J->saved1 || (Q > 0 && !J->saved2)) return set_err("save_control = 2 needs the saved per-size arrays");
+     if (N < 1) return set_err("no size points");
+     hipStream_t s = w->stream;
+     Timer tm{w, g_timing};
+     if (w->sw.get(sizeof(double) * (size_t) nch * N) || w->s1.get(sizeof(double) * 4 * (size_t) N) ||
+         (Q > 0 && w->s2.get(sizeof(double) * 4 * (size_t) N * Q)) ||
+         w->pf.get(sizeof(double) * (size_t) nch * 4 * (Q > 0 ? Q : 1)) || w->dots.get(sizeof(double) * 4 * nch))
+          return -1;
+     double *W = w->sw.as<double>();
+     HIPCHECK(hipMemcpyAsync(W, J->f, sizeof(double) * N, hipMemcpyHostToDevice, s));
+     if (nt) HIPCHECK(hipMemsetAsync(W + (size_t) N, 0, sizeof(double) * (size_t) nt * N, s));
+     if (nw)
+          HIPCHECK(hipMemcpyAsync(W + (size_t) (1 + nt) * N, J->df_wgt, sizeof(double) * (size_t) nw * N,
+                                  hipMemcpyHostToDevice, s));
+     HIPCHECK(hipMemcpyAsync(w->s1.p, J->saved1, sizeof(double) * 4 * (size_t) N, hipMemcpyHostToDevice, s));
+     if (Q > 0)
+          HIPCHECK(hipMemcpyAsync(w->s2.p, J->saved2, sizeof(double) * 4 * (size_t) N * Q, hipMemcpyHostToDevice, s));
+     tm.mark(LMIE_T_TOTAL, 0);
+     if (Q > 0) {
+          // Gauss rule and expansion tables (as the forward)
+          if (w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) || w->guess.get(sizeof(double) * H))
+               return -1;
+          w->hguess.resize(H > 0 ? H : 1);
+          gauss_guesses(Q, w->hguess.data());
+          HIPCHECK(hipMemcpyAsync(w->guess.p, w->hguess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
+          hipLaunchKernelGGL(gauss_nodes, dim3((H + 63) / 64), dim3(64), 0, s, Q, (const double *) w->guess.as<double>(),
+                             w->mu.as<double>(), w->w.as<double>());
+          if (J->calc_gc || J->calc_lc) {
+               const int planes = J->calc_lc ? 5 : 4;
+               if (w->T.get(sizeof(double) * planes * (size_t) C * Hp)) return -1;
+               hipLaunchKernelGGL(gsf_table, dim3(Hp / 64, planes), dim3(64), 0, s, H, Hp, C,
+                                  (const double *) w->mu.as<double>(), w->T.as<double>());
+          }
+     }
+     const int per = 4 * (Q + 1);
+     hipLaunchKernelGGL(saved_sums, dim3((per + 31) / 32, nch), dim3(256), 0, s, (const double *) W,
+                        (const double *) w->s1.as<double>(), (const double *) w->s2.as<double>(), N, Q,
+                        w->pf.as<double>(), w->dots.as<double>());
+     HIPCHECK(hipGetLastError());
+     if (Q > 0) {
+          if (w->E.get(sizeof(double) * (size_t) nch * 12 * Hp) || w->coefs.get(sizeof(double) * (size_t) nch * 6 * C))
+               return -1;
+          for (int gsf = 1; gsf >= 0; --gsf) {
+               double *dst = gsf ? R->gc_raw : R->lc_raw;
+               if (!(gsf ? J->calc_gc : J->calc_lc) || !dst) continue;
+               hipLaunchKernelGGL(expand_rows, dim3(Hp / 64, nch), dim3(64), 0, s, (const double *) w->pf.as<double>(),
+                                  nch, Q, H, Hp, (const double *) w->w.as<double>(), gsf, w->E.as<double>());
+               hipLaunchKernelGGL(expand_coefs, dim3((C + 3) / 4, nch), dim3(256), 0, s, (const double *) w->T.as<double>(),
+                                  (const double *) w->E.as<double>(), H, Hp, C, gsf, w->coefs.as<double>());
+               HIPCHECK(hipGetLastError());
+               HIPCHECK(hipMemcpyAsync(dst, w->coefs.p, sizeof(double) * (size_t) nch * 6 * C, hipMemcpyDeviceToHost, s));
+          }
+     }
+     tm.mark(LMIE_T_TOTAL, 1);
+     std::vector<double> dots((size_t) 4 * nch);
+     HIPCHECK(hipMemcpyAsync(dots.data(), w->dots.p, sizeof(double) * 4 * nch, hipMemcpyDeviceToHost, s));
+     HIPCHECK(hipStreamSynchronize(s));
+     for (int q = 0; q < 4; ++q) R->sums[q] = dots[q];
+     for (int c = 1; c < nch; ++c)
+          for (int q = 0; q < 4; ++q) R->sums_ch[4 * (c - 1) + q] = dots[4 * c + q];
+     const int ids[] = {LMIE_T_TOTAL};
+     tm.collect(ids, 1);
+     w->valid = 0;
+     return 0;
+}
+
 // Mixed reverse contraction: weights, split tables, G, its split operand and
 // U = G A (Uf, [4][Kq][Np]).
 int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
-     const int Np = w->Np, Hp = w->Hp, H = w->H, Q = w->Q, n1max = w->n1max;
+     const int Np = w->Np, H = w->H, Q = w->Q, n1max = w->n1max;
      const int Kq = rup(n1max, 64), Hq = rup(H, KM);
      if (rup(n1max, KM) > w->Kb * 8) return set_err("internal: split operand depth");
      if (w->wts.get(sizeof(double) * 8 * (size_t) Hq) || w->scal.get(sizeof(AdjScal)) ||
@@ -3586,10 +3686,11 @@ int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
      if (job->precision != LMIE_PRECISION_FP64 && job->precision != LMIE_PRECISION_MIXED &&
          job->precision != LMIE_PRECISION_MIXED_BF16)
           return set_err("unknown precision %d", job->precision);
+     if (job->save_control == 2 && job->keep) return set_err("save_control = 2 keeps no reverse-pass state");
      Workspace *w = nullptr;
      if (ws_acquire(job->device, &w)) return -1;
      res->state = nullptr;
-     int rc = forward_impl(job, res, w);
+     int rc = job->save_control == 2 ? saved_impl(job, res, w) : forward_impl(job, res, w);
      if (rc) {
           (void) hipStreamSynchronize(w->side);
           (void) hipStreamSynchronize(w->side2);

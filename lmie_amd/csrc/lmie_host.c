@@ -614,7 +614,7 @@ static int core_run(const core_args *c, core_out *o) {
      lmie_gpu_result res;
      job_mem_alloc(&jm, c->N, c->nd1, c->nd2);
      fill_job(c, &job, jm.x, jm.f, jm.n1, jm.n2, jm.dx, jm.dm, jm.dft, jm.dfw, jm.sx, jm.cx);
-     int C = job.n_coef, Q = job.n_qang;
+     int C = job.n_coef, Q = job.n_qang, rc = -1;
      o->n_qang = Q; o->n_coef = C;
      memset(&res, 0, sizeof res);
      res.sums_ch = vec((size_t) 4 * (nch - 1));
@@ -626,7 +626,18 @@ static int core_run(const core_args *c, core_out *o) {
           res.save1 = (*o->save1)[0];
           res.save2 = (*o->save2)[0][0];
      }
-     int rc = lmie_gpu_forward(&job, &res);
+     if (c->save_control == 2) {
+          /* the per-size arrays of an earlier save_control = 1 call on the same
+             grid (src/lmie_schedule.c:1087-1124) */
+          if (!o->save1 || !o->save2 || !*o->save1 || !**o->save1 || !*o->save2 || !**o->save2 ||
+              (Q > 0 && !***o->save2)) {
+               fail("lmie_core_solution", "save_control = 2 needs save1 and save2 of an earlier call");
+               goto done;
+          }
+          job.saved1 = (*o->save1)[0];
+          job.saved2 = Q > 0 ? (*o->save2)[0][0] : NULL;
+     }
+     rc = lmie_gpu_forward(&job, &res);
      if (rc) {
           fail("lmie_gpu_forward", lmie_gpu_last_error());
           goto done;
@@ -682,7 +693,6 @@ int lmie_core_solution_ex(int n_qsiz, int n_derivs1, int n_derivs2, int save_con
                           double accuracy, const lmie_opts *opts) {
      (void) r1;
      int nd = n_derivs1 + n_derivs2;
-     if (save_control == 2) return fail("lmie_core_solution", "save_control = 2 is not supported");
      core_args c;
      memset(&c, 0, sizeof c);
      c.N = n_qsiz; c.nd = nd; c.nd1 = n_derivs1; c.nd2 = n_derivs2;
@@ -844,9 +854,7 @@ static int solve_grid(const run_flags *rf, grid_t *g, const double *lambda_l, co
      double dummy[4];
      core_out o = {cext, csca, cbak, g_, gc, lc, nd ? cext_l : dummy, nd ? csca_l : dummy,
                    nd ? cbak_l : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}, NULL};
-     int rc = 0;
-     if (rf->save_control == 2) rc = fail("lmie_solution", "save_control = 2 is not supported");
-     else rc = core_run(&c, &o);
+     int rc = core_run(&c, &o);
      free(idx1); free(idx2);
      if (rc < 0) return fail("lmie_core_solution", lmie_last_error());
      *n_coef = rc;

@@ -12,6 +12,8 @@ Fixtures written (numpy .npz, allow_pickle never needed):
   example.npz     examples/example_c.c scenario, 4 seeds, gc + pf at 181 angles
   small_*.npz     tiny grids with save_control=1 (per-size save1/save2), one per
                   size-distribution kind, with Mie and weight-only tangents
+  resaved.npz     save_control=2: the small log-normal grid saved with one
+                  (r_g, sigma_g), re-weighted with another, weight-only seeds
   cfg0.npz        1000 points, x_max 22.8, no derivatives
   cfg1.npz        4000 points, x_max 228.5, 4 seeds
   cfg2.npz        16384 points, x_max 913.9, 4 seeds -> expansion-loss gradient
@@ -114,6 +116,17 @@ def gen_small(lib):
         save(name, p, seeds, r)
 
 
+# save_control = 2: saved with RESAVED_P1, re-weighted with RESAVED_P2
+RESAVED_P1 = dict(SMALL["small_lognormal"][0], save_control=1)
+RESAVED_P2 = dict(RESAVED_P1, a1=0.45, a2=0.18)
+RESAVED_SEEDS = {"a1": [1, 0], "a2": [0, 1]}
+
+
+def gen_resaved(lib):
+    r = abi.solve_resaved(lib, RESAVED_P1, RESAVED_P2, RESAVED_SEEDS, 1)
+    save("resaved", {"p1": RESAVED_P1, "p2": RESAVED_P2}, RESAVED_SEEDS, r)
+
+
 def gen_cfg0(lib):
     r = abi.solve(lib, W.CFG0, None, NT)
     save("cfg0", W.CFG0, None, r)
@@ -181,7 +194,7 @@ def gen_cfg4(lib):
     save("cfg4", {"grid": W.CFG4_GRID, "modes": W.CFG4_MODES}, None, arrs)
 
 
-GENS = {"example": gen_example, "small": gen_small, "cfg0": gen_cfg0, "cfg1": gen_cfg1,
+GENS = {"example": gen_example, "small": gen_small, "resaved": gen_resaved, "cfg0": gen_cfg0, "cfg1": gen_cfg1,
         "cfg2": gen_cfg2, "cfg3": gen_cfg3, "cfg4": gen_cfg4}
 
 

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import compare_all, load_golden, relnorm
-from lmie_amd import lmie, workloads as W
+from lmie_amd import abi, lmie, workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -98,6 +98,19 @@ def test_save_control_per_size_arrays(product):
     r = lmie.solve(product, p, seeds, "fp64")
     assert relnorm(r["save1"], z["save1"]) <= 1e-12
     assert relnorm(r["save2"], z["save2"]) <= 1e-12
+
+
+def test_resaved_per_size_arrays_match_reference(product, oracle):
+    say("case: save_control 2 re-weighting")
+    z = np.load(load_golden.__globals__["GOLDEN"] + "/resaved.npz")
+    meta = __import__("json").loads(str(z["inputs_json"]))
+    p1, p2, seeds = meta["params"]["p1"], meta["params"]["p2"], meta["seeds"]
+    r = abi.solve_resaved(product, p1, p2, seeds, n_threads=1)
+    compare_all(r, z, 1e-12)
+    # and equal to solving anew with the second distribution
+    f = abi.solve(product, dict(p2, save_control=0), seeds, n_threads=1)
+    for k in ("cext", "g", "gc", "gc_l", "lc", "pf"):
+        assert relnorm(r[k], f[k]) <= 1e-12, k
 
 
 def test_repeat_calls_are_deterministic(product):

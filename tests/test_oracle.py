@@ -39,6 +39,30 @@ def test_oracle_matches_reference_cfg2(oracle):
     compare_all(r, z, 1e-12)
 
 
+def _resaved_inputs():
+    z = np.load(os.path.join(GOLDEN, "resaved.npz"))
+    meta = __import__("json").loads(str(z["inputs_json"]))
+    return z, meta["params"]["p1"], meta["params"]["p2"], meta["seeds"]
+
+
+def test_oracle_resaved_matches_reference_fixture(oracle):
+    """save_control = 2 (src/lmie_schedule.c:1087-1124): the per-size arrays
+    saved with one distribution, re-weighted with another."""
+    z, p1, p2, seeds = _resaved_inputs()
+    r = abi.solve_resaved(oracle, p1, p2, seeds, n_threads=1)
+    compare_all(r, z, 1e-12)
+
+
+def test_oracle_resaved_equals_a_fresh_solution(oracle):
+    """The per-size Mie data do not depend on the distribution parameters on a
+    fixed grid, so re-weighting equals solving anew with them."""
+    z, p1, p2, seeds = _resaved_inputs()
+    r = abi.solve_resaved(oracle, p1, p2, seeds, n_threads=1)
+    f = abi.solve(oracle, dict(p2, save_control=0), seeds, n_threads=1)
+    for k in ("cext", "csca", "cbak", "g", "gc", "gc_l", "lc", "pf", "cext_l", "g_l"):
+        assert relnorm(r[k], f[k]) <= 1e-13, k
+
+
 def _parse_example_out(path):
     """Blocks of examples/example.out: scalars, gc rows, pf rows per section."""
     blocks, cur, mode = [], None, None
