@@ -1,26 +1,31 @@
 // lmie_gpu.hip -- gfx950 kernels of the size-integrated Mie path and the thin C
 // ABI of include/lmie_gpu.h.
 //
-// Pipeline of one forward call (one HIP stream per workspace):
-//   gauss_nodes   Gauss-Legendre angles on [-1, 1]             (misc/rtutil_math.c:231-316)
-//   pitau_table   pi_n +- tau_n at the positive angles         (src/lmie_core.c:669-691)
-//   gsf_table     P^{00}, P^{02}, P^{22}, P^{2-2} at the same  (misc/rtutil_math.c:867-1017)
-//   mie_coefs     per size point: D_n, psi_n, chi_n, a_n, b_n, series, tangents
-//                                                              (src/lmie_core.c:263-652)
-//   series_sums   f-weighted sums over sizes of the series      (src/lmie_core.c:537-611)
-//   angular_f64   S1/S2 contraction on FP64 matrix cores, Stokes epilogue and
-//                 weighted reduction over sizes                (src/lmie_core.c:658-791)
-//   angular_mix   the same contraction on BF16 matrix cores with split operands
-//   reduce_parts  deterministic sum of the per-block partials  (src/lmie_schedule.c:109-147)
-//   expand_*      generalized-spherical / Legendre expansion   (misc/rtutil_scat_int.c:20-480)
+// Forward call (main stream; angle tables on two side streams):
+//   gauss_nodes      Gauss-Legendre angles on [-1, 1]          (misc/rtutil_math.c:231-316)
+//   pitau_table16    split 16-bit pi_n +- tau_n at the positive angles (mixed path)
+//   pitau_table      FP64 pi_n +- tau_n (FP64 path)            (src/lmie_core.c:669-691)
+//   gsf_table        P^{00}, P^{02}, P^{22}, P^{2-2}, P_l       (misc/rtutil_math.c:867-1017)
+//   mie_chains       per size: D_n downward; ratio, psi_n, chi_n (src/lmie_core.c:298-405)
+//   mie_terms        per (size, 8 orders): a_n, b_n, contraction operand, series partials
+//                                                              (src/lmie_core.c:453-652)
+//   mie_sums         series per size                          (src/lmie_core.c:537-596)
+//   mie_coefs        per size with forward tangents (tangent passes)
+//   angular_mix2     S sums on the 16-bit matrix cores (split operands), Stokes in
+//                    FP64, weighted sums per angle            (src/lmie_core.c:658-791)
+//   angular_f64      the same on the FP64 matrix cores
+//   reduce_parts     fixed-order sum of the per-tile partials (src/lmie_schedule.c:109-147)
+//   expand_*         generalized-spherical / Legendre expansion (misc/rtutil_scat_int.c:20-480)
+//   saved_sums       save_control 2: re-weighting of saved per-size arrays
+//                                                              (src/lmie_schedule.c:1087-1124)
+// Reverse call: expand_adjoint_*, adj_weights, adj_tables, adj_gbuild,
+// adj_gsplit, adj_ugemm32 (the kernel matrix G and U = G A), rev_orders and
+// rev_carry (per-size reverse sweep); angular_adj_f64 on the FP64 path.
 //
-// HBM layouts (N_p = N rounded up to 64, K_p = n1_max rounded up to 16):
-//   A planes   [channel][4][K_p][N_p]  fv4_n (a_n + b_n) re, im; fv4_n (a_n - b_n) re, im,
-//              row n-1 holds order n; columns are size points, so the 64 lanes
-//              of a wave (64 consecutive sizes) store 512 contiguous bytes.
-//   B tables   [2][K_p][H_p]           pi_n + tau_n, pi_n - tau_n at the H = n_qang/2
-//              positive Gauss angles.
-//   GSF tables [4][n_coef][H_p].
+// HBM layouts: orders are rows and sizes (padded to N_p) columns, so the 64
+// lanes of a wave (64 consecutive sizes) move 512 contiguous bytes per row;
+// split 16-bit operands are blocked by 8 along the contraction index
+// ([plane][K / 8][column][8]), one 16-byte fragment per lane.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -1121,186 +1126,6 @@ __global__ __launch_bounds__(256) void angular_f64(AngArgs p) {
 }
 
 
-// Angular contraction, split 16-bit operands on the matrix cores.
-// The same four complex sums as angular_f64, each operand split into a high and
-// a low 16-bit part: x y ~ xh yh + xh yl + xl yh (three products, fp32
-// accumulation in the matrix core), v_mfma_f32_16x16x32_{f16,bf16}: lane l
-// supplies rows l % 16 and orders 8 (l / 16) .. 8 (l / 16) + 7 of the chunk for
-// both operands.  The mirror sign s_n = (-1)^(n-1) flips the odd elements of
-// each 8-order fragment.  Epilogue in double as angular_f64.
-struct MixArgs {
-     int Np, Kb, Hp, H, Q, N;
-     const unsigned short *A16;     // [8][Kb][Np][8]
-     const unsigned short *B16;     // [4][Kb][Hp][8]
-     const int *ktile;              // orders to contract per size tile (multiple of KM)
-     const double *W[4];            // weight channels [Np]
-     int tiles_per_block;
-     double *part;                  // [gridDim.y][nw][4][Q]
-     double *sst;                   // reverse-pass checkpoint [Np][Hp][8] or NULL
-};
-
-template <class E, int WM, int WN, int NW>
-__global__ __launch_bounds__(256) void angular_mix(MixArgs p) {
-     constexpr int TM = 32 * WM, TN = 32 * WN, G = KM / 8;
-     __shared__ v4u sA[8 * G * TM];
-     __shared__ v4u sB[4 * G * TN];
-     __shared__ double sR[NW * 8 * 32 * WN];
-
-     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-     const int wm = wave & 1, wn = wave >> 1;
-     const int l16 = lane & 15, lg = lane >> 4;
-     const int jt0 = blockIdx.x * TN;
-     const int tile0 = blockIdx.y * p.tiles_per_block;
-     const size_t Np = p.Np, Hp = p.Hp;
-     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hp * 8;
-     const double unscale = 1. / (kSA * kSB);
-     const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
-
-     double ew[NW][WN][8];
-#pragma unroll
-     for (int c = 0; c < NW; ++c)
-#pragma unroll
-          for (int b = 0; b < WN; ++b)
-#pragma unroll
-               for (int q = 0; q < 8; ++q) ew[c][b][q] = 0.;
-
-     for (int tt = 0; tt < p.tiles_per_block; ++tt) {
-          const int tile = tile0 + tt;
-          const int it0 = tile * TM;
-          if (it0 >= p.Np) break;
-          const int kmax = p.ktile[tile];
-          v4f acc[WM][WN][8];
-#pragma unroll
-          for (int a = 0; a < WM; ++a)
-#pragma unroll
-               for (int b = 0; b < WN; ++b)
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) acc[a][b][q] = v4f{0.f, 0.f, 0.f, 0.f};
-
-          for (int k0 = 0; k0 < kmax; k0 += KM) {
-               const int g0 = k0 / 8;
-               __syncthreads();
-               for (int e = threadIdx.x; e < 8 * G * TM; e += 256) {
-                    const int col = e % TM, rest = e / TM, g = rest % G, pl = rest / G;
-                    sA[e] = *(const v4u *) (p.A16 + pl * pA + ((size_t) (g0 + g) * Np + it0 + col) * 8);
-               }
-               for (int e = threadIdx.x; e < 4 * G * TN; e += 256) {
-                    const int col = e % TN, rest = e / TN, g = rest % G, pl = rest / G;
-                    sB[e] = *(const v4u *) (p.B16 + pl * pB + ((size_t) (g0 + g) * Hp + jt0 + col) * 8);
-               }
-               __syncthreads();
-               v4u bph[WN], bpl[WN], bmh[WN], bml[WN];
-#pragma unroll
-               for (int b = 0; b < WN; ++b) {
-                    const int jc = wn * 16 * WN + 16 * b + l16;
-                    bph[b] = sB[(0 * G + lg) * TN + jc];
-                    bpl[b] = sB[(1 * G + lg) * TN + jc];
-                    bmh[b] = sB[(2 * G + lg) * TN + jc];
-                    bml[b] = sB[(3 * G + lg) * TN + jc];
-               }
-#pragma unroll
-               for (int a = 0; a < WM; ++a) {
-                    const int ic = wm * 16 * WM + 16 * a + l16;
-                    v4u x[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) x[q] = sA[(q * G + lg) * TM + ic];
-#pragma unroll
-                    for (int b = 0; b < WN; ++b) {
-                         v4f *c = acc[a][b];
-                         // accumulators: sp+ re, im = S (pi+tau); sm+ = T (pi-tau);
-                         // sp- = s S (pi-tau); sm- = s T (pi+tau)
-                         const v4u y[8] = {x[0] ^ flip, x[1] ^ flip, x[2] ^ flip, x[3] ^ flip,
-                                           x[4] ^ flip, x[5] ^ flip, x[6] ^ flip, x[7] ^ flip};
-                         // hi x hi
-                         c[0] = E::mfma(x[0], bph[b], c[0]); c[1] = E::mfma(x[2], bph[b], c[1]);
-                         c[2] = E::mfma(x[4], bmh[b], c[2]); c[3] = E::mfma(x[6], bmh[b], c[3]);
-                         c[4] = E::mfma(y[0], bmh[b], c[4]); c[5] = E::mfma(y[2], bmh[b], c[5]);
-                         c[6] = E::mfma(y[4], bph[b], c[6]); c[7] = E::mfma(y[6], bph[b], c[7]);
-                         // hi x lo
-                         c[0] = E::mfma(x[0], bpl[b], c[0]); c[1] = E::mfma(x[2], bpl[b], c[1]);
-                         c[2] = E::mfma(x[4], bml[b], c[2]); c[3] = E::mfma(x[6], bml[b], c[3]);
-                         c[4] = E::mfma(y[0], bml[b], c[4]); c[5] = E::mfma(y[2], bml[b], c[5]);
-                         c[6] = E::mfma(y[4], bpl[b], c[6]); c[7] = E::mfma(y[6], bpl[b], c[7]);
-                         // lo x hi
-                         c[0] = E::mfma(x[1], bph[b], c[0]); c[1] = E::mfma(x[3], bph[b], c[1]);
-                         c[2] = E::mfma(x[5], bmh[b], c[2]); c[3] = E::mfma(x[7], bmh[b], c[3]);
-                         c[4] = E::mfma(y[1], bmh[b], c[4]); c[5] = E::mfma(y[3], bmh[b], c[5]);
-                         c[6] = E::mfma(y[5], bph[b], c[6]); c[7] = E::mfma(y[7], bph[b], c[7]);
-                    }
-               }
-          }
-#pragma unroll
-          for (int a = 0; a < WM; ++a)
-#pragma unroll
-               for (int v = 0; v < 4; ++v) {
-                    const int i = it0 + wm * 16 * WM + 16 * a + mfma16_row(lane, v);
-                    double wv[NW];
-#pragma unroll
-                    for (int c = 0; c < NW; ++c) wv[c] = p.W[c][i];
-#pragma unroll
-                    for (int b = 0; b < WN; ++b) {
-                         const v4f *q = acc[a][b];
-                         if (p.sst) {
-                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
-                              double *o = p.sst + ((size_t) i * p.Hp + j) * 8;
-#pragma unroll
-                              for (int c = 0; c < 8; ++c) o[c] = q[c][v] * unscale;
-                         }
-                         double Pp[4], Pm[4];
-                         stokes(q[0][v] * unscale, q[1][v] * unscale, q[2][v] * unscale, q[3][v] * unscale, Pp);
-                         stokes(q[4][v] * unscale, q[5][v] * unscale, q[6][v] * unscale, q[7][v] * unscale, Pm);
-#pragma unroll
-                         for (int c = 0; c < NW; ++c)
-#pragma unroll
-                              for (int k = 0; k < 4; ++k) {
-                                   ew[c][b][k] += wv[c] * Pp[k];
-                                   ew[c][b][4 + k] += wv[c] * Pm[k];
-                              }
-                    }
-               }
-     }
-#pragma unroll
-     for (int c = 0; c < NW; ++c)
-#pragma unroll
-          for (int b = 0; b < WN; ++b)
-#pragma unroll
-               for (int q = 0; q < 8; ++q) {
-                    double t = ew[c][b][q];
-                    t += __shfl_xor(t, 16);
-                    t += __shfl_xor(t, 32);
-                    ew[c][b][q] = t;
-               }
-     __syncthreads();
-     if (wm == 1 && lg == 0) {
-#pragma unroll
-          for (int c = 0; c < NW; ++c)
-#pragma unroll
-               for (int b = 0; b < WN; ++b)
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) sR[((c * WN + b) * 8 + q) * 32 + wn * 16 + l16] = ew[c][b][q];
-     }
-     __syncthreads();
-     if (wm == 0 && lg == 0) {
-          double *out = p.part + (size_t) blockIdx.y * NW * 4 * p.Q;
-#pragma unroll
-          for (int b = 0; b < WN; ++b) {
-               const int j = jt0 + wn * 16 * WN + 16 * b + l16;
-               if (j < p.H) {
-#pragma unroll
-                    for (int c = 0; c < NW; ++c)
-#pragma unroll
-                         for (int k = 0; k < 4; ++k) {
-                              out[((size_t) c * 4 + k) * p.Q + p.H + j] =
-                                   ew[c][b][k] + sR[((c * WN + b) * 8 + k) * 32 + wn * 16 + l16];
-                              out[((size_t) c * 4 + k) * p.Q + p.H - 1 - j] =
-                                   ew[c][b][4 + k] + sR[((c * WN + b) * 8 + 4 + k) * 32 + wn * 16 + l16];
-                         }
-               }
-          }
-     }
-}
-
-
 // Staging of one LDS stage of a blocked split operand: plane-major rows of
 // `cols` 16-byte columns, [plane][group of 8][column], `rows` = planes x G rows
 // per stage.  Thread tid of NT copies elements tid + NT t; with R = NT / cols
@@ -1324,13 +1149,18 @@ struct Stager {
      }
 };
 
-// The split 16-bit contraction, one workgroup per (size tile of 64, angle tile
-// of 32 WN) pair, two LDS stages: the next 32 orders of both operands are read
-// from HBM into registers while the matrix cores work on the current stage,
-// then written to the other stage (one barrier per stage).  Four waves in a
-// 2 x 2 grid, each 32 sizes x 16 WN angles, 8 accumulators per element as
-// angular_mix.  The (size tile, angle tile) of each workgroup comes from a
-// host-built list dealt over the XCDs (xcd_tasks).  Partials per size tile:
+// The split 16-bit contraction on the 16x16x32 instruction: the four complex
+// sums of the FP64 contraction, each operand split into a high and a low
+// 16-bit part, x y ~ xh yh + xh yl + xl yh (three products, fp32 accumulation).
+// One workgroup per (size tile of 64, angle tile of 16 WB NWV) task, two LDS
+// stages: the next 32 orders of both operands are read from HBM into
+// registers while the matrix cores work on the current stage, then written to
+// the other stage (one barrier per stage).  Waves in a 2 x NWV grid, each 32
+// sizes x 16 WB angles, 8 accumulators per element (the sums at +mu and -mu;
+// the -mu products flip the sign of the odd orders, s_n = (-1)^(n-1)).
+// NWV = 4, WB = 2: eight waves of 32 x 32, two per SIMD; NWV = 2, WB = 4: four
+// waves of 32 x 64.  The (size tile, angle tile) of each workgroup comes from
+// a host-built list dealt over the XCDs (xcd_tasks).  Partials per size tile:
 // part[tile][NW][4][Q].
 struct Mix2Args {
      int Np, Kb, Hb, H, Q;
@@ -2071,10 +1901,8 @@ __global__ __launch_bounds__(64) void adj_gsplit(const float *Gf, const AdjScal 
      }
 }
 
-// U = G A per size: one workgroup per (size tile of 64, order tile of 64)
-// task, contraction over m < the tile's depth in stages of 32, two LDS stages.
-// Matrix-core rows are orders n, columns sizes.  Accumulators: U_S re, im,
-// U_T re, im; output Uf[4][Kq][Np] in fp32, unscaled to U = Sbar / f.
+// Operands of U = G A (adj_ugemm32).  Accumulators: U_S re, im, U_T re, im;
+// output Uf[4][Kq][Np] in fp32, unscaled to U = Sbar / f.
 struct UArgs {
      int Np, Kb, Kq;
      const unsigned short *A16;     // [8][Kb][Np][8]
@@ -2085,112 +1913,15 @@ struct UArgs {
      float *Uf;                     // [4][Kq][Np]
 };
 
-template <class E>
-__global__ __launch_bounds__(256) void adj_ugemm(UArgs p) {
-     constexpr int T = 64, G = KM / 8;
-     constexpr int NA = 8 * G * T / 256, NG = 12 * G * T / 256;
-     __shared__ v4u sA[2][8 * G][T];
-     __shared__ v4u sG[2][12 * G][T];
-     const int2 tk = p.task[blockIdx.x];
-     if (tk.x < 0) return;
-     const int it0 = tk.x * T, n0 = tk.y * T, nch = p.ktile[tk.x] / KM;
-     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-     const int wm = wave & 1, wn = wave >> 1, l16 = lane & 15, lg = lane >> 4;
-     const size_t Np = p.Np, Kq = p.Kq;
-     const size_t pA = (size_t) p.Kb * Np * 8, pG = (Kq / 8) * Kq * 8;
-     const v4u neg = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-     v4u ra[NA], rg[NG];
-     const Stager<256, T, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
-     const Stager<256, T, G> stG(p.G16, pG, Kq * 8, n0, threadIdx.x);
-     auto load = [&](int c) {
-#pragma unroll
-          for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
-#pragma unroll
-          for (int t = 0; t < NG; ++t) rg[t] = *stG.at(t, c);
-     };
-     auto store = [&](int s) {
-#pragma unroll
-          for (int t = 0; t < NA; ++t) {
-               const int e = threadIdx.x + 256 * t;
-               sA[s][e / T][e % T] = ra[t];
-          }
-#pragma unroll
-          for (int t = 0; t < NG; ++t) {
-               const int e = threadIdx.x + 256 * t;
-               sG[s][e / T][e % T] = rg[t];
-          }
-     };
-     v4f acc[2][2][4];
-#pragma unroll
-     for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b)
-#pragma unroll
-               for (int k = 0; k < 4; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
-     if (nch > 0) {
-          load(0);
-          store(0);
-          if (nch > 1) load(1);
-     }
-     for (int c = 0; c < nch; ++c) {
-          const int s = c & 1;
-          __syncthreads();
-#pragma unroll
-          for (int b = 0; b < 2; ++b) {
-               const int nc = wn * 32 + 16 * b + l16;
-               v4u g[12];
-#pragma unroll
-               for (int q = 0; q < 12; ++q) g[q] = sG[s][q * G + lg][nc];
-#pragma unroll
-               for (int a = 0; a < 2; ++a) {
-                    const int ic = wm * 32 + 16 * a + l16;
-                    v4u x[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) x[q] = sA[s][q * G + lg][ic];
-                    const v4u nx0 = x[0] ^ neg, nx1 = x[1] ^ neg, nx4 = x[4] ^ neg, nx5 = x[5] ^ neg;
-                    v4f *cc = acc[a][b];
-#define UG(k, XH, XL, GP) cc[k] = E::mfma(g[2 * (GP)], XH, cc[k]); cc[k] = E::mfma(g[2 * (GP) + 1], XH, cc[k]); \
-                    cc[k] = E::mfma(g[2 * (GP)], XL, cc[k]);
-                    // U_S re = Gss A+r - Re Gst A-r + Im Gst A-i
-                    UG(0, x[0], x[1], 0) UG(0, x[4], x[5], 1) UG(0, x[6], x[7], 2)
-                    // U_S im = Gss A+i - Re Gst A-i - Im Gst A-r
-                    UG(1, x[2], x[3], 0) UG(1, x[6], x[7], 1) UG(1, nx4, nx5, 2)
-                    // U_T re = Gtt A-r - Re Gts A+r + Im Gts A+i
-                    UG(2, x[4], x[5], 3) UG(2, x[0], x[1], 4) UG(2, x[2], x[3], 5)
-                    // U_T im = Gtt A-i - Re Gts A+i - Im Gts A+r
-                    UG(3, x[6], x[7], 3) UG(3, x[2], x[3], 4) UG(3, nx0, nx1, 5)
-#undef UG
-               }
-               if (b == 0) {
-                    if (c + 1 < nch) store(s ^ 1);
-                    if (c + 2 < nch) load(c + 2);
-               }
-          }
-     }
-     // U = acc n (n+1) wmax / (2^(14 - k) kSB^2 kSA)
-     const double base = p.sc->wmax * ldexp(1., adj_gexp(p.sc) - 14) / (kSB * kSB * kSA);
-#pragma unroll
-     for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < 2; ++b)
-#pragma unroll
-               for (int v = 0; v < 4; ++v) {
-                    const int r = n0 + wn * 32 + 16 * b + mfma16_row(lane, v);
-                    const int i = it0 + wm * 32 + 16 * a + l16;
-                    const double f = base * (r + 1.) * (r + 2.);
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) p.Uf[((size_t) k * Kq + r) * Np + i] = (float) (acc[a][b][k][v] * f);
-               }
-}
-
 // U = G A with the 32x32x16 matrix instruction: one workgroup per (size tile
-// of 128, order tile of 64) task, stages of 16 orders in two LDS buffers
-// (112 KiB).  Waves in a grid of (4 / WA) size blocks x 2 order blocks, each
-// wave 32 orders (matrix rows) x 32 WA sizes, 4 accumulators per element as
-// adj_ugemm.  WA = 2: four waves, each operand fragment read for twice the
-// products; WA = 1: eight waves (two per SIMD), which hide each other's LDS
-// and barrier waits.
-template <class E, int WA, int SCH, int TS>
+// of TS, order tile of 64) task, stages of 16 orders in two LDS buffers
+// (112 KiB for TS = 128, 80 KiB for TS = 64).  Waves in a grid of TS / 32 / WA
+// size blocks x 2 order blocks, each wave 32 orders (matrix rows) x 32 WA
+// sizes, 4 accumulators per element (U_S, U_T re, im).  TS = 128, WA = 1:
+// eight waves, two per SIMD, which hide each other's LDS and barrier waits;
+// WA = 2: four waves, each operand fragment read for twice the products;
+// TS = 64: four waves, two workgroups per CU.
+template <class E, int WA, int TS>
 __global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
 void adj_ugemm32(UArgs p) {
      constexpr int TO = 64, G = 2;                           // orders, groups of 8 per stage
@@ -2286,30 +2017,6 @@ void adj_ugemm32(UArgs p) {
                     load(min(c + 2, nch - 1));
                }
           }
-          if constexpr (SCH == 1) {
-               // issue order of the stage: the first fragments, then one
-               // matrix product between consecutive LDS reads, LDS writes and
-               // HBM loads, so no run of memory instructions stalls the
-               // in-order issue of the products
-               constexpr int NR = 8 * WA + 12, NM = 36 * WA;
-               __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
-#pragma unroll
-               for (int q = 6; q < NR; ++q) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-               }
-#pragma unroll
-               for (int q = 0; q < NA + NG; ++q) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
-               }
-#pragma unroll
-               for (int q = 0; q < NA + NG; ++q) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
-               }
-               __builtin_amdgcn_sched_group_barrier(0x008, NM - (NR - 6) - 2 * (NA + NG), 0);
-          }
      }
      // U = acc n (n+1) wmax / (2^(14 - k) kSB^2 kSA); rows are orders
      const double base = p.sc->wmax * ldexp(1., adj_gexp(p.sc) - 14) / (kSB * kSB * kSA);
@@ -2325,27 +2032,13 @@ void adj_ugemm32(UArgs p) {
           }
 }
 
-// Shape of the reverse contraction (environment LMIE_UGEMM_WAVES): 8
-// (default) eight waves on 128 sizes, 4 four waves on 128 sizes, 2 four waves
-// on 64 sizes with two workgroups per CU; and its issue schedule
-// (LMIE_UGEMM_SCHED: 0 compiler order, 1 interleaved, default 0).
+// Shape of the reverse contraction (environment LMIE_UGEMM_WAVES, read at
+// every call): 8 (default) eight waves on 128 sizes, 4 four waves on 128
+// sizes, 2 four waves on 64 sizes with two workgroups per CU.
 int ugemm_waves() {
-     static int v = 0;
-     if (!v) {
-          const char *e = getenv("LMIE_UGEMM_WAVES");
-          const int r = e ? atoi(e) : 8;
-          v = (r == 4 || r == 2) ? r : 8;
-     }
-     return v;
-}
-
-int ugemm_sched() {
-     static int v = -1;
-     if (v < 0) {
-          const char *e = getenv("LMIE_UGEMM_SCHED");
-          v = (e && atoi(e) == 1) ? 1 : 0;
-     }
-     return v;
+     const char *e = getenv("LMIE_UGEMM_WAVES");
+     const int r = e ? atoi(e) : 8;
+     return (r == 4 || r == 2) ? r : 8;
 }
 
 // Per-size reverse sweep (adjoint of src/lmie_core.c:298-611) over the
@@ -2888,28 +2581,21 @@ void xcd_tasks(const std::vector<int> &kt, F ncol, int group, std::vector<int2> 
           for (size_t k = 0; k < per[x].size(); ++k) out[8 * k + x] = per[x][k];
 }
 
-// Kernel of the mixed contraction (environment LMIE_MIX_WAVES): 8 (default)
-// or 4 the 16x16x32 instruction with eight or four waves, 32 the 32x32x16
-// instruction with eight waves.
+// Kernel of the mixed contraction (environment LMIE_MIX_WAVES, read at every
+// call): 8 (default) or 4 the 16x16x32 instruction with eight or four waves,
+// 32 the 32x32x16 instruction with eight waves.
 int mix_waves() {
-     static int v = 0;
-     if (!v) {
-          const char *e = getenv("LMIE_MIX_WAVES");
-          const int r = e ? atoi(e) : 8;
-          v = (r == 4 || r == 32) ? r : 8;
-     }
-     return v;
+     const char *e = getenv("LMIE_MIX_WAVES");
+     const int r = e ? atoi(e) : 8;
+     return (r == 4 || r == 32) ? r : 8;
 }
 
-// Size tiles per group of xcd_tasks (environment LMIE_TILE_GROUP, default 4).
+// Size tiles per group of xcd_tasks (environment LMIE_TILE_GROUP, read at
+// every call, default 4).
 int tile_group() {
-     static int g = 0;
-     if (!g) {
-          const char *e = getenv("LMIE_TILE_GROUP");
-          g = e ? atoi(e) : 4;
-          if (g < 1 || g > 64) g = 4;
-     }
-     return g;
+     const char *e = getenv("LMIE_TILE_GROUP");
+     const int g = e ? atoi(e) : 4;
+     return (g < 1 || g > 64) ? 4 : g;
 }
 
 // Initial guesses of the Gauss-Legendre roots, cos(t + (n-1)/(8 n^3) / tan t)
@@ -3239,9 +2925,10 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                     if (nw == 0) LMIE_MIX(E, WB, NWV, 1); else if (nw == 1) LMIE_MIX(E, WB, NWV, 2);    \
                     else if (nw == 2) LMIE_MIX(E, WB, NWV, 3); else LMIE_MIX(E, WB, NWV, 4);            \
                } while (0)
-               if (mix_waves() == 4) {
+               const int mw = mix_waves();
+               if (mw == 4) {
                     if (bf16) LMIE_MIX_NW(BF16, 4, 2); else LMIE_MIX_NW(F16, 4, 2);
-               } else if (mix_waves() == 8) {
+               } else if (mw == 8) {
                     if (bf16) LMIE_MIX_NW(BF16, 2, 4); else LMIE_MIX_NW(F16, 2, 4);
                } else {
 #define LMIE_MIX32(E)                                                                                    \
@@ -3462,7 +3149,8 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
          w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np))
           return -1;
      // (size tile of 128 or 64, order tile of 64) tasks over the XCDs
-     const std::vector<int> &kts = ugemm_waves() == 2 ? w->kt64 : w->kt128;
+     const int shape = ugemm_waves();
+     const std::vector<int> &kts = shape == 2 ? w->kt64 : w->kt128;
      std::vector<int2> task;
      xcd_tasks(kts, [&](int t) { return (kts[t] + 63) / 64; }, tile_group(), task);
      if (w->ktile128.get(sizeof(int) * kts.size())) return -1;
@@ -3492,14 +3180,12 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
           hipLaunchKernelGGL(adj_gsplit<E>, dim3(Kq / 64, Kq / 8), dim3(64), 0, s, (const float *) w->Gf.as<float>(), \
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
           tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
-          if (ugemm_waves() == 4)                                                                               \
-               hipLaunchKernelGGL((adj_ugemm32<E, 2, 0, 128>), dim3((unsigned) task.size()), dim3(256), 0, s, u); \
-          else if (ugemm_waves() == 2)                                                                          \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 0, 64>), dim3((unsigned) task.size()), dim3(256), 0, s, u); \
-          else if (ugemm_sched() == 1)                                                                          \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 1, 128>), dim3((unsigned) task.size()), dim3(512), 0, s, u); \
+          if (shape == 4)                                                                                       \
+               hipLaunchKernelGGL((adj_ugemm32<E, 2, 128>), dim3((unsigned) task.size()), dim3(256), 0, s, u);  \
+          else if (shape == 2)                                                                                  \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 64>), dim3((unsigned) task.size()), dim3(256), 0, s, u);   \
           else                                                                                                  \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 0, 128>), dim3((unsigned) task.size()), dim3(512), 0, s, u); \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 128>), dim3((unsigned) task.size()), dim3(512), 0, s, u);  \
           tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);

@@ -6,11 +6,12 @@ Loss (workloads.expansion_loss_targets): L = sum_k w_k sum_l (c_kl - t_kl)^2,
 t = c_ref (1 + z), z ~ N(0,1), w_k = 1 / ||c_ref_k||^2.
 Per-component bar: |g - g_ref| <= tol * max(|g_ref|, 1e-3 ||g_ref||)."""
 import math
+import os
 
 import numpy as np
 import pytest
 
-from conftest import load_golden
+from conftest import load_golden, relnorm
 from lmie_amd import lmie, workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -142,3 +143,29 @@ def test_autograd_wrapper(product):
     out, g = lmie.adjoint(product, [p], None, lambda o: {"gc": 2.0 * o["gc"]}, "fp64")
     ref = np.array([g["mr"], g["mi"], g["a1"], g["a2"]])
     assert np.allclose(theta.grad.numpy(), ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("env", [("LMIE_MIX_WAVES", "4"), ("LMIE_MIX_WAVES", "32"),
+                                 ("LMIE_UGEMM_WAVES", "4"), ("LMIE_UGEMM_WAVES", "2"),
+                                 ("LMIE_TILE_GROUP", "1")])
+def test_contraction_shapes_agree(product, env):
+    """Every selectable shape of the mixed forward and reverse contractions
+    gives the default shape's outputs and gradients (cfg2: 16384 sizes,
+    x_max 914)."""
+    name, value = env
+    say("case: contraction shape %s=%s (cfg2)" % env)
+    z, p, _ = load_golden("cfg2")
+    fn = loss_fn(z["gc"], 0)
+    out0, g0 = lmie.adjoint(product, [p], None, fn, "mixed")
+    old = os.environ.get(name)
+    os.environ[name] = value
+    try:
+        out1, g1 = lmie.adjoint(product, [p], None, fn, "mixed")
+    finally:
+        if old is None:
+            del os.environ[name]
+        else:
+            os.environ[name] = old
+    assert relnorm(out1["gc"], out0["gc"]) <= 1e-6
+    keys = ("mr", "mi", "a1", "a2")
+    check([g1[k] for k in keys], [g0[k] for k in keys], 1e-5, "shape %s=%s" % env)
