@@ -2694,11 +2694,17 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                HIPCHECK(hipMemcpyAsync(w->dfw.as<double>() + (size_t) t * Np, J->df_wgt + (size_t) t * N,
                                        sizeof(double) * N, hipMemcpyHostToDevice, s));
      }
+     if (Q > 0) {
+          // initial guesses of the Gauss roots (host C library), with the inputs
+          if (w->guess.get(sizeof(double) * H)) return -1;
+          w->hguess.resize(H > 0 ? H : 1);
+          gauss_guesses(Q, w->hguess.data());
+          HIPCHECK(hipMemcpyAsync(w->guess.p, w->hguess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
+     }
      tm.mark(LMIE_T_TOTAL, 0);
 
-     // angle tables, on the side stream while the per-size recurrences run;
-     // the side stream forks here, its host preparation and launches follow
-     // the first per-size launch (side_tables) so the device starts at once
+     // angle tables, on the side streams while the per-size recurrences run
+     // (side_tables, launched after the first per-size kernel)
      hipStream_t ts = w->side;
      HIPCHECK(hipEventRecord(w->fork, s));
      HIPCHECK(hipStreamWaitEvent(ts, w->fork, 0));
@@ -2709,12 +2715,6 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           if (w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) ||
               (!mixed && (w->Bp.get(sizeof(double) * (size_t) Kp * Hp) || w->Bm.get(sizeof(double) * (size_t) Kp * Hp))))
                return -1;
-          {
-               w->hguess.resize(H > 0 ? H : 1);
-               gauss_guesses(Q, w->hguess.data());
-               if (w->guess.get(sizeof(double) * H)) return -1;
-               HIPCHECK(hipMemcpyAsync(w->guess.p, w->hguess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
-          }
           hipLaunchKernelGGL(gauss_nodes, dim3((H + 63) / 64), dim3(64), 0, s, Q, (const double *) w->guess.as<double>(),
                              w->mu.as<double>(), w->w.as<double>());
           // the expansion tables on a third stream from here: they run beside
