@@ -2094,32 +2094,45 @@ __global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
           const cd z = cscale(m, x), iz = crecip(z);
           const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
           // a_n, b_n recomputed from the kept recurrences by the forward's own
-          // code (the same bits), one order ahead for the asymmetry series
-          auto ab_at = [&](int n, cd *a, cd *b) {
-               mie_ab_at(n, x, m, im1, cmk(Dre[(size_t) n * Np + i], Dim[(size_t) n * Np + i]), Psi[(size_t) n * Np + i],
-                         Psi[(size_t) (n - 1) * Np + i], Chi[(size_t) n * Np + i], Chi[(size_t) (n - 1) * Np + i], a, b);
-          };
+          // code (the same bits), one order ahead for the asymmetry series.
+          // The recurrences are read through a window (psi, chi at j-1, j,
+          // j+1; D at j, j+1; the U row of j) whose next entries are loaded
+          // one order ahead of their use, off the dependence chain.
+          auto rd = [&](const double *a, int n, bool ok) { return ok ? a[(size_t) n * Np + i] : 0.; };
+          const size_t uq = (size_t) p.Kq * Np;
+          auto rdu = [&](int q, int n, bool ok) { return ok && p.Uf ? (double) p.Uf[q * uq + (size_t) (n - 1) * Np + i] : 0.; };
+          double psm = rd(Psi, lo - 1, lo <= n1 + 1), ps0 = rd(Psi, lo, lo <= n1), ps1 = rd(Psi, lo + 1, lo + 1 <= n1);
+          double chm = rd(Chi, lo - 1, lo <= n1 + 1), ch0 = rd(Chi, lo, lo <= n1), ch1 = rd(Chi, lo + 1, lo + 1 <= n1);
+          cd D0 = cmk(rd(Dre, lo, lo < n2), rd(Dim, lo, lo < n2));
+          cd D1 = cmk(rd(Dre, lo + 1, lo + 1 < n2), rd(Dim, lo + 1, lo + 1 < n2));
+          double u0 = rdu(0, lo, lo <= n1), u1 = rdu(1, lo, lo <= n1), u2 = rdu(2, lo, lo <= n1), u3 = rdu(3, lo, lo <= n1);
           cd am = cmk(0., 0.), bm = cmk(0., 0.), a = cmk(0., 0.), b = cmk(0., 0.), an = cmk(0., 0.), bn = cmk(0., 0.);
           if (lo <= n1) {
-               if (lo > 1) ab_at(lo - 1, &am, &bm);
-               ab_at(lo, &a, &b);
+               if (lo > 1)
+                    mie_ab_at(lo - 1, x, m, im1, cmk(Dre[(size_t) (lo - 1) * Np + i], Dim[(size_t) (lo - 1) * Np + i]),
+                              psm, Psi[(size_t) (lo - 2) * Np + i], chm, Chi[(size_t) (lo - 2) * Np + i], &am, &bm);
+               mie_ab_at(lo, x, m, im1, D0, ps0, psm, ch0, chm, &a, &b);
           }
           const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
           const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
           double sgn = (lo & 1) ? -1. : 1.;
           for (int j = lo; j < hi; ++j) {
+               // the window's next entries
+               const double ps2 = rd(Psi, j + 2, j + 2 <= n1), ch2 = rd(Chi, j + 2, j + 2 <= n1);
+               const cd D2 = cmk(rd(Dre, j + 2, j + 2 < n2), rd(Dim, j + 2, j + 2 < n2));
+               const double v0 = rdu(0, j + 1, j + 1 <= n1), v1 = rdu(1, j + 1, j + 1 <= n1);
+               const double v2 = rdu(2, j + 1, j + 1 <= n1), v3 = rdu(3, j + 1, j + 1 <= n1);
                cd lc = cmk(0., 0.);
                if (j <= n1) {
-                    if (j < n1) ab_at(j + 1, &an, &bn);
-                    const double psi = Psi[j * Np + i], psi0 = Psi[(j - 1) * Np + i];
-                    const cd zj = cmk(psi, Chi[j * Np + i]), zjm = cmk(psi0, Chi[(j - 1) * Np + i]);
-                    const cd Dj = cmk(Dre[j * Np + i], Dim[j * Np + i]);
+                    if (j < n1) mie_ab_at(j + 1, x, m, im1, D1, ps1, ps0, ch1, ch0, &an, &bn);
+                    const double psi = ps0;
+                    const cd zj = cmk(ps0, ch0), zjm = cmk(psm, chm);
+                    const cd Dj = D0;
                     const double jx = j / x;
                     const double fv1 = 2. * j + 1., fv4 = p.ft[FT_FV4 * ld + j];
                     cd Sb, Tb;
                     if (p.Uf) {
-                         const size_t uq = (size_t) p.Kq * Np, o = (size_t) (j - 1) * Np + i;
-                         const cd us = cmk(p.Uf[o], p.Uf[uq + o]), ut = cmk(p.Uf[2 * uq + o], p.Uf[3 * uq + o]);
+                         const cd us = cmk(u0, u1), ut = cmk(u2, u3);
                          // 1/2 Re(conj S U_S + conj T U_T), S = fv4 (a + b), T = fv4 (a - b)
                          fq += .5 * fv4 * ((a.r + b.r) * us.r + (a.i + b.i) * us.i + (a.r - b.r) * ut.r + (a.i - b.i) * ut.i);
                          Sb = cscale(us, f);
@@ -2166,8 +2179,7 @@ __global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
                // by the forward's own operations (the same bits).  D cotangent
                // = P c + (C0 + lc); carry = g D cotangent
                const cd c1 = cscale(iz, j + 1.);
-               const cd Dn = j + 1 < n2 ? cmk(Dre[(size_t) (j + 1) * Np + i], Dim[(size_t) (j + 1) * Np + i]) : cmk(0., 0.);
-               const cd rc = crecip_nr(cadd(Dn, c1));
+               const cd rc = crecip_nr(cadd(D1, c1));
                const cd ic22 = cmul(rc, rc);
                const cd g = cconj(ic22);
                const cd h = cconj(cmul(cadd(cmk(1., 0.), ic22), cscale(cmul(c1, iz), -1.)));
@@ -2176,6 +2188,10 @@ __global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
                Z0 = cadd(Z0, cmul(h, t));
                P = cmul(g, P);
                C0 = cmul(g, t);
+               psm = ps0; ps0 = ps1; ps1 = ps2;
+               chm = ch0; ch0 = ch1; ch1 = ch2;
+               D0 = D1; D1 = D2;
+               u0 = v0; u1 = v1; u2 = v2; u3 = v3;
           }
      }
      double *o = p.scan + (size_t) k * 11 * Np + i;
