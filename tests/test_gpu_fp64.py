@@ -120,3 +120,33 @@ def test_repeat_calls_are_deterministic(product):
     b = lmie.solve(product, p, seeds, "fp64")
     for k in ("gc", "gc_l", "cext", "g_l"):
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_cross_sections_only_no_angles(product, oracle, prec):
+    """calc_gc = calc_lc = 0: no Gauss angles, no contraction, no expansion;
+    the series and their tangents only."""
+    say("case: cross sections only (%s)" % prec)
+    p = dict(dist_type="gamma", n_int2=3, n_quad=7, calc_gc=0, calc_lc=0, **{"lambda": 0.63},
+             mr=1.41, mi=3e-3, a1=0.7, a2=0.25, r1=0.02, r2=5.0, accuracy=-1.0)
+    seeds = {"a1": [1, 0, 0], "mr": [0, 1, 0], "lambda": [0, 0, 1]}
+    r = lmie.solve(product, p, seeds, prec)
+    o = abi.solve(oracle, p, seeds, 1)
+    for k in ("cext", "csca", "cbak", "g", "reff", "veff"):
+        assert abs(r[k] - o[k]) <= 1e-12 * abs(o[k]), k
+    for k in ("cext_l", "csca_l", "cbak_l", "g_l"):
+        assert relnorm(r[k], o[k]) <= 1e-12, k
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_three_size_points(product, oracle, prec):
+    """A grid of 3 points (one padded tile, 61 padding lanes)."""
+    say("case: 3 size points (%s)" % prec)
+    p = dict(dist_type="log_normal", n_int2=1, n_quad=3, calc_gc=1, calc_lc=1, **{"lambda": 0.5},
+             mr=1.55, mi=0.01, a1=0.6, a2=0.2, r1=0.1, r2=2.0, accuracy=-1.0)
+    r = lmie.solve(product, p, None, prec)
+    o = abi.solve(oracle, p, None, 1)
+    tol = 1e-12 if prec == "fp64" else 1e-5
+    assert abs(r["csca"] - o["csca"]) <= 1e-12 * abs(o["csca"])
+    assert relnorm(r["gc"], o["gc"]) <= tol
+    assert relnorm(r["lc"], o["lc"]) <= tol
