@@ -806,6 +806,8 @@ __global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
           for (int r = max(8 * g, n1); r < min(8 * g + 8, kf); ++r)
                for (int c = 0; c < 4; ++c) p.A[c * plane + (size_t) r * Np + i] = 0.;
      }
+     // mie_sums reads the groups below (n1 + 7) / 8 only
+     if (j0 > n1) return;
      double *pt = p.part + (size_t) g * Np + i;
      pt[0] = se;
      pt[gp] = ss;
@@ -2194,6 +2196,8 @@ __global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
                u0 = v0; u1 = v1; u2 = v2; u3 = v3;
           }
      }
+     // rev_carry reads the chunks below (n2 - 1 + RCH - 1) / RCH only
+     if (lo >= n2) return;
      double *o = p.scan + (size_t) k * 11 * Np + i;
      o[0] = P.r; o[Np] = P.i; o[2 * Np] = C0.r; o[3 * Np] = C0.i;
      o[4 * Np] = Q.r; o[5 * Np] = Q.i; o[6 * Np] = Z0.r; o[7 * Np] = Z0.i;
