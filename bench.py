@@ -28,6 +28,13 @@ FP64 GPU forward of the same workload is timed beside it.  The JSON line reports
 
     python bench.py --calibrate-copy BYTES   one plain device copy of BYTES
                (calibration point of the traffic counters), then exit
+
+    python bench.py --workload retrieval-batch [--steps K] [--warmup W]
+               the 32 seeded retrieval cases (log-normal, 4000 sizes each, x_max
+               89-554): mixed forward + reverse of each case's L2 loss, all cases
+               in one lmie_solution_adjoint_batch call, against one call per case
+               and the plain FP64 forward one call per case; wall clock of whole
+               calls (host preparation included; the cases' inputs are scalars)
 """
 import argparse
 import ctypes as C
@@ -80,6 +87,57 @@ def cpu_baseline(threads):
                       "(1/2 of the grid, same r range and 3810 angles): %.1f s" % dt}
 
 
+def retrieval_batch(lib, steps, warmup):
+    """JSON line of the 32-case retrieval workload (--workload retrieval-batch)."""
+    from lmie_amd import lmie, workloads as W
+    cases = [p for p, _ in W.cfg3_cases()]
+    c_ref = [lmie.solve(lib, p, None, "fp64")["gc"] for p in cases]
+    tw = [W.expansion_loss_targets(c_ref[c], 100 + c) for c in range(len(cases))]
+    units = sum(sum_n1(p) for p in cases)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return (time.perf_counter() - t0) / steps
+
+    def batch(nw):
+        return lambda: lmie.adjoint_batch(lib, cases, [t for t, _ in tw], [w for _, w in tw],
+                                          "mixed", n_workers=nw)
+
+    def single():
+        for c, p in enumerate(cases):
+            t, w = tw[c]
+            lmie.adjoint(lib, [p], None, lambda o: {"gc": 2.0 * w * (o["gc"] - t)}, "mixed")
+
+    def fp64():
+        for p in cases:
+            lmie.solve(lib, p, None, "fp64")
+
+    t_batch = {nw: timed(batch(nw)) for nw in (1, 2, 4, 8)}
+    best = min(t_batch, key=t_batch.get)
+    t_single, t_fp64 = timed(single), timed(fp64)
+    dt = t_batch[4]
+    return {
+        "metric": METRIC, "value": units / dt, "unit": "size-pt*n-term/s", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f16-split/f32-acc+f64",
+        "data": "synthetic: 32 seeded retrieval cases",
+        "config": {"workload": "32 retrieval cases, log-normal, 40x100 sizes each, x_max 89-554, "
+                               "mixed forward + reverse of a per-case L2 loss on gc, one batch call, 4 workers",
+                   "sum_n1": units,
+                   "batch_ms_by_workers": {str(k): v * 1e3 for k, v in t_batch.items()},
+                   "best_workers": best,
+                   "one_call_per_case_ms": t_single * 1e3,
+                   "fp64_forward_one_call_per_case_ms": t_fp64 * 1e3,
+                   "grads_per_s": len(cases) / dt,
+                   "timing": "wall clock of whole calls, host preparation and transfers included"},
+        "roofline": None, "cpu_baseline": None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +149,7 @@ def main():
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="omit the separate plain-FP64 run (counter passes)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
+    ap.add_argument("--workload", default="two-mode", choices=["two-mode", "retrieval-batch"])
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the size points "
@@ -109,6 +168,9 @@ def main():
             raise SystemExit("copy failed: " + lmie.last_error(lib))
         print(json.dumps({"copy_bytes": a.calibrate_copy, "ms": ms.value,
                           "GB/s": 2 * a.calibrate_copy / ms.value / 1e6}), flush=True)
+        return
+    if a.workload == "retrieval-batch":
+        print(json.dumps(retrieval_batch(lib, a.steps, a.warmup)), flush=True)
         return
     modes = [(W.cfg4_mode_params(k), None) for k in range(2)]
     weights = [m["weight"] for m in W.CFG4_MODES]

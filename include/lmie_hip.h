@@ -148,6 +148,37 @@ int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int 
                           lmie_out_data *out, int alloc_out, const lmie_opts *opts,
                           lmie_cotangent_fn cot, void *user, lmie_grad *grad, double *grad_modes);
 
+/* Independent retrieval cases in one call: case c is the single distribution
+   cases[c] with its own wavelength, m and size grid; lmie_solution_adjoint is
+   applied to every case, n_workers cases at a time (0 = 4), each on its own set of
+   device streams, so the device overlaps the kernels of several small cases.
+   cot is called once per case, from the worker threads, with users[c] (users
+   may be NULL): it must be thread-safe.  outs[c], grads[c] and
+   grad_modes[5 c .. 5 c + 4] (grad_modes may be NULL) receive case c's
+   outputs and gradient; status[c] (may be NULL) is 0 for a case that
+   succeeded and -1 for one that failed.  With alloc_out the outputs of every
+   case with status 0 are allocated (free with lmie_out_free); a failed case
+   leaves nothing allocated.  Every case runs even if another fails; the
+   return value is -1 if any failed, and lmie_last_error() then names the
+   first failing case.  No counterpart in the reference. */
+int lmie_solution_adjoint_batch(int n_case, lmie_in_data *const *cases, lmie_out_data *const *outs,
+                                int alloc_out, const lmie_opts *opts, lmie_cotangent_fn cot,
+                                void *const *users, lmie_grad *grads, double *grad_modes,
+                                int *status, int n_workers);
+
+/* A ready-made cotangent function: the weighted L2 loss of the GSF
+   expansion, L = sum_kl weight_kl (gc_kl - target_kl)^2 over the [6][max_coef]
+   block of out->gc[0] (the row stride is taken from out->gc).  user points to
+   an lmie_gc_l2; gc_bar is caller memory of the same shape that receives
+   dL/d gc, and loss receives L. */
+typedef struct {
+     const double *target;  /* [6][max_coef] */
+     const double *weight;  /* [6][max_coef] */
+     double *gc_bar;        /* [6][max_coef], written */
+     double loss;           /* written */
+} lmie_gc_l2;
+int lmie_cotangent_gc_l2(const lmie_out_data *out, lmie_adj_in *adj, void *user);
+
 /* (3) Text of the last error on this thread ("" if none). */
 const char *lmie_last_error(void);
 

@@ -1136,6 +1136,13 @@ static void *dn_build(void *vp) {
      return NULL;
 }
 
+/* Frees what lmie_out_alloc gave an output without tangents and clears the
+   pointers, for the error paths of calls that allocated it. */
+static void out_release(lmie_out_data *out) {
+     lmie_out_free(out, 0);
+     out->gc = NULL; out->lc = NULL; out->pf = NULL; out->theta = NULL;
+}
+
 int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n_modes,
                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
                          lmie_adj_ctx **ctxp) {
@@ -1155,7 +1162,10 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      grid_t g;
      int rc = mix_grid(nm, weight, n_modes, 0, &g);
      free(nm); free(copies);
-     if (rc) return -1;
+     if (rc) {
+          if (alloc_out) out_release(out);
+          return -1;
+     }
      /* the number density tangents of the modes (for the reverse pass) on a
         helper thread while the forward runs on the GPU */
      dn_job dj = {modes, weight, n_modes, g.N, NULL};
@@ -1175,6 +1185,7 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
           for (int k = 0; k < n_modes; ++k) free(dj.dn[k]);
           free(dj.dn);
           grid_free(&g);
+          if (alloc_out) out_release(out);
           return -1;
      }
      lmie_adj_ctx *c = calloc(1, sizeof *c);
@@ -1254,4 +1265,107 @@ int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int 
      else rc = lmie_adjoint_reverse(c, &adj, grad, grad_modes);
      lmie_adjoint_free(c);
      return rc;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Independent cases in one call: worker threads take the next case and run
+   forward, cotangent and reverse on it.  Each worker holds its own device
+   workspace (streams and buffers) for the duration of a case, so the device
+   sees up to n_workers independent pipelines at once. */
+typedef struct {
+     int n_case, next, first_failed;
+     lmie_in_data *const *cases;
+     lmie_out_data *const *outs;
+     int alloc_out;
+     const lmie_opts *opts;
+     lmie_cotangent_fn cot;
+     void *const *users;
+     lmie_grad *grads;
+     double *grad_modes;
+     int *status;
+     pthread_mutex_t mu;
+     char err[512];
+} batch_job;
+
+static int batch_case(batch_job *b, int c) {
+     lmie_in_data *one[1] = {b->cases[c]};
+     lmie_out_data *out = b->outs[c];
+     if (!one[0] || !out) return fail("lmie_solution_adjoint_batch", "missing case");
+     lmie_adj_ctx *ctx = NULL;
+     if (lmie_adjoint_forward(one, NULL, 1, out, b->alloc_out, b->opts, &ctx)) return -1;
+     lmie_adj_in adj;
+     memset(&adj, 0, sizeof adj);
+     int rc = b->cot(out, &adj, b->users ? b->users[c] : NULL);
+     if (rc) rc = fail("lmie_solution_adjoint_batch", "the cotangent function failed");
+     else rc = lmie_adjoint_reverse(ctx, &adj, &b->grads[c], b->grad_modes ? b->grad_modes + 5 * c : NULL);
+     lmie_adjoint_free(ctx);
+     if (rc && b->alloc_out) out_release(out);
+     return rc ? -1 : 0;
+}
+
+static void *batch_worker(void *vp) {
+     batch_job *b = vp;
+     for (;;) {
+          pthread_mutex_lock(&b->mu);
+          const int c = b->next++;
+          pthread_mutex_unlock(&b->mu);
+          if (c >= b->n_case) return NULL;
+          const int rc = batch_case(b, c);
+          if (b->status) b->status[c] = rc;
+          if (rc) {
+               pthread_mutex_lock(&b->mu);
+               if (b->first_failed < 0 || c < b->first_failed) {
+                    b->first_failed = c;
+                    snprintf(b->err, sizeof b->err, "case %d: %s", c, lmie_last_error());
+               }
+               pthread_mutex_unlock(&b->mu);
+          }
+     }
+}
+
+int lmie_solution_adjoint_batch(int n_case, lmie_in_data *const *cases, lmie_out_data *const *outs,
+                                int alloc_out, const lmie_opts *opts, lmie_cotangent_fn cot,
+                                void *const *users, lmie_grad *grads, double *grad_modes,
+                                int *status, int n_workers) {
+     if (n_case < 0 || (n_case > 0 && (!cases || !outs || !grads)))
+          return fail("lmie_solution_adjoint_batch", "missing argument");
+     if (!cot) return fail("lmie_solution_adjoint_batch", "no cotangent function");
+     if (n_case == 0) return 0;
+     batch_job b;
+     memset(&b, 0, sizeof b);
+     b.n_case = n_case; b.first_failed = -1;
+     b.cases = cases; b.outs = outs; b.alloc_out = alloc_out; b.opts = opts; b.cot = cot;
+     b.users = users; b.grads = grads; b.grad_modes = grad_modes; b.status = status;
+     pthread_mutex_init(&b.mu, NULL);
+     int T = n_workers > 0 ? n_workers : 4;
+     if (T > n_case) T = n_case;
+     if (T > 64) T = 64;
+     pthread_t th[64];
+     int started = 0;
+     for (int t = 1; t < T; ++t) {
+          if (pthread_create(&th[started], NULL, batch_worker, &b) != 0) break;
+          started++;
+     }
+     batch_worker(&b);
+     for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+     pthread_mutex_destroy(&b.mu);
+     if (b.first_failed >= 0) return fail("lmie_solution_adjoint_batch", b.err);
+     return 0;
+}
+
+int lmie_cotangent_gc_l2(const lmie_out_data *out, lmie_adj_in *adj, void *user) {
+     lmie_gc_l2 *l = user;
+     if (!out || !adj || !l || !l->target || !l->weight || !l->gc_bar || !out->gc)
+          return fail("lmie_cotangent_gc_l2", "missing argument");
+     const size_t ld = (size_t) (out->gc[1] - out->gc[0]);
+     const double *c = out->gc[0];
+     double loss = 0.;
+     for (size_t q = 0; q < 6 * ld; ++q) {
+          const double d = c[q] - l->target[q];
+          loss += l->weight[q] * d * d;
+          l->gc_bar[q] = 2.0 * l->weight[q] * d;
+     }
+     l->loss = loss;
+     adj->gc = l->gc_bar;
+     return 0;
 }

@@ -52,6 +52,12 @@ def bind_all(lib):
     lib.lmie_adjoint_reverse.restype = C.c_int
     lib.lmie_adjoint_free.argtypes = [C.c_void_p]
     lib.lmie_adjoint_free.restype = None
+    PPOut = C.POINTER(POut)
+    lib.lmie_solution_adjoint_batch.argtypes = [C.c_int, C.POINTER(PIn), PPOut, C.c_int,
+                                                C.POINTER(LmieOpts), C.c_void_p,
+                                                C.POINTER(C.c_void_p), C.POINTER(LmieGrad),
+                                                C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
+    lib.lmie_solution_adjoint_batch.restype = C.c_int
     lib.lmie_last_error.restype = C.c_char_p
     lib.lmie_gpu_last_error.restype = C.c_char_p
     lib.lmie_set_timing.argtypes = [C.c_int]
@@ -208,6 +214,62 @@ def adjoint(lib, modes, weights, cot_fn, precision="fp64", device=0):
     finally:
         a.free()
     return out, grad
+
+
+class LmieGcL2(C.Structure):
+    """lmie_gc_l2, include/lmie_hip.h."""
+    _fields_ = [("target", C.POINTER(C.c_double)), ("weight", C.POINTER(C.c_double)),
+                ("gc_bar", C.POINTER(C.c_double)), ("loss", C.c_double)]
+
+
+def adjoint_batch(lib, cases, targets, weights, precision="mixed", device=0, n_workers=0):
+    """lmie_solution_adjoint_batch with the library's weighted L2 loss on gc
+    (lmie_cotangent_gc_l2): case c is the single distribution cases[c], with
+    L_c = sum_kl weights[c]_kl (gc_kl - targets[c]_kl)^2 over [6][max_coef].
+    Returns one (outputs, gradient, loss) triple per case, as ``adjoint``."""
+    n = len(cases)
+    ins = [abi.make_in(lib, p, None) for p in cases]
+    outs = [LmieOut() for _ in range(n)]
+    pin = (C.POINTER(LmieIn) * n)(*[C.pointer(x) for x in ins])
+    pout = (C.POINTER(LmieOut) * n)(*[C.pointer(o) for o in outs])
+    mc = [lib.lmie_calc_max_coef(x.lambda_, x.dist_type, x.a1, x.a2, x.a3, x.r1, x.r2) for x in ins]
+    keep, l2 = [], []
+    for c in range(n):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(targets[c], float), (6, mc[c])))
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(weights[c], float), (6, mc[c])))
+        b = np.zeros((6, mc[c]))
+        keep += [t, w, b]
+        dp = C.POINTER(C.c_double)
+        l2.append(LmieGcL2(t.ctypes.data_as(dp), w.ctypes.data_as(dp), b.ctypes.data_as(dp), 0.0))
+    users = (C.c_void_p * n)(*[C.cast(C.pointer(x), C.c_void_p) for x in l2])
+    grads = (LmieGrad * n)()
+    gm = (C.c_double * (5 * n))()
+    status = (C.c_int * n)()
+    o = _opts(precision, device)
+    cot = C.cast(lib.lmie_cotangent_gc_l2, C.c_void_p)
+    rc = lib.lmie_solution_adjoint_batch(n, pin, pout, 1, C.byref(o), cot, users, grads, gm,
+                                         status, n_workers)
+    res, err = [], None if rc == 0 else last_error(lib)
+    try:
+        for c in range(n):
+            if status[c] != 0:
+                res.append(None)
+                continue
+            fake = LmieIn.from_buffer_copy(ins[c])
+            fake.n_derivs = 0
+            out = abi.read_out(lib, cases[c], fake, outs[c], mc[c])
+            g = grads[c]
+            grad = {"lambda": g.lambda_, "mr": g.mr, "mi": g.mi, "a1": g.a1, "a2": g.a2,
+                    "a3": g.a3, "a4": g.a4, "a5": g.a5, "r1": g.r1, "r2": g.r2,
+                    "modes": np.array(list(gm[5 * c:5 * c + 5])).reshape(1, 5)}
+            res.append((out, grad, l2[c].loss))
+    finally:
+        for c in range(n):
+            if status[c] == 0:
+                lib.lmie_out_free(C.byref(outs[c]), 0)
+    if rc != 0:
+        raise RuntimeError("lmie_solution_adjoint_batch failed: " + err)
+    return res
 
 
 def nan_to_none(v):
