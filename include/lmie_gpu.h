@@ -84,6 +84,23 @@ typedef struct {
 int lmie_gpu_adjoint(void *state, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out);
 void lmie_gpu_release(void *state);
 
+/* Batches of independent cases, one kernel launch for all cases.  The host
+   thread that runs case `slot` of a batch calls lmie_gpu_batch_enter, then
+   the same lmie_gpu_forward / lmie_gpu_adjoint calls as for a single case, then
+   lmie_gpu_batch_leave (also after a failure).  Those calls block until every
+   case of the batch has made the same call or left; one of the threads then
+   runs the pass for all of them and each returns its own case's result.
+   Fused into single launches: mixed precision forwards with angles, no
+   tangent channels, save_control 0 and keep = 1, and their reverse passes with
+   a gc or lc cotangent; other cases run one after another inside the pass.
+   No counterpart in the reference (its cases are separate calls).
+     lmie_gpu_batch_fused   cases fused in pass 0 (forward) or 1 (reverse) */
+void *lmie_gpu_batch_open(int n_case);
+int lmie_gpu_batch_enter(void *batch, int slot);
+void lmie_gpu_batch_leave(void);
+int lmie_gpu_batch_fused(void *batch, int pass);
+void lmie_gpu_batch_close(void *batch);
+
 /* Scattering matrix at n_theta user angles from expansion coefficients
    (misc/rtutil_scat_int.c:510-797): coefs [n_set][6][ld], n_coef used,
    theta [n_theta] in degrees, pf [n_set][6][n_theta]; gsf = 1 for

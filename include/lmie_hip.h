@@ -150,8 +150,11 @@ int lmie_solution_adjoint(lmie_in_data *const *modes, const double *weight, int 
 
 /* Independent retrieval cases in one call: case c is the single distribution
    cases[c] with its own wavelength, m and size grid; lmie_solution_adjoint is
-   applied to every case, n_workers cases at a time (0 = 4), each on its own set of
-   device streams, so the device overlaps the kernels of several small cases.
+   applied to every case.  n_workers = 0: the cases run together, up to 64 at a
+   time, and each kernel of the device passes is launched once for all of
+   them (mixed precision with a gc or lc cotangent; other cases run one
+   after another).  n_workers > 0: that many cases at a time, each on its own
+   set of device streams.
    cot is called once per case, from the worker threads, with users[c] (users
    may be NULL): it must be thread-safe.  outs[c], grads[c] and
    grad_modes[5 c .. 5 c + 4] (grad_modes may be NULL) receive case c's
@@ -165,6 +168,10 @@ int lmie_solution_adjoint_batch(int n_case, lmie_in_data *const *cases, lmie_out
                                 int alloc_out, const lmie_opts *opts, lmie_cotangent_fn cot,
                                 void *const *users, lmie_grad *grads, double *grad_modes,
                                 int *status, int n_workers);
+
+/* Cases of this thread's last lmie_solution_adjoint_batch call whose forward
+   (pass 0) or reverse (pass 1) ran in the single launches of the fused mode. */
+int lmie_last_batch_fused(int pass);
 
 /* A ready-made cotangent function: the weighted L2 loss of the GSF
    expansion, L = sum_kl weight_kl (gc_kl - target_kl)^2 over the [6][max_coef]

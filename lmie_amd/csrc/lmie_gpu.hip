@@ -37,6 +37,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -201,7 +202,7 @@ __device__ __forceinline__ cd cdiv_nr(cd a, cd b) {
 // initial guesses come from the host (the C library's cos and tan, as the
 // reference) and every operation here is a separately rounded IEEE operation
 // (no contraction): the rule is the reference's rule bit for bit.
-__global__ void gauss_nodes(int n, const double *guess, double *mu, double *w) {
+__device__ __forceinline__ void gauss_nodes_body(int n, const double *guess, double *mu, double *w) {
 #pragma clang fp contract(off)
      int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= (n + 1) / 2) return;
@@ -232,6 +233,22 @@ __global__ void gauss_nodes(int n, const double *guess, double *mu, double *w) {
      mu[n - 1 - i] = __dadd_rn(0., off);
      w[i] = w[n - 1 - i] = __dmul_rn(__dsub_rn(1., __dmul_rn(off, y)), c);
 }
+__global__ void gauss_nodes(int n, const double *guess, double *mu, double *w) { gauss_nodes_body(n, guess, mu, w); }
+// gauss_nodes over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct gauss_nodes_B {
+     int n;
+     const double *guess;
+     double *mu;
+     double *w;
+     unsigned gx, gy;
+};
+__global__ void gauss_nodes_b(const gauss_nodes_B *cases) {
+     const gauss_nodes_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     gauss_nodes_body(a.n, a.guess, a.mu, a.w);
+}
+
 
 // pi_n +- tau_n at the positive angles mu_{H+j}: one thread per angle, upward
 // recurrences pi_{n+1} = mu pi_n + (n+1)/n (mu pi_n - pi_{n-1}),
@@ -263,7 +280,7 @@ __global__ void pitau_table(int H, int Hp, int n1max, int Kp, const double *mu, 
 // (pi_n +- tau_n) / (n (n+1)) * kSB.  A lane (angle) writes 16 bytes per plane
 // per group of 8 orders; consecutive lanes are consecutive 16-byte chunks.
 template <class E>
-__global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) {
+__device__ __forceinline__ void pitau_table16_body(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) {
      const int j = blockIdx.x * blockDim.x + threadIdx.x;
      if (j >= Hp) return;
      const size_t plane = (size_t) Kb * Hp * 8;
@@ -293,6 +310,26 @@ __global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu
           }
      }
 }
+template <class E>
+__global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) { pitau_table16_body<E>(H, Hp, n1max, Kb, mu, B16); }
+// pitau_table16 over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct pitau_table16_B {
+     int H;
+     int Hp;
+     int n1max;
+     int Kb;
+     const double *mu;
+     unsigned short *B16;
+     unsigned gx, gy;
+};
+template <class E>
+__global__ void pitau_table16_b(const pitau_table16_B *cases) {
+     const pitau_table16_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     pitau_table16_body<E>(a.H, a.Hp, a.n1max, a.Kb, a.mu, a.B16);
+}
+
 
 // Generalized spherical functions at the positive angles, l = 0 .. C-1, by
 // their three-term recurrences in l (misc/rtutil_math.c:956-1016).  Layout
@@ -301,7 +338,7 @@ __global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu
 // The five recurrences are independent: blockIdx.y picks one (0: P00, 1: P02,
 // 2: P22, 3: P2-2, 4: P_l), so five times as many threads run one fifth of the
 // steps.  Columns H .. Hp-1 are written as zeros.
-__global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) {
+__device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const double *mu, double *T) {
      // separately rounded operations in the reference's order (bitwise the
      // reference's functions)
 #pragma clang fp contract(off)
@@ -359,6 +396,23 @@ __global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) {
           b0 = v;
      }
 }
+__global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) { gsf_table_body(H, Hp, C, mu, T); }
+// gsf_table over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct gsf_table_B {
+     int H;
+     int Hp;
+     int C;
+     const double *mu;
+     double *T;
+     unsigned gx, gy;
+};
+__global__ void gsf_table_b(const gsf_table_B *cases) {
+     const gsf_table_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     gsf_table_body(a.H, a.Hp, a.C, a.mu, a.T);
+}
+
 
 // ---------------------------------------------------------------------------
 // Per-size Mie coefficients.  One thread per size point (NT tangent channels
@@ -615,7 +669,7 @@ struct ChainArgs {
      double *PC;                // [2][n1max + 1][Np]: psi_j, chi_j at row j
 };
 
-__global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
+__device__ __forceinline__ void mie_chains_body(ChainArgs p) {
 #pragma clang fp contract(off)
      const int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= p.N) return;
@@ -676,6 +730,19 @@ __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) {
           }
      }
 }
+__global__ __launch_bounds__(64) void mie_chains(ChainArgs p) { mie_chains_body(p); }
+// mie_chains over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct mie_chains_B {
+     ChainArgs p;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(64) void mie_chains_b(const mie_chains_B *cases) {
+     const mie_chains_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     mie_chains_body(a.p);
+}
+
 
 // a_j, b_j (src/lmie_core.c:557-583), operations as mie_coefs
 __device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, double psi, double psi0, double chi1,
@@ -698,7 +765,7 @@ __device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, d
 // instead of dividing in every lane.
 constexpr int FT_FV4 = 0, FT_FV3M = 1, FT_FV4M = 2, FT_FV3 = 3, FT_ROWS = 4;
 
-__global__ void order_factors(int n, double *ft) {
+__device__ __forceinline__ void order_factors_body(int n, double *ft) {
 #pragma clang fp contract(off)
      const int j = blockIdx.x * blockDim.x + threadIdx.x;
      if (j > n) return;
@@ -709,6 +776,20 @@ __global__ void order_factors(int n, double *ft) {
      ft[FT_FV4M * ld + j] = j > 1 ? (2. * j - 1.) / ((j - 1.) * j) : 0.;
      ft[FT_FV3 * ld + j] = j > 0 ? (j + 2.) / ((j + 1.) / j) : 0.;
 }
+__global__ void order_factors(int n, double *ft) { order_factors_body(n, ft); }
+// order_factors over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct order_factors_B {
+     int n;
+     double *ft;
+     unsigned gx, gy;
+};
+__global__ void order_factors_b(const order_factors_B *cases) {
+     const order_factors_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     order_factors_body(a.n, a.ft);
+}
+
 
 struct TermArgs {
      int N, Np, n1max, n2rows, Kp, Kb, ng;
@@ -724,7 +805,7 @@ struct TermArgs {
 };
 
 template <class E>
-__global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
+__device__ __forceinline__ void mie_terms_body(TermArgs p) {
 #pragma clang fp contract(off)
      constexpr bool MIX = !std::is_void<E>::value;
      const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
@@ -815,12 +896,26 @@ __global__ __launch_bounds__(64) void mie_terms(TermArgs p) {
      pt[3 * gp] = cb.i;
      pt[4 * gp] = sg;
 }
+template <class E>
+__global__ __launch_bounds__(64) void mie_terms(TermArgs p) { mie_terms_body<E>(p); }
+// mie_terms over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct mie_terms_B {
+     TermArgs p;
+     unsigned gx, gy;
+};
+template <class E>
+__global__ __launch_bounds__(64) void mie_terms_b(const mie_terms_B *cases) {
+     const mie_terms_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     mie_terms_body<E>(a.p);
+}
+
 
 // Series per size from the group partials (src/lmie_core.c:586-596): a block
 // of 32 sizes x 8 lanes, lane l adds the groups g = l mod 8, then the eight
 // lane sums are added in a fixed order (deterministic).
-__global__ __launch_bounds__(256) void mie_sums(const double *part, int N, int Np, int ng, const int *n1, double *S,
-                                                double *CB) {
+__device__ __forceinline__ void mie_sums_body(const double *part, int N, int Np, int ng, const int *n1, double *S, double *CB) {
      __shared__ double red[5][8][32];
      const int il = threadIdx.x & 31, l = threadIdx.x >> 5, i = blockIdx.x * 32 + il;
      const size_t gp = (size_t) ng * Np;
@@ -854,6 +949,25 @@ __global__ __launch_bounds__(256) void mie_sums(const double *part, int N, int N
           CB[Np + i] = cb.i;
      }
 }
+__global__ __launch_bounds__(256) void mie_sums(const double *part, int N, int Np, int ng, const int *n1, double *S, double *CB) { mie_sums_body(part, N, Np, ng, n1, S, CB); }
+// mie_sums over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct mie_sums_B {
+     const double *part;
+     int N;
+     int Np;
+     int ng;
+     const int *n1;
+     double *S;
+     double *CB;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(256) void mie_sums_b(const mie_sums_B *cases) {
+     const mie_sums_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     mie_sums_body(a.part, a.N, a.Np, a.ng, a.n1, a.S, a.CB);
+}
+
 
 // ---------------------------------------------------------------------------
 // Weighted sums over sizes: out[k] = sum_i u_k[i] v_k[i] (+ sum_i u2_k[i] v2_k[i]),
@@ -862,7 +976,7 @@ struct DotJob {
      const double *u, *v, *u2, *v2;
 };
 
-__global__ __launch_bounds__(256) void weighted_dots(const DotJob *jobs, int n, double *out) {
+__device__ __forceinline__ void weighted_dots_body(const DotJob *jobs, int n, double *out) {
      __shared__ double red[256];
      const DotJob jb = jobs[blockIdx.x];
      double s = 0.;
@@ -878,6 +992,21 @@ __global__ __launch_bounds__(256) void weighted_dots(const DotJob *jobs, int n, 
      }
      if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
+__global__ __launch_bounds__(256) void weighted_dots(const DotJob *jobs, int n, double *out) { weighted_dots_body(jobs, n, out); }
+// weighted_dots over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct weighted_dots_B {
+     const DotJob *jobs;
+     int n;
+     double *out;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(256) void weighted_dots_b(const weighted_dots_B *cases) {
+     const weighted_dots_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     weighted_dots_body(a.jobs, a.n, a.out);
+}
+
 
 // ---------------------------------------------------------------------------
 // Angular contraction, FP64 matrix cores.
@@ -1176,7 +1305,7 @@ struct Mix2Args {
 };
 
 template <class E, int WB, int NWV, int NW>
-__global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
+__device__ __forceinline__ void angular_mix2_body(const Mix2Args &p, int2 tk) {
      constexpr int NT = 128 * NWV, TM = 64, TN = 16 * WB * NWV, G = KM / 8;
      constexpr int NA = 8 * G * TM / NT, NB = 4 * G * TN / NT;   // staged v4u per thread
      static_assert(NA * NT == 8 * G * TM && NB * NT == 4 * G * TN, "staging split");
@@ -1184,7 +1313,6 @@ __global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
      __shared__ v4u sB[2][4 * G][TN];
      static_assert(sizeof(sA) >= sizeof(double) * 2 * NW * 8 * TN, "epilogue buffer");
 
-     const int2 tk = p.task[blockIdx.x];
      if (tk.x < 0) return;
      const int st = tk.x, jt = tk.y;
      const int it0 = st * TM, jt0 = jt * TN, kmax = p.ktile[st], nch = kmax / KM;
@@ -1332,6 +1460,19 @@ __global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
           out[((size_t) c * 4 + (k & 3)) * p.Q + (k < 4 ? p.H + j : p.H - 1 - j)] = v;
      }
 }
+template <class E, int WB, int NWV, int NW>
+__global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
+     angular_mix2_body<E, WB, NWV, NW>(p, p.task[blockIdx.x]);
+}
+// angular_mix2 over the cases of a batch: one task list for all cases, task
+// (size tile, angle tile | case << 16), padding tasks (-1, case << 16).
+template <class E, int WB, int NWV, int NW>
+__global__ __launch_bounds__(128 * NWV) void angular_mix2_b(const Mix2Args *cases, const int2 *task) {
+     const int2 t = task[blockIdx.x];
+     const Mix2Args p = cases[t.y >> 16];
+     angular_mix2_body<E, WB, NWV, NW>(p, make_int2(t.x, t.y & 0xffff));
+}
+
 
 
 // The split 16-bit contraction on the 32x32x16 matrix instruction: one
@@ -1484,8 +1625,7 @@ __global__ __launch_bounds__(512) void angular_mix32(Mix2Args p) {
 // partials [chunk][24][Hp]), then the fixed-order sum over chunks.
 constexpr int EXA_L = 128;
 
-__global__ __launch_bounds__(64) void expand_adjoint_part(const double *T, const double *gbar, const double *lbar,
-                                                          int H, int Hp, int C, double *part) {
+__device__ __forceinline__ void expand_adjoint_part_body(const double *T, const double *gbar, const double *lbar, int H, int Hp, int C, double *part) {
      const int j = blockIdx.x * blockDim.x + threadIdx.x, ck = blockIdx.y;
      if (j >= Hp) return;
      const size_t plane = (size_t) C * Hp;
@@ -1524,8 +1664,27 @@ __global__ __launch_bounds__(64) void expand_adjoint_part(const double *T, const
           out[(size_t) (18 + k) * Hp] = Lp[k];
      }
 }
+__global__ __launch_bounds__(64) void expand_adjoint_part(const double *T, const double *gbar, const double *lbar, int H, int Hp, int C, double *part) { expand_adjoint_part_body(T, gbar, lbar, H, Hp, C, part); }
+// expand_adjoint_part over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct expand_adjoint_part_B {
+     const double *T;
+     const double *gbar;
+     const double *lbar;
+     int H;
+     int Hp;
+     int C;
+     double *part;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(64) void expand_adjoint_part_b(const expand_adjoint_part_B *cases) {
+     const expand_adjoint_part_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     expand_adjoint_part_body(a.T, a.gbar, a.lbar, a.H, a.Hp, a.C, a.part);
+}
 
-__global__ void expand_adjoint_fin(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) {
+
+__device__ __forceinline__ void expand_adjoint_fin_body(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) {
      const int j = blockIdx.x * blockDim.x + threadIdx.x;
      if (j >= H) return;
      double v[24];
@@ -1543,6 +1702,25 @@ __global__ void expand_adjoint_fin(const double *part, int nck, const double *w,
           pfbar[3 * Q + a] = wq * (E[5] + L[5]);
      }
 }
+__global__ void expand_adjoint_fin(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) { expand_adjoint_fin_body(part, nck, w, H, Hp, Q, pfbar); }
+// expand_adjoint_fin over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct expand_adjoint_fin_B {
+     const double *part;
+     int nck;
+     const double *w;
+     int H;
+     int Hp;
+     int Q;
+     double *pfbar;
+     unsigned gx, gy;
+};
+__global__ void expand_adjoint_fin_b(const expand_adjoint_fin_B *cases) {
+     const expand_adjoint_fin_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     expand_adjoint_fin_body(a.part, a.nck, a.w, a.H, a.Hp, a.Q, a.pfbar);
+}
+
 
 // Cotangents of the four complex sums at one (size, angle) from those of the
 // four matrix elements (gradient convention d/dRe + i d/dIm):
@@ -1692,8 +1870,7 @@ struct AdjScal {
 // Weights of the eight right-hand tables per positive node j (row q of wts):
 // a, a', c, c', beta_r, beta_i, beta'_r, beta'_i, divided by their largest
 // magnitude.  One workgroup.
-__global__ __launch_bounds__(1024) void adj_weights(const double *pfbar, int H, int Hq, int Q, double *wts,
-                                                    AdjScal *sc) {
+__device__ __forceinline__ void adj_weights_body(const double *pfbar, int H, int Hq, int Q, double *wts, AdjScal *sc) {
      __shared__ double red[1024];
      double mx = 0.;
      for (int j = threadIdx.x; j < H; j += blockDim.x)
@@ -1728,6 +1905,24 @@ __global__ __launch_bounds__(1024) void adj_weights(const double *pfbar, int H, 
           sc->gmax = 0u;
      }
 }
+__global__ __launch_bounds__(1024) void adj_weights(const double *pfbar, int H, int Hq, int Q, double *wts, AdjScal *sc) { adj_weights_body(pfbar, H, Hq, Q, wts, sc); }
+// adj_weights over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct adj_weights_B {
+     const double *pfbar;
+     int H;
+     int Hq;
+     int Q;
+     double *wts;
+     AdjScal *sc;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(1024) void adj_weights_b(const adj_weights_B *cases) {
+     const adj_weights_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     adj_weights_body(a.pfbar, a.H, a.Hq, a.Q, a.wts, a.sc);
+}
+
 
 // Split tables with angles along the contraction: [plane][Hq / 8][Kq][8],
 // 8 consecutive positive nodes per 16 bytes, from the forward's split tables
@@ -1737,8 +1932,7 @@ __global__ __launch_bounds__(1024) void adj_weights(const double *pfbar, int H, 
 // 7) or P- (q = 1, 2, 4, 5), from P = high + low (relative error about 2^-22),
 // high and low parts.  One thread per (order, group of 8 nodes).
 template <class E>
-__global__ __launch_bounds__(64) void adj_tables(const unsigned short *B16, int Kb, int Hb, const double *wts, int H,
-                                                 int Hq, int Kq, int n1max, unsigned short *L16, unsigned short *R16) {
+__device__ __forceinline__ void adj_tables_body(const unsigned short *B16, int Kb, int Hb, const double *wts, int H, int Hq, int Kq, int n1max, unsigned short *L16, unsigned short *R16) {
      const int r = blockIdx.x * 64 + threadIdx.x, jg = blockIdx.y;
      if (r >= Kq) return;
      const size_t plane = (size_t) (Hq / 8) * Kq * 8, o = ((size_t) jg * Kq + r) * 8, bpl = (size_t) Kb * Hb * 8;
@@ -1776,6 +1970,30 @@ __global__ __launch_bounds__(64) void adj_tables(const unsigned short *B16, int 
           *(v4u *) (R16 + q * plane + o) = w;
      }
 }
+template <class E>
+__global__ __launch_bounds__(64) void adj_tables(const unsigned short *B16, int Kb, int Hb, const double *wts, int H, int Hq, int Kq, int n1max, unsigned short *L16, unsigned short *R16) { adj_tables_body<E>(B16, Kb, Hb, wts, H, Hq, Kq, n1max, L16, R16); }
+// adj_tables over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct adj_tables_B {
+     const unsigned short *B16;
+     int Kb;
+     int Hb;
+     const double *wts;
+     int H;
+     int Hq;
+     int Kq;
+     int n1max;
+     unsigned short *L16;
+     unsigned short *R16;
+     unsigned gx, gy;
+};
+template <class E>
+__global__ __launch_bounds__(64) void adj_tables_b(const adj_tables_B *cases) {
+     const adj_tables_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     adj_tables_body<E>(a.B16, a.Kb, a.Hb, a.wts, a.H, a.Hq, a.Kq, a.n1max, a.L16, a.R16);
+}
+
 
 // G on the 32x32x16 matrix instruction: one workgroup per 64 x 64 tile of
 // (n, m), four waves of 32 x 32, contraction over the positive nodes in
@@ -1789,8 +2007,7 @@ __global__ __launch_bounds__(64) void adj_tables(const unsigned short *B16, int 
 // Im Gst (row n, column m, scaled by kSB^2 / (wmax n (n+1) m (m+1))), and the
 // largest magnitude.
 template <class E>
-__global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, const unsigned short *R16, int Hq,
-                                                  int Kq, float *Gf, AdjScal *sc) {
+__device__ __forceinline__ void adj_gbuild_body(const unsigned short *L16, const unsigned short *R16, int Hq, int Kq, float *Gf, AdjScal *sc) {
      constexpr int T = 64, G = 2, KS = 8 * G;
      constexpr int NL = 4 * G * T / 256, NR = 16 * G * T / 256;
      __shared__ v4u sL[2][4 * G][T];
@@ -1868,6 +2085,26 @@ __global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, con
      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
      if (lane == 0) atomicMax(&sc->gmax, __float_as_uint(mx));
 }
+template <class E>
+__global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, const unsigned short *R16, int Hq, int Kq, float *Gf, AdjScal *sc) { adj_gbuild_body<E>(L16, R16, Hq, Kq, Gf, sc); }
+// adj_gbuild over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct adj_gbuild_B {
+     const unsigned short *L16;
+     const unsigned short *R16;
+     int Hq;
+     int Kq;
+     float *Gf;
+     AdjScal *sc;
+     unsigned gx, gy;
+};
+template <class E>
+__global__ __launch_bounds__(256) void adj_gbuild_b(const adj_gbuild_B *cases) {
+     const adj_gbuild_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     adj_gbuild_body<E>(a.L16, a.R16, a.Hq, a.Kq, a.Gf, a.sc);
+}
+
 
 // Exponent k with |G| * 2^(14 - k) < 2^14 for every entry (0 for G = 0).
 __device__ __forceinline__ int adj_gexp(const AdjScal *sc) {
@@ -1883,7 +2120,7 @@ __device__ __forceinline__ int adj_gexp(const AdjScal *sc) {
 // low) with Gts(n, m) = conj Gst(m, n), all times 2^(14 - k).  One thread per
 // (group of 8 m, n).
 template <class E>
-__global__ __launch_bounds__(64) void adj_gsplit(const float *Gf, const AdjScal *sc, int Kq, unsigned short *G16) {
+__device__ __forceinline__ void adj_gsplit_body(const float *Gf, const AdjScal *sc, int Kq, unsigned short *G16) {
      const int n = blockIdx.x * 64 + threadIdx.x, mg = blockIdx.y;
      if (n >= Kq) return;
      const double s = ldexp(1., 14 - adj_gexp(sc));
@@ -1902,6 +2139,24 @@ __global__ __launch_bounds__(64) void adj_gsplit(const float *Gf, const AdjScal 
           *(v4u *) (G16 + q * plane + o) = w;
      }
 }
+template <class E>
+__global__ __launch_bounds__(64) void adj_gsplit(const float *Gf, const AdjScal *sc, int Kq, unsigned short *G16) { adj_gsplit_body<E>(Gf, sc, Kq, G16); }
+// adj_gsplit over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct adj_gsplit_B {
+     const float *Gf;
+     const AdjScal *sc;
+     int Kq;
+     unsigned short *G16;
+     unsigned gx, gy;
+};
+template <class E>
+__global__ __launch_bounds__(64) void adj_gsplit_b(const adj_gsplit_B *cases) {
+     const adj_gsplit_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     adj_gsplit_body<E>(a.Gf, a.sc, a.Kq, a.G16);
+}
+
 
 // Operands of U = G A (adj_ugemm32).  Accumulators: U_S re, im, U_T re, im;
 // output Uf[4][Kq][Np] in fp32, unscaled to U = Sbar / f.
@@ -1924,15 +2179,13 @@ struct UArgs {
 // WA = 2: four waves, each operand fragment read for twice the products;
 // TS = 64: four waves, two workgroups per CU.
 template <class E, int WA, int TS>
-__global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
-void adj_ugemm32(UArgs p) {
+__device__ __forceinline__ void adj_ugemm32_body(const UArgs &p, int2 tk) {
      constexpr int TO = 64, G = 2;                           // orders, groups of 8 per stage
      constexpr int NT = TS * 4 / WA, WS = TS / 32 / WA;      // threads, size-block waves
      constexpr int NA = 8 * G * TS / NT, NG = 12 * G * TO / NT;
      static_assert(NA * NT == 8 * G * TS && NG * NT == 12 * G * TO, "staging split");
      __shared__ v4u sA[2][8 * G][TS];
      __shared__ v4u sG[2][12 * G][TO];
-     const int2 tk = p.task[blockIdx.x];
      if (tk.x < 0) return;
      const int it0 = tk.x * TS, n0 = tk.y * TO, nch = p.ktile[tk.x] / (8 * G);
      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -2033,6 +2286,20 @@ void adj_ugemm32(UArgs p) {
                for (int k = 0; k < 4; ++k) p.Uf[((size_t) k * Kq + r) * Np + i] = (float) (acc[a][k][v] * f);
           }
 }
+template <class E, int WA, int TS>
+__global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
+void adj_ugemm32(UArgs p) {
+     adj_ugemm32_body<E, WA, TS>(p, p.task[blockIdx.x]);
+}
+// adj_ugemm32 over the cases of a batch: task (size tile, order tile | case << 16).
+template <class E, int WA, int TS>
+__global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
+void adj_ugemm32_b(const UArgs *cases, const int2 *task) {
+     const int2 t = task[blockIdx.x];
+     const UArgs p = cases[t.y >> 16];
+     adj_ugemm32_body<E, WA, TS>(p, make_int2(t.x, t.y & 0xffff));
+}
+
 
 // Shape of the reverse contraction (environment LMIE_UGEMM_WAVES, read at
 // every call): 8 (default) eight waves on 128 sizes, 4 four waves on 128
@@ -2081,7 +2348,7 @@ struct RevArgs {
 
 // Per chunk, with the carry entering it as c, every carry and D cotangent
 // inside is affine in c: carry out = P c + C0, z cotangent = Q c + Z0.
-__global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
+__device__ __forceinline__ void rev_orders_body(RevArgs p) {
      const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
      if (i >= p.Np) return;
      const size_t Np = p.Np, plane = (size_t) p.Kp * Np, dpl = (size_t) p.n2rows * Np;
@@ -2203,8 +2470,21 @@ __global__ __launch_bounds__(64) void rev_orders(RevArgs p) {
      o[4 * Np] = Q.r; o[5 * Np] = Q.i; o[6 * Np] = Z0.r; o[7 * Np] = Z0.i;
      o[8 * Np] = mb.r; o[9 * Np] = mb.i; o[10 * Np] = fq;
 }
+__global__ __launch_bounds__(64) void rev_orders(RevArgs p) { rev_orders_body(p); }
+// rev_orders over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct rev_orders_B {
+     RevArgs p;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(64) void rev_orders_b(const rev_orders_B *cases) {
+     const rev_orders_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     rev_orders_body(a.p);
+}
 
-__global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
+
+__device__ __forceinline__ void rev_carry_body(RevArgs p) {
      const int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= p.Np) return;
      const size_t Np = p.Np;
@@ -2234,6 +2514,19 @@ __global__ __launch_bounds__(64) void rev_carry(RevArgs p) {
      const double fser = p.sb0 * p.S[i] + p.sb1 * p.S[Np + i] + p.sb2 * p.S[2 * Np + i] + p.sb3 * p.S[3 * Np + i];
      p.fbar[i] = (p.Uf ? fq : p.fang[i]) + fser;
 }
+__global__ __launch_bounds__(64) void rev_carry(RevArgs p) { rev_carry_body(p); }
+// rev_carry over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct rev_carry_B {
+     RevArgs p;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(64) void rev_carry_b(const rev_carry_B *cases) {
+     const rev_carry_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     rev_carry_body(a.p);
+}
+
 
 // Sum of the per-block partials per (pass channel, element, angle); channel c
 // of the pass goes to channel map.g[c] of the output.
@@ -2244,8 +2537,7 @@ struct ChMap {
 // Eight lanes per output element, each summing every eighth block, then the
 // eight partials in a fixed order: deterministic, and eight times the
 // threads of a plain per-element loop.
-__global__ __launch_bounds__(256) void reduce_parts(const double *part, int nblk, int nchp, int Q, ChMap map,
-                                                    double *out) {
+__device__ __forceinline__ void reduce_parts_body(const double *part, int nblk, int nchp, int Q, ChMap map, double *out) {
      const int per = nchp * 4 * Q;
      const int e = blockIdx.x * 32 + (threadIdx.x >> 3), l = threadIdx.x & 7;
      double s = 0.;
@@ -2258,6 +2550,24 @@ __global__ __launch_bounds__(256) void reduce_parts(const double *part, int nblk
      const int c = e / (4 * Q), rem = e % (4 * Q);
      out[(size_t) map.g[c] * 4 * Q + rem] = s;
 }
+__global__ __launch_bounds__(256) void reduce_parts(const double *part, int nblk, int nchp, int Q, ChMap map, double *out) { reduce_parts_body(part, nblk, nchp, Q, map, out); }
+// reduce_parts over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct reduce_parts_B {
+     const double *part;
+     int nblk;
+     int nchp;
+     int Q;
+     ChMap map;
+     double *out;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(256) void reduce_parts_b(const reduce_parts_B *cases) {
+     const reduce_parts_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     reduce_parts_body(a.part, a.nblk, a.nchp, a.Q, a.map, a.out);
+}
+
 
 // ---------------------------------------------------------------------------
 // Expansion.  For channel c the quadrature-weighted element rows at the
@@ -2265,8 +2575,7 @@ __global__ __launch_bounds__(256) void reduce_parts(const double *part, int nblk
 //   GSF rows  0..5: F11, F11+F33, F11-F33, F33, F12, F34 at -mu_j
 //            6..11: the same at +mu_j            (misc/rtutil_scat_int.c:173-183)
 //   Legendre 0..5: F11, F11, F33, F33, F12, F34 at -mu_j, 6..11 at +mu_j.
-__global__ void expand_rows(const double *pf, int nch, int Q, int H, int Hp, const double *w, int gsf,
-                            double *E) {
+__device__ __forceinline__ void expand_rows_body(const double *pf, int nch, int Q, int H, int Hp, const double *w, int gsf, double *E) {
      const int j = blockIdx.x * blockDim.x + threadIdx.x;
      const int c = blockIdx.y;
      if (j >= Hp || c >= nch) return;
@@ -2297,12 +2606,31 @@ __global__ void expand_rows(const double *pf, int nch, int Q, int H, int Hp, con
           row[5 * Hp + j] = f34;
      }
 }
+__global__ void expand_rows(const double *pf, int nch, int Q, int H, int Hp, const double *w, int gsf, double *E) { expand_rows_body(pf, nch, Q, H, Hp, w, gsf, E); }
+// expand_rows over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct expand_rows_B {
+     const double *pf;
+     int nch;
+     int Q;
+     int H;
+     int Hp;
+     const double *w;
+     int gsf;
+     double *E;
+     unsigned gx, gy;
+};
+__global__ void expand_rows_b(const expand_rows_B *cases) {
+     const expand_rows_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     expand_rows_body(a.pf, a.nch, a.Q, a.H, a.Hp, a.w, a.gsf, a.E);
+}
+
 
 // One wave per (l, channel): twelve dot products over the positive nodes,
 // folded with (-1)^l for the mirrored half, then the alpha/zeta recombination
 // and the (2l+1)/2 factor (misc/rtutil_scat_int.c:185-213, :436-469).
-__global__ __launch_bounds__(256) void expand_coefs(const double *T, const double *E, int H, int Hp, int C,
-                                                    int gsf, double *coefs) {
+__device__ __forceinline__ void expand_coefs_body(const double *T, const double *E, int H, int Hp, int C, int gsf, double *coefs) {
      const int lane = threadIdx.x & 63;
      const int l = blockIdx.x * 4 + (threadIdx.x >> 6);
      const int c = blockIdx.y;
@@ -2353,6 +2681,25 @@ __global__ __launch_bounds__(256) void expand_coefs(const double *T, const doubl
           for (int k = 0; k < 6; ++k) out[(size_t) k * C + l] = e[k] * fl;
      }
 }
+__global__ __launch_bounds__(256) void expand_coefs(const double *T, const double *E, int H, int Hp, int C, int gsf, double *coefs) { expand_coefs_body(T, E, H, Hp, C, gsf, coefs); }
+// expand_coefs over the cases of a batch: blockIdx.z picks the case, whose grid may be
+// smaller than the launch grid (gx, gy blocks).
+struct expand_coefs_B {
+     const double *T;
+     const double *E;
+     int H;
+     int Hp;
+     int C;
+     int gsf;
+     double *coefs;
+     unsigned gx, gy;
+};
+__global__ __launch_bounds__(256) void expand_coefs_b(const expand_coefs_B *cases) {
+     const expand_coefs_B a = cases[blockIdx.z];
+     if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
+     expand_coefs_body(a.T, a.E, a.H, a.Hp, a.C, a.gsf, a.coefs);
+}
+
 
 // Scattering matrix at user angles from coefficients: one thread per
 // (set, angle) (misc/rtutil_scat_int.c:510-589, :662-710).
@@ -2441,6 +2788,20 @@ struct Buf {
      template <class T> T *as() const { return (T *) p; }
 };
 
+struct PinBuf {      // pinned host memory, grown on demand
+     void *p = nullptr;
+     size_t n = 0;
+     int get(size_t bytes) {
+          if (bytes <= n) return 0;
+          if (p) (void) hipHostFree(p);
+          p = nullptr;
+          n = 0;
+          if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) return set_err("hipHostMalloc(%zu bytes) failed", bytes);
+          n = bytes;
+          return 0;
+     }
+};
+
 struct Workspace {
      int device = -1;
      hipStream_t stream = nullptr;
@@ -2459,6 +2820,9 @@ struct Workspace {
      Buf chD, chPC, chPart, revScan;   // per-size recurrences of the value channel
      Buf ftab;                      // order_factors of the value channel
      Buf sw, s1, s2;                // save_control 2: channel weights, saved per-size arrays
+     Buf bargs, bup, bdown;         // per-case arguments and staging of a batch launched from here
+     PinBuf pup, pdown;
+     int home = -1, busy = 0;       // batch slot of this workspace (-1: general pool)
      int n2rows = 0, ng = 0;
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
@@ -2470,6 +2834,31 @@ struct Workspace {
 
 pthread_mutex_t g_pool_mu = PTHREAD_MUTEX_INITIALIZER;
 std::vector<Workspace *> g_pool;
+std::vector<Workspace *> g_home;     // workspaces of batch slots (Workspace::home)
+
+// Makes device d current on this thread (hipSetDevice only when it is not).
+int use_device(int d) {
+     int cur = -1;
+     if (hipGetDevice(&cur) != hipSuccess || cur != d) HIPCHECK(hipSetDevice(d));
+     return 0;
+}
+
+int ws_create(int device, Workspace **out) {
+     Workspace *w = new Workspace();
+     w->device = device;
+     HIPCHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+     HIPCHECK(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
+     HIPCHECK(hipStreamCreateWithFlags(&w->side2, hipStreamNonBlocking));
+     HIPCHECK(hipEventCreateWithFlags(&w->fork, hipEventDisableTiming));
+     HIPCHECK(hipEventCreateWithFlags(&w->join, hipEventDisableTiming));
+     HIPCHECK(hipEventCreateWithFlags(&w->join2, hipEventDisableTiming));
+     HIPCHECK(hipEventCreateWithFlags(&w->sums, hipEventDisableTiming));
+     HIPCHECK(hipEventCreateWithFlags(&w->nodes, hipEventDisableTiming));
+     HIPCHECK(hipEventCreateWithFlags(&w->join3, hipEventDisableTiming));
+     for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
+     *out = w;
+     return 0;
+}
 
 
 int ws_acquire(int device, Workspace **out) {
@@ -2482,20 +2871,32 @@ int ws_acquire(int device, Workspace **out) {
                break;
           }
      pthread_mutex_unlock(&g_pool_mu);
-     HIPCHECK(hipSetDevice(device));
+     if (use_device(device)) return -1;
+     if (!w && ws_create(device, &w)) return -1;
+     *out = w;
+     return 0;
+}
+
+// The workspace of slot `slot` of a batch on `device` (the current device):
+// the same one at every batch call while it is free, so a retrieval loop that
+// passes its cases in the same order finds buffers of the right sizes.
+int ws_acquire_home(int device, int slot, Workspace **out) {
+     pthread_mutex_lock(&g_pool_mu);
+     Workspace *w = nullptr;
+     for (Workspace *h : g_home)
+          if (h->device == device && h->home == slot && !h->busy) {
+               w = h;
+               break;
+          }
+     if (w) w->busy = 1;
+     pthread_mutex_unlock(&g_pool_mu);
      if (!w) {
-          w = new Workspace();
-          w->device = device;
-          HIPCHECK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
-          HIPCHECK(hipStreamCreateWithFlags(&w->side, hipStreamNonBlocking));
-          HIPCHECK(hipStreamCreateWithFlags(&w->side2, hipStreamNonBlocking));
-          HIPCHECK(hipEventCreateWithFlags(&w->fork, hipEventDisableTiming));
-          HIPCHECK(hipEventCreateWithFlags(&w->join, hipEventDisableTiming));
-          HIPCHECK(hipEventCreateWithFlags(&w->join2, hipEventDisableTiming));
-          HIPCHECK(hipEventCreateWithFlags(&w->sums, hipEventDisableTiming));
-          HIPCHECK(hipEventCreateWithFlags(&w->nodes, hipEventDisableTiming));
-          HIPCHECK(hipEventCreateWithFlags(&w->join3, hipEventDisableTiming));
-          for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
+          if (ws_create(device, &w)) return -1;
+          w->home = slot;
+          w->busy = 1;
+          pthread_mutex_lock(&g_pool_mu);
+          g_home.push_back(w);
+          pthread_mutex_unlock(&g_pool_mu);
      }
      *out = w;
      return 0;
@@ -2503,7 +2904,8 @@ int ws_acquire(int device, Workspace **out) {
 
 void ws_release(Workspace *w) {
      pthread_mutex_lock(&g_pool_mu);
-     g_pool.push_back(w);
+     if (w->home >= 0) w->busy = 0;
+     else g_pool.push_back(w);
      pthread_mutex_unlock(&g_pool_mu);
 }
 
@@ -3305,6 +3707,645 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Batches of independent cases in one launch per kernel.  The host thread
+// that runs case k of a batch (lmie_gpu_batch_enter) hands its forward and
+// later its reverse pass to the batch instead of running them; when every
+// case has arrived (or left), one thread runs the pass for all cases: each
+// case keeps its own workspace (buffers, reverse-pass state), and every kernel
+// is launched once, with blockIdx.z (or the task list) picking the case.
+// Fused: mixed precision, value channel only, angles, save_control 0, kept
+// state (the forward of a reverse pass); other cases run one by one.
+
+// Many small copies as one: each job moves n doubles (src NULL: writes zeros).
+struct CopyJob {
+     const double *src;
+     double *dst;
+     size_t n;
+};
+__global__ __launch_bounds__(256) void copy_jobs(const CopyJob *jobs) {
+     const CopyJob j = jobs[blockIdx.y];
+     for (size_t e = (size_t) blockIdx.x * 256 + threadIdx.x; e < j.n; e += (size_t) gridDim.x * 256)
+          j.dst[e] = j.src ? j.src[e] : 0.;
+}
+
+// Uploads of a batch gathered into one host-to-device copy and one copy_jobs
+// launch.  Sizes are whole doubles.
+struct Upload {
+     struct Item { size_t off; void *dst; size_t n8; bool zero; };
+     std::vector<unsigned char> h;
+     std::vector<Item> items;
+     int add(void *dst, const void *src, size_t bytes) {
+          if (bytes % 8) return set_err("internal: upload of %zu bytes", bytes);
+          const size_t off = h.size();
+          h.resize(off + bytes);
+          memcpy(h.data() + off, src, bytes);
+          items.push_back(Item{off, dst, bytes / 8, false});
+          return 0;
+     }
+     int zero(void *dst, size_t bytes) {
+          if (bytes % 8) return set_err("internal: fill of %zu bytes", bytes);
+          items.push_back(Item{0, dst, bytes / 8, true});
+          return 0;
+     }
+     int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
+          if (items.empty()) return 0;
+          const size_t data = (h.size() + 255) & ~(size_t) 255, total = data + sizeof(CopyJob) * items.size();
+          if (dev.get(total) || pin.get(total)) return -1;
+          unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
+          memcpy(hp, h.data(), h.size());
+          CopyJob *jobs = (CopyJob *) (hp + data);
+          for (size_t k = 0; k < items.size(); ++k)
+               jobs[k] = CopyJob{items[k].zero ? nullptr : (const double *) (base + items[k].off),
+                                 (double *) items[k].dst, items[k].n8};
+          HIPCHECK(hipMemcpyAsync(base, hp, total, hipMemcpyHostToDevice, s));
+          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) items.size()), dim3(256), 0, s,
+                             (const CopyJob *) (base + data));
+          HIPCHECK(hipGetLastError());
+          return 0;
+     }
+};
+
+// Results of a batch gathered by one copy_jobs launch into one device-to-host
+// copy; flush waits for the stream and scatters into the callers' arrays.
+struct Download {
+     struct Item { const void *src; void *dst; size_t off, n8; };
+     std::vector<Item> items;
+     size_t total = 0;
+     int add(void *dst, const void *src, size_t bytes) {
+          if (bytes % 8) return set_err("internal: download of %zu bytes", bytes);
+          items.push_back(Item{src, dst, total, bytes / 8});
+          total += bytes;
+          return 0;
+     }
+     int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
+          if (items.empty()) return 0;
+          const size_t data = (total + 255) & ~(size_t) 255, jb = sizeof(CopyJob) * items.size();
+          if (dev.get(data + jb) || pin.get(data + jb)) return -1;
+          unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
+          CopyJob *jobs = (CopyJob *) (hp + data);
+          for (size_t k = 0; k < items.size(); ++k)
+               jobs[k] = CopyJob{(const double *) items[k].src, (double *) (base + items[k].off), items[k].n8};
+          HIPCHECK(hipMemcpyAsync(base + data, hp + data, jb, hipMemcpyHostToDevice, s));
+          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) items.size()), dim3(256), 0, s,
+                             (const CopyJob *) (base + data));
+          HIPCHECK(hipGetLastError());
+          HIPCHECK(hipMemcpyAsync(hp, base, total, hipMemcpyDeviceToHost, s));
+          HIPCHECK(hipStreamSynchronize(s));
+          for (const Item &it : items) memcpy(it.dst, hp + it.off, 8 * it.n8);
+          return 0;
+     }
+};
+
+struct BatchBlob {   // host staging of per-case argument arrays, one upload
+     std::vector<unsigned char> h;
+     template <class T> size_t add(const std::vector<T> &v) {
+          const size_t off = (h.size() + 255) & ~(size_t) 255;
+          h.resize(off + sizeof(T) * (v.empty() ? 1 : v.size()));
+          if (!v.empty()) memcpy(h.data() + off, v.data(), sizeof(T) * v.size());
+          return off;
+     }
+};
+
+template <class T> unsigned max_g(const std::vector<T> &v, unsigned T::*g) {
+     unsigned m = 0;
+     for (const T &x : v) m = std::max(m, x.*g);
+     return m;
+}
+
+struct FwdCase {
+     int slot;
+     const lmie_gpu_job *J;
+     lmie_gpu_result *R;
+     Workspace *w;
+     int N, Np, n1max, Q, C, H, Hp, Kp, Kb, Hb, nt64, n2rows, ng, planes;
+     std::vector<int2> task;
+};
+
+int fused_forward_ok(const lmie_gpu_job *J) {
+     return (J->precision == LMIE_PRECISION_MIXED || J->precision == LMIE_PRECISION_MIXED_BF16) && J->n_tan == 0 &&
+            J->n_wgt == 0 && J->save_control == 0 && J->keep && J->n_qang > 0 && J->n_qang % 2 == 0 &&
+            (J->calc_gc || J->calc_lc);
+}
+
+// Buffers and uploads of one case (the mixed value-channel branch of
+// forward_impl); the uploads are queued on up, no kernels.
+int fused_prepare(FwdCase &c, Upload &up) {
+     const lmie_gpu_job *J = c.J;
+     Workspace *w = c.w;
+     const int N = J->n_size, Np = rup(N, 2 * TILE), n1max = J->n1_max, Q = J->n_qang, C = J->n_coef;
+     const int H = Q / 2, Hp = rup(H, TILE), Kp = rup(n1max, 32), Kb = rup(n1max, KM) / 8, Hb = rup(H, 128);
+     if (N <= 0) return set_err("no size points");
+     for (int i = 0; i < N; ++i)
+          if (J->n1[i] > n1max || J->n1[i] < 1 || J->n2[i] < J->n1[i])
+               return set_err("size point %d: order n1 = %d outside [1, %d] (n2 = %d)", i, J->n1[i], n1max, J->n2[i]);
+     w->valid = 0;
+     const int nt32 = Np / 32, nt64 = Np / 64;
+     std::vector<int> n1h(Np, 0), n2h(Np, 0), kt32(nt32), kt64(nt64), kfill(nt64), kt128(nt64 / 2);
+     for (int i = 0; i < N; ++i) { n1h[i] = J->n1[i]; n2h[i] = J->n2[i]; }
+     for (int t = 0; t < nt32; ++t) {
+          int m = 0;
+          for (int i = t * 32; i < t * 32 + 32; ++i) m = std::max(m, n1h[i]);
+          kt32[t] = rup(m, KM);
+     }
+     for (int t = 0; t < nt64; ++t) kt64[t] = std::max(kt32[2 * t], kt32[2 * t + 1]);
+     for (int t = 0; t < nt64 / 2; ++t) kt128[t] = std::max(kt64[2 * t], kt64[2 * t + 1]);
+     for (int t = 0; t < nt64; ++t) kfill[t] = kt128[t / 2];
+     int n2max = 0, kfmax = 0;
+     for (int i = 0; i < N; ++i) n2max = std::max(n2max, J->n2[i]);
+     for (int t = 0; t < nt64; ++t) kfmax = std::max(kfmax, kfill[t]);
+     const int n2rows = n2max + 1, ng = kfmax / 8, planes = J->calc_lc ? 5 : 4;
+     if (w->x.get(sizeof(double) * Np) || w->f.get(sizeof(double) * Np) || w->n1.get(sizeof(int) * Np) ||
+         w->n2.get(sizeof(int) * Np) || w->kfill.get(sizeof(int) * nt64) || w->ktile32.get(sizeof(int) * nt32) ||
+         w->ktile64.get(sizeof(int) * nt64) || w->sx.get(sizeof(double) * Np) || w->cx.get(sizeof(double) * Np) ||
+         w->guess.get(sizeof(double) * H) || w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) ||
+         w->T.get(sizeof(double) * planes * (size_t) C * Hp) ||
+         w->B16.get(sizeof(unsigned short) * 4 * (size_t) Kb * Hb * 8) || w->S.get(sizeof(double) * 4 * Np) ||
+         w->A16.get(sizeof(unsigned short) * 8 * (size_t) Kb * Np * 8) || w->CB.get(sizeof(double) * 2 * Np) ||
+         w->chD.get(sizeof(double) * 2 * (size_t) n2rows * Np) ||
+         w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) ||
+         w->chPart.get(sizeof(double) * 5 * (size_t) (ng > 0 ? ng : 1) * Np) ||
+         w->ftab.get(sizeof(double) * FT_ROWS * (size_t) (n1max + 2)) || w->dotjobs.get(sizeof(DotJob) * 4) ||
+         w->dots.get(sizeof(double) * 4) || w->pf.get(sizeof(double) * 4 * (size_t) Q) ||
+         w->part.get(sizeof(double) * (size_t) nt64 * 4 * Q) || w->E.get(sizeof(double) * 12 * (size_t) Hp) ||
+         w->coefs.get(sizeof(double) * 12 * (size_t) C))
+          return -1;
+     // x, f zero-padded to Np; sin x, cos x from the host C library (padding 0, 1)
+     w->hsc.assign(4 * (size_t) Np, 0.);
+     double *hx = w->hsc.data(), *hf = hx + Np, *hs = hf + Np, *hc = hs + Np;
+     for (int i = 0; i < Np; ++i) {
+          hx[i] = i < N ? J->x[i] : 0.;
+          hf[i] = i < N ? J->f[i] : 0.;
+          hs[i] = i < N ? (J->sx ? J->sx[i] : sin(J->x[i])) : 0.;
+          hc[i] = i < N ? (J->cx ? J->cx[i] : cos(J->x[i])) : 1.;
+     }
+     w->hguess.resize(H);
+     gauss_guesses(Q, w->hguess.data());
+     DotJob jobs[4];
+     for (int q = 0; q < 4; ++q) jobs[q] = DotJob{w->f.as<double>(), w->S.as<double>() + (size_t) q * Np, nullptr, nullptr};
+     if (up.add(w->x.p, hx, sizeof(double) * Np) || up.add(w->f.p, hf, sizeof(double) * Np) ||
+         up.add(w->sx.p, hs, sizeof(double) * Np) || up.add(w->cx.p, hc, sizeof(double) * Np) ||
+         up.add(w->n1.p, n1h.data(), sizeof(int) * Np) || up.add(w->n2.p, n2h.data(), sizeof(int) * Np) ||
+         up.add(w->kfill.p, kfill.data(), sizeof(int) * nt64) || up.add(w->ktile32.p, kt32.data(), sizeof(int) * nt32) ||
+         up.add(w->ktile64.p, kt64.data(), sizeof(int) * nt64) ||
+         up.add(w->guess.p, w->hguess.data(), sizeof(double) * H) || up.add(w->dotjobs.p, jobs, sizeof(jobs)))
+          return -1;
+     xcd_tasks(kt64, [&](int) { return Hb / 128; }, tile_group(), c.task);
+     c.N = N; c.Np = Np; c.n1max = n1max; c.Q = Q; c.C = C; c.H = H; c.Hp = Hp; c.Kp = Kp; c.Kb = Kb; c.Hb = Hb;
+     c.nt64 = nt64; c.n2rows = n2rows; c.ng = ng; c.planes = planes;
+     w->N = N; w->Np = Np; w->Kp = Kp; w->Hp = Hp; w->H = H; w->Q = Q; w->C = C; w->n1max = n1max;
+     w->n2max = n2max; w->calc_gc = J->calc_gc; w->calc_lc = J->calc_lc; w->mr = J->mr; w->mi = J->mi;
+     w->mixed = 1; w->bf16 = J->precision == LMIE_PRECISION_MIXED_BF16; w->Kb = Kb; w->Hb = Hb;
+     w->kt64 = kt64; w->kt128 = kt128; w->n2rows = n2rows; w->ng = ng;
+     return 0;
+}
+
+// One launch per kernel over the prepared cases, on the streams of workspace L.
+template <class E>
+int fused_launch(std::vector<FwdCase> &cs, Upload &up, Workspace *L) {
+     hipStream_t s = L->stream, ts = L->side, t2 = L->side2;
+     if (up.flush(L->bup, L->pup, s)) return -1;
+     const unsigned n = (unsigned) cs.size();
+     std::vector<mie_chains_B> chains(n);
+     std::vector<gauss_nodes_B> gauss(n);
+     std::vector<gsf_table_B> gsf(n);
+     std::vector<pitau_table16_B> pitau(n);
+     std::vector<order_factors_B> fac(n);
+     std::vector<mie_terms_B> terms(n);
+     std::vector<mie_sums_B> sums(n);
+     std::vector<weighted_dots_B> dots(n);
+     std::vector<Mix2Args> mix(n);
+     std::vector<reduce_parts_B> red(n);
+     std::vector<expand_rows_B> rows[2];
+     std::vector<expand_coefs_B> coefs[2];
+     std::vector<int2> task;
+     for (unsigned k = 0; k < n; ++k) {
+          const FwdCase &c = cs[k];
+          Workspace *w = c.w;
+          ChainArgs ca;
+          ca.N = c.N; ca.Np = c.Np; ca.n1max = c.n1max; ca.n2rows = c.n2rows;
+          ca.x = w->x.as<double>(); ca.sx = w->sx.as<double>(); ca.cx = w->cx.as<double>();
+          ca.n1 = w->n1.as<int>(); ca.n2 = w->n2.as<int>(); ca.mr = c.J->mr; ca.mi = c.J->mi;
+          ca.d_all = 1; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
+          chains[k] = mie_chains_B{ca, (unsigned) c.Np / 64, 2};
+          gauss[k] = gauss_nodes_B{c.Q, w->guess.as<double>(), w->mu.as<double>(), w->w.as<double>(),
+                                   (unsigned) (c.H + 63) / 64, 1};
+          gsf[k] = gsf_table_B{c.H, c.Hp, c.C, w->mu.as<double>(), w->T.as<double>(), (unsigned) c.Hp / 64,
+                               (unsigned) c.planes};
+          pitau[k] = pitau_table16_B{c.H, c.Hb, c.n1max, c.Kb, w->mu.as<double>(), w->B16.as<unsigned short>(),
+                                     (unsigned) c.Hb / 64, 1};
+          fac[k] = order_factors_B{c.n1max + 1, w->ftab.as<double>(), (unsigned) (c.n1max + 2 + 255) / 256, 1};
+          TermArgs ta;
+          ta.N = c.N; ta.Np = c.Np; ta.n1max = c.n1max; ta.n2rows = c.n2rows; ta.Kp = c.Kp; ta.Kb = c.Kb; ta.ng = c.ng;
+          ta.ft = w->ftab.as<double>(); ta.x = w->x.as<double>(); ta.n1 = w->n1.as<int>();
+          ta.kfill = w->kfill.as<int>(); ta.mr = c.J->mr; ta.mi = c.J->mi; ta.D = w->chD.as<double>();
+          ta.PC = w->chPC.as<double>(); ta.A = nullptr; ta.A16 = w->A16.as<unsigned short>();
+          ta.part = w->chPart.as<double>();
+          terms[k] = mie_terms_B{ta, (unsigned) c.Np / 64, (unsigned) c.ng};
+          sums[k] = mie_sums_B{w->chPart.as<double>(), c.N, c.Np, c.ng, w->n1.as<int>(), w->S.as<double>(),
+                               w->CB.as<double>(), (unsigned) (c.Np + 31) / 32, 1};
+          dots[k] = weighted_dots_B{w->dotjobs.as<DotJob>(), c.Np, w->dots.as<double>(), 4, 1};
+          Mix2Args &m = mix[k];
+          memset(&m, 0, sizeof m);
+          m.Np = c.Np; m.Kb = c.Kb; m.Hb = c.Hb; m.H = c.H; m.Q = c.Q;
+          m.A16 = w->A16.as<unsigned short>(); m.B16 = w->B16.as<unsigned short>();
+          m.ktile = w->ktile64.as<int>(); m.task = nullptr; m.W[0] = w->f.as<double>();
+          m.part = w->part.as<double>(); m.sst = nullptr;
+          for (const int2 &t : c.task)
+               task.push_back(make_int2(t.x, ((t.x < 0 ? 0 : t.y) & 0xffff) | (int) (k << 16)));
+          ChMap map;
+          memset(&map, 0, sizeof map);
+          map.g[0] = 0;
+          red[k] = reduce_parts_B{w->part.as<double>(), c.nt64, 1, c.Q, map, w->pf.as<double>(),
+                                  (unsigned) (4 * c.Q + 31) / 32, 1};
+          for (int gs = 0; gs < 2; ++gs) {
+               const unsigned on = (gs ? c.J->calc_gc : c.J->calc_lc) ? 1u : 0u;
+               rows[gs].push_back(expand_rows_B{w->pf.as<double>(), 1, c.Q, c.H, c.Hp, w->w.as<double>(), gs,
+                                                w->E.as<double>(), (unsigned) c.Hp / 64, on});
+               // generalized spherical into coefs[0 .. 6C), Legendre into [6C .. 12C)
+               coefs[gs].push_back(expand_coefs_B{w->T.as<double>(), w->E.as<double>(), c.H, c.Hp, c.C, gs,
+                                                  w->coefs.as<double>() + (gs ? 0 : 6 * (size_t) c.C),
+                                                  (unsigned) (c.C + 3) / 4, on});
+          }
+     }
+     BatchBlob b;
+     const size_t o_chains = b.add(chains), o_gauss = b.add(gauss), o_gsf = b.add(gsf), o_pitau = b.add(pitau),
+                  o_fac = b.add(fac), o_terms = b.add(terms), o_sums = b.add(sums), o_dots = b.add(dots),
+                  o_mix = b.add(mix), o_task = b.add(task), o_red = b.add(red), o_rows0 = b.add(rows[0]),
+                  o_rows1 = b.add(rows[1]), o_coefs0 = b.add(coefs[0]), o_coefs1 = b.add(coefs[1]);
+     if (L->bargs.get(b.h.size())) return -1;
+     unsigned char *d = L->bargs.as<unsigned char>();
+     HIPCHECK(hipMemcpyAsync(d, b.h.data(), b.h.size(), hipMemcpyHostToDevice, s));
+#define LMIE_B(T, off) ((const T *) (d + (off)))
+     // main stream: per-size recurrences; side streams: the angle tables
+     HIPCHECK(hipEventRecord(L->fork, s));
+     HIPCHECK(hipStreamWaitEvent(ts, L->fork, 0));
+     hipLaunchKernelGGL(mie_chains_b, dim3(max_g(chains, &mie_chains_B::gx), 2, n), dim3(64), 0, s,
+                        LMIE_B(mie_chains_B, o_chains));
+     hipLaunchKernelGGL(gauss_nodes_b, dim3(max_g(gauss, &gauss_nodes_B::gx), 1, n), dim3(64), 0, ts,
+                        LMIE_B(gauss_nodes_B, o_gauss));
+     HIPCHECK(hipEventRecord(L->nodes, ts));
+     HIPCHECK(hipStreamWaitEvent(t2, L->nodes, 0));
+     hipLaunchKernelGGL(gsf_table_b, dim3(max_g(gsf, &gsf_table_B::gx), max_g(gsf, &gsf_table_B::gy), n), dim3(64), 0,
+                        t2, LMIE_B(gsf_table_B, o_gsf));
+     HIPCHECK(hipEventRecord(L->join3, t2));
+     hipLaunchKernelGGL(pitau_table16_b<E>, dim3(max_g(pitau, &pitau_table16_B::gx), 1, n), dim3(64), 0, ts,
+                        LMIE_B(pitau_table16_B, o_pitau));
+     HIPCHECK(hipEventRecord(L->join, ts));
+     hipLaunchKernelGGL(order_factors_b, dim3(max_g(fac, &order_factors_B::gx), 1, n), dim3(256), 0, s,
+                        LMIE_B(order_factors_B, o_fac));
+     const unsigned tgy = max_g(terms, &mie_terms_B::gy);
+     if (tgy > 0)
+          hipLaunchKernelGGL(mie_terms_b<E>, dim3(max_g(terms, &mie_terms_B::gx), tgy, n), dim3(64), 0, s,
+                             LMIE_B(mie_terms_B, o_terms));
+     hipLaunchKernelGGL(mie_sums_b, dim3(max_g(sums, &mie_sums_B::gx), 1, n), dim3(256), 0, s,
+                        LMIE_B(mie_sums_B, o_sums));
+     hipLaunchKernelGGL(weighted_dots_b, dim3(4, 1, n), dim3(256), 0, s, LMIE_B(weighted_dots_B, o_dots));
+     // contraction after the split tables, one task list for all cases
+     HIPCHECK(hipStreamWaitEvent(s, L->join, 0));
+     if (!task.empty())
+          hipLaunchKernelGGL((angular_mix2_b<E, 2, 4, 1>), dim3((unsigned) task.size()), dim3(512), 0, s,
+                             LMIE_B(Mix2Args, o_mix), LMIE_B(int2, o_task));
+     hipLaunchKernelGGL(reduce_parts_b, dim3(max_g(red, &reduce_parts_B::gx), 1, n), dim3(256), 0, s,
+                        LMIE_B(reduce_parts_B, o_red));
+     // expansion after the expansion tables: generalized spherical, then Legendre
+     HIPCHECK(hipStreamWaitEvent(s, L->join3, 0));
+     for (int gs = 1; gs >= 0; --gs) {
+          if (max_g(rows[gs], &expand_rows_B::gy) == 0) continue;
+          hipLaunchKernelGGL(expand_rows_b, dim3(max_g(rows[gs], &expand_rows_B::gx), 1, n), dim3(64), 0, s,
+                             LMIE_B(expand_rows_B, gs ? o_rows1 : o_rows0));
+          hipLaunchKernelGGL(expand_coefs_b, dim3(max_g(coefs[gs], &expand_coefs_B::gx), 1, n), dim3(256), 0, s,
+                             LMIE_B(expand_coefs_B, gs ? o_coefs1 : o_coefs0));
+          HIPCHECK(hipGetLastError());
+     }
+#undef LMIE_B
+     HIPCHECK(hipGetLastError());
+     std::vector<double> hd(4 * (size_t) n);
+     Download dn;
+     for (unsigned k = 0; k < n; ++k) {
+          const FwdCase &c = cs[k];
+          const size_t cb = sizeof(double) * 6 * c.C;
+          if ((c.J->calc_gc && c.R->gc_raw && dn.add(c.R->gc_raw, c.w->coefs.p, cb)) ||
+              (c.J->calc_lc && c.R->lc_raw && dn.add(c.R->lc_raw, c.w->coefs.as<double>() + 6 * (size_t) c.C, cb)) ||
+              dn.add(hd.data() + 4 * k, c.w->dots.p, sizeof(double) * 4))
+               return -1;
+     }
+     if (dn.flush(L->bdown, L->pdown, s)) return -1;
+     for (unsigned k = 0; k < n; ++k) {
+          for (int q = 0; q < 4; ++q) cs[k].R->sums[q] = hd[4 * k + q];
+          cs[k].w->valid = 1;
+     }
+     return 0;
+}
+
+struct RevCase {
+     int slot;
+     Workspace *w;
+     const lmie_gpu_adj_in *in;
+     lmie_gpu_adj_out *out;
+     int nck, Kq, Hq, nch, use_gc, use_lc;
+     std::vector<int2> task;
+};
+
+int fused_reverse_ok(Workspace *w, const lmie_gpu_adj_in *in) {
+     return w->valid && w->mixed && w->Q > 0 && ((in->gc_bar && w->calc_gc) || (in->lc_bar && w->calc_lc));
+}
+
+// Buffers and uploads of one reverse pass (the mixed branch of adjoint_impl),
+// queued on up.
+int fused_rev_prepare(RevCase &c, Upload &up) {
+     Workspace *w = c.w;
+     const int Np = w->Np, Hp = w->Hp, H = w->H, Q = w->Q, C = w->C, n1max = w->n1max;
+     const int Kq = rup(n1max, 64), Hq = rup(H, KM);
+     if (rup(n1max, KM) > w->Kb * 8) return set_err("internal: split operand depth");
+     c.use_gc = c.in->gc_bar && w->calc_gc;
+     c.use_lc = c.in->lc_bar && w->calc_lc;
+     c.nck = (C + EXA_L - 1) / EXA_L;
+     c.Kq = Kq; c.Hq = Hq;
+     c.nch = (w->n2rows - 1 + RCH - 1) / RCH;
+     const size_t nk = w->kt128.size(), nk2 = nk + (nk & 1);   // whole doubles
+     if (w->pfbar.get(sizeof(double) * 4 * (size_t) Q) || w->fang.get(sizeof(double) * Np) ||
+         w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * Np) ||
+         w->gbar.get(sizeof(double) * 12 * (size_t) C) ||
+         w->expart.get(sizeof(double) * 24 * (size_t) c.nck * Hp) || w->wts.get(sizeof(double) * 8 * (size_t) Hq) ||
+         w->scal.get(sizeof(AdjScal)) || w->L16.get(sizeof(unsigned short) * 4 * (size_t) Hq * Kq) ||
+         w->R16.get(sizeof(unsigned short) * 16 * (size_t) Hq * Kq) ||
+         w->Gf.get(sizeof(float) * 4 * (size_t) Kq * Kq) ||
+         w->G16.get(sizeof(unsigned short) * 12 * (size_t) Kq * Kq) ||
+         w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np) ||
+         w->revScan.get(sizeof(double) * 11 * (size_t) (c.nch > 0 ? c.nch : 1) * Np) ||
+         w->ktile128.get(sizeof(int) * nk2) || w->dotjobs.get(sizeof(DotJob) * 2) || w->dots.get(sizeof(double) * 2))
+          return -1;
+     double *g = w->gbar.as<double>();
+     std::vector<int> kt(w->kt128);
+     kt.resize(nk2, 0);
+     const std::vector<int> &kts = w->kt128;
+     xcd_tasks(kts, [&](int t) { return (kts[t] + 63) / 64; }, tile_group(), c.task);
+     DotJob jobs[2] = {{nullptr, w->mbar.as<double>(), nullptr, nullptr},
+                       {nullptr, w->mbar.as<double>() + Np, nullptr, nullptr}};
+     if ((c.use_gc && up.add(g, c.in->gc_bar, sizeof(double) * 6 * C)) ||
+         (c.use_lc && up.add(g + 6 * C, c.in->lc_bar, sizeof(double) * 6 * C)) ||
+         up.zero(w->fang.p, sizeof(double) * Np) || up.add(w->ktile128.p, kt.data(), sizeof(int) * nk2) ||
+         (c.task.empty() && up.zero(w->Uf.p, sizeof(float) * 4 * (size_t) Kq * Np)) ||
+         up.add(w->dotjobs.p, jobs, sizeof(jobs)))
+          return -1;
+     return 0;
+}
+
+template <class E>
+int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
+     hipStream_t s = L->stream;
+     if (up.flush(L->bup, L->pup, s)) return -1;
+     const unsigned n = (unsigned) cs.size();
+     std::vector<expand_adjoint_part_B> epart(n);
+     std::vector<expand_adjoint_fin_B> efin(n);
+     std::vector<adj_weights_B> wts(n);
+     std::vector<adj_tables_B> tabs(n);
+     std::vector<adj_gbuild_B> gb(n);
+     std::vector<adj_gsplit_B> gsp(n);
+     std::vector<UArgs> ua(n);
+     std::vector<int2> task;
+     std::vector<rev_orders_B> ro(n);
+     std::vector<rev_carry_B> rc(n);
+     std::vector<weighted_dots_B> dots(n);
+     for (unsigned k = 0; k < n; ++k) {
+          const RevCase &c = cs[k];
+          Workspace *w = c.w;
+          const int H = w->H, Hp = w->Hp, Q = w->Q, C = w->C, Np = w->Np;
+          const double *g = w->gbar.as<double>();
+          AdjScal *sc = w->scal.as<AdjScal>();
+          epart[k] = expand_adjoint_part_B{w->T.as<double>(), c.use_gc ? g : nullptr, c.use_lc ? g + 6 * C : nullptr,
+                                           H, Hp, C, w->expart.as<double>(), (unsigned) Hp / 64, (unsigned) c.nck};
+          efin[k] = expand_adjoint_fin_B{w->expart.as<double>(), c.nck, w->w.as<double>(), H, Hp, Q,
+                                         w->pfbar.as<double>(), (unsigned) (H + 63) / 64, 1};
+          wts[k] = adj_weights_B{w->pfbar.as<double>(), H, c.Hq, Q, w->wts.as<double>(), sc, 1, 1};
+          tabs[k] = adj_tables_B{w->B16.as<unsigned short>(), w->Kb, w->Hb, w->wts.as<double>(), H, c.Hq, c.Kq,
+                                 w->n1max, w->L16.as<unsigned short>(), w->R16.as<unsigned short>(),
+                                 (unsigned) c.Kq / 64, (unsigned) c.Hq / 8};
+          gb[k] = adj_gbuild_B{w->L16.as<unsigned short>(), w->R16.as<unsigned short>(), c.Hq, c.Kq,
+                               w->Gf.as<float>(), sc, (unsigned) c.Kq / 64, (unsigned) c.Kq / 64};
+          gsp[k] = adj_gsplit_B{w->Gf.as<float>(), sc, c.Kq, w->G16.as<unsigned short>(), (unsigned) c.Kq / 64,
+                                (unsigned) c.Kq / 8};
+          UArgs &u = ua[k];
+          u.Np = Np; u.Kb = w->Kb; u.Kq = c.Kq;
+          u.A16 = w->A16.as<unsigned short>(); u.G16 = w->G16.as<unsigned short>();
+          u.ktile = w->ktile128.as<int>(); u.task = nullptr; u.sc = sc; u.Uf = w->Uf.as<float>();
+          for (const int2 &t : c.task)
+               task.push_back(make_int2(t.x, ((t.x < 0 ? 0 : t.y) & 0xffff) | (int) (k << 16)));
+          RevArgs r;
+          r.N = w->N; r.Np = Np; r.Kp = w->Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.nch = c.nch;
+          r.ft = w->ftab.as<double>(); r.x = w->x.as<double>(); r.f = w->f.as<double>();
+          r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
+          r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>();
+          r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = nullptr;   // read only without Uf
+          r.Uf = w->Uf.as<float>(); r.Kq = c.Kq; r.fang = w->fang.as<double>();
+          r.sb0 = c.in->sums_bar[0]; r.sb1 = c.in->sums_bar[1]; r.sb2 = c.in->sums_bar[2]; r.sb3 = c.in->sums_bar[3];
+          r.scan = w->revScan.as<double>(); r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
+          ro[k] = rev_orders_B{r, (unsigned) Np / 64, (unsigned) c.nch};
+          rc[k] = rev_carry_B{r, (unsigned) Np / 64, 1};
+          dots[k] = weighted_dots_B{w->dotjobs.as<DotJob>(), Np, w->dots.as<double>(), 2, 1};
+     }
+     BatchBlob b;
+     const size_t o_ep = b.add(epart), o_ef = b.add(efin), o_w = b.add(wts), o_t = b.add(tabs), o_gb = b.add(gb),
+                  o_gs = b.add(gsp), o_u = b.add(ua), o_task = b.add(task), o_ro = b.add(ro), o_rc = b.add(rc),
+                  o_d = b.add(dots);
+     if (L->bargs.get(b.h.size())) return -1;
+     unsigned char *d = L->bargs.as<unsigned char>();
+     HIPCHECK(hipMemcpyAsync(d, b.h.data(), b.h.size(), hipMemcpyHostToDevice, s));
+#define LMIE_B(T, off) ((const T *) (d + (off)))
+     hipLaunchKernelGGL(expand_adjoint_part_b, dim3(max_g(epart, &expand_adjoint_part_B::gx),
+                                                    max_g(epart, &expand_adjoint_part_B::gy), n),
+                        dim3(64), 0, s, LMIE_B(expand_adjoint_part_B, o_ep));
+     hipLaunchKernelGGL(expand_adjoint_fin_b, dim3(max_g(efin, &expand_adjoint_fin_B::gx), 1, n), dim3(64), 0, s,
+                        LMIE_B(expand_adjoint_fin_B, o_ef));
+     hipLaunchKernelGGL(adj_weights_b, dim3(1, 1, n), dim3(1024), 0, s, LMIE_B(adj_weights_B, o_w));
+     hipLaunchKernelGGL(adj_tables_b<E>, dim3(max_g(tabs, &adj_tables_B::gx), max_g(tabs, &adj_tables_B::gy), n),
+                        dim3(64), 0, s, LMIE_B(adj_tables_B, o_t));
+     hipLaunchKernelGGL(adj_gbuild_b<E>, dim3(max_g(gb, &adj_gbuild_B::gx), max_g(gb, &adj_gbuild_B::gy), n),
+                        dim3(256), 0, s, LMIE_B(adj_gbuild_B, o_gb));
+     hipLaunchKernelGGL(adj_gsplit_b<E>, dim3(max_g(gsp, &adj_gsplit_B::gx), max_g(gsp, &adj_gsplit_B::gy), n),
+                        dim3(64), 0, s, LMIE_B(adj_gsplit_B, o_gs));
+     if (!task.empty())
+          hipLaunchKernelGGL((adj_ugemm32_b<E, 1, 128>), dim3((unsigned) task.size()), dim3(512), 0, s,
+                             LMIE_B(UArgs, o_u), LMIE_B(int2, o_task));
+     const unsigned rgy = max_g(ro, &rev_orders_B::gy);
+     if (rgy > 0)
+          hipLaunchKernelGGL(rev_orders_b, dim3(max_g(ro, &rev_orders_B::gx), rgy, n), dim3(64), 0, s,
+                             LMIE_B(rev_orders_B, o_ro));
+     hipLaunchKernelGGL(rev_carry_b, dim3(max_g(rc, &rev_carry_B::gx), 1, n), dim3(64), 0, s,
+                        LMIE_B(rev_carry_B, o_rc));
+     hipLaunchKernelGGL(weighted_dots_b, dim3(2, 1, n), dim3(256), 0, s, LMIE_B(weighted_dots_B, o_d));
+#undef LMIE_B
+     HIPCHECK(hipGetLastError());
+     std::vector<double> mb(2 * (size_t) n);
+     Download dn;
+     for (unsigned k = 0; k < n; ++k)
+          if (dn.add(mb.data() + 2 * k, cs[k].w->dots.p, sizeof(double) * 2) ||
+              dn.add(cs[k].out->f_bar, cs[k].w->fbar.p, sizeof(double) * cs[k].w->N))
+               return -1;
+     if (dn.flush(L->bdown, L->pdown, s)) return -1;
+     for (unsigned k = 0; k < n; ++k) {
+          // m = mr - i mi: dL/dmr = Re(mbar), dL/dmi = -Im(mbar)
+          cs[k].out->m_bar[0] = mb[2 * k];
+          cs[k].out->m_bar[1] = -mb[2 * k + 1];
+     }
+     return 0;
+}
+
+// The collective: cases register their pass; the last to arrive (or to
+// leave without arriving) runs it for all.
+struct Batch {
+     int n = 0, device = 0;
+     pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+     pthread_cond_t cv = PTHREAD_COND_INITIALIZER;
+     int arrived[2] = {0, 0}, left[2] = {0, 0}, state[2] = {0, 0};   // state: 0 open, 1 running, 2 done
+     std::vector<char> in[2];
+     std::vector<const lmie_gpu_job *> job;
+     std::vector<lmie_gpu_result *> res;
+     std::vector<Workspace *> ws;
+     std::vector<const lmie_gpu_adj_in *> ain;
+     std::vector<lmie_gpu_adj_out *> aout;
+     std::vector<int> rc[2];
+     std::vector<std::string> err[2];
+     int fused_cases[2] = {0, 0};
+};
+
+thread_local Batch *t_batch = nullptr;
+thread_local int t_slot = -1, t_phase = 0;
+
+int forward_one(const lmie_gpu_job *job, lmie_gpu_result *res);
+int adjoint_one(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out);
+
+void batch_run_forward(Batch *b) {
+     std::vector<FwdCase> f16, b16;
+     Upload up16, upb;
+     Workspace *L = nullptr;           // the workspace whose streams launch the batch
+     for (int k = 0; k < b->n; ++k) {
+          if (!b->in[0][k]) continue;
+          const lmie_gpu_job *J = b->job[k];
+          g_err[0] = 0;
+          int rc;
+          if (!fused_forward_ok(J) || J->device != b->device) {
+               rc = forward_one(J, b->res[k]);
+          } else {
+               FwdCase c;
+               c.slot = k; c.J = J; c.R = b->res[k]; c.w = nullptr;
+               rc = ws_acquire_home(J->device, k, &c.w);
+               if (rc == 0) {
+                    if (!L) L = c.w;
+                    const bool bf = J->precision == LMIE_PRECISION_MIXED_BF16;
+                    rc = fused_prepare(c, bf ? upb : up16);
+                    if (rc == 0) {
+                         (bf ? b16 : f16).push_back(std::move(c));
+                         continue;
+                    }
+                    if (c.w != L) ws_release(c.w);
+               }
+          }
+          b->rc[0][k] = rc;
+          b->err[0][k] = rc ? g_err : "";
+     }
+     if (!L) return;
+     bool L_kept = false;
+     for (int t = 0; t < 2; ++t) {     // one arithmetic type per launch
+          std::vector<FwdCase> &v = t ? b16 : f16;
+          if (v.empty()) continue;
+          g_err[0] = 0;
+          const int rc = t ? fused_launch<BF16>(v, upb, L) : fused_launch<F16>(v, up16, L);
+          if (rc) (void) hipStreamSynchronize(L->stream);
+          for (FwdCase &c : v) {
+               b->rc[0][c.slot] = rc;
+               b->err[0][c.slot] = rc ? g_err : "";
+               c.R->state = rc ? nullptr : c.w;
+               if (rc == 0 && c.w == L) L_kept = true;
+               if (rc && c.w != L) ws_release(c.w);
+               b->fused_cases[0]++;
+          }
+     }
+     if (!L_kept) ws_release(L);
+}
+
+void batch_run_reverse(Batch *b) {
+     std::vector<RevCase> f16, b16;
+     Upload up16, upb;
+     Workspace *L = nullptr;
+     for (int k = 0; k < b->n; ++k) {
+          if (!b->in[1][k]) continue;
+          Workspace *w = b->ws[k];
+          g_err[0] = 0;
+          int rc;
+          if (!fused_reverse_ok(w, b->ain[k]) || w->device != b->device) {
+               rc = adjoint_one(w, b->ain[k], b->aout[k]);
+          } else {
+               RevCase c;
+               c.slot = k; c.w = w; c.in = b->ain[k]; c.out = b->aout[k];
+               if (!L) L = w;
+               rc = fused_rev_prepare(c, w->bf16 ? upb : up16);
+               if (rc == 0) {
+                    (w->bf16 ? b16 : f16).push_back(std::move(c));
+                    continue;
+               }
+          }
+          b->rc[1][k] = rc;
+          b->err[1][k] = rc ? g_err : "";
+     }
+     for (int t = 0; t < 2; ++t) {
+          std::vector<RevCase> &v = t ? b16 : f16;
+          if (v.empty()) continue;
+          g_err[0] = 0;
+          const int rc = t ? fused_rev_launch<BF16>(v, upb, L) : fused_rev_launch<F16>(v, up16, L);
+          if (rc) (void) hipStreamSynchronize(L->stream);
+          for (RevCase &c : v) {
+               b->rc[1][c.slot] = rc;
+               b->err[1][c.slot] = rc ? g_err : "";
+               b->fused_cases[1]++;
+          }
+     }
+}
+
+void batch_fire(Batch *b, int p) {   // with b->mu held; runs phase p if complete
+     if (b->state[p] != 0 || b->arrived[p] + b->left[p] != b->n) return;
+     if (b->arrived[p] == 0) {
+          b->state[p] = 2;
+          return;
+     }
+     b->state[p] = 1;
+     pthread_mutex_unlock(&b->mu);
+     (void) use_device(b->device);
+     if (p == 0) batch_run_forward(b);
+     else batch_run_reverse(b);
+     pthread_mutex_lock(&b->mu);
+     b->state[p] = 2;
+     pthread_cond_broadcast(&b->cv);
+}
+
+int batch_arrive(int p, const lmie_gpu_job *job, lmie_gpu_result *res, Workspace *w, const lmie_gpu_adj_in *ain,
+                 lmie_gpu_adj_out *aout) {
+     Batch *b = t_batch;
+     const int k = t_slot;
+     pthread_mutex_lock(&b->mu);
+     if (p == 0) {
+          b->job[k] = job;
+          b->res[k] = res;
+          if (b->arrived[0] == 0) b->device = job->device;
+     } else {
+          b->ws[k] = w;
+          b->ain[k] = ain;
+          b->aout[k] = aout;
+          if (b->arrived[1] == 0) b->device = w->device;
+     }
+     b->in[p][k] = 1;
+     b->arrived[p]++;
+     t_phase = p + 1;
+     batch_fire(b, p);
+     while (b->state[p] != 2) pthread_cond_wait(&b->cv, &b->mu);
+     const int rc = b->rc[p][k];
+     const std::string e = b->err[p][k];
+     pthread_mutex_unlock(&b->mu);
+     if (rc) return set_err("%s", e.c_str());
+     return 0;
+}
+
 // Plain 16-byte-per-lane copy: the calibration point for HBM traffic counters
 // (n16 elements of 16 bytes read and written once).
 __global__ __launch_bounds__(256) void copy16(const v4u *src, v4u *dst, size_t n16) {
@@ -3389,6 +4430,74 @@ int lmie_get_timing(double *ms, int n) {
 int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
      g_err[0] = 0;
      if (!job || !res) return set_err("null argument");
+     if (t_batch && t_phase == 0) {
+          res->state = nullptr;
+          return batch_arrive(0, job, res, nullptr, nullptr, nullptr);
+     }
+     return forward_one(job, res);
+}
+
+int lmie_gpu_adjoint(void *state, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
+     g_err[0] = 0;
+     if (!state || !in || !out || !in->sums_bar || !out->f_bar) return set_err("null argument");
+     if (t_batch && t_phase == 1) return batch_arrive(1, nullptr, nullptr, (Workspace *) state, in, out);
+     return adjoint_one((Workspace *) state, in, out);
+}
+
+void *lmie_gpu_batch_open(int n_case) {
+     g_err[0] = 0;
+     if (n_case < 1 || n_case > 32767) {
+          set_err("batch of %d cases (1 .. 32767)", n_case);
+          return nullptr;
+     }
+     Batch *b = new Batch();
+     b->n = n_case;
+     for (int p = 0; p < 2; ++p) {
+          b->in[p].assign(n_case, 0);
+          b->rc[p].assign(n_case, 0);
+          b->err[p].assign(n_case, std::string());
+     }
+     b->job.assign(n_case, nullptr);
+     b->res.assign(n_case, nullptr);
+     b->ws.assign(n_case, nullptr);
+     b->ain.assign(n_case, nullptr);
+     b->aout.assign(n_case, nullptr);
+     return b;
+}
+
+int lmie_gpu_batch_enter(void *batch, int slot) {
+     g_err[0] = 0;
+     Batch *b = (Batch *) batch;
+     if (!b || slot < 0 || slot >= b->n) return set_err("batch slot %d", slot);
+     if (t_batch) return set_err("this thread is already in a batch");
+     t_batch = b;
+     t_slot = slot;
+     t_phase = 0;
+     return 0;
+}
+
+void lmie_gpu_batch_leave(void) {
+     Batch *b = t_batch;
+     if (!b) return;
+     pthread_mutex_lock(&b->mu);
+     for (int p = t_phase; p < 2; ++p) b->left[p]++;
+     for (int p = 0; p < 2; ++p) batch_fire(b, p);
+     pthread_mutex_unlock(&b->mu);
+     t_batch = nullptr;
+     t_slot = -1;
+     t_phase = 0;
+}
+
+int lmie_gpu_batch_fused(void *batch, int pass) {
+     Batch *b = (Batch *) batch;
+     return b && pass >= 0 && pass < 2 ? b->fused_cases[pass] : 0;
+}
+
+void lmie_gpu_batch_close(void *batch) { delete (Batch *) batch; }
+}
+
+namespace {
+int forward_one(const lmie_gpu_job *job, lmie_gpu_result *res) {
      if (job->precision != LMIE_PRECISION_FP64 && job->precision != LMIE_PRECISION_MIXED &&
          job->precision != LMIE_PRECISION_MIXED_BF16)
           return set_err("unknown precision %d", job->precision);
@@ -3410,15 +4519,15 @@ int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res) {
      return rc;
 }
 
-int lmie_gpu_adjoint(void *state, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
-     g_err[0] = 0;
-     if (!state || !in || !out || !in->sums_bar || !out->f_bar) return set_err("null argument");
-     Workspace *w = (Workspace *) state;
-     HIPCHECK(hipSetDevice(w->device));
+int adjoint_one(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
+     if (use_device(w->device)) return -1;
      int rc = adjoint_impl(w, in, out);
      if (rc) (void) hipStreamSynchronize(w->stream);
      return rc;
 }
+}  // namespace
+
+extern "C" {
 
 void lmie_gpu_release(void *state) {
      if (state) ws_release((Workspace *) state);

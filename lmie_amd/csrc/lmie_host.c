@@ -66,8 +66,12 @@ static void *pool_worker(void *arg) {
      return NULL;
 }
 
+/* set on the threads of a fused batch: there the cases are the parallelism,
+   and each runs its host loops alone */
+static __thread int g_batch_thread;
+
 static void par_for(int n, par_fn fn, void *ctx) {
-     if (n < 2048 || pthread_mutex_trylock(&g_pool.region) != 0) {
+     if (n < 2048 || g_batch_thread || pthread_mutex_trylock(&g_pool.region) != 0) {
           fn(ctx, 0, n);
           return;
      }
@@ -201,11 +205,38 @@ static double sd(const double *v, int d) { return v ? v[d] : 0.; }
 /* n_int equal intervals on [r1, r2] with n_quad nodes each; the map of the
    [-1, 1] rule onto each interval follows misc/rtutil_math.c:329-387 (the
    mirrored node sets the offset). */
+/* gl_rule of the last few orders asked for, kept for the next calls (a
+   batch of cases builds the same rule once per case). */
+static void gl_rule_cached(int n, double *x, double *w) {
+     static pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+     static struct { int n; double *x, *w; } memo[4];
+     static int next;
+     pthread_mutex_lock(&mu);
+     for (int k = 0; k < 4; ++k)
+          if (memo[k].n == n && memo[k].x) {
+               memcpy(x, memo[k].x, sizeof(double) * n);
+               memcpy(w, memo[k].w, sizeof(double) * n);
+               pthread_mutex_unlock(&mu);
+               return;
+          }
+     pthread_mutex_unlock(&mu);
+     gl_rule(n, x, w);
+     double *cx = malloc(sizeof(double) * n), *cw = malloc(sizeof(double) * n);
+     if (!cx || !cw) { free(cx); free(cw); return; }
+     memcpy(cx, x, sizeof(double) * n);
+     memcpy(cw, w, sizeof(double) * n);
+     pthread_mutex_lock(&mu);
+     const int k = next++ % 4;
+     free(memo[k].x); free(memo[k].w);
+     memo[k].n = n; memo[k].x = cx; memo[k].w = cw;
+     pthread_mutex_unlock(&mu);
+}
+
 static void grid_intervals(int n_int, int n_quad, int nd, double r1, double r2,
                            const double *r1_l, const double *r2_l,
                            double *qx, double *qw, double *qx_l, double *qw_l) {
      double *tx = vec(n_quad), *tw = vec(n_quad);
-     gl_rule(n_quad, tx, tw);
+     gl_rule_cached(n_quad, tx, tw);
      /* the reference scales the [-1,1] weights by the half-width of each
         interval with its own weight formula ((half - off*y) * c); on [-1, 1]
         that is (1 - y^2) * c, as gl_rule computes */
@@ -1170,7 +1201,7 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
         helper thread while the forward runs on the GPU */
      dn_job dj = {modes, weight, n_modes, g.N, NULL};
      pthread_t dth;
-     const int dthr = pthread_create(&dth, NULL, dn_build, &dj) == 0;
+     const int dthr = !g_batch_thread && pthread_create(&dth, NULL, dn_build, &dj) == 0;
      if (!dthr) dn_build(&dj);
      run_flags rf = {m0->calc_gc, m0->calc_lc, 0, 0, 0, m0->lambda, m0->mr, m0->mi, m0->accuracy, 1};
      core_out keep;
@@ -1303,6 +1334,76 @@ static int batch_case(batch_job *b, int c) {
      return rc ? -1 : 0;
 }
 
+/* Fused mode: one host thread per case of a device batch; the device passes
+   of all cases run as single launches (include/lmie_gpu.h). */
+typedef struct {
+     batch_job *b;
+     void *gpu_batch;
+     int first, slot;
+} batch_member;
+
+static void batch_record(batch_job *b, int c, int rc) {
+     if (b->status) b->status[c] = rc;
+     if (rc) {
+          pthread_mutex_lock(&b->mu);
+          if (b->first_failed < 0 || c < b->first_failed) {
+               b->first_failed = c;
+               snprintf(b->err, sizeof b->err, "case %d: %s", c, lmie_last_error());
+          }
+          pthread_mutex_unlock(&b->mu);
+     }
+}
+
+static void *batch_member_run(void *vp) {
+     batch_member *m = vp;
+     const int c = m->first + m->slot;
+     g_batch_thread = 1;
+     int rc = lmie_gpu_batch_enter(m->gpu_batch, m->slot);
+     if (rc) rc = fail("lmie_gpu_batch_enter", lmie_gpu_last_error());
+     else rc = batch_case(m->b, c);
+     lmie_gpu_batch_leave();
+     batch_record(m->b, c, rc);
+     return NULL;
+}
+
+#define LMIE_BATCH_GROUP 64
+
+static __thread int g_batch_fused[2];
+
+int lmie_last_batch_fused(int pass) { return pass >= 0 && pass < 2 ? g_batch_fused[pass] : 0; }
+
+static void batch_fused(batch_job *b) {
+     g_batch_fused[0] = g_batch_fused[1] = 0;
+     for (int g0 = 0; g0 < b->n_case; g0 += LMIE_BATCH_GROUP) {
+          const int m = b->n_case - g0 < LMIE_BATCH_GROUP ? b->n_case - g0 : LMIE_BATCH_GROUP;
+          void *gb = lmie_gpu_batch_open(m);
+          if (!gb) {
+               fail("lmie_gpu_batch_open", lmie_gpu_last_error());
+               for (int k = 0; k < m; ++k) batch_record(b, g0 + k, -1);
+               continue;
+          }
+          batch_member mem[LMIE_BATCH_GROUP];
+          pthread_t th[LMIE_BATCH_GROUP];
+          int started[LMIE_BATCH_GROUP];
+          for (int k = 0; k < m; ++k) {
+               mem[k].b = b; mem[k].gpu_batch = gb; mem[k].first = g0; mem[k].slot = k;
+               started[k] = pthread_create(&th[k], NULL, batch_member_run, &mem[k]) == 0;
+          }
+          /* a case whose thread did not start leaves the device batch from here */
+          for (int k = 0; k < m; ++k)
+               if (!started[k]) {
+                    if (lmie_gpu_batch_enter(gb, k) == 0) lmie_gpu_batch_leave();
+                    fail("lmie_solution_adjoint_batch", "could not start a thread");
+                    batch_record(b, g0 + k, -1);
+               }
+          for (int k = 0; k < m; ++k)
+               if (started[k]) pthread_join(th[k], NULL);
+          g_batch_fused[0] += lmie_gpu_batch_fused(gb, 0);
+          g_batch_fused[1] += lmie_gpu_batch_fused(gb, 1);
+          lmie_gpu_batch_close(gb);
+     }
+}
+
 static void *batch_worker(void *vp) {
      batch_job *b = vp;
      for (;;) {
@@ -1310,16 +1411,7 @@ static void *batch_worker(void *vp) {
           const int c = b->next++;
           pthread_mutex_unlock(&b->mu);
           if (c >= b->n_case) return NULL;
-          const int rc = batch_case(b, c);
-          if (b->status) b->status[c] = rc;
-          if (rc) {
-               pthread_mutex_lock(&b->mu);
-               if (b->first_failed < 0 || c < b->first_failed) {
-                    b->first_failed = c;
-                    snprintf(b->err, sizeof b->err, "case %d: %s", c, lmie_last_error());
-               }
-               pthread_mutex_unlock(&b->mu);
-          }
+          batch_record(b, c, batch_case(b, c));
      }
 }
 
@@ -1337,7 +1429,14 @@ int lmie_solution_adjoint_batch(int n_case, lmie_in_data *const *cases, lmie_out
      b.cases = cases; b.outs = outs; b.alloc_out = alloc_out; b.opts = opts; b.cot = cot;
      b.users = users; b.grads = grads; b.grad_modes = grad_modes; b.status = status;
      pthread_mutex_init(&b.mu, NULL);
-     int T = n_workers > 0 ? n_workers : 4;
+     g_batch_fused[0] = g_batch_fused[1] = 0;
+     if (n_workers <= 0) {
+          batch_fused(&b);
+          pthread_mutex_destroy(&b.mu);
+          if (b.first_failed >= 0) return fail("lmie_solution_adjoint_batch", b.err);
+          return 0;
+     }
+     int T = n_workers;
      if (T > n_case) T = n_case;
      if (T > 64) T = 64;
      pthread_t th[64];

@@ -8,6 +8,7 @@ lmie_amd.abi and lmie_amd.workloads.
 """
 import ctypes as C
 import math
+import time
 
 import numpy as np
 
@@ -58,6 +59,8 @@ def bind_all(lib):
                                                 C.POINTER(C.c_void_p), C.POINTER(LmieGrad),
                                                 C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_int]
     lib.lmie_solution_adjoint_batch.restype = C.c_int
+    lib.lmie_last_batch_fused.argtypes = [C.c_int]
+    lib.lmie_last_batch_fused.restype = C.c_int
     lib.lmie_last_error.restype = C.c_char_p
     lib.lmie_gpu_last_error.restype = C.c_char_p
     lib.lmie_set_timing.argtypes = [C.c_int]
@@ -226,7 +229,9 @@ def adjoint_batch(lib, cases, targets, weights, precision="mixed", device=0, n_w
     """lmie_solution_adjoint_batch with the library's weighted L2 loss on gc
     (lmie_cotangent_gc_l2): case c is the single distribution cases[c], with
     L_c = sum_kl weights[c]_kl (gc_kl - targets[c]_kl)^2 over [6][max_coef].
-    Returns one (outputs, gradient, loss) triple per case, as ``adjoint``."""
+    n_workers 0: one launch per kernel for all cases; > 0: that many cases at a
+    time on their own streams.  Returns one (outputs, gradient, loss) triple per
+    case, as ``adjoint``."""
     n = len(cases)
     ins = [abi.make_in(lib, p, None) for p in cases]
     outs = [LmieOut() for _ in range(n)]
@@ -247,8 +252,10 @@ def adjoint_batch(lib, cases, targets, weights, precision="mixed", device=0, n_w
     status = (C.c_int * n)()
     o = _opts(precision, device)
     cot = C.cast(lib.lmie_cotangent_gc_l2, C.c_void_p)
+    t0 = time.perf_counter()
     rc = lib.lmie_solution_adjoint_batch(n, pin, pout, 1, C.byref(o), cot, users, grads, gm,
                                          status, n_workers)
+    adjoint_batch.call_seconds = time.perf_counter() - t0
     res, err = [], None if rc == 0 else last_error(lib)
     try:
         for c in range(n):

@@ -171,16 +171,18 @@ def test_contraction_shapes_agree(product, env):
     check([g1[k] for k in keys], [g0[k] for k in keys], 1e-5, "shape %s=%s" % env)
 
 
-def test_batch_of_cfg3_cases_in_one_call(product):
-    """The 32 retrieval cases in one lmie_solution_adjoint_batch call (mixed,
-    4 workers): every case's outputs and gradient equal those of its own
+@pytest.mark.parametrize("workers", [0, 4])
+def test_batch_of_cfg3_cases_in_one_call(product, workers):
+    """The 32 retrieval cases in one lmie_solution_adjoint_batch call (mixed;
+    workers 0: every kernel launched once for all cases, 4: four cases at a
+    time on their own streams): every case's outputs and gradient equal those of its own
     single call bit for bit, the library's L2 loss equals the loss formed
     here, and the gradients meet the reference tangents at the mixed bar.
     The mixed path's gradient errors are normwise (about 1e-7 of ||g||), and
     in some of these cases one component cancels to 1e-3 of ||g||, so the
     per-component floor here is 1e-2 ||g_ref||: an absolute error up to
     1e-6 ||g_ref|| on such a component."""
-    say("case: cfg3 batch, 32 cases in one call, mixed, 4 workers")
+    say("case: cfg3 batch, 32 cases in one call, mixed, workers", workers)
     z = np.load(load_golden.__globals__["GOLDEN"] + "/cfg3.npz")
     cases = W.cfg3_cases()
     params = [p for p, _ in cases]
@@ -188,7 +190,10 @@ def test_batch_of_cfg3_cases_in_one_call(product):
               for c, p in enumerate(params)]
     tw = [W.expansion_loss_targets(c_refs[c], 100 + c) for c in range(len(params))]
     res = lmie.adjoint_batch(product, params, [t for t, _ in tw], [w for _, w in tw], "mixed",
-                             n_workers=4)
+                             n_workers=workers)
+    fused = [product.lmie_last_batch_fused(p) for p in (0, 1)]
+    say("  cases fused: forward %d, reverse %d" % tuple(fused))
+    assert fused == ([len(params)] * 2 if workers == 0 else [0, 0])
     for c, (p, seeds) in enumerate(cases):
         out_b, g_b, loss_b = res[c]
         out, g = lmie.adjoint(product, [p], None, loss_fn(c_refs[c], 100 + c), "mixed")

@@ -104,8 +104,9 @@ def test_batch_reports_each_failing_case(lib):
     good = dict(dist_type="log_normal", n_int2=2, n_quad=4, calc_gc=1, calc_lc=0, n_angles=0,
                 **{"lambda": 0.5}, mr=1.5, mi=0.01, a1=0.3, a2=0.2, r1=0.01, r2=2.0)
     cases = [dict(good, dist_type=42), dict(good, dist_type=43), dict(good, dist_type=44)]
-    with pytest.raises(RuntimeError, match="case 0"):
-        lmie.adjoint_batch(lib, cases, [0.0] * 3, [1.0] * 3, "mixed", n_workers=2)
+    for workers in (0, 2):
+        with pytest.raises(RuntimeError, match="case 0"):
+            lmie.adjoint_batch(lib, cases, [0.0] * 3, [1.0] * 3, "mixed", n_workers=workers)
     ins = [abi.make_in(lib, p) for p in cases]
     outs = [abi.LmieOut() for _ in cases]
     pin = (C.POINTER(abi.LmieIn) * 3)(*[C.pointer(x) for x in ins])
@@ -120,3 +121,6 @@ def test_batch_reports_each_failing_case(lib):
     assert lib.lmie_solution_adjoint_batch(3, pin, pout, 1, None, None, None, grads, None, status, 3) == -1
     assert "cotangent" in lmie.last_error(lib)
     assert lib.lmie_solution_adjoint_batch(0, None, None, 1, None, cot, None, None, None, None, 0) == 0
+    status = (C.c_int * 3)(7, 7, 7)
+    assert lib.lmie_solution_adjoint_batch(3, pin, pout, 1, None, cot, None, grads, None, status, 0) == -1
+    assert list(status) == [-1, -1, -1] and all(not bool(o.gc) for o in outs)
