@@ -234,8 +234,9 @@ __device__ __forceinline__ void gauss_nodes_body(int n, const double *guess, dou
      w[i] = w[n - 1 - i] = __dmul_rn(__dsub_rn(1., __dmul_rn(off, y)), c);
 }
 __global__ void gauss_nodes(int n, const double *guess, double *mu, double *w) { gauss_nodes_body(n, guess, mu, w); }
-// gauss_nodes over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
+// Batched form of each kernel used by fused batches (name_b): one launch for
+// all cases, blockIdx.z picks the case, whose own grid (gx, gy blocks) may be
+// smaller than the launch grid; its surplus blocks exit.
 struct gauss_nodes_B {
      int n;
      const double *guess;
@@ -312,8 +313,6 @@ __device__ __forceinline__ void pitau_table16_body(int H, int Hp, int n1max, int
 }
 template <class E>
 __global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) { pitau_table16_body<E>(H, Hp, n1max, Kb, mu, B16); }
-// pitau_table16 over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct pitau_table16_B {
      int H;
      int Hp;
@@ -397,8 +396,6 @@ __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const doubl
      }
 }
 __global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) { gsf_table_body(H, Hp, C, mu, T); }
-// gsf_table over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct gsf_table_B {
      int H;
      int Hp;
@@ -731,8 +728,6 @@ __device__ __forceinline__ void mie_chains_body(ChainArgs p) {
      }
 }
 __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) { mie_chains_body(p); }
-// mie_chains over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct mie_chains_B {
      ChainArgs p;
      unsigned gx, gy;
@@ -777,8 +772,6 @@ __device__ __forceinline__ void order_factors_body(int n, double *ft) {
      ft[FT_FV3 * ld + j] = j > 0 ? (j + 2.) / ((j + 1.) / j) : 0.;
 }
 __global__ void order_factors(int n, double *ft) { order_factors_body(n, ft); }
-// order_factors over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct order_factors_B {
      int n;
      double *ft;
@@ -898,8 +891,6 @@ __device__ __forceinline__ void mie_terms_body(TermArgs p) {
 }
 template <class E>
 __global__ __launch_bounds__(64) void mie_terms(TermArgs p) { mie_terms_body<E>(p); }
-// mie_terms over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct mie_terms_B {
      TermArgs p;
      unsigned gx, gy;
@@ -950,8 +941,6 @@ __device__ __forceinline__ void mie_sums_body(const double *part, int N, int Np,
      }
 }
 __global__ __launch_bounds__(256) void mie_sums(const double *part, int N, int Np, int ng, const int *n1, double *S, double *CB) { mie_sums_body(part, N, Np, ng, n1, S, CB); }
-// mie_sums over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct mie_sums_B {
      const double *part;
      int N;
@@ -993,8 +982,6 @@ __device__ __forceinline__ void weighted_dots_body(const DotJob *jobs, int n, do
      if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 __global__ __launch_bounds__(256) void weighted_dots(const DotJob *jobs, int n, double *out) { weighted_dots_body(jobs, n, out); }
-// weighted_dots over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct weighted_dots_B {
      const DotJob *jobs;
      int n;
@@ -1665,8 +1652,6 @@ __device__ __forceinline__ void expand_adjoint_part_body(const double *T, const 
      }
 }
 __global__ __launch_bounds__(64) void expand_adjoint_part(const double *T, const double *gbar, const double *lbar, int H, int Hp, int C, double *part) { expand_adjoint_part_body(T, gbar, lbar, H, Hp, C, part); }
-// expand_adjoint_part over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct expand_adjoint_part_B {
      const double *T;
      const double *gbar;
@@ -1703,8 +1688,6 @@ __device__ __forceinline__ void expand_adjoint_fin_body(const double *part, int 
      }
 }
 __global__ void expand_adjoint_fin(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) { expand_adjoint_fin_body(part, nck, w, H, Hp, Q, pfbar); }
-// expand_adjoint_fin over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct expand_adjoint_fin_B {
      const double *part;
      int nck;
@@ -1906,8 +1889,6 @@ __device__ __forceinline__ void adj_weights_body(const double *pfbar, int H, int
      }
 }
 __global__ __launch_bounds__(1024) void adj_weights(const double *pfbar, int H, int Hq, int Q, double *wts, AdjScal *sc) { adj_weights_body(pfbar, H, Hq, Q, wts, sc); }
-// adj_weights over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct adj_weights_B {
      const double *pfbar;
      int H;
@@ -1972,8 +1953,6 @@ __device__ __forceinline__ void adj_tables_body(const unsigned short *B16, int K
 }
 template <class E>
 __global__ __launch_bounds__(64) void adj_tables(const unsigned short *B16, int Kb, int Hb, const double *wts, int H, int Hq, int Kq, int n1max, unsigned short *L16, unsigned short *R16) { adj_tables_body<E>(B16, Kb, Hb, wts, H, Hq, Kq, n1max, L16, R16); }
-// adj_tables over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct adj_tables_B {
      const unsigned short *B16;
      int Kb;
@@ -2087,8 +2066,6 @@ __device__ __forceinline__ void adj_gbuild_body(const unsigned short *L16, const
 }
 template <class E>
 __global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, const unsigned short *R16, int Hq, int Kq, float *Gf, AdjScal *sc) { adj_gbuild_body<E>(L16, R16, Hq, Kq, Gf, sc); }
-// adj_gbuild over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct adj_gbuild_B {
      const unsigned short *L16;
      const unsigned short *R16;
@@ -2141,8 +2118,6 @@ __device__ __forceinline__ void adj_gsplit_body(const float *Gf, const AdjScal *
 }
 template <class E>
 __global__ __launch_bounds__(64) void adj_gsplit(const float *Gf, const AdjScal *sc, int Kq, unsigned short *G16) { adj_gsplit_body<E>(Gf, sc, Kq, G16); }
-// adj_gsplit over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct adj_gsplit_B {
      const float *Gf;
      const AdjScal *sc;
@@ -2471,8 +2446,6 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      o[8 * Np] = mb.r; o[9 * Np] = mb.i; o[10 * Np] = fq;
 }
 __global__ __launch_bounds__(64) void rev_orders(RevArgs p) { rev_orders_body(p); }
-// rev_orders over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct rev_orders_B {
      RevArgs p;
      unsigned gx, gy;
@@ -2515,8 +2488,6 @@ __device__ __forceinline__ void rev_carry_body(RevArgs p) {
      p.fbar[i] = (p.Uf ? fq : p.fang[i]) + fser;
 }
 __global__ __launch_bounds__(64) void rev_carry(RevArgs p) { rev_carry_body(p); }
-// rev_carry over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct rev_carry_B {
      RevArgs p;
      unsigned gx, gy;
@@ -2551,8 +2522,6 @@ __device__ __forceinline__ void reduce_parts_body(const double *part, int nblk, 
      out[(size_t) map.g[c] * 4 * Q + rem] = s;
 }
 __global__ __launch_bounds__(256) void reduce_parts(const double *part, int nblk, int nchp, int Q, ChMap map, double *out) { reduce_parts_body(part, nblk, nchp, Q, map, out); }
-// reduce_parts over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct reduce_parts_B {
      const double *part;
      int nblk;
@@ -2607,8 +2576,6 @@ __device__ __forceinline__ void expand_rows_body(const double *pf, int nch, int 
      }
 }
 __global__ void expand_rows(const double *pf, int nch, int Q, int H, int Hp, const double *w, int gsf, double *E) { expand_rows_body(pf, nch, Q, H, Hp, w, gsf, E); }
-// expand_rows over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct expand_rows_B {
      const double *pf;
      int nch;
@@ -2682,8 +2649,6 @@ __device__ __forceinline__ void expand_coefs_body(const double *T, const double 
      }
 }
 __global__ __launch_bounds__(256) void expand_coefs(const double *T, const double *E, int H, int Hp, int C, int gsf, double *coefs) { expand_coefs_body(T, E, H, Hp, C, gsf, coefs); }
-// expand_coefs over the cases of a batch: blockIdx.z picks the case, whose grid may be
-// smaller than the launch grid (gx, gy blocks).
 struct expand_coefs_B {
      const double *T;
      const double *E;

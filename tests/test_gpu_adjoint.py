@@ -207,3 +207,26 @@ def test_batch_of_cfg3_cases_in_one_call(product, workers):
         assert abs(loss_b - loss) <= 1e-12 * abs(loss), (c, loss_b, loss)
         ours = [g_b["a1"], g_b["a2"] * seeds["a2"][1], g_b["mr"], g_b["mi"]]
         check(ours, z["loss_grad"][c], TOL["mixed"], "case %d" % c, floor_frac=1e-2)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed-bf16"])
+def test_batch_other_precisions_equal_single_calls(product, prec):
+    """A batch of six small cases in FP64 (run one by one inside the batch's
+    passes) and in the bf16 split (fused with the bf16 kernels): each case equals
+    its single call bit for bit."""
+    say("case: batch of 6 small log-normal cases,", prec)
+    cases = [dict(FD_CASE, n_int2=4 + k, mr=1.40 + 0.03 * k, mi=0.001 * (k + 1), a1=0.5 + 0.2 * k,
+                  **{"lambda": 0.5 + 0.07 * k}) for k in range(6)]
+    refs = [lmie.solve(product, p, None, "fp64")["gc"] for p in cases]
+    tw = [W.expansion_loss_targets(refs[k], 50 + k) for k in range(6)]
+    res = lmie.adjoint_batch(product, cases, [t for t, _ in tw], [w for _, w in tw], prec)
+    fused = [product.lmie_last_batch_fused(p) for p in (0, 1)]
+    say("  cases fused: forward %d, reverse %d" % tuple(fused))
+    assert fused == ([0, 0] if prec == "fp64" else [6, 6])
+    for k, p in enumerate(cases):
+        out_b, g_b, _ = res[k]
+        out, g = lmie.adjoint(product, [p], None, loss_fn(refs[k], 50 + k), prec)
+        assert np.array_equal(out_b["gc"], out["gc"]) and np.array_equal(out_b["lc"], out["lc"]), k
+        for key in ("cext", "csca", "cbak", "g", "mr", "mi", "a1", "a2"):
+            a, b = (out_b, out) if key in out else (g_b, g)
+            assert a[key] == b[key], (k, key, a[key], b[key])
