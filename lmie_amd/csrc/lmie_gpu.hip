@@ -202,22 +202,28 @@ __device__ __forceinline__ cd cdiv_nr(cd a, cd b) {
 // initial guesses come from the host (the C library's cos and tan, as the
 // reference) and every operation here is a separately rounded IEEE operation
 // (no contraction): the rule is the reference's rule bit for bit.
-__device__ __forceinline__ void gauss_nodes_body(int n, const double *guess, double *mu, double *w) {
+// The factors j / (j + 1) of the Legendre recurrence come from LDS when the
+// degree fits (the same correctly rounded quotients, computed once per
+// workgroup), else they are divided in the loop.
+constexpr int GAUSS_LDS = 8192;
+
+template <bool TAB>
+__device__ __forceinline__ void gauss_newton(int n, const double *cj, double &y, double &pa, double &pb) {
 #pragma clang fp contract(off)
-     int i = blockIdx.x * blockDim.x + threadIdx.x;
-     if (i >= (n + 1) / 2) return;
-     double y = guess[i], y0, pa, pb;
      const double dn = (double) n, nn1 = (double) (n * (n + 1));
+     double y0;
      int it = 0;
      do {
           double pm;
           pa = 1.;
           pb = 0.;
+#pragma unroll 4
           for (int j = 0; j < n; ++j) {
                pm = pb;
                pb = pa;
                const double u = __dmul_rn(y, pb);
-               pa = __dadd_rn(__dmul_rn(__ddiv_rn((double) j, __dadd_rn((double) j, 1.)), __dsub_rn(u, pm)), u);
+               const double c = TAB ? cj[j] : __ddiv_rn((double) j, __dadd_rn((double) j, 1.));
+               pa = __dadd_rn(__dmul_rn(c, __dsub_rn(u, pm)), u);
           }
           const double s = __dsub_rn(1., __dmul_rn(y, y));
           const double d1 = __ddiv_rn(__dmul_rn(dn, __dsub_rn(pb, __dmul_rn(y, pa))), s);
@@ -226,6 +232,22 @@ __device__ __forceinline__ void gauss_nodes_body(int n, const double *guess, dou
           y0 = y;
           y = __dsub_rn(y0, __dmul_rn(h, __dadd_rn(1., __ddiv_rn(__dmul_rn(h, d2), __dmul_rn(2., d1)))));
      } while (fabs(__dsub_rn(y, y0)) > 2.220446049250313e-16 && ++it < 100);
+}
+
+__device__ __forceinline__ void gauss_nodes_body(int n, const double *guess, double *mu, double *w) {
+#pragma clang fp contract(off)
+     __shared__ double cj[GAUSS_LDS];
+     const bool tab = n <= GAUSS_LDS;
+     if (tab) {
+          for (int j = threadIdx.x; j < n; j += blockDim.x) cj[j] = __ddiv_rn((double) j, __dadd_rn((double) j, 1.));
+          __syncthreads();
+     }
+     int i = blockIdx.x * blockDim.x + threadIdx.x;
+     if (i >= (n + 1) / 2) return;
+     double y = guess[i], pa, pb;
+     if (tab) gauss_newton<true>(n, cj, y, pa, pb);
+     else gauss_newton<false>(n, nullptr, y, pa, pb);
+     const double dn = (double) n;
      double c = __dmul_rn(dn, pb);
      c = __ddiv_rn(2., __dmul_rn(c, c));
      const double off = __dmul_rn(1., y);
@@ -280,10 +302,13 @@ __global__ void pitau_table(int H, int Hp, int n1max, int Kp, const double *mu, 
 // [4 planes: (pi+tau) hi, lo, (pi-tau) hi, lo][Kb][Hp][8], value
 // (pi_n +- tau_n) / (n (n+1)) * kSB.  A lane (angle) writes 16 bytes per plane
 // per group of 8 orders; consecutive lanes are consecutive 16-byte chunks.
-template <class E>
-__device__ __forceinline__ void pitau_table16_body(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) {
-     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-     if (j >= Hp) return;
+// The per-order factors (n+1)/n and kSB/(n(n+1)) come from LDS when n1max
+// fits, else they are divided in the loop.
+constexpr int PITAU_LDS = 4096;
+
+template <class E, bool TAB>
+__device__ __forceinline__ void pitau16_rows(int H, int Hp, int n1max, int Kb, const double *mu, const double *fq,
+                                             const double *fs, unsigned short *B16, int j) {
      const size_t plane = (size_t) Kb * Hp * 8;
      const double m = j < H ? mu[H + j] : 0.;
      double p0 = 0., p1 = 1.;
@@ -295,11 +320,11 @@ __device__ __forceinline__ void pitau_table16_body(int H, int Hp, int n1max, int
                if (j < H && n <= n1max) {
                     const double a1 = m * p1, a2 = a1 - p0;
                     const double tau = n * a2 - p0;
-                    const double sc = kSB / (n * (n + 1.));
+                    const double sc = TAB ? fs[n] : kSB / (n * (n + 1.));
                     bp = (p1 + tau) * sc;
                     bm = (p1 - tau) * sc;
                     p0 = p1;
-                    p1 = a1 + (n + 1.) / n * a2;
+                    p1 = a1 + (TAB ? fq[n] : (n + 1.) / n) * a2;
                }
                split16<E>(bp, &v[0][e], &v[1][e]);
                split16<E>(bm, &v[2][e], &v[3][e]);
@@ -310,6 +335,23 @@ __device__ __forceinline__ void pitau_table16_body(int H, int Hp, int n1max, int
                *(v4u *) (B16 + q * plane + ((size_t) g * Hp + j) * 8) = w;
           }
      }
+}
+
+template <class E>
+__device__ __forceinline__ void pitau_table16_body(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) {
+     __shared__ double fq[PITAU_LDS + 1], fs[PITAU_LDS + 1];
+     const bool tab = n1max <= PITAU_LDS;
+     if (tab) {
+          for (int n = threadIdx.x + 1; n <= n1max; n += blockDim.x) {
+               fq[n] = (n + 1.) / n;
+               fs[n] = kSB / (n * (n + 1.));
+          }
+          __syncthreads();
+     }
+     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+     if (j >= Hp) return;
+     if (tab) pitau16_rows<E, true>(H, Hp, n1max, Kb, mu, fq, fs, B16, j);
+     else pitau16_rows<E, false>(H, Hp, n1max, Kb, mu, nullptr, nullptr, B16, j);
 }
 template <class E>
 __global__ void pitau_table16(int H, int Hp, int n1max, int Kb, const double *mu, unsigned short *B16) { pitau_table16_body<E>(H, Hp, n1max, Kb, mu, B16); }
@@ -2153,14 +2195,26 @@ struct UArgs {
 // eight waves, two per SIMD, which hide each other's LDS and barrier waits;
 // WA = 2: four waves, each operand fragment read for twice the products;
 // TS = 64: four waves, two workgroups per CU.
-template <class E, int WA, int TS>
-__device__ __forceinline__ void adj_ugemm32_body(const UArgs &p, int2 tk) {
+template <class E, int WA, int TS, bool B>
+__global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
+void adj_ugemm32(UArgs p0, const UArgs *cases, const int2 *btask) {
      constexpr int TO = 64, G = 2;                           // orders, groups of 8 per stage
      constexpr int NT = TS * 4 / WA, WS = TS / 32 / WA;      // threads, size-block waves
      constexpr int NA = 8 * G * TS / NT, NG = 12 * G * TO / NT;
      static_assert(NA * NT == 8 * G * TS && NG * NT == 12 * G * TO, "staging split");
      __shared__ v4u sA[2][8 * G][TS];
      __shared__ v4u sG[2][12 * G][TO];
+     // B: one task list for the cases of a batch, task (size tile, order tile | case << 16)
+     int2 tk;
+     UArgs p;
+     if constexpr (B) {
+          const int2 t = btask[blockIdx.x];
+          p = cases[t.y >> 16];
+          tk = make_int2(t.x, t.y & 0xffff);
+     } else {
+          p = p0;
+          tk = p.task[blockIdx.x];
+     }
      if (tk.x < 0) return;
      const int it0 = tk.x * TS, n0 = tk.y * TO, nch = p.ktile[tk.x] / (8 * G);
      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -2260,19 +2314,6 @@ __device__ __forceinline__ void adj_ugemm32_body(const UArgs &p, int2 tk) {
 #pragma unroll
                for (int k = 0; k < 4; ++k) p.Uf[((size_t) k * Kq + r) * Np + i] = (float) (acc[a][k][v] * f);
           }
-}
-template <class E, int WA, int TS>
-__global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
-void adj_ugemm32(UArgs p) {
-     adj_ugemm32_body<E, WA, TS>(p, p.task[blockIdx.x]);
-}
-// adj_ugemm32 over the cases of a batch: task (size tile, order tile | case << 16).
-template <class E, int WA, int TS>
-__global__ __launch_bounds__(TS * 4 / WA) __attribute__((amdgpu_waves_per_eu(TS == 64 ? 2 : 1)))
-void adj_ugemm32_b(const UArgs *cases, const int2 *task) {
-     const int2 t = task[blockIdx.x];
-     const UArgs p = cases[t.y >> 16];
-     adj_ugemm32_body<E, WA, TS>(p, make_int2(t.x, t.y & 0xffff));
 }
 
 
@@ -3568,11 +3609,11 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
           tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
           if (shape == 4)                                                                                       \
-               hipLaunchKernelGGL((adj_ugemm32<E, 2, 128>), dim3((unsigned) task.size()), dim3(256), 0, s, u);  \
+               hipLaunchKernelGGL((adj_ugemm32<E, 2, 128, false>), dim3((unsigned) task.size()), dim3(256), 0, s, u, nullptr, nullptr);  \
           else if (shape == 2)                                                                                  \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 64>), dim3((unsigned) task.size()), dim3(256), 0, s, u);   \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 64, false>), dim3((unsigned) task.size()), dim3(256), 0, s, u, nullptr, nullptr);   \
           else                                                                                                  \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 128>), dim3((unsigned) task.size()), dim3(512), 0, s, u);  \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 128, false>), dim3((unsigned) task.size()), dim3(512), 0, s, u, nullptr, nullptr);  \
           tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);
@@ -4131,8 +4172,8 @@ int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
      hipLaunchKernelGGL(adj_gsplit_b<E>, dim3(max_g(gsp, &adj_gsplit_B::gx), max_g(gsp, &adj_gsplit_B::gy), n),
                         dim3(64), 0, s, LMIE_B(adj_gsplit_B, o_gs));
      if (!task.empty())
-          hipLaunchKernelGGL((adj_ugemm32_b<E, 1, 128>), dim3((unsigned) task.size()), dim3(512), 0, s,
-                             LMIE_B(UArgs, o_u), LMIE_B(int2, o_task));
+          hipLaunchKernelGGL((adj_ugemm32<E, 1, 128, true>), dim3((unsigned) task.size()), dim3(512), 0, s,
+                             UArgs{}, LMIE_B(UArgs, o_u), LMIE_B(int2, o_task));
      const unsigned rgy = max_g(ro, &rev_orders_B::gy);
      if (rgy > 0)
           hipLaunchKernelGGL(rev_orders_b, dim3(max_g(ro, &rev_orders_B::gx), rgy, n), dim3(64), 0, s,
