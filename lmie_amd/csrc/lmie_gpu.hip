@@ -735,16 +735,19 @@ __device__ __forceinline__ void mie_chains_body(ChainArgs p) {
           return;
      }
      // the other chains in other waves (blockIdx.y = 1): the downward ratio
-     // r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows
+     // r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows; (2j+1)/x
+     // as (2j+1) (1/x) and the refined reciprocal (about 1 ulp each, damped by the
+     // downward recurrence), so no division sits in the loops
+     const double ax = 1. / x;
      double r = 0.;
      Psi[n1 * Np + i] = 0.;
      for (int j = n1 - 1; j >= 1; --j) {
-          const double c = (2. * j + 1.) / x;
-          r = 1. / (c - r);
+          const double c = (2. * j + 1.) * ax;
+          r = rcp_nr(c - r);
           Psi[j * Np + i] = r;
      }
      // upward psi_j = r_j psi_{j-1}, chi_{j+1} = (2j+1)/x chi_j - chi_{j-1}
-     const double sx = p.sx[i], cx = p.cx[i], ax = 1. / x;
+     const double sx = p.sx[i], cx = p.cx[i];
      double psi0 = sx, chi0 = cx, chi1 = ax * cx + sx;
      Psi[i] = psi0;
      Chi[i] = chi0;
@@ -761,7 +764,7 @@ __device__ __forceinline__ void mie_chains_body(ChainArgs p) {
                     Psi[(size_t) j * Np + i] = psi;
                     Chi[(size_t) j * Np + i] = chi1;
                     const double fv1 = 2. * j + 1.;
-                    const double chi2 = fv1 / x * chi1 - chi0;
+                    const double chi2 = fv1 * ax * chi1 - chi0;
                     chi0 = chi1;
                     chi1 = chi2;
                     psi0 = psi;
