@@ -2833,6 +2833,7 @@ struct Workspace {
      PinBuf pup, pdown;
      int home = -1, busy = 0;       // batch slot of this workspace (-1: general pool)
      int n2rows = 0, ng = 0;
+     int ushape = 8, ntask = 0;     // reverse contraction: shape and task count of the pending pass
      int valid = 0;
      int N = 0, Np = 0, Kp = 0, Hp = 0, H = 0, Q = 0, C = 0, n1max = 0, n2max = 0, calc_gc = 0, calc_lc = 0;
      int mixed = 0, bf16 = 0, Kb = 0, Hb = 0;
@@ -3132,6 +3133,31 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           gauss_guesses(Q, w->hguess.data());
           HIPCHECK(hipMemcpyAsync(w->guess.p, w->hguess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
      }
+     if (mixed) {
+          // the contraction's task list, also with the inputs
+          xcd_tasks(kt64, [&](int) { return Hb / 128; }, tile_group(), w->htask);
+          if (w->order.get(sizeof(int2) * w->htask.size())) return -1;
+          HIPCHECK(hipMemcpyAsync(w->order.p, w->htask.data(), sizeof(int2) * w->htask.size(),
+                                  hipMemcpyHostToDevice, s));
+     }
+     // series dot products: channel 0, then tangent, then weight-only channels
+     std::vector<DotJob> dot_jobs;
+     {
+          if (w->S.get(sizeof(double) * 4 * Np * (1 + nt))) return -1;
+          const double *S0 = w->S.as<double>(), *f = w->f.as<double>();
+          for (int q = 0; q < 4; ++q) dot_jobs.push_back(DotJob{f, S0 + (size_t) q * Np, nullptr, nullptr});
+          for (int t = 0; t < nt; ++t)
+               for (int q = 0; q < 4; ++q)
+                    dot_jobs.push_back(DotJob{w->dft.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, f,
+                                              w->S.as<double>() + (size_t) 4 * Np * (1 + t) + (size_t) q * Np});
+          for (int t = 0; t < nw; ++t)
+               for (int q = 0; q < 4; ++q)
+                    dot_jobs.push_back(DotJob{w->dfw.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, nullptr, nullptr});
+          if (w->dotjobs.get(sizeof(DotJob) * dot_jobs.size()) || w->dots.get(sizeof(double) * dot_jobs.size()))
+               return -1;
+          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, dot_jobs.data(), sizeof(DotJob) * dot_jobs.size(),
+                                  hipMemcpyHostToDevice, s));
+     }
      tm.mark(LMIE_T_TOTAL, 0);
 
      // angle tables, on the side streams while the per-size recurrences run
@@ -3267,26 +3293,14 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      }
      tm.mark(LMIE_T_COEFS, 1);
 
-     // series sums: channel 0, then tangent, then weight-only channels
+     // series sums (jobs uploaded with the inputs)
      {
-          std::vector<DotJob> jobs;
-          const double *S0 = w->S.as<double>(), *f = w->f.as<double>();
-          for (int q = 0; q < 4; ++q) jobs.push_back(DotJob{f, S0 + (size_t) q * Np, nullptr, nullptr});
-          for (int t = 0; t < nt; ++t)
-               for (int q = 0; q < 4; ++q)
-                    jobs.push_back(DotJob{w->dft.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, f,
-                                          w->S.as<double>() + (size_t) 4 * Np * (1 + t) + (size_t) q * Np});
-          for (int t = 0; t < nw; ++t)
-               for (int q = 0; q < 4; ++q)
-                    jobs.push_back(DotJob{w->dfw.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, nullptr, nullptr});
-          if (w->dotjobs.get(sizeof(DotJob) * jobs.size()) || w->dots.get(sizeof(double) * jobs.size())) return -1;
           // on the side stream after the angle tables (beside the contraction);
           // the main stream joins it (join2) before the expansion
           HIPCHECK(hipEventRecord(w->sums, s));
           HIPCHECK(hipStreamWaitEvent(ts, w->sums, 0));
-          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs.data(), sizeof(DotJob) * jobs.size(), hipMemcpyHostToDevice, ts));
           tm.mark(LMIE_T_REDUCE, 0);
-          hipLaunchKernelGGL(weighted_dots, dim3(jobs.size()), dim3(256), 0, ts, w->dotjobs.as<DotJob>(), Np,
+          hipLaunchKernelGGL(weighted_dots, dim3(dot_jobs.size()), dim3(256), 0, ts, w->dotjobs.as<DotJob>(), Np,
                              w->dots.as<double>());
           HIPCHECK(hipGetLastError());
           HIPCHECK(hipEventRecord(w->join2, ts));
@@ -3327,13 +3341,6 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                tot = (size_t) plan[0].nblk * plan[0].nwp * 4 * Q;
           }
           if (w->part.get(sizeof(double) * tot)) return -1;
-          const int nJ = Hb / 128;
-          if (mixed) {
-               xcd_tasks(kt64, [&](int) { return nJ; }, tile_group(), w->htask);
-               if (w->order.get(sizeof(int2) * w->htask.size())) return -1;
-               HIPCHECK(hipMemcpyAsync(w->order.p, w->htask.data(), sizeof(int2) * w->htask.size(),
-                                       hipMemcpyHostToDevice, s));
-          }
           tm.mark(LMIE_T_ANGULAR, 0);
           if (mixed) {
                Mix2Args m;
@@ -3569,8 +3576,10 @@ int saved_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
 
 // Mixed reverse contraction: weights, split tables, G, its split operand and
 // U = G A (Uf, [4][Kq][Np]).
-int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
-     const int Np = w->Np, H = w->H, Q = w->Q, n1max = w->n1max;
+// Buffers, task list and depth uploads of the mixed reverse contraction, before
+// the timed pipeline (host work and transfers the kernels do not wait on).
+int adjoint_mixed_prepare(Workspace *w, hipStream_t s) {
+     const int Np = w->Np, H = w->H, n1max = w->n1max;
      const int Kq = rup(n1max, 64), Hq = rup(H, KM);
      if (rup(n1max, KM) > w->Kb * 8) return set_err("internal: split operand depth");
      if (w->wts.get(sizeof(double) * 8 * (size_t) Hq) || w->scal.get(sizeof(AdjScal)) ||
@@ -3580,18 +3589,29 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
          w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np))
           return -1;
      // (size tile of 128 or 64, order tile of 64) tasks over the XCDs
-     const int shape = ugemm_waves();
-     const std::vector<int> &kts = shape == 2 ? w->kt64 : w->kt128;
+     w->ushape = ugemm_waves();
+     const std::vector<int> &kts = w->ushape == 2 ? w->kt64 : w->kt128;
      std::vector<int2> task;
      xcd_tasks(kts, [&](int t) { return (kts[t] + 63) / 64; }, tile_group(), task);
+     w->ntask = (int) task.size();
      if (w->ktile128.get(sizeof(int) * kts.size())) return -1;
      HIPCHECK(hipMemcpyAsync(w->ktile128.p, kts.data(), sizeof(int) * kts.size(), hipMemcpyHostToDevice, s));
-     if (task.empty()) {
+     if (!task.empty()) {
+          if (w->tasks.get(sizeof(int2) * task.size())) return -1;
+          HIPCHECK(hipMemcpyAsync(w->tasks.p, task.data(), sizeof(int2) * task.size(), hipMemcpyHostToDevice, s));
+     }
+     return 0;
+}
+
+int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
+     const int Np = w->Np, H = w->H, Q = w->Q, n1max = w->n1max;
+     const int Kq = rup(n1max, 64), Hq = rup(H, KM);
+     const int shape = w->ushape;
+     const unsigned ntask = (unsigned) w->ntask;
+     if (ntask == 0) {
           HIPCHECK(hipMemsetAsync(w->Uf.p, 0, sizeof(float) * 4 * (size_t) Kq * Np, s));
           return 0;
      }
-     if (w->tasks.get(sizeof(int2) * task.size())) return -1;
-     HIPCHECK(hipMemcpyAsync(w->tasks.p, task.data(), sizeof(int2) * task.size(), hipMemcpyHostToDevice, s));
      AdjScal *sc = w->scal.as<AdjScal>();
      hipLaunchKernelGGL(adj_weights, dim3(1), dim3(1024), 0, s, (const double *) w->pfbar.as<double>(), H, Hq, Q,
                         w->wts.as<double>(), sc);
@@ -3612,11 +3632,11 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
                              (const AdjScal *) sc, Kq, w->G16.as<unsigned short>());                           \
           tm.mark(LMIE_T_ADJ_GEMM, 0);                                                                          \
           if (shape == 4)                                                                                       \
-               hipLaunchKernelGGL((adj_ugemm32<E, 2, 128, false>), dim3((unsigned) task.size()), dim3(256), 0, s, u, nullptr, nullptr);  \
+               hipLaunchKernelGGL((adj_ugemm32<E, 2, 128, false>), dim3(ntask), dim3(256), 0, s, u, nullptr, nullptr);  \
           else if (shape == 2)                                                                                  \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 64, false>), dim3((unsigned) task.size()), dim3(256), 0, s, u, nullptr, nullptr);   \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 64, false>), dim3(ntask), dim3(256), 0, s, u, nullptr, nullptr);   \
           else                                                                                                  \
-               hipLaunchKernelGGL((adj_ugemm32<E, 1, 128, false>), dim3((unsigned) task.size()), dim3(512), 0, s, u, nullptr, nullptr);  \
+               hipLaunchKernelGGL((adj_ugemm32<E, 1, 128, false>), dim3(ntask), dim3(512), 0, s, u, nullptr, nullptr);  \
           tm.mark(LMIE_T_ADJ_GEMM, 1);                                                                          \
      } while (0)
      if (w->bf16) LMIE_ADJ(BF16);
@@ -3637,7 +3657,10 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
          w->bar.get(sizeof(double) * 4 * (size_t) Kp * Np))
           return -1;
      const int use_gc = in->gc_bar && w->calc_gc, use_lc = in->lc_bar && w->calc_lc;
-     // the cotangent inputs go to the device before the timed pipeline
+     const int use_u = w->mixed && Q > 0 && (use_gc || use_lc);
+     // the cotangent inputs and the contraction's task list go to the device
+     // before the timed pipeline
+     if (use_u && adjoint_mixed_prepare(w, s)) return -1;
      double *g = w->gbar.as<double>();
      if (Q > 0 && use_gc) HIPCHECK(hipMemcpyAsync(g, in->gc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
      if (Q > 0 && use_lc)
@@ -3673,7 +3696,6 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           HIPCHECK(hipMemsetAsync(w->bar.p, 0, sizeof(double) * 4 * (size_t) Kp * Np, s));
           tm.mark(LMIE_T_ADJ_ANGULAR, 1);
      }
-     const int use_u = w->mixed && Q > 0 && (use_gc || use_lc);
      tm.mark(LMIE_T_ADJ_COEFS, 0);
      const int nch = (w->n2rows - 1 + RCH - 1) / RCH;
      if (w->revScan.get(sizeof(double) * 11 * (size_t) (nch > 0 ? nch : 1) * Np)) return -1;
