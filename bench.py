@@ -25,6 +25,8 @@ FP64 GPU forward of the same workload is timed beside it.  The JSON line reports
                (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), if present
   cpu_baseline the reference's pthreads CPU path (oracle/_ref) on a bounded
                sample of the same workload, on this host's cores
+  config.retrieval_batch  the 32-case retrieval workload in one fused batch call
+               against one call per case (wall clock; --no-batch omits it)
 
     python bench.py --calibrate-copy BYTES   one plain device copy of BYTES
                (calibration point of the traffic counters), then exit
@@ -88,8 +90,9 @@ def cpu_baseline(threads):
                       "(1/2 of the grid, same r range and 3810 angles): %.1f s" % dt}
 
 
-def retrieval_batch(lib, steps, warmup):
-    """JSON line of the 32-case retrieval workload (--workload retrieval-batch)."""
+def retrieval_batch(lib, steps, warmup, quick=False):
+    """JSON line of the 32-case retrieval workload (--workload retrieval-batch);
+    quick: only the fused batch and one call per case (a summary for the main line)."""
     from lmie_amd import lmie, workloads as W
     cases = [p for p, _ in W.cfg3_cases()]
     c_ref = [lmie.solve(lib, p, None, "fp64")["gc"] for p in cases]
@@ -117,8 +120,13 @@ def retrieval_batch(lib, steps, warmup):
         for p in cases:
             lmie.solve(lib, p, None, "fp64")
 
-    t_batch = {nw: timed(batch(nw)) for nw in (0, 1, 4, 8)}
-    t_single, t_fp64 = timed(single), timed(fp64)
+    t_batch = {nw: timed(batch(nw)) for nw in ((0,) if quick else (0, 1, 4, 8))}
+    t_single = timed(single)
+    if quick:
+        return {"workload": "32 retrieval cases, mixed forward + reverse, wall clock of whole calls",
+                "fused_batch_ms": t_batch[0] * 1e3, "one_call_per_case_ms": t_single * 1e3,
+                "grads_per_s": len(cases) / t_batch[0]}
+    t_fp64 = timed(fp64)
     dt = t_batch[0]
     return {
         "metric": METRIC, "value": units / dt, "unit": "size-pt*n-term/s", "n_gpus": 1,
@@ -149,6 +157,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-compare", action="store_true",
                     help="omit the separate plain-FP64 run (counter passes)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="omit the summary of the 32-case retrieval batch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES")
     ap.add_argument("--workload", default="two-mode", choices=["two-mode", "retrieval-batch"])
     a = ap.parse_args()
@@ -262,6 +272,8 @@ def main():
                      "kernels": {k: {"ms": v[1], "TFLOP/s": v[0] / (v[1] * 1e-3) / 1e12} for k, v in kern.items()}},
         "cpu_baseline": None,
     }
+    if a.path == "mixed-adjoint" and not a.no_batch and not a.no_fp64_compare:
+        line["config"]["retrieval_batch"] = retrieval_batch(lib, 5, 2, quick=True)
     if not a.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
