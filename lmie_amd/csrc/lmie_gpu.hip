@@ -2831,6 +2831,7 @@ struct Workspace {
      Buf sw, s1, s2;                // save_control 2: channel weights, saved per-size arrays
      Buf bargs, bup, bdown;         // per-case arguments and staging of a batch launched from here
      PinBuf pup, pdown;
+     PinBuf pres;                   // pinned landing area of a call's results (one DMA each)
      int home = -1, busy = 0;       // batch slot of this workspace (-1: general pool)
      int n2rows = 0, ng = 0;
      int ushape = 8, ntask = 0;     // reverse contraction: shape and task count of the pending pass
@@ -3133,6 +3134,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           gauss_guesses(Q, w->hguess.data());
           HIPCHECK(hipMemcpyAsync(w->guess.p, w->hguess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
      }
+     // pinned landing area of the results: expansions [2][nch][6][C], then the sums
+     if (w->pres.get(sizeof(double) * ((size_t) nch * 12 * C + 4 * nch))) return -1;
      if (mixed) {
           // the contraction's task list, also with the inputs
           xcd_tasks(kt64, [&](int) { return Hb / 128; }, tile_group(), w->htask);
@@ -3437,12 +3440,13 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           for (int gsf = 1; gsf >= 0; --gsf) {
                double *dst = gsf ? R->gc_raw : R->lc_raw;
                if (!(gsf ? J->calc_gc : J->calc_lc) || !dst) continue;
+               double *land = (double *) w->pres.p + (gsf ? 0 : (size_t) nch * 6 * C);
                hipLaunchKernelGGL(expand_rows, dim3(Hp / 64, nch), dim3(64), 0, s, (const double *) w->pf.as<double>(),
                                   nch, Q, H, Hp, (const double *) w->w.as<double>(), gsf, w->E.as<double>());
                hipLaunchKernelGGL(expand_coefs, dim3((C + 3) / 4, nch), dim3(256), 0, s, (const double *) w->T.as<double>(),
                                   (const double *) w->E.as<double>(), H, Hp, C, gsf, w->coefs.as<double>());
                HIPCHECK(hipGetLastError());
-               HIPCHECK(hipMemcpyAsync(dst, w->coefs.p, sizeof(double) * (size_t) nch * 6 * C, hipMemcpyDeviceToHost, s));
+               HIPCHECK(hipMemcpyAsync(land, w->coefs.p, sizeof(double) * (size_t) nch * 6 * C, hipMemcpyDeviceToHost, s));
           }
           tm.mark(LMIE_T_EXPAND, 1);
           if (J->save_control == 1 && R->save2)
@@ -3453,14 +3457,21 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      HIPCHECK(hipStreamWaitEvent(s, w->join2, 0));   // the series dots
      HIPCHECK(hipStreamWaitEvent(s, w->join3, 0));   // the expansion tables
      tm.mark(LMIE_T_TOTAL, 1);
-     // results
-     std::vector<double> dots((size_t) 4 * nch);
-     HIPCHECK(hipMemcpyAsync(dots.data(), w->dots.p, sizeof(double) * 4 * nch, hipMemcpyDeviceToHost, s));
+     // results: through the pinned landing area (the expansions above, the sums here)
+     const double *dots = (const double *) w->pres.p + (size_t) nch * 12 * C;
+     HIPCHECK(hipMemcpyAsync((void *) dots, w->dots.p, sizeof(double) * 4 * nch, hipMemcpyDeviceToHost, s));
      if (J->save_control == 1 && R->save1)
           for (int q = 0; q < 4; ++q)
                HIPCHECK(hipMemcpyAsync(R->save1 + (size_t) q * N, w->S.as<double>() + (size_t) q * Np,
                                        sizeof(double) * N, hipMemcpyDeviceToHost, s));
      HIPCHECK(hipStreamSynchronize(s));
+     if (Q > 0)
+          for (int gsf = 1; gsf >= 0; --gsf) {
+               double *dst = gsf ? R->gc_raw : R->lc_raw;
+               if ((gsf ? J->calc_gc : J->calc_lc) && dst)
+                    memcpy(dst, (const double *) w->pres.p + (gsf ? 0 : (size_t) nch * 6 * C),
+                           sizeof(double) * (size_t) nch * 6 * C);
+          }
      for (int q = 0; q < 4; ++q) R->sums[q] = dots[q];
      for (int c = 1; c < nch; ++c)
           for (int q = 0; q < 4; ++q) R->sums_ch[4 * (c - 1) + q] = dots[4 * c + q];
@@ -3661,6 +3672,15 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      // the cotangent inputs and the contraction's task list go to the device
      // before the timed pipeline
      if (use_u && adjoint_mixed_prepare(w, s)) return -1;
+     {
+          // the m-cotangent sums at the end of the pass
+          DotJob jobs[2] = {{nullptr, w->mbar.as<double>(), nullptr, nullptr},
+                            {nullptr, w->mbar.as<double>() + Np, nullptr, nullptr}};
+          if (w->dotjobs.get(sizeof(jobs)) || w->dots.get(sizeof(double) * 2) ||
+              w->pres.get(sizeof(double) * (2 + (size_t) N)))
+               return -1;
+          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs, sizeof(jobs), hipMemcpyHostToDevice, s));
+     }
      double *g = w->gbar.as<double>();
      if (Q > 0 && use_gc) HIPCHECK(hipMemcpyAsync(g, in->gc_bar, sizeof(double) * 6 * C, hipMemcpyHostToDevice, s));
      if (Q > 0 && use_lc)
@@ -3715,23 +3735,20 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r);
      HIPCHECK(hipGetLastError());
      tm.mark(LMIE_T_ADJ_COEFS, 1);
-     // sum of the per-size m cotangents
+     // sum of the per-size m cotangents; results through the pinned landing area
      {
-          DotJob jobs[2] = {{nullptr, w->mbar.as<double>(), nullptr, nullptr},
-                            {nullptr, w->mbar.as<double>() + Np, nullptr, nullptr}};
-          if (w->dotjobs.get(sizeof(jobs)) || w->dots.get(sizeof(double) * 2)) return -1;
-          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs, sizeof(jobs), hipMemcpyHostToDevice, s));
           hipLaunchKernelGGL(weighted_dots, dim3(2), dim3(256), 0, s, (const DotJob *) w->dotjobs.as<DotJob>(), Np,
                              w->dots.as<double>());
           HIPCHECK(hipGetLastError());
-          double mb[2];
-          HIPCHECK(hipMemcpyAsync(mb, w->dots.p, sizeof(mb), hipMemcpyDeviceToHost, s));
-          HIPCHECK(hipMemcpyAsync(out->f_bar, w->fbar.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+          double *land = (double *) w->pres.p;
+          HIPCHECK(hipMemcpyAsync(land, w->dots.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+          HIPCHECK(hipMemcpyAsync(land + 2, w->fbar.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
           tm.mark(LMIE_T_TOTAL, 1);
           HIPCHECK(hipStreamSynchronize(s));
+          memcpy(out->f_bar, land + 2, sizeof(double) * N);
           // m = mr - i mi: dL/dmr = Re(mbar), dL/dmi = -Im(mbar)
-          out->m_bar[0] = mb[0];
-          out->m_bar[1] = -mb[1];
+          out->m_bar[0] = land[0];
+          out->m_bar[1] = -land[1];
      }
      const int ids[] = {LMIE_T_ADJ_ANGULAR, LMIE_T_ADJ_COEFS, LMIE_T_TOTAL, LMIE_T_ADJ_GEMM};
      tm.collect(ids, use_u ? 4 : 3);
