@@ -22,6 +22,10 @@
 // adj_gsplit, adj_ugemm32 (the kernel matrix G and U = G A), rev_orders and
 // rev_carry (per-size reverse sweep); angular_adj_f64 on the FP64 path.
 //
+// Batches (lmie_gpu_batch_*): the same kernels in batched form (name_b, and
+// the batch flag of adj_ugemm32), one launch for all cases of a batch, with
+// copy_jobs moving the staged uploads and results.
+//
 // HBM layouts: orders are rows and sizes (padded to N_p) columns, so the 64
 // lanes of a wave (64 consecutive sizes) move 512 contiguous bytes per row;
 // split 16-bit operands are blocked by 8 along the contraction index
