@@ -3802,6 +3802,7 @@ struct Upload {
      }
      int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
           if (items.empty()) return 0;
+          if (items.size() > 65535) return set_err("batch of %zu uploads (at most 65535)", items.size());
           const size_t data = (h.size() + 255) & ~(size_t) 255, total = data + sizeof(CopyJob) * items.size();
           if (dev.get(total) || pin.get(total)) return -1;
           unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
@@ -3832,6 +3833,7 @@ struct Download {
      }
      int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
           if (items.empty()) return 0;
+          if (items.size() > 65535) return set_err("batch of %zu results (at most 65535)", items.size());
           const size_t data = (total + 255) & ~(size_t) 255, jb = sizeof(CopyJob) * items.size();
           if (dev.get(data + jb) || pin.get(data + jb)) return -1;
           unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
