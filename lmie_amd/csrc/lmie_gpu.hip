@@ -3800,6 +3800,11 @@ struct Upload {
           items.push_back(Item{0, dst, bytes / 8, true});
           return 0;
      }
+     // drops what was queued after the given sizes (a case that failed)
+     void rewind(size_t n_items, size_t n_bytes) {
+          items.resize(n_items);
+          h.resize(n_bytes);
+     }
      int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
           if (items.empty()) return 0;
           if (items.size() > 65535) return set_err("batch of %zu uploads (at most 65535)", items.size());
@@ -4288,11 +4293,14 @@ void batch_run_forward(Batch *b) {
                if (rc == 0) {
                     if (!L) L = c.w;
                     const bool bf = J->precision == LMIE_PRECISION_MIXED_BF16;
-                    rc = fused_prepare(c, bf ? upb : up16);
+                    Upload &up = bf ? upb : up16;
+                    const size_t ni = up.items.size(), nb = up.h.size();
+                    rc = fused_prepare(c, up);
                     if (rc == 0) {
                          (bf ? b16 : f16).push_back(std::move(c));
                          continue;
                     }
+                    up.rewind(ni, nb);
                     if (c.w != L) ws_release(c.w);
                }
           }
@@ -4334,11 +4342,14 @@ void batch_run_reverse(Batch *b) {
                RevCase c;
                c.slot = k; c.w = w; c.in = b->ain[k]; c.out = b->aout[k];
                if (!L) L = w;
-               rc = fused_rev_prepare(c, w->bf16 ? upb : up16);
+               Upload &up = w->bf16 ? upb : up16;
+               const size_t ni = up.items.size(), nb = up.h.size();
+               rc = fused_rev_prepare(c, up);
                if (rc == 0) {
                     (w->bf16 ? b16 : f16).push_back(std::move(c));
                     continue;
                }
+               up.rewind(ni, nb);
           }
           b->rc[1][k] = rc;
           b->err[1][k] = rc ? g_err : "";
