@@ -230,3 +230,44 @@ def test_batch_other_precisions_equal_single_calls(product, prec):
         for key in ("cext", "csca", "cbak", "g", "mr", "mi", "a1", "a2"):
             a, b = (out_b, out) if key in out else (g_b, g)
             assert a[key] == b[key], (k, key, a[key], b[key])
+
+
+def test_batch_with_a_failing_case(product):
+    """A fused batch whose middle case fails host validation: the other cases
+    still run fused and equal their single calls; the failing case is
+    reported and leaves nothing allocated."""
+    say("case: fused batch of 3, the middle case invalid")
+    cases = [dict(FD_CASE, n_int2=6), dict(FD_CASE, dist_type=42), dict(FD_CASE, n_int2=7, mr=1.5)]
+    refs = [lmie.solve(product, p, None, "fp64")["gc"] for p in (cases[0], cases[2])]
+    tw = [W.expansion_loss_targets(refs[0], 7), (0.0, 1.0), W.expansion_loss_targets(refs[1], 8)]
+    with pytest.raises(RuntimeError, match="case 1"):
+        lmie.adjoint_batch(product, cases, [t for t, _ in tw], [w for _, w in tw], "mixed")
+    assert [product.lmie_last_batch_fused(p) for p in (0, 1)] == [2, 2]
+    ins = [lmie.abi.make_in(product, p) for p in cases]
+    outs = [lmie.LmieOut() for _ in cases]
+    import ctypes as C
+    pin = (C.POINTER(lmie.LmieIn) * 3)(*[C.pointer(x) for x in ins])
+    pout = (C.POINTER(lmie.LmieOut) * 3)(*[C.pointer(o) for o in outs])
+    mc = [product.lmie_calc_max_coef(x.lambda_, x.dist_type, x.a1, x.a2, x.a3, x.r1, x.r2) for x in ins]
+    keep, l2 = [], []
+    for k in range(3):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(tw[k][0], float), (6, max(mc[k], 1))))
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(tw[k][1], float), (6, max(mc[k], 1))))
+        b = np.zeros((6, max(mc[k], 1)))
+        keep += [t, w, b]
+        dp = C.POINTER(C.c_double)
+        l2.append(lmie.LmieGcL2(t.ctypes.data_as(dp), w.ctypes.data_as(dp), b.ctypes.data_as(dp), 0.0))
+    users = (C.c_void_p * 3)(*[C.cast(C.pointer(x), C.c_void_p) for x in l2])
+    grads = (lmie.LmieGrad * 3)()
+    status = (C.c_int * 3)(7, 7, 7)
+    o = lmie._opts("mixed")
+    rc = product.lmie_solution_adjoint_batch(3, pin, pout, 1, C.byref(o), C.cast(product.lmie_cotangent_gc_l2, C.c_void_p),
+                                             users, grads, None, status, 0)
+    assert rc == -1 and list(status) == [0, -1, 0] and not bool(outs[1].gc)
+    for k, ref_k in ((0, 0), (2, 1)):
+        fake = lmie.LmieIn.from_buffer_copy(ins[k])
+        gc_b = lmie.abi.read_out(product, cases[k], fake, outs[k], mc[k])["gc"]
+        out, g = lmie.adjoint(product, [cases[k]], None, loss_fn(refs[ref_k], 7 if k == 0 else 8), "mixed")
+        assert np.array_equal(gc_b, out["gc"]), k
+        assert grads[k].mr == g["mr"] and grads[k].a1 == g["a1"], k
+        product.lmie_out_free(C.byref(outs[k]), 0)
