@@ -18,7 +18,9 @@ FP64 GPU forward of the same workload is timed beside it.  The JSON line reports
   value        sum_i n1(x_i) over the size points per second (size-pt * n-term / s)
   roofline     the dominant kernel of the step, algorithmic FLOP per launch over
                its device time (HIP events on the library's stream) against the
-               MI355X peak of its arithmetic: the forward angular contraction
+               MI355X peak of its arithmetic (executed_frac: the matrix-core work
+               actually issued, three split products per algorithmic product on
+               the mixed path): the forward angular contraction
                (8 n_qang sum_i n1(x_i)) or the reverse contraction U = G A of the
                mixed path (24 sum_i n1(x_i)^2); traffic = HBM bytes per launch
                from the committed counter summary profiles/traffic.json
@@ -269,6 +271,9 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel": kname, "kernel_ms": kms, "flop_per_launch": flop,
+                     # the split 16-bit operands take three matrix products per
+                     # algorithmic product (hi.hi, lo.hi, hi.lo)
+                     "executed_frac": (3.0 if mixed else 1.0) * achieved / peak,
                      "kernels": {k: {"ms": v[1], "TFLOP/s": v[0] / (v[1] * 1e-3) / 1e12} for k, v in kern.items()}},
         "cpu_baseline": None,
     }
