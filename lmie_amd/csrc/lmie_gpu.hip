@@ -3028,11 +3028,12 @@ int mix_waves() {
 }
 
 // Size tiles per group of xcd_tasks (environment LMIE_TILE_GROUP, read at
-// every call, default 4).
+// every call, default 1: the workgroups of one XCD take the column tiles of
+// one size tile together).
 int tile_group() {
      const char *e = getenv("LMIE_TILE_GROUP");
-     const int g = e ? atoi(e) : 4;
-     return (g < 1 || g > 64) ? 4 : g;
+     const int g = e ? atoi(e) : 1;
+     return (g < 1 || g > 64) ? 1 : g;
 }
 
 // Initial guesses of the Gauss-Legendre roots, cos(t + (n-1)/(8 n^3) / tan t)

@@ -147,7 +147,7 @@ def test_autograd_wrapper(product):
 
 @pytest.mark.parametrize("env", [("LMIE_MIX_WAVES", "4"), ("LMIE_MIX_WAVES", "32"),
                                  ("LMIE_UGEMM_WAVES", "4"), ("LMIE_UGEMM_WAVES", "2"),
-                                 ("LMIE_TILE_GROUP", "1")])
+                                 ("LMIE_TILE_GROUP", "4")])
 def test_contraction_shapes_agree(product, env):
     """Every selectable shape of the mixed forward and reverse contractions
     gives the default shape's outputs and gradients (cfg2: 16384 sizes,
