@@ -1358,10 +1358,14 @@ static void *batch_member_run(void *vp) {
      batch_member *m = vp;
      const int c = m->first + m->slot;
      g_batch_thread = 1;
+     /* a fresh thread holding a valid slot: entering cannot fail */
      int rc = lmie_gpu_batch_enter(m->gpu_batch, m->slot);
-     if (rc) rc = fail("lmie_gpu_batch_enter", lmie_gpu_last_error());
-     else rc = batch_case(m->b, c);
-     lmie_gpu_batch_leave();
+     if (rc) {
+          rc = fail("lmie_gpu_batch_enter", lmie_gpu_last_error());
+     } else {
+          rc = batch_case(m->b, c);
+          lmie_gpu_batch_leave();
+     }
      batch_record(m->b, c, rc);
      return NULL;
 }
