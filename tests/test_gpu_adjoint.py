@@ -271,3 +271,20 @@ def test_batch_with_a_failing_case(product):
         assert np.array_equal(gc_b, out["gc"]), k
         assert grads[k].mr == g["mr"] and grads[k].a1 == g["a1"], k
         product.lmie_out_free(C.byref(outs[k]), 0)
+
+
+def test_batch_larger_than_one_group(product):
+    """70 tiny cases: the fused mode runs them as two device batches (64 + 6);
+    every case equals its single call bit for bit."""
+    say("case: fused batch of 70 tiny cases (two groups)")
+    cases = [dict(FD_CASE, n_int2=2, n_quad=16, calc_lc=0, a1=0.3 + 0.01 * k, mr=1.35 + 0.002 * k,
+                  **{"lambda": 0.4 + 0.005 * k}) for k in range(70)]
+    refs = [lmie.solve(product, p, None, "fp64")["gc"] for p in cases]
+    tw = [W.expansion_loss_targets(refs[k], 200 + k) for k in range(70)]
+    res = lmie.adjoint_batch(product, cases, [t for t, _ in tw], [w for _, w in tw], "mixed")
+    assert [product.lmie_last_batch_fused(p) for p in (0, 1)] == [70, 70]
+    for k in (0, 31, 63, 64, 69):
+        out_b, g_b, _ = res[k]
+        out, g = lmie.adjoint(product, [cases[k]], None, loss_fn(refs[k], 200 + k), "mixed")
+        assert np.array_equal(out_b["gc"], out["gc"]), k
+        assert g_b["mr"] == g["mr"] and g_b["a1"] == g["a1"] and g_b["a2"] == g["a2"], k
