@@ -150,3 +150,27 @@ def test_three_size_points(product, oracle, prec):
     assert abs(r["csca"] - o["csca"]) <= 1e-12 * abs(o["csca"])
     assert relnorm(r["gc"], o["gc"]) <= tol
     assert relnorm(r["lc"], o["lc"]) <= tol
+
+
+def test_compiled_c_caller_matches_reference(product, tmp_path):
+    """tests/c/dropin_caller.c, built against the reference's interface and
+    linked to the product library, run as a separate process on the cfg0
+    inputs: every scalar and the expansion equal the reference's fixture at the
+    FP64 bar."""
+    import subprocess
+    from test_library import _build_dropin
+    say("case: compiled C caller of lmie_solution, cfg0")
+    z, p, _ = load_golden("cfg0")
+    exe = _build_dropin(tmp_path)
+    args = [str(abi.SIZE_DIST[p["dist_type"]]), str(p["n_int2"]), str(p["n_quad"])] + \
+           ["%.17g" % p[k] for k in ("lambda", "mr", "mi", "a1", "a2", "r1", "r2")]
+    txt = subprocess.run([exe] + args, check=True, capture_output=True, text=True, timeout=120).stdout
+    rows = {ln.split()[0]: ln.split()[1:] for ln in txt.splitlines()}
+    assert int(rows["n_coef"][0]) == int(z["n_coef"])
+    sc = [float(v) for v in rows["scalars"]]
+    for k, v in zip(("r1", "r2", "norm", "reff", "veff", "gavg", "vavg", "ravg", "rvw", "cext", "csca",
+                     "cbak", "g"), sc):
+        assert relnorm(v, z[k]) <= 1e-12, (k, v, float(z[k]))
+    gc = np.array([[float(v) for v in rows["gc%d" % k]] for k in range(6)])
+    n = gc.shape[1]
+    assert relnorm(gc, z["gc"][:, :n]) <= 1e-12

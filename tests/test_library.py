@@ -124,3 +124,20 @@ def test_batch_reports_each_failing_case(lib):
     status = (C.c_int * 3)(7, 7, 7)
     assert lib.lmie_solution_adjoint_batch(3, pin, pout, 1, None, cot, None, grads, None, status, 0) == -1
     assert list(status) == [-1, -1, -1] and all(not bool(o.gc) for o in outs)
+
+
+def _build_dropin(tmpdir):
+    """Compiles tests/c/dropin_caller.c against include/ and links it to the
+    product library (as an existing LMie C caller would be built)."""
+    exe = os.path.join(str(tmpdir), "dropin_caller")
+    libdir = os.path.dirname(LIB)
+    subprocess.check_call(["gcc", "-O1", "-std=gnu99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                           "-o", exe, os.path.join(ROOT, "tests", "c", "dropin_caller.c"),
+                           "-L", libdir, "-llmie_amd", "-Wl,-rpath," + libdir])
+    return exe
+
+
+def test_c_caller_builds_against_the_drop_in_abi(lib, tmp_path):
+    """A C program using only the reference's interface compiles and links
+    against the product library (it runs in tests/test_gpu_fp64.py)."""
+    assert os.path.exists(_build_dropin(tmp_path))
