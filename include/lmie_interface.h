@@ -18,6 +18,30 @@
 #ifndef LMIE_INTERFACE_H
 #define LMIE_INTERFACE_H
 
+/* Platform identifiers that the reference's headers give its callers
+   (misc/gutil.h:60-77, reached through src/lmie_interface.h:13), for code
+   built with the reference's -DPLATFORM=... flag (common.inc:5) that compares
+   against them; same values. */
+#ifndef LINUX_GCC
+#define AIX_CC       1
+#define AIX_GCC      2
+#define CYGWIN_GCC   3
+#define DEC_UNIX_CC  4
+#define DEC_UNIX_GCC 5
+#define HP_CC        6
+#define HP_GCC       7
+#define IRIX_CC      8
+#define IRIX_GCC     9
+#define LINUX_GCC    10
+#define LINUX_ICC    11
+#define LINUX_PGCC   12
+#define MACOSX_GCC   13
+#define MACOSX_XLC   14
+#define SOLARIS_CC   15
+#define SOLARIS_GCC  16
+#define WIN32_MSVC   17
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif

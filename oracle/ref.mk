@@ -24,7 +24,18 @@ SRCS     := $(REF)/src/lmie_core.c $(REF)/src/lmie_interface.c $(REF)/src/lmie_s
             $(REF)/misc/rtutil_support.c $(REF)/misc/rtutil_scat_io.c $(REF)/misc/zeroin.c
 OBJS     := $(patsubst $(REF)/%.c,$(OUT)/obj/%.o,$(SRCS))
 
-all: $(OUT)/liblmie_ref.so
+# The reference's example program (examples/example_c.c), unmodified, compiled
+# with the reference's own platform flags (common.inc:5) against this
+# repository's include/lmie_interface.h and linked to the product library: the
+# drop-in check of tests/test_gpu_fp64.py runs it on the GPU.
+EXAMPLE  := $(OUT)/example_c_amd
+
+all: $(OUT)/liblmie_ref.so $(EXAMPLE)
+
+$(EXAMPLE): $(REF)/examples/example_c.c lmie_amd/lib/liblmie_amd.so
+	@mkdir -p $(OUT)
+	$(CC) -O2 -DPLATFORM=LINUX_GCC -DUNIX -Iinclude -o $@ $(REF)/examples/example_c.c -Llmie_amd/lib -llmie_amd \
+	      -Wl,-rpath,'$$ORIGIN/../../lmie_amd/lib' -lm
 
 $(OUT)/obj/%.o: $(REF)/%.c
 	@mkdir -p $(dir $@)

@@ -174,3 +174,33 @@ def test_compiled_c_caller_matches_reference(product, tmp_path):
     gc = np.array([[float(v) for v in rows["gc%d" % k]] for k in range(6)])
     n = gc.shape[1]
     assert relnorm(gc, z["gc"][:, :n]) <= 1e-12
+
+
+def test_reference_example_program_runs_unchanged(product, tmp_path):
+    """The reference's own example program (examples/example_c.c), unmodified,
+    built against include/lmie_interface.h with the reference's platform
+    flags and linked to the product library (oracle/ref.mk, where the
+    reference sources are present): run on the GPU, its printed output equals
+    the reference's examples/example.out to the printed 7 digits."""
+    import os
+    import subprocess
+    from conftest import ROOT, GOLDEN
+    from test_oracle import _parse_example_out
+    exe = os.path.join(ROOT, "oracle", "_ref", "example_c_amd")
+    if not os.path.exists(exe):
+        pytest.skip("the reference's example was not built here (oracle/ref.mk needs the reference sources)")
+    say("case: the reference's example_c linked to the product library")
+    txt = subprocess.run([exe], check=True, capture_output=True, text=True, timeout=120).stdout
+    out = tmp_path / "example.out"
+    out.write_text(txt)
+    ours, ref = _parse_example_out(str(out)), _parse_example_out(os.path.join(GOLDEN, "example.out"))
+    assert len(ours) == len(ref) == 5
+    for b, (o, r) in enumerate(zip(ours, ref)):
+        assert o["scal"].keys() == r["scal"].keys(), b
+        for k, v in r["scal"].items():
+            assert abs(o["scal"][k] - v) <= 6e-7 * abs(v) + 1e-300, (b, k, o["scal"][k], v)
+        for key in ("gc", "pf"):
+            a, e = np.array(o[key]), np.array(r[key])
+            assert a.shape == e.shape, (b, key)
+            scale = np.abs(e).max()
+            assert np.all(np.abs(a - e) <= 6e-7 * np.abs(e) + 1e-9 * scale), (b, key)
