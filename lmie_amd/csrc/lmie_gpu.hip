@@ -1659,7 +1659,7 @@ __global__ __launch_bounds__(512) void angular_mix32(Mix2Args p) {
 // matrix elements F11, F33, F12, F34 at the Gauss angles.  Two kernels: the
 // sums over l in chunks of EXA_L (one thread per positive node j and chunk,
 // partials [chunk][24][Hp]), then the fixed-order sum over chunks.
-constexpr int EXA_L = 128;
+constexpr int EXA_L = 32;
 
 __device__ __forceinline__ void expand_adjoint_part_body(const double *T, const double *gbar, const double *lbar, int H, int Hp, int C, double *part) {
      const int j = blockIdx.x * blockDim.x + threadIdx.x, ck = blockIdx.y;
@@ -1718,17 +1718,36 @@ __global__ __launch_bounds__(64) void expand_adjoint_part_b(const expand_adjoint
 }
 
 
+// Sums of the chunk partials: a workgroup of 32 nodes x 24 rows, one thread
+// per (node, row) adding the chunks in order, then the row combinations per
+// node through LDS.
+constexpr int EXF_J = 32;
 __device__ __forceinline__ void expand_adjoint_fin_body(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) {
-     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-     if (j >= H) return;
-     double v[24];
-     for (int k = 0; k < 24; ++k) v[k] = 0.;
-     for (int ck = 0; ck < nck; ++ck)
-          for (int k = 0; k < 24; ++k) v[k] += part[((size_t) ck * 24 + k) * Hp + j];
+     __shared__ double sv[24][EXF_J];
+     const int jj = threadIdx.x % EXF_J, k = threadIdx.x / EXF_J, j = blockIdx.x * EXF_J + jj;
+     double v = 0.;
+     if (j < H) {
+          const double *pk = part + (size_t) k * Hp + j;
+          const size_t step = (size_t) 24 * Hp;
+          int ck = 0;
+          for (; ck + 8 <= nck; ck += 8) {   // eight loads in flight, added in order
+               double t[8];
+#pragma unroll
+               for (int e = 0; e < 8; ++e) t[e] = pk[(size_t) (ck + e) * step];
+#pragma unroll
+               for (int e = 0; e < 8; ++e) v += t[e];
+          }
+          for (; ck < nck; ++ck) v += pk[(size_t) ck * step];
+     }
+     sv[k][jj] = v;
+     __syncthreads();
+     if (k || j >= H) return;
+     double vv[24];
+     for (int r = 0; r < 24; ++r) vv[r] = sv[r][jj];
      const int ip = H + j, im = H - 1 - j;
      const double wq = w[ip];
      for (int s = 0; s < 2; ++s) {
-          const double *E = v + (s ? 6 : 0), *L = v + (s ? 18 : 12);
+          const double *E = vv + (s ? 6 : 0), *L = vv + (s ? 18 : 12);
           const int a = s ? ip : im;
           pfbar[a] = wq * (E[0] + E[1] + E[2] + L[0] + L[1]);
           pfbar[Q + a] = wq * (E[1] - E[2] + E[3] + L[2] + L[3]);
@@ -1736,7 +1755,7 @@ __device__ __forceinline__ void expand_adjoint_fin_body(const double *part, int 
           pfbar[3 * Q + a] = wq * (E[5] + L[5]);
      }
 }
-__global__ void expand_adjoint_fin(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) { expand_adjoint_fin_body(part, nck, w, H, Hp, Q, pfbar); }
+__global__ __launch_bounds__(24 * EXF_J) void expand_adjoint_fin(const double *part, int nck, const double *w, int H, int Hp, int Q, double *pfbar) { expand_adjoint_fin_body(part, nck, w, H, Hp, Q, pfbar); }
 struct expand_adjoint_fin_B {
      const double *part;
      int nck;
@@ -1747,7 +1766,7 @@ struct expand_adjoint_fin_B {
      double *pfbar;
      unsigned gx, gy;
 };
-__global__ void expand_adjoint_fin_b(const expand_adjoint_fin_B *cases) {
+__global__ __launch_bounds__(24 * EXF_J) void expand_adjoint_fin_b(const expand_adjoint_fin_B *cases) {
      const expand_adjoint_fin_B a = cases[blockIdx.z];
      if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
      expand_adjoint_fin_body(a.part, a.nck, a.w, a.H, a.Hp, a.Q, a.pfbar);
@@ -3699,7 +3718,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           hipLaunchKernelGGL(expand_adjoint_part, dim3(Hp / 64, nck), dim3(64), 0, s, (const double *) w->T.as<double>(),
                              use_gc ? (const double *) g : nullptr, use_lc ? (const double *) (g + 6 * C) : nullptr,
                              H, Hp, C, w->expart.as<double>());
-          hipLaunchKernelGGL(expand_adjoint_fin, dim3((H + 63) / 64), dim3(64), 0, s, (const double *) w->expart.as<double>(),
+          hipLaunchKernelGGL(expand_adjoint_fin, dim3((H + EXF_J - 1) / EXF_J), dim3(24 * EXF_J), 0, s, (const double *) w->expart.as<double>(),
                              nck, (const double *) w->w.as<double>(), H, Hp, Q, w->pfbar.as<double>());
      }
      if (w->mixed && Q > 0 && (use_gc || use_lc)) {
@@ -4177,7 +4196,7 @@ int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
           epart[k] = expand_adjoint_part_B{w->T.as<double>(), c.use_gc ? g : nullptr, c.use_lc ? g + 6 * C : nullptr,
                                            H, Hp, C, w->expart.as<double>(), (unsigned) Hp / 64, (unsigned) c.nck};
           efin[k] = expand_adjoint_fin_B{w->expart.as<double>(), c.nck, w->w.as<double>(), H, Hp, Q,
-                                         w->pfbar.as<double>(), (unsigned) (H + 63) / 64, 1};
+                                         w->pfbar.as<double>(), (unsigned) (H + EXF_J - 1) / EXF_J, 1};
           wts[k] = adj_weights_B{w->pfbar.as<double>(), H, c.Hq, Q, w->wts.as<double>(), sc, 1, 1};
           tabs[k] = adj_tables_B{w->B16.as<unsigned short>(), w->Kb, w->Hb, w->wts.as<double>(), H, c.Hq, c.Kq,
                                  w->n1max, w->L16.as<unsigned short>(), w->R16.as<unsigned short>(),
@@ -4216,7 +4235,7 @@ int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
      hipLaunchKernelGGL(expand_adjoint_part_b, dim3(max_g(epart, &expand_adjoint_part_B::gx),
                                                     max_g(epart, &expand_adjoint_part_B::gy), n),
                         dim3(64), 0, s, LMIE_B(expand_adjoint_part_B, o_ep));
-     hipLaunchKernelGGL(expand_adjoint_fin_b, dim3(max_g(efin, &expand_adjoint_fin_B::gx), 1, n), dim3(64), 0, s,
+     hipLaunchKernelGGL(expand_adjoint_fin_b, dim3(max_g(efin, &expand_adjoint_fin_B::gx), 1, n), dim3(24 * EXF_J), 0, s,
                         LMIE_B(expand_adjoint_fin_B, o_ef));
      hipLaunchKernelGGL(adj_weights_b, dim3(1, 1, n), dim3(1024), 0, s, LMIE_B(adj_weights_B, o_w));
      hipLaunchKernelGGL(adj_tables_b<E>, dim3(max_g(tabs, &adj_tables_B::gx), max_g(tabs, &adj_tables_B::gy), n),
