@@ -1018,7 +1018,24 @@ __device__ __forceinline__ void weighted_dots_body(const DotJob *jobs, int n, do
      __shared__ double red[256];
      const DotJob jb = jobs[blockIdx.x];
      double s = 0.;
-     for (int i = threadIdx.x; i < n; i += 256) {
+     int i = threadIdx.x;
+     // eight elements in flight, added in the order of the plain loop
+     for (; i + 7 * 256 < n; i += 8 * 256) {
+          double a[8], b[8], a2[8], b2[8];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+               a[e] = jb.u ? jb.u[i + 256 * e] : 1.;
+               b[e] = jb.v[i + 256 * e];
+               a2[e] = jb.u2 ? jb.u2[i + 256 * e] : 0.;
+               b2[e] = jb.u2 ? jb.v2[i + 256 * e] : 0.;
+          }
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+               s += jb.u ? a[e] * b[e] : b[e];
+               if (jb.u2) s += a2[e] * b2[e];
+          }
+     }
+     for (; i < n; i += 256) {
           s += jb.u ? jb.u[i] * jb.v[i] : jb.v[i];
           if (jb.u2) s += jb.u2[i] * jb.v2[i];
      }
@@ -2579,8 +2596,17 @@ __device__ __forceinline__ void reduce_parts_body(const double *part, int nblk, 
      const int per = nchp * 4 * Q;
      const int e = blockIdx.x * 32 + (threadIdx.x >> 3), l = threadIdx.x & 7;
      double s = 0.;
-     if (e < per)
-          for (int b = l; b < nblk; b += 8) s += part[(size_t) b * per + e];
+     if (e < per) {
+          int b = l;
+          for (; b + 56 < nblk; b += 64) {   // eight partials in flight, added in order
+               double t[8];
+#pragma unroll
+               for (int k = 0; k < 8; ++k) t[k] = part[(size_t) (b + 8 * k) * per + e];
+#pragma unroll
+               for (int k = 0; k < 8; ++k) s += t[k];
+          }
+          for (; b < nblk; b += 8) s += part[(size_t) b * per + e];
+     }
      s += __shfl_xor(s, 1);
      s += __shfl_xor(s, 2);
      s += __shfl_xor(s, 4);
