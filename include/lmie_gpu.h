@@ -68,8 +68,9 @@ int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res);
 /* Reverse pass on the state kept by lmie_gpu_forward (job->keep = 1, value
    channel only): given the cotangents of the raw sums (4) and of the raw
    expansions (gc_raw / lc_raw of channel 0, [6][n_coef], NULL = zero), return
-   per size the cotangent of f_i, and the cotangent of m as (d/d mr, d/d mi)
-   summed over sizes.  The state stays valid until lmie_gpu_release. */
+   per size the cotangents of f_i and (x_bar non-NULL) of the size parameter
+   x_i, and the cotangent of m as (d/d mr, d/d mi) summed over sizes.  The
+   state stays valid until lmie_gpu_release. */
 typedef struct {
      const double *sums_bar;  /* [4]                                              */
      const double *gc_bar;    /* [6][n_coef] or NULL                              */
@@ -79,6 +80,7 @@ typedef struct {
 typedef struct {
      double *f_bar;           /* [N]                                      (caller) */
      double m_bar[2];         /* dL/d mr, dL/d mi                                  */
+     double *x_bar;           /* [N] dL/d x_i, or NULL (not formed)       (caller) */
 } lmie_gpu_adj_out;
 
 int lmie_gpu_adjoint(void *state, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out);

@@ -129,9 +129,11 @@ typedef struct {
                            may be called again on the same context;
      lmie_adjoint_free     releases the device state.
    modes / weight / n_modes as lmie_solution_mix (n_modes = 1, weight = NULL
-   for a single distribution).  The gradient covers mr, mi and the
-   distribution parameters a1..a5; lambda, r1 and r2 are reported as NaN (their
-   reverse pass is not built).  No counterpart in the reference. */
+   for a single distribution).  The gradient covers every scalar input:
+   lambda, mr, mi, the distribution parameters a1..a5, r1 and r2 (lambda, r1,
+   r2 and grids set by a1, a2 move the size parameters x_i = 2 pi r_i / lambda,
+   whose cotangents the reverse pass forms per size).  No counterpart in the
+   reference. */
 typedef struct lmie_adj_ctx lmie_adj_ctx;
 
 int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n_modes,

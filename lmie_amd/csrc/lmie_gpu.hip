@@ -2400,10 +2400,13 @@ struct RevArgs {
      int Kq;
      const double *fang;        // [Np] (FP64 path)
      double sb0, sb1, sb2, sb3; // cotangents of the raw sums
-     double *scan;              // [nch][11][Np]: P, C0, Q, Z0 (re, im), m cotangent (re, im), f quadratic form
+     double *scan;              // [nch][REV_ROWS][Np]: P, C0, Q, Z0 (re, im), m cotangent (re, im),
+                                // f quadratic form, local x cotangent
      double *mbar;              // [2][Np]
      double *fbar;              // [Np]
+     double *xbar;              // [Np] cotangent of x_i, or null (not formed)
 };
+constexpr int REV_ROWS = 12;
 
 // Per chunk, with the carry entering it as c, every carry and D cotangent
 // inside is affine in c: carry out = P c + C0, z cotangent = Q c + Z0.
@@ -2413,7 +2416,7 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      const size_t Np = p.Np, plane = (size_t) p.Kp * Np, dpl = (size_t) p.n2rows * Np;
      const size_t ppl = (size_t) (p.n1max + 1) * Np, ld = (size_t) p.n1max + 2;
      cd P = cmk(1., 0.), C0 = cmk(0., 0.), Q = cmk(0., 0.), Z0 = cmk(0., 0.), mb = cmk(0., 0.);
-     double fq = 0.;
+     double fq = 0., xq = 0.;
      const int n1 = i < p.N ? p.n1[i] : 0, n2 = i < p.N ? p.n2[i] : 0;
      const int lo = 1 + k * RCH, hi = min(n2, lo + RCH);
      if (lo < hi) {
@@ -2444,6 +2447,7 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
           const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
           double sgn = (lo & 1) ? -1. : 1.;
+          const double ix2 = 1. / (x * x);
           for (int j = lo; j < hi; ++j) {
                // the window's next entries
                const double ps2 = rd(Psi, j + 2, j + 2 <= n1), ch2 = rd(Chi, j + 2, j + 2 <= n1);
@@ -2494,6 +2498,19 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                     const cd dua = cdiv_nr(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
                     const cd dvb = cdiv_nr(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
                     const cd ub = cmul(cconj(dua), ab), vb = cmul(cconj(dvb), bb);
+                    if (p.xbar) {
+                         // d a / d x at fixed D_j (psi_j, psi_{j-1}, chi_j, chi_{j-1}
+                         // and j / x move): with P = psi_j - a zeta_j the definition
+                         // of a gives psi_{j-1} - a zeta_{j-1} = u P, and with
+                         // psi_j' = psi_{j-1} - j psi_j / x, psi_{j-1}' = j psi_{j-1} / x - psi_j
+                         // (the same for chi) it reduces to
+                         // (d a / d u) (1 + (D / m)^2 - j (j + 1) / x^2); b likewise
+                         // with m D.  The D_j part goes through the z cotangent.
+                         const double c0 = 1. - j * (j + 1.) * ix2;
+                         const cd dm = cmul(Dj, im1), md = cmul(m, Dj);
+                         const cd wu = cmul(dm, dm), wv = cmul(md, md);
+                         xq += (c0 + wu.r) * ub.r + wu.i * ub.i + (c0 + wv.r) * vb.r + wv.i * vb.i;
+                    }
                     // u = D/m + j/x, v = m D + j/x
                     lc = cadd(cmul(cconj(im1), ub), cmul(cconj(m), vb));
                     mb = cadd(mb, cadd(cmul(cconj(cscale(cmul(Dj, im2), -1.)), ub), cmul(cconj(Dj), vb)));
@@ -2524,10 +2541,10 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      }
      // rev_carry reads the chunks below (n2 - 1 + RCH - 1) / RCH only
      if (lo >= n2) return;
-     double *o = p.scan + (size_t) k * 11 * Np + i;
+     double *o = p.scan + (size_t) k * REV_ROWS * Np + i;
      o[0] = P.r; o[Np] = P.i; o[2 * Np] = C0.r; o[3 * Np] = C0.i;
      o[4 * Np] = Q.r; o[5 * Np] = Q.i; o[6 * Np] = Z0.r; o[7 * Np] = Z0.i;
-     o[8 * Np] = mb.r; o[9 * Np] = mb.i; o[10 * Np] = fq;
+     o[8 * Np] = mb.r; o[9 * Np] = mb.i; o[10 * Np] = fq; o[11 * Np] = xq;
 }
 __global__ __launch_bounds__(64) void rev_orders(RevArgs p) { rev_orders_body(p); }
 struct rev_orders_B {
@@ -2548,23 +2565,27 @@ __device__ __forceinline__ void rev_carry_body(RevArgs p) {
      if (i >= p.N) {
           p.mbar[i] = p.mbar[Np + i] = 0.;
           p.fbar[i] = 0.;
+          if (p.xbar) p.xbar[i] = 0.;
           return;
      }
      const double x = p.x[i];
      const int nk = (p.n2[i] - 1 + RCH - 1) / RCH;
      // chain the chunks upward from j = 1 with carry 0
      cd carry = cmk(0., 0.), zb = cmk(0., 0.), mb = cmk(0., 0.);
-     double fq = 0.;
+     double fq = 0., xq = 0.;
      for (int k = 0; k < nk; ++k) {
-          const double *q = p.scan + (size_t) k * 11 * Np + i;
+          const double *q = p.scan + (size_t) k * REV_ROWS * Np + i;
           const cd P = cmk(q[0], q[Np]), C0 = cmk(q[2 * Np], q[3 * Np]);
           const cd Q = cmk(q[4 * Np], q[5 * Np]), Z0 = cmk(q[6 * Np], q[7 * Np]);
           zb = cadd(zb, cadd(cmul(Q, carry), Z0));
           carry = cadd(cmul(P, carry), C0);
           mb = cadd(mb, cmk(q[8 * Np], q[9 * Np]));
           fq += q[10 * Np];
+          xq += q[11 * Np];
      }
      mb = cadd(mb, cscale(zb, x));
+     // z = m x with m = mr - i mi: the x cotangent of z is Re(conj(m) zbar)
+     if (p.xbar) p.xbar[i] = xq + p.mr * zb.r - p.mi * zb.i;
      p.mbar[i] = mb.r;
      p.mbar[Np + i] = mb.i;
      // size-weight cotangent: angular part plus the series part
@@ -3713,7 +3734,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      hipStream_t s = w->stream;
      Timer tm{w, g_timing};
      if (w->pfbar.get(sizeof(double) * 4 * (size_t) (Q > 0 ? Q : 1)) || w->fang.get(sizeof(double) * Np) ||
-         w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * Np) ||
+         w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * 2 * Np) ||
          w->gbar.get(sizeof(double) * 12 * (size_t) (C > 0 ? C : 1)) ||
          w->bar.get(sizeof(double) * 4 * (size_t) Kp * Np))
           return -1;
@@ -3727,7 +3748,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           DotJob jobs[2] = {{nullptr, w->mbar.as<double>(), nullptr, nullptr},
                             {nullptr, w->mbar.as<double>() + Np, nullptr, nullptr}};
           if (w->dotjobs.get(sizeof(jobs)) || w->dots.get(sizeof(double) * 2) ||
-              w->pres.get(sizeof(double) * (2 + (size_t) N)))
+              w->pres.get(sizeof(double) * (2 + 2 * (size_t) N)))
                return -1;
           HIPCHECK(hipMemcpyAsync(w->dotjobs.p, jobs, sizeof(jobs), hipMemcpyHostToDevice, s));
      }
@@ -3768,7 +3789,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      }
      tm.mark(LMIE_T_ADJ_COEFS, 0);
      const int nch = (w->n2rows - 1 + RCH - 1) / RCH;
-     if (w->revScan.get(sizeof(double) * 11 * (size_t) (nch > 0 ? nch : 1) * Np)) return -1;
+     if (w->revScan.get(sizeof(double) * REV_ROWS * (size_t) (nch > 0 ? nch : 1) * Np)) return -1;
      RevArgs r;
      r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.nch = nch;
      r.ft = w->ftab.as<double>();
@@ -3781,6 +3802,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.sb0 = in->sums_bar[0]; r.sb1 = in->sums_bar[1]; r.sb2 = in->sums_bar[2]; r.sb3 = in->sums_bar[3];
      r.scan = w->revScan.as<double>();
      r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
+     r.xbar = out->x_bar ? w->fbar.as<double>() + Np : nullptr;
      if (nch > 0) hipLaunchKernelGGL(rev_orders, dim3(Np / 64, nch), dim3(64), 0, s, r);
      hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r);
      HIPCHECK(hipGetLastError());
@@ -3793,9 +3815,13 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
           double *land = (double *) w->pres.p;
           HIPCHECK(hipMemcpyAsync(land, w->dots.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
           HIPCHECK(hipMemcpyAsync(land + 2, w->fbar.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+          if (out->x_bar)
+               HIPCHECK(hipMemcpyAsync(land + 2 + N, w->fbar.as<double>() + Np, sizeof(double) * N,
+                                       hipMemcpyDeviceToHost, s));
           tm.mark(LMIE_T_TOTAL, 1);
           HIPCHECK(hipStreamSynchronize(s));
           memcpy(out->f_bar, land + 2, sizeof(double) * N);
+          if (out->x_bar) memcpy(out->x_bar, land + 2 + N, sizeof(double) * N);
           // m = mr - i mi: dL/dmr = Re(mbar), dL/dmi = -Im(mbar)
           out->m_bar[0] = land[0];
           out->m_bar[1] = -land[1];
@@ -4170,7 +4196,7 @@ int fused_rev_prepare(RevCase &c, Upload &up) {
      c.nch = (w->n2rows - 1 + RCH - 1) / RCH;
      const size_t nk = w->kt128.size(), nk2 = nk + (nk & 1);   // whole doubles
      if (w->pfbar.get(sizeof(double) * 4 * (size_t) Q) || w->fang.get(sizeof(double) * Np) ||
-         w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * Np) ||
+         w->mbar.get(sizeof(double) * 2 * Np) || w->fbar.get(sizeof(double) * 2 * Np) ||
          w->gbar.get(sizeof(double) * 12 * (size_t) C) ||
          w->expart.get(sizeof(double) * 24 * (size_t) c.nck * Hp) || w->wts.get(sizeof(double) * 8 * (size_t) Hq) ||
          w->scal.get(sizeof(AdjScal)) || w->L16.get(sizeof(unsigned short) * 4 * (size_t) Hq * Kq) ||
@@ -4178,7 +4204,7 @@ int fused_rev_prepare(RevCase &c, Upload &up) {
          w->Gf.get(sizeof(float) * 4 * (size_t) Kq * Kq) ||
          w->G16.get(sizeof(unsigned short) * 12 * (size_t) Kq * Kq) ||
          w->Uf.get(sizeof(float) * 4 * (size_t) Kq * Np) ||
-         w->revScan.get(sizeof(double) * 11 * (size_t) (c.nch > 0 ? c.nch : 1) * Np) ||
+         w->revScan.get(sizeof(double) * REV_ROWS * (size_t) (c.nch > 0 ? c.nch : 1) * Np) ||
          w->ktile128.get(sizeof(int) * nk2) || w->dotjobs.get(sizeof(DotJob) * 2) || w->dots.get(sizeof(double) * 2))
           return -1;
      double *g = w->gbar.as<double>();
@@ -4246,6 +4272,7 @@ int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
           r.Uf = w->Uf.as<float>(); r.Kq = c.Kq; r.fang = w->fang.as<double>();
           r.sb0 = c.in->sums_bar[0]; r.sb1 = c.in->sums_bar[1]; r.sb2 = c.in->sums_bar[2]; r.sb3 = c.in->sums_bar[3];
           r.scan = w->revScan.as<double>(); r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
+          r.xbar = c.out->x_bar ? w->fbar.as<double>() + Np : nullptr;
           ro[k] = rev_orders_B{r, (unsigned) Np / 64, (unsigned) c.nch};
           rc[k] = rev_carry_B{r, (unsigned) Np / 64, 1};
           dots[k] = weighted_dots_B{w->dotjobs.as<DotJob>(), Np, w->dots.as<double>(), 2, 1};
@@ -4286,7 +4313,9 @@ int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
      Download dn;
      for (unsigned k = 0; k < n; ++k)
           if (dn.add(mb.data() + 2 * k, cs[k].w->dots.p, sizeof(double) * 2) ||
-              dn.add(cs[k].out->f_bar, cs[k].w->fbar.p, sizeof(double) * cs[k].w->N))
+              dn.add(cs[k].out->f_bar, cs[k].w->fbar.p, sizeof(double) * cs[k].w->N) ||
+              (cs[k].out->x_bar &&
+               dn.add(cs[k].out->x_bar, cs[k].w->fbar.as<double>() + cs[k].w->Np, sizeof(double) * cs[k].w->N)))
                return -1;
      if (dn.flush(L->bdown, L->pdown, s)) return -1;
      for (unsigned k = 0; k < n; ++k) {

@@ -1117,51 +1117,73 @@ int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_mo
 /* Reverse mode.  The forward keeps its device state; the reverse pass pulls
    the output cotangents back through the normalisation here
    (src/lmie_schedule.c:1158-1199 read backwards), through the kernels to
-   f_i and m, and from f_i to the distribution parameters with the number
-   density tangents of each mode. */
+   f_i, x_i and m, and from f_i = qw_i sum_k w_k n_k(r_i) and
+   x_i = 2 pi r_i / lambda to the distribution parameters, r1 and r2 with the
+   grid tangents of each mode. */
+#define ADJ_SEEDS 7                 /* a1 .. a5, r1, r2 */
 struct lmie_adj_ctx {
      void *state;
      int n_modes, N, C, calc_gc, calc_lc;
-     double lambda, Csca, g;
+     double lambda, Csca, g, cext, csca, cbak;
      double *gc, *lc;               /* normalised outputs [6][C] */
      double *w;                     /* mode weights */
-     double **dn;                   /* per mode: w_k qw_i d n_k(r_i) / d a_p, [N][5] */
+     double **df;                   /* per mode: d (w_k qw_i n_k(r_i)) / d seed, [N][ADJ_SEEDS] */
+     double **dx;                   /* per mode: d x_i / d seed, [N][ADJ_SEEDS] */
+     double *xl;                    /* [N] x_i / lambda */
 };
+
+static void adj_tangents_free(double **v, int n) {
+     if (v)
+          for (int k = 0; k < n; ++k) free(v[k]);
+     free(v);
+}
 
 void lmie_adjoint_free(lmie_adj_ctx *ctx) {
      if (!ctx) return;
      lmie_gpu_release(ctx->state);
-     free(ctx->gc); free(ctx->lc); free(ctx->w);
-     if (ctx->dn)
-          for (int k = 0; k < ctx->n_modes; ++k) free(ctx->dn[k]);
-     free(ctx->dn);
+     free(ctx->gc); free(ctx->lc); free(ctx->w); free(ctx->xl);
+     adj_tangents_free(ctx->df, ctx->n_modes);
+     adj_tangents_free(ctx->dx, ctx->n_modes);
      free(ctx);
 }
 
-/* Per mode k: w_k qw_i d n_k(r_i) / d a_p, [N][5] (unit seeds on a1..a5). */
+/* Per mode k, unit seeds on a1..a5, r1, r2: the tangents of the size weight
+   w_k qw_i n_k(r_i) (node weight and normalised density both move) and of
+   x_i = 2 pi r_i / lambda. */
 typedef struct {
      lmie_in_data *const *modes;
      const double *weight;
      int n_modes, N;
-     double **dn;
+     double lambda;
+     double **df, **dx;
 } dn_job;
 
 static void *dn_build(void *vp) {
      dn_job *j = vp;
      const lmie_in_data *m0 = j->modes[0];
-     j->dn = calloc(j->n_modes, sizeof(double *));
+     const int S = ADJ_SEEDS;
+     j->df = calloc(j->n_modes, sizeof(double *));
+     j->dx = calloc(j->n_modes, sizeof(double *));
      for (int k = 0; k < j->n_modes; ++k) {
           const lmie_in_data *m = j->modes[k];
           const double w = j->weight ? j->weight[k] : 1.;
-          double e[5][5] = {{1, 0, 0, 0, 0}, {0, 1, 0, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 1, 0}, {0, 0, 0, 0, 1}};
-          seeds sv = {5, {e[0], e[1], e[2], e[3], e[4]}, NULL, NULL};
+          double e[ADJ_SEEDS][ADJ_SEEDS];
+          memset(e, 0, sizeof e);
+          for (int q = 0; q < S; ++q) e[q][q] = 1.;
+          seeds sv = {S, {e[0], e[1], e[2], e[3], e[4]}, e[5], e[6]};
           double a[5] = {m->a1, m->a2, m->a3, m->a4, m->a5};
           grid_t gk;
           memset(&gk, 0, sizeof gk);
-          j->dn[k] = vec((size_t) 5 * j->N);
-          if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0)
+          j->df[k] = vec((size_t) S * j->N);
+          j->dx[k] = vec((size_t) S * j->N);
+          if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0 &&
+              gk.N == j->N)
                for (int i = 0; i < gk.N; ++i)
-                    for (int q = 0; q < 5; ++q) j->dn[k][(size_t) i * 5 + q] = w * gk.nr_l[(size_t) i * 5 + q] * gk.qw[i];
+                    for (int q = 0; q < S; ++q) {
+                         const size_t o = (size_t) i * S + q;
+                         j->df[k][o] = w * (gk.nr_l[o] * gk.qw[i] + gk.nr[i] * gk.qw_l[o]);
+                         j->dx[k][o] = 2. * LM_PI * gk.qx_l[o] / j->lambda;
+                    }
           grid_free(&gk);
      }
      return NULL;
@@ -1199,7 +1221,7 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      }
      /* the number density tangents of the modes (for the reverse pass) on a
         helper thread while the forward runs on the GPU */
-     dn_job dj = {modes, weight, n_modes, g.N, NULL};
+     dn_job dj = {modes, weight, n_modes, g.N, m0->lambda, NULL, NULL};
      pthread_t dth;
      const int dthr = !g_batch_thread && pthread_create(&dth, NULL, dn_build, &dj) == 0;
      if (!dthr) dn_build(&dj);
@@ -1213,8 +1235,8 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
                      NULL, NULL, NULL, NULL, NULL, NULL, NULL, &out->save1, &out->save2, &keep);
      if (dthr) pthread_join(dth, NULL);
      if (rc) {
-          for (int k = 0; k < n_modes; ++k) free(dj.dn[k]);
-          free(dj.dn);
+          adj_tangents_free(dj.df, n_modes);
+          adj_tangents_free(dj.dx, n_modes);
           grid_free(&g);
           if (alloc_out) out_release(out);
           return -1;
@@ -1224,11 +1246,14 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      c->n_modes = n_modes; c->N = g.N; c->C = keep.n_coef;
      c->calc_gc = m0->calc_gc; c->calc_lc = m0->calc_lc;
      c->lambda = m0->lambda; c->Csca = keep.sums[1]; c->g = out->g;
+     c->cext = out->cext; c->csca = out->csca; c->cbak = out->cbak;
      if (c->calc_gc) { c->gc = vec((size_t) 6 * c->C); memcpy(c->gc, out->gc[0], sizeof(double) * 6 * c->C); }
      if (c->calc_lc) { c->lc = vec((size_t) 6 * c->C); memcpy(c->lc, out->lc[0], sizeof(double) * 6 * c->C); }
      c->w = vec(n_modes);
      for (int k = 0; k < n_modes; ++k) c->w[k] = weight ? weight[k] : 1.;
-     c->dn = dj.dn;
+     c->df = dj.df; c->dx = dj.dx;
+     c->xl = vec(g.N);
+     for (int i = 0; i < g.N; ++i) c->xl[i] = 2. * LM_PI * g.qx[i] / (m0->lambda * m0->lambda);
      grid_free(&g);
      *ctxp = c;
      return 0;
@@ -1262,24 +1287,41 @@ int lmie_adjoint_reverse(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *gra
      lmie_gpu_adj_in ain = {sb, gcb, lcb};
      lmie_gpu_adj_out aout;
      aout.f_bar = vec(c->N);
+     aout.x_bar = vec(c->N);
      int rc = lmie_gpu_adjoint(c->state, &ain, &aout);
      if (rc) {
           fail("lmie_gpu_adjoint", lmie_gpu_last_error());
      } else {
-          grad->lambda = NAN; grad->r1 = NAN; grad->r2 = NAN;
+          const int S = ADJ_SEEDS;
           grad->mr = aout.m_bar[0];
           grad->mi = aout.m_bar[1];
-          /* f_i = qw_i sum_k w_k n_k(r_i): a_p of mode k enters through n_k */
-          double *gp = vec((size_t) 5 * c->n_modes);
+          /* a_p of mode k moves f_i through n_k (and, for grids set by the
+             parameters, the nodes and weights); r1 and r2 move the shared grid
+             of every mode: its weights and densities per mode, its nodes once */
+          double *gp = vec((size_t) 5 * c->n_modes), gr[2] = {0., 0.}, lx = 0.;
           for (int k = 0; k < c->n_modes; ++k)
-               for (int i = 0; i < c->N; ++i)
-                    for (int q = 0; q < 5; ++q) gp[5 * k + q] += aout.f_bar[i] * c->dn[k][(size_t) i * 5 + q];
+               for (int i = 0; i < c->N; ++i) {
+                    const double *df = c->df[k] + (size_t) i * S, *dx = c->dx[k] + (size_t) i * S;
+                    for (int q = 0; q < 5; ++q) gp[5 * k + q] += aout.f_bar[i] * df[q] + aout.x_bar[i] * dx[q];
+                    gr[0] += aout.f_bar[i] * df[5];
+                    gr[1] += aout.f_bar[i] * df[6];
+               }
+          for (int i = 0; i < c->N; ++i) {
+               gr[0] += aout.x_bar[i] * c->dx[0][(size_t) i * S + 5];
+               gr[1] += aout.x_bar[i] * c->dx[0][(size_t) i * S + 6];
+               lx += aout.x_bar[i] * c->xl[i];
+          }
+          /* lambda: x_i = 2 pi r_i / lambda, and cext, csca, cbak carry
+             lambda^2 / (2 pi) (src/lmie_schedule.c:1158-1199) */
+          grad->lambda = 2. / c->lambda * (adj->cext * c->cext + adj->csca * c->csca + adj->cbak * c->cbak) - lx;
+          grad->r1 = gr[0];
+          grad->r2 = gr[1];
           grad->a1 = gp[0]; grad->a2 = gp[1]; grad->a3 = gp[2]; grad->a4 = gp[3]; grad->a5 = gp[4];
           if (grad_modes)
                for (int q = 0; q < 5 * c->n_modes; ++q) grad_modes[q] = gp[q];
           free(gp);
      }
-     free(aout.f_bar); free(gcb); free(lcb);
+     free(aout.f_bar); free(aout.x_bar); free(gcb); free(lcb);
      return rc ? -1 : 0;
 }
 

@@ -5,7 +5,6 @@
 Loss (workloads.expansion_loss_targets): L = sum_k w_k sum_l (c_kl - t_kl)^2,
 t = c_ref (1 + z), z ~ N(0,1), w_k = 1 / ||c_ref_k||^2.
 Per-component bar: |g - g_ref| <= tol * max(|g_ref|, 1e-3 ||g_ref||)."""
-import math
 import os
 
 import numpy as np
@@ -98,7 +97,7 @@ def test_adjoint_against_central_differences(product, part):
         return (L(lmie.solve(product, dict(p, **{key: p[key] + h}), None, "fp64")) -
                 L(lmie.solve(product, dict(p, **{key: p[key] - h}), None, "fp64"))) / (2 * h)
 
-    for key in ("mr", "mi", "a1", "a2"):
+    for key in ("mr", "mi", "a1", "a2", "lambda", "r1", "r2"):
         # Richardson on two steps cancels the h^2 term: the Mie resonances make
         # the third derivative in mr large enough that a plain central
         # difference at h = 1e-5 mr is off by ~1e-5 relative
@@ -109,7 +108,38 @@ def test_adjoint_against_central_differences(product, part):
         if err > 1e-6:
             bad.append(key)
     assert not bad, bad
-    assert math.isnan(g["lambda"]) and math.isnan(g["r1"]) and math.isnan(g["r2"])
+
+
+PL_CASE = dict(FD_CASE, dist_type="power_law", n_int2=4, n_quad=32, a1=0.8, a2=0.15, r1=0.0, r2=0.0)
+
+
+@pytest.mark.parametrize("case", ["log_normal", "power_law"])
+def test_adjoint_every_input_matches_forward_tangents(product, case):
+    """The reverse pass against the FP64 forward tangents of all ten scalar
+    inputs (lambda, m, a1..a5, r1, r2) contracted with one random cotangent on
+    every output.  The power law's grid (r1, r2, nodes, weights) is set by a1
+    and a2, so there a1 and a2 also move the size parameters."""
+    p = FD_CASE if case == "log_normal" else PL_CASE
+    say("case: adjoint vs forward tangents of every input,", case)
+    names = ("lambda", "mr", "mi", "a1", "a2", "a3", "a4", "a5", "r1", "r2")
+    eye = np.eye(len(names))
+    seeds = {k: list(eye[i]) for i, k in enumerate(names)}
+    tan = lmie.solve(product, p, seeds, "fp64")
+    rng = np.random.Generator(np.random.PCG64(13))
+    cot = {k: float(rng.standard_normal()) for k in ("cext", "csca", "cbak", "g")}
+    for k in ("gc", "lc"):
+        cot[k] = rng.standard_normal(tan[k].shape) / np.abs(tan[k]).max(axis=1, keepdims=True)
+    ref = np.zeros(len(names))
+    for k in ("cext", "csca", "cbak", "g"):
+        ref += cot[k] * tan[k + "_l"]
+    for k in ("gc", "lc"):
+        ref += np.einsum("dkl,kl->d", tan[k + "_l"], cot[k])
+    for prec, tol in (("fp64", TOL["fp64"]), ("mixed", TOL["mixed"])):
+        _, g = lmie.adjoint(product, [p], None, lambda o: cot, prec)
+        ours = np.array([g[k] for k in names])
+        say("  %s lambda %.6e r1 %.6e r2 %.6e (tangents %.6e %.6e %.6e)" % (prec, g["lambda"], g["r1"], g["r2"],
+                                                                             ref[0], ref[8], ref[9]))
+        check(ours, ref, tol, "%s %s" % (case, prec))
 
 
 @pytest.mark.parametrize("part", ["cext", "csca", "cbak", "g", "gc", "lc"])
@@ -127,7 +157,7 @@ def test_mixed_adjoint_matches_fp64_adjoint(product, part):
         cot[part] = 1.0
     _, g64 = lmie.adjoint(product, [p], None, lambda o: cot, "fp64")
     _, gmx = lmie.adjoint(product, [p], None, lambda o: cot, "mixed")
-    keys = ("mr", "mi", "a1", "a2")
+    keys = ("mr", "mi", "a1", "a2", "lambda", "r1", "r2")
     check([gmx[k] for k in keys], [g64[k] for k in keys], TOL["mixed"], "mixed " + part)
 
 
