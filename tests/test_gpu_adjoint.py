@@ -173,6 +173,12 @@ def test_autograd_wrapper(product):
     out, g = lmie.adjoint(product, [p], None, lambda o: {"gc": 2.0 * o["gc"]}, "fp64")
     ref = np.array([g["mr"], g["mi"], g["a1"], g["a2"]])
     assert np.allclose(theta.grad.numpy(), ref, rtol=1e-12, atol=0)
+    # with the wavelength and the radius range
+    theta = torch.tensor([p["mr"], p["mi"], p["a1"], p["a2"], p["lambda"], p["r1"], p["r2"]], dtype=torch.float64,
+                         requires_grad=True)
+    (autograd.expansion(theta, product, [p], grid=True) ** 2).sum().backward()
+    ref = np.array([g["mr"], g["mi"], g["a1"], g["a2"], g["lambda"], g["r1"], g["r2"]])
+    assert np.allclose(theta.grad.numpy(), ref, rtol=1e-12, atol=0)
 
 
 @pytest.mark.parametrize("env", [("LMIE_MIX_WAVES", "4"), ("LMIE_MIX_WAVES", "32"),
