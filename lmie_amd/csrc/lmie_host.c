@@ -1176,13 +1176,19 @@ static void *dn_build(void *vp) {
           memset(&gk, 0, sizeof gk);
           j->df[k] = vec((size_t) S * j->N);
           j->dx[k] = vec((size_t) S * j->N);
+          /* a seed moves the size parameters only where the forward makes it a
+             Mie tangent channel: where it moves the radius range
+             (src/lmie_interface.c:327-331), so the mono distribution's a1,
+             whose node moves with r1, r2 held (misc/rtutil_size_dist.c:267-290),
+             acts on the weights only, as in the reference's tangents */
           if (grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk) == 0 &&
               gk.N == j->N)
                for (int i = 0; i < gk.N; ++i)
                     for (int q = 0; q < S; ++q) {
                          const size_t o = (size_t) i * S + q;
+                         const int moves = gk.r1_l[q] != 0. || gk.r2_l[q] != 0.;
                          j->df[k][o] = w * (gk.nr_l[o] * gk.qw[i] + gk.nr[i] * gk.qw_l[o]);
-                         j->dx[k][o] = 2. * LM_PI * gk.qx_l[o] / j->lambda;
+                         j->dx[k][o] = moves ? 2. * LM_PI * gk.qx_l[o] / j->lambda : 0.;
                     }
           grid_free(&gk);
      }

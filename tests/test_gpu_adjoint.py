@@ -111,15 +111,19 @@ def test_adjoint_against_central_differences(product, part):
 
 
 PL_CASE = dict(FD_CASE, dist_type="power_law", n_int2=4, n_quad=32, a1=0.8, a2=0.15, r1=0.0, r2=0.0)
+MONO_CASE = dict(FD_CASE, dist_type="mono", n_int2=1, n_quad=1, a1=1.3, a2=0.0, r1=1.3, r2=1.3)
 
 
-@pytest.mark.parametrize("case", ["log_normal", "power_law"])
+@pytest.mark.parametrize("case", ["log_normal", "power_law", "mono"])
 def test_adjoint_every_input_matches_forward_tangents(product, case):
     """The reverse pass against the FP64 forward tangents of all ten scalar
     inputs (lambda, m, a1..a5, r1, r2) contracted with one random cotangent on
     every output.  The power law's grid (r1, r2, nodes, weights) is set by a1
-    and a2, so there a1 and a2 also move the size parameters."""
-    p = FD_CASE if case == "log_normal" else PL_CASE
+    and a2, so there a1 and a2 also move the size parameters.  The mono
+    distribution's one size is a1, but a1 does not move its range, so its
+    tangent acts on the weight only, as in the reference's forward tangents
+    (src/lmie_interface.c:327-331): the reverse pass follows that."""
+    p = {"log_normal": FD_CASE, "power_law": PL_CASE, "mono": MONO_CASE}[case]
     say("case: adjoint vs forward tangents of every input,", case)
     names = ("lambda", "mr", "mi", "a1", "a2", "a3", "a4", "a5", "r1", "r2")
     eye = np.eye(len(names))
@@ -236,7 +240,7 @@ def test_batch_of_cfg3_cases_in_one_call(product, workers):
         for k in ("cext", "csca", "cbak", "g"):
             assert out_b[k] == out[k], (c, k)
         assert np.array_equal(out_b["gc"], out["gc"]), c
-        for k in ("mr", "mi", "a1", "a2"):
+        for k in ("lambda", "mr", "mi", "a1", "a2", "r1", "r2"):
             assert g_b[k] == g[k], (c, k, g_b[k], g[k])
         t, w = tw[c]
         loss = float(np.sum(w * (out["gc"] - t) ** 2))
@@ -263,9 +267,10 @@ def test_batch_other_precisions_equal_single_calls(product, prec):
         out_b, g_b, _ = res[k]
         out, g = lmie.adjoint(product, [p], None, loss_fn(refs[k], 50 + k), prec)
         assert np.array_equal(out_b["gc"], out["gc"]) and np.array_equal(out_b["lc"], out["lc"]), k
-        for key in ("cext", "csca", "cbak", "g", "mr", "mi", "a1", "a2"):
-            a, b = (out_b, out) if key in out else (g_b, g)
-            assert a[key] == b[key], (k, key, a[key], b[key])
+        for key in ("cext", "csca", "cbak", "g"):
+            assert out_b[key] == out[key], (k, key, out_b[key], out[key])
+        for key in ("lambda", "mr", "mi", "a1", "a2", "r1", "r2"):
+            assert g_b[key] == g[key], (k, key, g_b[key], g[key])
 
 
 def test_batch_with_a_failing_case(product):
