@@ -2383,7 +2383,8 @@ int ugemm_waves() {
 //              which the chunk returns as an affine function of the carry
 //              entering it;
 //   rev_carry  one thread per size: chains the chunks, then the per-size totals.
-// Outputs per size: the cotangent of m (re, im as d/dRe + i d/dIm) and of f_i.
+// Outputs per size: the cotangent of m (re, im as d/dRe + i d/dIm), of f_i and
+// (when asked for) of x_i.
 constexpr int RCH = 64;
 
 struct RevArgs {
