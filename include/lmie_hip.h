@@ -29,6 +29,8 @@
 #ifndef LMIE_HIP_H
 #define LMIE_HIP_H
 
+#include <stdio.h>
+
 #include "lmie_interface.h"
 
 #ifdef __cplusplus
@@ -187,6 +189,18 @@ typedef struct {
      double loss;           /* written */
 } lmie_gc_l2;
 int lmie_cotangent_gc_l2(const lmie_out_data *out, lmie_adj_in *adj, void *user);
+
+/* Scattering files in the reference's calllmie text format (the default build
+   of misc/rtutil_scat_io.c): header (misc/rtutil_scat_io.c:250-294), expansion
+   coefficients (:823-889) and phase function (:636-702).  The header holds the
+   inputs of in and the radius range and statistics of out, as
+   utils/calllmie.c:758-760 fills it.  legendre: 0 = out->gc, 1 = out->lc;
+   deriv: -1 = the values, d >= 0 = the tangents of seed d (gc_l / lc_l /
+   pf_l).  The coefficient file has out->n_coef rows, the phase function
+   in->n_angles. */
+int lmie_scat_header_write(FILE *fp, const lmie_in_data *in, const lmie_out_data *out);
+int lmie_scat_coefs_write(FILE *fp, const lmie_in_data *in, const lmie_out_data *out, int legendre, int deriv);
+int lmie_phase_func_write(FILE *fp, const lmie_in_data *in, const lmie_out_data *out, int deriv);
 
 /* (3) Text of the last error on this thread ("" if none). */
 const char *lmie_last_error(void);

@@ -8,6 +8,8 @@ import os
 import re
 import subprocess
 
+import numpy as np
+
 import pytest
 
 from conftest import ROOT
@@ -141,3 +143,82 @@ def test_c_caller_builds_against_the_drop_in_abi(lib, tmp_path):
     """A C program using only the reference's interface compiles and links
     against the product library (it runs in tests/test_gpu_fp64.py)."""
     assert os.path.exists(_build_dropin(tmp_path))
+
+
+def test_scattering_files_match_the_reference_writers(lib, reflib, tmp_path):
+    """The calllmie text files written by the library (header, gc and lc
+    coefficients, phase function; the values and a tangent) equal byte for byte
+    those of the reference's own writers (misc/rtutil_scat_io.c, built in
+    oracle/_ref) on the same data, with values over the whole exponent range."""
+    libc = C.CDLL(None)
+    libc.fopen.restype, libc.fopen.argtypes = C.c_void_p, [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    DPP = C.POINTER(C.POINTER(C.c_double))
+    for name, args in (("lmie_scat_coefs_write", [C.c_int, C.c_int]), ("lmie_phase_func_write", [C.c_int])):
+        getattr(lib, name).argtypes = [C.c_void_p, C.POINTER(abi.LmieIn), C.POINTER(abi.LmieOut)] + args
+        getattr(lib, name).restype = C.c_int
+    reflib.scat_header_import.argtypes = [C.c_void_p] + [C.c_double] * 3 + [C.c_int] + [C.c_double] * 7 + \
+        [C.c_int] * 3 + [C.c_double] * 12
+    reflib.scat_header_import.restype = None
+    reflib.scat_coefs_write_fp.argtypes = [C.c_void_p, C.c_int, C.c_void_p, DPP, C.c_int]
+    reflib.phase_func_write_fp.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_double), DPP, C.c_int, C.c_int]
+    p = dict(dist_type="power_law", n_int2=2, n_quad=8, calc_gc=1, calc_lc=1, calc_pf=1, n_angles=7,
+             **{"lambda": 0.63}, mr=1.5, mi=0.01, a1=0.8, a2=0.15, r1=0.0, r2=0.0, accuracy=1e-7)
+    inp = abi.make_in(lib, p, {"mr": [1.0, 0.0], "a1": [0.0, 1.0]})
+    out = abi.LmieOut()
+    max_coef, n_coef = 40, 37
+    assert lib.lmie_out_alloc(C.byref(inp), C.byref(out), max_coef) == 0
+    rng = np.random.Generator(np.random.PCG64(5))
+
+    def val():
+        return float(rng.standard_normal() * 10.0 ** int(rng.integers(-300, 300)))
+
+    out.n_coef = n_coef
+    for k in ("r1", "r2", "norm", "reff", "veff", "gavg", "vavg", "ravg", "rvw", "cext", "csca", "cbak", "g"):
+        setattr(out, k, val())
+    for i in range(p["n_angles"]):
+        out.theta[i] = float(rng.uniform(0, 180))
+    for k in range(6):
+        for i in range(max_coef):
+            out.gc[k][i] = val(); out.lc[k][i] = val()
+            for d in range(2):
+                out.gc_l[d][k][i] = val(); out.lc_l[d][k][i] = val()
+        for i in range(p["n_angles"]):
+            out.pf[k][i] = val()
+            for d in range(2):
+                out.pf_l[d][k][i] = val()
+    h = C.create_string_buffer(4096)
+    reflib.scat_header_import(h, inp.lambda_, inp.mr, inp.mi, inp.dist_type, inp.a1, inp.a2, inp.a3, inp.a4,
+                              inp.a5, out.r1, out.r2, inp.n_int1, inp.n_int2, inp.n_quad, inp.accuracy,
+                              out.norm, out.reff, out.veff, out.gavg, out.vavg, out.ravg, out.rvw,
+                              out.cext, out.csca, out.cbak, out.g)
+    cases = []
+    for deriv in (-1, 0, 1):
+        for leg in (0, 1):
+            src = (out.lc if leg else out.gc) if deriv < 0 else (out.lc_l if leg else out.gc_l)[deriv]
+            cases.append(("coefs_%d_%d" % (deriv, leg),
+                          lambda fp, leg=leg, deriv=deriv: lib.lmie_scat_coefs_write(fp, C.byref(inp), C.byref(out),
+                                                                                     leg, deriv),
+                          lambda fp, src=src: reflib.scat_coefs_write_fp(fp, n_coef, h, src, 1)))
+        pf = out.pf if deriv < 0 else out.pf_l[deriv]
+        cases.append(("pf_%d" % deriv,
+                      lambda fp, deriv=deriv: lib.lmie_phase_func_write(fp, C.byref(inp), C.byref(out), deriv),
+                      lambda fp, pf=pf: reflib.phase_func_write_fp(fp, h, out.theta, pf, p["n_angles"], 1)))
+    for name, ours, theirs in cases:
+        texts = []
+        for tag, fn in (("ours", ours), ("ref", theirs)):
+            path = str(tmp_path / ("%s_%s.txt" % (name, tag)))
+            fp = libc.fopen(path.encode(), b"w")
+            assert fp
+            assert fn(fp) == 0, (name, tag)
+            libc.fclose(fp)
+            texts.append(open(path, "rb").read())
+        assert texts[0] == texts[1], name
+        # 25 header lines, the row count, the rows
+        assert texts[0].count(b"\n") == 25 + 1 + (n_coef if name.startswith("coefs") else p["n_angles"]), name
+    # errors: a derivative that does not exist
+    fp = libc.fopen(str(tmp_path / "bad.txt").encode(), b"w")
+    assert lib.lmie_scat_coefs_write(fp, C.byref(inp), C.byref(out), 0, 2) == -1
+    libc.fclose(fp)
+    lib.lmie_out_free(C.byref(out), 1)
+    lib.lmie_in_free(C.byref(inp), 1)
