@@ -6,7 +6,9 @@ size-integrated solution as a differentiable function of the scalar inputs
 radius range r1, r2 shared by the modes): the forward calls lmie_adjoint_forward,
 the backward lmie_adjoint_reverse with the incoming gradient as the
 cotangent of the coefficients.  torch only carries the tensors; the
-solution and its gradient are computed by the library.
+solution and its gradient are computed by the library.  The forward's device
+state belongs to the graph node: backward may run more than once
+(retain_graph=True), and the state is released when the node is freed.
 """
 import numpy as np
 import torch
@@ -34,8 +36,10 @@ class _Expansion(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, grad_gc):
+        # the device state stays with the graph node: a second backward
+        # (retain_graph=True) runs another reverse pass on it, and it is
+        # released with the node (Adjoint.__del__) when the graph is dropped
         g = ctx.adj.reverse({"gc": grad_gc.detach().cpu().numpy().astype(np.float64)})
-        ctx.adj.free()
         out = [g["mr"], g["mi"]]
         for k in range(ctx.n_modes):
             out += [g["modes"][k][0], g["modes"][k][1]]

@@ -418,6 +418,11 @@ static int grid_build(int dist, int n_int1, int n_int2, int n_quad, const double
      g->qx_l = vec(L); g->qw_l = vec(L); g->nr_l = vec(L);
      g->r1_l = vec(nd); g->r2_l = vec(nd); g->norm_l = vec(nd);
      double *tot_l = vec(nd), tot = 0.;
+     if (!g->qx || !g->qw || !g->nr || !g->qx_l || !g->qw_l || !g->nr_l || !g->r1_l || !g->r2_l ||
+         !g->norm_l || !tot_l) {
+          free(tot_l);
+          return fail("size distribution", "out of memory");   /* the caller frees g */
+     }
      g->r1 = r1; g->r2 = r2;
      for (int d = 0; d < nd; ++d) { g->r1_l[d] = sd(sv->r1, d); g->r2_l[d] = sd(sv->r2, d); }
      double a1 = a[0], a2 = a[1], a3 = a[2], a4 = a[3], a5 = a[4];
@@ -783,25 +788,48 @@ void lmie_in_free(lmie_in_data *in, int flag) {
      free(in->a3_l); free(in->a4_l); free(in->a5_l); free(in->r1_l); free(in->r2_l);
 }
 
+/* Every block is checked; on a failure everything allocated here is freed,
+   the pointers are left NULL and -1 is returned (src/lmie_interface.c:79-125
+   returns -1 from the first failing allocation).  Arrays of outputs that are
+   not requested are set to NULL, so writers and lmie_out_free can test them. */
 int lmie_out_alloc(lmie_in_data *in, lmie_out_data *out, int max_coef) {
-     int n = in->n_derivs;
-     if (n > 0) {
-          double **v[13] = {&out->r1_l, &out->r2_l, &out->norm_l, &out->reff_l, &out->veff_l,
-                            &out->gavg_l, &out->vavg_l, &out->ravg_l, &out->rvw_l, &out->cext_l,
-                            &out->csca_l, &out->cbak_l, &out->g_l};
-          for (int k = 0; k < 13; ++k) *v[k] = vec(n);
+     if (!in || !out) return fail("lmie_out_alloc", "missing argument");
+     const int n = in->n_derivs > 0 ? in->n_derivs : 0;
+     if (max_coef < 0 || (in->calc_pf && in->n_angles < 1))
+          return fail("lmie_out_alloc", "invalid array size");
+     double **v[13] = {&out->r1_l, &out->r2_l, &out->norm_l, &out->reff_l, &out->veff_l,
+                       &out->gavg_l, &out->vavg_l, &out->ravg_l, &out->rvw_l, &out->cext_l,
+                       &out->csca_l, &out->cbak_l, &out->g_l};
+     int bad = 0;
+     for (int k = 0; k < 13; ++k) {
+          *v[k] = n > 0 ? vec(n) : NULL;
+          bad |= n > 0 && !*v[k];
      }
-     out->gc = NULL;
-     if (in->calc_gc) { out->gc = table2(6, max_coef); if (n > 0) out->gc_l = table3(n, 6, max_coef); }
-     out->lc = NULL;
-     if (in->calc_lc) { out->lc = table2(6, max_coef); if (n > 0) out->lc_l = table3(n, 6, max_coef); }
-     out->pf = NULL;
+     out->gc = NULL; out->gc_l = NULL;
+     out->lc = NULL; out->lc_l = NULL;
+     out->pf = NULL; out->pf_l = NULL; out->theta = NULL;
+     if (in->calc_gc) {
+          bad |= !(out->gc = table2(6, max_coef));
+          if (n > 0) bad |= !(out->gc_l = table3(n, 6, max_coef));
+     }
+     if (in->calc_lc) {
+          bad |= !(out->lc = table2(6, max_coef));
+          if (n > 0) bad |= !(out->lc_l = table3(n, 6, max_coef));
+     }
      if (in->calc_pf) {
-          out->theta = vec(in->n_angles);
-          out->pf = table2(6, in->n_angles);
-          if (n > 0) out->pf_l = table3(n, 6, in->n_angles);
+          bad |= !(out->theta = vec(in->n_angles));
+          bad |= !(out->pf = table2(6, in->n_angles));
+          if (n > 0) bad |= !(out->pf_l = table3(n, 6, in->n_angles));
      }
      if (in->save_control != 2) { out->save1 = NULL; out->save2 = NULL; }
+     if (bad) {
+          for (int k = 0; k < 13; ++k) { free(*v[k]); *v[k] = NULL; }
+          free(out->gc); free(out->gc_l); free(out->lc); free(out->lc_l);
+          free(out->pf); free(out->pf_l); free(out->theta);
+          out->gc = NULL; out->gc_l = NULL; out->lc = NULL; out->lc_l = NULL;
+          out->pf = NULL; out->pf_l = NULL; out->theta = NULL;
+          return fail("lmie_out_alloc", "out of memory");
+     }
      return 0;
 }
 
@@ -1012,6 +1040,14 @@ int lmie_solution_ex(lmie_in_data *in, lmie_out_data *out, int alloc_out, const 
 
 int lmie_solution_slave(void) { return fail("lmie_solution_slave", "MPI is not supported (one GPU)"); }
 
+/* Frees what lmie_out_alloc gave an output (with nd tangent columns) and
+   clears the pointers, for the error paths of calls that allocated it. */
+static void out_release_nd(lmie_out_data *out, int nd) {
+     lmie_out_free(out, nd > 0);
+     out->gc = NULL; out->lc = NULL; out->pf = NULL; out->theta = NULL;
+     out->gc_l = NULL; out->lc_l = NULL; out->pf_l = NULL;
+}
+
 /* ------------------------------------------------------------------------- */
 /* Mixtures of distributions on one grid. */
 
@@ -1021,8 +1057,11 @@ int lmie_solution_slave(void) { return fail("lmie_solution_slave", "MPI is not s
 static int mix_grid(lmie_in_data *const *modes, const double *weight, int n_modes, int nd,
                     grid_t *mix) {
      const lmie_in_data *m0 = modes[0];
-     if (m0->dist_type == SIZE_DIST_POWER_LAW || m0->dist_type == SIZE_DIST_MONO)
-          if (n_modes > 1) return fail("lmie_solution_mix", "power-law and mono modes cannot be mixed");
+     /* the power law and mono place their nodes from their own parameters, so
+        such a mode has a grid of its own and cannot share one */
+     for (int k = 0; k < n_modes && n_modes > 1; ++k)
+          if (modes[k]->dist_type == SIZE_DIST_POWER_LAW || modes[k]->dist_type == SIZE_DIST_MONO)
+               return fail("lmie_solution_mix", "power-law and mono modes cannot be mixed");
      int D = nd > 0 ? nd : 1;
      double **col = calloc(12, sizeof(double *));
      for (int q = 0; q < 7; ++q) col[q] = vec(D);    /* a1..a5, r1, r2 */
@@ -1047,6 +1086,7 @@ static int mix_grid(lmie_in_data *const *modes, const double *weight, int n_mode
           grid_t gk;
           memset(&gk, 0, sizeof gk);
           rc = grid_build(m->dist_type, m0->n_int1, m0->n_int2, m0->n_quad, a, m0->r1, m0->r2, &sv, &gk);
+          if (rc) grid_free(&gk);
           if (!rc) {
                double w = weight ? weight[k] : 1.;
                if (k == 0) {
@@ -1056,7 +1096,8 @@ static int mix_grid(lmie_in_data *const *modes, const double *weight, int n_mode
                     mix->norm *= w;
                     for (int d = 0; d < nd; ++d) mix->norm_l[d] *= w;
                } else {
-                    if (gk.N != mix->N) rc = fail("lmie_solution_mix", "modes have different grids");
+                    if (gk.N != mix->N || memcmp(gk.qx, mix->qx, sizeof(double) * gk.N))
+                         rc = fail("lmie_solution_mix", "modes have different grids");
                     for (int i = 0; i < gk.N && !rc; ++i) mix->nr[i] += w * gk.nr[i];
                     for (size_t i = 0; i < (size_t) gk.N * nd && !rc; ++i) mix->nr_l[i] += w * gk.nr_l[i];
                     mix->norm += w * gk.norm;
@@ -1069,6 +1110,10 @@ static int mix_grid(lmie_in_data *const *modes, const double *weight, int n_mode
      (void) off;
      for (int q = 0; q < 7; ++q) free(col[q]);
      free(col);
+     if (rc) {
+          grid_free(mix);
+          memset(mix, 0, sizeof *mix);
+     }
      return rc;
 }
 
@@ -1094,6 +1139,8 @@ int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_mo
           if (lmie_out_alloc(&tmp, out, mc)) return fail("lmie_out_alloc", "failed");
      }
      grid_t g;
+     /* as lmie_solution (src/lmie_interface.c:208-248): on a failure after
+        alloc_out the output stays allocated and the caller frees it */
      if (mix_grid(modes, weight, n_modes, nd, &g)) return -1;
      double *lam_l = vec(nd), *mr_l = vec(nd), *mi_l = vec(nd);
      mix_seed_column(modes, n_modes, 0, lam_l);
@@ -1124,6 +1171,7 @@ int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_mo
 struct lmie_adj_ctx {
      void *state;
      int n_modes, N, C, calc_gc, calc_lc;
+     size_t ld_gc, ld_lc;           /* row strides of the caller's gc / lc blocks (and cotangents) */
      double lambda, Csca, g, cext, csca, cbak;
      double *gc, *lc;               /* normalised outputs [6][C] */
      double *w;                     /* mode weights */
@@ -1195,12 +1243,7 @@ static void *dn_build(void *vp) {
      return NULL;
 }
 
-/* Frees what lmie_out_alloc gave an output without tangents and clears the
-   pointers, for the error paths of calls that allocated it. */
-static void out_release(lmie_out_data *out) {
-     lmie_out_free(out, 0);
-     out->gc = NULL; out->lc = NULL; out->pf = NULL; out->theta = NULL;
-}
+static void out_release(lmie_out_data *out) { out_release_nd(out, 0); }
 
 int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n_modes,
                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
@@ -1253,8 +1296,19 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      c->calc_gc = m0->calc_gc; c->calc_lc = m0->calc_lc;
      c->lambda = m0->lambda; c->Csca = keep.sums[1]; c->g = out->g;
      c->cext = out->cext; c->csca = out->csca; c->cbak = out->cbak;
-     if (c->calc_gc) { c->gc = vec((size_t) 6 * c->C); memcpy(c->gc, out->gc[0], sizeof(double) * 6 * c->C); }
-     if (c->calc_lc) { c->lc = vec((size_t) 6 * c->C); memcpy(c->lc, out->lc[0], sizeof(double) * 6 * c->C); }
+     /* the caller's blocks may be wider than the coefficient count (alloc_out
+        = 0 with a larger max_coef): rows are copied with their own stride, and
+        the cotangents are read with the same stride */
+     if (c->calc_gc) {
+          c->ld_gc = (size_t) (out->gc[1] - out->gc[0]);
+          c->gc = vec((size_t) 6 * c->C);
+          for (int k = 0; k < 6; ++k) memcpy(c->gc + (size_t) k * c->C, out->gc[k], sizeof(double) * c->C);
+     }
+     if (c->calc_lc) {
+          c->ld_lc = (size_t) (out->lc[1] - out->lc[0]);
+          c->lc = vec((size_t) 6 * c->C);
+          for (int k = 0; k < 6; ++k) memcpy(c->lc + (size_t) k * c->C, out->lc[k], sizeof(double) * c->C);
+     }
      c->w = vec(n_modes);
      for (int k = 0; k < n_modes; ++k) c->w[k] = weight ? weight[k] : 1.;
      c->df = dj.df; c->dx = dj.dx;
@@ -1277,17 +1331,21 @@ int lmie_adjoint_reverse(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *gra
      double *gcb = NULL, *lcb = NULL;
      if (adj->gc && c->calc_gc) {
           gcb = vec((size_t) 6 * C);
-          for (int q = 0; q < 6 * C; ++q) {
-               gcb[q] = adj->gc[q] * 2. / Csca;
-               dot += adj->gc[q] * c->gc[q];
-          }
+          for (int k = 0; k < 6; ++k)
+               for (int l = 0; l < C; ++l) {
+                    const double v = adj->gc[k * c->ld_gc + l];
+                    gcb[k * C + l] = v * 2. / Csca;
+                    dot += v * c->gc[k * C + l];
+               }
      }
      if (adj->lc && c->calc_lc) {
           lcb = vec((size_t) 6 * C);
-          for (int q = 0; q < 6 * C; ++q) {
-               lcb[q] = adj->lc[q] * 2. / Csca;
-               dot += adj->lc[q] * c->lc[q];
-          }
+          for (int k = 0; k < 6; ++k)
+               for (int l = 0; l < C; ++l) {
+                    const double v = adj->lc[k * c->ld_lc + l];
+                    lcb[k * C + l] = v * 2. / Csca;
+                    dot += v * c->lc[k * C + l];
+               }
      }
      sb[1] = s * adj->csca - dot / Csca;
      lmie_gpu_adj_in ain = {sb, gcb, lcb};
@@ -1571,7 +1629,8 @@ static int scat_rows_write(FILE *fp, const char *who, const double *x, double *c
 int lmie_scat_coefs_write(FILE *fp, const lmie_in_data *in, const lmie_out_data *out, int legendre, int deriv) {
      if (!fp || !in || !out) return fail("lmie_scat_coefs_write", "missing argument");
      if (deriv >= in->n_derivs) return fail("lmie_scat_coefs_write", "no such derivative");
-     double **c = deriv < 0 ? (legendre ? out->lc : out->gc) : (legendre ? out->lc_l : out->gc_l)[deriv];
+     double ***cl = legendre ? out->lc_l : out->gc_l, **c0 = legendre ? out->lc : out->gc;
+     double **c = deriv < 0 ? c0 : c0 && cl ? cl[deriv] : NULL;
      if (!c) return fail("lmie_scat_coefs_write", legendre ? "no Legendre coefficients" : "no expansion coefficients");
      if (lmie_scat_header_write(fp, in, out)) return -1;
      return scat_rows_write(fp, "lmie_scat_coefs_write", NULL, c, out->n_coef);
@@ -1580,7 +1639,7 @@ int lmie_scat_coefs_write(FILE *fp, const lmie_in_data *in, const lmie_out_data 
 int lmie_phase_func_write(FILE *fp, const lmie_in_data *in, const lmie_out_data *out, int deriv) {
      if (!fp || !in || !out) return fail("lmie_phase_func_write", "missing argument");
      if (deriv >= in->n_derivs) return fail("lmie_phase_func_write", "no such derivative");
-     double **p = deriv < 0 ? out->pf : out->pf_l ? out->pf_l[deriv] : NULL;
+     double **p = deriv < 0 ? out->pf : out->pf && out->pf_l ? out->pf_l[deriv] : NULL;
      if (!p || !out->theta) return fail("lmie_phase_func_write", "no phase function");
      if (lmie_scat_header_write(fp, in, out)) return -1;
      return scat_rows_write(fp, "lmie_phase_func_write", out->theta, p, in->n_angles);

@@ -99,8 +99,9 @@ def solve(lib, p, seeds=None, precision="fp64", device=0):
             raise RuntimeError("lmie_solution_ex failed: " + last_error(lib))
         return abi.read_out(lib, p, inp, out, max_coef)
     finally:
-        if rc == 0:
-            lib.lmie_out_free(C.byref(out), 1 if inp.n_derivs > 0 else 0)
+        # a failed call leaves what it allocated to the caller, as the reference
+        # (the pointers of a failed allocation are NULL)
+        lib.lmie_out_free(C.byref(out), 1 if inp.n_derivs > 0 else 0)
         if inp.n_derivs > 0:
             lib.lmie_in_free(C.byref(inp), 1)
 
@@ -129,8 +130,7 @@ def solve_mix(lib, modes, weights, precision="fp64", device=0):
         fake.n_derivs = nd
         return abi.read_out(lib, p0, fake, out, max_coef)
     finally:
-        if rc == 0:
-            lib.lmie_out_free(C.byref(out), 1 if nd > 0 else 0)
+        lib.lmie_out_free(C.byref(out), 1 if nd > 0 else 0)
         for x in ins:
             if x.n_derivs > 0:
                 lib.lmie_in_free(C.byref(x), 1)

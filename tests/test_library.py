@@ -222,3 +222,60 @@ def test_scattering_files_match_the_reference_writers(lib, reflib, tmp_path):
     libc.fclose(fp)
     lib.lmie_out_free(C.byref(out), 1)
     lib.lmie_in_free(C.byref(inp), 1)
+
+
+def test_out_alloc_failure_frees_everything(lib):
+    """An allocation that cannot be satisfied returns -1 and leaves every array
+    pointer NULL (src/lmie_interface.c:79-125 returns -1 on the first failing
+    allocation); lmie_out_free on the result is then a no-op."""
+    p = dict(dist_type="log_normal", n_int2=2, n_quad=4, calc_gc=1, calc_lc=1, calc_pf=1, n_angles=7,
+             **{"lambda": 0.5}, mr=1.5, mi=0.01, a1=0.3, a2=0.2, r1=0.01, r2=2.0)
+    inp = abi.make_in(lib, p, {"mr": [1.0] * 4096})
+    out = abi.LmieOut()
+    # 4096 x 6 x 2^31 doubles cannot be allocated
+    assert lib.lmie_out_alloc(C.byref(inp), C.byref(out), 2 ** 31 - 1) == -1
+    assert "out of memory" in lmie.last_error(lib)
+    for k in ("gc", "lc", "pf", "theta", "gc_l", "lc_l", "pf_l", "cext_l", "g_l"):
+        assert not bool(getattr(out, k)), k
+    lib.lmie_out_free(C.byref(out), 1)
+    lib.lmie_in_free(C.byref(inp), 1)
+
+
+def test_writers_refuse_outputs_that_were_not_computed(lib, tmp_path):
+    """The tangent file of an output that was not requested (calc_lc = 0 or
+    calc_pf = 0 with seeds) is an error, not a dereference of a NULL block."""
+    libc = C.CDLL(None)
+    libc.fopen.restype, libc.fopen.argtypes = C.c_void_p, [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    lib.lmie_scat_coefs_write.argtypes = [C.c_void_p, C.POINTER(abi.LmieIn), C.POINTER(abi.LmieOut), C.c_int,
+                                          C.c_int]
+    lib.lmie_phase_func_write.argtypes = [C.c_void_p, C.POINTER(abi.LmieIn), C.POINTER(abi.LmieOut), C.c_int]
+    p = dict(dist_type="log_normal", n_int2=2, n_quad=4, calc_gc=1, calc_lc=0, calc_pf=0, n_angles=7,
+             **{"lambda": 0.5}, mr=1.5, mi=0.01, a1=0.3, a2=0.2, r1=0.01, r2=2.0)
+    inp = abi.make_in(lib, p, {"mr": [1.0, 0.0]})
+    out = abi.LmieOut()
+    assert lib.lmie_out_alloc(C.byref(inp), C.byref(out), 12) == 0
+    assert not bool(out.lc_l) and not bool(out.pf_l) and bool(out.gc_l)
+    out.n_coef = 12
+    fp = libc.fopen(str(tmp_path / "x.txt").encode(), b"w")
+    for deriv in (-1, 0, 1):
+        assert lib.lmie_scat_coefs_write(fp, C.byref(inp), C.byref(out), 1, deriv) == -1
+        assert lib.lmie_phase_func_write(fp, C.byref(inp), C.byref(out), deriv) == -1
+        assert lib.lmie_scat_coefs_write(fp, C.byref(inp), C.byref(out), 0, deriv) == 0
+    libc.fclose(fp)
+    lib.lmie_out_free(C.byref(out), 1)
+    lib.lmie_in_free(C.byref(inp), 1)
+
+
+@pytest.mark.parametrize("kind", ["power_law", "mono"])
+@pytest.mark.parametrize("position", [0, 1])
+def test_mixture_rejects_modes_with_their_own_grid(lib, kind, position):
+    """A power-law or mono mode places its nodes from its own parameters, so it
+    cannot share a mixture's grid, in any position of the mode list."""
+    base = dict(dist_type="log_normal", n_int2=2, n_quad=4, calc_gc=1, calc_lc=0, n_angles=0,
+                **{"lambda": 0.5}, mr=1.5, mi=0.01, a1=0.3, a2=0.2, r1=0.01, r2=2.0)
+    other = dict(base, dist_type=kind, a1=0.5, a2=0.1)
+    modes = [(base, None), (base, None)]
+    modes[position] = (other, None)
+    with pytest.raises(RuntimeError, match="cannot be mixed"):
+        lmie.solve_mix(lib, modes, [0.5, 0.5], "fp64")
