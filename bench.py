@@ -25,8 +25,10 @@ FP64 GPU forward of the same workload is timed beside it.  The JSON line reports
                mixed path (24 sum_i n1(x_i)^2); traffic = HBM bytes per launch
                from the committed counter summary profiles/traffic.json
                (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), if present
-  cpu_baseline the reference's pthreads CPU path (oracle/_ref) on a bounded
-               sample of the same workload, on this host's cores
+  cpu_baseline the reference's pthreads CPU path (oracle/_ref) on the same
+               workload (forward only; the reference has no reverse pass), on
+               this host's cores (up to 16 threads: one GPU's share of a node),
+               with the CPU model and core counts
   config.retrieval_batch  the 32-case retrieval workload in one fused batch call
                against one call per case (wall clock; --no-batch omits it)
 
@@ -72,24 +74,49 @@ def sum_n1(p):
     return int(np.sum(grid_n1(p)))
 
 
-def cpu_baseline(threads):
-    """The reference build on a bounded sample: both modes, 256 x 64 = 16384 size
-    points each (1/2 of the grid, same radius range, same 3810 angles)."""
+def cpu_model():
+    """CPU model name of this host (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def host_threads():
+    """Worker threads for the CPU baseline: every core this process may run on,
+    capped at 16, the host-core share of one GPU on a shared multi-GPU node."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return max(1, min(16, avail)), avail
+
+
+def cpu_baseline():
+    """The reference build (pthreads) on the whole workload: both modes on the
+    full 512 x 64 grid (the two-mode solution is the weighted sum of the two
+    single-mode solutions on that grid)."""
     from lmie_amd import abi, workloads as W
     so = os.path.join(ROOT, "oracle", "_ref", "liblmie_ref.so")
     if not os.path.exists(so):
         return None
+    threads, avail = host_threads()
     ref = abi.bind(C.CDLL(so))
     tot_units, t0 = 0, time.perf_counter()
     for k in range(2):
-        p = dict(W.cfg4_mode_params(k), n_int2=256)
+        p = W.cfg4_mode_params(k)
         abi.solve(ref, p, None, threads)
         tot_units += sum_n1(p)
     dt = time.perf_counter() - t0
     return {"value": tot_units / dt, "unit": "size-pt*n-term/s", "cores": threads,
-            "kind": "reference",
-            "sample": "reference lmie_solution (pthreads) on both modes at 256x64 size points each "
-                      "(1/2 of the grid, same r range and 3810 angles): %.1f s" % dt}
+            "kind": "reference", "host_cpus": os.cpu_count(), "cpus_available": avail,
+            "cpu_model": cpu_model(),
+            "sample": "the whole workload: reference lmie_solution (pthreads, %d threads) on both modes "
+                      "at 512x64 size points each, 3810 angles, the forward only: %.1f s" % (threads, dt)}
 
 
 def retrieval_batch(lib, steps, warmup, quick=False):
@@ -280,7 +307,7 @@ def main():
     if a.path == "mixed-adjoint" and not a.no_batch and not a.no_fp64_compare:
         line["config"]["retrieval_batch"] = retrieval_batch(lib, 5, 2, quick=True)
     if not a.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(min(16, os.cpu_count() or 1))
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 
