@@ -30,7 +30,14 @@ OBJS     := $(patsubst $(REF)/%.c,$(OUT)/obj/%.o,$(SRCS))
 # drop-in check of tests/test_gpu_fp64.py runs it on the GPU.
 EXAMPLE  := $(OUT)/example_c_amd
 
-all: $(OUT)/liblmie_ref.so $(EXAMPLE)
+# The reference's command line program (utils/calllmie.c), unmodified, linked
+# statically with the objects above: the checker of the product's calllmie.
+CALLLMIE := $(OUT)/calllmie_ref
+
+all: $(OUT)/liblmie_ref.so $(EXAMPLE) $(CALLLMIE)
+
+$(CALLLMIE): $(REF)/utils/calllmie.c $(OBJS)
+	$(CC) $(CFLAGS) -o $@ $(REF)/utils/calllmie.c $(OBJS) -lpthread -lm
 
 $(EXAMPLE): $(REF)/examples/example_c.c lmie_amd/lib/liblmie_amd.so
 	@mkdir -p $(OUT)

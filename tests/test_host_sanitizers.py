@@ -1,7 +1,7 @@
 """The host C layer and the oracle under AddressSanitizer, UndefinedBehavior-
 Sanitizer and the leak checker, on the CPU.
 
-The product's host layer (lmie_amd/csrc/lmie_host.c) is linked with a test
+The product's host layer (lmie_amd/csrc/lmie_host.c, lmie_scat_io.c) is linked with a test
 double of the device layer (tests/c/gpu_double.c), so every entry point of the
 drop-in interface and of the extensions runs without a GPU; then each device
 call is forced to fail from the host, and the -1 return, the error text
@@ -36,8 +36,8 @@ def _run(exe):
 
 
 def test_host_layer_under_sanitizers_with_forced_device_failures(tmp_path):
-    exe = _build(tmp_path, "host_checks", ["lmie_amd/csrc/lmie_host.c", "tests/c/gpu_double.c",
-                                           "tests/c/host_checks.c"], ["-DWITH_EXTENSIONS"])
+    exe = _build(tmp_path, "host_checks", ["lmie_amd/csrc/lmie_host.c", "lmie_amd/csrc/lmie_scat_io.c",
+                                           "tests/c/gpu_double.c", "tests/c/host_checks.c"], ["-DWITH_EXTENSIONS"])
     _run(exe)
 
 
