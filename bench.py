@@ -19,12 +19,16 @@ FP64 GPU forward of the same workload is timed beside it.  The JSON line reports
   roofline     the dominant kernel of the step, algorithmic FLOP per launch over
                its device time (HIP events on the library's stream) against the
                MI355X peak of its arithmetic (executed_frac: the matrix-core work
-               actually issued, three split products per algorithmic product on
-               the mixed path): the forward angular contraction
+               actually issued, five split products per algorithmic product in the
+               forward contraction and three in the reverse one): the forward angular contraction
                (8 n_qang sum_i n1(x_i)) or the reverse contraction U = G A of the
                mixed path (24 sum_i n1(x_i)^2); traffic = HBM bytes per launch
                from the committed counter summary profiles/traffic.json
                (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy), if present
+  config.wall_speedup_vs_fp64  whole calls (host preparation and PCIe included)
+               of the plain FP64 forward over the default path's whole calls
+  config.fp64_angular  the FP64 forward's angular contraction: device time, the
+               time its algorithmic FLOP take at the FP64 matrix peak, their ratio
   cpu_baseline the reference's pthreads CPU path (oracle/_ref) on the same
                workload (forward only; the reference has no reverse pass), on
                this host's cores (up to 16 threads: one GPU's share of a node),
@@ -265,12 +269,25 @@ def main():
     mixed = a.path.startswith("mixed")
     peak = PEAK["bf16"] if mixed else PEAK["fp64"]
     # dominant kernel: forward contraction, or the reverse contraction U = G A
-    kern = {"angular_mix2" if mixed else "angular_f64": (8.0 * nq * units, ang_ms)}
+    kern = {"angular_mix" if mixed else "angular_f64": (8.0 * nq * units, ang_ms)}
+    # matrix products issued per algorithmic product: the forward contraction
+    # splits the size operand in two parts and the angle operand in three and
+    # takes five of their products (three with LMIE_MIX_PRODUCTS=3); the
+    # reverse contraction takes three (two parts of each operand)
+    issued = {"angular_mix": 3.0 if os.environ.get("LMIE_MIX_PRODUCTS") == "3" else 5.0,
+              "adj_ugemm": 3.0, "angular_f64": 1.0}
     if mixed and a.path.endswith("-adjoint") and stages.get("adj_gemm", 0) > 0:
         kern["adj_ugemm"] = (24.0 * float(np.sum(n1s * n1s)), stages["adj_gemm"])
     kname = max(kern, key=lambda k: kern[k][1])
     flop, kms = kern[kname]
     achieved = flop / (kms * 1e-3)
+    # the FP64 forward's angular contraction against the FP64 matrix peak
+    fp64_angular = None
+    if ref_dt is not None:
+        f64_ms = ref_stages["angular"]
+        f64_roof_ms = 8.0 * nq * units / PEAK["fp64"] * 1e3
+        fp64_angular = {"kernel": "angular_f64", "ms": f64_ms, "roofline_ms": f64_roof_ms,
+                        "frac": f64_roof_ms / f64_ms}
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
@@ -292,16 +309,18 @@ def main():
                    "speedup_vs_fp64": None if ref_dt is None else ref_dt / dt,
                    "wall_ms_per_step": wall * 1e3,
                    "fp64_wall_ms_per_step": None if ref_wall is None else ref_wall * 1e3,
+                   "wall_speedup_vs_fp64": None if ref_wall is None else ref_wall / wall,
+                   "fp64_angular": fp64_angular,
                    "timing": "ms_per_step, value and fp64_ms_per_step: device pipeline per step with "
                              "inputs in HBM (HIP events on the library stream, forward + reverse); "
                              "wall_*: whole calls including host preparation and PCIe transfers"},
         "roofline": {"bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel": kname, "kernel_ms": kms, "flop_per_launch": flop,
-                     # the split 16-bit operands take three matrix products per
-                     # algorithmic product (hi.hi, lo.hi, hi.lo)
-                     "executed_frac": (3.0 if mixed else 1.0) * achieved / peak,
-                     "kernels": {k: {"ms": v[1], "TFLOP/s": v[0] / (v[1] * 1e-3) / 1e12} for k, v in kern.items()}},
+                     "executed_frac": issued[kname] * achieved / peak,
+                     "kernels": {k: {"ms": v[1], "TFLOP/s": v[0] / (v[1] * 1e-3) / 1e12,
+                                     "executed_frac": issued[k] * v[0] / (v[1] * 1e-3) / peak}
+                                 for k, v in kern.items()}},
         "cpu_baseline": None,
     }
     if a.path == "mixed-adjoint" and not a.no_batch and not a.no_fp64_compare:

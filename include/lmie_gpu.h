@@ -61,6 +61,11 @@ typedef struct {
      double *save1;           /* [4][N]          if save_control == 1            (caller) */
      double *save2;           /* [4][N][n_qang]  if save_control == 1            (caller) */
      void *state;             /* job->keep: device state for lmie_gpu_adjoint     */
+     int precision;           /* arithmetic the angular contraction ran in: the
+                                 job's, or LMIE_PRECISION_FP64 where the mixed
+                                 kernels do not cover the job (forward tangent
+                                 channels, more than 3 weight-only channels,
+                                 save_control 1)                                   */
 } lmie_gpu_result;
 
 int lmie_gpu_forward(const lmie_gpu_job *job, lmie_gpu_result *res);

@@ -40,10 +40,13 @@ extern "C" {
 /* Arithmetic of the angular contraction.  FP64: every stage in double.
    MIXED: the Mie coefficients, cross sections, reductions, normalisation and
    expansion stay in double; the angular sums S1/S2 (and their adjoint) run on
-   the matrix cores with each operand split into a high and a low 16-bit part,
-   three products, fp32 accumulation; MIXED uses fp16 parts (operands scaled
-   into the fp16 range by exact powers of two), MIXED_BF16 bf16 parts.
-   Forward tangents are always computed in FP64. */
+   the matrix cores with fp32 accumulation, the operands split into 16-bit
+   parts (forward: the size operand in two, the angle operand in three, five
+   products; reverse: two and two, three products); MIXED uses fp16 parts
+   (operands scaled into the fp16 range by exact powers of two), MIXED_BF16
+   bf16 parts.  Forward
+   tangents are always computed in FP64 (lmie_last_precision reports the
+   arithmetic a call ran in). */
 enum lmie_precision {
      LMIE_PRECISION_FP64       = 0,
      LMIE_PRECISION_MIXED      = 1,
@@ -204,6 +207,14 @@ int lmie_phase_func_write(FILE *fp, const lmie_in_data *in, const lmie_out_data 
 
 /* (3) Text of the last error on this thread ("" if none). */
 const char *lmie_last_error(void);
+
+/* Arithmetic (enum lmie_precision) the angular contraction of this thread's
+   last solution ran in.  A MIXED request runs in FP64 where the split 16-bit
+   kernels do not cover it: with forward tangent channels (seeds that move the
+   size parameters or m), more than 3 weight-only seed channels, or
+   save_control 1 (per-size outputs); the re-weighting of save_control 2 is
+   FP64.  No counterpart in the reference. */
+int lmie_last_precision(void);
 
 /* Per-kernel device time of the last call on this thread, in milliseconds,
    measured with HIP events on the library's stream when enabled. */

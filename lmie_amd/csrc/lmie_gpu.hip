@@ -11,7 +11,7 @@
 //                                                              (src/lmie_core.c:453-652)
 //   mie_sums         series per size                          (src/lmie_core.c:537-596)
 //   mie_coefs        per size with forward tangents (tangent passes)
-//   angular_mix2     S sums on the 16-bit matrix cores (split operands), Stokes in
+//   angular_mix      S sums on the 16-bit matrix cores (split operands), Stokes in
 //                    FP64, weighted sums per angle            (src/lmie_core.c:658-791)
 //   angular_f64      the same on the FP64 matrix cores
 //   reduce_parts     fixed-order sum of the per-tile partials (src/lmie_schedule.c:109-147)
@@ -68,6 +68,8 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 // scaling is exact.
 constexpr double kSA = 4.0, kSB = 16384.0;
 constexpr int KM = 32;     // orders per LDS stage of the mixed contraction (4 groups of 8)
+// planes of the split angle table: (pi+tau) high, middle, low, (pi-tau) high, middle, low
+constexpr int B16_PLANES = 6;
 
 // 16-bit element types of the split operands
 struct F16 {
@@ -113,6 +115,15 @@ __device__ __forceinline__ void split16(double v, unsigned short *hi, unsigned s
      double r, r2;
      *hi = E::bits(v, &r);
      *lo = E::bits(v - r, &r2);
+}
+// three parts: hi = round(v), mid = round(v - hi), lo = round(v - hi - mid)
+// (hi and mid are split16's two parts)
+template <class E>
+__device__ __forceinline__ void split16x3(double v, unsigned short *hi, unsigned short *mid, unsigned short *lo) {
+     double r, r2, r3;
+     *hi = E::bits(v, &r);
+     *mid = E::bits(v - r, &r2);
+     *lo = E::bits((v - r) - r2, &r3);
 }
 
 // Row of the 16x16 f32 result held in lane l, element v, for the 16-bit
@@ -303,7 +314,7 @@ __global__ void pitau_table(int H, int Hp, int n1max, int Kp, const double *mu, 
 }
 
 // Split 16-bit tables for the mixed path, blocked by 8 orders:
-// [4 planes: (pi+tau) hi, lo, (pi-tau) hi, lo][Kb][Hp][8], value
+// [6 planes: (pi+tau) hi, mid, lo, (pi-tau) hi, mid, lo][Kb][Hp][8], value
 // (pi_n +- tau_n) / (n (n+1)) * kSB.  A lane (angle) writes 16 bytes per plane
 // per group of 8 orders; consecutive lanes are consecutive 16-byte chunks.
 // The per-order factors (n+1)/n and kSB/(n(n+1)) come from LDS when n1max
@@ -317,7 +328,7 @@ __device__ __forceinline__ void pitau16_rows(int H, int Hp, int n1max, int Kb, c
      const double m = j < H ? mu[H + j] : 0.;
      double p0 = 0., p1 = 1.;
      for (int g = 0; g < Kb; ++g) {
-          unsigned short v[4][8];
+          unsigned short v[B16_PLANES][8];
           for (int e = 0; e < 8; ++e) {
                const int n = 8 * g + e + 1;
                double bp = 0., bm = 0.;
@@ -330,10 +341,10 @@ __device__ __forceinline__ void pitau16_rows(int H, int Hp, int n1max, int Kb, c
                     p0 = p1;
                     p1 = a1 + (TAB ? fq[n] : (n + 1.) / n) * a2;
                }
-               split16<E>(bp, &v[0][e], &v[1][e]);
-               split16<E>(bm, &v[2][e], &v[3][e]);
+               split16x3<E>(bp, &v[0][e], &v[1][e], &v[2][e]);
+               split16x3<E>(bm, &v[3][e], &v[4][e], &v[5][e]);
           }
-          for (int q = 0; q < 4; ++q) {
+          for (int q = 0; q < B16_PLANES; ++q) {
                v4u w;
                for (int k = 0; k < 4; ++k) w[k] = (unsigned) v[q][2 * k] | ((unsigned) v[q][2 * k + 1] << 16);
                *(v4u *) (B16 + q * plane + ((size_t) g * Hp + j) * 8) = w;
@@ -1334,159 +1345,268 @@ struct Stager {
 };
 
 // The split 16-bit contraction on the 16x16x32 instruction: the four complex
-// sums of the FP64 contraction, each operand split into a high and a low
-// 16-bit part, x y ~ xh yh + xh yl + xl yh (three products, fp32 accumulation).
-// One workgroup per (size tile of 64, angle tile of 16 WB NWV) task, two LDS
-// stages: the next 32 orders of both operands are read from HBM into
-// registers while the matrix cores work on the current stage, then written to
-// the other stage (one barrier per stage).  Waves in a 2 x NWV grid, each 32
-// sizes x 16 WB angles, 8 accumulators per element (the sums at +mu and -mu;
-// the -mu products flip the sign of the odd orders, s_n = (-1)^(n-1)).
-// NWV = 4, WB = 2: eight waves of 32 x 32, two per SIMD; NWV = 2, WB = 4: four
-// waves of 32 x 64.  The (size tile, angle tile) of each workgroup comes from
-// a host-built list dealt over the XCDs (xcd_tasks).  Partials per size tile:
-// part[tile][NW][4][Q].
+// sums of the FP64 contraction with the size operand in two 16-bit parts
+// (A = Ah + Al, 22 bits) and the angle operand in three (B = Bh + Bm + Bl,
+// 33 bits), on the matrix cores with fp32 accumulation:
+//   NP = 5:  A B ~ Ah Bh + [Ah Bm + Al Bh + Ah Bl + Al Bm]   (dropped: Al Bl ~ 2^-34)
+//   NP = 3:  A B ~ Ah Bh + [Ah Bm + Al Bh]                   (dropped: ~ 2^-24 of A B)
+// The leading product and the bracketed corrections accumulate separately and
+// meet in FP64 in the epilogue: an fp32 accumulator rounds at every product
+// it takes, relative to its own running value, so the corrections (2^-11 of
+// the sums) would otherwise add four roundings of the sums' size per stage.
+// The angle operand is the same for every size point, so its rounding errors
+// add up coherently over the size integral (two parts of B leave expansion
+// coefficients with relative errors of a few 1e-5); the size operand's
+// rounding is independent from size to size.
+// One workgroup of four waves (one per SIMD) per (size tile of 64, angle tile
+// of 64) task, two LDS stages of 32 orders filled straight from memory
+// (global_load_lds_dwordx4: a wave moves one 1 KiB row, 64 lanes x 16 bytes,
+// into LDS without registers) while the matrix cores work on the other stage,
+// one barrier per stage.  Waves in a 2 x 2 grid, each 32 sizes x 32 angles
+// (2 x 2 blocks of 16 x 16), 8 sums per element: at +mu and at -mu, whose
+// products flip the sign of the odd orders of the size operand
+// (s_n = (-1)^(n-1)).  The (size tile, angle tile) of each workgroup comes
+// from a host-built list dealt over the XCDs (xcd_tasks).  Partials per size
+// tile: part[tile][NW][4][Q].
 struct Mix2Args {
      int Np, Kb, Hb, H, Q;
-     const unsigned short *A16;     // [8][Kb][Np][8]
-     const unsigned short *B16;     // [4][Kb][Hb][8]
+     const unsigned short *A16;     // [8][Kb][Np][8]: S re h/l, S im h/l, T re h/l, T im h/l
+     const unsigned short *B16;     // [6][Kb][Hb][8]: (pi+tau) h/m/l, (pi-tau) h/m/l
      const int *ktile;              // orders per size tile of 64 (multiple of KM)
      const int2 *task;              // (size tile, angle tile) per workgroup, (-1, -1) = none
      const double *W[4];            // weight channels [Np]
      double *part;                  // [nT][NW][4][Q]
-     double *sst;                   // reverse-pass checkpoint [Np][Hb][8] or NULL
 };
 
-template <class E, int WB, int NWV, int NW>
-__device__ __forceinline__ void angular_mix2_body(const Mix2Args &p, int2 tk) {
-     constexpr int NT = 128 * NWV, TM = 64, TN = 16 * WB * NWV, G = KM / 8;
-     constexpr int NA = 8 * G * TM / NT, NB = 4 * G * TN / NT;   // staged v4u per thread
-     static_assert(NA * NT == 8 * G * TM && NB * NT == 4 * G * TN, "staging split");
-     __shared__ v4u sA[2][8 * G][TM];
-     __shared__ v4u sB[2][4 * G][TN];
-     static_assert(sizeof(sA) >= sizeof(double) * 2 * NW * 8 * TN, "epilogue buffer");
+// Shapes of the forward contraction (environment LMIE_MIX_SHAPE, read at every
+// call).  A workgroup owns TM sizes x TN angles; its NT / 64 waves form a
+// WM x WN grid, each wave 16 MA sizes x 16 NB angles; SEP: the corrections
+// have accumulators of their own (else every product goes to one
+// accumulator per sum); ST LDS stages, loads ST - 1 stages ahead; OCC
+// workgroups per CU.
+//   0: 64 x 128, eight waves (two per SIMD), one accumulator, 2 x 80 KiB
+//   1: 64 x 64, four waves (one per SIMD), separate corrections, 2 x 56 KiB
+//   2: 32 x 64, four waves, separate corrections, 2 x 40 KiB, two workgroups per CU
+//   3: 64 x 64, eight waves (two per SIMD), separate corrections, 2 x 56 KiB
+// DT > 0: every stage also touches each 128-byte line of the stage DT + 1
+// ahead (one 4-byte direct load per line into a scratch slot of LDS that is
+// never read), so that line is in L2 when its stage is loaded; the stage loads
+// then wait for an L2 hit instead of a trip to memory.
+constexpr int MIX_G = KM / 8;
+constexpr int MIX_SHAPE_DEFAULT = 3;
+struct MixS0 { static constexpr int NT = 512, TM = 64, TN = 128, WM = 2, WN = 4, MA = 2, NB = 2, ST = 2, OCC = 1, DT = 0; static constexpr bool SEP = false; };
+struct MixS1 { static constexpr int NT = 256, TM = 64, TN = 64, WM = 2, WN = 2, MA = 2, NB = 2, ST = 2, OCC = 1, DT = 0; static constexpr bool SEP = true; };
+struct MixS2 { static constexpr int NT = 256, TM = 32, TN = 64, WM = 2, WN = 2, MA = 1, NB = 2, ST = 2, OCC = 2, DT = 0; static constexpr bool SEP = true; };
+struct MixS3 { static constexpr int NT = 512, TM = 64, TN = 64, WM = 4, WN = 2, MA = 1, NB = 2, ST = 2, OCC = 1, DT = 0; static constexpr bool SEP = true; };
+template <class S> constexpr int mix_sa() { return 8 * MIX_G * S::TM; }              // v4u of A per stage
+template <class S> constexpr int mix_sb() { return B16_PLANES * MIX_G * S::TN; }     // v4u of B per stage
+// direct loads one wave issues per stage (its vmcnt events)
+template <class S> constexpr int mix_loads() {
+     return (8 * MIX_G * ((S::TM + 63) / 64) + B16_PLANES * MIX_G * ((S::TN + 63) / 64)) / (S::NT / 64);
+}
+
+typedef __attribute__((address_space(3))) void *lds_ptr;
+
+// One wave's share of a stage: the A rows (plane, group) of the size tile and
+// the B rows of the angle tile, cut into segments of at most 64 columns of 16
+// bytes; wave w takes segments w, w + NT / 64, ...  Each segment is one direct
+// load (lane l: column l of the segment).
+// lines a stage touches, and direct loads per wave for them
+template <class S> constexpr int mix_touch_lines() { return 8 * MIX_G * S::TM / 8 + B16_PLANES * MIX_G * S::TN / 8; }
+template <class S> constexpr int mix_touch_loads() { return (mix_touch_lines<S>() + S::NT - 1) / S::NT; }
+
+// Touch of stage c: thread tid loads one dword of lines tid, tid + NT, ...
+// (the last load of a thread repeats a line when the count is not a multiple)
+template <class S>
+__device__ __forceinline__ void mix_stage_touch(const Mix2Args &p, unsigned *sink, int c, int it0, int jt0, int tid) {
+     constexpr int LA = 8 * MIX_G * S::TM / 8, L = mix_touch_lines<S>();
+     const size_t Np = p.Np, Hb = p.Hb;
+     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hb * 8;
+#pragma unroll
+     for (int t = 0; t < mix_touch_loads<S>(); ++t) {
+          int li = tid + S::NT * t;
+          li = li < L ? li : L - 1;
+          const unsigned short *src;
+          if (li < LA) {
+               const int row = li / (S::TM / 8), seg = li % (S::TM / 8), q = row / MIX_G, g = row % MIX_G;
+               src = p.A16 + q * pA + ((size_t) (c * MIX_G + g) * Np + it0 + 8 * seg) * 8;
+          } else {
+               const int lb = li - LA, row = lb / (S::TN / 8), seg = lb % (S::TN / 8), q = row / MIX_G, g = row % MIX_G;
+               src = p.B16 + q * pB + ((size_t) (c * MIX_G + g) * Hb + jt0 + 8 * seg) * 8;
+          }
+          __builtin_amdgcn_global_load_lds((const void *) src, (lds_ptr) sink, 4, 0, 0);
+     }
+}
+
+template <class S>
+__device__ __forceinline__ void mix_stage_load(const Mix2Args &p, v4u *sA, v4u *sB, int c, int it0, int jt0,
+                                               int wave, int lane) {
+     constexpr int NWAVE = S::NT / 64, SEGA = (S::TM + 63) / 64, SEGB = (S::TN + 63) / 64;
+     constexpr int LA = S::TM < 64 ? S::TM : 64, LB = S::TN < 64 ? S::TN : 64;
+     const size_t Np = p.Np, Hb = p.Hb;
+     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hb * 8;
+     if (lane < LA) {
+#pragma unroll
+          for (int t = 0; t < 8 * MIX_G * SEGA / NWAVE; ++t) {
+               const int r = wave + NWAVE * t, row = r / SEGA, seg = r % SEGA, q = row / MIX_G, g = row % MIX_G;
+               const unsigned short *src = p.A16 + q * pA + ((size_t) (c * MIX_G + g) * Np + it0 + 64 * seg + lane) * 8;
+               __builtin_amdgcn_global_load_lds((const void *) src, (lds_ptr) (sA + row * S::TM + 64 * seg), 16, 0, 0);
+          }
+     }
+     if (lane < LB) {
+#pragma unroll
+          for (int t = 0; t < B16_PLANES * MIX_G * SEGB / NWAVE; ++t) {
+               const int r = wave + NWAVE * t, row = r / SEGB, seg = r % SEGB, q = row / MIX_G, g = row % MIX_G;
+               const unsigned short *src = p.B16 + q * pB + ((size_t) (c * MIX_G + g) * Hb + jt0 + 64 * seg + lane) * 8;
+               __builtin_amdgcn_global_load_lds((const void *) src, (lds_ptr) (sB + row * S::TN + 64 * seg), 16, 0, 0);
+          }
+     }
+}
+
+// The products of four sums (x_k times b_k, k = 0..3; x = (h, l) parts,
+// b = (h, m, l) parts), interleaved so that consecutive instructions are
+// independent: the leading product into m, the corrections into c.
+template <class E, int NP>
+__device__ __forceinline__ void mix_products(const v4u *x0, const v4u *b0, const v4u *x1, const v4u *b1,
+                                             const v4u *x2, const v4u *b2, const v4u *x3, const v4u *b3,
+                                             v4f *m, v4f *c) {
+     m[0] = E::mfma(x0[0], b0[0], m[0]); m[1] = E::mfma(x1[0], b1[0], m[1]);
+     m[2] = E::mfma(x2[0], b2[0], m[2]); m[3] = E::mfma(x3[0], b3[0], m[3]);
+     c[0] = E::mfma(x0[0], b0[1], c[0]); c[1] = E::mfma(x1[0], b1[1], c[1]);
+     c[2] = E::mfma(x2[0], b2[1], c[2]); c[3] = E::mfma(x3[0], b3[1], c[3]);
+     c[0] = E::mfma(x0[1], b0[0], c[0]); c[1] = E::mfma(x1[1], b1[0], c[1]);
+     c[2] = E::mfma(x2[1], b2[0], c[2]); c[3] = E::mfma(x3[1], b3[0], c[3]);
+     if constexpr (NP == 5) {
+          c[0] = E::mfma(x0[0], b0[2], c[0]); c[1] = E::mfma(x1[0], b1[2], c[1]);
+          c[2] = E::mfma(x2[0], b2[2], c[2]); c[3] = E::mfma(x3[0], b3[2], c[3]);
+          c[0] = E::mfma(x0[1], b0[1], c[0]); c[1] = E::mfma(x1[1], b1[1], c[1]);
+          c[2] = E::mfma(x2[1], b2[1], c[2]); c[3] = E::mfma(x3[1], b3[1], c[3]);
+     }
+}
+
+template <class E, int NP, int NW, class S>
+__device__ __forceinline__ void angular_mix_body(const Mix2Args &p, int2 tk) {
+     constexpr int NT = S::NT, TM = S::TM, TN = S::TN, G = MIX_G, MA = S::MA, NB = S::NB, ST = S::ST;
+     constexpr int SA = mix_sa<S>(), SB = mix_sb<S>();
+     __shared__ v4u lds[ST * (SA + SB) + (S::DT > 0 ? 16 : 0)];
+     static_assert(sizeof(lds) >= sizeof(double) * S::WM * NW * 8 * TN, "epilogue buffer");
+     unsigned *sink = (unsigned *) (lds + ST * (SA + SB));   // touch loads land here (never read)
+     static_assert(S::WM * 16 * MA == TM && S::WN * 16 * NB == TN && S::WM * S::WN == NT / 64, "wave grid");
 
      if (tk.x < 0) return;
      const int st = tk.x, jt = tk.y;
-     const int it0 = st * TM, jt0 = jt * TN, kmax = p.ktile[st], nch = kmax / KM;
-
-     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-     const int wm = wave & 1, wn = wave >> 1;
+     const int it0 = st * TM, jt0 = jt * TN, nch = p.ktile[st] / KM;
+     const int lane = threadIdx.x & 63;
+     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+     const int wm = wave % S::WM, wn = wave / S::WM;
      const int l16 = lane & 15, lg = lane >> 4;
-     const size_t Np = p.Np, Hb = p.Hb;
-     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hb * 8;
      const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
+     // stage buffer b: A rows at lds + b (SA + SB), B rows after them
+     auto sA = [&](int b_) { return lds + b_ * (SA + SB); };
+     auto sB = [&](int b_) { return lds + b_ * (SA + SB) + SA; };
 
-     v4u ra[NA], rb[NB];
-     const Stager<NT, TM, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
-     const Stager<NT, TN, G> stB(p.B16, pB, Hb * 8, jt0, threadIdx.x);
-     auto load = [&](int c) {
+     // leading product and corrections (the same accumulators without SEP)
+     v4f am[MA][NB][8], ac_[S::SEP ? MA : 1][S::SEP ? NB : 1][8];
 #pragma unroll
-          for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
+     for (int a = 0; a < MA; ++a)
 #pragma unroll
-          for (int t = 0; t < NB; ++t) rb[t] = *stB.at(t, c);
-     };
-     auto store = [&](int s) {
+          for (int b = 0; b < NB; ++b)
 #pragma unroll
-          for (int t = 0; t < NA; ++t) {
-               const int e = threadIdx.x + NT * t;
-               sA[s][e / TM][e % TM] = ra[t];
-          }
+               for (int k = 0; k < 8; ++k) am[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int t = 0; t < NB; ++t) {
-               const int e = threadIdx.x + NT * t;
-               sB[s][e / TN][e % TN] = rb[t];
-          }
-     };
+     for (int a = 0; a < (S::SEP ? MA : 1); ++a)
+#pragma unroll
+          for (int b = 0; b < (S::SEP ? NB : 1); ++b)
+#pragma unroll
+               for (int k = 0; k < 8; ++k) ac_[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
+     auto acc_c = [&](int a, int b) -> v4f * { if constexpr (S::SEP) return ac_[a][b]; else return am[a][b]; };
 
-     v4f acc[2][WB][8];
+     // stages 0 .. ST-2 in flight before the loop; stage c + ST - 1 is issued
+     // at stage c into the buffer of stage c - 1
 #pragma unroll
-     for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int b = 0; b < WB; ++b)
-#pragma unroll
-               for (int k = 0; k < 8; ++k) acc[a][b][k] = v4f{0.f, 0.f, 0.f, 0.f};
-
-     // stage c+1 goes to LDS in the middle of stage c's products and the
-     // loads of stage c+2 follow it, so both overlap the matrix cores
-     if (nch > 0) {
-          load(0);
-          store(0);
-          if (nch > 1) load(1);
-     }
+     for (int c = 0; c < ST - 1; ++c)
+          if (c < nch) mix_stage_load<S>(p, sA(c), sB(c), c, it0, jt0, wave, lane);
+     if constexpr (S::DT > 0)
+          if (nch > 0) mix_stage_touch<S>(p, sink, min(ST - 1 + S::DT, nch - 1), it0, jt0, threadIdx.x);
      for (int c = 0; c < nch; ++c) {
-          const int s = c & 1;
-          __syncthreads();
-          v4u bph[WB], bpl[WB], bmh[WB], bml[WB];
+          const int buf = c % ST;
+          // this wave's rows of stage c have landed (the younger stages may
+          // still be in flight); after the barrier every wave's have, and
+          // every wave is done with stage c - 1
+          // (the touch loads issued after them may stay in flight)
+          if constexpr (S::DT > 0) __builtin_amdgcn_s_waitcnt(0x0F70 | mix_touch_loads<S>());
+          else if (ST == 3 && c + 1 < nch) __builtin_amdgcn_s_waitcnt(0x0F70 | mix_loads<S>());
+          else __builtin_amdgcn_s_waitcnt(0);
+          // the plain barrier: each wave has waited for its own rows above, and
+          // a workgroup fence here would also wait for the younger stages
+          __builtin_amdgcn_s_barrier();
+          __asm__ volatile("" ::: "memory");
+          if (c + ST - 1 < nch) {
+               const int nb = (c + ST - 1) % ST;
+               mix_stage_load<S>(p, sA(nb), sB(nb), c + ST - 1, it0, jt0, wave, lane);
+          }
+          if constexpr (S::DT > 0)
+               mix_stage_touch<S>(p, sink, min(c + ST + S::DT, nch - 1), it0, jt0, threadIdx.x);
+          // B fragments of the angle blocks: (pi+tau) h/m/l, (pi-tau) h/m/l
+          v4u bp[NB][3], bm[NB][3];
 #pragma unroll
-          for (int b = 0; b < WB; ++b) {
-               const int jc = wn * 16 * WB + 16 * b + l16;
-               bph[b] = sB[s][0 * G + lg][jc];
-               bpl[b] = sB[s][1 * G + lg][jc];
-               bmh[b] = sB[s][2 * G + lg][jc];
-               bml[b] = sB[s][3 * G + lg][jc];
+          for (int b = 0; b < NB; ++b) {
+               const int jc = wn * 16 * NB + 16 * b + l16;
+#pragma unroll
+               for (int q = 0; q < 3; ++q) {
+                    bp[b][q] = sB(buf)[(q * G + lg) * TN + jc];
+                    bm[b][q] = sB(buf)[((3 + q) * G + lg) * TN + jc];
+               }
           }
 #pragma unroll
-          for (int a = 0; a < 2; ++a) {
-               const int ic = wm * 32 + 16 * a + l16;
+          for (int a = 0; a < MA; ++a) {
+               const int ic = wm * 16 * MA + 16 * a + l16;
                v4u x[8];
 #pragma unroll
-               for (int k = 0; k < 8; ++k) x[k] = sA[s][k * G + lg][ic];
+               for (int k = 0; k < 8; ++k) x[k] = sA(buf)[(k * G + lg) * TM + ic];
                // +mu: sp = S (pi+tau), sm = T (pi-tau)
 #pragma unroll
-               for (int b = 0; b < WB; ++b) {
-                    v4f *cc = acc[a][b];
-                    cc[0] = E::mfma(x[0], bph[b], cc[0]); cc[1] = E::mfma(x[2], bph[b], cc[1]);
-                    cc[2] = E::mfma(x[4], bmh[b], cc[2]); cc[3] = E::mfma(x[6], bmh[b], cc[3]);
-                    cc[0] = E::mfma(x[0], bpl[b], cc[0]); cc[1] = E::mfma(x[2], bpl[b], cc[1]);
-                    cc[2] = E::mfma(x[4], bml[b], cc[2]); cc[3] = E::mfma(x[6], bml[b], cc[3]);
-                    cc[0] = E::mfma(x[1], bph[b], cc[0]); cc[1] = E::mfma(x[3], bph[b], cc[1]);
-                    cc[2] = E::mfma(x[5], bmh[b], cc[2]); cc[3] = E::mfma(x[7], bmh[b], cc[3]);
-               }
+               for (int b = 0; b < NB; ++b)
+                    mix_products<E, NP>(x + 0, bp[b], x + 2, bp[b], x + 4, bm[b], x + 6, bm[b], am[a][b], acc_c(a, b));
                // -mu: the odd orders change sign, sp = s S (pi-tau), sm = s T (pi+tau)
 #pragma unroll
                for (int k = 0; k < 8; ++k) x[k] ^= flip;
 #pragma unroll
-               for (int b = 0; b < WB; ++b) {
-                    v4f *cc = acc[a][b];
-                    cc[4] = E::mfma(x[0], bmh[b], cc[4]); cc[5] = E::mfma(x[2], bmh[b], cc[5]);
-                    cc[6] = E::mfma(x[4], bph[b], cc[6]); cc[7] = E::mfma(x[6], bph[b], cc[7]);
-                    cc[4] = E::mfma(x[0], bml[b], cc[4]); cc[5] = E::mfma(x[2], bml[b], cc[5]);
-                    cc[6] = E::mfma(x[4], bpl[b], cc[6]); cc[7] = E::mfma(x[6], bpl[b], cc[7]);
-                    cc[4] = E::mfma(x[1], bmh[b], cc[4]); cc[5] = E::mfma(x[3], bmh[b], cc[5]);
-                    cc[6] = E::mfma(x[5], bph[b], cc[6]); cc[7] = E::mfma(x[7], bph[b], cc[7]);
-               }
-               if (a == 0) {
-                    if (c + 1 < nch) store(s ^ 1);
-                    if (c + 2 < nch) load(c + 2);
-               }
+               for (int b = 0; b < NB; ++b)
+                    mix_products<E, NP>(x + 0, bm[b], x + 2, bm[b], x + 4, bp[b], x + 6, bp[b], am[a][b] + 4,
+                                        acc_c(a, b) + 4);
           }
      }
 
-     // epilogue: Stokes elements in double, weighted sums over this wave's 32
-     // sizes per angle, combined over the two size halves (wm) through LDS
+     // epilogue: the sums in double (leading product plus corrections), the
+     // Stokes elements, weighted sums over this wave's sizes per angle,
+     // combined over the WM size blocks of the waves through LDS
      const double unscale = 1. / (kSA * kSB);
-     double *sR = (double *) &sA[0][0][0];   // [2][NW][8][TN]
+     double *sR = (double *) lds;            // [WM][NW][8][TN]
      __syncthreads();                        // every wave is done with the stages
      // one weight channel at a time (the Stokes elements are recomputed per
      // channel), so few doubles are live beside the accumulators
 #pragma unroll
-     for (int b = 0; b < WB; ++b) {
-          const int jl = wn * 16 * WB + 16 * b + l16;
+     for (int b = 0; b < NB; ++b) {
+          const int jl = wn * 16 * NB + 16 * b + l16;
 #pragma unroll
           for (int c = 0; c < NW; ++c) {
                double ew[8];
 #pragma unroll
                for (int k = 0; k < 8; ++k) ew[k] = 0.;
 #pragma unroll
-               for (int a = 0; a < 2; ++a)
+               for (int a = 0; a < MA; ++a)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
-                         const v4f *qq = acc[a][b];
-                         const int i = it0 + wm * 32 + 16 * a + mfma16_row(lane, v);
+                         const v4f *qm = am[a][b], *qc = acc_c(a, b);
+                         double q[8];
+#pragma unroll
+                         for (int k = 0; k < 8; ++k)
+                              q[k] = (S::SEP ? (double) qm[k][v] + (double) qc[k][v] : (double) qm[k][v]) * unscale;
+                         const int i = it0 + wm * 16 * MA + 16 * a + mfma16_row(lane, v);
                          double Pp[4], Pm[4];
-                         stokes(qq[0][v] * unscale, qq[1][v] * unscale, qq[2][v] * unscale, qq[3][v] * unscale, Pp);
-                         stokes(qq[4][v] * unscale, qq[5][v] * unscale, qq[6][v] * unscale, qq[7][v] * unscale, Pm);
+                         stokes(q[0], q[1], q[2], q[3], Pp);
+                         stokes(q[4], q[5], q[6], q[7], Pm);
                          const double wv = p.W[c][i];
 #pragma unroll
                          for (int k = 0; k < 4; ++k) {
@@ -1508,163 +1628,26 @@ __device__ __forceinline__ void angular_mix2_body(const Mix2Args &p, int2 tk) {
      for (int e = threadIdx.x; e < NW * 8 * TN; e += NT) {
           const int jl = e % TN, ck = e / TN, c = ck / 8, k = ck % 8, j = jt0 + jl;
           if (j >= p.H) continue;
-          const double v = sR[(c * 8 + k) * TN + jl] + sR[((NW + c) * 8 + k) * TN + jl];
+          double v = 0.;
+#pragma unroll
+          for (int m = 0; m < S::WM; ++m) v += sR[((m * NW + c) * 8 + k) * TN + jl];
           // k < 4: direct angle H + j; k >= 4: mirrored angle H - 1 - j
           out[((size_t) c * 4 + (k & 3)) * p.Q + (k < 4 ? p.H + j : p.H - 1 - j)] = v;
      }
 }
-template <class E, int WB, int NWV, int NW>
-__global__ __launch_bounds__(128 * NWV) void angular_mix2(Mix2Args p) {
-     angular_mix2_body<E, WB, NWV, NW>(p, p.task[blockIdx.x]);
+template <class E, int NP, int NW, class S>
+__global__ __launch_bounds__(S::NT) __attribute__((amdgpu_waves_per_eu(S::NT / 256 * S::OCC)))
+void angular_mix(Mix2Args p) {
+     angular_mix_body<E, NP, NW, S>(p, p.task[blockIdx.x]);
 }
-// angular_mix2 over the cases of a batch: one task list for all cases, task
+// angular_mix over the cases of a batch: one task list for all cases, task
 // (size tile, angle tile | case << 16), padding tasks (-1, case << 16).
-template <class E, int WB, int NWV, int NW>
-__global__ __launch_bounds__(128 * NWV) void angular_mix2_b(const Mix2Args *cases, const int2 *task) {
+template <class E, int NP, int NW, class S>
+__global__ __launch_bounds__(S::NT) __attribute__((amdgpu_waves_per_eu(S::NT / 256 * S::OCC)))
+void angular_mix_b(const Mix2Args *cases, const int2 *task) {
      const int2 t = task[blockIdx.x];
      const Mix2Args p = cases[t.y >> 16];
-     angular_mix2_body<E, WB, NWV, NW>(p, make_int2(t.x, t.y & 0xffff));
-}
-
-
-
-// The split 16-bit contraction on the 32x32x16 matrix instruction: one
-// workgroup of eight waves (two per SIMD) per (size tile of 64, angle tile of
-// 128) task, stages of 32 orders (two K-steps of 16) in two LDS buffers.
-// Waves in a 2 x 4 grid, each one 32 x 32 block (rows sizes, columns angles),
-// 8 accumulators per element as angular_mix2.  The -mu products flip the sign
-// of the odd orders on the angle operand (the same products as flipping the
-// size operand).  Partials per size tile: part[tile][NW][4][Q].
-template <class E, int NW>
-__global__ __launch_bounds__(512) void angular_mix32(Mix2Args p) {
-     constexpr int NT = 512, TM = 64, TN = 128, G = KM / 8;
-     static_assert(G == 4, "two K-steps of 16 orders per stage");
-     constexpr int NA = 8 * G * TM / NT, NB = 4 * G * TN / NT;   // staged v4u per thread
-     static_assert(NA * NT == 8 * G * TM && NB * NT == 4 * G * TN, "staging split");
-     __shared__ v4u sA[2][8 * G][TM];
-     __shared__ v4u sB[2][4 * G][TN];
-     static_assert(sizeof(sA) >= sizeof(double) * 2 * NW * 8 * TN, "epilogue buffer");
-
-     const int2 tk = p.task[blockIdx.x];
-     if (tk.x < 0) return;
-     const int st = tk.x, jt = tk.y;
-     const int it0 = st * TM, jt0 = jt * TN, nch = p.ktile[st] / KM;
-     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-     const int wm = wave & 1, wn = wave >> 1, l32 = lane & 31, lh = lane >> 5;
-     const size_t Np = p.Np, Hb = p.Hb;
-     const size_t pA = (size_t) p.Kb * Np * 8, pB = (size_t) p.Kb * Hb * 8;
-     const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
-
-     v4u ra[NA], rb[NB];
-     const Stager<NT, TM, G> stA(p.A16, pA, Np * 8, it0, threadIdx.x);
-     const Stager<NT, TN, G> stB(p.B16, pB, Hb * 8, jt0, threadIdx.x);
-     auto load = [&](int c) {
-#pragma unroll
-          for (int t = 0; t < NA; ++t) ra[t] = *stA.at(t, c);
-#pragma unroll
-          for (int t = 0; t < NB; ++t) rb[t] = *stB.at(t, c);
-     };
-     auto store = [&](int s) {
-#pragma unroll
-          for (int t = 0; t < NA; ++t) {
-               const int e = threadIdx.x + NT * t;
-               sA[s][e / TM][e % TM] = ra[t];
-          }
-#pragma unroll
-          for (int t = 0; t < NB; ++t) {
-               const int e = threadIdx.x + NT * t;
-               sB[s][e / TN][e % TN] = rb[t];
-          }
-     };
-
-     v16f acc[8];
-#pragma unroll
-     for (int k = 0; k < 8; ++k)
-#pragma unroll
-          for (int v = 0; v < 16; ++v) acc[k][v] = 0.f;
-     if (nch > 0) {
-          load(0);
-          store(0);
-          if (nch > 1) load(1);
-     }
-     for (int c = 0; c < nch; ++c) {
-          const int s = c & 1;
-          __syncthreads();
-#pragma unroll
-          for (int ks = 0; ks < 2; ++ks) {
-               // lane l: row (size or angle) l % 32, orders 8 (2 ks + l / 32) + e
-               const int row = 2 * ks + lh;
-               v4u x[8], b[4];
-#pragma unroll
-               for (int q = 0; q < 8; ++q) x[q] = sA[s][q * G + row][wm * 32 + l32];
-#pragma unroll
-               for (int q = 0; q < 4; ++q) b[q] = sB[s][q * G + row][wn * 32 + l32];
-               // +mu: sp = S (pi+tau), sm = T (pi-tau); b = (pi+tau hi, lo, pi-tau hi, lo)
-               acc[0] = E::mfma32(x[0], b[0], acc[0]); acc[1] = E::mfma32(x[2], b[0], acc[1]);
-               acc[2] = E::mfma32(x[4], b[2], acc[2]); acc[3] = E::mfma32(x[6], b[2], acc[3]);
-               acc[0] = E::mfma32(x[0], b[1], acc[0]); acc[1] = E::mfma32(x[2], b[1], acc[1]);
-               acc[2] = E::mfma32(x[4], b[3], acc[2]); acc[3] = E::mfma32(x[6], b[3], acc[3]);
-               acc[0] = E::mfma32(x[1], b[0], acc[0]); acc[1] = E::mfma32(x[3], b[0], acc[1]);
-               acc[2] = E::mfma32(x[5], b[2], acc[2]); acc[3] = E::mfma32(x[7], b[2], acc[3]);
-               // -mu: the odd orders change sign, sp = s S (pi-tau), sm = s T (pi+tau)
-#pragma unroll
-               for (int q = 0; q < 4; ++q) b[q] ^= flip;
-               acc[4] = E::mfma32(x[0], b[2], acc[4]); acc[5] = E::mfma32(x[2], b[2], acc[5]);
-               acc[6] = E::mfma32(x[4], b[0], acc[6]); acc[7] = E::mfma32(x[6], b[0], acc[7]);
-               acc[4] = E::mfma32(x[0], b[3], acc[4]); acc[5] = E::mfma32(x[2], b[3], acc[5]);
-               acc[6] = E::mfma32(x[4], b[1], acc[6]); acc[7] = E::mfma32(x[6], b[1], acc[7]);
-               acc[4] = E::mfma32(x[1], b[2], acc[4]); acc[5] = E::mfma32(x[3], b[2], acc[5]);
-               acc[6] = E::mfma32(x[5], b[0], acc[6]); acc[7] = E::mfma32(x[7], b[0], acc[7]);
-               if (ks == 0) {
-                    if (c + 1 < nch) store(s ^ 1);
-                    if (c + 2 < nch) load(c + 2);
-               }
-          }
-     }
-
-     // epilogue: Stokes elements in double, weighted sums over this wave's 32
-     // sizes per angle (16 rows per lane, the other 16 in lane l ^ 32),
-     // combined over the two size halves (wm) through LDS
-     const double unscale = 1. / (kSA * kSB);
-     double *sR = (double *) &sA[0][0][0];   // [2][NW][8][TN]
-     __syncthreads();                        // every wave is done with the stages
-     const int jl = wn * 32 + l32;
-     // one weight channel at a time (the Stokes elements are recomputed per
-     // channel), so few doubles are live beside the accumulators
-#pragma unroll
-     for (int c = 0; c < NW; ++c) {
-          double ew[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) ew[k] = 0.;
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-               const int i = it0 + wm * 32 + mfma32_row(lane, v);
-               double Pp[4], Pm[4];
-               stokes(acc[0][v] * unscale, acc[1][v] * unscale, acc[2][v] * unscale, acc[3][v] * unscale, Pp);
-               stokes(acc[4][v] * unscale, acc[5][v] * unscale, acc[6][v] * unscale, acc[7][v] * unscale, Pm);
-               const double wv = p.W[c][i];
-#pragma unroll
-               for (int k = 0; k < 4; ++k) {
-                    ew[k] += wv * Pp[k];
-                    ew[4 + k] += wv * Pm[k];
-               }
-          }
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-               double t = ew[k];
-               t += __shfl_xor(t, 32);
-               if (lh == 0) sR[((wm * NW + c) * 8 + k) * TN + jl] = t;
-          }
-     }
-     __syncthreads();
-     double *out = p.part + (size_t) st * NW * 4 * p.Q;
-     for (int e = threadIdx.x; e < NW * 8 * TN; e += NT) {
-          const int jl2 = e % TN, ck = e / TN, c = ck / 8, k = ck % 8, j2 = jt0 + jl2;
-          if (j2 >= p.H) continue;
-          const double v = sR[(c * 8 + k) * TN + jl2] + sR[((NW + c) * 8 + k) * TN + jl2];
-          // k < 4: direct angle H + j; k >= 4: mirrored angle H - 1 - j
-          out[((size_t) c * 4 + (k & 3)) * p.Q + (k < 4 ? p.H + j2 : p.H - 1 - j2)] = v;
-     }
+     angular_mix_body<E, NP, NW, S>(p, make_int2(t.x, t.y & 0xffff));
 }
 
 
@@ -1992,10 +1975,10 @@ __global__ __launch_bounds__(1024) void adj_weights_b(const adj_weights_B *cases
 
 // Split tables with angles along the contraction: [plane][Hq / 8][Kq][8],
 // 8 consecutive positive nodes per 16 bytes, from the forward's split tables
-// B16 ([4][Kb][Hb][8], orders along the contraction): left (4 planes) P+ and
-// P- of order r + 1 over r (r + 1) (r + 2) times kSB, high and low parts (the
-// B16 halves, transposed); right (16 planes) weight q times P+ (q = 0, 3, 6,
-// 7) or P- (q = 1, 2, 4, 5), from P = high + low (relative error about 2^-22),
+// B16 ([6][Kb][Hb][8], orders along the contraction): left (4 planes) P+ and
+// P- of order r + 1 over r (r + 1) (r + 2) times kSB, high and middle parts
+// (two of the B16 parts, transposed); right (16 planes) weight q times P+
+// (q = 0, 3, 6, 7) or P- (q = 1, 2, 4, 5), from P = high + middle + low,
 // high and low parts.  One thread per (order, group of 8 nodes).
 template <class E>
 __device__ __forceinline__ void adj_tables_body(const unsigned short *B16, int Kb, int Hb, const double *wts, int H, int Hq, int Kq, int n1max, unsigned short *L16, unsigned short *R16) {
@@ -2008,10 +1991,12 @@ __device__ __forceinline__ void adj_tables_body(const unsigned short *B16, int K
           double pp = 0., pm = 0.;
           if (j < H && r < n1max) {
                const size_t bo = ((size_t) (r / 8) * Hb + j) * 8 + (r % 8);
-#pragma unroll
-               for (int q = 0; q < 4; ++q) lv[q][e] = B16[q * bpl + bo];
-               pp = E::value(lv[0][e]) + E::value(lv[1][e]);
-               pm = E::value(lv[2][e]) + E::value(lv[3][e]);
+               lv[0][e] = B16[0 * bpl + bo];
+               lv[1][e] = B16[1 * bpl + bo];
+               lv[2][e] = B16[3 * bpl + bo];
+               lv[3][e] = B16[4 * bpl + bo];
+               pp = E::value(lv[0][e]) + E::value(lv[1][e]) + E::value(B16[2 * bpl + bo]);
+               pm = E::value(lv[2][e]) + E::value(lv[3][e]) + E::value(B16[5 * bpl + bo]);
           } else {
 #pragma unroll
                for (int q = 0; q < 4; ++q) lv[q][e] = 0;
@@ -3085,13 +3070,34 @@ void xcd_tasks(const std::vector<int> &kt, F ncol, int group, std::vector<int2> 
           for (size_t k = 0; k < per[x].size(); ++k) out[8 * k + x] = per[x][k];
 }
 
-// Kernel of the mixed contraction (environment LMIE_MIX_WAVES, read at every
-// call): 8 (default) or 4 the 16x16x32 instruction with eight or four waves,
-// 32 the 32x32x16 instruction with eight waves.
-int mix_waves() {
-     const char *e = getenv("LMIE_MIX_WAVES");
-     const int r = e ? atoi(e) : 8;
-     return (r == 4 || r == 32) ? r : 8;
+// Split products of the forward mixed contraction (environment
+// LMIE_MIX_PRODUCTS, read at every call): 5 (default) or 3.
+int mix_products() {
+     const char *e = getenv("LMIE_MIX_PRODUCTS");
+     return (e && atoi(e) == 3) ? 3 : 5;
+}
+
+// Shape of the forward mixed contraction (environment LMIE_MIX_SHAPE, read at
+// every call; the shapes are listed at MixS0): size tile, angle tile.
+struct MixShapeInfo { int id, tm, tn, nt; };
+MixShapeInfo mix_shape() {
+     const char *e = getenv("LMIE_MIX_SHAPE");
+     const int id = e ? atoi(e) : MIX_SHAPE_DEFAULT;
+     if (id == 0) return {0, MixS0::TM, MixS0::TN, MixS0::NT};
+     if (id == 2) return {2, MixS2::TM, MixS2::TN, MixS2::NT};
+     if (id == 3) return {3, MixS3::TM, MixS3::TN, MixS3::NT};
+     return {1, MixS1::TM, MixS1::TN, MixS1::NT};
+}
+
+// Size tiles per group of xcd_tasks in the forward mixed contraction
+// (environment LMIE_MIX_GROUP, read at every call, default 8): the 32
+// workgroups in flight on one XCD are then 8 size tiles x 4 angle tiles, so
+// each angle tile's operand is read into that XCD's L2 for 8 size tiles and
+// each size tile's for 4 angle tiles.
+int mix_group() {
+     const char *e = getenv("LMIE_MIX_GROUP");
+     const int g = e ? atoi(e) : 8;
+     return (g < 1 || g > 64) ? 8 : g;
 }
 
 // Size tiles per group of xcd_tasks (environment LMIE_TILE_GROUP, read at
@@ -3134,7 +3140,9 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      const int bf16 = J->precision == LMIE_PRECISION_MIXED_BF16;
      const int kgran = mixed ? KM : KC;
      const int Kb = rup(n1max, KM) / 8;
-     // angle padding of the split tables: tiles of 128 angles
+     // angle padding of the split tables: tiles of 128 angles (whole angle
+     // tiles of every contraction shape)
+     const MixShapeInfo ms = mix_shape();
      const int Hb = rup(H > 0 ? H : 1, 128);
 
      // host-side tables: per-tile contraction depths
@@ -3210,7 +3218,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      if (w->pres.get(sizeof(double) * ((size_t) nch * 12 * C + 4 * nch))) return -1;
      if (mixed) {
           // the contraction's task list, also with the inputs
-          xcd_tasks(kt64, [&](int) { return Hb / 128; }, tile_group(), w->htask);
+          xcd_tasks(ms.tm == 64 ? kt64 : kt32, [&](int) { return (H + ms.tn - 1) / ms.tn; }, mix_group(), w->htask);
           if (w->order.get(sizeof(int2) * w->htask.size())) return -1;
           HIPCHECK(hipMemcpyAsync(w->order.p, w->htask.data(), sizeof(int2) * w->htask.size(),
                                   hipMemcpyHostToDevice, s));
@@ -3261,7 +3269,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           }
           HIPCHECK(hipEventRecord(w->join3, w->side2));
           if (mixed) {
-               if (w->B16.get(sizeof(unsigned short) * 4 * (size_t) Kb * Hb * 8)) return -1;
+               if (w->B16.get(sizeof(unsigned short) * B16_PLANES * (size_t) Kb * Hb * 8)) return -1;
                if (bf16)
                     hipLaunchKernelGGL(pitau_table16<BF16>, dim3(Hb / 64), dim3(64), 0, s, H, Hb, n1max, Kb,
                                        (const double *) w->mu.as<double>(), w->B16.as<unsigned short>());
@@ -3411,7 +3419,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                plan[0].ntp = 0;
                plan[0].nwp = 1 + nw;
                plan[0].t0 = plan[0].w0 = 0;
-               plan[0].nblk = nt64;
+               plan[0].nblk = Np / ms.tm;
                plan[0].off = 0;
                tot = (size_t) plan[0].nblk * plan[0].nwp * 4 * Q;
           }
@@ -3423,37 +3431,31 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                m.Np = Np; m.Kb = Kb; m.Hb = Hb; m.H = H; m.Q = Q;
                m.A16 = w->A16.as<unsigned short>();
                m.B16 = w->B16.as<unsigned short>();
-               m.ktile = w->ktile64.as<int>();
+               m.ktile = ms.tm == 64 ? w->ktile64.as<int>() : w->ktile32.as<int>();
                m.task = w->order.as<int2>();
                m.W[0] = w->f.as<double>();
                for (int q = 0; q < nw; ++q) m.W[1 + q] = w->dfw.as<double>() + (size_t) q * Np;
                m.part = w->part.as<double>();
-               m.sst = nullptr;
                const dim3 grid((unsigned) w->htask.size());
-               // tile 64 sizes x 128 angles: 8 waves of 32 x 32, or 4 waves of
-               // 32 x 64 (LMIE_MIX_WAVES=4)
-#define LMIE_MIX(E, WB, NWV, NWC) hipLaunchKernelGGL((angular_mix2<E, WB, NWV, NWC>), grid, dim3(128 * NWV), 0, s, m)
-#define LMIE_MIX_NW(E, WB, NWV)                                                                          \
+               // five split products (three with LMIE_MIX_PRODUCTS=3), in the
+               // shape of LMIE_MIX_SHAPE
+#define LMIE_MIX(E, NP, NWC, S) hipLaunchKernelGGL((angular_mix<E, NP, NWC, S>), grid, dim3(S::NT), 0, s, m)
+#define LMIE_MIX_NW(E, NP, S)                                                                            \
                do {                                                                                      \
-                    if (nw == 0) LMIE_MIX(E, WB, NWV, 1); else if (nw == 1) LMIE_MIX(E, WB, NWV, 2);    \
-                    else if (nw == 2) LMIE_MIX(E, WB, NWV, 3); else LMIE_MIX(E, WB, NWV, 4);            \
+                    if (nw == 0) LMIE_MIX(E, NP, 1, S); else if (nw == 1) LMIE_MIX(E, NP, 2, S);        \
+                    else if (nw == 2) LMIE_MIX(E, NP, 3, S); else LMIE_MIX(E, NP, 4, S);                \
                } while (0)
-               const int mw = mix_waves();
-               if (mw == 4) {
-                    if (bf16) LMIE_MIX_NW(BF16, 4, 2); else LMIE_MIX_NW(F16, 4, 2);
-               } else if (mw == 8) {
-                    if (bf16) LMIE_MIX_NW(BF16, 2, 4); else LMIE_MIX_NW(F16, 2, 4);
-               } else {
-#define LMIE_MIX32(E)                                                                                    \
-                    do {                                                                                 \
-                         if (nw == 0) hipLaunchKernelGGL((angular_mix32<E, 1>), grid, dim3(512), 0, s, m); \
-                         else if (nw == 1) hipLaunchKernelGGL((angular_mix32<E, 2>), grid, dim3(512), 0, s, m); \
-                         else if (nw == 2) hipLaunchKernelGGL((angular_mix32<E, 3>), grid, dim3(512), 0, s, m); \
-                         else hipLaunchKernelGGL((angular_mix32<E, 4>), grid, dim3(512), 0, s, m);        \
-                    } while (0)
-                    if (bf16) LMIE_MIX32(BF16); else LMIE_MIX32(F16);
-#undef LMIE_MIX32
-               }
+#define LMIE_MIX_NP(E, S)                                                                                \
+               do {                                                                                      \
+                    if (mix_products() == 3) LMIE_MIX_NW(E, 3, S); else LMIE_MIX_NW(E, 5, S);            \
+               } while (0)
+#define LMIE_MIX_S(S) do { if (bf16) LMIE_MIX_NP(BF16, S); else LMIE_MIX_NP(F16, S); } while (0)
+               if (ms.id == 0) LMIE_MIX_S(MixS0);
+               else if (ms.id == 2) LMIE_MIX_S(MixS2);
+               else if (ms.id == 3) LMIE_MIX_S(MixS3);
+               else LMIE_MIX_S(MixS1);
+#undef LMIE_MIX_S
+#undef LMIE_MIX_NP
 #undef LMIE_MIX_NW
 #undef LMIE_MIX
                HIPCHECK(hipGetLastError());
@@ -3555,6 +3557,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
      for (int i = 0; i < N; ++i) w->n2max = J->n2[i] > w->n2max ? J->n2[i] : w->n2max;
      w->calc_gc = J->calc_gc; w->calc_lc = J->calc_lc; w->mr = J->mr; w->mi = J->mi;
      w->mixed = mixed; w->bf16 = bf16; w->Kb = Kb; w->Hb = Hb; w->kt64 = kt64; w->kt128 = kt128;
+     R->precision = mixed ? J->precision : LMIE_PRECISION_FP64;
      return 0;
 }
 
@@ -3951,6 +3954,7 @@ struct FwdCase {
      lmie_gpu_result *R;
      Workspace *w;
      int N, Np, n1max, Q, C, H, Hp, Kp, Kb, Hb, nt64, n2rows, ng, planes;
+     int nmix;                      // size tiles of the forward contraction's shape
      std::vector<int2> task;
 };
 
@@ -3966,7 +3970,8 @@ int fused_prepare(FwdCase &c, Upload &up) {
      const lmie_gpu_job *J = c.J;
      Workspace *w = c.w;
      const int N = J->n_size, Np = rup(N, 2 * TILE), n1max = J->n1_max, Q = J->n_qang, C = J->n_coef;
-     const int H = Q / 2, Hp = rup(H, TILE), Kp = rup(n1max, 32), Kb = rup(n1max, KM) / 8, Hb = rup(H, 128);
+     const int H = Q / 2, Hp = rup(H, TILE), Kp = rup(n1max, 32), Kb = rup(n1max, KM) / 8,
+               Hb = rup(H, 128);
      if (N <= 0) return set_err("no size points");
      for (int i = 0; i < N; ++i)
           if (J->n1[i] > n1max || J->n1[i] < 1 || J->n2[i] < J->n1[i])
@@ -3992,14 +3997,14 @@ int fused_prepare(FwdCase &c, Upload &up) {
          w->ktile64.get(sizeof(int) * nt64) || w->sx.get(sizeof(double) * Np) || w->cx.get(sizeof(double) * Np) ||
          w->guess.get(sizeof(double) * H) || w->mu.get(sizeof(double) * Q) || w->w.get(sizeof(double) * Q) ||
          w->T.get(sizeof(double) * planes * (size_t) C * Hp) ||
-         w->B16.get(sizeof(unsigned short) * 4 * (size_t) Kb * Hb * 8) || w->S.get(sizeof(double) * 4 * Np) ||
+         w->B16.get(sizeof(unsigned short) * B16_PLANES * (size_t) Kb * Hb * 8) || w->S.get(sizeof(double) * 4 * Np) ||
          w->A16.get(sizeof(unsigned short) * 8 * (size_t) Kb * Np * 8) || w->CB.get(sizeof(double) * 2 * Np) ||
          w->chD.get(sizeof(double) * 2 * (size_t) n2rows * Np) ||
          w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) ||
          w->chPart.get(sizeof(double) * 5 * (size_t) (ng > 0 ? ng : 1) * Np) ||
          w->ftab.get(sizeof(double) * FT_ROWS * (size_t) (n1max + 2)) || w->dotjobs.get(sizeof(DotJob) * 4) ||
          w->dots.get(sizeof(double) * 4) || w->pf.get(sizeof(double) * 4 * (size_t) Q) ||
-         w->part.get(sizeof(double) * (size_t) nt64 * 4 * Q) || w->E.get(sizeof(double) * 12 * (size_t) Hp) ||
+         w->part.get(sizeof(double) * (size_t) (Np / 32) * 4 * Q) || w->E.get(sizeof(double) * 12 * (size_t) Hp) ||
          w->coefs.get(sizeof(double) * 12 * (size_t) C))
           return -1;
      // x, f zero-padded to Np; sin x, cos x from the host C library (padding 0, 1)
@@ -4022,7 +4027,9 @@ int fused_prepare(FwdCase &c, Upload &up) {
          up.add(w->ktile64.p, kt64.data(), sizeof(int) * nt64) ||
          up.add(w->guess.p, w->hguess.data(), sizeof(double) * H) || up.add(w->dotjobs.p, jobs, sizeof(jobs)))
           return -1;
-     xcd_tasks(kt64, [&](int) { return Hb / 128; }, tile_group(), c.task);
+     const MixShapeInfo ms = mix_shape();
+     xcd_tasks(ms.tm == 64 ? kt64 : kt32, [&](int) { return (H + ms.tn - 1) / ms.tn; }, mix_group(), c.task);
+     c.nmix = Np / ms.tm;
      c.N = N; c.Np = Np; c.n1max = n1max; c.Q = Q; c.C = C; c.H = H; c.Hp = Hp; c.Kp = Kp; c.Kb = Kb; c.Hb = Hb;
      c.nt64 = nt64; c.n2rows = n2rows; c.ng = ng; c.planes = planes;
      w->N = N; w->Np = Np; w->Kp = Kp; w->Hp = Hp; w->H = H; w->Q = Q; w->C = C; w->n1max = n1max;
@@ -4081,14 +4088,15 @@ int fused_launch(std::vector<FwdCase> &cs, Upload &up, Workspace *L) {
           memset(&m, 0, sizeof m);
           m.Np = c.Np; m.Kb = c.Kb; m.Hb = c.Hb; m.H = c.H; m.Q = c.Q;
           m.A16 = w->A16.as<unsigned short>(); m.B16 = w->B16.as<unsigned short>();
-          m.ktile = w->ktile64.as<int>(); m.task = nullptr; m.W[0] = w->f.as<double>();
-          m.part = w->part.as<double>(); m.sst = nullptr;
+          m.ktile = c.nmix == c.Np / 64 ? w->ktile64.as<int>() : w->ktile32.as<int>();
+          m.task = nullptr; m.W[0] = w->f.as<double>();
+          m.part = w->part.as<double>();
           for (const int2 &t : c.task)
                task.push_back(make_int2(t.x, ((t.x < 0 ? 0 : t.y) & 0xffff) | (int) (k << 16)));
           ChMap map;
           memset(&map, 0, sizeof map);
           map.g[0] = 0;
-          red[k] = reduce_parts_B{w->part.as<double>(), c.nt64, 1, c.Q, map, w->pf.as<double>(),
+          red[k] = reduce_parts_B{w->part.as<double>(), c.nmix, 1, c.Q, map, w->pf.as<double>(),
                                   (unsigned) (4 * c.Q + 31) / 32, 1};
           for (int gs = 0; gs < 2; ++gs) {
                const unsigned on = (gs ? c.J->calc_gc : c.J->calc_lc) ? 1u : 0u;
@@ -4136,8 +4144,16 @@ int fused_launch(std::vector<FwdCase> &cs, Upload &up, Workspace *L) {
      // contraction after the split tables, one task list for all cases
      HIPCHECK(hipStreamWaitEvent(s, L->join, 0));
      if (!task.empty())
-          hipLaunchKernelGGL((angular_mix2_b<E, 2, 4, 1>), dim3((unsigned) task.size()), dim3(512), 0, s,
-                             LMIE_B(Mix2Args, o_mix), LMIE_B(int2, o_task));
+     {
+          const dim3 g((unsigned) task.size());
+          const int np = mix_products(), id = mix_shape().id;
+#define LMIE_MIXB(NP, S) hipLaunchKernelGGL((angular_mix_b<E, NP, 1, S>), g, dim3(S::NT), 0, s, LMIE_B(Mix2Args, o_mix), LMIE_B(int2, o_task))
+          if (id == 0) { if (np == 3) LMIE_MIXB(3, MixS0); else LMIE_MIXB(5, MixS0); }
+          else if (id == 2) { if (np == 3) LMIE_MIXB(3, MixS2); else LMIE_MIXB(5, MixS2); }
+          else if (id == 3) { if (np == 3) LMIE_MIXB(3, MixS3); else LMIE_MIXB(5, MixS3); }
+          else { if (np == 3) LMIE_MIXB(3, MixS1); else LMIE_MIXB(5, MixS1); }
+#undef LMIE_MIXB
+     }
      hipLaunchKernelGGL(reduce_parts_b, dim3(max_g(red, &reduce_parts_B::gx), 1, n), dim3(256), 0, s,
                         LMIE_B(reduce_parts_B, o_red));
      // expansion after the expansion tables: generalized spherical, then Legendre
@@ -4395,6 +4411,7 @@ void batch_run_forward(Batch *b) {
                b->rc[0][c.slot] = rc;
                b->err[0][c.slot] = rc ? g_err : "";
                c.R->state = rc ? nullptr : c.w;
+               c.R->precision = c.J->precision;
                if (rc == 0 && c.w == L) L_kept = true;
                if (rc && c.w != L) ws_release(c.w);
                b->fused_cases[0]++;
@@ -4646,6 +4663,7 @@ int forward_one(const lmie_gpu_job *job, lmie_gpu_result *res) {
      Workspace *w = nullptr;
      if (ws_acquire(job->device, &w)) return -1;
      res->state = nullptr;
+     res->precision = LMIE_PRECISION_FP64;   /* the re-weighting of save_control 2 is FP64 */
      int rc = job->save_control == 2 ? saved_impl(job, res, w) : forward_impl(job, res, w);
      if (rc) {
           (void) hipStreamSynchronize(w->side);

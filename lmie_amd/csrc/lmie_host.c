@@ -119,6 +119,10 @@ static int fail(const char *where, const char *why) {
 
 const char *lmie_last_error(void) { return g_err; }
 
+/* arithmetic the angular contraction of this thread's last solution ran in */
+static __thread int g_last_precision = LMIE_PRECISION_FP64;
+int lmie_last_precision(void) { return g_last_precision; }
+
 /* ------------------------------------------------------------------------- */
 /* Pointer-table blocks (misc/gutil_alloc.c:19-75 layout): row pointers, then
    contiguous row-major data, one malloc, released with free(). */
@@ -679,6 +683,7 @@ static int core_run(const core_args *c, core_out *o) {
           goto done;
      }
      o->state = res.state;
+     g_last_precision = res.precision;
      /* per-channel raw tangent sums -> seed order */
      double *rs_l = vec((size_t) 4 * (nd ? nd : 1));
      for (int t = 0; t < c->nd1; ++t)

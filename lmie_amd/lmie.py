@@ -62,6 +62,7 @@ def bind_all(lib):
     lib.lmie_last_batch_fused.argtypes = [C.c_int]
     lib.lmie_last_batch_fused.restype = C.c_int
     lib.lmie_last_error.restype = C.c_char_p
+    lib.lmie_last_precision.restype = C.c_int
     lib.lmie_gpu_last_error.restype = C.c_char_p
     lib.lmie_set_timing.argtypes = [C.c_int]
     lib.lmie_get_timing.argtypes = [C.POINTER(C.c_double), C.c_int]

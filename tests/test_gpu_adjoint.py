@@ -15,7 +15,7 @@ from lmie_amd import lmie, workloads as W
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp64": 1e-8, "mixed": 1e-4}
+TOL = {"fp64": 1e-8, "mixed": 1e-5}
 
 
 def say(*a):
@@ -37,6 +37,7 @@ def check(g, ref, tol, label, floor_frac=1e-3):
     err = np.abs(g - ref) / np.maximum(np.abs(ref), floor)
     say("  %s max rel err %.2e  (%s)" % (label, err.max(), " ".join("%.3e" % v for v in g)))
     assert err.max() <= tol, (label, g, ref)
+    return err
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
@@ -185,7 +186,7 @@ def test_autograd_wrapper(product):
     assert np.allclose(theta.grad.numpy(), ref, rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("env", [("LMIE_MIX_WAVES", "4"), ("LMIE_MIX_WAVES", "32"),
+@pytest.mark.parametrize("env", [("LMIE_MIX_PRODUCTS", "3"),
                                  ("LMIE_UGEMM_WAVES", "4"), ("LMIE_UGEMM_WAVES", "2"),
                                  ("LMIE_TILE_GROUP", "4")])
 def test_contraction_shapes_agree(product, env):
@@ -219,9 +220,10 @@ def test_batch_of_cfg3_cases_in_one_call(product, workers):
     single call bit for bit, the library's L2 loss equals the loss formed
     here, and the gradients meet the reference tangents at the mixed bar.
     The mixed path's gradient errors are normwise (about 1e-7 of ||g||), and
-    in some of these cases one component cancels to 1e-3 of ||g||, so the
-    per-component floor here is 1e-2 ||g_ref||: an absolute error up to
-    1e-6 ||g_ref|| on such a component."""
+    in some of these cases one component cancels to 1e-3 of ||g||: bars
+    ||g - g_ref|| <= 1e-6 ||g_ref|| per case, and per component
+    |g - g_ref| <= 1e-5 max(|g_ref|, 1e-2 ||g_ref||) and
+    <= 1e-4 max(|g_ref|, 1e-3 ||g_ref||)."""
     say("case: cfg3 batch, 32 cases in one call, mixed, workers", workers)
     z = np.load(load_golden.__globals__["GOLDEN"] + "/cfg3.npz")
     cases = W.cfg3_cases()
@@ -246,7 +248,12 @@ def test_batch_of_cfg3_cases_in_one_call(product, workers):
         loss = float(np.sum(w * (out["gc"] - t) ** 2))
         assert abs(loss_b - loss) <= 1e-12 * abs(loss), (c, loss_b, loss)
         ours = [g_b["a1"], g_b["a2"] * seeds["a2"][1], g_b["mr"], g_b["mi"]]
-        check(ours, z["loss_grad"][c], TOL["mixed"], "case %d" % c, floor_frac=1e-2)
+        ref = np.asarray(z["loss_grad"][c], float)
+        nw = np.linalg.norm(np.asarray(ours) - ref) / np.linalg.norm(ref)
+        say("  case %d normwise %.2e" % (c, nw))
+        assert nw <= 1e-6, (c, nw)
+        check(ours, ref, TOL["mixed"], "case %d" % c, floor_frac=1e-2)
+        check(ours, ref, 1e-4, "case %d, floor 1e-3" % c, floor_frac=1e-3)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed-bf16"])
