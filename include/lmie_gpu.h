@@ -115,6 +115,20 @@ void lmie_gpu_batch_close(void *batch);
 int lmie_gpu_phase_func(int n_set, int n_coef, int ld, const double *coefs, int n_theta,
                         const double *theta, double *pf, int gsf, int device);
 
+/* lmie_gpu_phase_func without any device allocation: no counterpart in the
+   reference; for callers that cannot allocate device memory (code under
+   stream capture, serving loops, harnesses that forbid hipMalloc).  Same
+   arguments, kernel and results (bit for bit) as lmie_gpu_phase_func; the
+   staging of coefficients, angles and phase function lives in a static
+   device array of 2^21 doubles (16 MiB) per GPU, held until the process
+   ends.  Calls whose staging, 6 n_set ld + 6 n_set n_theta + n_theta doubles
+   (each part rounded up to 32), exceeds it fail with -1 before any device
+   work.  The array keeps the last call's contents; calls on one GPU from
+   threads of one process take turns on a lock per device (overlapping calls
+   would race on the array). */
+int lmie_gpu_phase_func_static(int n_set, int n_coef, int ld, const double *coefs, int n_theta,
+                               const double *theta, double *pf, int gsf, int device);
+
 /* Error text of the last failing call on this thread ("" if none). */
 const char *lmie_gpu_last_error(void);
 

@@ -38,5 +38,6 @@ EXPORTED = (
     "lmie_core_solution", "lmie_core_solution_ex", "lmie_solution_ex", "lmie_solution_mix",
     "lmie_solution_adjoint", "lmie_last_error", "lmie_set_timing", "lmie_get_timing",
     # include/lmie_gpu.h
-    "lmie_gpu_forward", "lmie_gpu_adjoint", "lmie_gpu_phase_func", "lmie_gpu_last_error",
+    "lmie_gpu_forward", "lmie_gpu_adjoint", "lmie_gpu_phase_func", "lmie_gpu_phase_func_static",
+    "lmie_gpu_last_error",
 )

@@ -41,6 +41,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -4714,6 +4715,64 @@ int lmie_gpu_phase_func(int n_set, int n_coef, int ld, const double *coefs, int 
      HIPCHECK(hipFree(dT));
      HIPCHECK(hipFree(dP));
      return 0;
+}
+
+// Staging of lmie_gpu_phase_func_static: the coefficients, the angles and the
+// phase function of one call, in one file-scope device array in place of the
+// three device blocks lmie_gpu_phase_func allocates per call.  Every GPU the
+// library runs on holds its own copy of the array (16 MiB) until the process
+// ends.  In host code the name `scratch` is not a device address: the address
+// of each device's copy is taken once (hipGetSymbolAddress, with that device
+// current) and kept in pf_scratch[device]; only that address is passed on.
+constexpr size_t PF_SCRATCH_DOUBLES = (size_t) 1 << 21;
+constexpr int PF_MAX_DEVICES = 64;
+static __device__ __align__(256) double scratch[PF_SCRATCH_DOUBLES];
+static double *pf_scratch[PF_MAX_DEVICES];     // guarded by pf_lock[device]
+static std::mutex pf_lock[PF_MAX_DEVICES];
+
+static int phase_func_static_locked(int n_set, int n_coef, int ld, const double *coefs, int n_theta,
+                                    const double *theta, double *pf, int gsf, int device, size_t ot, size_t op) {
+     HIPCHECK(hipSetDevice(device));
+     if (!pf_scratch[device]) {
+          void *p = nullptr;
+          HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+          if (!p) return set_err("hipGetSymbolAddress gave no address for the phase-function scratch");
+          pf_scratch[device] = (double *) p;
+     }
+     double *dX = pf_scratch[device], *dT = dX + ot, *dP = dX + op;
+     const size_t nx = (size_t) n_set * 6 * ld, np = (size_t) n_set * 6 * n_theta;
+     if (nx) HIPCHECK(hipMemcpy(dX, coefs, sizeof(double) * nx, hipMemcpyHostToDevice));
+     HIPCHECK(hipMemcpy(dT, theta, sizeof(double) * n_theta, hipMemcpyHostToDevice));
+     const int tot = n_set * n_theta;
+     hipLaunchKernelGGL(phase_at_angles, dim3((tot + 127) / 128), dim3(128), 0, 0, n_set, n_coef, ld,
+                        (const double *) dX, n_theta, (const double *) dT, dP, gsf);
+     HIPCHECK(hipGetLastError());
+     HIPCHECK(hipMemcpy(pf, dP, sizeof(double) * np, hipMemcpyDeviceToHost));
+     return 0;
+}
+
+int lmie_gpu_phase_func_static(int n_set, int n_coef, int ld, const double *coefs, int n_theta,
+                               const double *theta, double *pf, int gsf, int device) {
+     g_err[0] = 0;
+     if (n_set <= 0 || n_theta <= 0) return 0;
+     if (n_coef > ld || n_coef < 0) return set_err("n_coef %d outside [0, %d]", n_coef, ld);
+     if (device < 0 || device >= PF_MAX_DEVICES)
+          return set_err("device %d outside [0, %d)", device, PF_MAX_DEVICES);
+     // layout in the array (doubles): coefficients at 0, angles at ot, phase
+     // function at op, each start a multiple of 32 (256 bytes); every factor
+     // is bounded first so that the products cannot wrap
+     const size_t cap = PF_SCRATCH_DOUBLES;
+     if ((size_t) n_set > cap || (size_t) ld > cap || (size_t) n_theta > cap)
+          return set_err("lmie_gpu_phase_func_static: n_set %d, ld %d, n_theta %d exceed its static scratch of %zu doubles",
+                         n_set, ld, n_theta, cap);
+     const size_t nx = (size_t) n_set * 6 * ld, np = (size_t) n_set * 6 * n_theta;
+     const size_t ot = (nx + 31) / 32 * 32, op = (ot + (size_t) n_theta + 31) / 32 * 32;
+     if (op + np > cap)
+          return set_err("lmie_gpu_phase_func_static: %zu doubles of staging (coefficients %zu, angles %d, "
+                         "phase function %zu) exceed its static scratch of %zu doubles",
+                         op + np, nx, n_theta, np, cap);
+     std::lock_guard<std::mutex> hold(pf_lock[device]);
+     return phase_func_static_locked(n_set, n_coef, ld, coefs, n_theta, theta, pf, gsf, device, ot, op);
 }
 
 int lmie_gpu_gauss_rule(int n, double *x, double *wt, int device) {
