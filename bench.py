@@ -29,6 +29,9 @@ FP64 GPU forward of the same workload is timed beside it.  The JSON line reports
                of the plain FP64 forward over the default path's whole calls
   config.fp64_angular  the FP64 forward's angular contraction: device time, the
                time its algorithmic FLOP take at the FP64 matrix peak, their ratio
+  config.accuracy  the timed run's expansion coefficients against the FP64
+               solution of the same workload: per set, normwise and the largest
+               elementwise relative error over |c| >= 1e-3 max |c| of the set
   cpu_baseline the reference's pthreads CPU path (oracle/_ref) on the same
                workload (forward only; the reference has no reverse pass), on
                this host's cores (up to 16 threads: one GPU's share of a node),
@@ -240,9 +243,11 @@ def main():
             for k in ("adj_angular", "adj_coefs", "adj_gemm"):
                 tm[k] = t_rev[k]
             tm["total"] = t_fwd["total"] + t_rev["total"]
-            return tm, grad
-        lmie.solve_mix(lib, modes, weights, path)
-        return lmie.timings(lib), None
+            tm["fwd_call"], tm["rev_call"] = t_fwd["call"], t_rev["call"]
+            tm["call"] = t_fwd["call"] + t_rev["call"]
+            return tm, grad, out
+        out = lmie.solve_mix(lib, modes, weights, path)
+        return lmie.timings(lib), None, out
 
     def run(path, steps, warmup):
         """Mean device time per step (inputs resident in HBM: the library's own
@@ -254,17 +259,17 @@ def main():
         t_all = []
         t0 = time.perf_counter()
         for _ in range(steps):
-            tm, grad = step(path)
+            tm, grad, out = step(path)
             t_all.append(tm)
         wall = (time.perf_counter() - t0) / steps
         lib.lmie_set_timing(0)
         stages = {k: float(np.mean([t[k] for t in t_all])) for k in t_all[0]}
-        return stages["total"] * 1e-3, wall, stages["angular"], stages, grad
+        return stages["total"] * 1e-3, wall, stages["angular"], stages, grad, out
 
-    dt, wall, ang_ms, stages, grad = run(a.path, a.steps, a.warmup)
+    dt, wall, ang_ms, stages, grad, out_last = run(a.path, a.steps, a.warmup)
     ref_dt = ref_wall = None
     if a.path != "fp64" and not a.no_fp64_compare:
-        ref_dt, ref_wall, _, ref_stages, _ = run("fp64", max(2, a.steps // 3), 1)
+        ref_dt, ref_wall, _, ref_stages, _, _ = run("fp64", max(2, a.steps // 3), 1)
 
     mixed = a.path.startswith("mixed")
     peak = PEAK["bf16"] if mixed else PEAK["fp64"]
@@ -293,6 +298,20 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as fh:
             traffic = json.load(fh).get(kname, {}).get("bytes_per_launch")
+    # error of the last timed step's coefficients against the FP64 solution of
+    # the same workload (c_ref): per set, normwise and the largest elementwise
+    # relative error over |c| >= 1e-3 max |c| of the set
+    acc = {}
+    for k, name in enumerate(("alpha1", "alpha2", "alpha3", "alpha4", "beta1", "beta2")):
+        r = np.asarray(c_ref[k], float)
+        c = np.asarray(out_last["gc"][k], float)[:r.size]
+        big = np.abs(r) >= 1e-3 * np.abs(r).max()
+        acc[name] = {"normwise": float(np.linalg.norm(c - r) / np.linalg.norm(r)),
+                     "elem_max": float(np.max(np.abs(c[big] - r[big]) / np.abs(r[big])))}
+    accuracy = {"vs": "FP64 GPU solution of the same workload", "sets": acc,
+                "normwise_max": max(v["normwise"] for v in acc.values()),
+                "elem_max": max(v["elem_max"] for v in acc.values()),
+                "elem_max_without_beta1": max(v["elem_max"] for k, v in acc.items() if k != "beta1")}
     line = {
         "metric": METRIC, "value": units / dt, "unit": "size-pt*n-term/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
@@ -311,6 +330,7 @@ def main():
                    "fp64_wall_ms_per_step": None if ref_wall is None else ref_wall * 1e3,
                    "wall_speedup_vs_fp64": None if ref_wall is None else ref_wall / wall,
                    "fp64_angular": fp64_angular,
+                   "accuracy": accuracy,
                    "timing": "ms_per_step, value and fp64_ms_per_step: device pipeline per step with "
                              "inputs in HBM (HIP events on the library stream, forward + reverse); "
                              "wall_*: whole calls including host preparation and PCIe transfers"},

@@ -132,6 +132,12 @@ int lmie_gpu_phase_func_static(int n_set, int n_coef, int ld, const double *coef
 /* Error text of the last failing call on this thread ("" if none). */
 const char *lmie_gpu_last_error(void);
 
+/* Host wall-clock time of entry k >= LMIE_T_HOST_GRID of enum lmie_timer
+   (include/lmie_hip.h) for this thread's current call, kept when timing is on;
+   lmie_gpu_host_times_clear zeroes those entries at the start of a call. */
+void lmie_gpu_host_time(int k, double ms);
+void lmie_gpu_host_times_clear(void);
+
 /* Diagnostics (tests and benchmarks; no counterpart in the reference):
      lmie_gpu_gauss_rule        the Gauss-Legendre rule of the angle kernels,
                                 n nodes ascending and weights (misc/rtutil_math.c:73-129)

@@ -217,7 +217,9 @@ const char *lmie_last_error(void);
 int lmie_last_precision(void);
 
 /* Per-kernel device time of the last call on this thread, in milliseconds,
-   measured with HIP events on the library's stream when enabled. */
+   measured with HIP events on the library's stream when enabled; the
+   LMIE_T_HOST_* and LMIE_T_CALL entries are host wall-clock times of the
+   same call. */
 enum lmie_timer {
      LMIE_T_TABLES = 0,    /* Gauss nodes + angular and expansion tables  */
      LMIE_T_COEFS,         /* per-size recurrences, a_n/b_n, series        */
@@ -228,6 +230,11 @@ enum lmie_timer {
      LMIE_T_ADJ_COEFS,     /* adjoint per-size recurrences                 */
      LMIE_T_TOTAL,         /* whole device pipeline of the call            */
      LMIE_T_ADJ_GEMM,      /* reverse contraction kernel alone (mixed)     */
+     LMIE_T_HOST_GRID,     /* host: size grid and number densities          */
+     LMIE_T_HOST_PREP,     /* host: device-layer set-up before the first launch */
+     LMIE_T_HOST_WAIT,     /* host: wait for the reverse pass's density
+                              tangents (built beside the device pipeline)  */
+     LMIE_T_CALL,          /* host: the whole call, entry to return         */
      LMIE_N_TIMERS
 };
 int lmie_set_timing(int on);

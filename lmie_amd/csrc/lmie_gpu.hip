@@ -41,6 +41,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -198,14 +199,19 @@ __device__ __forceinline__ cd crecip_nr(cd b) {
      const double q = rcp_nr(b.r * b.r + b.i * b.i);
      return cd{b.r * q, -b.i * q};
 }
-// Smith's division with the reciprocals of rcp_nr (about 1 ulp)
+// Smith's division with the reciprocals of rcp_nr (about 1 ulp).  Both of
+// Smith's cases in one instruction stream: with (p, q) the divisor's larger
+// and smaller component and (u, v) the dividend's matching pair, the quotient
+// is ((u + v t) / d, s (v - u t) / d), t = q / p, d = p + q t, s = +1 when
+// the real part is the larger (else -1); the same operations as the two
+// branches, so the same bits, without divergent lanes.
 __device__ __forceinline__ cd cdiv_nr(cd a, cd b) {
-     if (fabs(b.r) >= fabs(b.i)) {
-          const double t = b.i * rcp_nr(b.r), rd = rcp_nr(b.r + b.i * t);
-          return cd{(a.r + a.i * t) * rd, (a.i - a.r * t) * rd};
-     }
-     const double t = b.r * rcp_nr(b.i), rd = rcp_nr(b.r * t + b.i);
-     return cd{(a.r * t + a.i) * rd, (a.i * t - a.r) * rd};
+     const bool rr = fabs(b.r) >= fabs(b.i);
+     const double p = rr ? b.r : b.i, q = rr ? b.i : b.r;
+     const double u = rr ? a.r : a.i, v = rr ? a.i : a.r;
+     const double t = q * rcp_nr(p), rd = rcp_nr(p + q * t);
+     const double im = (v - u * t) * rd;
+     return cd{(u + v * t) * rd, rr ? im : -im};
 }
 
 // ---------------------------------------------------------------------------
@@ -801,10 +807,9 @@ __global__ __launch_bounds__(64) void mie_chains_b(const mie_chains_B *cases) {
 
 
 // a_j, b_j (src/lmie_core.c:557-583), operations as mie_coefs
-__device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, double psi, double psi0, double chi1,
+__device__ __forceinline__ void mie_ab_jx(double jx, cd m, cd im, cd Dj, double psi, double psi0, double chi1,
                                           double chi0, cd *a, cd *b) {
 #pragma clang fp contract(off)
-     const double jx = j / x;
      const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
      const cd u = cadd(cmul(Dj, im), cmk(jx, 0.));   // D/m with im = 1/m
      const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
@@ -812,6 +817,11 @@ __device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, d
      const cd nb = csub(cscale(v, psi), cmk(psi0, 0.)), db = csub(cmul(v, zj), zjm);
      *a = cdiv_nr(na, da);
      *b = cdiv_nr(nb, db);
+}
+// a_j, b_j of size parameter x (jx = j / x)
+__device__ __forceinline__ void mie_ab_at(int j, double x, cd m, cd im, cd Dj, double psi, double psi0, double chi1,
+                                          double chi0, cd *a, cd *b) {
+     mie_ab_jx(j / x, m, im, Dj, psi, psi0, chi1, chi0, a, b);
 }
 
 // Size-independent factors of order j (the reference's expressions, so the
@@ -2435,7 +2445,10 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
           double sgn = (lo & 1) ? -1. : 1.;
           const double ix2 = 1. / (x * x);
+          double jx_next = lo / x;        // j / x of the loop's order, formed one order ahead
           for (int j = lo; j < hi; ++j) {
+               const double jx = jx_next;
+               jx_next = (j + 1) / x;
                // the window's next entries
                const double ps2 = rd(Psi, j + 2, j + 2 <= n1), ch2 = rd(Chi, j + 2, j + 2 <= n1);
                const cd D2 = cmk(rd(Dre, j + 2, j + 2 < n2), rd(Dim, j + 2, j + 2 < n2));
@@ -2443,11 +2456,10 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                const double v2 = rdu(2, j + 1, j + 1 <= n1), v3 = rdu(3, j + 1, j + 1 <= n1);
                cd lc = cmk(0., 0.);
                if (j <= n1) {
-                    if (j < n1) mie_ab_at(j + 1, x, m, im1, D1, ps1, ps0, ch1, ch0, &an, &bn);
+                    if (j < n1) mie_ab_jx(jx_next, m, im1, D1, ps1, ps0, ch1, ch0, &an, &bn);
                     const double psi = ps0;
                     const cd zj = cmk(ps0, ch0), zjm = cmk(psm, chm);
                     const cd Dj = D0;
-                    const double jx = j / x;
                     const double fv1 = 2. * j + 1., fv4 = p.ft[FT_FV4 * ld + j];
                     cd Sb, Tb;
                     if (p.Uf) {
@@ -2868,6 +2880,93 @@ struct PinBuf {      // pinned host memory, grown on demand
      }
 };
 
+// Many small copies as one: each job moves n doubles (src NULL: writes zeros).
+struct CopyJob {
+     const double *src;
+     double *dst;
+     size_t n;
+};
+__global__ __launch_bounds__(256) void copy_jobs(const CopyJob *jobs) {
+     const CopyJob j = jobs[blockIdx.y];
+     for (size_t e = (size_t) blockIdx.x * 256 + threadIdx.x; e < j.n; e += (size_t) gridDim.x * 256)
+          j.dst[e] = j.src ? j.src[e] : 0.;
+}
+
+// Uploads of a batch gathered into one host-to-device copy and one copy_jobs
+// launch.  Sizes are whole doubles.
+struct Upload {
+     struct Item { size_t off; void *dst; size_t n8; bool zero; };
+     std::vector<unsigned char> h;
+     std::vector<Item> items;
+     int add(void *dst, const void *src, size_t bytes) {
+          if (bytes % 8) return set_err("internal: upload of %zu bytes", bytes);
+          const size_t off = h.size();
+          h.resize(off + bytes);
+          memcpy(h.data() + off, src, bytes);
+          items.push_back(Item{off, dst, bytes / 8, false});
+          return 0;
+     }
+     int zero(void *dst, size_t bytes) {
+          if (bytes % 8) return set_err("internal: fill of %zu bytes", bytes);
+          items.push_back(Item{0, dst, bytes / 8, true});
+          return 0;
+     }
+     // drops what was queued after the given sizes (a case that failed)
+     void rewind(size_t n_items, size_t n_bytes) {
+          items.resize(n_items);
+          h.resize(n_bytes);
+     }
+     int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
+          if (items.empty()) return 0;
+          if (items.size() > 65535) return set_err("batch of %zu uploads (at most 65535)", items.size());
+          const size_t data = (h.size() + 255) & ~(size_t) 255, total = data + sizeof(CopyJob) * items.size();
+          if (dev.get(total) || pin.get(total)) return -1;
+          unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
+          memcpy(hp, h.data(), h.size());
+          CopyJob *jobs = (CopyJob *) (hp + data);
+          for (size_t k = 0; k < items.size(); ++k)
+               jobs[k] = CopyJob{items[k].zero ? nullptr : (const double *) (base + items[k].off),
+                                 (double *) items[k].dst, items[k].n8};
+          HIPCHECK(hipMemcpyAsync(base, hp, total, hipMemcpyHostToDevice, s));
+          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) items.size()), dim3(256), 0, s,
+                             (const CopyJob *) (base + data));
+          HIPCHECK(hipGetLastError());
+          return 0;
+     }
+};
+
+// Results of a batch gathered by one copy_jobs launch into one device-to-host
+// copy; flush waits for the stream and scatters into the callers' arrays.
+struct Download {
+     struct Item { const void *src; void *dst; size_t off, n8; };
+     std::vector<Item> items;
+     size_t total = 0;
+     int add(void *dst, const void *src, size_t bytes) {
+          if (bytes % 8) return set_err("internal: download of %zu bytes", bytes);
+          items.push_back(Item{src, dst, total, bytes / 8});
+          total += bytes;
+          return 0;
+     }
+     int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
+          if (items.empty()) return 0;
+          if (items.size() > 65535) return set_err("batch of %zu results (at most 65535)", items.size());
+          const size_t data = (total + 255) & ~(size_t) 255, jb = sizeof(CopyJob) * items.size();
+          if (dev.get(data + jb) || pin.get(data + jb)) return -1;
+          unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
+          CopyJob *jobs = (CopyJob *) (hp + data);
+          for (size_t k = 0; k < items.size(); ++k)
+               jobs[k] = CopyJob{(const double *) items[k].src, (double *) (base + items[k].off), items[k].n8};
+          HIPCHECK(hipMemcpyAsync(base + data, hp + data, jb, hipMemcpyHostToDevice, s));
+          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) items.size()), dim3(256), 0, s,
+                             (const CopyJob *) (base + data));
+          HIPCHECK(hipGetLastError());
+          HIPCHECK(hipMemcpyAsync(hp, base, total, hipMemcpyDeviceToHost, s));
+          HIPCHECK(hipStreamSynchronize(s));
+          for (const Item &it : items) memcpy(it.dst, hp + it.off, 8 * it.n8);
+          return 0;
+     }
+};
+
 struct Workspace {
      int device = -1;
      hipStream_t stream = nullptr;
@@ -2889,6 +2988,9 @@ struct Workspace {
      Buf bargs, bup, bdown;         // per-case arguments and staging of a batch launched from here
      PinBuf pup, pdown;
      PinBuf pres;                   // pinned landing area of a call's results (one DMA each)
+     Upload fin;                    // a single call's inputs, one copy (forward_impl)
+     Buf finDev;
+     PinBuf finPin;
      int home = -1, busy = 0;       // batch slot of this workspace (-1: general pool)
      int n2rows = 0, ng = 0;
      int ushape = 8, ntask = 0;     // reverse contraction: shape and task count of the pending pass
@@ -2984,7 +3086,7 @@ struct Timer {
           if (on) (void) hipEventRecord(w->ev[2 * id + end], st ? st : w->stream);
      }
      void collect(const int *ids, int n) {
-          for (int k = 0; k < LMIE_N_TIMERS; ++k) g_times[k] = 0.;
+          for (int k = 0; k < LMIE_T_HOST_GRID; ++k) g_times[k] = 0.;
           if (!on) return;
           for (int k = 0; k < n; ++k) {
                float ms = 0.f;
@@ -3091,14 +3193,14 @@ MixShapeInfo mix_shape() {
 }
 
 // Size tiles per group of xcd_tasks in the forward mixed contraction
-// (environment LMIE_MIX_GROUP, read at every call, default 8): the 32
-// workgroups in flight on one XCD are then 8 size tiles x 4 angle tiles, so
-// each angle tile's operand is read into that XCD's L2 for 8 size tiles and
-// each size tile's for 4 angle tiles.
+// (environment LMIE_MIX_GROUP, read at every call, default 4): the 32
+// workgroups in flight on one XCD are then 4 size tiles x 8 angle tiles, so
+// each angle tile's operand is read into that XCD's L2 for 4 size tiles and
+// each size tile's (the larger operand per tile) for 8 angle tiles.
 int mix_group() {
      const char *e = getenv("LMIE_MIX_GROUP");
-     const int g = e ? atoi(e) : 8;
-     return (g < 1 || g > 64) ? 8 : g;
+     const int g = e ? atoi(e) : 4;
+     return (g < 1 || g > 64) ? 4 : g;
 }
 
 // Size tiles per group of xcd_tasks (environment LMIE_TILE_GROUP, read at
@@ -3121,7 +3223,9 @@ void gauss_guesses(int n, double *g) {
      }
 }
 
+
 int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
+     const auto t_entry = std::chrono::steady_clock::now();
      const int N = J->n_size, Np = rup(N, 2 * TILE);   // whole tiles of 128 sizes
      const int n1max = J->n1_max, Q = J->n_qang, C = J->n_coef;
      const int H = Q / 2, Hp = rup(H > 0 ? H : 1, TILE), Kp = rup(n1max, 32);
@@ -3169,65 +3273,63 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
          w->n2.get(sizeof(int) * Np) || w->kfill.get(sizeof(int) * nt64) || w->ktile32.get(sizeof(int) * nt32) ||
          w->ktile64.get(sizeof(int) * nt64) || w->dx.get(sizeof(double) * Np * (nt ? nt : 1)) ||
          w->dm.get(sizeof(double) * 2 * (nt ? nt : 1)) || w->dft.get(sizeof(double) * Np * (nt ? nt : 1)) ||
-         w->dfw.get(sizeof(double) * Np * (nw ? nw : 1)))
+         w->dfw.get(sizeof(double) * Np * (nw ? nw : 1)) || w->sx.get(sizeof(double) * Np) ||
+         w->cx.get(sizeof(double) * Np) || w->S.get(sizeof(double) * 4 * Np * (1 + nt)))
           return -1;
-     HIPCHECK(hipMemsetAsync(w->x.p, 0, sizeof(double) * Np, s));
-     HIPCHECK(hipMemsetAsync(w->f.p, 0, sizeof(double) * Np, s));
-     HIPCHECK(hipMemcpyAsync(w->x.p, J->x, sizeof(double) * N, hipMemcpyHostToDevice, s));
+     // every input of the pipeline goes up in one pinned host-to-device copy
+     // and one copy_jobs launch (a dozen small pageable copies cost more host
+     // time than the data)
+     Upload &up = w->fin;
+     up.items.clear();
+     up.h.clear();
+     // x, f zero-padded to Np; sin x, cos x of every size from the host C
+     // library, as the reference (padding 0, 1)
+     w->hsc.assign(4 * (size_t) Np, 0.);
      {
-          // sin x, cos x of every size from the host C library, as the reference
-          if (w->sx.get(sizeof(double) * Np) || w->cx.get(sizeof(double) * Np)) return -1;
-          w->hsc.resize(2 * (size_t) Np);
+          double *hx = w->hsc.data(), *hf = hx + Np, *hs = hf + Np, *hc = hs + Np;
           for (int i = 0; i < Np; ++i) {
-               w->hsc[i] = i < N ? (J->sx ? J->sx[i] : sin(J->x[i])) : 0.;
-               w->hsc[Np + i] = i < N ? (J->cx ? J->cx[i] : cos(J->x[i])) : 1.;
+               hx[i] = i < N ? J->x[i] : 0.;
+               hf[i] = i < N ? J->f[i] : 0.;
+               hs[i] = i < N ? (J->sx ? J->sx[i] : sin(J->x[i])) : 0.;
+               hc[i] = i < N ? (J->cx ? J->cx[i] : cos(J->x[i])) : 1.;
           }
-          HIPCHECK(hipMemcpyAsync(w->sx.p, w->hsc.data(), sizeof(double) * Np, hipMemcpyHostToDevice, s));
-          HIPCHECK(hipMemcpyAsync(w->cx.p, w->hsc.data() + Np, sizeof(double) * Np, hipMemcpyHostToDevice, s));
+          if (up.add(w->x.p, hx, sizeof(double) * Np) || up.add(w->f.p, hf, sizeof(double) * Np) ||
+              up.add(w->sx.p, hs, sizeof(double) * Np) || up.add(w->cx.p, hc, sizeof(double) * Np) ||
+              up.add(w->n1.p, n1h.data(), sizeof(int) * Np) || up.add(w->n2.p, n2h.data(), sizeof(int) * Np) ||
+              up.add(w->kfill.p, kfill.data(), sizeof(int) * nt64) ||
+              up.add(w->ktile32.p, kt32.data(), sizeof(int) * nt32) ||
+              up.add(w->ktile64.p, kt64.data(), sizeof(int) * nt64))
+               return -1;
      }
-     HIPCHECK(hipMemcpyAsync(w->f.p, J->f, sizeof(double) * N, hipMemcpyHostToDevice, s));
-     HIPCHECK(hipMemcpyAsync(w->n1.p, n1h.data(), sizeof(int) * Np, hipMemcpyHostToDevice, s));
-     HIPCHECK(hipMemcpyAsync(w->n2.p, n2h.data(), sizeof(int) * Np, hipMemcpyHostToDevice, s));
-     HIPCHECK(hipMemcpyAsync(w->kfill.p, kfill.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
-     HIPCHECK(hipMemcpyAsync(w->ktile32.p, kt32.data(), sizeof(int) * nt32, hipMemcpyHostToDevice, s));
-     HIPCHECK(hipMemcpyAsync(w->ktile64.p, kt64.data(), sizeof(int) * nt64, hipMemcpyHostToDevice, s));
-     if (nt) {
-          HIPCHECK(hipMemsetAsync(w->dx.p, 0, sizeof(double) * Np * nt, s));
-          HIPCHECK(hipMemsetAsync(w->dft.p, 0, sizeof(double) * Np * nt, s));
-          for (int t = 0; t < nt; ++t) {
-               HIPCHECK(hipMemcpyAsync(w->dx.as<double>() + (size_t) t * Np, J->dx + (size_t) t * N,
-                                       sizeof(double) * N, hipMemcpyHostToDevice, s));
-               HIPCHECK(hipMemcpyAsync(w->dft.as<double>() + (size_t) t * Np, J->df_tan + (size_t) t * N,
-                                       sizeof(double) * N, hipMemcpyHostToDevice, s));
-          }
-          HIPCHECK(hipMemcpyAsync(w->dm.p, J->dm, sizeof(double) * 2 * nt, hipMemcpyHostToDevice, s));
-     }
-     if (nw) {
-          HIPCHECK(hipMemsetAsync(w->dfw.p, 0, sizeof(double) * Np * nw, s));
-          for (int t = 0; t < nw; ++t)
-               HIPCHECK(hipMemcpyAsync(w->dfw.as<double>() + (size_t) t * Np, J->df_wgt + (size_t) t * N,
-                                       sizeof(double) * N, hipMemcpyHostToDevice, s));
-     }
+     // seed columns zero-padded to Np
+     auto add_padded = [&](Buf &dst, const double *src, int rows) -> int {
+          std::vector<double> pad((size_t) Np * rows, 0.);
+          for (int t = 0; t < rows; ++t) memcpy(pad.data() + (size_t) t * Np, src + (size_t) t * N, sizeof(double) * N);
+          return up.add(dst.p, pad.data(), sizeof(double) * pad.size());
+     };
+     if (nt && (add_padded(w->dx, J->dx, nt) || add_padded(w->dft, J->df_tan, nt) ||
+                up.add(w->dm.p, J->dm, sizeof(double) * 2 * nt)))
+          return -1;
+     if (nw && add_padded(w->dfw, J->df_wgt, nw)) return -1;
      if (Q > 0) {
-          // initial guesses of the Gauss roots (host C library), with the inputs
+          // initial guesses of the Gauss roots (host C library)
           if (w->guess.get(sizeof(double) * H)) return -1;
           w->hguess.resize(H > 0 ? H : 1);
           gauss_guesses(Q, w->hguess.data());
-          HIPCHECK(hipMemcpyAsync(w->guess.p, w->hguess.data(), sizeof(double) * H, hipMemcpyHostToDevice, s));
+          if (up.add(w->guess.p, w->hguess.data(), sizeof(double) * H)) return -1;
      }
      // pinned landing area of the results: expansions [2][nch][6][C], then the sums
      if (w->pres.get(sizeof(double) * ((size_t) nch * 12 * C + 4 * nch))) return -1;
      if (mixed) {
-          // the contraction's task list, also with the inputs
+          // the contraction's task list
           xcd_tasks(ms.tm == 64 ? kt64 : kt32, [&](int) { return (H + ms.tn - 1) / ms.tn; }, mix_group(), w->htask);
-          if (w->order.get(sizeof(int2) * w->htask.size())) return -1;
-          HIPCHECK(hipMemcpyAsync(w->order.p, w->htask.data(), sizeof(int2) * w->htask.size(),
-                                  hipMemcpyHostToDevice, s));
+          if (w->order.get(sizeof(int2) * w->htask.size()) ||
+              up.add(w->order.p, w->htask.data(), sizeof(int2) * w->htask.size()))
+               return -1;
      }
      // series dot products: channel 0, then tangent, then weight-only channels
      std::vector<DotJob> dot_jobs;
      {
-          if (w->S.get(sizeof(double) * 4 * Np * (1 + nt))) return -1;
           const double *S0 = w->S.as<double>(), *f = w->f.as<double>();
           for (int q = 0; q < 4; ++q) dot_jobs.push_back(DotJob{f, S0 + (size_t) q * Np, nullptr, nullptr});
           for (int t = 0; t < nt; ++t)
@@ -3237,11 +3339,12 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           for (int t = 0; t < nw; ++t)
                for (int q = 0; q < 4; ++q)
                     dot_jobs.push_back(DotJob{w->dfw.as<double>() + (size_t) t * Np, S0 + (size_t) q * Np, nullptr, nullptr});
-          if (w->dotjobs.get(sizeof(DotJob) * dot_jobs.size()) || w->dots.get(sizeof(double) * dot_jobs.size()))
+          if (w->dotjobs.get(sizeof(DotJob) * dot_jobs.size()) || w->dots.get(sizeof(double) * dot_jobs.size()) ||
+              up.add(w->dotjobs.p, dot_jobs.data(), sizeof(DotJob) * dot_jobs.size()))
                return -1;
-          HIPCHECK(hipMemcpyAsync(w->dotjobs.p, dot_jobs.data(), sizeof(DotJob) * dot_jobs.size(),
-                                  hipMemcpyHostToDevice, s));
      }
+     if (up.flush(w->finDev, w->finPin, s)) return -1;
+     const double prep_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_entry).count();
      tm.mark(LMIE_T_TOTAL, 0);
 
      // angle tables, on the side streams while the per-size recurrences run
@@ -3552,6 +3655,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           for (int q = 0; q < 4; ++q) R->sums_ch[4 * (c - 1) + q] = dots[4 * c + q];
      const int ids[] = {LMIE_T_TABLES, LMIE_T_COEFS, LMIE_T_ANGULAR, LMIE_T_REDUCE, LMIE_T_EXPAND, LMIE_T_TOTAL};
      tm.collect(ids, Q > 0 ? 6 : 2);
+     lmie_gpu_host_time(LMIE_T_HOST_PREP, prep_ms);
      w->valid = 1;
      w->N = N; w->Np = Np; w->Kp = Kp; w->Hp = Hp; w->H = H; w->Q = Q; w->C = C; w->n1max = n1max;
      w->n2max = 0;
@@ -3846,92 +3950,6 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
 // Fused: mixed precision, value channel only, angles, save_control 0, kept
 // state (the forward of a reverse pass); other cases run one by one.
 
-// Many small copies as one: each job moves n doubles (src NULL: writes zeros).
-struct CopyJob {
-     const double *src;
-     double *dst;
-     size_t n;
-};
-__global__ __launch_bounds__(256) void copy_jobs(const CopyJob *jobs) {
-     const CopyJob j = jobs[blockIdx.y];
-     for (size_t e = (size_t) blockIdx.x * 256 + threadIdx.x; e < j.n; e += (size_t) gridDim.x * 256)
-          j.dst[e] = j.src ? j.src[e] : 0.;
-}
-
-// Uploads of a batch gathered into one host-to-device copy and one copy_jobs
-// launch.  Sizes are whole doubles.
-struct Upload {
-     struct Item { size_t off; void *dst; size_t n8; bool zero; };
-     std::vector<unsigned char> h;
-     std::vector<Item> items;
-     int add(void *dst, const void *src, size_t bytes) {
-          if (bytes % 8) return set_err("internal: upload of %zu bytes", bytes);
-          const size_t off = h.size();
-          h.resize(off + bytes);
-          memcpy(h.data() + off, src, bytes);
-          items.push_back(Item{off, dst, bytes / 8, false});
-          return 0;
-     }
-     int zero(void *dst, size_t bytes) {
-          if (bytes % 8) return set_err("internal: fill of %zu bytes", bytes);
-          items.push_back(Item{0, dst, bytes / 8, true});
-          return 0;
-     }
-     // drops what was queued after the given sizes (a case that failed)
-     void rewind(size_t n_items, size_t n_bytes) {
-          items.resize(n_items);
-          h.resize(n_bytes);
-     }
-     int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
-          if (items.empty()) return 0;
-          if (items.size() > 65535) return set_err("batch of %zu uploads (at most 65535)", items.size());
-          const size_t data = (h.size() + 255) & ~(size_t) 255, total = data + sizeof(CopyJob) * items.size();
-          if (dev.get(total) || pin.get(total)) return -1;
-          unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
-          memcpy(hp, h.data(), h.size());
-          CopyJob *jobs = (CopyJob *) (hp + data);
-          for (size_t k = 0; k < items.size(); ++k)
-               jobs[k] = CopyJob{items[k].zero ? nullptr : (const double *) (base + items[k].off),
-                                 (double *) items[k].dst, items[k].n8};
-          HIPCHECK(hipMemcpyAsync(base, hp, total, hipMemcpyHostToDevice, s));
-          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) items.size()), dim3(256), 0, s,
-                             (const CopyJob *) (base + data));
-          HIPCHECK(hipGetLastError());
-          return 0;
-     }
-};
-
-// Results of a batch gathered by one copy_jobs launch into one device-to-host
-// copy; flush waits for the stream and scatters into the callers' arrays.
-struct Download {
-     struct Item { const void *src; void *dst; size_t off, n8; };
-     std::vector<Item> items;
-     size_t total = 0;
-     int add(void *dst, const void *src, size_t bytes) {
-          if (bytes % 8) return set_err("internal: download of %zu bytes", bytes);
-          items.push_back(Item{src, dst, total, bytes / 8});
-          total += bytes;
-          return 0;
-     }
-     int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
-          if (items.empty()) return 0;
-          if (items.size() > 65535) return set_err("batch of %zu results (at most 65535)", items.size());
-          const size_t data = (total + 255) & ~(size_t) 255, jb = sizeof(CopyJob) * items.size();
-          if (dev.get(data + jb) || pin.get(data + jb)) return -1;
-          unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
-          CopyJob *jobs = (CopyJob *) (hp + data);
-          for (size_t k = 0; k < items.size(); ++k)
-               jobs[k] = CopyJob{(const double *) items[k].src, (double *) (base + items[k].off), items[k].n8};
-          HIPCHECK(hipMemcpyAsync(base + data, hp + data, jb, hipMemcpyHostToDevice, s));
-          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) items.size()), dim3(256), 0, s,
-                             (const CopyJob *) (base + data));
-          HIPCHECK(hipGetLastError());
-          HIPCHECK(hipMemcpyAsync(hp, base, total, hipMemcpyDeviceToHost, s));
-          HIPCHECK(hipStreamSynchronize(s));
-          for (const Item &it : items) memcpy(it.dst, hp + it.off, 8 * it.n8);
-          return 0;
-     }
-};
 
 struct BatchBlob {   // host staging of per-case argument arrays, one upload
      std::vector<unsigned char> h;
@@ -4579,6 +4597,14 @@ const char *lmie_gpu_last_error(void) { return g_err; }
 int lmie_set_timing(int on) {
      g_timing = on ? 1 : 0;
      return 0;
+}
+
+void lmie_gpu_host_time(int k, double ms) {
+     if (g_timing && k >= LMIE_T_HOST_GRID && k < LMIE_N_TIMERS) g_times[k] = ms;
+}
+
+void lmie_gpu_host_times_clear(void) {
+     for (int k = LMIE_T_HOST_GRID; k < LMIE_N_TIMERS; ++k) g_times[k] = 0.;
 }
 
 int lmie_get_timing(double *ms, int n) {

@@ -19,11 +19,19 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include <unistd.h>
 
 #include "../../include/lmie_interface.h"
 #include "../../include/lmie_hip.h"
 #include "../../include/lmie_gpu.h"
+
+/* host wall clock in ms, for the LMIE_T_HOST_* and LMIE_T_CALL timers */
+static double wall_ms(void) {
+     struct timespec t;
+     clock_gettime(CLOCK_MONOTONIC, &t);
+     return t.tv_sec * 1e3 + t.tv_nsec * 1e-6;
+}
 
 #define LM_PI 3.14159265358979323846264338327950288
 
@@ -544,6 +552,8 @@ typedef struct {
      double accuracy;
      const lmie_opts *opts;
      int keep;                      /* keep the device state for a reverse pass */
+     void (*on_device)(void *);     /* called once the host inputs are ready, just */
+     void *on_device_arg;           /* before the device pipeline (or NULL)        */
 } core_args;
 
 typedef struct {
@@ -677,6 +687,7 @@ static int core_run(const core_args *c, core_out *o) {
           job.saved1 = (*o->save1)[0];
           job.saved2 = Q > 0 ? (*o->save2)[0][0] : NULL;
      }
+     if (c->on_device) c->on_device(c->on_device_arg);
      rc = lmie_gpu_forward(&job, &res);
      if (rc) {
           fail("lmie_gpu_forward", lmie_gpu_last_error());
@@ -873,6 +884,8 @@ typedef struct {
      int calc_gc, calc_lc, calc_pf, n_angles, save_control;
      double lambda, mr, mi, accuracy;
      int keep;
+     void (*on_device)(void *);     /* core_args.on_device */
+     void *on_device_arg;
 } run_flags;
 
 static int solve_grid(const run_flags *rf, grid_t *g, const double *lambda_l, const double *mr_l,
@@ -915,6 +928,8 @@ static int solve_grid(const run_flags *rf, grid_t *g, const double *lambda_l, co
      c.r2 = g->r2; c.calc_gc = rf->calc_gc; c.calc_lc = rf->calc_lc; c.accuracy = rf->accuracy;
      c.opts = opts;
      c.keep = rf->keep;
+     c.on_device = rf->on_device;
+     c.on_device_arg = rf->on_device_arg;
      double dummy[4];
      core_out o = {cext, csca, cbak, g_, gc, lc, nd ? cext_l : dummy, nd ? csca_l : dummy,
                    nd ? cbak_l : dummy, nd ? g_l : dummy, gc_l, lc_l, save1, save2, 0, 0, {0, 0, 0, 0}, NULL};
@@ -956,11 +971,13 @@ static int solution2_l_opts(int calc_gc, int calc_lc, int calc_pf, enum size_dis
      seeds sv = {nd, {a1_l, a2_l, a3_l, a4_l, a5_l}, r1_l, r2_l};
      grid_t gr;
      memset(&gr, 0, sizeof gr);
+     const double tg = wall_ms();
      if (grid_build(dist_type, n_int1, n_int2, n_quad, a, r1, r2, &sv, &gr)) {
           grid_free(&gr);
           return fail("get_particle_dist", lmie_last_error());
      }
-     run_flags rf = {calc_gc, calc_lc, calc_pf, n_angles, save_control, lambda, mr, mi, accuracy, 0};
+     lmie_gpu_host_time(LMIE_T_HOST_GRID, wall_ms() - tg);
+     run_flags rf = {calc_gc, calc_lc, calc_pf, n_angles, save_control, lambda, mr, mi, accuracy, 0, NULL, NULL};
      int rc = solve_grid(&rf, &gr, lambda_l, mr_l, mi_l, opts, n_coef, r21, r22, norm, reff, veff,
                          gavg, vavg, ravg, rvw, cext, csca, cbak, g, gc, lc, theta, pf, r21_l, r22_l,
                          norm_l, reff_l, veff_l, gavg_l, vavg_l, ravg_l, rvw_l, cext_l, csca_l,
@@ -1012,8 +1029,20 @@ int lmie_solution2(int calc_gc, int calc_lc, int calc_pf, enum size_dist_type di
                              n_threads, use_mpi);
 }
 
+static int solution_opts_run(lmie_in_data *in, lmie_out_data *out, int alloc_out, int use_mpi,
+                             const lmie_opts *opts);
+
 static int solution_opts(lmie_in_data *in, lmie_out_data *out, int alloc_out, int use_mpi,
                          const lmie_opts *opts) {
+     const double t0 = wall_ms();
+     lmie_gpu_host_times_clear();
+     const int rc = solution_opts_run(in, out, alloc_out, use_mpi, opts);
+     lmie_gpu_host_time(LMIE_T_CALL, wall_ms() - t0);
+     return rc;
+}
+
+static int solution_opts_run(lmie_in_data *in, lmie_out_data *out, int alloc_out, int use_mpi,
+                             const lmie_opts *opts) {
      if (alloc_out) {
           int mc = lmie_calc_max_coef(in->lambda, in->dist_type, in->a1, in->a2, in->a3, in->r1, in->r2);
           if (lmie_out_alloc(in, out, mc)) return fail("lmie_out_alloc", "failed");
@@ -1129,8 +1158,20 @@ static void mix_seed_column(lmie_in_data *const *modes, int n_modes, int which, 
      }
 }
 
+static int solution_mix_run(lmie_in_data *const *modes, const double *weight, int n_modes,
+                            lmie_out_data *out, int alloc_out, const lmie_opts *opts);
+
 int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_modes,
                       lmie_out_data *out, int alloc_out, const lmie_opts *opts) {
+     const double t0 = wall_ms();
+     lmie_gpu_host_times_clear();
+     const int rc = solution_mix_run(modes, weight, n_modes, out, alloc_out, opts);
+     lmie_gpu_host_time(LMIE_T_CALL, wall_ms() - t0);
+     return rc;
+}
+
+static int solution_mix_run(lmie_in_data *const *modes, const double *weight, int n_modes,
+                            lmie_out_data *out, int alloc_out, const lmie_opts *opts) {
      if (n_modes < 1 || !modes || !modes[0]) return fail("lmie_solution_mix", "no modes");
      const lmie_in_data *m0 = modes[0];
      int nd = 0;
@@ -1146,13 +1187,15 @@ int lmie_solution_mix(lmie_in_data *const *modes, const double *weight, int n_mo
      grid_t g;
      /* as lmie_solution (src/lmie_interface.c:208-248): on a failure after
         alloc_out the output stays allocated and the caller frees it */
+     const double tg = wall_ms();
      if (mix_grid(modes, weight, n_modes, nd, &g)) return -1;
+     lmie_gpu_host_time(LMIE_T_HOST_GRID, wall_ms() - tg);
      double *lam_l = vec(nd), *mr_l = vec(nd), *mi_l = vec(nd);
      mix_seed_column(modes, n_modes, 0, lam_l);
      mix_seed_column(modes, n_modes, 1, mr_l);
      mix_seed_column(modes, n_modes, 2, mi_l);
      run_flags rf = {m0->calc_gc, m0->calc_lc, m0->calc_pf, m0->n_angles, m0->save_control,
-                     m0->lambda, m0->mr, m0->mi, m0->accuracy, 0};
+                     m0->lambda, m0->mr, m0->mi, m0->accuracy, 0, NULL, NULL};
      int rc = solve_grid(&rf, &g, lam_l, mr_l, mi_l, opts, &out->n_coef, &out->r1, &out->r2,
                          &out->norm, &out->reff, &out->veff, &out->gavg, &out->vavg, &out->ravg,
                          &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g, out->gc, out->lc,
@@ -1248,11 +1291,44 @@ static void *dn_build(void *vp) {
      return NULL;
 }
 
+/* dn_build on a helper thread, started by core_run's on_device hook (or
+   inline where no thread can be had, and on the threads of a fused batch) */
+typedef struct {
+     dn_job *job;
+     int running, done;
+     pthread_t th;
+} dn_start;
+
+static void dn_launch(void *vp) {
+     dn_start *st = vp;
+     if (st->running || st->done) return;
+     if (!g_batch_thread && pthread_create(&st->th, NULL, dn_build, st->job) == 0) {
+          st->running = 1;
+          return;
+     }
+     dn_build(st->job);
+     st->done = 1;
+}
+
 static void out_release(lmie_out_data *out) { out_release_nd(out, 0); }
+
+static int adjoint_forward_run(lmie_in_data *const *modes, const double *weight, int n_modes,
+                               lmie_out_data *out, int alloc_out, const lmie_opts *opts,
+                               lmie_adj_ctx **ctxp);
 
 int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n_modes,
                          lmie_out_data *out, int alloc_out, const lmie_opts *opts,
                          lmie_adj_ctx **ctxp) {
+     const double t0 = wall_ms();
+     lmie_gpu_host_times_clear();
+     const int rc = adjoint_forward_run(modes, weight, n_modes, out, alloc_out, opts, ctxp);
+     lmie_gpu_host_time(LMIE_T_CALL, wall_ms() - t0);
+     return rc;
+}
+
+static int adjoint_forward_run(lmie_in_data *const *modes, const double *weight, int n_modes,
+                               lmie_out_data *out, int alloc_out, const lmie_opts *opts,
+                               lmie_adj_ctx **ctxp) {
      if (n_modes < 1 || !modes || !modes[0] || !ctxp) return fail("lmie_adjoint_forward", "missing argument");
      *ctxp = NULL;
      const lmie_in_data *m0 = modes[0];
@@ -1267,19 +1343,23 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      lmie_in_data **nm = calloc(n_modes, sizeof(*nm));
      for (int k = 0; k < n_modes; ++k) { copies[k] = *modes[k]; copies[k].n_derivs = 0; nm[k] = &copies[k]; }
      grid_t g;
+     const double tg = wall_ms();
      int rc = mix_grid(nm, weight, n_modes, 0, &g);
+     lmie_gpu_host_time(LMIE_T_HOST_GRID, wall_ms() - tg);
      free(nm); free(copies);
      if (rc) {
           if (alloc_out) out_release(out);
           return -1;
      }
      /* the number density tangents of the modes (for the reverse pass) on a
-        helper thread while the forward runs on the GPU */
+        helper thread while the forward runs on the GPU: started once the
+        host's own per-size loops are done, so the two do not share the cores */
      dn_job dj = {modes, weight, n_modes, g.N, m0->lambda, NULL, NULL};
-     pthread_t dth;
-     const int dthr = !g_batch_thread && pthread_create(&dth, NULL, dn_build, &dj) == 0;
-     if (!dthr) dn_build(&dj);
-     run_flags rf = {m0->calc_gc, m0->calc_lc, 0, 0, 0, m0->lambda, m0->mr, m0->mi, m0->accuracy, 1};
+     dn_start st;
+     memset(&st, 0, sizeof st);
+     st.job = &dj;
+     run_flags rf = {m0->calc_gc, m0->calc_lc, 0, 0, 0, m0->lambda, m0->mr, m0->mi, m0->accuracy, 1,
+                     dn_launch, &st};
      core_out keep;
      double zero = 0.;
      rc = solve_grid(&rf, &g, &zero, &zero, &zero, opts, &out->n_coef, &out->r1, &out->r2,
@@ -1287,7 +1367,10 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
                      &out->rvw, &out->cext, &out->csca, &out->cbak, &out->g, out->gc, out->lc,
                      out->theta, out->pf, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
                      NULL, NULL, NULL, NULL, NULL, NULL, NULL, &out->save1, &out->save2, &keep);
-     if (dthr) pthread_join(dth, NULL);
+     const double tw = wall_ms();
+     if (st.running) pthread_join(st.th, NULL);
+     else if (!st.done && rc == 0) dn_build(&dj);     /* not started (cannot happen on success) */
+     lmie_gpu_host_time(LMIE_T_HOST_WAIT, wall_ms() - tw);
      if (rc) {
           adj_tangents_free(dj.df, n_modes);
           adj_tangents_free(dj.dx, n_modes);
@@ -1324,7 +1407,17 @@ int lmie_adjoint_forward(lmie_in_data *const *modes, const double *weight, int n
      return 0;
 }
 
+static int adjoint_reverse_run(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes);
+
 int lmie_adjoint_reverse(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes) {
+     const double t0 = wall_ms();
+     lmie_gpu_host_times_clear();
+     const int rc = adjoint_reverse_run(c, adj, grad, grad_modes);
+     lmie_gpu_host_time(LMIE_T_CALL, wall_ms() - t0);
+     return rc;
+}
+
+static int adjoint_reverse_run(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes) {
      if (!c || !adj || !grad) return fail("lmie_adjoint_reverse", "missing argument");
      int C = c->C;
      double Csca = c->Csca, s = c->lambda * c->lambda / (2. * LM_PI);

@@ -17,7 +17,8 @@ from .abi import LmieIn, LmieOut
 
 PRECISION = {"fp64": 0, "mixed": 1, "mixed-bf16": 2}
 
-TIMERS = ("tables", "coefs", "angular", "reduce", "expand", "adj_angular", "adj_coefs", "total", "adj_gemm")
+TIMERS = ("tables", "coefs", "angular", "reduce", "expand", "adj_angular", "adj_coefs", "total", "adj_gemm",
+          "host_grid", "host_prep", "host_wait", "call")
 
 
 class LmieOpts(C.Structure):

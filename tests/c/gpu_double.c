@@ -117,6 +117,9 @@ int lmie_gpu_phase_func(int n_set, int n_coef, int ld, const double *coefs, int 
      return 0;
 }
 
+void lmie_gpu_host_time(int k, double ms) { (void) k; (void) ms; }
+void lmie_gpu_host_times_clear(void) {}
+
 /* timing entry points of the device layer (include/lmie_hip.h) */
 int lmie_set_timing(int on) { (void) on; return 0; }
 int lmie_get_timing(double *ms, int n) {
