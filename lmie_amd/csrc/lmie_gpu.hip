@@ -3096,6 +3096,235 @@ struct Timer {
      }
 };
 
+// The FP64 contraction of a pass without tangent channels, with its operand
+// stages moved by direct loads (global_load_lds_dwordx4, 16 bytes a lane
+// straight into LDS, no registers) into two LDS buffers: the next stage is
+// loaded while the matrix cores work on this one, one barrier per stage, and
+// the stream of stages runs on across the size tiles of the workgroup (the
+// next tile's first stage loads during this tile's last stage and epilogue).
+// The products, their order and the epilogue are those of angular_f64, so the
+// results are the same bits.
+template <int WM, int WN, int NW>
+__global__ __launch_bounds__(256) void angular_f64_dma(AngArgs p) {
+     constexpr int TM = 32 * WM, TN = 32 * WN;
+     constexpr int SA = TM + 16, SB = TN + 16;       // LDS row strides (bank spread)
+     constexpr int AST = 4 * KC * SA, BST = 2 * KC * SB;
+     static_assert(TM * 8 <= 1024 && TN * 8 <= 1024 && KC % 4 == 0, "a row is one direct load of at most 64 lanes");
+     __shared__ double sA[2 * AST];
+     __shared__ double sB[2 * BST];
+     __shared__ double sR[NW * 8 * 32 * WN];
+
+     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+     const int wm = wave & 1, wn = wave >> 1;
+     const int l16 = lane & 15, lg = lane >> 4;
+     const int jt0 = blockIdx.x * TN;
+     // size blocks from the last (largest sizes, deepest) to the first, so
+     // the longest workgroups start first and the short ones fill the tail
+     const int tile0 = (gridDim.y - 1 - blockIdx.y) * p.tiles_per_block;
+     const size_t Np = p.Np, Hp = p.Hp, plane = (size_t) p.Kp * Np;
+     const int ntile = max(0, min(p.tiles_per_block, (int) (Np / TM) - tile0));
+     auto nch_of = [&](int t) { return p.ktile[tile0 + t] / KC; };
+
+     // stage (tile t, orders k0 .. k0 + KC - 1) into buffer b_: A rows
+     // (plane, order) of TM sizes and B rows ((pi+tau) / (pi-tau), order) of
+     // TN angles, one direct load per row, rows dealt over the four waves
+     auto stage_load = [&](int b_, int t, int k0) {
+          const int it0 = (tile0 + t) * TM;
+#pragma unroll
+          for (int u = 0; u < KC; ++u) {
+               const int row = wave + 4 * u, pl = row / KC, r = row % KC;
+               if (lane < TM / 2)
+                    __builtin_amdgcn_global_load_lds(
+                         (const void *) (p.A[0] + pl * plane + (size_t) (k0 + r) * Np + it0 + 2 * lane),
+                         (lds_ptr) (sA + b_ * AST + row * SA), 16, 0, 0);
+          }
+#pragma unroll
+          for (int u = 0; u < KC / 2; ++u) {
+               const int row = wave + 4 * u, pm = row / KC, r = row % KC;
+               const double *src = pm ? p.Bm : p.Bp;
+               if (lane < TN / 2)
+                    __builtin_amdgcn_global_load_lds((const void *) (src + (size_t) (k0 + r) * Hp + jt0 + 2 * lane),
+                                                     (lds_ptr) (sB + b_ * BST + row * SB), 16, 0, 0);
+          }
+     };
+
+     double ew[NW][WN][8];
+#pragma unroll
+     for (int c = 0; c < NW; ++c)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int q = 0; q < 8; ++q) ew[c][b][q] = 0.;
+
+     // issue cursor over the stream of stages (tile it_, chunk ik_)
+     int it_ = 0, ik_ = 0;
+     while (it_ < ntile && nch_of(it_) == 0) ++it_;
+     auto advance = [&]() {
+          if (++ik_ >= nch_of(it_)) {
+               ik_ = 0;
+               ++it_;
+               while (it_ < ntile && nch_of(it_) == 0) ++it_;
+          }
+     };
+     if (it_ < ntile) {
+          stage_load(0, it_, 0);
+          advance();
+     }
+     int gs = 0;                        // stages computed so far: buffer gs & 1
+     for (int t = 0; t < ntile; ++t) {
+          const int it0 = (tile0 + t) * TM, nch = nch_of(t);
+          v4d acc[WM][WN][8];
+#pragma unroll
+          for (int a = 0; a < WM; ++a)
+#pragma unroll
+               for (int b = 0; b < WN; ++b)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) acc[a][b][q] = v4d{0., 0., 0., 0.};
+          for (int kc = 0; kc < nch; ++kc, ++gs) {
+               const int buf = gs & 1, k0 = kc * KC;
+               // this wave's rows of the stage have landed (it is the only one
+               // in flight), then everyone's; every wave is also done with the
+               // other buffer, which takes the next stage
+               __builtin_amdgcn_s_waitcnt(0);
+               __builtin_amdgcn_s_barrier();
+               __asm__ volatile("" ::: "memory");
+               if (it_ < ntile) {
+                    stage_load(buf ^ 1, it_, ik_ * KC);
+                    advance();
+               }
+               const double *a0 = sA + buf * AST, *b0 = sB + buf * BST;
+#pragma unroll
+               for (int ks = 0; ks < KC / 4; ++ks) {
+                    const int r = ks * 4 + lg;                   // this lane's k row
+                    const double sg = ((k0 + r) & 1) ? -1. : 1.;   // s_n, n = k0 + r + 1
+                    // the -mu products take s_n on the angle operand: the same
+                    // products as s_n on the size operand (a sign), fewer multiplies
+                    double bp[WN], bm[WN], sbp[WN], sbm[WN];
+#pragma unroll
+                    for (int b = 0; b < WN; ++b) {
+                         const int jc = wn * 16 * WN + 16 * b + l16;
+                         bp[b] = b0[r * SB + jc];
+                         bm[b] = b0[(KC + r) * SB + jc];
+                         sbp[b] = sg * bp[b];
+                         sbm[b] = sg * bm[b];
+                    }
+#pragma unroll
+                    for (int a = 0; a < WM; ++a) {
+                         const int ic = wm * 16 * WM + 16 * a + l16;
+                         const double Sr = a0[(0 * KC + r) * SA + ic];
+                         const double Si = a0[(1 * KC + r) * SA + ic];
+                         const double Tr = a0[(2 * KC + r) * SA + ic];
+                         const double Ti = a0[(3 * KC + r) * SA + ic];
+#pragma unroll
+                         for (int b = 0; b < WN; ++b) {
+                              v4d *q = acc[a][b];
+                              q[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sr, bp[b], q[0], 0, 0, 0);
+                              q[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Si, bp[b], q[1], 0, 0, 0);
+                              q[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tr, bm[b], q[2], 0, 0, 0);
+                              q[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti, bm[b], q[3], 0, 0, 0);
+                              q[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sr, sbm[b], q[4], 0, 0, 0);
+                              q[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(Si, sbm[b], q[5], 0, 0, 0);
+                              q[6] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tr, sbp[b], q[6], 0, 0, 0);
+                              q[7] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti, sbp[b], q[7], 0, 0, 0);
+                         }
+                    }
+               }
+          }
+          // epilogue of the tile: Stokes elements, weights, reduction over
+          // this lane's sizes (the next stage is loading meanwhile)
+#pragma unroll
+          for (int a = 0; a < WM; ++a)
+#pragma unroll
+               for (int v = 0; v < 4; ++v) {
+                    const int i = it0 + wm * 16 * WM + 16 * a + lg + 4 * v;
+                    double wv[NW];
+#pragma unroll
+                    for (int c = 0; c < NW; ++c) wv[c] = p.W[c][i];
+#pragma unroll
+                    for (int b = 0; b < WN; ++b) {
+                         const v4d *q = acc[a][b];
+                         double Pp[4], Pm[4];
+                         stokes(q[0][v], q[1][v], q[2][v], q[3][v], Pp);
+                         stokes(q[4][v], q[5][v], q[6][v], q[7][v], Pm);
+#pragma unroll
+                         for (int c = 0; c < NW; ++c)
+#pragma unroll
+                              for (int k = 0; k < 4; ++k) {
+                                   ew[c][b][k] += wv[c] * Pp[k];
+                                   ew[c][b][4 + k] += wv[c] * Pm[k];
+                              }
+                         if (p.sst) {
+                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+                              double *o = p.sst + ((size_t) i * p.Hp + j) * 8;
+#pragma unroll
+                              for (int c = 0; c < 8; ++c) o[c] = q[c][v];
+                         }
+                         if (p.save2 && i < p.N) {
+                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+                              if (j < p.H) {
+#pragma unroll
+                                   for (int k = 0; k < 4; ++k) {
+                                        p.save2[((size_t) k * p.N + i) * p.Q + p.H + j] = Pp[k];
+                                        p.save2[((size_t) k * p.N + i) * p.Q + p.H - 1 - j] = Pm[k];
+                                   }
+                              }
+                         }
+                    }
+               }
+     }
+     // reduce over the 4 lane groups (same angle column), then over the two
+     // waves sharing the columns, then write this block's partial
+#pragma unroll
+     for (int c = 0; c < NW; ++c)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int q = 0; q < 8; ++q) {
+                    double v = ew[c][b][q];
+                    v += __shfl_xor(v, 16);
+                    v += __shfl_xor(v, 32);
+                    ew[c][b][q] = v;
+               }
+     __syncthreads();
+     if (wm == 1 && lg == 0) {
+#pragma unroll
+          for (int c = 0; c < NW; ++c)
+#pragma unroll
+               for (int b = 0; b < WN; ++b)
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) sR[((c * WN + b) * 8 + q) * 32 + wn * 16 + l16] = ew[c][b][q];
+     }
+     __syncthreads();
+     if (wm == 0 && lg == 0) {
+          // the partial of size block tile0 / tiles_per_block, in its own slot
+          double *out = p.part + (size_t) (tile0 / p.tiles_per_block) * NW * 4 * p.Q;
+#pragma unroll
+          for (int b = 0; b < WN; ++b) {
+               const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+               if (j < p.H) {
+#pragma unroll
+                    for (int c = 0; c < NW; ++c)
+#pragma unroll
+                         for (int k = 0; k < 4; ++k) {
+                              const double vp = ew[c][b][k] + sR[((c * WN + b) * 8 + k) * 32 + wn * 16 + l16];
+                              const double vm = ew[c][b][4 + k] + sR[((c * WN + b) * 8 + 4 + k) * 32 + wn * 16 + l16];
+                              out[((size_t) c * 4 + k) * p.Q + p.H + j] = vp;
+                              out[((size_t) c * 4 + k) * p.Q + p.H - 1 - j] = vm;
+                         }
+               }
+          }
+     }
+}
+
+// Operand staging of the FP64 contraction without tangent channels
+// (environment LMIE_F64_STAGING, read at every call): 1 (default) direct
+// loads into two LDS buffers (angular_f64_dma), 0 the loads through registers
+// into one buffer (angular_f64).
+int f64_staging() {
+     const char *e = getenv("LMIE_F64_STAGING");
+     return e && atoi(e) == 0 ? 0 : 1;
+}
+
 // Launch sequence of the contraction for one pass (up to 2 tangent channels
 // and up to 4 weight channels).
 template <int WM, int WN, int NT, int NW>
@@ -3107,7 +3336,12 @@ int launch_angular(const AngArgs &a, int n_tiles, hipStream_t s, int *nblk_out) 
      AngArgs b = a;
      b.tiles_per_block = tpb;
      dim3 grid(a.Hp / TN, nblk);
-     hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+     if constexpr (NT == 0) {
+          if (f64_staging()) hipLaunchKernelGGL((angular_f64_dma<WM, WN, NW>), grid, dim3(256), 0, s, b);
+          else hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+     } else {
+          hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+     }
      HIPCHECK(hipGetLastError());
      *nblk_out = nblk;
      return 0;
