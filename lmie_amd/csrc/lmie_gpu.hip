@@ -3098,38 +3098,44 @@ struct Timer {
 
 // The FP64 contraction of a pass without tangent channels, with its operand
 // stages moved by direct loads (global_load_lds_dwordx4, 16 bytes a lane
-// straight into LDS, no registers) into two LDS buffers: the next stage is
-// loaded while the matrix cores work on this one, one barrier per stage, and
-// the stream of stages runs on across the size tiles of the workgroup (the
-// next tile's first stage loads during this tile's last stage and epilogue).
-// The products, their order and the epilogue are those of angular_f64, so the
-// results are the same bits.
-template <int WM, int WN, int NW>
-__global__ __launch_bounds__(256) void angular_f64_dma(AngArgs p) {
+// straight into LDS, no registers) into ST LDS buffers: stages up to ST - 1
+// ahead load while the matrix cores work on this one, one barrier per stage.
+// One size tile per workgroup, the largest size tiles first.  On 32 x 32
+// tiles (WM = WN = 1) each wave holds 8 accumulators of 16 x 16 in ordinary
+// registers and two workgroups share a CU; with 16 accumulators (64 x 32
+// tiles) the compiler keeps them in ordinary registers across the stage loop
+// and copies them to the matrix-core registers and back at every stage,
+// which costs as much as the loads saved.  The products and their order are
+// those of angular_f64; the partials are per size tile (angular_f64 sums four
+// tiles per partial), so the sums group differently (1e-15 relative).
+template <int WM, int WN, int NW, int ST, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC))) void angular_f64_dma(AngArgs p) {
      constexpr int TM = 32 * WM, TN = 32 * WN;
      constexpr int SA = TM + 16, SB = TN + 16;       // LDS row strides (bank spread)
      constexpr int AST = 4 * KC * SA, BST = 2 * KC * SB;
+     // direct loads one wave issues per stage (its vmcnt events)
+     constexpr int NLOAD = KC + KC / 2;
      static_assert(TM * 8 <= 1024 && TN * 8 <= 1024 && KC % 4 == 0, "a row is one direct load of at most 64 lanes");
-     __shared__ double sA[2 * AST];
-     __shared__ double sB[2 * BST];
-     __shared__ double sR[NW * 8 * 32 * WN];
+     static_assert(ST == 2 || ST == 3, "stages");
+     static_assert(NLOAD < 64, "vmcnt field");
+     __shared__ double sA[ST * AST];
+     __shared__ double sB[ST * BST];
+     double *sR = sA;                    // [NW][WN][8][32], after the stages
+     static_assert(NW * 8 * 32 * WN <= ST * AST, "epilogue buffer");
 
      const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
      const int wm = wave & 1, wn = wave >> 1;
      const int l16 = lane & 15, lg = lane >> 4;
      const int jt0 = blockIdx.x * TN;
-     // size blocks from the last (largest sizes, deepest) to the first, so
-     // the longest workgroups start first and the short ones fill the tail
-     const int tile0 = (gridDim.y - 1 - blockIdx.y) * p.tiles_per_block;
+     const int tile = gridDim.y - 1 - blockIdx.y;
+     const int it0 = tile * TM;
      const size_t Np = p.Np, Hp = p.Hp, plane = (size_t) p.Kp * Np;
-     const int ntile = max(0, min(p.tiles_per_block, (int) (Np / TM) - tile0));
-     auto nch_of = [&](int t) { return p.ktile[tile0 + t] / KC; };
+     const int nch = p.ktile[tile] / KC;
 
-     // stage (tile t, orders k0 .. k0 + KC - 1) into buffer b_: A rows
-     // (plane, order) of TM sizes and B rows ((pi+tau) / (pi-tau), order) of
-     // TN angles, one direct load per row, rows dealt over the four waves
-     auto stage_load = [&](int b_, int t, int k0) {
-          const int it0 = (tile0 + t) * TM;
+     // stage of orders k0 .. k0 + KC - 1 into buffer b_: A rows (plane, order)
+     // of TM sizes and B rows ((pi+tau) / (pi-tau), order) of TN angles, one
+     // direct load per row, rows dealt over the four waves
+     auto stage_load = [&](int b_, int k0) {
 #pragma unroll
           for (int u = 0; u < KC; ++u) {
                const int row = wave + 4 * u, pl = row / KC, r = row % KC;
@@ -3148,6 +3154,70 @@ __global__ __launch_bounds__(256) void angular_f64_dma(AngArgs p) {
           }
      };
 
+     v4d acc[WM][WN][8];
+#pragma unroll
+     for (int a = 0; a < WM; ++a)
+#pragma unroll
+          for (int b = 0; b < WN; ++b)
+#pragma unroll
+               for (int q = 0; q < 8; ++q) acc[a][b][q] = v4d{0., 0., 0., 0.};
+
+     // stages 0 .. ST - 2 in flight before the first is computed
+#pragma unroll
+     for (int k = 0; k < ST - 1; ++k)
+          if (k < nch) stage_load(k, k * KC);
+     for (int kc = 0; kc < nch; ++kc) {
+          const int buf = kc % ST, k0 = kc * KC;
+          // this wave's rows of the stage have landed (the younger stage may
+          // stay in flight: the loads complete in order), then everyone's;
+          // every wave is also done with the buffer of stage kc - 1, which
+          // takes stage kc + ST - 1
+          // (vmcnt in bits 3:0 and 15:14, lgkmcnt and expcnt left open)
+          if (ST == 3 && kc + 1 < nch) __builtin_amdgcn_s_waitcnt(0x0F70 | (NLOAD & 15) | ((NLOAD >> 4) << 14));
+          else __builtin_amdgcn_s_waitcnt(0);
+          __builtin_amdgcn_s_barrier();
+          __asm__ volatile("" ::: "memory");
+          if (kc + ST - 1 < nch) stage_load((kc + ST - 1) % ST, (kc + ST - 1) * KC);
+          const double *a0 = sA + buf * AST, *b0 = sB + buf * BST;
+#pragma unroll
+          for (int ks = 0; ks < KC / 4; ++ks) {
+               const int r = ks * 4 + lg;                   // this lane's k row
+               const double sg = ((k0 + r) & 1) ? -1. : 1.;   // s_n, n = k0 + r + 1
+               // the -mu products take s_n on the angle operand: the same
+               // products as s_n on the size operand (a sign), fewer multiplies
+               double bp[WN], bm[WN], sbp[WN], sbm[WN];
+#pragma unroll
+               for (int b = 0; b < WN; ++b) {
+                    const int jc = wn * 16 * WN + 16 * b + l16;
+                    bp[b] = b0[r * SB + jc];
+                    bm[b] = b0[(KC + r) * SB + jc];
+                    sbp[b] = sg * bp[b];
+                    sbm[b] = sg * bm[b];
+               }
+#pragma unroll
+               for (int a = 0; a < WM; ++a) {
+                    const int ic = wm * 16 * WM + 16 * a + l16;
+                    const double Sr = a0[(0 * KC + r) * SA + ic];
+                    const double Si = a0[(1 * KC + r) * SA + ic];
+                    const double Tr = a0[(2 * KC + r) * SA + ic];
+                    const double Ti = a0[(3 * KC + r) * SA + ic];
+#pragma unroll
+                    for (int b = 0; b < WN; ++b) {
+                         v4d *q = acc[a][b];
+                         q[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sr, bp[b], q[0], 0, 0, 0);
+                         q[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Si, bp[b], q[1], 0, 0, 0);
+                         q[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tr, bm[b], q[2], 0, 0, 0);
+                         q[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti, bm[b], q[3], 0, 0, 0);
+                         q[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sr, sbm[b], q[4], 0, 0, 0);
+                         q[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(Si, sbm[b], q[5], 0, 0, 0);
+                         q[6] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tr, sbp[b], q[6], 0, 0, 0);
+                         q[7] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti, sbp[b], q[7], 0, 0, 0);
+                    }
+               }
+          }
+     }
+
+     // epilogue: Stokes elements, weights, reduction over this lane's sizes
      double ew[NW][WN][8];
 #pragma unroll
      for (int c = 0; c < NW; ++c)
@@ -3155,123 +3225,45 @@ __global__ __launch_bounds__(256) void angular_f64_dma(AngArgs p) {
           for (int b = 0; b < WN; ++b)
 #pragma unroll
                for (int q = 0; q < 8; ++q) ew[c][b][q] = 0.;
-
-     // issue cursor over the stream of stages (tile it_, chunk ik_)
-     int it_ = 0, ik_ = 0;
-     while (it_ < ntile && nch_of(it_) == 0) ++it_;
-     auto advance = [&]() {
-          if (++ik_ >= nch_of(it_)) {
-               ik_ = 0;
-               ++it_;
-               while (it_ < ntile && nch_of(it_) == 0) ++it_;
-          }
-     };
-     if (it_ < ntile) {
-          stage_load(0, it_, 0);
-          advance();
-     }
-     int gs = 0;                        // stages computed so far: buffer gs & 1
-     for (int t = 0; t < ntile; ++t) {
-          const int it0 = (tile0 + t) * TM, nch = nch_of(t);
-          v4d acc[WM][WN][8];
 #pragma unroll
-          for (int a = 0; a < WM; ++a)
+     for (int a = 0; a < WM; ++a)
 #pragma unroll
-               for (int b = 0; b < WN; ++b)
+          for (int v = 0; v < 4; ++v) {
+               const int i = it0 + wm * 16 * WM + 16 * a + lg + 4 * v;
+               double wv[NW];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q) acc[a][b][q] = v4d{0., 0., 0., 0.};
-          for (int kc = 0; kc < nch; ++kc, ++gs) {
-               const int buf = gs & 1, k0 = kc * KC;
-               // this wave's rows of the stage have landed (it is the only one
-               // in flight), then everyone's; every wave is also done with the
-               // other buffer, which takes the next stage
-               __builtin_amdgcn_s_waitcnt(0);
-               __builtin_amdgcn_s_barrier();
-               __asm__ volatile("" ::: "memory");
-               if (it_ < ntile) {
-                    stage_load(buf ^ 1, it_, ik_ * KC);
-                    advance();
-               }
-               const double *a0 = sA + buf * AST, *b0 = sB + buf * BST;
+               for (int c = 0; c < NW; ++c) wv[c] = p.W[c][i];
 #pragma unroll
-               for (int ks = 0; ks < KC / 4; ++ks) {
-                    const int r = ks * 4 + lg;                   // this lane's k row
-                    const double sg = ((k0 + r) & 1) ? -1. : 1.;   // s_n, n = k0 + r + 1
-                    // the -mu products take s_n on the angle operand: the same
-                    // products as s_n on the size operand (a sign), fewer multiplies
-                    double bp[WN], bm[WN], sbp[WN], sbm[WN];
+               for (int b = 0; b < WN; ++b) {
+                    const v4d *q = acc[a][b];
+                    double Pp[4], Pm[4];
+                    stokes(q[0][v], q[1][v], q[2][v], q[3][v], Pp);
+                    stokes(q[4][v], q[5][v], q[6][v], q[7][v], Pm);
 #pragma unroll
-                    for (int b = 0; b < WN; ++b) {
-                         const int jc = wn * 16 * WN + 16 * b + l16;
-                         bp[b] = b0[r * SB + jc];
-                         bm[b] = b0[(KC + r) * SB + jc];
-                         sbp[b] = sg * bp[b];
-                         sbm[b] = sg * bm[b];
-                    }
+                    for (int c = 0; c < NW; ++c)
 #pragma unroll
-                    for (int a = 0; a < WM; ++a) {
-                         const int ic = wm * 16 * WM + 16 * a + l16;
-                         const double Sr = a0[(0 * KC + r) * SA + ic];
-                         const double Si = a0[(1 * KC + r) * SA + ic];
-                         const double Tr = a0[(2 * KC + r) * SA + ic];
-                         const double Ti = a0[(3 * KC + r) * SA + ic];
-#pragma unroll
-                         for (int b = 0; b < WN; ++b) {
-                              v4d *q = acc[a][b];
-                              q[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sr, bp[b], q[0], 0, 0, 0);
-                              q[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(Si, bp[b], q[1], 0, 0, 0);
-                              q[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tr, bm[b], q[2], 0, 0, 0);
-                              q[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti, bm[b], q[3], 0, 0, 0);
-                              q[4] = __builtin_amdgcn_mfma_f64_16x16x4f64(Sr, sbm[b], q[4], 0, 0, 0);
-                              q[5] = __builtin_amdgcn_mfma_f64_16x16x4f64(Si, sbm[b], q[5], 0, 0, 0);
-                              q[6] = __builtin_amdgcn_mfma_f64_16x16x4f64(Tr, sbp[b], q[6], 0, 0, 0);
-                              q[7] = __builtin_amdgcn_mfma_f64_16x16x4f64(Ti, sbp[b], q[7], 0, 0, 0);
+                         for (int k = 0; k < 4; ++k) {
+                              ew[c][b][k] += wv[c] * Pp[k];
+                              ew[c][b][4 + k] += wv[c] * Pm[k];
                          }
+                    if (p.sst) {
+                         const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+                         double *o = p.sst + ((size_t) i * p.Hp + j) * 8;
+#pragma unroll
+                         for (int c = 0; c < 8; ++c) o[c] = q[c][v];
                     }
-               }
-          }
-          // epilogue of the tile: Stokes elements, weights, reduction over
-          // this lane's sizes (the next stage is loading meanwhile)
-#pragma unroll
-          for (int a = 0; a < WM; ++a)
-#pragma unroll
-               for (int v = 0; v < 4; ++v) {
-                    const int i = it0 + wm * 16 * WM + 16 * a + lg + 4 * v;
-                    double wv[NW];
-#pragma unroll
-                    for (int c = 0; c < NW; ++c) wv[c] = p.W[c][i];
-#pragma unroll
-                    for (int b = 0; b < WN; ++b) {
-                         const v4d *q = acc[a][b];
-                         double Pp[4], Pm[4];
-                         stokes(q[0][v], q[1][v], q[2][v], q[3][v], Pp);
-                         stokes(q[4][v], q[5][v], q[6][v], q[7][v], Pm);
-#pragma unroll
-                         for (int c = 0; c < NW; ++c)
+                    if (p.save2 && i < p.N) {
+                         const int j = jt0 + wn * 16 * WN + 16 * b + l16;
+                         if (j < p.H) {
 #pragma unroll
                               for (int k = 0; k < 4; ++k) {
-                                   ew[c][b][k] += wv[c] * Pp[k];
-                                   ew[c][b][4 + k] += wv[c] * Pm[k];
-                              }
-                         if (p.sst) {
-                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
-                              double *o = p.sst + ((size_t) i * p.Hp + j) * 8;
-#pragma unroll
-                              for (int c = 0; c < 8; ++c) o[c] = q[c][v];
-                         }
-                         if (p.save2 && i < p.N) {
-                              const int j = jt0 + wn * 16 * WN + 16 * b + l16;
-                              if (j < p.H) {
-#pragma unroll
-                                   for (int k = 0; k < 4; ++k) {
-                                        p.save2[((size_t) k * p.N + i) * p.Q + p.H + j] = Pp[k];
-                                        p.save2[((size_t) k * p.N + i) * p.Q + p.H - 1 - j] = Pm[k];
-                                   }
+                                   p.save2[((size_t) k * p.N + i) * p.Q + p.H + j] = Pp[k];
+                                   p.save2[((size_t) k * p.N + i) * p.Q + p.H - 1 - j] = Pm[k];
                               }
                          }
                     }
                }
-     }
+          }
      // reduce over the 4 lane groups (same angle column), then over the two
      // waves sharing the columns, then write this block's partial
 #pragma unroll
@@ -3285,7 +3277,7 @@ __global__ __launch_bounds__(256) void angular_f64_dma(AngArgs p) {
                     v += __shfl_xor(v, 32);
                     ew[c][b][q] = v;
                }
-     __syncthreads();
+     __syncthreads();                    // every wave is done with the stages (sR aliases them)
      if (wm == 1 && lg == 0) {
 #pragma unroll
           for (int c = 0; c < NW; ++c)
@@ -3296,8 +3288,7 @@ __global__ __launch_bounds__(256) void angular_f64_dma(AngArgs p) {
      }
      __syncthreads();
      if (wm == 0 && lg == 0) {
-          // the partial of size block tile0 / tiles_per_block, in its own slot
-          double *out = p.part + (size_t) (tile0 / p.tiles_per_block) * NW * 4 * p.Q;
+          double *out = p.part + (size_t) tile * NW * 4 * p.Q;
 #pragma unroll
           for (int b = 0; b < WN; ++b) {
                const int j = jt0 + wn * 16 * WN + 16 * b + l16;
@@ -3318,27 +3309,40 @@ __global__ __launch_bounds__(256) void angular_f64_dma(AngArgs p) {
 
 // Operand staging of the FP64 contraction without tangent channels
 // (environment LMIE_F64_STAGING, read at every call): 1 (default) direct
-// loads into two LDS buffers (angular_f64_dma), 0 the loads through registers
-// into one buffer (angular_f64).
+// loads into two LDS buffers on 32 x 32 tiles (angular_f64_dma, 16 x 16 per
+// wave, two workgroups per CU); 0 the loads through registers into one
+// buffer on 64 x 32 tiles (angular_f64).  Measured on the two-mode workload
+// (32768 sizes, 3810 angles): 20.9 against 43.7 ms; the direct loads on
+// 64 x 32 tiles took 26.1 ms with three buffers and 27.0 with two.
 int f64_staging() {
      const char *e = getenv("LMIE_F64_STAGING");
      return e && atoi(e) == 0 ? 0 : 1;
 }
 
+// Size tile of the FP64 contraction of a pass: 32 with tangent channels or
+// the direct loads, else 64
+int f64_tile(int n_tan, int staging) { return n_tan > 0 || staging != 0 ? 32 : 64; }
+
+// Size tiles per workgroup of the FP64 contraction: one for the direct-load
+// kernel, four for angular_f64 (each workgroup's partial takes one slot)
+int f64_tiles_per_block(int n_tan, int staging) { return n_tan == 0 && staging != 0 ? 1 : 4; }
+
 // Launch sequence of the contraction for one pass (up to 2 tangent channels
 // and up to 4 weight channels).
 template <int WM, int WN, int NT, int NW>
-int launch_angular(const AngArgs &a, int n_tiles, hipStream_t s, int *nblk_out) {
+int launch_angular(const AngArgs &a, int n_tiles, int staging, hipStream_t s, int *nblk_out) {
      constexpr int TM = 32 * WM, TN = 32 * WN;
      (void) TM;
-     const int tpb = 4;
+     const int tpb = f64_tiles_per_block(NT, staging);
      const int nblk = (n_tiles + tpb - 1) / tpb;
      AngArgs b = a;
      b.tiles_per_block = tpb;
      dim3 grid(a.Hp / TN, nblk);
      if constexpr (NT == 0) {
-          if (f64_staging()) hipLaunchKernelGGL((angular_f64_dma<WM, WN, NW>), grid, dim3(256), 0, s, b);
-          else hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+          if (staging != 0 && WM == 1 && WN == 1)
+               hipLaunchKernelGGL((angular_f64_dma<WM, WN, NW, 2, 2>), grid, dim3(256), 0, s, b);
+          else if (staging == 0) hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+          else return set_err("internal: FP64 staging %d with %d x %d tiles", staging, TM, TN);
      } else {
           hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
      }
@@ -3348,15 +3352,15 @@ int launch_angular(const AngArgs &a, int n_tiles, hipStream_t s, int *nblk_out) 
 }
 
 template <int WM, int WN, int NT>
-int launch_angular_nw(int nw, const AngArgs &a, int n_tiles, hipStream_t s, int *nblk) {
+int launch_angular_nw(int nw, const AngArgs &a, int n_tiles, int staging, hipStream_t s, int *nblk) {
      switch (nw) {
      case 0:
           if (NT == 0) return set_err("internal: contraction pass without channels");
-          return launch_angular<WM, WN, NT, (NT > 0 ? 0 : 1)>(a, n_tiles, s, nblk);
-     case 1: return launch_angular<WM, WN, NT, 1>(a, n_tiles, s, nblk);
-     case 2: return launch_angular<WM, WN, NT, 2>(a, n_tiles, s, nblk);
-     case 3: return launch_angular<WM, WN, NT, 3>(a, n_tiles, s, nblk);
-     default: return launch_angular<WM, WN, NT, 4>(a, n_tiles, s, nblk);
+          return launch_angular<WM, WN, NT, (NT > 0 ? 0 : 1)>(a, n_tiles, staging, s, nblk);
+     case 1: return launch_angular<WM, WN, NT, 1>(a, n_tiles, staging, s, nblk);
+     case 2: return launch_angular<WM, WN, NT, 2>(a, n_tiles, staging, s, nblk);
+     case 3: return launch_angular<WM, WN, NT, 3>(a, n_tiles, staging, s, nblk);
+     default: return launch_angular<WM, WN, NT, 4>(a, n_tiles, staging, s, nblk);
      }
 }
 
@@ -3737,6 +3741,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           struct Pass { int ntp, nwp, t0, w0, nblk; size_t off; };
           std::vector<Pass> plan;
           size_t tot = 0;
+          const int f64st = f64_staging();
           for (int td = 0, wd = 0, ps = 0; ps == 0 || td < nt || wd < nw; ++ps) {
                Pass P;
                P.ntp = nt - td < 2 ? nt - td : 2;
@@ -3745,7 +3750,8 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                P.nwp = base + ww;
                P.t0 = td;
                P.w0 = wd;
-               P.nblk = ((Np / (P.ntp ? 32 : 64)) + 3) / 4;
+               const int tpb = f64_tiles_per_block(P.ntp, f64st);
+               P.nblk = ((Np / f64_tile(P.ntp, f64st)) + tpb - 1) / tpb;
                P.off = tot;
                tot += (size_t) P.nblk * (P.nwp + P.ntp) * 4 * Q;
                plan.push_back(P);
@@ -3815,13 +3821,16 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                a.save2 = (ps == 0 && J->save_control == 1) ? w->save2.as<double>() : nullptr;
                a.sst = (ps == 0 && keep) ? w->sst.as<double>() : nullptr;
                int nblk = 0, rc;
-               if (P.ntp == 0) {
+               if (P.ntp == 0 && f64_tile(0, f64st) == 32) {
+                    a.ktile = w->ktile32.as<int>();
+                    rc = launch_angular_nw<1, 1, 0>(P.nwp, a, Np / 32, f64st, s, &nblk);
+               } else if (P.ntp == 0) {
                     a.ktile = w->ktile64.as<int>();
-                    rc = launch_angular_nw<2, 1, 0>(P.nwp, a, Np / 64, s, &nblk);
+                    rc = launch_angular_nw<2, 1, 0>(P.nwp, a, Np / 64, f64st, s, &nblk);
                } else {
                     a.ktile = w->ktile32.as<int>();
-                    rc = P.ntp == 1 ? launch_angular_nw<1, 1, 1>(P.nwp, a, Np / 32, s, &nblk)
-                                    : launch_angular_nw<1, 1, 2>(P.nwp, a, Np / 32, s, &nblk);
+                    rc = P.ntp == 1 ? launch_angular_nw<1, 1, 1>(P.nwp, a, Np / 32, f64st, s, &nblk)
+                                    : launch_angular_nw<1, 1, 2>(P.nwp, a, Np / 32, f64st, s, &nblk);
                }
                if (rc) return -1;
                if (nblk != P.nblk) return set_err("internal: partial block count");
