@@ -2973,6 +2973,7 @@ struct Workspace {
      hipStream_t side = nullptr;     // angle tables, concurrent with the per-size recurrences
      hipStream_t side2 = nullptr;    // expansion tables, from the Gauss rule on
      hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr, sums = nullptr, nodes = nullptr, join3 = nullptr;
+     hipEvent_t uploaded = nullptr;  // a batch case's inputs are on the device (its own stream)
      std::vector<double> hguess;     // host staging of the Gauss root guesses
      std::vector<int2> htask;        // host staging of the contraction task list
      hipEvent_t ev[2 * LMIE_N_TIMERS + 2];
@@ -3025,6 +3026,7 @@ int ws_create(int device, Workspace **out) {
      HIPCHECK(hipEventCreateWithFlags(&w->sums, hipEventDisableTiming));
      HIPCHECK(hipEventCreateWithFlags(&w->nodes, hipEventDisableTiming));
      HIPCHECK(hipEventCreateWithFlags(&w->join3, hipEventDisableTiming));
+     HIPCHECK(hipEventCreateWithFlags(&w->uploaded, hipEventDisableTiming));
      for (auto &e : w->ev) HIPCHECK(hipEventCreate(&e));
      *out = w;
      return 0;
@@ -3339,10 +3341,13 @@ int launch_angular(const AngArgs &a, int n_tiles, int staging, hipStream_t s, in
      b.tiles_per_block = tpb;
      dim3 grid(a.Hp / TN, nblk);
      if constexpr (NT == 0) {
-          if (staging != 0 && WM == 1 && WN == 1)
-               hipLaunchKernelGGL((angular_f64_dma<WM, WN, NW, 2, 2>), grid, dim3(256), 0, s, b);
-          else if (staging == 0) hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
-          else return set_err("internal: FP64 staging %d with %d x %d tiles", staging, TM, TN);
+          if constexpr (WM == 1 && WN == 1) {
+               if (staging != 0) hipLaunchKernelGGL((angular_f64_dma<WM, WN, NW, 2, 2>), grid, dim3(256), 0, s, b);
+               else hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+          } else {
+               if (staging == 0) hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
+               else return set_err("internal: FP64 staging %d with %d x %d tiles", staging, TM, TN);
+          }
      } else {
           hipLaunchKernelGGL((angular_f64<WM, WN, NT, NW>), grid, dim3(256), 0, s, b);
      }
@@ -4621,6 +4626,20 @@ struct Batch {
      std::vector<int> rc[2];
      std::vector<std::string> err[2];
      int fused_cases[2] = {0, 0};
+     // forward cases prepared by their own threads before they arrive
+     // (buffers, host tables and staged uploads), so the thread that runs
+     // the pass only launches
+     std::vector<char> prepped;
+     std::vector<FwdCase> pre;
+     std::vector<Upload> preup;
+     std::vector<int> prerc;
+     std::vector<std::string> preerr;
+     // the same for the reverse pass
+     std::vector<char> rprepped;
+     std::vector<RevCase> rpre;
+     std::vector<Upload> rpreup;
+     std::vector<int> rprerc;
+     std::vector<std::string> rpreerr;
 };
 
 thread_local Batch *t_batch = nullptr;
@@ -4628,6 +4647,54 @@ thread_local int t_slot = -1, t_phase = 0;
 
 int forward_one(const lmie_gpu_job *job, lmie_gpu_result *res);
 int adjoint_one(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out);
+
+// A forward case of a fused batch prepared on its own thread, before it
+// arrives: its workspace, host tables and staged uploads (fused_prepare).
+void batch_prepare(Batch *b, int k, const lmie_gpu_job *J, lmie_gpu_result *res) {
+     g_err[0] = 0;
+     FwdCase &c = b->pre[k];
+     c.slot = k; c.J = J; c.R = res; c.w = nullptr;
+     int rc = use_device(J->device);
+     if (rc == 0) rc = ws_acquire_home(J->device, k, &c.w);
+     if (rc == 0) {
+          // the case's inputs go up from here, on its own stream, one pinned
+          // copy; the launch waits for the event
+          Upload &up = b->preup[k];
+          up.items.clear();
+          up.h.clear();
+          rc = fused_prepare(c, up);
+          if (rc == 0) rc = up.flush(c.w->finDev, c.w->finPin, c.w->stream);
+          if (rc == 0 && hipEventRecord(c.w->uploaded, c.w->stream) != hipSuccess)
+               rc = set_err("hipEventRecord failed");
+          if (rc) {
+               (void) hipStreamSynchronize(c.w->stream);
+               ws_release(c.w);
+               c.w = nullptr;
+          }
+     }
+     b->prerc[k] = rc;
+     b->preerr[k] = rc ? g_err : "";
+     b->prepped[k] = 1;
+}
+
+// A reverse case of a fused batch prepared on its own thread, before it
+// arrives (fused_rev_prepare), its uploads sent on its own stream.
+void batch_rev_prepare(Batch *b, int k, Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
+     g_err[0] = 0;
+     RevCase &c = b->rpre[k];
+     c.slot = k; c.w = w; c.in = in; c.out = out;
+     Upload &up = b->rpreup[k];
+     up.items.clear();
+     up.h.clear();
+     int rc = use_device(w->device);
+     if (rc == 0) rc = fused_rev_prepare(c, up);
+     if (rc == 0) rc = up.flush(w->finDev, w->finPin, w->stream);
+     if (rc == 0 && hipEventRecord(w->uploaded, w->stream) != hipSuccess) rc = set_err("hipEventRecord failed");
+     if (rc) (void) hipStreamSynchronize(w->stream);
+     b->rprerc[k] = rc;
+     b->rpreerr[k] = rc ? g_err : "";
+     b->rprepped[k] = 1;
+}
 
 void batch_run_forward(Batch *b) {
      std::vector<FwdCase> f16, b16;
@@ -4638,7 +4705,31 @@ void batch_run_forward(Batch *b) {
           const lmie_gpu_job *J = b->job[k];
           g_err[0] = 0;
           int rc;
-          if (!fused_forward_ok(J) || J->device != b->device) {
+          if (b->prepped[k] && (b->prerc[k] || J->device != b->device)) {
+               // prepared on another device, or the preparation failed
+               if (b->prerc[k] == 0) {
+                    ws_release(b->pre[k].w);
+                    rc = forward_one(J, b->res[k]);
+               } else {
+                    b->rc[0][k] = b->prerc[k];
+                    b->err[0][k] = b->preerr[k];
+                    continue;
+               }
+          } else if (b->prepped[k]) {
+               // its inputs are on the way on its own stream (batch_prepare)
+               FwdCase &c = b->pre[k];
+               if (!L) L = c.w;
+               if (hipStreamWaitEvent(L->stream, c.w->uploaded, 0) != hipSuccess) {
+                    (void) hipStreamSynchronize(c.w->stream);
+                    if (c.w != L) ws_release(c.w);
+                    b->rc[0][k] = set_err("hipStreamWaitEvent failed");
+                    b->err[0][k] = g_err;
+                    continue;
+               }
+               const bool bf = J->precision == LMIE_PRECISION_MIXED_BF16;
+               (bf ? b16 : f16).push_back(std::move(c));
+               continue;
+          } else if (!fused_forward_ok(J) || J->device != b->device) {
                rc = forward_one(J, b->res[k]);
           } else {
                FwdCase c;
@@ -4691,7 +4782,23 @@ void batch_run_reverse(Batch *b) {
           Workspace *w = b->ws[k];
           g_err[0] = 0;
           int rc;
-          if (!fused_reverse_ok(w, b->ain[k]) || w->device != b->device) {
+          if (b->rprepped[k] && b->rprerc[k]) {
+               b->rc[1][k] = b->rprerc[k];
+               b->err[1][k] = b->rpreerr[k];
+               continue;
+          } else if (b->rprepped[k] && w->device == b->device) {
+               // its uploads are on the way on its own stream (batch_rev_prepare)
+               RevCase &c = b->rpre[k];
+               if (!L) L = w;
+               if (hipStreamWaitEvent(L->stream, w->uploaded, 0) != hipSuccess) {
+                    (void) hipStreamSynchronize(w->stream);
+                    b->rc[1][k] = set_err("hipStreamWaitEvent failed");
+                    b->err[1][k] = g_err;
+                    continue;
+               }
+               (w->bf16 ? b16 : f16).push_back(std::move(c));
+               continue;
+          } else if (!fused_reverse_ok(w, b->ain[k]) || w->device != b->device) {
                rc = adjoint_one(w, b->ain[k], b->aout[k]);
           } else {
                RevCase c;
@@ -4743,6 +4850,9 @@ int batch_arrive(int p, const lmie_gpu_job *job, lmie_gpu_result *res, Workspace
                  lmie_gpu_adj_out *aout) {
      Batch *b = t_batch;
      const int k = t_slot;
+     // a fused case does its own preparation first, beside the others
+     if (p == 0 && fused_forward_ok(job)) batch_prepare(b, k, job, res);
+     if (p == 1 && fused_reverse_ok(w, ain)) batch_rev_prepare(b, k, w, ain, aout);
      pthread_mutex_lock(&b->mu);
      if (p == 0) {
           b->job[k] = job;
@@ -4890,6 +5000,16 @@ void *lmie_gpu_batch_open(int n_case) {
      b->ws.assign(n_case, nullptr);
      b->ain.assign(n_case, nullptr);
      b->aout.assign(n_case, nullptr);
+     b->prepped.assign(n_case, 0);
+     b->pre.resize(n_case);
+     b->preup.resize(n_case);
+     b->prerc.assign(n_case, 0);
+     b->preerr.assign(n_case, std::string());
+     b->rprepped.assign(n_case, 0);
+     b->rpre.resize(n_case);
+     b->rpreup.resize(n_case);
+     b->rprerc.assign(n_case, 0);
+     b->rpreerr.assign(n_case, std::string());
      return b;
 }
 

@@ -1580,6 +1580,69 @@ static __thread int g_batch_fused[2];
 
 int lmie_last_batch_fused(int pass) { return pass >= 0 && pass < 2 ? g_batch_fused[pass] : 0; }
 
+/* Persistent threads for the cases of fused batches: a batch call hands each
+   case to one of them instead of creating a thread per case and call (thread
+   creation costs about as much as a small case's host work).  One batch call
+   uses them at a time (region); a call that finds them busy, or cases beyond
+   the threads that could be started, get threads of their own as before. */
+static struct {
+     pthread_mutex_t region;
+     pthread_mutex_t mu;
+     pthread_cond_t go, done;
+     int n;                                  /* threads started */
+     unsigned gen;                           /* generation of the current work */
+     int m, pending;                         /* members of the generation, not yet done */
+     batch_member *mem[LMIE_BATCH_GROUP];
+} g_bpool = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER,
+             PTHREAD_COND_INITIALIZER, 0, 0, 0, 0, {0}};
+
+static void *bpool_worker(void *vp) {
+     const int k = (int) (intptr_t) vp;
+     unsigned seen = 0;
+     pthread_mutex_lock(&g_bpool.mu);
+     for (;;) {
+          while (g_bpool.gen == seen) pthread_cond_wait(&g_bpool.go, &g_bpool.mu);
+          seen = g_bpool.gen;
+          if (k < g_bpool.m) {
+               batch_member *mb = g_bpool.mem[k];
+               pthread_mutex_unlock(&g_bpool.mu);
+               batch_member_run(mb);
+               pthread_mutex_lock(&g_bpool.mu);
+               if (--g_bpool.pending == 0) pthread_cond_signal(&g_bpool.done);
+          }
+     }
+     return NULL;
+}
+
+/* Runs members 0 .. m-1 on the persistent threads; returns how many it took
+   (0 when the threads are in use by another call), the caller starts the rest. */
+static int bpool_start(batch_member *mem, int m) {
+     if (pthread_mutex_trylock(&g_bpool.region) != 0) return 0;
+     pthread_mutex_lock(&g_bpool.mu);
+     while (g_bpool.n < m) {
+          pthread_t th;
+          if (pthread_create(&th, NULL, bpool_worker, (void *) (intptr_t) g_bpool.n) != 0) break;
+          pthread_detach(th);
+          g_bpool.n++;
+     }
+     const int take = m < g_bpool.n ? m : g_bpool.n;
+     for (int k = 0; k < take; ++k) g_bpool.mem[k] = &mem[k];
+     g_bpool.m = take;
+     g_bpool.pending = take;
+     g_bpool.gen++;
+     pthread_cond_broadcast(&g_bpool.go);
+     pthread_mutex_unlock(&g_bpool.mu);
+     if (take == 0) pthread_mutex_unlock(&g_bpool.region);
+     return take;
+}
+
+static void bpool_wait(void) {
+     pthread_mutex_lock(&g_bpool.mu);
+     while (g_bpool.pending > 0) pthread_cond_wait(&g_bpool.done, &g_bpool.mu);
+     pthread_mutex_unlock(&g_bpool.mu);
+     pthread_mutex_unlock(&g_bpool.region);
+}
+
 static void batch_fused(batch_job *b) {
      g_batch_fused[0] = g_batch_fused[1] = 0;
      for (int g0 = 0; g0 < b->n_case; g0 += LMIE_BATCH_GROUP) {
@@ -1595,8 +1658,10 @@ static void batch_fused(batch_job *b) {
           int started[LMIE_BATCH_GROUP];
           for (int k = 0; k < m; ++k) {
                mem[k].b = b; mem[k].gpu_batch = gb; mem[k].first = g0; mem[k].slot = k;
-               started[k] = pthread_create(&th[k], NULL, batch_member_run, &mem[k]) == 0;
           }
+          const int pooled = bpool_start(mem, m);
+          for (int k = 0; k < m; ++k)
+               started[k] = k < pooled ? 1 : pthread_create(&th[k], NULL, batch_member_run, &mem[k]) == 0;
           /* a case whose thread did not start leaves the device batch from here */
           for (int k = 0; k < m; ++k)
                if (!started[k]) {
@@ -1604,8 +1669,9 @@ static void batch_fused(batch_job *b) {
                     fail("lmie_solution_adjoint_batch", "could not start a thread");
                     batch_record(b, g0 + k, -1);
                }
-          for (int k = 0; k < m; ++k)
+          for (int k = pooled; k < m; ++k)
                if (started[k]) pthread_join(th[k], NULL);
+          if (pooled) bpool_wait();
           g_batch_fused[0] += lmie_gpu_batch_fused(gb, 0);
           g_batch_fused[1] += lmie_gpu_batch_fused(gb, 1);
           lmie_gpu_batch_close(gb);
