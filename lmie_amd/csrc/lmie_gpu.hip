@@ -2405,6 +2405,27 @@ struct RevArgs {
 };
 constexpr int REV_ROWS = 12;
 
+// a_j, b_j for the reverse sweep, with the reciprocals the adjoint reuses:
+// ra = 1/(u zeta_j - zeta_{j-1}), rb = 1/(v zeta_j - zeta_{j-1}) of the
+// denominators of a and b (Smith's reciprocal), u = D/m + j/x, v = m D + j/x;
+// a = (u psi_j - psi_{j-1}) ra.  Not the forward's bits (a product with the
+// reciprocal instead of a division): the reverse pass needs the values, and
+// d a / d u = (psi_j - a zeta_j) ra reuses ra.
+struct AbRev {
+     cd a, b, ra, rb;
+};
+__device__ __forceinline__ AbRev mie_ab_rev(double jx, cd m, cd im, cd Dj, double psi, double psi0, double chi1,
+                                            double chi0) {
+     AbRev o;
+     const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
+     const cd u = cadd(cmul(Dj, im), cmk(jx, 0.)), v = cadd(cmul(m, Dj), cmk(jx, 0.));
+     o.ra = cdiv_nr(cmk(1., 0.), csub(cmul(u, zj), zjm));
+     o.rb = cdiv_nr(cmk(1., 0.), csub(cmul(v, zj), zjm));
+     o.a = cmul(csub(cscale(u, psi), cmk(psi0, 0.)), o.ra);
+     o.b = cmul(csub(cscale(v, psi), cmk(psi0, 0.)), o.rb);
+     return o;
+}
+
 // Per chunk, with the carry entering it as c, every carry and D cotangent
 // inside is affine in c: carry out = P c + C0, z cotangent = Q c + Z0.
 __device__ __forceinline__ void rev_orders_body(RevArgs p) {
@@ -2434,21 +2455,25 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           cd D0 = cmk(rd(Dre, lo, lo < n2), rd(Dim, lo, lo < n2));
           cd D1 = cmk(rd(Dre, lo + 1, lo + 1 < n2), rd(Dim, lo + 1, lo + 1 < n2));
           double u0 = rdu(0, lo, lo <= n1), u1 = rdu(1, lo, lo <= n1), u2 = rdu(2, lo, lo <= n1), u3 = rdu(3, lo, lo <= n1);
-          cd am = cmk(0., 0.), bm = cmk(0., 0.), a = cmk(0., 0.), b = cmk(0., 0.), an = cmk(0., 0.), bn = cmk(0., 0.);
+          cd am = cmk(0., 0.), bm = cmk(0., 0.);
+          AbRev cur, nxt;
+          cur.a = cur.b = cur.ra = cur.rb = cmk(0., 0.);
+          nxt = cur;
           if (lo <= n1) {
-               if (lo > 1)
-                    mie_ab_at(lo - 1, x, m, im1, cmk(Dre[(size_t) (lo - 1) * Np + i], Dim[(size_t) (lo - 1) * Np + i]),
-                              psm, Psi[(size_t) (lo - 2) * Np + i], chm, Chi[(size_t) (lo - 2) * Np + i], &am, &bm);
-               mie_ab_at(lo, x, m, im1, D0, ps0, psm, ch0, chm, &a, &b);
+               if (lo > 1) {
+                    const AbRev pr = mie_ab_rev((lo - 1) / x, m, im1,
+                                                cmk(Dre[(size_t) (lo - 1) * Np + i], Dim[(size_t) (lo - 1) * Np + i]),
+                                                psm, Psi[(size_t) (lo - 2) * Np + i], chm, Chi[(size_t) (lo - 2) * Np + i]);
+                    am = pr.a;
+                    bm = pr.b;
+               }
+               cur = mie_ab_rev(lo / x, m, im1, D0, ps0, psm, ch0, chm);
           }
           const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
           const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
           double sgn = (lo & 1) ? -1. : 1.;
           const double ix2 = 1. / (x * x);
-          double jx_next = lo / x;        // j / x of the loop's order, formed one order ahead
           for (int j = lo; j < hi; ++j) {
-               const double jx = jx_next;
-               jx_next = (j + 1) / x;
                // the window's next entries
                const double ps2 = rd(Psi, j + 2, j + 2 <= n1), ch2 = rd(Chi, j + 2, j + 2 <= n1);
                const cd D2 = cmk(rd(Dre, j + 2, j + 2 < n2), rd(Dim, j + 2, j + 2 < n2));
@@ -2456,9 +2481,10 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                const double v2 = rdu(2, j + 1, j + 1 <= n1), v3 = rdu(3, j + 1, j + 1 <= n1);
                cd lc = cmk(0., 0.);
                if (j <= n1) {
-                    if (j < n1) mie_ab_jx(jx_next, m, im1, D1, ps1, ps0, ch1, ch0, &an, &bn);
+                    if (j < n1) nxt = mie_ab_rev((j + 1) / x, m, im1, D1, ps1, ps0, ch1, ch0);
+                    const cd a = cur.a, b = cur.b, an = nxt.a, bn = nxt.b;
                     const double psi = ps0;
-                    const cd zj = cmk(ps0, ch0), zjm = cmk(psm, chm);
+                    const cd zj = cmk(ps0, ch0);
                     const cd Dj = D0;
                     const double fv1 = 2. * j + 1., fv4 = p.ft[FT_FV4 * ld + j];
                     cd Sb, Tb;
@@ -2492,10 +2518,8 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                          bb = cadd(bb, cscale(bm, sb3 * fv3m));
                     }
                     // a = (u psi - psi0) / (u zeta - zeta0), da/du = (psi - a zeta) / den
-                    const cd u = cadd(cmul(Dj, im1), cmk(jx, 0.));
-                    const cd v = cadd(cmul(m, Dj), cmk(jx, 0.));
-                    const cd dua = cdiv_nr(csub(cmk(psi, 0.), cmul(a, zj)), csub(cmul(u, zj), zjm));
-                    const cd dvb = cdiv_nr(csub(cmk(psi, 0.), cmul(b, zj)), csub(cmul(v, zj), zjm));
+                    const cd dua = cmul(csub(cmk(psi, 0.), cmul(a, zj)), cur.ra);
+                    const cd dvb = cmul(csub(cmk(psi, 0.), cmul(b, zj)), cur.rb);
                     const cd ub = cmul(cconj(dua), ab), vb = cmul(cconj(dvb), bb);
                     if (p.xbar) {
                          // d a / d x at fixed D_j (psi_j, psi_{j-1}, chi_j, chi_{j-1}
@@ -2516,8 +2540,7 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                     sgn = -sgn;
                     am = a;
                     bm = b;
-                    a = an;
-                    b = bn;
+                    cur = nxt;
                }
                // D_j = c1 - 1/c2, c2 = D_{j+1} + c1, c1 = (j+1)/z, D_{n2} = 0; 1/c2
                // by the forward's own operations (the same bits).  D cotangent
