@@ -2472,7 +2472,7 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
           const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
           double sgn = (lo & 1) ? -1. : 1.;
-          const double ix2 = 1. / (x * x);
+          const double ix2 = 1. / (x * x), ix = 1. / x;
           for (int j = lo; j < hi; ++j) {
                // the window's next entries
                const double ps2 = rd(Psi, j + 2, j + 2 <= n1), ch2 = rd(Chi, j + 2, j + 2 <= n1);
@@ -2481,7 +2481,7 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                const double v2 = rdu(2, j + 1, j + 1 <= n1), v3 = rdu(3, j + 1, j + 1 <= n1);
                cd lc = cmk(0., 0.);
                if (j <= n1) {
-                    if (j < n1) nxt = mie_ab_rev((j + 1) / x, m, im1, D1, ps1, ps0, ch1, ch0);
+                    if (j < n1) nxt = mie_ab_rev((j + 1) * ix, m, im1, D1, ps1, ps0, ch1, ch0);
                     const cd a = cur.a, b = cur.b, an = nxt.a, bn = nxt.b;
                     const double psi = ps0;
                     const cd zj = cmk(ps0, ch0);
