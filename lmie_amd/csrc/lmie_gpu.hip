@@ -2407,10 +2407,12 @@ constexpr int REV_ROWS = 12;
 
 // a_j, b_j for the reverse sweep, with the reciprocals the adjoint reuses:
 // ra = 1/(u zeta_j - zeta_{j-1}), rb = 1/(v zeta_j - zeta_{j-1}) of the
-// denominators of a and b (Smith's reciprocal), u = D/m + j/x, v = m D + j/x;
-// a = (u psi_j - psi_{j-1}) ra.  Not the forward's bits (a product with the
-// reciprocal instead of a division): the reverse pass needs the values, and
-// d a / d u = (psi_j - a zeta_j) ra reuses ra.
+// denominators of a and b, u = D/m + j/x, v = m D + j/x; a = (u psi_j -
+// psi_{j-1}) ra.  Not the forward's bits (conj(d) / |d|^2 and a product with
+// the reciprocal instead of Smith's division; a few ulp): the reverse pass
+// needs the values, and d a / d u = (psi_j - a zeta_j) ra reuses ra.  |d|^2
+// stays far inside the double range: |d| >= |a numerator| and zeta_j of the
+// orders j <= n1(x) grows at most like (2j-1)!! / x^j.
 struct AbRev {
      cd a, b, ra, rb;
 };
@@ -2419,20 +2421,47 @@ __device__ __forceinline__ AbRev mie_ab_rev(double jx, cd m, cd im, cd Dj, doubl
      AbRev o;
      const cd zj = cmk(psi, chi1), zjm = cmk(psi0, chi0);
      const cd u = cadd(cmul(Dj, im), cmk(jx, 0.)), v = cadd(cmul(m, Dj), cmk(jx, 0.));
-     o.ra = cdiv_nr(cmk(1., 0.), csub(cmul(u, zj), zjm));
-     o.rb = cdiv_nr(cmk(1., 0.), csub(cmul(v, zj), zjm));
+     o.ra = crecip_nr(csub(cmul(u, zj), zjm));
+     o.rb = crecip_nr(csub(cmul(v, zj), zjm));
      o.a = cmul(csub(cscale(u, psi), cmk(psi0, 0.)), o.ra);
      o.b = cmul(csub(cscale(v, psi), cmk(psi0, 0.)), o.rb);
      return o;
 }
 
+// Element i of row `row` of a [rows][Np] array: the row pointer is uniform
+// over the wave (row and the base are), the lane's byte offset a 32-bit
+// register, so each load is one scalar-based address and no 64-bit lane
+// arithmetic.
+template <class T>
+__device__ __forceinline__ T ld_row(const T *base, int row, size_t Np, unsigned ioff) {
+     // the row pointer through readfirstlane: scalar registers, and the loop
+     // optimiser cannot fold the lane offset into a 64-bit pointer per lane
+     const unsigned long long a = (unsigned long long) (base + (size_t) row * Np);
+     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned) a);
+     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned) (a >> 32));
+     typedef const char __attribute__((address_space(1))) *gchar;
+     typedef const T __attribute__((address_space(1))) *gT;
+     const gchar rp = (gchar) (((unsigned long long) hi << 32) | lo);
+     return *(gT) (rp + ioff);
+}
+
 // Per chunk, with the carry entering it as c, every carry and D cotangent
 // inside is affine in c: carry out = P c + C0, z cotangent = Q c + Z0.
+// The order j of the loop is the same in every lane (the chunk is the
+// workgroup's), so the rows read are uniform: each window load takes a row
+// clamped to the array's extent (in bounds for every lane), unconditionally,
+// so no wait follows it before the value's use an order later.  Rows of psi,
+// chi and U past a lane's n1 are read but never used (every use is under
+// j <= n1); D past n2 - 1 is replaced by 0 (D_{n2} = 0 starts the
+// recurrence).  UP: the angular cotangents come from U (mixed path), else
+// from the FP64 cotangent planes.
+template <bool UP>
 __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
      if (i >= p.Np) return;
      const size_t Np = p.Np, plane = (size_t) p.Kp * Np, dpl = (size_t) p.n2rows * Np;
      const size_t ppl = (size_t) (p.n1max + 1) * Np, ld = (size_t) p.n1max + 2;
+     const unsigned o8 = (unsigned) i * 8u, o4 = (unsigned) i * 4u;
      cd P = cmk(1., 0.), C0 = cmk(0., 0.), Q = cmk(0., 0.), Z0 = cmk(0., 0.), mb = cmk(0., 0.);
      double fq = 0., xq = 0.;
      const int n1 = i < p.N ? p.n1[i] : 0, n2 = i < p.N ? p.n2[i] : 0;
@@ -2442,28 +2471,31 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           const cd m = cmk(p.mr, -p.mi), m2 = cmul(m, m), im1 = crecip(m), im2 = crecip(m2);
           const cd z = cscale(m, x), iz = crecip(z);
           const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
-          // a_n, b_n recomputed from the kept recurrences by the forward's own
-          // code (the same bits), one order ahead for the asymmetry series.
-          // The recurrences are read through a window (psi, chi at j-1, j,
-          // j+1; D at j, j+1; the U row of j) whose next entries are loaded
-          // one order ahead of their use, off the dependence chain.
-          auto rd = [&](const double *a, int n, bool ok) { return ok ? a[(size_t) n * Np + i] : 0.; };
+          const int pmax = p.n1max, dmax = p.n2rows - 1;
+          // a_n, b_n recomputed from the kept recurrences, one order ahead for
+          // the asymmetry series.  The recurrences are read through a window
+          // (psi, chi at j-1, j, j+1; D at j, j+1; the U row of j) whose next
+          // entries are loaded one order ahead of their use.
+          auto rps = [&](const double *a, int n) { return ld_row(a, min(n, pmax), Np, o8); };
+          auto rD = [&](int n) {
+               const int r = min(n, dmax);
+               const double re = ld_row(Dre, r, Np, o8), im = ld_row(Dim, r, Np, o8);
+               return n < n2 ? cmk(re, im) : cmk(0., 0.);
+          };
           const size_t uq = (size_t) p.Kq * Np;
-          auto rdu = [&](int q, int n, bool ok) { return ok && p.Uf ? (double) p.Uf[q * uq + (size_t) (n - 1) * Np + i] : 0.; };
-          double psm = rd(Psi, lo - 1, lo <= n1 + 1), ps0 = rd(Psi, lo, lo <= n1), ps1 = rd(Psi, lo + 1, lo + 1 <= n1);
-          double chm = rd(Chi, lo - 1, lo <= n1 + 1), ch0 = rd(Chi, lo, lo <= n1), ch1 = rd(Chi, lo + 1, lo + 1 <= n1);
-          cd D0 = cmk(rd(Dre, lo, lo < n2), rd(Dim, lo, lo < n2));
-          cd D1 = cmk(rd(Dre, lo + 1, lo + 1 < n2), rd(Dim, lo + 1, lo + 1 < n2));
-          double u0 = rdu(0, lo, lo <= n1), u1 = rdu(1, lo, lo <= n1), u2 = rdu(2, lo, lo <= n1), u3 = rdu(3, lo, lo <= n1);
+          auto rdu = [&](int q, int n) { return UP ? ld_row(p.Uf + q * uq, min(n, p.Kq) - 1, Np, o4) : 0.f; };
+          double psm = rps(Psi, lo - 1), ps0 = rps(Psi, lo), ps1 = rps(Psi, lo + 1);
+          double chm = rps(Chi, lo - 1), ch0 = rps(Chi, lo), ch1 = rps(Chi, lo + 1);
+          cd D0 = rD(lo), D1 = rD(lo + 1);
+          float u0 = rdu(0, lo), u1 = rdu(1, lo), u2 = rdu(2, lo), u3 = rdu(3, lo);
           cd am = cmk(0., 0.), bm = cmk(0., 0.);
           AbRev cur, nxt;
           cur.a = cur.b = cur.ra = cur.rb = cmk(0., 0.);
           nxt = cur;
           if (lo <= n1) {
                if (lo > 1) {
-                    const AbRev pr = mie_ab_rev((lo - 1) / x, m, im1,
-                                                cmk(Dre[(size_t) (lo - 1) * Np + i], Dim[(size_t) (lo - 1) * Np + i]),
-                                                psm, Psi[(size_t) (lo - 2) * Np + i], chm, Chi[(size_t) (lo - 2) * Np + i]);
+                    const AbRev pr = mie_ab_rev((lo - 1) / x, m, im1, rD(lo - 1), psm, rps(Psi, lo - 2), chm,
+                                                rps(Chi, lo - 2));
                     am = pr.a;
                     bm = pr.b;
                }
@@ -2475,10 +2507,9 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           const double ix2 = 1. / (x * x), ix = 1. / x;
           for (int j = lo; j < hi; ++j) {
                // the window's next entries
-               const double ps2 = rd(Psi, j + 2, j + 2 <= n1), ch2 = rd(Chi, j + 2, j + 2 <= n1);
-               const cd D2 = cmk(rd(Dre, j + 2, j + 2 < n2), rd(Dim, j + 2, j + 2 < n2));
-               const double v0 = rdu(0, j + 1, j + 1 <= n1), v1 = rdu(1, j + 1, j + 1 <= n1);
-               const double v2 = rdu(2, j + 1, j + 1 <= n1), v3 = rdu(3, j + 1, j + 1 <= n1);
+               const double ps2 = rps(Psi, j + 2), ch2 = rps(Chi, j + 2);
+               const cd D2 = rD(j + 2);
+               const float v0 = rdu(0, j + 1), v1 = rdu(1, j + 1), v2 = rdu(2, j + 1), v3 = rdu(3, j + 1);
                cd lc = cmk(0., 0.);
                if (j <= n1) {
                     if (j < n1) nxt = mie_ab_rev((j + 1) * ix, m, im1, D1, ps1, ps0, ch1, ch0);
@@ -2488,15 +2519,15 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                     const cd Dj = D0;
                     const double fv1 = 2. * j + 1., fv4 = p.ft[FT_FV4 * ld + j];
                     cd Sb, Tb;
-                    if (p.Uf) {
+                    if constexpr (UP) {
                          const cd us = cmk(u0, u1), ut = cmk(u2, u3);
                          // 1/2 Re(conj S U_S + conj T U_T), S = fv4 (a + b), T = fv4 (a - b)
                          fq += .5 * fv4 * ((a.r + b.r) * us.r + (a.i + b.i) * us.i + (a.r - b.r) * ut.r + (a.i - b.i) * ut.i);
                          Sb = cscale(us, f);
                          Tb = cscale(ut, f);
                     } else {
-                         Sb = cmk(p.bar[(size_t) (j - 1) * Np + i], p.bar[plane + (size_t) (j - 1) * Np + i]);
-                         Tb = cmk(p.bar[2 * plane + (size_t) (j - 1) * Np + i], p.bar[3 * plane + (size_t) (j - 1) * Np + i]);
+                         Sb = cmk(ld_row(p.bar, j - 1, Np, o8), ld_row(p.bar + plane, j - 1, Np, o8));
+                         Tb = cmk(ld_row(p.bar + 2 * plane, j - 1, Np, o8), ld_row(p.bar + 3 * plane, j - 1, Np, o8));
                     }
                     // angular part
                     cd ab = cscale(cadd(Sb, Tb), fv4), bb = cscale(csub(Sb, Tb), fv4);
@@ -2568,7 +2599,8 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      o[4 * Np] = Q.r; o[5 * Np] = Q.i; o[6 * Np] = Z0.r; o[7 * Np] = Z0.i;
      o[8 * Np] = mb.r; o[9 * Np] = mb.i; o[10 * Np] = fq; o[11 * Np] = xq;
 }
-__global__ __launch_bounds__(64) void rev_orders(RevArgs p) { rev_orders_body(p); }
+template <bool UP>
+__global__ __launch_bounds__(64) void rev_orders(RevArgs p) { rev_orders_body<UP>(p); }
 struct rev_orders_B {
      RevArgs p;
      unsigned gx, gy;
@@ -2576,7 +2608,7 @@ struct rev_orders_B {
 __global__ __launch_bounds__(64) void rev_orders_b(const rev_orders_B *cases) {
      const rev_orders_B a = cases[blockIdx.z];
      if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
-     rev_orders_body(a.p);
+     rev_orders_body<true>(a.p);
 }
 
 
@@ -4183,7 +4215,10 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.scan = w->revScan.as<double>();
      r.mbar = w->mbar.as<double>(); r.fbar = w->fbar.as<double>();
      r.xbar = out->x_bar ? w->fbar.as<double>() + Np : nullptr;
-     if (nch > 0) hipLaunchKernelGGL(rev_orders, dim3(Np / 64, nch), dim3(64), 0, s, r);
+     if (nch > 0) {
+          if (use_u) hipLaunchKernelGGL(rev_orders<true>, dim3(Np / 64, nch), dim3(64), 0, s, r);
+          else hipLaunchKernelGGL(rev_orders<false>, dim3(Np / 64, nch), dim3(64), 0, s, r);
+     }
      hipLaunchKernelGGL(rev_carry, dim3(Np / 64), dim3(64), 0, s, r);
      HIPCHECK(hipGetLastError());
      tm.mark(LMIE_T_ADJ_COEFS, 1);
