@@ -712,87 +712,122 @@ __global__ __launch_bounds__(64) void mie_coefs(CoefArgs p) {
      }
 }
 
+// Element i of row `row` of a [rows][Np] array: the row pointer is uniform
+// over the wave (row and the base are), the lane's byte offset a 32-bit
+// register, so each load is one scalar-based address and no 64-bit lane
+// arithmetic.
+// (double2: one 16-byte load, read as a built-in vector type.)
+template <class T>
+struct LdRowType {
+     typedef T type;
+};
+template <>
+struct LdRowType<double2> {
+     typedef double type __attribute__((ext_vector_type(2)));
+};
+template <class T>
+__device__ __forceinline__ T ld_row(const T *base, int row, size_t Np, unsigned ioff) {
+     // the row pointer through readfirstlane: scalar registers, and the loop
+     // optimiser cannot fold the lane offset into a 64-bit pointer per lane
+     typedef typename LdRowType<T>::type L;
+     const unsigned long long a = (unsigned long long) (base + (size_t) row * Np);
+     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned) a);
+     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned) (a >> 32));
+     typedef const char __attribute__((address_space(1))) *gchar;
+     typedef const L __attribute__((address_space(1))) *gL;
+     const gchar rp = (gchar) (((unsigned long long) hi << 32) | lo);
+     const L v = *(gL) (rp + ioff);
+     if constexpr (std::is_same<T, double2>::value) return double2{v.x, v.y};
+     else return v;
+}
+
+// 1/q from the hardware reciprocal r (about 2^-24 relative) and the series
+// r (1 + e + e^2), e = 1 - q r: three dependent operations after the
+// reciprocal instead of the four of two Newton steps, error e^3 (< 2^-72)
+// plus one rounding.  For the D chain, whose steps are latency bound.
+__device__ __forceinline__ double rcp_s3(double q) {
+     const double r = __builtin_amdgcn_rcp(q);
+     const double e = __builtin_fma(-q, r, 1.);
+     return __builtin_fma(r, __builtin_fma(e, e, e), r);
+}
+
 // ---------------------------------------------------------------------------
-// Value channel in three kernels (the same operations as mie_coefs<0>, so the
-// same bits): mie_chains runs the three recurrences that are serial in n, two
-// threads per size in different waves (the D chain; the ratio chain and then
-// the psi, chi chain); mie_terms forms a_n, b_n, the contraction operand and the
-// partial series sums in parallel, one thread per (size, group of 8 orders);
-// mie_sums adds the partial sums per size.  The serial part is a few
-// operations per order, the parallel part fills the GPU.
+// Value channel in three kernels: mie_chains runs the three recurrences that
+// are serial in n, two threads per size in different waves (the D chain; the
+// ratio chain and then the psi, chi chain); mie_terms forms a_n, b_n, the
+// contraction operand and the partial series sums in parallel, one thread
+// per (size, group of 8 orders); mie_sums adds the partial sums per size.  The
+// serial part is a few operations per order, the parallel part fills the GPU.
+// D_j is kept as a complex pair, one 16-byte element per (order, size), so
+// each step of the D chain stores one element.
 struct ChainArgs {
      int N, Np, n1max, n2rows;
      const double *x, *sx, *cx;
      const int *n1, *n2;
      double mr, mi;
      int d_all;                 // store D_j up to n2 - 1 (reverse pass) instead of n1
-     double *D;                 // [2][n2rows][Np]: D_j re, im at row j
-     double *PC;                // [2][n1max + 1][Np]: psi_j, chi_j at row j
+     double *D;                 // [n2rows][Np][2]: D_j (re, im) at row j
+     double *PC;                // [2][n1max + 1][Np]: U_j (psi_j = s U_j), chi_j at row j
+     double *PS;                // [Np]: s
 };
 
 __device__ __forceinline__ void mie_chains_body(ChainArgs p) {
 #pragma clang fp contract(off)
      const int i = blockIdx.x * blockDim.x + threadIdx.x;
      if (i >= p.N) return;
-     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
+     const size_t Np = p.Np;
      const double x = p.x[i];
      const int n1 = p.n1[i], n2 = p.n2[i], dtop = p.d_all ? n2 - 1 : n1;
-     double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
+     double2 *Dv = (double2 *) p.D;
      if (blockIdx.y == 0) {
-          // downward D_j(z) = (j+1)/z - 1 / (D_{j+1} + (j+1)/z), D_{n2} = 0; one
-          // reciprocal per step (|D_{j+1} + (j+1)/z| stays between about 1/|z|
-          // and n2/|z|), the recurrence damps its last-bit differences
+          // downward D_j(z) = (j+1)/z - 1 / w_j, w_j = D_{j+1} + (j+1)/z,
+          // D_{n2} = 0.  The chain carries w: w_{j-1} = (2j+1)/z - 1 / w_j,
+          // w_{n2-1} = n2 / z, with 1 / w = conj(w) / |w|^2 (|w| stays between
+          // about 1/|z| and n2/|z|) and each update one fused multiply-add per
+          // component; D_j = (j+1)/z - 1 / w_j is formed beside the chain.  The
+          // recurrence damps its last-bit differences.
           const cd z = cscale(cmk(p.mr, -p.mi), x), iz = crecip(z);
-          cd D = cmk(0., 0.);
+          cd w = cscale(iz, (double) n2);
           for (int j = n2 - 1; j >= 1; --j) {
-               const cd c1 = cscale(iz, j + 1.);
-               const cd c2 = cadd(D, c1);
-               const cd rc = crecip_nr(c2);
-               D = csub(c1, rc);
-               if (j <= dtop) {
-                    Dre[j * Np + i] = D.r;
-                    Dim[j * Np + i] = D.i;
-               }
+               const cd c1 = cscale(iz, j + 1.), e = cscale(iz, 2. * j + 1.);
+               const double q = rcp_s3(__builtin_fma(w.r, w.r, w.i * w.i));
+               if (j <= dtop)
+                    Dv[j * Np + i] = double2{__builtin_fma(-w.r, q, c1.r), __builtin_fma(w.i, q, c1.i)};
+               w = cmk(__builtin_fma(-w.r, q, e.r), __builtin_fma(w.i, q, e.i));
           }
           return;
      }
-     // the other chains in other waves (blockIdx.y = 1): the downward ratio
-     // r_j = 1 / ((2j+1)/x - r_{j+1}), r_{n1} = 0, kept in the psi rows; (2j+1)/x
-     // as (2j+1) (1/x) and the refined reciprocal (about 1 ulp each, damped by the
-     // downward recurrence), so no division sits in the loops
-     const double ax = 1. / x;
-     double r = 0.;
-     Psi[n1 * Np + i] = 0.;
-     for (int j = n1 - 1; j >= 1; --j) {
-          const double c = (2. * j + 1.) * ax;
-          r = rcp_nr(c - r);
-          Psi[j * Np + i] = r;
-     }
-     // upward psi_j = r_j psi_{j-1}, chi_{j+1} = (2j+1)/x chi_j - chi_{j-1}
-     const double sx = p.sx[i], cx = p.cx[i];
-     double psi0 = sx, chi0 = cx, chi1 = ax * cx + sx;
-     Psi[i] = psi0;
+     // the other chains in another wave (blockIdx.y = 1), one loop: psi by
+     // Miller's downward recurrence, U_{n1} = 0, U_{n1-1} = 1,
+     // U_{j-1} = (2j+1)/x U_j - U_{j+1}, psi_j = s U_j with s = sin x / U_0 (the
+     // ratios r_j = U_j / U_{j-1} of the reference's ratio chain, whose
+     // product from psi_0 = sin x is the same psi_j), beside the upward
+     // chi_{j+1} = (2j+1)/x chi_j - chi_{j-1}; two independent dependence
+     // chains of two operations per order and no round trip through memory.
+     // U grows downward by at most the decay of psi_j between n1 - 1 and 0.
+     const size_t ppl = (size_t) (p.n1max + 1) * Np;
+     double *U = p.PC, *Chi = p.PC + ppl;
+     const double ax = 1. / x, sx = p.sx[i], cx = p.cx[i];
+     double ua = 0., ub = 1.;               // U_{j+1}, U_j, starting at j = n1 - 1
+     U[n1 * Np + i] = 0.;
+     U[(n1 - 1) * Np + i] = 1.;
+     double chi0 = cx, chi1 = ax * cx + sx;
      Chi[i] = chi0;
-     // ratios read 16 rows ahead so the loads are off the dependence chain
-     for (int jb = 1; jb <= n1; jb += 16) {
-          double rr[16];
-#pragma unroll
-          for (int e = 0; e < 16; ++e) rr[e] = jb + e <= n1 ? Psi[(size_t) (jb + e) * Np + i] : 0.;
-#pragma unroll
-          for (int e = 0; e < 16; ++e) {
-               const int j = jb + e;
-               if (j <= n1) {
-                    const double psi = rr[e] * psi0;
-                    Psi[(size_t) j * Np + i] = psi;
-                    Chi[(size_t) j * Np + i] = chi1;
-                    const double fv1 = 2. * j + 1.;
-                    const double chi2 = fv1 * ax * chi1 - chi0;
-                    chi0 = chi1;
-                    chi1 = chi2;
-                    psi0 = psi;
-               }
+     for (int t = 1; t <= n1; ++t) {
+          Chi[t * Np + i] = chi1;
+          const double fv1 = 2. * t + 1.;
+          const double chi2 = fv1 * ax * chi1 - chi0;
+          chi0 = chi1;
+          chi1 = chi2;
+          const int j = n1 - t;
+          if (j >= 1) {
+               const double un = (2. * j + 1.) * ax * ub - ua;
+               U[(j - 1) * Np + i] = un;
+               ua = ub;
+               ub = un;
           }
      }
+     p.PS[i] = sx / ub;
 }
 __global__ __launch_bounds__(64) void mie_chains(ChainArgs p) { mie_chains_body(p); }
 struct mie_chains_B {
@@ -862,7 +897,8 @@ struct TermArgs {
      const int *n1;
      const int *kfill;          // [Np / TILE]
      double mr, mi;
-     const double *D, *PC;      // from mie_chains
+     const double *D, *PC, *PS; // from mie_chains: [n2rows][Np][2] D_j, [2][n1max + 1][Np] U_j and
+                                // chi_j, [Np] s (psi_j = s U_j)
      double *A;                 // FP64 planes [4][Kp][Np] (E = void)
      unsigned short *A16;       // split planes [8][Kb][Np][8] (E = F16 / BF16)
      double *part;              // [5][ng][Np]: se, ss, cb re, cb im, sg of each group
@@ -874,7 +910,7 @@ __device__ __forceinline__ void mie_terms_body(TermArgs p) {
      constexpr bool MIX = !std::is_void<E>::value;
      const int i = blockIdx.x * blockDim.x + threadIdx.x, g = blockIdx.y;
      if (i >= p.Np || g >= p.ng) return;
-     const size_t Np = p.Np, dpl = (size_t) p.n2rows * Np, ppl = (size_t) (p.n1max + 1) * Np;
+     const size_t Np = p.Np;
      const size_t plane = (size_t) p.Kp * Np, p16 = (size_t) p.Kb * Np * 8, gp = (size_t) p.ng * Np;
      const size_t ld = (size_t) p.n1max + 2;
      const int kf = p.kfill[i / TILE];
@@ -891,21 +927,39 @@ __device__ __forceinline__ void mie_terms_body(TermArgs p) {
      if (j0 <= j1) {
           const double x = p.x[i];
           const cd m = cmk(p.mr, -p.mi), im = crecip(m);
-          const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
+          // the group's rows are the same in every lane: all of them are
+          // loaded at once (rows clamped to the arrays; those past a lane's
+          // n1 are read, not used), so the loads overlap
+          const double2 *Dv = (const double2 *) p.D;
+          const double *U = p.PC, *Chi = p.PC + (size_t) (p.n1max + 1) * Np;
+          const unsigned o8 = (unsigned) i * 8u, o16 = (unsigned) i * 16u;
+          const int pmax = p.n1max, dmax = p.n2rows - 1;
+          const double sc = p.PS[i];
+          double ps[10], ch[10], dr[9], di[9];   // rows j0 - 2 .. j0 + 7 (psi, chi), j0 - 1 .. j0 + 7 (D)
+#pragma unroll
+          for (int e = 0; e < 10; ++e) {
+               const int r = min(max(j0 - 2 + e, 0), pmax);
+               ps[e] = sc * ld_row(U, r, Np, o8);
+               ch[e] = ld_row(Chi, r, Np, o8);
+          }
+#pragma unroll
+          for (int e = 0; e < 9; ++e) {
+               const double2 v = ld_row(Dv, min(j0 - 1 + e, dmax), Np, o16);
+               dr[e] = v.x;
+               di[e] = v.y;
+          }
           cd ap = cmk(0., 0.), bp = cmk(0., 0.);
-          double psi0 = Psi[(j0 - 1) * Np + i], chi0 = Chi[(j0 - 1) * Np + i];
-          if (j0 > 1)
-               mie_ab_at(j0 - 1, x, m, im, cmk(Dre[(j0 - 1) * Np + i], Dim[(j0 - 1) * Np + i]), psi0,
-                         Psi[(j0 - 2) * Np + i], chi0, Chi[(j0 - 2) * Np + i], &ap, &bp);
+          double psi0 = ps[1], chi0 = ch[1];
+          if (j0 > 1) mie_ab_at(j0 - 1, x, m, im, cmk(dr[0], di[0]), psi0, ps[0], chi0, ch[0], &ap, &bp);
           double sgn = (j0 & 1) ? -1. : 1.;
           // the 8 orders unrolled: the split halves land in registers
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
                const int j = j0 + e;
                if (j <= j1) {
-                    const double psi = Psi[j * Np + i], chi = Chi[j * Np + i];
+                    const double psi = ps[e + 2], chi = ch[e + 2];
                     cd a, b;
-                    mie_ab_at(j, x, m, im, cmk(Dre[j * Np + i], Dim[j * Np + i]), psi, psi0, chi, chi0, &a, &b);
+                    mie_ab_at(j, x, m, im, cmk(dr[e + 1], di[e + 1]), psi, psi0, chi, chi0, &a, &b);
                     const double fv1 = 2. * j + 1.;
                     se += fv1 * (a.r + b.r);
                     ss += fv1 * (cabs2(a) + cabs2(b));
@@ -2389,7 +2443,7 @@ struct RevArgs {
      const double *x, *f;
      const int *n1, *n2;
      double mr, mi;
-     const double *D, *PC;      // forward recurrences (mie_chains, D up to n2 - 1)
+     const double *D, *PC, *PS; // forward recurrences (mie_chains, D up to n2 - 1)
      const double *CB;          // [2][Np]
      const double *S;           // series [4][Np]
      const double *bar;         // [4][Kp][Np] f-weighted cotangents (FP64 path), or
@@ -2428,23 +2482,6 @@ __device__ __forceinline__ AbRev mie_ab_rev(double jx, cd m, cd im, cd Dj, doubl
      return o;
 }
 
-// Element i of row `row` of a [rows][Np] array: the row pointer is uniform
-// over the wave (row and the base are), the lane's byte offset a 32-bit
-// register, so each load is one scalar-based address and no 64-bit lane
-// arithmetic.
-template <class T>
-__device__ __forceinline__ T ld_row(const T *base, int row, size_t Np, unsigned ioff) {
-     // the row pointer through readfirstlane: scalar registers, and the loop
-     // optimiser cannot fold the lane offset into a 64-bit pointer per lane
-     const unsigned long long a = (unsigned long long) (base + (size_t) row * Np);
-     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned) a);
-     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned) (a >> 32));
-     typedef const char __attribute__((address_space(1))) *gchar;
-     typedef const T __attribute__((address_space(1))) *gT;
-     const gchar rp = (gchar) (((unsigned long long) hi << 32) | lo);
-     return *(gT) (rp + ioff);
-}
-
 // Per chunk, with the carry entering it as c, every carry and D cotangent
 // inside is affine in c: carry out = P c + C0, z cotangent = Q c + Z0.
 // The order j of the loop is the same in every lane (the chunk is the
@@ -2459,9 +2496,8 @@ template <bool UP>
 __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      const int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
      if (i >= p.Np) return;
-     const size_t Np = p.Np, plane = (size_t) p.Kp * Np, dpl = (size_t) p.n2rows * Np;
-     const size_t ppl = (size_t) (p.n1max + 1) * Np, ld = (size_t) p.n1max + 2;
-     const unsigned o8 = (unsigned) i * 8u, o4 = (unsigned) i * 4u;
+     const size_t Np = p.Np, plane = (size_t) p.Kp * Np, ld = (size_t) p.n1max + 2;
+     const unsigned o4 = (unsigned) i * 4u, o8 = (unsigned) i * 8u, o16 = (unsigned) i * 16u;
      cd P = cmk(1., 0.), C0 = cmk(0., 0.), Q = cmk(0., 0.), Z0 = cmk(0., 0.), mb = cmk(0., 0.);
      double fq = 0., xq = 0.;
      const int n1 = i < p.N ? p.n1[i] : 0, n2 = i < p.N ? p.n2[i] : 0;
@@ -2470,22 +2506,27 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           const double x = p.x[i], f = p.f[i];
           const cd m = cmk(p.mr, -p.mi), m2 = cmul(m, m), im1 = crecip(m), im2 = crecip(m2);
           const cd z = cscale(m, x), iz = crecip(z);
-          const double *Dre = p.D, *Dim = p.D + dpl, *Psi = p.PC, *Chi = p.PC + ppl;
+          const double2 *Dv = (const double2 *) p.D;
+          const double *U = p.PC, *Chi = p.PC + (size_t) (p.n1max + 1) * Np;
+          const double sc = p.PS[i];
           const int pmax = p.n1max, dmax = p.n2rows - 1;
           // a_n, b_n recomputed from the kept recurrences, one order ahead for
           // the asymmetry series.  The recurrences are read through a window
           // (psi, chi at j-1, j, j+1; D at j, j+1; the U row of j) whose next
           // entries are loaded one order ahead of their use.
-          auto rps = [&](const double *a, int n) { return ld_row(a, min(n, pmax), Np, o8); };
+          auto rpc = [&](int n) {   // (psi_n, chi_n)
+               const int r = min(n, pmax);
+               return double2{sc * ld_row(U, r, Np, o8), ld_row(Chi, r, Np, o8)};
+          };
           auto rD = [&](int n) {
-               const int r = min(n, dmax);
-               const double re = ld_row(Dre, r, Np, o8), im = ld_row(Dim, r, Np, o8);
-               return n < n2 ? cmk(re, im) : cmk(0., 0.);
+               const double2 v = ld_row(Dv, min(n, dmax), Np, o16);
+               return n < n2 ? cmk(v.x, v.y) : cmk(0., 0.);
           };
           const size_t uq = (size_t) p.Kq * Np;
           auto rdu = [&](int q, int n) { return UP ? ld_row(p.Uf + q * uq, min(n, p.Kq) - 1, Np, o4) : 0.f; };
-          double psm = rps(Psi, lo - 1), ps0 = rps(Psi, lo), ps1 = rps(Psi, lo + 1);
-          double chm = rps(Chi, lo - 1), ch0 = rps(Chi, lo), ch1 = rps(Chi, lo + 1);
+          const double2 pcm = rpc(lo - 1), pc0 = rpc(lo), pc1 = rpc(lo + 1);
+          double psm = pcm.x, ps0 = pc0.x, ps1 = pc1.x;
+          double chm = pcm.y, ch0 = pc0.y, ch1 = pc1.y;
           cd D0 = rD(lo), D1 = rD(lo + 1);
           float u0 = rdu(0, lo), u1 = rdu(1, lo), u2 = rdu(2, lo), u3 = rdu(3, lo);
           cd am = cmk(0., 0.), bm = cmk(0., 0.);
@@ -2494,8 +2535,8 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           nxt = cur;
           if (lo <= n1) {
                if (lo > 1) {
-                    const AbRev pr = mie_ab_rev((lo - 1) / x, m, im1, rD(lo - 1), psm, rps(Psi, lo - 2), chm,
-                                                rps(Chi, lo - 2));
+                    const double2 pc2 = rpc(lo - 2);
+                    const AbRev pr = mie_ab_rev((lo - 1) / x, m, im1, rD(lo - 1), psm, pc2.x, chm, pc2.y);
                     am = pr.a;
                     bm = pr.b;
                }
@@ -2507,7 +2548,8 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
           const double ix2 = 1. / (x * x), ix = 1. / x;
           for (int j = lo; j < hi; ++j) {
                // the window's next entries
-               const double ps2 = rps(Psi, j + 2), ch2 = rps(Chi, j + 2);
+               const double2 pc2 = rpc(j + 2);
+               const double ps2 = pc2.x, ch2 = pc2.y;
                const cd D2 = rD(j + 2);
                const float v0 = rdu(0, j + 1), v1 = rdu(1, j + 1), v2 = rdu(2, j + 1), v3 = rdu(3, j + 1);
                cd lc = cmk(0., 0.);
@@ -3038,7 +3080,7 @@ struct Workspace {
      // reverse-pass checkpoint and buffers
      Buf CB, sst, pfbar, bar, fang, mbar, fbar, gbar, expart;
      Buf wts, scal, L16, R16, Gf, G16, Uf, tasks;   // mixed reverse pass
-     Buf chD, chPC, chPart, revScan;   // per-size recurrences of the value channel
+     Buf chD, chPC, chPS, chPart, revScan;   // per-size recurrences of the value channel
      Buf ftab;                      // order_factors of the value channel
      Buf sw, s1, s2;                // save_control 2: channel weights, saved per-size arrays
      Buf bargs, bup, bdown;         // per-case arguments and staging of a batch launched from here
@@ -3719,14 +3761,14 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           for (int t = 0; t < nt64; ++t) kfmax = kfill[t] > kfmax ? kfill[t] : kfmax;
           const int n2rows = n2max + 1, ng = kfmax / 8;
           if (w->chD.get(sizeof(double) * 2 * (size_t) n2rows * Np) ||
-              w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) ||
+              w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) || w->chPS.get(sizeof(double) * Np) ||
               w->chPart.get(sizeof(double) * 5 * (size_t) (ng > 0 ? ng : 1) * Np))
                return -1;
           ChainArgs ca;
           ca.N = N; ca.Np = Np; ca.n1max = n1max; ca.n2rows = n2rows;
           ca.x = w->x.as<double>(); ca.sx = w->sx.as<double>(); ca.cx = w->cx.as<double>();
           ca.n1 = w->n1.as<int>(); ca.n2 = w->n2.as<int>(); ca.mr = J->mr; ca.mi = J->mi;
-          ca.d_all = keep; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
+          ca.d_all = keep; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>(); ca.PS = w->chPS.as<double>();
           hipLaunchKernelGGL(mie_chains, dim3(Np / 64, 2), dim3(64), 0, s, ca);
           if (side_tables()) return -1;
           if (w->ftab.get(sizeof(double) * FT_ROWS * (size_t) (n1max + 2))) return -1;
@@ -3737,6 +3779,7 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
           ta.ft = w->ftab.as<double>();
           ta.x = w->x.as<double>(); ta.n1 = w->n1.as<int>(); ta.kfill = w->kfill.as<int>();
           ta.mr = J->mr; ta.mi = J->mi; ta.D = w->chD.as<double>(); ta.PC = w->chPC.as<double>();
+          ta.PS = w->chPS.as<double>();
           ta.A = mixed ? nullptr : w->A.as<double>();
           ta.A16 = mixed ? w->A16.as<unsigned short>() : nullptr;
           ta.part = w->chPart.as<double>();
@@ -4207,7 +4250,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.ft = w->ftab.as<double>();
      r.x = w->x.as<double>(); r.f = w->f.as<double>();
      r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
-     r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>();
+     r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>(); r.PS = w->chPS.as<double>();
      r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
      r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
      r.fang = w->fang.as<double>();
@@ -4325,7 +4368,7 @@ int fused_prepare(FwdCase &c, Upload &up) {
          w->B16.get(sizeof(unsigned short) * B16_PLANES * (size_t) Kb * Hb * 8) || w->S.get(sizeof(double) * 4 * Np) ||
          w->A16.get(sizeof(unsigned short) * 8 * (size_t) Kb * Np * 8) || w->CB.get(sizeof(double) * 2 * Np) ||
          w->chD.get(sizeof(double) * 2 * (size_t) n2rows * Np) ||
-         w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) ||
+         w->chPC.get(sizeof(double) * 2 * (size_t) (n1max + 1) * Np) || w->chPS.get(sizeof(double) * Np) ||
          w->chPart.get(sizeof(double) * 5 * (size_t) (ng > 0 ? ng : 1) * Np) ||
          w->ftab.get(sizeof(double) * FT_ROWS * (size_t) (n1max + 2)) || w->dotjobs.get(sizeof(DotJob) * 4) ||
          w->dots.get(sizeof(double) * 4) || w->pf.get(sizeof(double) * 4 * (size_t) Q) ||
@@ -4390,7 +4433,7 @@ int fused_launch(std::vector<FwdCase> &cs, Upload &up, Workspace *L) {
           ca.N = c.N; ca.Np = c.Np; ca.n1max = c.n1max; ca.n2rows = c.n2rows;
           ca.x = w->x.as<double>(); ca.sx = w->sx.as<double>(); ca.cx = w->cx.as<double>();
           ca.n1 = w->n1.as<int>(); ca.n2 = w->n2.as<int>(); ca.mr = c.J->mr; ca.mi = c.J->mi;
-          ca.d_all = 1; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>();
+          ca.d_all = 1; ca.D = w->chD.as<double>(); ca.PC = w->chPC.as<double>(); ca.PS = w->chPS.as<double>();
           chains[k] = mie_chains_B{ca, (unsigned) c.Np / 64, 2};
           gauss[k] = gauss_nodes_B{c.Q, w->guess.as<double>(), w->mu.as<double>(), w->w.as<double>(),
                                    (unsigned) (c.H + 63) / 64, 1};
@@ -4403,7 +4446,7 @@ int fused_launch(std::vector<FwdCase> &cs, Upload &up, Workspace *L) {
           ta.N = c.N; ta.Np = c.Np; ta.n1max = c.n1max; ta.n2rows = c.n2rows; ta.Kp = c.Kp; ta.Kb = c.Kb; ta.ng = c.ng;
           ta.ft = w->ftab.as<double>(); ta.x = w->x.as<double>(); ta.n1 = w->n1.as<int>();
           ta.kfill = w->kfill.as<int>(); ta.mr = c.J->mr; ta.mi = c.J->mi; ta.D = w->chD.as<double>();
-          ta.PC = w->chPC.as<double>(); ta.A = nullptr; ta.A16 = w->A16.as<unsigned short>();
+          ta.PC = w->chPC.as<double>(); ta.PS = w->chPS.as<double>(); ta.A = nullptr; ta.A16 = w->A16.as<unsigned short>();
           ta.part = w->chPart.as<double>();
           terms[k] = mie_terms_B{ta, (unsigned) c.Np / 64, (unsigned) c.ng};
           sums[k] = mie_sums_B{w->chPart.as<double>(), c.N, c.Np, c.ng, w->n1.as<int>(), w->S.as<double>(),
@@ -4609,7 +4652,7 @@ int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
           r.N = w->N; r.Np = Np; r.Kp = w->Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.nch = c.nch;
           r.ft = w->ftab.as<double>(); r.x = w->x.as<double>(); r.f = w->f.as<double>();
           r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
-          r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>();
+          r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>(); r.PS = w->chPS.as<double>();
           r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = nullptr;   // read only without Uf
           r.Uf = w->Uf.as<float>(); r.Kq = c.Kq; r.fang = w->fang.as<double>();
           r.sb0 = c.in->sums_bar[0]; r.sb1 = c.in->sums_bar[1]; r.sb2 = c.in->sums_bar[2]; r.sb3 = c.in->sums_bar[3];
