@@ -394,6 +394,15 @@ __global__ void pitau_table16_b(const pitau_table16_B *cases) {
 }
 
 
+// Value of lane `lane` (uniform) of a double: a scalar operand for the whole
+// wave.
+__device__ __forceinline__ double readlane_d(double v, int lane) {
+     const unsigned long long u = (unsigned long long) __double_as_longlong(v);
+     const unsigned lo = __builtin_amdgcn_readlane((unsigned) u, lane);
+     const unsigned hi = __builtin_amdgcn_readlane((unsigned) (u >> 32), lane);
+     return __longlong_as_double((long long) (((unsigned long long) hi << 32) | lo));
+}
+
 // Generalized spherical functions at the positive angles, l = 0 .. C-1, by
 // their three-term recurrences in l (misc/rtutil_math.c:956-1016).  Layout
 // [4][C][Hp] (+ plane 4: P_l for the Legendre expansion,
@@ -401,62 +410,86 @@ __global__ void pitau_table16_b(const pitau_table16_B *cases) {
 // The five recurrences are independent: blockIdx.y picks one (0: P00, 1: P02,
 // 2: P22, 3: P2-2, 4: P_l), so five times as many threads run one fifth of the
 // steps.  Columns H .. Hp-1 are written as zeros.
+// Each step divides the reference's numerator by its divisor d_l (L1, sqrt(L1^2
+// - 4) or L (L1^2 - 4)) in the reference's order of operations; the quotient
+// is q = RN(x y) corrected once, q + y RN(x - q d), with y = RN(1 / d_l):
+// Markstein's correction, which gives the correctly rounded x / d (the IEEE
+// division's bits) when y is the correctly rounded reciprocal and q is
+// within an ulp.  The divisors' reciprocals, the square roots and the
+// Legendre factors (l-1)/l of 64 consecutive l are formed one per lane (in
+// parallel) and each step reads its l's from that lane as scalars, so the
+// dependence chain of a step is the numerator and three operations instead
+// of a full division.
 __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const double *mu, double *T) {
      // separately rounded operations in the reference's order (bitwise the
      // reference's functions)
 #pragma clang fp contract(off)
-     const int j = blockIdx.x * blockDim.x + threadIdx.x, fn = blockIdx.y;
+     const int j = blockIdx.x * blockDim.x + threadIdx.x, fn = blockIdx.y, lane = threadIdx.x & 63;
      if (j >= Hp) return;
      double *P = T + (size_t) fn * C * Hp;
-     if (j >= H) {
-          for (int l = 0; l < C; ++l) P[(size_t) l * Hp + j] = 0.;
-          return;
-     }
-     const double m = mu[H + j], m2 = m * m;
-     if (fn == 4) {
-          double lp0 = 1., lp1 = m;
-          for (int l = 0; l < C; ++l) {
+     // every lane stays in the loops (its l's are read by the others); the
+     // columns past H run with mu = 0 and store zeros
+     const bool live = j < H;
+     const double m = live ? mu[H + j] : 0., m2 = m * m;
+     double a0 = 0., b0 = 0.;
+     for (int l0 = 0; l0 < C; l0 += 64) {
+          // this lane's order l0 + lane: divisor, its reciprocal, square root
+          // (fn 1), Legendre factor (fn 4)
+          double dv = 1., yv = 1., sav = 0.;
+          {
+               const int lc = l0 + lane;
+               const double L = lc - 1, L1 = lc;
+               if (fn == 4) {
+                    yv = lc >= 2 ? (lc - 1) / (double) lc : 0.;
+               } else if (lc >= 3) {
+                    if (fn == 0) dv = L1;
+                    else if (fn == 1) {
+                         sav = sqrt(L * L - 4.);
+                         dv = sqrt(L1 * L1 - 4.);
+                    } else dv = L * (L1 * L1 - 4);
+                    yv = 1. / dv;
+               }
+          }
+          const int ne = min(64, C - l0);
+          for (int e = 0; e < ne; ++e) {
+               const int l = l0 + e;
                double v;
-               if (l == 0) v = 1.;
-               else if (l == 1) v = m;
-               else {
-                    const double a = (l - 1) / (double) l, b = m * lp1;
-                    v = a * (b - lp0) + b;
-                    lp0 = lp1;
-                    lp1 = v;
-               }
-               P[(size_t) l * Hp + j] = v;
-          }
-          return;
-     }
-     double a0 = fn == 0 ? 1. : 0., b0 = fn == 0 ? m : 0.;
-     for (int l = 0; l < C; ++l) {
-          double v;
-          if (l == 0) v = fn == 0 ? 1. : 0.;
-          else if (l == 1) v = fn == 0 ? m : 0.;
-          else if (l == 2) {
-               if (fn == 0) v = (3. * m2 - 1.) / 2.;
-               else if (fn == 1) v = .25 * sqrt(6.) * (m2 - 1.);
-               else if (fn == 2) v = .25 * (1. + m) * (1. + m);
-               else v = .25 * (1. - m) * (1. - m);
-          } else {
-               const double L = l - 1, L1 = l, t = 2. * L + 1.;
-               if (fn == 0) {
-                    const double da = t * m;
-                    v = (da * b0 - L * a0) / L1;
-               } else if (fn == 1) {
-                    const double sa = sqrt(L * L - 4.), sb = sqrt(L1 * L1 - 4.);
-                    const double da = t * m;
-                    v = (da * b0 - sa * a0) / sb;
+               if (fn == 4) {
+                    if (l == 0) v = 1.;
+                    else if (l == 1) v = m;
+                    else {
+                         const double a = readlane_d(yv, e), bb = m * b0;
+                         v = a * (bb - a0) + bb;
+                    }
+               } else if (l < 3) {
+                    if (l == 0) v = fn == 0 ? 1. : 0.;
+                    else if (l == 1) v = fn == 0 ? m : 0.;
+                    else if (fn == 0) v = (3. * m2 - 1.) / 2.;
+                    else if (fn == 1) v = .25 * sqrt(6.) * (m2 - 1.);
+                    else if (fn == 2) v = .25 * (1. + m) * (1. + m);
+                    else v = .25 * (1. - m) * (1. - m);
                } else {
-                    const double k4 = t * 4, kb = L1 * (L * L - 4), kd = L * (L1 * L1 - 4);
-                    const double dc = t * (L * L1) * m;
-                    v = fn == 2 ? ((dc - k4) * b0 - kb * a0) / kd : ((dc + k4) * b0 - kb * a0) / kd;
+                    const double L = l - 1, L1 = l, t = 2. * L + 1.;
+                    const double d = readlane_d(dv, e), y = readlane_d(yv, e);
+                    double x;
+                    if (fn == 0) {
+                         const double da = t * m;
+                         x = da * b0 - L * a0;
+                    } else if (fn == 1) {
+                         const double da = t * m;
+                         x = da * b0 - readlane_d(sav, e) * a0;
+                    } else {
+                         const double k4 = t * 4, kb = L1 * (L * L - 4);
+                         const double dc = t * (L * L1) * m;
+                         x = fn == 2 ? (dc - k4) * b0 - kb * a0 : (dc + k4) * b0 - kb * a0;
+                    }
+                    const double q = x * y;
+                    v = __builtin_fma(__builtin_fma(-q, d, x), y, q);
                }
+               P[(size_t) l * Hp + j] = live ? v : 0.;
+               a0 = b0;
+               b0 = v;
           }
-          P[(size_t) l * Hp + j] = v;
-          a0 = b0;
-          b0 = v;
      }
 }
 __global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) { gsf_table_body(H, Hp, C, mu, T); }
