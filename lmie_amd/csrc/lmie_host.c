@@ -1291,20 +1291,84 @@ static void *dn_build(void *vp) {
      return NULL;
 }
 
-/* dn_build on a helper thread, started by core_run's on_device hook (or
-   inline where no thread can be had, and on the threads of a fused batch) */
+/* One persistent helper thread for dn_build, created on first use: a call
+   hands it its job instead of creating a thread (thread creation sat on the
+   forward's path).  A second concurrent caller finds it busy and creates a
+   thread of its own. */
+static struct {
+     pthread_mutex_t mu;
+     pthread_cond_t go, done;
+     int started, busy, has_job;
+     dn_job *job;
+} g_helper = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, PTHREAD_COND_INITIALIZER, 0, 0, 0, NULL};
+
+static void *helper_main(void *vp) {
+     (void) vp;
+     pthread_mutex_lock(&g_helper.mu);
+     for (;;) {
+          while (!g_helper.has_job) pthread_cond_wait(&g_helper.go, &g_helper.mu);
+          dn_job *j = g_helper.job;
+          pthread_mutex_unlock(&g_helper.mu);
+          dn_build(j);
+          pthread_mutex_lock(&g_helper.mu);
+          g_helper.has_job = 0;
+          pthread_cond_broadcast(&g_helper.done);
+     }
+     return NULL;
+}
+
+/* hand j to the helper: 0 if it took it (then helper_join), -1 if busy */
+static int helper_post(dn_job *j) {
+     pthread_mutex_lock(&g_helper.mu);
+     if (g_helper.busy) {
+          pthread_mutex_unlock(&g_helper.mu);
+          return -1;
+     }
+     if (!g_helper.started) {
+          pthread_t th;
+          if (pthread_create(&th, NULL, helper_main, NULL) != 0) {
+               pthread_mutex_unlock(&g_helper.mu);
+               return -1;
+          }
+          pthread_detach(th);
+          g_helper.started = 1;
+     }
+     g_helper.busy = 1;
+     g_helper.job = j;
+     g_helper.has_job = 1;
+     pthread_cond_signal(&g_helper.go);
+     pthread_mutex_unlock(&g_helper.mu);
+     return 0;
+}
+
+static void helper_join(void) {
+     pthread_mutex_lock(&g_helper.mu);
+     while (g_helper.has_job) pthread_cond_wait(&g_helper.done, &g_helper.mu);
+     g_helper.busy = 0;
+     pthread_mutex_unlock(&g_helper.mu);
+}
+
+/* dn_build on the helper thread (or a thread of its own), started by
+   core_run's on_device hook; inline where no thread can be had, and on the
+   threads of a fused batch */
 typedef struct {
      dn_job *job;
-     int running, done;
+     int running, pooled, done;
      pthread_t th;
 } dn_start;
 
 static void dn_launch(void *vp) {
      dn_start *st = vp;
-     if (st->running || st->done) return;
-     if (!g_batch_thread && pthread_create(&st->th, NULL, dn_build, st->job) == 0) {
-          st->running = 1;
-          return;
+     if (st->running || st->pooled || st->done) return;
+     if (!g_batch_thread) {
+          if (helper_post(st->job) == 0) {
+               st->pooled = 1;
+               return;
+          }
+          if (pthread_create(&st->th, NULL, dn_build, st->job) == 0) {
+               st->running = 1;
+               return;
+          }
      }
      dn_build(st->job);
      st->done = 1;
@@ -1368,7 +1432,8 @@ static int adjoint_forward_run(lmie_in_data *const *modes, const double *weight,
                      out->theta, out->pf, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL,
                      NULL, NULL, NULL, NULL, NULL, NULL, NULL, &out->save1, &out->save2, &keep);
      const double tw = wall_ms();
-     if (st.running) pthread_join(st.th, NULL);
+     if (st.pooled) helper_join();
+     else if (st.running) pthread_join(st.th, NULL);
      else if (!st.done && rc == 0) dn_build(&dj);     /* not started (cannot happen on success) */
      lmie_gpu_host_time(LMIE_T_HOST_WAIT, wall_ms() - tw);
      if (rc) {
