@@ -2476,6 +2476,7 @@ struct RevArgs {
      const double *x, *f;
      const int *n1, *n2;
      double mr, mi;
+     double im1r, im1i, im2r, im2i;   // 1/m, 1/m^2 (uniform, formed on the host)
      const double *D, *PC, *PS; // forward recurrences (mie_chains, D up to n2 - 1)
      const double *CB;          // [2][Np]
      const double *S;           // series [4][Np]
@@ -2491,6 +2492,24 @@ struct RevArgs {
      double *xbar;              // [Np] cotangent of x_i, or null (not formed)
 };
 constexpr int REV_ROWS = 12;
+
+// 1/m and 1/m^2 of m = mr - i mi for the reverse sweep, by Smith's division
+static void host_crecip(double br, double bi, double *rr, double *ri) {
+     if (fabs(br) >= fabs(bi)) {
+          const double t = bi / br, d = br + bi * t;
+          *rr = 1. / d;
+          *ri = -t / d;
+     } else {
+          const double t = br / bi, d = br * t + bi;
+          *rr = t / d;
+          *ri = -1. / d;
+     }
+}
+static void rev_recips(RevArgs &r) {
+     const double mr = r.mr, mi = -r.mi;
+     host_crecip(mr, mi, &r.im1r, &r.im1i);
+     host_crecip(mr * mr - mi * mi, 2. * mr * mi, &r.im2r, &r.im2i);
+}
 
 // a_j, b_j for the reverse sweep, with the reciprocals the adjoint reuses:
 // ra = 1/(u zeta_j - zeta_{j-1}), rb = 1/(v zeta_j - zeta_{j-1}) of the
@@ -2537,8 +2556,9 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      const int lo = 1 + k * RCH, hi = min(n2, lo + RCH);
      if (lo < hi) {
           const double x = p.x[i], f = p.f[i];
-          const cd m = cmk(p.mr, -p.mi), m2 = cmul(m, m), im1 = crecip(m), im2 = crecip(m2);
-          const cd z = cscale(m, x), iz = crecip(z);
+          // m, 1/m, 1/m^2 from the arguments: scalar operands, no registers per lane
+          const cd m = cmk(p.mr, -p.mi), im1 = cmk(p.im1r, p.im1i), im2 = cmk(p.im2r, p.im2i);
+          const cd iz = cscale(im1, 1. / x);   // 1/z, z = m x
           const double2 *Dv = (const double2 *) p.D;
           const double *U = p.PC, *Chi = p.PC + (size_t) (p.n1max + 1) * Np;
           const double sc = p.PS[i];
@@ -2576,9 +2596,13 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                cur = mie_ab_rev(lo / x, m, im1, D0, ps0, psm, ch0, chm);
           }
           const cd cbv = cmk(p.CB[i], p.CB[Np + i]);
-          const double sb0 = p.sb0 * f, sb1 = p.sb1 * f, sb2 = p.sb2 * f, sb3 = p.sb3 * f;
+          // U path: every cotangent below is linear in f (U = Sbar / f and the
+          // series parts carry f), so the sweep runs with f = 1 and the chunk's
+          // f-linear results are scaled once at the end
+          const double fw = UP ? 1. : f;
+          const double sb0 = p.sb0 * fw, sb1 = p.sb1 * fw, sb2 = p.sb2 * fw, sb3 = p.sb3 * fw;
           double sgn = (lo & 1) ? -1. : 1.;
-          const double ix2 = 1. / (x * x), ix = 1. / x;
+          const double ix = 1. / x;
           for (int j = lo; j < hi; ++j) {
                // the window's next entries
                const double2 pc2 = rpc(j + 2);
@@ -2598,8 +2622,8 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                          const cd us = cmk(u0, u1), ut = cmk(u2, u3);
                          // 1/2 Re(conj S U_S + conj T U_T), S = fv4 (a + b), T = fv4 (a - b)
                          fq += .5 * fv4 * ((a.r + b.r) * us.r + (a.i + b.i) * us.i + (a.r - b.r) * ut.r + (a.i - b.i) * ut.i);
-                         Sb = cscale(us, f);
-                         Tb = cscale(ut, f);
+                         Sb = us;
+                         Tb = ut;
                     } else {
                          Sb = cmk(ld_row(p.bar, j - 1, Np, o8), ld_row(p.bar + plane, j - 1, Np, o8));
                          Tb = cmk(ld_row(p.bar + 2 * plane, j - 1, Np, o8), ld_row(p.bar + 3 * plane, j - 1, Np, o8));
@@ -2635,7 +2659,7 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
                          // (the same for chi) it reduces to
                          // (d a / d u) (1 + (D / m)^2 - j (j + 1) / x^2); b likewise
                          // with m D.  The D_j part goes through the z cotangent.
-                         const double c0 = 1. - j * (j + 1.) * ix2;
+                         const double c0 = 1. - j * (j + 1.) * (ix * ix);
                          const cd dm = cmul(Dj, im1), md = cmul(m, Dj);
                          const cd wu = cmul(dm, dm), wv = cmul(md, md);
                          xq += (c0 + wu.r) * ub.r + wu.i * ub.i + (c0 + wv.r) * vb.r + wv.i * vb.i;
@@ -2669,6 +2693,13 @@ __device__ __forceinline__ void rev_orders_body(RevArgs p) {
      }
      // rev_carry reads the chunks below (n2 - 1 + RCH - 1) / RCH only
      if (lo >= n2) return;
+     if constexpr (UP) {
+          const double f = p.f[i];
+          C0 = cscale(C0, f);
+          Z0 = cscale(Z0, f);
+          mb = cscale(mb, f);
+          xq *= f;
+     }
      double *o = p.scan + (size_t) k * REV_ROWS * Np + i;
      o[0] = P.r; o[Np] = P.i; o[2 * Np] = C0.r; o[3 * Np] = C0.i;
      o[4 * Np] = Q.r; o[5 * Np] = Q.i; o[6 * Np] = Z0.r; o[7 * Np] = Z0.i;
@@ -4282,7 +4313,7 @@ int adjoint_impl(Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out)
      r.N = N; r.Np = Np; r.Kp = Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.nch = nch;
      r.ft = w->ftab.as<double>();
      r.x = w->x.as<double>(); r.f = w->f.as<double>();
-     r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
+     r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi; rev_recips(r);
      r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>(); r.PS = w->chPS.as<double>();
      r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = w->bar.as<double>();
      r.Uf = use_u ? w->Uf.as<float>() : nullptr; r.Kq = rup(w->n1max, 64);
@@ -4684,7 +4715,7 @@ int fused_rev_launch(std::vector<RevCase> &cs, Upload &up, Workspace *L) {
           RevArgs r;
           r.N = w->N; r.Np = Np; r.Kp = w->Kp; r.n1max = w->n1max; r.n2rows = w->n2rows; r.nch = c.nch;
           r.ft = w->ftab.as<double>(); r.x = w->x.as<double>(); r.f = w->f.as<double>();
-          r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi;
+          r.n1 = w->n1.as<int>(); r.n2 = w->n2.as<int>(); r.mr = w->mr; r.mi = w->mi; rev_recips(r);
           r.D = w->chD.as<double>(); r.PC = w->chPC.as<double>(); r.PS = w->chPS.as<double>();
           r.CB = w->CB.as<double>(); r.S = w->S.as<double>(); r.bar = nullptr;   // read only without Uf
           r.Uf = w->Uf.as<float>(); r.Kq = c.Kq; r.fang = w->fang.as<double>();
