@@ -1587,7 +1587,7 @@ __device__ __forceinline__ void angular_mix_body(const Mix2Args &p, int2 tk) {
      constexpr int NT = S::NT, TM = S::TM, TN = S::TN, G = MIX_G, MA = S::MA, NB = S::NB, ST = S::ST;
      constexpr int SA = mix_sa<S>(), SB = mix_sb<S>();
      __shared__ v4u lds[ST * (SA + SB) + (S::DT > 0 ? 16 : 0)];
-     static_assert(sizeof(lds) >= sizeof(double) * S::WM * NW * 8 * TN, "epilogue buffer");
+     static_assert(sizeof(lds) >= sizeof(double) * S::WM * 4 * 8 * TN, "epilogue buffer");
      unsigned *sink = (unsigned *) (lds + ST * (SA + SB));   // touch loads land here (never read)
      static_assert(S::WM * 16 * MA == TM && S::WN * 16 * NB == TN && S::WM * S::WN == NT / 64, "wave grid");
 
@@ -1598,6 +1598,13 @@ __device__ __forceinline__ void angular_mix_body(const Mix2Args &p, int2 tk) {
      const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
      const int wm = wave % S::WM, wn = wave / S::WM;
      const int l16 = lane & 15, lg = lane >> 4;
+     // the first weight channel of this lane's sizes, loaded now so the
+     // epilogue does not wait for it
+     double w0[MA][4];
+#pragma unroll
+     for (int a = 0; a < MA; ++a)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) w0[a][v] = p.W[0][it0 + wm * 16 * MA + 16 * a + mfma16_row(lane, v)];
      const v4u flip = {0x80000000u, 0x80000000u, 0x80000000u, 0x80000000u};
      // stage buffer b: A rows at lds + b (SA + SB), B rows after them
      auto sA = [&](int b_) { return lds + b_ * (SA + SB); };
@@ -1677,18 +1684,21 @@ __device__ __forceinline__ void angular_mix_body(const Mix2Args &p, int2 tk) {
      }
 
      // epilogue: the sums in double (leading product plus corrections), the
-     // Stokes elements, weighted sums over this wave's sizes per angle,
-     // combined over the WM size blocks of the waves through LDS
+     // Stokes elements, weighted sums over this lane's sizes per angle; the
+     // partial sums of every (wave row block, lane group) go to LDS and one
+     // thread per output adds them (no cross-lane shuffles), one weight
+     // channel at a time (the Stokes elements are recomputed per channel), so
+     // few doubles are live beside the accumulators
      const double unscale = 1. / (kSA * kSB);
-     double *sR = (double *) lds;            // [WM][NW][8][TN]
+     double *sR = (double *) lds;            // [WM][4 lane groups][8][TN]
      __syncthreads();                        // every wave is done with the stages
-     // one weight channel at a time (the Stokes elements are recomputed per
-     // channel), so few doubles are live beside the accumulators
+     double *out = p.part + (size_t) st * NW * 4 * p.Q;
 #pragma unroll
-     for (int b = 0; b < NB; ++b) {
-          const int jl = wn * 16 * NB + 16 * b + l16;
+     for (int c = 0; c < NW; ++c) {
+          if (c > 0) __syncthreads();        // the previous channel's sums are read
 #pragma unroll
-          for (int c = 0; c < NW; ++c) {
+          for (int b = 0; b < NB; ++b) {
+               const int jl = wn * 16 * NB + 16 * b + l16;
                double ew[8];
 #pragma unroll
                for (int k = 0; k < 8; ++k) ew[k] = 0.;
@@ -1705,7 +1715,7 @@ __device__ __forceinline__ void angular_mix_body(const Mix2Args &p, int2 tk) {
                          double Pp[4], Pm[4];
                          stokes(q[0], q[1], q[2], q[3], Pp);
                          stokes(q[4], q[5], q[6], q[7], Pm);
-                         const double wv = p.W[c][i];
+                         const double wv = c == 0 ? w0[a][v] : p.W[c][i];
 #pragma unroll
                          for (int k = 0; k < 4; ++k) {
                               ew[k] += wv * Pp[k];
@@ -1713,24 +1723,18 @@ __device__ __forceinline__ void angular_mix_body(const Mix2Args &p, int2 tk) {
                          }
                     }
 #pragma unroll
-               for (int k = 0; k < 8; ++k) {
-                    double t = ew[k];
-                    t += __shfl_xor(t, 16);
-                    t += __shfl_xor(t, 32);
-                    if (lg == 0) sR[((wm * NW + c) * 8 + k) * TN + jl] = t;
-               }
+               for (int k = 0; k < 8; ++k) sR[((wm * 4 + lg) * 8 + k) * TN + jl] = ew[k];
           }
-     }
-     __syncthreads();
-     double *out = p.part + (size_t) st * NW * 4 * p.Q;
-     for (int e = threadIdx.x; e < NW * 8 * TN; e += NT) {
-          const int jl = e % TN, ck = e / TN, c = ck / 8, k = ck % 8, j = jt0 + jl;
-          if (j >= p.H) continue;
-          double v = 0.;
+          __syncthreads();
+          for (int e = threadIdx.x; e < 8 * TN; e += NT) {
+               const int jl = e % TN, k = e / TN, j = jt0 + jl;
+               if (j >= p.H) continue;
+               double v = 0.;
 #pragma unroll
-          for (int m = 0; m < S::WM; ++m) v += sR[((m * NW + c) * 8 + k) * TN + jl];
-          // k < 4: direct angle H + j; k >= 4: mirrored angle H - 1 - j
-          out[((size_t) c * 4 + (k & 3)) * p.Q + (k < 4 ? p.H + j : p.H - 1 - j)] = v;
+               for (int m = 0; m < S::WM * 4; ++m) v += sR[(m * 8 + k) * TN + jl];
+               // k < 4: direct angle H + j; k >= 4: mirrored angle H - 1 - j
+               out[((size_t) c * 4 + (k & 3)) * p.Q + (k < 4 ? p.H + j : p.H - 1 - j)] = v;
+          }
      }
 }
 template <class E, int NP, int NW, class S>
