@@ -960,20 +960,44 @@ __device__ __forceinline__ void mie_terms_body(TermArgs p) {
      if (j0 <= j1) {
           const double x = p.x[i];
           const cd m = cmk(p.mr, -p.mi), im = crecip(m);
-          // the group's rows are the same in every lane: all of them are
-          // loaded at once (rows clamped to the arrays; those past a lane's
+          // the group's rows are the same in every lane: the D rows are all
+          // loaded at once (rows clamped to the array; those past a lane's
           // n1 are read, not used), so the loads overlap
           const double2 *Dv = (const double2 *) p.D;
           const double *U = p.PC, *Chi = p.PC + (size_t) (p.n1max + 1) * Np;
           const unsigned o8 = (unsigned) i * 8u, o16 = (unsigned) i * 16u;
           const int pmax = p.n1max, dmax = p.n2rows - 1;
-          const double sc = p.PS[i];
+          const double sc = p.PS[i], ax = 1. / x;
           double ps[10], ch[10], dr[9], di[9];   // rows j0 - 2 .. j0 + 7 (psi, chi), j0 - 1 .. j0 + 7 (D)
+          // U and chi of the group's rows from two seed rows each, by the
+          // chains' own steps (the same bits as the stored rows): U downward
+          // from rows j0 + 8, j0 + 7 (U_{n1} = 0, U_{n1-1} = 1 where those rows
+          // fall inside, as the chain starts), chi upward from rows j0 - 2,
+          // j0 - 1 (rows 0 and 1 for the first group).  Rows past a lane's n1
+          // are computed from unused values and never used.
+          {
+               double u[11];                  // rows j0 - 2 .. j0 + 8
+               u[10] = ld_row(U, min(j0 + 8, pmax), Np, o8);
+               u[9] = ld_row(U, min(j0 + 7, pmax), Np, o8);
 #pragma unroll
-          for (int e = 0; e < 10; ++e) {
-               const int r = min(max(j0 - 2 + e, 0), pmax);
-               ps[e] = sc * ld_row(U, r, Np, o8);
-               ch[e] = ld_row(Chi, r, Np, o8);
+               for (int e = 8; e >= 0; --e) {
+                    const int r = j0 - 2 + e, jj = r + 1;
+                    const double un = (2. * jj + 1.) * ax * u[e + 1] - u[e + 2];
+                    u[e] = r == n1 ? 0. : r == n1 - 1 ? 1. : un;
+               }
+#pragma unroll
+               for (int e = 0; e < 10; ++e) ps[e] = sc * u[e];
+               const bool first = j0 == 1;
+               ch[0] = first ? 0. : ld_row(Chi, max(j0 - 2, 0), Np, o8);
+               ch[1] = ld_row(Chi, j0 - 1, Np, o8);
+               ch[2] = first ? ld_row(Chi, 1, Np, o8) : 0.;
+#pragma unroll
+               for (int e = 2; e < 10; ++e) {
+                    const int t = j0 - 2 + e - 1;
+                    const double fv1 = 2. * t + 1.;
+                    const double c2 = fv1 * ax * ch[e - 1] - ch[e - 2];
+                    if (e > 2 || !first) ch[e] = c2;
+               }
           }
 #pragma unroll
           for (int e = 0; e < 9; ++e) {
