@@ -78,8 +78,10 @@ static void *pool_worker(void *arg) {
    and each runs its host loops alone */
 static __thread int g_batch_thread;
 
-static void par_for(int n, par_fn fn, void *ctx) {
-     if (n < 2048 || g_batch_thread || pthread_mutex_trylock(&g_pool.region) != 0) {
+/* fn over [0, n) on the pool when n >= min_n (else, on a batch thread or
+   with the pool busy, in the calling thread) */
+static void par_run(int n, par_fn fn, void *ctx, int min_n) {
+     if (n < min_n || g_batch_thread || pthread_mutex_trylock(&g_pool.region) != 0) {
           fn(ctx, 0, n);
           return;
      }
@@ -111,6 +113,12 @@ static void par_for(int n, par_fn fn, void *ctx) {
      pthread_mutex_unlock(&g_pool.lk);
      pthread_mutex_unlock(&g_pool.region);
 }
+
+static void par_for(int n, par_fn fn, void *ctx) { par_run(n, fn, ctx, 2048); }
+
+/* fixed blocks 0 .. nb-1 spread over the pool: each block's result is the
+   same whichever thread runs it */
+static void par_blocks(int nb, par_fn fn, void *ctx) { par_run(nb, fn, ctx, 2); }
 
 /* ------------------------------------------------------------------------- */
 /* errors: one message per thread; -1 returns leave a breadcrumb like the
@@ -1482,6 +1490,38 @@ int lmie_adjoint_reverse(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *gra
      return rc;
 }
 
+/* The reverse pass's sums over sizes of the mode-parameter gradients, in
+   GRAD_BLOCKS fixed blocks: part[b] = (5 per mode, r1, r2, lambda) */
+#define GRAD_BLOCKS 32
+typedef struct {
+     const lmie_adj_ctx *c;
+     const double *f_bar, *x_bar;
+     double *part;
+} grad_ctx;
+
+static void grad_block(void *vp, int b0, int b1) {
+     const grad_ctx *g = vp;
+     const lmie_adj_ctx *c = g->c;
+     const int S = ADJ_SEEDS, W = 5 * c->n_modes + 3;
+     for (int b = b0; b < b1; ++b) {
+          double *pb = g->part + (size_t) b * W;
+          const int i0 = (int) ((long long) c->N * b / GRAD_BLOCKS), i1 = (int) ((long long) c->N * (b + 1) / GRAD_BLOCKS);
+          for (int q = 0; q < W; ++q) pb[q] = 0.;
+          for (int k = 0; k < c->n_modes; ++k)
+               for (int i = i0; i < i1; ++i) {
+                    const double *df = c->df[k] + (size_t) i * S, *dx = c->dx[k] + (size_t) i * S;
+                    for (int q = 0; q < 5; ++q) pb[5 * k + q] += g->f_bar[i] * df[q] + g->x_bar[i] * dx[q];
+                    pb[W - 3] += g->f_bar[i] * df[5];
+                    pb[W - 2] += g->f_bar[i] * df[6];
+               }
+          for (int i = i0; i < i1; ++i) {
+               pb[W - 3] += g->x_bar[i] * c->dx[0][(size_t) i * S + 5];
+               pb[W - 2] += g->x_bar[i] * c->dx[0][(size_t) i * S + 6];
+               pb[W - 1] += g->x_bar[i] * c->xl[i];
+          }
+     }
+}
+
 static int adjoint_reverse_run(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_grad *grad, double *grad_modes) {
      if (!c || !adj || !grad) return fail("lmie_adjoint_reverse", "missing argument");
      int C = c->C;
@@ -1519,25 +1559,24 @@ static int adjoint_reverse_run(lmie_adj_ctx *c, const lmie_adj_in *adj, lmie_gra
      if (rc) {
           fail("lmie_gpu_adjoint", lmie_gpu_last_error());
      } else {
-          const int S = ADJ_SEEDS;
           grad->mr = aout.m_bar[0];
           grad->mi = aout.m_bar[1];
           /* a_p of mode k moves f_i through n_k (and, for grids set by the
              parameters, the nodes and weights); r1 and r2 move the shared grid
              of every mode: its weights and densities per mode, its nodes once */
           double *gp = vec((size_t) 5 * c->n_modes), gr[2] = {0., 0.}, lx = 0.;
-          for (int k = 0; k < c->n_modes; ++k)
-               for (int i = 0; i < c->N; ++i) {
-                    const double *df = c->df[k] + (size_t) i * S, *dx = c->dx[k] + (size_t) i * S;
-                    for (int q = 0; q < 5; ++q) gp[5 * k + q] += aout.f_bar[i] * df[q] + aout.x_bar[i] * dx[q];
-                    gr[0] += aout.f_bar[i] * df[5];
-                    gr[1] += aout.f_bar[i] * df[6];
-               }
-          for (int i = 0; i < c->N; ++i) {
-               gr[0] += aout.x_bar[i] * c->dx[0][(size_t) i * S + 5];
-               gr[1] += aout.x_bar[i] * c->dx[0][(size_t) i * S + 6];
-               lx += aout.x_bar[i] * c->xl[i];
+          /* per fixed block of sizes on the pool, then the blocks in order:
+             the sums do not depend on the threads */
+          grad_ctx gx = {c, aout.f_bar, aout.x_bar, vec((size_t) GRAD_BLOCKS * (5 * c->n_modes + 3))};
+          par_blocks(GRAD_BLOCKS, grad_block, &gx);
+          for (int b = 0; b < GRAD_BLOCKS; ++b) {
+               const double *pb = gx.part + (size_t) b * (5 * c->n_modes + 3);
+               for (int q = 0; q < 5 * c->n_modes; ++q) gp[q] += pb[q];
+               gr[0] += pb[5 * c->n_modes];
+               gr[1] += pb[5 * c->n_modes + 1];
+               lx += pb[5 * c->n_modes + 2];
           }
+          free(gx.part);
           /* lambda: x_i = 2 pi r_i / lambda, and cext, csca, cbak carry
              lambda^2 / (2 pi) (src/lmie_schedule.c:1158-1199) */
           grad->lambda = 2. / c->lambda * (adj->cext * c->cext + adj->csca * c->csca + adj->cbak * c->cbak) - lx;
