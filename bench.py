@@ -279,9 +279,9 @@ def main():
     kern = {"angular_mix" if mixed else "angular_f64_dma": (8.0 * nq * units, ang_ms)}
     # matrix products issued per algorithmic product: the forward contraction
     # splits the size operand in two parts and the angle operand in three and
-    # takes five of their products (three with LMIE_MIX_PRODUCTS=3); the
-    # reverse contraction takes three (two parts of each operand)
-    issued = {"angular_mix": 3.0 if os.environ.get("LMIE_MIX_PRODUCTS") == "3" else 5.0,
+    # takes five of their products (three or four with LMIE_MIX_PRODUCTS=3 or
+    # 4); the reverse contraction takes three (two parts of each operand)
+    issued = {"angular_mix": {"3": 3.0, "4": 4.0}.get(os.environ.get("LMIE_MIX_PRODUCTS", ""), 5.0),
               "adj_ugemm": 3.0, "angular_f64_dma": 1.0}
     if mixed and a.path.endswith("-adjoint") and stages.get("adj_gemm", 0) > 0:
         kern["adj_ugemm"] = (24.0 * float(np.sum(n1s * n1s)), stages["adj_gemm"])

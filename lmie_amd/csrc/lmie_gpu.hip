@@ -1598,9 +1598,11 @@ __device__ __forceinline__ void mix_products(const v4u *x0, const v4u *b0, const
      c[2] = E::mfma(x2[0], b2[1], c[2]); c[3] = E::mfma(x3[0], b3[1], c[3]);
      c[0] = E::mfma(x0[1], b0[0], c[0]); c[1] = E::mfma(x1[1], b1[0], c[1]);
      c[2] = E::mfma(x2[1], b2[0], c[2]); c[3] = E::mfma(x3[1], b3[0], c[3]);
-     if constexpr (NP == 5) {
+     if constexpr (NP >= 4) {
           c[0] = E::mfma(x0[0], b0[2], c[0]); c[1] = E::mfma(x1[0], b1[2], c[1]);
           c[2] = E::mfma(x2[0], b2[2], c[2]); c[3] = E::mfma(x3[0], b3[2], c[3]);
+     }
+     if constexpr (NP == 5) {
           c[0] = E::mfma(x0[1], b0[1], c[0]); c[1] = E::mfma(x1[1], b1[1], c[1]);
           c[2] = E::mfma(x2[1], b2[1], c[2]); c[3] = E::mfma(x3[1], b3[1], c[3]);
      }
@@ -3606,10 +3608,12 @@ void xcd_tasks(const std::vector<int> &kt, F ncol, int group, std::vector<int2> 
 }
 
 // Split products of the forward mixed contraction (environment
-// LMIE_MIX_PRODUCTS, read at every call): 5 (default) or 3.
+// LMIE_MIX_PRODUCTS, read at every call): 5 (default), 4 (without A_lo B_mid;
+// shape 3 only, else 5) or 3.
 int mix_products() {
      const char *e = getenv("LMIE_MIX_PRODUCTS");
-     return (e && atoi(e) == 3) ? 3 : 5;
+     const int r = e ? atoi(e) : 5;
+     return (r == 3 || r == 4) ? r : 5;
 }
 
 // Shape of the forward mixed contraction (environment LMIE_MIX_SHAPE, read at
@@ -3986,7 +3990,9 @@ int forward_impl(const lmie_gpu_job *J, lmie_gpu_result *R, Workspace *w) {
                } while (0)
 #define LMIE_MIX_NP(E, S)                                                                                \
                do {                                                                                      \
-                    if (mix_products() == 3) LMIE_MIX_NW(E, 3, S); else LMIE_MIX_NW(E, 5, S);            \
+                    if (mix_products() == 3) LMIE_MIX_NW(E, 3, S);                                       \
+                    else if (mix_products() == 4 && std::is_same<S, MixS3>::value) LMIE_MIX_NW(E, 4, MixS3); \
+                    else LMIE_MIX_NW(E, 5, S);                                                          \
                } while (0)
 #define LMIE_MIX_S(S) do { if (bf16) LMIE_MIX_NP(BF16, S); else LMIE_MIX_NP(F16, S); } while (0)
                if (ms.id == 0) LMIE_MIX_S(MixS0);
@@ -4610,7 +4616,7 @@ int fused_launch(std::vector<FwdCase> &cs, Upload &up, Workspace *L) {
 #define LMIE_MIXB(NP, S) hipLaunchKernelGGL((angular_mix_b<E, NP, 1, S>), g, dim3(S::NT), 0, s, LMIE_B(Mix2Args, o_mix), LMIE_B(int2, o_task))
           if (id == 0) { if (np == 3) LMIE_MIXB(3, MixS0); else LMIE_MIXB(5, MixS0); }
           else if (id == 2) { if (np == 3) LMIE_MIXB(3, MixS2); else LMIE_MIXB(5, MixS2); }
-          else if (id == 3) { if (np == 3) LMIE_MIXB(3, MixS3); else LMIE_MIXB(5, MixS3); }
+          else if (id == 3) { if (np == 3) LMIE_MIXB(3, MixS3); else if (np == 4) LMIE_MIXB(4, MixS3); else LMIE_MIXB(5, MixS3); }
           else { if (np == 3) LMIE_MIXB(3, MixS1); else LMIE_MIXB(5, MixS1); }
 #undef LMIE_MIXB
      }
