@@ -451,24 +451,37 @@ __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const doubl
                }
           }
           const int ne = min(64, C - l0);
-          for (int e = 0; e < ne; ++e) {
-               const int l = l0 + e;
-               double v;
-               if (fn == 4) {
-                    if (l == 0) v = 1.;
-                    else if (l == 1) v = m;
-                    else {
-                         const double a = readlane_d(yv, e), bb = m * b0;
-                         v = a * (bb - a0) + bb;
-                    }
-               } else if (l < 3) {
-                    if (l == 0) v = fn == 0 ? 1. : 0.;
+          auto put = [&](int l, double v) {
+               P[(size_t) l * Hp + j] = live ? v : 0.;
+               a0 = b0;
+               b0 = v;
+          };
+          // l = 0, 1, 2 by their closed forms, then the recurrence steps (no
+          // per-step test of l)
+          int e0 = 0;
+          if (l0 == 0) {
+               for (int l = 0; l < 3 && l < C; ++l) {
+                    double v;
+                    if (fn == 4) v = l == 0 ? 1. : l == 1 ? m : readlane_d(yv, l) * (m * b0 - a0) + m * b0;
+                    else if (l == 0) v = fn == 0 ? 1. : 0.;
                     else if (l == 1) v = fn == 0 ? m : 0.;
                     else if (fn == 0) v = (3. * m2 - 1.) / 2.;
                     else if (fn == 1) v = .25 * sqrt(6.) * (m2 - 1.);
                     else if (fn == 2) v = .25 * (1. + m) * (1. + m);
                     else v = .25 * (1. - m) * (1. - m);
-               } else {
+                    put(l, v);
+               }
+               e0 = 3;
+          }
+          if (fn == 4) {
+               // P_l = a (m P_{l-1} - P_{l-2}) + m P_{l-1}, a = (l-1)/l
+               for (int e = e0; e < ne; ++e) {
+                    const double a = readlane_d(yv, e), bb = m * b0;
+                    put(l0 + e, a * (bb - a0) + bb);
+               }
+          } else {
+               for (int e = e0; e < ne; ++e) {
+                    const int l = l0 + e;
                     const double L = l - 1, L1 = l, t = 2. * L + 1.;
                     const double d = readlane_d(dv, e), y = readlane_d(yv, e);
                     double x;
@@ -484,11 +497,8 @@ __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const doubl
                          x = fn == 2 ? (dc - k4) * b0 - kb * a0 : (dc + k4) * b0 - kb * a0;
                     }
                     const double q = x * y;
-                    v = __builtin_fma(__builtin_fma(-q, d, x), y, q);
+                    put(l, __builtin_fma(__builtin_fma(-q, d, x), y, q));
                }
-               P[(size_t) l * Hp + j] = live ? v : 0.;
-               a0 = b0;
-               b0 = v;
           }
      }
 }
