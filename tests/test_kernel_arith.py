@@ -1,8 +1,10 @@
 """CPU checks of the arithmetic identities the gfx950 kernels rely on
 (tests/c/kernel_arith.c): the Markstein-corrected quotients of the expansion
-tables equal the IEEE division bit for bit, and the per-size psi recurrence in
+tables equal the IEEE division bit for bit; the per-size psi recurrence in
 Miller's normalised form equals the reference's ratio chain and product
-(src/lmie_core.c:298-330) to rounding."""
+(src/lmie_core.c:298-330) to rounding; the reciprocal refinement of the D
+chain is within an ulp; the D chain's w form equals the reference's
+(src/lmie_core.c:386-405) to rounding."""
 import os
 import subprocess
 
