@@ -44,8 +44,9 @@ def test_gauss_rule_is_bitwise_the_reference_rule(product, oracle, n):
     xo, wo = np.array(xo), np.array(wo)
     if not (np.array_equal(x, xo) and np.array_equal(w, wo)):
         import os
-        os.makedirs("gpurun_out", exist_ok=True)
-        np.savez("gpurun_out/gauss_rule_%d.npz" % n, x=x, w=w, xo=xo, wo=wo)
+        out = os.environ.get("LMIE_TEST_OUT", "test_out")
+        os.makedirs(out, exist_ok=True)
+        np.savez(os.path.join(out, "gauss_rule_%d.npz" % n), x=x, w=w, xo=xo, wo=wo)
         say("  nodes differ at %d, weights at %d; max |dx| %.3e, max |dw|/w %.3e" % (
             np.sum(x != xo), np.sum(w != wo), np.abs(x - xo).max(), (np.abs(w - wo) / wo).max()))
     assert np.array_equal(x, xo)

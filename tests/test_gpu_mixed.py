@@ -137,8 +137,9 @@ def test_mixed_precision_error(product, case):
         table[prec] = errors(fn(), ref)
         say("  %-16s " % prec + " ".join("%s %.1e/%.1e" % (k, v["normwise"], v["elem_max"])
                                          for k, v in table[prec].items() if isinstance(v, dict)))
-    os.makedirs("gpurun_out", exist_ok=True)
-    with open("gpurun_out/mixed_error_%s.json" % case, "w") as fh:
+    out = os.environ.get("LMIE_TEST_OUT", "test_out")   # diagnostic tables
+    os.makedirs(out, exist_ok=True)
+    with open(os.path.join(out, "mixed_error_%s.json" % case), "w") as fh:
         json.dump(table, fh, indent=1)
     e = table["mixed"]
     for q in ("cext", "csca", "cbak", "g", "omega"):
