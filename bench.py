@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--workload", default="two-mode", choices=["two-mode", "retrieval-batch"])
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the size points "
+        print("bench.py measures one GPU: the library has no multi-GPU path (the size points "
               "would shard across GPUs with one all-reduce of the raw sums); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
