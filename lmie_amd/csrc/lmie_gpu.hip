@@ -2194,12 +2194,13 @@ __global__ __launch_bounds__(64) void adj_tables_b(const adj_tables_B *cases) {
 // Im Gst (row n, column m, scaled by kSB^2 / (wmax n (n+1) m (m+1))), and the
 // largest magnitude.
 template <class E>
-__device__ __forceinline__ void adj_gbuild_body(const unsigned short *L16, const unsigned short *R16, int Hq, int Kq, float *Gf, AdjScal *sc) {
+__device__ __forceinline__ void adj_gbuild_body(const unsigned short *L16, const unsigned short *R16, int Hq, int Kq, float *Gf, AdjScal *sc,
+                                                int tn, int tm) {
      constexpr int T = 64, G = 2, KS = 8 * G;
      constexpr int NL = 4 * G * T / 256, NR = 16 * G * T / 256;
      __shared__ v4u sL[2][4 * G][T];
      __shared__ v4u sR[2][16 * G][T];
-     const int n0 = blockIdx.x * T, m0 = blockIdx.y * T, nch = Hq / KS;
+     const int n0 = tn * T, m0 = tm * T, nch = Hq / KS;
      const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
      const int wa = wave & 1, wb = wave >> 1, l32 = lane & 31, lh = lane >> 5;
      const size_t plane = (size_t) (Hq / 8) * Kq * 8;
@@ -2272,8 +2273,22 @@ __device__ __forceinline__ void adj_gbuild_body(const unsigned short *L16, const
      for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
      if (lane == 0) atomicMax(&sc->gmax, __float_as_uint(mx));
 }
+// Tile (row tile, column tile) of workgroup id L of a g x g tile grid.  The
+// hardware deals workgroup ids round robin over the 8 XCDs, so XCD x runs ids
+// x, x + 8, ...; those take the consecutive tiles start_x, start_x + 1, ... of
+// the column-major order, so the workgroups in flight on one XCD share one or
+// two right-hand column tiles (4 MB each at Kq 1920) and stream the left-hand
+// rows together, from that XCD's L2.
+__device__ __forceinline__ int2 xcd_tile(int L, int g) {
+     const int nt = g * g, x = L & 7, k = L >> 3, q = nt >> 3, r = nt & 7;
+     const int t = x * q + min(x, r) + k;
+     return make_int2(t % g, t / g);
+}
 template <class E>
-__global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, const unsigned short *R16, int Hq, int Kq, float *Gf, AdjScal *sc) { adj_gbuild_body<E>(L16, R16, Hq, Kq, Gf, sc); }
+__global__ __launch_bounds__(256) void adj_gbuild(const unsigned short *L16, const unsigned short *R16, int Hq, int Kq, float *Gf, AdjScal *sc) {
+     const int2 t = xcd_tile(blockIdx.x, Kq / 64);
+     adj_gbuild_body<E>(L16, R16, Hq, Kq, Gf, sc, t.x, t.y);
+}
 struct adj_gbuild_B {
      const unsigned short *L16;
      const unsigned short *R16;
@@ -2287,7 +2302,7 @@ template <class E>
 __global__ __launch_bounds__(256) void adj_gbuild_b(const adj_gbuild_B *cases) {
      const adj_gbuild_B a = cases[blockIdx.z];
      if (blockIdx.x >= a.gx || blockIdx.y >= a.gy) return;
-     adj_gbuild_body<E>(a.L16, a.R16, a.Hq, a.Kq, a.Gf, a.sc);
+     adj_gbuild_body<E>(a.L16, a.R16, a.Hq, a.Kq, a.Gf, a.sc, blockIdx.x, blockIdx.y);
 }
 
 
@@ -4270,7 +4285,7 @@ int adjoint_mixed_angular(Workspace *w, hipStream_t s, Timer &tm) {
                              (const unsigned short *) w->B16.as<unsigned short>(), w->Kb, w->Hb,                      \
                              (const double *) w->wts.as<double>(), H, Hq, Kq, n1max, w->L16.as<unsigned short>(),    \
                              w->R16.as<unsigned short>());                                                            \
-          hipLaunchKernelGGL(adj_gbuild<E>, dim3(Kq / 64, Kq / 64), dim3(256), 0, s,                            \
+          hipLaunchKernelGGL(adj_gbuild<E>, dim3((Kq / 64) * (Kq / 64)), dim3(256), 0, s,                       \
                              (const unsigned short *) w->L16.as<unsigned short>(),                             \
                              (const unsigned short *) w->R16.as<unsigned short>(), Hq, Kq, w->Gf.as<float>(), sc); \
           hipLaunchKernelGGL(adj_gsplit<E>, dim3(Kq / 64, Kq / 8), dim3(64), 0, s, (const float *) w->Gf.as<float>(), \
