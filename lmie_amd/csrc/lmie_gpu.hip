@@ -420,13 +420,12 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
 // parallel) and each step reads its l's from that lane as scalars, so the
 // dependence chain of a step is the numerator and three operations instead
 // of a full division.
-__device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const double *mu, double *T) {
+// FN is the recurrence (a template argument, so a step has no branch on it).
+template <int FN>
+__device__ __forceinline__ void gsf_plane(int H, int Hp, int C, const double *mu, double *P, int j, int lane) {
      // separately rounded operations in the reference's order (bitwise the
      // reference's functions)
 #pragma clang fp contract(off)
-     const int j = blockIdx.x * blockDim.x + threadIdx.x, fn = blockIdx.y, lane = threadIdx.x & 63;
-     if (j >= Hp) return;
-     double *P = T + (size_t) fn * C * Hp;
      // every lane stays in the loops (its l's are read by the others); the
      // columns past H run with mu = 0 and store zeros
      const bool live = j < H;
@@ -434,16 +433,16 @@ __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const doubl
      double a0 = 0., b0 = 0.;
      for (int l0 = 0; l0 < C; l0 += 64) {
           // this lane's order l0 + lane: divisor, its reciprocal, square root
-          // (fn 1), Legendre factor (fn 4)
+          // (FN 1), Legendre factor (FN 4)
           double dv = 1., yv = 1., sav = 0.;
           {
                const int lc = l0 + lane;
                const double L = lc - 1, L1 = lc;
-               if (fn == 4) {
+               if constexpr (FN == 4) {
                     yv = lc >= 2 ? (lc - 1) / (double) lc : 0.;
                } else if (lc >= 3) {
-                    if (fn == 0) dv = L1;
-                    else if (fn == 1) {
+                    if constexpr (FN == 0) dv = L1;
+                    else if constexpr (FN == 1) {
                          sav = sqrt(L * L - 4.);
                          dv = sqrt(L1 * L1 - 4.);
                     } else dv = L * (L1 * L1 - 4);
@@ -451,8 +450,9 @@ __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const doubl
                }
           }
           const int ne = min(64, C - l0);
-          auto put = [&](int l, double v) {
-               P[(size_t) l * Hp + j] = live ? v : 0.;
+          double *row = P + (size_t) l0 * Hp + j;
+          auto put = [&](int e, double v) {
+               row[(size_t) e * Hp] = live ? v : 0.;
                a0 = b0;
                b0 = v;
           };
@@ -462,22 +462,22 @@ __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const doubl
           if (l0 == 0) {
                for (int l = 0; l < 3 && l < C; ++l) {
                     double v;
-                    if (fn == 4) v = l == 0 ? 1. : l == 1 ? m : readlane_d(yv, l) * (m * b0 - a0) + m * b0;
-                    else if (l == 0) v = fn == 0 ? 1. : 0.;
-                    else if (l == 1) v = fn == 0 ? m : 0.;
-                    else if (fn == 0) v = (3. * m2 - 1.) / 2.;
-                    else if (fn == 1) v = .25 * sqrt(6.) * (m2 - 1.);
-                    else if (fn == 2) v = .25 * (1. + m) * (1. + m);
+                    if constexpr (FN == 4) v = l == 0 ? 1. : l == 1 ? m : readlane_d(yv, l) * (m * b0 - a0) + m * b0;
+                    else if (l == 0) v = FN == 0 ? 1. : 0.;
+                    else if (l == 1) v = FN == 0 ? m : 0.;
+                    else if constexpr (FN == 0) v = (3. * m2 - 1.) / 2.;
+                    else if constexpr (FN == 1) v = .25 * sqrt(6.) * (m2 - 1.);
+                    else if constexpr (FN == 2) v = .25 * (1. + m) * (1. + m);
                     else v = .25 * (1. - m) * (1. - m);
                     put(l, v);
                }
                e0 = 3;
           }
-          if (fn == 4) {
+          if constexpr (FN == 4) {
                // P_l = a (m P_{l-1} - P_{l-2}) + m P_{l-1}, a = (l-1)/l
                for (int e = e0; e < ne; ++e) {
                     const double a = readlane_d(yv, e), bb = m * b0;
-                    put(l0 + e, a * (bb - a0) + bb);
+                    put(e, a * (bb - a0) + bb);
                }
           } else {
                for (int e = e0; e < ne; ++e) {
@@ -485,21 +485,33 @@ __device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const doubl
                     const double L = l - 1, L1 = l, t = 2. * L + 1.;
                     const double d = readlane_d(dv, e), y = readlane_d(yv, e);
                     double x;
-                    if (fn == 0) {
+                    if constexpr (FN == 0) {
                          const double da = t * m;
                          x = da * b0 - L * a0;
-                    } else if (fn == 1) {
+                    } else if constexpr (FN == 1) {
                          const double da = t * m;
                          x = da * b0 - readlane_d(sav, e) * a0;
                     } else {
                          const double k4 = t * 4, kb = L1 * (L * L - 4);
                          const double dc = t * (L * L1) * m;
-                         x = fn == 2 ? (dc - k4) * b0 - kb * a0 : (dc + k4) * b0 - kb * a0;
+                         x = FN == 2 ? (dc - k4) * b0 - kb * a0 : (dc + k4) * b0 - kb * a0;
                     }
                     const double q = x * y;
-                    put(l, __builtin_fma(__builtin_fma(-q, d, x), y, q));
+                    put(e, __builtin_fma(__builtin_fma(-q, d, x), y, q));
                }
           }
+     }
+}
+__device__ __forceinline__ void gsf_table_body(int H, int Hp, int C, const double *mu, double *T) {
+     const int j = blockIdx.x * blockDim.x + threadIdx.x, fn = blockIdx.y, lane = threadIdx.x & 63;
+     if (j >= Hp) return;
+     double *P = T + (size_t) fn * C * Hp;
+     switch (fn) {
+     case 0: gsf_plane<0>(H, Hp, C, mu, P, j, lane); break;
+     case 1: gsf_plane<1>(H, Hp, C, mu, P, j, lane); break;
+     case 2: gsf_plane<2>(H, Hp, C, mu, P, j, lane); break;
+     case 3: gsf_plane<3>(H, Hp, C, mu, P, j, lane); break;
+     default: gsf_plane<4>(H, Hp, C, mu, P, j, lane); break;
      }
 }
 __global__ void gsf_table(int H, int Hp, int C, const double *mu, double *T) { gsf_table_body(H, Hp, C, mu, T); }
