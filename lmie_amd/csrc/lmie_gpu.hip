@@ -3144,20 +3144,32 @@ struct Upload {
           items.resize(n_items);
           h.resize(n_bytes);
      }
-     int flush(Buf &dev, PinBuf &pin, hipStream_t s) {
-          if (items.empty()) return 0;
-          if (items.size() > 65535) return set_err("batch of %zu uploads (at most 65535)", items.size());
-          const size_t data = (h.size() + 255) & ~(size_t) 255, total = data + sizeof(CopyJob) * items.size();
+     int flush(Buf &dev, PinBuf &pin, hipStream_t s) { return flush_all({this}, dev, pin, s); }
+     // the staged uploads of several callers as one copy and one copy_jobs
+     // launch, each one's bytes going straight into the pinned buffer (which
+     // must not be refilled before the copy has run)
+     static int flush_all(const std::vector<const Upload *> &ups, Buf &dev, PinBuf &pin, hipStream_t s) {
+          size_t bytes = 0, n = 0;
+          for (const Upload *u : ups) {
+               bytes += u->h.size();
+               n += u->items.size();
+          }
+          if (n == 0) return 0;
+          if (n > 65535) return set_err("batch of %zu uploads (at most 65535)", n);
+          const size_t data = (bytes + 255) & ~(size_t) 255, total = data + sizeof(CopyJob) * n;
           if (dev.get(total) || pin.get(total)) return -1;
           unsigned char *base = dev.as<unsigned char>(), *hp = (unsigned char *) pin.p;
-          memcpy(hp, h.data(), h.size());
           CopyJob *jobs = (CopyJob *) (hp + data);
-          for (size_t k = 0; k < items.size(); ++k)
-               jobs[k] = CopyJob{items[k].zero ? nullptr : (const double *) (base + items[k].off),
-                                 (double *) items[k].dst, items[k].n8};
+          size_t off = 0, k = 0;
+          for (const Upload *u : ups) {
+               memcpy(hp + off, u->h.data(), u->h.size());
+               for (const Item &it : u->items)
+                    jobs[k++] = CopyJob{it.zero ? nullptr : (const double *) (base + off + it.off), (double *) it.dst,
+                                        it.n8};
+               off += u->h.size();
+          }
           HIPCHECK(hipMemcpyAsync(base, hp, total, hipMemcpyHostToDevice, s));
-          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) items.size()), dim3(256), 0, s,
-                             (const CopyJob *) (base + data));
+          hipLaunchKernelGGL(copy_jobs, dim3(32, (unsigned) n), dim3(256), 0, s, (const CopyJob *) (base + data));
           HIPCHECK(hipGetLastError());
           return 0;
      }
@@ -4914,7 +4926,10 @@ void batch_prepare(Batch *b, int k, const lmie_gpu_job *J, lmie_gpu_result *res)
 }
 
 // A reverse case of a fused batch prepared on its own thread, before it
-// arrives (fused_rev_prepare), its uploads sent on its own stream.
+// arrives (fused_rev_prepare).  Its uploads stay staged on the host: the pass
+// sends those of all cases as one copy (a few tens of KB per case; a copy, a
+// scatter launch and an event per case from every case's thread contend for
+// the runtime).
 void batch_rev_prepare(Batch *b, int k, Workspace *w, const lmie_gpu_adj_in *in, lmie_gpu_adj_out *out) {
      g_err[0] = 0;
      RevCase &c = b->rpre[k];
@@ -4924,9 +4939,6 @@ void batch_rev_prepare(Batch *b, int k, Workspace *w, const lmie_gpu_adj_in *in,
      up.h.clear();
      int rc = use_device(w->device);
      if (rc == 0) rc = fused_rev_prepare(c, up);
-     if (rc == 0) rc = up.flush(w->finDev, w->finPin, w->stream);
-     if (rc == 0 && hipEventRecord(w->uploaded, w->stream) != hipSuccess) rc = set_err("hipEventRecord failed");
-     if (rc) (void) hipStreamSynchronize(w->stream);
      b->rprerc[k] = rc;
      b->rpreerr[k] = rc ? g_err : "";
      b->rprepped[k] = 1;
@@ -5012,6 +5024,7 @@ void batch_run_forward(Batch *b) {
 void batch_run_reverse(Batch *b) {
      std::vector<RevCase> f16, b16;
      Upload up16, upb;
+     std::vector<const Upload *> staged;
      Workspace *L = nullptr;
      for (int k = 0; k < b->n; ++k) {
           if (!b->in[1][k]) continue;
@@ -5023,15 +5036,10 @@ void batch_run_reverse(Batch *b) {
                b->err[1][k] = b->rpreerr[k];
                continue;
           } else if (b->rprepped[k] && w->device == b->device) {
-               // its uploads are on the way on its own stream (batch_rev_prepare)
+               // its uploads are staged (batch_rev_prepare)
                RevCase &c = b->rpre[k];
                if (!L) L = w;
-               if (hipStreamWaitEvent(L->stream, w->uploaded, 0) != hipSuccess) {
-                    (void) hipStreamSynchronize(w->stream);
-                    b->rc[1][k] = set_err("hipStreamWaitEvent failed");
-                    b->err[1][k] = g_err;
-                    continue;
-               }
+               staged.push_back(&b->rpreup[k]);
                (w->bf16 ? b16 : f16).push_back(std::move(c));
                continue;
           } else if (!fused_reverse_ok(w, b->ain[k]) || w->device != b->device) {
@@ -5052,11 +5060,24 @@ void batch_run_reverse(Batch *b) {
           b->rc[1][k] = rc;
           b->err[1][k] = rc ? g_err : "";
      }
+     // every case's uploads in one copy on the launching stream, before
+     // either arithmetic's launches
+     int up_rc = 0;
+     if (L) {
+          g_err[0] = 0;
+          staged.push_back(&up16);
+          staged.push_back(&upb);
+          up_rc = Upload::flush_all(staged, L->bup, L->pup, L->stream);
+          up16.items.clear(); up16.h.clear();
+          upb.items.clear(); upb.h.clear();
+     }
+     const std::string up_err = up_rc ? g_err : "";
      for (int t = 0; t < 2; ++t) {
           std::vector<RevCase> &v = t ? b16 : f16;
           if (v.empty()) continue;
           g_err[0] = 0;
-          const int rc = t ? fused_rev_launch<BF16>(v, upb, L) : fused_rev_launch<F16>(v, up16, L);
+          if (up_rc) snprintf(g_err, sizeof g_err, "%s", up_err.c_str());
+          const int rc = up_rc ? up_rc : t ? fused_rev_launch<BF16>(v, upb, L) : fused_rev_launch<F16>(v, up16, L);
           if (rc) (void) hipStreamSynchronize(L->stream);
           for (RevCase &c : v) {
                b->rc[1][c.slot] = rc;

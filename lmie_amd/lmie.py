@@ -241,9 +241,14 @@ def adjoint_batch(lib, cases, targets, weights, precision="mixed", device=0, n_w
     pout = (C.POINTER(LmieOut) * n)(*[C.pointer(o) for o in outs])
     mc = [lib.lmie_calc_max_coef(x.lambda_, x.dist_type, x.a1, x.a2, x.a3, x.r1, x.r2) for x in ins]
     keep, l2 = [], []
+
+    def full(a, m):
+        # (6, m) contiguous doubles; arrays already of that form are used as they are
+        a = np.asarray(a, float)
+        return np.ascontiguousarray(a if a.shape == (6, m) else np.broadcast_to(a, (6, m)))
+
     for c in range(n):
-        t = np.ascontiguousarray(np.broadcast_to(np.asarray(targets[c], float), (6, mc[c])))
-        w = np.ascontiguousarray(np.broadcast_to(np.asarray(weights[c], float), (6, mc[c])))
+        t, w = full(targets[c], mc[c]), full(weights[c], mc[c])
         b = np.zeros((6, mc[c]))
         keep += [t, w, b]
         dp = C.POINTER(C.c_double)
@@ -264,8 +269,11 @@ def adjoint_batch(lib, cases, targets, weights, precision="mixed", device=0, n_w
             if status[c] != 0:
                 res.append(None)
                 continue
-            fake = LmieIn.from_buffer_copy(ins[c])
-            fake.n_derivs = 0
+            # the outputs carry no derivatives (read_out reads n_derivs from the input)
+            fake = ins[c]
+            if fake.n_derivs:
+                fake = LmieIn.from_buffer_copy(ins[c])
+                fake.n_derivs = 0
             out = abi.read_out(lib, cases[c], fake, outs[c], mc[c])
             g = grads[c]
             grad = {"lambda": g.lambda_, "mr": g.mr, "mi": g.mi, "a1": g.a1, "a2": g.a2,
