@@ -5438,6 +5438,25 @@ int lmie_gpu_gauss_rule(int n, double *x, double *wt, int device) {
      return 0;
 }
 
+int lmie_gpu_gsf_table(int n_mu, const double *mu, int n_l, double *out, int device) {
+     if (n_mu < 1 || n_l < 1) return set_err("n_mu < 1 or n_l < 1");
+     if (!mu || !out) return set_err("missing argument");
+     HIPCHECK(hipSetDevice(device));
+     // gsf_table reads the angles at mu[H + j], j < H, and writes [5][n_l][Hp]
+     const int H = n_mu, Hp = rup(H, 64);
+     const size_t tab = (size_t) 5 * n_l * Hp;
+     double *d = nullptr;
+     HIPCHECK(hipMalloc(&d, sizeof(double) * (2 * (size_t) H + tab)));
+     HIPCHECK(hipMemcpy(d + H, mu, sizeof(double) * H, hipMemcpyHostToDevice));
+     hipLaunchKernelGGL(gsf_table, dim3(Hp / 64, 5), dim3(64), 0, 0, H, Hp, n_l, (const double *) d, d + 2 * H);
+     HIPCHECK(hipGetLastError());
+     std::vector<double> t(tab);
+     HIPCHECK(hipMemcpy(t.data(), d + 2 * H, sizeof(double) * tab, hipMemcpyDeviceToHost));
+     HIPCHECK(hipFree(d));
+     for (size_t r = 0; r < (size_t) 5 * n_l; ++r) memcpy(out + r * H, t.data() + r * Hp, sizeof(double) * H);
+     return 0;
+}
+
 int lmie_gpu_selftest_mfma16(int bf16, const float *A, const float *B, float *D, int device) {
      HIPCHECK(hipSetDevice(device));
      float *d = nullptr;

@@ -53,6 +53,43 @@ def test_gauss_rule_is_bitwise_the_reference_rule(product, oracle, n):
     assert np.array_equal(w, wo)
 
 
+def test_gsf_tables_are_bitwise_the_oracle_functions(product, oracle):
+    """The expansion tables of the angle kernels (gsf_table: each division a
+    corrected product) equal the oracle's generalized spherical functions,
+    computed with plain IEEE divisions in the reference's order, bit for bit,
+    and P_l its Legendre recurrence; at the nodes of a 3810-point rule, at
+    mu = 0 and +-1 and at random angles, to l = 3809."""
+    n_l = 3810
+    rng = np.random.Generator(np.random.PCG64(31))
+    x, w = np.zeros(n_l), np.zeros(n_l)
+    g = product.lmie_gpu_gauss_rule
+    g.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    assert g(n_l, x.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)), 0) == 0
+    mu = np.concatenate([x[n_l // 2:][::37], [0.0, 1.0, -1.0], rng.uniform(-1.0, 1.0, 40)])
+    say("case: gsf tables at %d angles, l < %d" % (mu.size, n_l))
+    out = np.zeros((5, n_l, mu.size))
+    f = product.lmie_gpu_gsf_table
+    f.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.c_int]
+    m_c = np.ascontiguousarray(mu)
+    assert f(mu.size, m_c.ctypes.data_as(C.POINTER(C.c_double)), n_l, out.ctypes.data_as(C.POINTER(C.c_double)), 0) == 0
+    bad = []
+    for k, m in enumerate(mu):
+        P = [(C.c_double * n_l)() for _ in range(4)]
+        oracle.oracle_gsf(n_l, float(m), *P)
+        for q in range(4):
+            if not np.array_equal(out[q, :, k], np.array(P[q])):
+                bad.append((q, float(m)))
+        # P_{l+1} = l / (l + 1) (mu P_l - P_{l-1}) + mu P_l
+        leg = np.zeros(n_l)
+        leg[0], leg[1] = 1.0, m
+        for l in range(1, n_l - 1):
+            u = m * leg[l]
+            leg[l + 1] = l / (l + 1.0) * (u - leg[l - 1]) + u
+        if not np.array_equal(out[4, :, k], leg):
+            bad.append((4, float(m)))
+    assert not bad, bad[:8]
+
+
 @pytest.mark.parametrize("name", SINGLE)
 def test_fp64_matches_reference(product, name):
     say("case: fp64 parity", name)
