@@ -153,9 +153,13 @@ void lmie_gpu_host_times_clear(void);
      lmie_gpu_gsf_table         the expansion tables of the angle kernels at n_mu
                                 angles mu (any values in [-1, 1]), l = 0..n_l-1:
                                 out[5][n_l][n_mu] = P^00, P^02, P^22, P^2-2, P_l
-                                (misc/rtutil_math.c:867-1017, 506-513) */
+                                (misc/rtutil_math.c:867-1017, 506-513)
+     lmie_gpu_pitau_table       the FP64 angle tables at n_mu angles mu, orders
+                                1..n: bp[n][n_mu] = pi + tau, bm = pi - tau
+                                (src/lmie_core.c:669-691) */
 int lmie_gpu_gauss_rule(int n, double *x, double *wt, int device);
 int lmie_gpu_gsf_table(int n_mu, const double *mu, int n_l, double *out, int device);
+int lmie_gpu_pitau_table(int n_mu, const double *mu, int n, double *bp, double *bm, int device);
 int lmie_gpu_selftest_mfma16(int bf16, const float *A, const float *B, float *D, int device);
 int lmie_gpu_selftest_mfma32(const float *A, const float *B, float *D, int device);
 int lmie_gpu_selftest_mfma_f64(const double *A, const double *B, double *D, int device);

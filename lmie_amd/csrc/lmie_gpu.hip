@@ -5457,6 +5457,29 @@ int lmie_gpu_gsf_table(int n_mu, const double *mu, int n_l, double *out, int dev
      return 0;
 }
 
+int lmie_gpu_pitau_table(int n_mu, const double *mu, int n, double *bp, double *bm, int device) {
+     if (n_mu < 1 || n < 1) return set_err("n_mu < 1 or n < 1");
+     if (!mu || !bp || !bm) return set_err("missing argument");
+     HIPCHECK(hipSetDevice(device));
+     // pitau_table reads the angles at mu[H + j], j < H, and writes [n][Hp] twice
+     const int H = n_mu, Hp = rup(H, 64);
+     const size_t tab = (size_t) n * Hp;
+     double *d = nullptr;
+     HIPCHECK(hipMalloc(&d, sizeof(double) * (2 * (size_t) H + 2 * tab)));
+     HIPCHECK(hipMemcpy(d + H, mu, sizeof(double) * H, hipMemcpyHostToDevice));
+     hipLaunchKernelGGL(pitau_table, dim3(Hp / 64), dim3(64), 0, 0, H, Hp, n, n, (const double *) d, d + 2 * H,
+                        d + 2 * H + tab);
+     HIPCHECK(hipGetLastError());
+     std::vector<double> t(2 * tab);
+     HIPCHECK(hipMemcpy(t.data(), d + 2 * H, sizeof(double) * 2 * tab, hipMemcpyDeviceToHost));
+     HIPCHECK(hipFree(d));
+     for (size_t r = 0; r < (size_t) n; ++r) {
+          memcpy(bp + r * H, t.data() + r * Hp, sizeof(double) * H);
+          memcpy(bm + r * H, t.data() + tab + r * Hp, sizeof(double) * H);
+     }
+     return 0;
+}
+
 int lmie_gpu_selftest_mfma16(int bf16, const float *A, const float *B, float *D, int device) {
      HIPCHECK(hipSetDevice(device));
      float *d = nullptr;

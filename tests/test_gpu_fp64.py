@@ -90,6 +90,33 @@ def test_gsf_tables_are_bitwise_the_oracle_functions(product, oracle):
     assert not bad, bad[:8]
 
 
+def test_pitau_tables_are_bitwise_the_oracle_functions(product, oracle):
+    """The FP64 angle tables (pitau_table: pi_n + tau_n, pi_n - tau_n) equal
+    the oracle's pi_n, tau_n recurrence in the reference's order, added and
+    subtracted, bit for bit; at 95 angles (nodes, 0, +-1, random), n <= 1905."""
+    n, q = 1905, 3810
+    rng = np.random.Generator(np.random.PCG64(32))
+    x, w = np.zeros(q), np.zeros(q)
+    g = product.lmie_gpu_gauss_rule
+    g.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    assert g(q, x.ctypes.data_as(C.POINTER(C.c_double)), w.ctypes.data_as(C.POINTER(C.c_double)), 0) == 0
+    mu = np.ascontiguousarray(np.concatenate([x[q // 2:][::37], [0.0, 1.0, -1.0], rng.uniform(-1.0, 1.0, 40)]))
+    say("case: pi/tau tables at %d angles, n <= %d" % (mu.size, n))
+    bp, bm = np.zeros((n, mu.size)), np.zeros((n, mu.size))
+    f = product.lmie_gpu_pitau_table
+    f.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int]
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    assert f(mu.size, dp(mu), n, dp(bp), dp(bm), 0) == 0
+    bad = []
+    for k, m in enumerate(mu):
+        pi, tau = (C.c_double * n)(), (C.c_double * n)()
+        oracle.oracle_pitau(float(m), n, pi, tau)
+        pi, tau = np.array(pi), np.array(tau)
+        if not (np.array_equal(bp[:, k], pi + tau) and np.array_equal(bm[:, k], pi - tau)):
+            bad.append(float(m))
+    assert not bad, bad[:8]
+
+
 @pytest.mark.parametrize("name", SINGLE)
 def test_fp64_matches_reference(product, name):
     say("case: fp64 parity", name)
